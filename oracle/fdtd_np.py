@@ -1,0 +1,762 @@
+"""numpy restatement of rfx's FDTD time-stepping path — TEST INFRASTRUCTURE ONLY.
+
+Used by ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg
+of ``bench.py`` as the checker; never imported by the product package.
+
+Every function names the reference file:line it restates (paths relative to
+the rfx repository root).  Stepping arithmetic runs in float64 by default
+(``dtype=np.float64``), the "FP64 run" that float32 results are quoted
+against; ``dtype=np.float32`` gives a float32 run for bit-pattern studies.
+
+Set-up quantities keep the reference's own float32 rounding wherever the
+reference computes them in float32 even under x64 (material arrays, CPML
+b/c/kappa, source waveform tables, node coordinates): the reference's
+float64 mode keeps those float32 too (rfx/api/__init__.py:320-324,
+rfx/simulation.py:273 ``jnp.arange(n_steps, dtype=jnp.float32)``).
+
+Pinning (what checks each function; see tests/test_oracle_*.py):
+  * uniform step, CPML, lumped-port fold, V/I DFT, S11 extraction:
+    golden vector tests/fixtures/golden_forward_no_rlc_s11.npy (rtol 1e-5).
+  * soft J-source, eps override, probe series, forward lane (no outer PEC):
+    tests/fixtures/gradient_dx_ladder/ad_fd_ladder.json J0 / J+ / J- values.
+  * DFT plane accumulator: equality with a direct DFT of the stored probe
+    series at the same cell (closed form).
+  * Box rasterisation: brute-force per-node comparison on tiny grids and
+    the README quick-start footprint.
+  * CPML profile: closed forms of rfx/boundaries/cpml.py:101-171
+    (sigma_max formula, b = exp(...), c = sigma (b-1)/(sigma k + k^2 a)).
+  * graded mesh: a uniform profile must reproduce the uniform lane after the
+    documented source-normalisation factor; PEC-cavity resonance closed
+    form.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# rfx/grid.py:8
+C0 = 299_792_458.0
+# rfx/core/yee.py:117-118
+EPS_0 = 8.8541878128e-12
+MU_0 = 1.25663706212e-6
+# rfx/nonuniform.py:34 (speed of light the graded mesh derives from eps0/mu0)
+C0_NU = 1.0 / np.sqrt(float(EPS_0) * float(MU_0))
+
+F32 = np.float32
+COMPS = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+
+# ---------------------------------------------------------------------------
+# Grid (rfx/grid.py:11-211)
+# ---------------------------------------------------------------------------
+
+class UniformGrid:
+    """Uniform grid: shape incl. CPML pads, dt, position->index.
+
+    rfx/grid.py:91-96 (dx default lambda/20, dt = dx/(C0 sqrt 3) * 0.99),
+    :150-164 (n = ceil(L/dx) + 1 + pad_lo + pad_hi), :190-211 (index =
+    Python round(pos/dx) + pad_lo, round-half-even).
+    """
+
+    def __init__(self, freq_max, domain, dx=None, cpml_layers=8):
+        self.freq_max = float(freq_max)
+        self.domain = tuple(float(d) for d in domain)
+        self.dx = float(dx) if dx is not None else C0 / freq_max / 20.0
+        self.dt = self.dx / (C0 * np.sqrt(3.0)) * 0.99
+        self.pad = int(cpml_layers)
+        self.cpml_layers = int(cpml_layers)
+        self.shape = tuple(int(np.ceil(d / self.dx)) + 1 + 2 * self.pad
+                           for d in self.domain)
+
+    def position_to_index(self, pos):
+        idx = tuple(int(round(p / self.dx)) + self.pad for p in pos)
+        for a in range(3):
+            if not 0 <= idx[a] < self.shape[a]:
+                raise ValueError(f"position {pos} maps outside the grid")
+        return idx
+
+    def num_timesteps(self, num_periods=20.0):
+        # rfx/grid.py:185-188
+        return int(np.ceil(num_periods * (1.0 / self.freq_max) / self.dt))
+
+    def node_coords_f32(self):
+        """rfx/geometry/csg.py:61-69: f32(f32(i - pad) * f32(dx))."""
+        return [(np.arange(n) - self.pad).astype(F32) * F32(self.dx)
+                for n in self.shape]
+
+    # per-axis spacing seen by the stepper
+    def spacing(self):
+        return dict(kind="uniform", dx=self.dx)
+
+
+class GradedGrid:
+    """Graded (non-uniform) grid: rfx/nonuniform.py:187-224 and 274-422.
+
+    Profiles are the PHYSICAL interior cell sizes; each axis is padded by
+    ``cpml_layers`` boundary-size cells per face (_pad_profile :98-128)
+    and gets one duplicated bounding cell (_append_bounding_node :130-155).
+    dt = 0.99 / (c sqrt(sum 1/dmin^2)) with c = 1/sqrt(eps0 mu0) (:386-401).
+    """
+
+    def __init__(self, dz_profile, dx, domain_xy, cpml_layers=8,
+                 dx_profile=None, dy_profile=None):
+        p = int(cpml_layers)
+        self.pad = p
+        self.cpml_layers = p
+
+        def _full(prof):
+            prof = np.asarray(prof, dtype=np.float64)
+            lo = np.full(p, float(prof[0]))
+            hi = np.full(p, float(prof[-1]))
+            full = np.concatenate([lo, prof, hi])
+            return np.concatenate([full, full[-1:]])
+
+        if dx_profile is None:
+            dx_profile = np.full(int(round(domain_xy[0] / dx)), float(dx))
+        if dy_profile is None:
+            dy_profile = np.full(int(round(domain_xy[1] / dx)), float(dx))
+        self.d_full = [_full(dx_profile), _full(dy_profile), _full(dz_profile)]
+        self.shape = tuple(len(d) for d in self.d_full)
+        dmins = [float(np.min(d)) for d in self.d_full]
+        self.dt = float(0.99 / (C0_NU * np.sqrt(sum(1.0 / m ** 2 for m in dmins))))
+        self.dx = float(dx)
+        self.dy = float(np.asarray(dy_profile)[0])
+        # rfx/nonuniform.py:212-224 (float32 arrays)
+        self.inv_e, self.inv_h = [], []
+        for d in self.d_full:
+            a = d.astype(F32)
+            inv_local = F32(1.0) / a
+            inv_mean = F32(2.0) / (a[:-1] + a[1:])
+            self.inv_h.append(np.concatenate([inv_local[:-1], np.zeros(1, F32)]))
+            self.inv_e.append(np.concatenate([inv_local[:1], inv_mean]))
+        # CPML face cell sizes (rfx/boundaries/cpml.py:174-197)
+        dz = self.d_full[2]
+        self.face_d = {"x": (self.dx, self.dx), "y": (self.dy, self.dy),
+                       "z": (float(dz[0]), float(dz[-1]))}
+
+    def node_positions(self, axis):
+        """rfx/geometry/rasterize_grid.py:84 node coordinates (float32).
+
+        Node 0 sits at -(sum of the lo pad cells); node k at the cumulative
+        sum of the first k cells, offset so the first interior node is 0.
+        """
+        d = self.d_full[axis]
+        edges = np.concatenate([[0.0], np.cumsum(d[:-1])])
+        off = float(np.sum(d[:self.pad]))
+        return (edges - off).astype(F32)
+
+    def spacing(self):
+        return dict(kind="graded", inv_e=self.inv_e, inv_h=self.inv_h)
+
+
+# ---------------------------------------------------------------------------
+# Geometry and materials (rfx/geometry/csg.py, rfx/api/_compile.py,
+# rfx/geometry/rasterize_grid.py)
+# ---------------------------------------------------------------------------
+
+def _axis_mask(coords, lo, hi):
+    """rfx/geometry/csg.py:243-302 (float32 comparisons, thin-sheet branch)."""
+    coords = np.asarray(coords, dtype=F32)
+    mid = (lo + hi) * 0.5
+    extent = float(hi - lo)
+    n = coords.size
+    if n <= 1:
+        dc_local = F32(1e-3)
+    else:
+        k_mid = int(np.clip(np.searchsorted(coords, F32(mid)) - 1, 0, n - 2))
+        im1 = int(np.clip(k_mid - 1, 0, n - 1))
+        ip1 = int(np.clip(k_mid + 1, 0, n - 1))
+        s_left = coords[k_mid] - coords[im1]
+        s_right = coords[ip1] - coords[k_mid]
+        if s_left > 0 and s_right > 0:
+            dc_local = min(s_left, s_right)
+        else:
+            dc_local = max(s_left, s_right)
+    nearest = int(np.argmin(np.abs(coords - F32(mid))))
+    thin = np.zeros(n, dtype=bool)
+    thin[nearest] = True
+    vol = (coords >= F32(lo)) & (coords < F32(hi))
+    is_thin = F32(extent) <= F32(dc_local) * F32(1.01)
+    return thin if is_thin else vol
+
+
+def box_mask(coords, corner_lo, corner_hi):
+    """rfx/geometry/csg.py:296-302: separable per-axis masks, outer-ANDed."""
+    mx = _axis_mask(coords[0], corner_lo[0], corner_hi[0])
+    my = _axis_mask(coords[1], corner_lo[1], corner_hi[1])
+    mz = _axis_mask(coords[2], corner_lo[2], corner_hi[2])
+    return mx[:, None, None] & my[None, :, None] & mz[None, None, :]
+
+
+def extend_cpml_pad(eps_r, sigma, mu_r, p):
+    """rfx/geometry/rasterize_grid.py:237-414 with equal pads on all faces.
+
+    Lo faces replicate the first interior slice outward; hi faces replicate
+    the last interior slice, falling back one slice inward (and repairing
+    the dropped boundary node) where the last slice is vacuum but the one
+    inside it is not.
+    """
+    arrays = [eps_r.copy(), sigma.copy(), mu_r.copy()]
+    if p <= 0:
+        return tuple(arrays)
+    for ax in range(3):
+        n = arrays[0].shape[ax]
+
+        def sl(a, b):
+            s = [slice(None)] * 3
+            s[ax] = slice(a, b)
+            return tuple(s)
+
+        # lo face
+        for i, a in enumerate(arrays):
+            src = a[sl(p, p + 1)]
+            a[sl(0, p)] = src
+        # hi face
+        outer, inner, dst = sl(n - p - 1, n - p), sl(n - p - 2, n - p - 1), sl(n - p, n)
+        e, s, m = arrays
+        outer_vac = (e[outer] == 1.0) & (s[outer] == 0.0) & (m[outer] == 1.0)
+        use_inner = None
+        if n - 2 * p >= 2:
+            inner_vac = (e[inner] == 1.0) & (s[inner] == 0.0) & (m[inner] == 1.0)
+            use_inner = outer_vac & ~inner_vac
+        for a in arrays:
+            src = a[outer].copy()
+            if use_inner is not None:
+                src = np.where(use_inner, a[inner], src)
+                a[outer] = src
+            a[dst] = src
+    return tuple(arrays)
+
+
+PEC_SIGMA_THRESHOLD = 1e6  # rfx/api/_compile.py:118
+
+
+def assemble_materials(shape, coords, geometry, materials, cpml_pad):
+    """rfx/api/_compile.py:121-367 (no dispersion / thin conductors).
+
+    geometry: list of (corner_lo, corner_hi, material_name) in insertion
+    order (later wins); materials: name -> (eps_r, sigma, mu_r).
+    Returns float32 eps_r, sigma, mu_r and a bool pec_mask or None.
+    """
+    eps_r = np.ones(shape, F32)
+    sigma = np.zeros(shape, F32)
+    mu_r = np.ones(shape, F32)
+    pec = np.zeros(shape, bool)
+    for lo, hi, name in geometry:
+        e, s, m = materials[name]
+        mask = box_mask(coords, lo, hi)
+        if s >= PEC_SIGMA_THRESHOLD:
+            pec |= mask
+        else:
+            eps_r = np.where(mask, F32(e), eps_r)
+            sigma = np.where(mask, F32(s), sigma)
+            mu_r = np.where(mask, F32(m), mu_r)
+    if cpml_pad > 0:
+        eps_r, sigma, mu_r = extend_cpml_pad(eps_r, sigma, mu_r, cpml_pad)
+    return eps_r, sigma, mu_r, (pec if pec.any() else None)
+
+
+def tangential_edge_masks(cell_mask):
+    """rfx/boundaries/pec.py:90-176 (non-periodic axes: zero-padded shifts)."""
+    out = []
+    for ax in range(3):
+        nb = shift_bwd(cell_mask, ax) | shift_fwd(cell_mask, ax)
+        out.append(cell_mask & nb)
+    return tuple(out)
+
+
+# ---------------------------------------------------------------------------
+# CPML profile (rfx/boundaries/cpml.py:101-171, 346-478)
+# ---------------------------------------------------------------------------
+
+def cpml_profile(n_layers, dt, d, kappa_max=1.0, order=3, R=1e-15):
+    """rfx/boundaries/cpml.py:101-171 (float64 work, float32 result).
+
+    Returns (b, c, kappa) float32 arrays for the lo face (index 0 =
+    outermost cell).
+    """
+    eta = float(np.sqrt(MU_0 / EPS_0))
+    thick = n_layers * d
+    sigma_max = -float(np.log(R)) * (order + 1) / (2.0 * eta * thick)
+    sigma_max = sigma_max * kappa_max
+    rho = 1.0 - np.arange(n_layers, dtype=np.float64) / max(n_layers - 1, 1)
+    sigma = sigma_max * rho ** order
+    kappa = 1.0 + (kappa_max - 1.0) * rho ** order
+    alpha = 0.05 * (1.0 - rho)
+    denom = sigma * kappa + kappa ** 2 * alpha
+    b = np.exp(-(sigma / kappa + alpha) * dt / EPS_0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c = np.where(denom > 1e-30, sigma * (b - 1.0) / denom, 0.0)
+    return b.astype(F32), c.astype(F32), kappa.astype(F32)
+
+
+# ---------------------------------------------------------------------------
+# Sources (rfx/sources/sources.py, rfx/simulation.py:181-315)
+# ---------------------------------------------------------------------------
+
+def gaussian_pulse_f32(times, f0, bandwidth=0.5, amplitude=1.0, cutoff=3.0):
+    """rfx/sources/sources.py:56-104, evaluated in float32 like the JAX call.
+
+    tau = 1/(f0 bw pi), t0 = cutoff tau, s = A (-2u) exp(-u^2), u=(t-t0)/tau.
+    """
+    tau = 1.0 / (f0 * bandwidth * math.pi)
+    t0 = cutoff * tau
+    t = np.asarray(times, dtype=F32)
+    arg = (t - F32(t0)) / F32(tau)
+    return (F32(amplitude) * (F32(-2.0) * arg) * np.exp(-(arg ** 2))).astype(F32)
+
+
+def times_f32(n_steps, dt):
+    """rfx/simulation.py:273: jnp.arange(n, float32) * dt -> float32."""
+    return np.arange(n_steps, dtype=F32) * F32(dt)
+
+
+def cell_cb_f32(dt, eps_r, sigma, idx):
+    """rfx/simulation.py:181-194 in float32."""
+    eps = eps_r[idx] * F32(EPS_0)
+    sig = sigma[idx]
+    loss = sig * F32(dt) / (F32(2.0) * eps)
+    return (F32(dt) / eps) / (F32(1.0) + loss)
+
+
+# ---------------------------------------------------------------------------
+# Stencil helpers (rfx/core/yee.py:121-138)
+# ---------------------------------------------------------------------------
+
+def shift_fwd(a, ax):
+    """a[i+1] with zero at the last index (yee.py:121-128)."""
+    out = np.zeros_like(a)
+    s_dst = [slice(None)] * 3
+    s_src = [slice(None)] * 3
+    s_dst[ax] = slice(0, -1)
+    s_src[ax] = slice(1, None)
+    out[tuple(s_dst)] = a[tuple(s_src)]
+    return out
+
+
+def shift_bwd(a, ax):
+    """a[i-1] with zero at the first index (yee.py:131-138)."""
+    out = np.zeros_like(a)
+    s_dst = [slice(None)] * 3
+    s_src = [slice(None)] * 3
+    s_dst[ax] = slice(1, None)
+    s_src[ax] = slice(0, -1)
+    out[tuple(s_dst)] = a[tuple(s_src)]
+    return out
+
+
+def _bcast(v, ax):
+    shp = [1, 1, 1]
+    shp[ax] = -1
+    return np.asarray(v).reshape(shp)
+
+
+# ---------------------------------------------------------------------------
+# The step (rfx/simulation.py:1256-1681; rfx/nonuniform.py:1256-1565)
+# ---------------------------------------------------------------------------
+
+# CPML correction table: (field, derivative-source field, sign) per axis.
+# H: rfx/boundaries/cpml.py:875-1043; E: cpml.py:608-777.
+CPML_H_TERMS = {
+    0: (("hy", "ez", +1.0), ("hz", "ey", -1.0)),
+    1: (("hx", "ez", -1.0), ("hz", "ex", +1.0)),
+    2: (("hx", "ey", +1.0), ("hy", "ex", -1.0)),
+}
+CPML_E_TERMS = {
+    0: (("ey", "hz", -1.0), ("ez", "hy", +1.0)),
+    1: (("ex", "hz", +1.0), ("ez", "hx", -1.0)),
+    2: (("ex", "hy", -1.0), ("ey", "hx", +1.0)),
+}
+
+
+@dataclass
+class StepConfig:
+    """Everything the time loop needs, already set up on the host."""
+    shape: tuple
+    dt: float
+    spacing: dict                     # {"kind": "uniform", "dx"} or graded inv arrays
+    eps_r: np.ndarray                 # float32 (nx,ny,nz)
+    sigma: np.ndarray
+    mu_r: np.ndarray
+    cpml_n: int = 0                   # layers per face (0 = no CPML)
+    cpml_axes: str = "xyz"
+    cpml_profiles: dict = field(default_factory=dict)  # (axis, side) -> (b,c,kappa) per slab index
+    cpml_face_d: dict = field(default_factory=dict)    # axis -> (d_lo, d_hi)
+    pec_axes: str = "xyz"
+    pec_mask: np.ndarray | None = None
+    sources: list = field(default_factory=list)        # (idx, comp, waveform f32 (n_steps,))
+    probes: list = field(default_factory=list)         # (idx, comp)
+    dft_planes: list = field(default_factory=list)     # (axis, index, comp, freqs)
+    ports_vi: list = field(default_factory=list)       # (idx, comp, freqs, d_par, d_loop(dict))
+    record_order: str = "uniform"                      # "uniform" | "graded"
+
+
+class Stepper:
+    """One FDTD time loop in the reference's sub-step order."""
+
+    def __init__(self, cfg: StepConfig, dtype=np.float64):
+        self.cfg = cfg
+        self.dt = cfg.dt
+        self.dtype = dtype
+        dt = dtype(cfg.dt)
+        eps = cfg.eps_r.astype(dtype) * dtype(EPS_0)
+        sig = cfg.sigma.astype(dtype)
+        mu = cfg.mu_r.astype(dtype) * dtype(MU_0)
+        # yee.py:386-393 and :272/:289
+        L = sig * dt / (dtype(2.0) * eps)
+        self.ca = (dtype(1.0) - L) / (dtype(1.0) + L)
+        self.cb = (dt / eps) / (dtype(1.0) + L)
+        self.ch = dt / mu
+        # cpml.py:538-546 (ce) and :817-826 (ch)
+        self.ce_cpml = dt / (cfg.eps_r.astype(dtype) * dtype(EPS_0))
+        self.ch_cpml = dt / (cfg.mu_r.astype(dtype) * dtype(MU_0))
+        self.f = {c: np.zeros(cfg.shape, dtype) for c in COMPS}
+        self.psi = {}
+        self.step_count = 0
+        if cfg.pec_mask is not None:
+            self.mask = tangential_edge_masks(cfg.pec_mask)
+        else:
+            self.mask = None
+        sp = cfg.spacing
+        if sp["kind"] == "graded":
+            self.inv_e = [_bcast(v.astype(dtype), a) for a, v in enumerate(sp["inv_e"])]
+            self.inv_h = [_bcast(v.astype(dtype), a) for a, v in enumerate(sp["inv_h"])]
+
+    # -- differences ------------------------------------------------------
+    def _dfwd(self, a, ax):
+        sp = self.cfg.spacing
+        if sp["kind"] == "uniform":
+            return (shift_fwd(a, ax) - a) / self.dtype(sp["dx"])
+        return (shift_fwd(a, ax) - a) * self.inv_h[ax]
+
+    def _dbwd(self, a, ax):
+        sp = self.cfg.spacing
+        if sp["kind"] == "uniform":
+            return (a - shift_bwd(a, ax)) / self.dtype(sp["dx"])
+        return (a - shift_bwd(a, ax)) * self.inv_e[ax]
+
+    # -- sub-steps ---------------------------------------------------------
+    def update_h(self):
+        """yee.py:265-291 (uniform) / yee.py:429-451 (graded)."""
+        f = self.f
+        cx = self._dfwd(f["ez"], 1) - self._dfwd(f["ey"], 2)
+        cy = self._dfwd(f["ex"], 2) - self._dfwd(f["ez"], 0)
+        cz = self._dfwd(f["ey"], 0) - self._dfwd(f["ex"], 1)
+        f["hx"] = f["hx"] - self.ch * cx
+        f["hy"] = f["hy"] - self.ch * cy
+        f["hz"] = f["hz"] - self.ch * cz
+
+    def update_e(self):
+        """yee.py:379-404 (uniform) / yee.py:470-492 (graded)."""
+        f = self.f
+        cx = self._dbwd(f["hz"], 1) - self._dbwd(f["hy"], 2)
+        cy = self._dbwd(f["hx"], 2) - self._dbwd(f["hz"], 0)
+        cz = self._dbwd(f["hy"], 0) - self._dbwd(f["hx"], 1)
+        f["ex"] = self.ca * f["ex"] + self.cb * cx
+        f["ey"] = self.ca * f["ey"] + self.cb * cy
+        f["ez"] = self.ca * f["ez"] + self.cb * cz
+
+    def _slab(self, ax, side):
+        n = self.cfg.cpml_n
+        s = [slice(None)] * 3
+        N = self.cfg.shape[ax]
+        s[ax] = slice(0, n) if side == 0 else slice(N - n, N)
+        return tuple(s)
+
+    def _cpml(self, terms, is_h):
+        cfg = self.cfg
+        if cfg.cpml_n <= 0:
+            return
+        dt_ = self.dtype
+        coef = self.ch_cpml if is_h else self.ce_cpml
+        for ax in range(3):
+            name = "xyz"[ax]
+            if name not in cfg.cpml_axes:
+                continue
+            for side in (0, 1):
+                sl = self._slab(ax, side)
+                b, c, k = cfg.cpml_profiles[(ax, side)]
+                b = _bcast(b.astype(dt_), ax)
+                c = _bcast(c.astype(dt_), ax)
+                kk = _bcast(k.astype(dt_), ax)
+                d = dt_(cfg.cpml_face_d[ax][side])
+                for fld, src, sign in terms[ax]:
+                    a = self.f[src]
+                    if is_h:
+                        g = (shift_fwd(a, ax)[sl] - a[sl]) / d
+                    else:
+                        g = (a[sl] - shift_bwd(a, ax)[sl]) / d
+                    key = (fld, ax, side)
+                    psi = self.psi.get(key)
+                    if psi is None:
+                        psi = np.zeros_like(g)
+                    psi = b * psi + c * g
+                    self.psi[key] = psi
+                    cc = coef[sl]
+                    upd = self.f[fld][sl]
+                    upd = upd + dt_(sign) * cc * psi
+                    upd = upd + dt_(sign) * cc * (dt_(1.0) / kk - dt_(1.0)) * g
+                    self.f[fld][sl] = upd
+
+    def apply_pec(self):
+        """rfx/boundaries/pec.py:13-48."""
+        f = self.f
+        ax = self.cfg.pec_axes
+        if "x" in ax:
+            for c in ("ey", "ez"):
+                f[c][0, :, :] = 0.0
+                f[c][-1, :, :] = 0.0
+        if "y" in ax:
+            for c in ("ex", "ez"):
+                f[c][:, 0, :] = 0.0
+                f[c][:, -1, :] = 0.0
+        if "z" in ax:
+            for c in ("ex", "ey"):
+                f[c][:, :, 0] = 0.0
+                f[c][:, :, -1] = 0.0
+            f["ez"][:, :, -1] = 0.0
+        if self.mask is not None:  # pec.py:179-208
+            for c, m in zip(("ex", "ey", "ez"), self.mask):
+                f[c] = f[c] * (1.0 - m.astype(self.dtype))
+
+    def step_fields(self):
+        """H -> CPML-H -> E -> CPML-E -> PEC (simulation.py:1256-1407)."""
+        self.update_h()
+        self._cpml(CPML_H_TERMS, True)
+        self.update_e()
+        self._cpml(CPML_E_TERMS, False)
+        self.apply_pec()
+        self.step_count += 1
+
+
+def port_current(f, idx, comp, loop):
+    """Ampere loop of H around the port edge (simulation.py:1484-1493).
+
+    ``loop`` is the per-axis length multiplying each H difference: uniform
+    lane dx (simulation.py:1484-1493); graded wire ports use the dual
+    spacings (rfx/nonuniform.py:720-777).
+    """
+    i, j, k = idx
+    if comp == "ez":
+        return ((f["hy"][i, j, k] - f["hy"][i - 1, j, k]) * loop[1]
+                - (f["hx"][i, j, k] - f["hx"][i, j - 1, k]) * loop[0])
+    if comp == "ex":
+        return ((f["hz"][i, j, k] - f["hz"][i, j - 1, k]) * loop[2]
+                - (f["hy"][i, j, k] - f["hy"][i, j, k - 1]) * loop[1])
+    return ((f["hx"][i, j, k] - f["hx"][i, j, k - 1]) * loop[0]
+            - (f["hz"][i, j, k] - f["hz"][i - 1, j, k]) * loop[2])
+
+
+@dataclass
+class RunOutput:
+    time_series: np.ndarray
+    dft_planes: list
+    port_v: list        # per port: (n_steps,) V samples (pre-injection)
+    port_i: list
+    port_v_dft: list
+    port_i_dft: list
+    fields: dict
+    psi: dict
+
+
+def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
+    """rfx/simulation.py:1977 scan body order, per step n:
+    fields -> port V/I sample (t = n dt, before injection, :1449-1497) ->
+    sources (:1520-1523) -> probes (:1565-1567) -> DFT planes
+    (t = (n+1) dt, :1574-1588).  Graded order (rfx/nonuniform.py:1354-1542):
+    sources -> port V/I -> DFT planes (t = n dt) -> probes.
+
+    DFT sums are accumulated in float64 (complex128).
+    """
+    st = stepper if stepper is not None else Stepper(cfg, dtype)
+    f = st.f
+    npr = len(cfg.probes)
+    ts = np.zeros((n_steps, npr), dtype=np.float64)
+    accs = [np.zeros((len(fr),) + _plane_shape(cfg.shape, ax), np.complex128)
+            for ax, _, _, fr in cfg.dft_planes]
+    pv = [np.zeros(n_steps) for _ in cfg.ports_vi]
+    pi = [np.zeros(n_steps) for _ in cfg.ports_vi]
+    dt = cfg.dt
+    for n in range(n_steps):
+        st.step_fields()
+        if cfg.record_order == "uniform":
+            _sample_ports(f, cfg, pv, pi, n)
+            _inject(f, cfg, n)
+            _probe(f, cfg, ts, n)
+            _planes(f, cfg, accs, (n + 1) * dt, dt)
+        else:
+            _inject(f, cfg, n)
+            _sample_ports(f, cfg, pv, pi, n)
+            _planes(f, cfg, accs, n * dt, dt)
+            _probe(f, cfg, ts, n)
+    tvec = np.arange(n_steps) * dt
+    vd, idft = [], []
+    for p, (idx, comp, freqs, d_par, loop) in enumerate(cfg.ports_vi):
+        ph = np.exp(-1j * 2.0 * np.pi * np.outer(np.asarray(freqs, np.float64), tvec)) * dt
+        vd.append(ph @ pv[p])
+        idft.append(ph @ pi[p])
+    return RunOutput(ts, accs, pv, pi, vd, idft, f, st.psi)
+
+
+def _plane_shape(shape, ax):
+    return tuple(s for a, s in enumerate(shape) if a != ax)
+
+
+def _plane(f, ax, index):
+    if ax == 0:
+        return f[index, :, :]
+    if ax == 1:
+        return f[:, index, :]
+    return f[:, :, index]
+
+
+def _sample_ports(f, cfg, pv, pi, n):
+    for p, (idx, comp, freqs, d_par, loop) in enumerate(cfg.ports_vi):
+        pv[p][n] = -f[comp][idx] * d_par
+        pi[p][n] = port_current(f, idx, comp, loop)
+
+
+def _inject(f, cfg, n):
+    for idx, comp, wave in cfg.sources:
+        f[comp][idx] += wave[n]
+
+
+def _probe(f, cfg, ts, n):
+    for p, (idx, comp) in enumerate(cfg.probes):
+        ts[n, p] = f[comp][idx]
+
+
+def _planes(f, cfg, accs, t, dt):
+    for q, (ax, index, comp, freqs) in enumerate(cfg.dft_planes):
+        ph = np.exp(-1j * 2.0 * np.pi * np.asarray(freqs, np.float64) * t) * dt
+        accs[q] += ph[:, None, None] * _plane(f[comp], ax, index)[None].astype(np.float64)
+
+
+def extract_lumped_s11(v_dft, i_dft, z0=50.0):
+    """rfx/probes/probes.py:1239-1273: S11 = (V + Z0 I) / (V - Z0 I)."""
+    den = v_dft - z0 * i_dft
+    den = np.where(np.abs(den) > 0.0, den, 1.0)
+    return (v_dft + z0 * i_dft) / den
+
+
+# ---------------------------------------------------------------------------
+# High-level uniform-lane set-up (rfx/api/_compile.py, rfx/runners/uniform.py,
+# rfx/api/_execute.py:917-1669)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Pulse:
+    f0: float
+    bandwidth: float = 0.5
+    amplitude: float = 1.0
+    cutoff: float = 3.0
+
+
+def build_uniform(spec, n_steps):
+    """Build a StepConfig for the uniform lane from a high-level spec.
+
+    spec keys: freq_max, domain, dx, boundary ("cpml"|"pec"), cpml_layers,
+    kappa_max, materials {name: (eps_r, sigma, mu_r)}, geometry
+    [(lo, hi, name)], sources [(pos, comp, Pulse)], ports
+    [(pos, comp, z0, Pulse|None, excite)], probes [(pos, comp)],
+    dft_planes [(axis, coord, comp, freqs|None, n_freqs)], lane
+    ("run"|"forward"), eps_override (array|None), s11_freqs.
+
+    Lane semantics: ``run`` applies outer-face PEC on every axis
+    (rfx/runners/uniform.py:321 passes pec_axes=None -> all axes,
+    rfx/simulation.py:684-691); ``forward`` passes pec_axes = axes without
+    CPML, i.e. none when CPML covers x, y and z (rfx/api/_execute.py:
+    1420-1421).  Soft sources are Cb-normalised J sources on CPML
+    (uniform.py:339-360) and always on forward() (_execute.py:1040-1051);
+    raw field adds on the run() PEC lane (simulation.py:205-236).
+    """
+    boundary = spec.get("boundary", "cpml")
+    layers = spec.get("cpml_layers", 16) if boundary == "cpml" else 0
+    g = UniformGrid(spec["freq_max"], spec["domain"], spec.get("dx"), layers)
+    shape = g.shape
+    coords = g.node_coords_f32()
+    eps_r, sigma, mu_r, pec = assemble_materials(
+        shape, coords, spec.get("geometry", []), spec.get("materials", {}),
+        layers)
+    lane = spec.get("lane", "run")
+    if lane == "forward" and spec.get("eps_override") is not None:
+        # _execute.py:2800-2805: override replaces eps_r wholesale
+        eps_r = np.asarray(spec["eps_override"], F32).reshape(shape).copy()
+    if lane == "forward" and spec.get("sigma_override") is not None:
+        sigma = np.asarray(spec["sigma_override"], F32).reshape(shape).copy()
+    dt = g.dt
+    tms = times_f32(n_steps, dt)
+    sources, ports_vi = [], []
+    s11_freqs = spec.get("s11_freqs")
+    # add_source() and add_port() both append to Simulation._ports in
+    # registration order (rfx/api/__init__.py:1008-1016, 1609-1624); a soft
+    # source is the impedance-0 entry.  Sources are built in that order.
+    for pos, comp, z0, pulse, excite in spec.get("ports", []):
+        idx = g.position_to_index(pos)
+        if pulse is None and excite:
+            pulse = Pulse(spec["freq_max"] / 2, 0.8)
+        if z0 == 0.0:
+            # simulation.py:239-293 (J source) / :205-236 (raw add)
+            wave = gaussian_pulse_f32(tms, pulse.f0, pulse.bandwidth,
+                                      pulse.amplitude, pulse.cutoff)
+            if boundary == "cpml" or lane == "forward":
+                wave = cell_cb_f32(dt, eps_r, sigma, idx) * wave
+            sources.append((idx, comp, wave.astype(F32)))
+            continue
+        # sources.py:31-45 (port_sigma) + :238-247 (fold into sigma)
+        sp = g.dx / (z0 * g.dx * g.dx)
+        sigma = sigma.copy()
+        sigma[idx] = F32(sigma[idx] + F32(sp))
+        if pec is not None:
+            pec = pec.copy()
+            pec[idx] = False
+        if excite:
+            # simulation.py:296-315
+            cb = cell_cb_f32(dt, eps_r, sigma, idx)
+            w = (cb / F32(g.dx)) * gaussian_pulse_f32(
+                tms, pulse.f0, pulse.bandwidth, pulse.amplitude, pulse.cutoff)
+            sources.append((idx, comp, w.astype(F32)))
+        if s11_freqs is not None:
+            ports_vi.append((idx, comp, np.asarray(s11_freqs, F32),
+                             g.dx, (g.dx, g.dx, g.dx)))
+    probes = [(g.position_to_index(p), c) for p, c in spec.get("probes", [])]
+    if not probes and lane == "forward" and spec.get("ports"):
+        # _execute.py:1319-1321: no probes -> one probe per port entry
+        probes = [(g.position_to_index(p[0]), p[1]) for p in spec["ports"]]
+    planes = []
+    for axis, coord, comp, freqs, nf in spec.get("dft_planes", []):
+        a = "xyz".index(axis)
+        pos = [0.0, 0.0, 0.0]
+        pos[a] = coord
+        index = g.position_to_index(tuple(pos))[a]
+        if freqs is None:
+            freqs = np.linspace(spec["freq_max"] / 10, spec["freq_max"], nf).astype(F32)
+        planes.append((a, index, comp, np.asarray(freqs, F32)))
+    profiles, face_d = {}, {}
+    if layers > 0:
+        b, c, k = cpml_profile(layers, dt, g.dx, spec.get("kappa_max", 1.0))
+        for ax in range(3):
+            profiles[(ax, 0)] = (b, c, k)
+            profiles[(ax, 1)] = (b[::-1].copy(), c[::-1].copy(), k[::-1].copy())
+            face_d[ax] = (g.dx, g.dx)
+    if lane == "run":
+        pec_axes = "xyz"
+    else:
+        cpml_axes = "xyz" if layers > 0 else ""
+        pec_axes = "".join(a for a in "xyz" if a not in cpml_axes)
+    cfg = StepConfig(shape=shape, dt=dt, spacing=g.spacing(), eps_r=eps_r,
+                     sigma=sigma, mu_r=mu_r, cpml_n=layers,
+                     cpml_axes="xyz" if layers > 0 else "",
+                     cpml_profiles=profiles, cpml_face_d=face_d,
+                     pec_axes=pec_axes, pec_mask=pec, sources=sources,
+                     probes=probes, dft_planes=planes, ports_vi=ports_vi)
+    return g, cfg
+
+
+def simulate_uniform(spec, n_steps, dtype=np.float64):
+    """Set up and run the uniform lane; returns (grid, cfg, RunOutput)."""
+    g, cfg = build_uniform(spec, n_steps)
+    out = run_loop(cfg, n_steps, dtype)
+    return g, cfg, out

@@ -1,0 +1,80 @@
+"""Shared problem descriptions for the oracle and parity tests.
+
+Each case is described once, as the high-level inputs a user gives the
+``Simulation`` API, and is turned into an oracle spec here and into
+``rfx_amd.Simulation`` calls in the GPU tests.
+"""
+import json
+import os
+
+import numpy as np
+
+from oracle import fdtd_np as o
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# rfx tests/test_lumped_rlc_ad.py:135-151
+S11_CASE = dict(freq_max=10e9, domain=(0.02, 0.02, 0.02), dx=0.02 / 15,
+                boundary="cpml", cpml_layers=6,
+                ports=[((0.0093,) * 3, "ez", 50.0, o.Pulse(5e9, 0.9), True)],
+                lane="forward", s11_freqs=np.array([3., 4., 5., 6., 7.]) * 1e9)
+S11_STEPS = 1200
+
+
+def golden_s11():
+    return np.load(os.path.join(GOLDEN, "golden_forward_no_rlc_s11.npy"))
+
+
+def ladder():
+    with open(os.path.join(GOLDEN, "ad_fd_ladder_values.json")) as fh:
+        return json.load(fh)
+
+
+# scripts/diagnostics/gradient_dx_ladder/run_ladder.py:60-165
+LADDER_LAMBDA = 2.998e8 / 10e9
+
+
+def ladder_case(n_per_lambda, eps_val):
+    dx = LADDER_LAMBDA / n_per_lambda
+    g = o.UniformGrid(10e9, (0.06, 0.012, 0.012), dx, 8)
+    n_steps = g.num_timesteps(20.0)
+    xs = (np.arange(g.shape[0]) - g.pad) * dx
+    m1d = (xs >= 30e-3) & (xs < 36e-3)
+    eps = np.where(m1d[:, None, None], np.float32(eps_val), np.float32(1.0)) * np.ones(g.shape, np.float32)
+    spec = dict(freq_max=10e9, domain=(0.06, 0.012, 0.012), dx=dx, boundary="cpml",
+                cpml_layers=8,
+                ports=[((12e-3, 6e-3, 6e-3), "ez", 0.0, o.Pulse(8e9, 0.9), True)],
+                probes=[((48e-3, 6e-3, 6e-3), "ez")], lane="forward", eps_override=eps)
+    return spec, n_steps, m1d
+
+
+def ladder_objective(ts, dt, f_obj=8e9):
+    t = np.arange(ts.shape[0]) * dt
+    acc = np.sum(ts * np.exp(-1j * 2 * np.pi * f_obj * t))
+    return float(abs(acc) ** 2)
+
+
+# README quick start (rfx README.md:49-74)
+def quickstart_case():
+    return dict(freq_max=5e9, domain=(0.14, 0.06, 0.05), dx=2e-3, boundary="cpml",
+                cpml_layers=8, materials={"slab": (2.2, 0.01, 1.0)},
+                geometry=[((0.07, 0.018, 0.018), (0.09, 0.042, 0.032), "slab")],
+                ports=[((0.03, 0.03, 0.025), "ez", 0.0, o.Pulse(3e9, 0.8), True)],
+                probes=[((0.11, 0.03, 0.025), "ez")], lane="run")
+
+
+# a small case touching every feature of the uniform lane
+def mixed_case(lane="run"):
+    return dict(freq_max=10e9, domain=(0.016, 0.012, 0.010), dx=1e-3, boundary="cpml",
+                cpml_layers=4,
+                materials={"d": (3.0, 0.02, 1.0), "m": (1.0, 1e8, 1.0), "mag": (1.5, 0.0, 2.0)},
+                geometry=[((0.004, 0.003, 0.002), (0.009, 0.008, 0.006), "d"),
+                          ((0.011, 0.002, 0.003), (0.0125, 0.009, 0.0035), "m"),
+                          ((0.002, 0.008, 0.006), (0.005, 0.0105, 0.009), "mag")],
+                ports=[((0.003, 0.006, 0.005), "ez", 0.0, o.Pulse(8e9, 0.8), True),
+                       ((0.0125, 0.006, 0.005), "ey", 0.0, o.Pulse(6e9, 0.5), True),
+                       ((0.008, 0.004, 0.003), "ex", 50.0, o.Pulse(7e9, 0.9), True)],
+                probes=[((0.013, 0.006, 0.005), "ez"), ((0.010, 0.005, 0.004), "hx"),
+                        ((0.003, 0.006, 0.005), "ez"), ((0.006, 0.007, 0.002), "hz")],
+                dft_planes=[("z", 0.005, "ez", None, 4), ("x", 0.008, "hy", None, 3)],
+                s11_freqs=np.array([5e9, 7e9, 9e9]), lane=lane)
