@@ -1,0 +1,202 @@
+"""Benchmark: Mcell-steps/s of the FDTD time loop on one MI355X.
+
+Workload (default): 512^3-cell interior, dx = 1 mm, 10-layer CPML on every
+face (533^3 = 151,419,437 cells incl. pads), 64 axis-aligned dielectric
+boxes at seeded random positions/sizes (8-64 cells per side, eps_r ~ U(1,10),
+sigma ~ U(0, 0.05) S/m), an Ez Gaussian-pulse source (f0 = 10 GHz) at the
+centre, 16 Ez point probes, one running DFT of Ez on the 533x533 z-plane
+through the centre at 32 frequencies, 2000 time steps, float32 fields.
+
+One bench step = one full 2000-time-step run of that problem (fields, CPML
+memory and accumulators re-zeroed first); value = steps x 2000 x cells /
+wall seconds of the timed region.  Inputs (material ids, coefficient
+tables, source table) are resident in HBM before timing starts.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one
+GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_STEPS = {"c2": 2000, "c1": 1200}
+BYTES_PER_CELL_STEP = 48.0   # 6 float32 field components, one read + one write each
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_spec(n_int=512):
+    """The 512^3 workload as high-level inputs (shared by the GPU run and the
+    CPU baseline)."""
+    rng = np.random.default_rng(2)
+    dx = 1e-3
+    L = n_int * dx
+    mats, geom = {}, []
+    for b in range(64):
+        size = rng.integers(8, 65, size=3)
+        lo = np.array([rng.integers(0, n_int - s + 1) for s in size])
+        eps = float(rng.uniform(1.0, 10.0))
+        sig = float(rng.uniform(0.0, 0.05))
+        name = f"m{b}"
+        mats[name] = (eps, sig, 1.0)
+        geom.append((tuple((lo + 0.5) * dx), tuple((lo + size + 0.5) * dx), name))
+    c = n_int // 2 * dx
+    probes = [(tuple((rng.integers(0, n_int + 1, size=3)) * dx), "ez") for _ in range(16)]
+    return dict(freq_max=20e9, domain=(L, L, L), dx=dx, boundary="cpml", cpml_layers=10,
+                materials=mats, geometry=geom,
+                sources=[((c, c, c), "ez", 10e9, 0.5)],
+                probes=probes,
+                dft_planes=[("z", c, "ez", np.linspace(2e9, 20e9, 32), 32)])
+
+
+def c1_spec():
+    """README quick start (87x47x42 cells, 1200 steps)."""
+    return dict(freq_max=5e9, domain=(0.14, 0.06, 0.05), dx=2e-3, boundary="cpml",
+                cpml_layers=8, materials={"slab": (2.2, 0.01, 1.0)},
+                geometry=[((0.07, 0.018, 0.018), (0.09, 0.042, 0.032), "slab")],
+                sources=[((0.03, 0.03, 0.025), "ez", 3e9, 0.8)],
+                probes=[((0.11, 0.03, 0.025), "ez")], dft_planes=[])
+
+
+def build_sim(spec):
+    import rfx_amd as R
+    sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec["dx"],
+                       boundary=spec["boundary"], cpml_layers=spec["cpml_layers"])
+    for name, (e, s, m) in spec["materials"].items():
+        sim.add_material(name, eps_r=e, sigma=s, mu_r=m)
+    for lo, hi, name in spec["geometry"]:
+        sim.add(R.Box(lo, hi), material=name)
+    for pos, comp, f0, bw in spec["sources"]:
+        sim.add_source(pos, comp, waveform=R.GaussianPulse(f0=f0, bandwidth=bw))
+    for pos, comp in spec["probes"]:
+        sim.add_probe(pos, comp)
+    for axis, coord, comp, freqs, nf in spec["dft_planes"]:
+        sim.add_dft_plane_probe(axis=axis, coordinate=coord, component=comp, freqs=freqs,
+                                n_freqs=nf)
+    return sim
+
+
+def oracle_spec(spec):
+    from oracle import fdtd_np as o
+    return dict(freq_max=spec["freq_max"], domain=spec["domain"], dx=spec["dx"],
+                boundary=spec["boundary"], cpml_layers=spec["cpml_layers"],
+                materials=spec["materials"], geometry=spec["geometry"],
+                ports=[(pos, comp, 0.0, o.Pulse(f0, bw), True)
+                       for pos, comp, f0, bw in spec["sources"]],
+                probes=spec["probes"], dft_planes=spec["dft_planes"], lane="run")
+
+
+def cpu_baseline(spec, n_total, cpu_steps):
+    """The C restatement (oracle/cstep.c, float32, OpenMP) timed on a prefix
+    of the same workload."""
+    from oracle import cstep
+    from oracle import fdtd_np as o
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    g, cfg = o.build_uniform(oracle_spec(spec), cpu_steps)
+    t0 = time.perf_counter()
+    cstep.run_c(cfg, cpu_steps, double=False, want_fields=False)
+    el = time.perf_counter() - t0
+    cells = int(np.prod(g.shape))
+    return {"value": round(cpu_steps * cells / el / 1e6, 3), "unit": "Mcell-steps/s",
+            "cores": cores, "kind": "port",
+            "sample": f"first {cpu_steps} of {n_total} time steps of the same "
+                      f"{g.shape[0]}x{g.shape[1]}x{g.shape[2]} problem, C restatement "
+                      f"(oracle/cstep.c) in float32 with {cores} OpenMP threads, "
+                      f"{el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(N_STEPS))
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU slab decomposition is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from rfx_amd.engine import DeviceRun
+    spec = c2_spec() if args.config == "c2" else c1_spec()
+    n_total = N_STEPS[args.config]
+    t_setup = time.perf_counter()
+    sim = build_sim(spec)
+    grid, prob, _ = sim._build_problem(n_total, "run")
+    dr = DeviceRun(prob, n_total, "cuda:0")
+    torch.cuda.synchronize()
+    print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s, grid {grid.shape}, "
+          f"materials {len(prob.mat.table)}", file=sys.stderr, flush=True)
+    cells = int(np.prod(grid.shape))
+    for w in range(args.warmup):
+        dr.reset()
+        dr.run(0, n_total, use_graph=not args.no_graph)
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    dr.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        dr.reset()
+        dr.run(0, n_total, use_graph=not args.no_graph)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} runs in {el:.3f} s", file=sys.stderr, flush=True)
+    ms_h, n_h = dr.kernel_time_ms(0)
+    ms_e, n_e = dr.kernel_time_ms(1)
+    ms_d, n_d = dr.kernel_time_ms(2)
+    value = args.steps * n_total * cells / el / 1e6
+    which, ms = ("k_update_e", ms_e) if ms_e >= ms_h else ("k_update_h", ms_h)
+    half_bytes = 0.5 * BYTES_PER_CELL_STEP * cells
+    achieved = half_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(f"{args.config}:{which}")
+    out = {
+        "metric": "Mcell-steps/s fp32 + % HBM roofline; fwd+adjoint grad time vs plain forward",
+        "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"uniform Yee + 10-layer CPML, {grid.shape[0]}x{grid.shape[1]}x"
+                                f"{grid.shape[2]} cells, 64 random dielectric boxes, Ez source, "
+                                f"16 probes, 32-freq DFT plane, {n_total} time steps")
+                   if args.config == "c2" else
+                   f"README quick start {grid.shape}, {n_total} time steps",
+                   "cells": cells, "time_steps": n_total,
+                   "pct_hbm_roofline_48B": round(100 * value * 1e6 * BYTES_PER_CELL_STEP
+                                                 / (HBM_PEAK_GBS * 1e9), 2),
+                   "kernel_ms": {"k_update_h": round(ms_h, 4), "k_update_e": round(ms_e, 4),
+                                 "k_dft_plane": round(ms_d, 4), "samples": n_e}},
+        "roofline": {"bound": "hbm", "kernel": which,
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": traffic},
+        "cpu_baseline": None,
+    }
+    dr.close()
+    del dr
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(spec, n_total, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/* rfxhip — C ABI of the MI355X (gfx950) FDTD time-stepping backend.
+ *
+ * The reference (bk-squared/rfx) has no FFI: its hot path is the Python
+ * operator boundary
+ *   rfx.simulation.run(grid, materials, n_steps, *, boundary, cpml_axes,
+ *                      pec_axes, sources, probes, dft_planes,
+ *                      lumped_port_sparams, pec_mask, ...) -> SimResult
+ *   (rfx/simulation.py:1686-1725, result :2125-2138) and
+ *   rfx.nonuniform.run_nonuniform(...) (rfx/nonuniform.py:1589-1618).
+ * This header is the seam that replaces the body of those two functions
+ * (the jax.lax.scan at rfx/simulation.py:1977 and rfx/nonuniform.py:1796):
+ * the host sets the problem up (grid, materials, CPML profiles, source
+ * tables) and hands device buffers to a plan that advances the fields.
+ *
+ * Conventions: every call returns 0 on success and a negative code on
+ * failure; rfxh_last_error() gives a thread-local message.  No C++
+ * exception crosses this boundary.  Device pointers ("d_" prefix) are owned
+ * by the caller and must stay alive for the plan's lifetime; host arrays are
+ * copied at rfxh_plan_create.  Field arrays are float32, [nx][ny][nz]
+ * row-major (z fastest).  A plan is used from one host thread at a time;
+ * distinct plans are independent.  Stream arguments are hipStream_t passed
+ * as void* (NULL = the default stream); all work is stream-ordered and
+ * asynchronous.
+ */
+#ifndef RFXHIP_H
+#define RFXHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RFXH_OK 0
+#define RFXH_EINVAL -1
+#define RFXH_EHIP -2
+#define RFXH_ENOMEM -3
+
+/* Material flag bits (per material id). */
+#define RFXH_ZERO_EX 1u   /* tangential-edge PEC mask on Ex (rfx/boundaries/pec.py:179-208) */
+#define RFXH_ZERO_EY 2u
+#define RFXH_ZERO_EZ 4u
+#define RFXH_POINT 8u     /* cell carries point operations (source/probe/port) */
+
+/* Point operation kinds. */
+#define RFXH_OP_PORT 0    /* record V = -E_c * d_par and the Ampere-loop current */
+#define RFXH_OP_INJECT 1  /* E_c += wave[n][slot]   (rfx/simulation.py:1520-1523) */
+#define RFXH_OP_PROBE 2   /* out[n][slot] = F_c     (rfx/simulation.py:1565-1567) */
+
+typedef struct {
+  int64_t cell;    /* linear index (i*ny + j)*nz + k */
+  int32_t kind;    /* RFXH_OP_* */
+  int32_t comp;    /* 0..5 = ex, ey, ez, hx, hy, hz */
+  int32_t slot;    /* column in the source table / probe output / port output */
+  int32_t order;   /* execution order within one cell (ascending) */
+  float d_par;     /* port: length along the port component */
+  float loop[3];   /* port: per-axis length multiplying each H difference */
+} rfxh_point_op;
+
+typedef struct {
+  int32_t axis;        /* plane normal 0/1/2 */
+  int32_t index;       /* plane index along the normal */
+  int32_t comp;        /* 0..5 */
+  int32_t n_freq;
+  const double* freqs; /* host, n_freq Hz */
+  double* d_acc;       /* device complex128 [n_freq][n1][n2] (re, im interleaved) */
+} rfxh_plane;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t graded;          /* 0: uniform spacing; 1: graded (per-index inverse spacings) */
+  const float* inv_e[3];   /* host, length n_axis: E-update inverse spacings
+                              (uniform: 1/dx; graded: rfx/nonuniform.py:212-224 inv_d_e) */
+  const float* inv_h[3];   /* host: H-update inverse spacings (graded: inv_d_h, last = 0) */
+  double dt;
+  /* materials: id per cell + per-id coefficient table */
+  const uint16_t* d_mat;   /* device [nx*ny*nz] */
+  int32_t n_mat;
+  const float* mat_coef;   /* host [n_mat][4] = ca, cb, ce (= dt/(eps_r eps0)), ch (= dt/(mu_r mu0)) */
+  const uint32_t* mat_flags; /* host [n_mat] RFXH_ZERO_* | RFXH_POINT */
+  /* CPML, faces ordered x_lo, x_hi, y_lo, y_hi, z_lo, z_hi */
+  int32_t cpml_n[6];       /* slab depth per face (0 = no CPML on that face) */
+  const float* cpml_b[6];  /* host [cpml_n[f]] in slab-local order (index 0 = slab's first cell) */
+  const float* cpml_c[6];
+  const float* cpml_km1[6];/* 1/kappa - 1 */
+  float cpml_inv_d[6];     /* 1 / face cell size */
+  int32_t pec_axes;        /* bit a: zero tangential E on both faces of axis a (rfx/boundaries/pec.py:13-48) */
+  /* state (device, caller-owned) */
+  float* d_f[6];           /* ex, ey, ez, hx, hy, hz */
+  float* d_psi;            /* CPML memory, rfxh_psi_floats() floats, zeroed by the caller */
+  /* point operations */
+  int32_t n_ops;
+  const rfxh_point_op* ops;/* host, any order (sorted internally by cell, order) */
+  int32_t n_src;
+  const float* d_src_wave; /* device [max_steps][n_src] */
+  int32_t n_probe;
+  float* d_probe_out;      /* device [max_steps][n_probe] */
+  int32_t n_port;
+  float* d_port_out;       /* device [max_steps][n_port][2] = (V, I) */
+  int64_t max_steps;       /* rows of the tables above */
+  /* DFT planes (running sums in float64) */
+  int32_t n_plane;
+  const rfxh_plane* planes;
+  int32_t plane_t_offset;  /* t = (n + offset) dt: 1 uniform lane (rfx/simulation.py:1575),
+                              0 graded lane (rfx/nonuniform.py:1458-1476) */
+} rfxh_desc;
+
+typedef struct rfxh_plan rfxh_plan;
+
+/* Number of floats of CPML memory the descriptor needs. */
+int64_t rfxh_psi_floats(const rfxh_desc* desc);
+
+/* Validate the descriptor, copy host tables to device, build the kernels'
+ * parameter block.  Replaces the set-up half of rfx/simulation.py:626-1086. */
+int rfxh_plan_create(const rfxh_desc* desc, rfxh_plan** out);
+int rfxh_plan_destroy(rfxh_plan* plan);
+
+/* Advance the fields by n_steps time steps starting at absolute step step0
+ * (rows step0 .. step0+n_steps-1 of the source/probe/port tables).  The loop
+ * body is rfx/simulation.py:1256-1681 (uniform) or rfx/nonuniform.py:
+ * 1256-1565 (graded).  use_graph != 0 replays chunks of the step sequence
+ * from a captured hipGraph. */
+int rfxh_run(rfxh_plan* plan, int64_t step0, int64_t n_steps, void* stream, int use_graph);
+
+/* Average duration in milliseconds of each kernel of the last rfxh_run made
+ * with timing enabled (rfxh_set_timing(plan, 1)), measured with HIP events on
+ * the stream the kernels run on.  which: 0 = H pass, 1 = E pass, 2 = DFT. */
+int rfxh_set_timing(rfxh_plan* plan, int enable);
+int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* count);
+
+const char* rfxh_last_error(void);
+int rfxh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -246,13 +246,16 @@ def assemble_materials(shape, coords, geometry, materials, cpml_pad):
     pec = np.zeros(shape, bool)
     for lo, hi, name in geometry:
         e, s, m = materials[name]
-        mask = box_mask(coords, lo, hi)
+        # box_mask is an outer product of per-axis masks, so the reference's
+        # jnp.where(mask, value, array) equals assignment on the index box
+        sel = np.ix_(*[np.nonzero(_axis_mask(coords[a], lo[a], hi[a]))[0]
+                       for a in range(3)])
         if s >= PEC_SIGMA_THRESHOLD:
-            pec |= mask
+            pec[sel] = True
         else:
-            eps_r = np.where(mask, F32(e), eps_r)
-            sigma = np.where(mask, F32(s), sigma)
-            mu_r = np.where(mask, F32(m), mu_r)
+            eps_r[sel] = F32(e)
+            sigma[sel] = F32(s)
+            mu_r[sel] = F32(m)
     if cpml_pad > 0:
         eps_r, sigma, mu_r = extend_cpml_pad(eps_r, sigma, mu_r, cpml_pad)
     return eps_r, sigma, mu_r, (pec if pec.any() else None)
@@ -741,11 +744,10 @@ def build_uniform(spec, n_steps):
             profiles[(ax, 0)] = (b, c, k)
             profiles[(ax, 1)] = (b[::-1].copy(), c[::-1].copy(), k[::-1].copy())
             face_d[ax] = (g.dx, g.dx)
-    if lane == "run":
-        pec_axes = "xyz"
-    else:
-        cpml_axes = "xyz" if layers > 0 else ""
-        pec_axes = "".join(a for a in "xyz" if a not in cpml_axes)
+    # forward(): pec_axes = axes outside grid.cpml_axes, and Grid keeps
+    # cpml_axes = "xyz" even with zero layers (rfx/grid.py:86), so forward()
+    # applies no outer-face PEC on either boundary kind.
+    pec_axes = "xyz" if lane == "run" else ""
     cfg = StepConfig(shape=shape, dt=dt, spacing=g.spacing(), eps_r=eps_r,
                      sigma=sigma, mu_r=mu_r, cpml_n=layers,
                      cpml_axes="xyz" if layers > 0 else "",

@@ -1,0 +1,414 @@
+"""``Simulation`` facade: the reference's user API over the HIP backend.
+
+Mirrors rfx/api/__init__.py:280-2600 (builder), rfx/api/_compile.py:60-367
+(grid and materials), rfx/runners/uniform.py:140-760 (run lane) and
+rfx/api/_execute.py:917-1669, 2447-2858 (forward lane) for the FDTD
+time-stepping path: uniform and graded Yee grids, CPML, PEC faces and PEC
+geometry masks, soft sources, lumped ports, point probes, DFT planes and
+lumped-port S11.  Everything else the reference offers raises
+NotImplementedError instead of running a different algorithm.
+"""
+from __future__ import annotations
+
+import warnings
+from dataclasses import dataclass, field
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from .constants import EPS_0
+from .cpml import face_tables
+from .engine import DeviceRun, Problem
+from .geometry import Box
+from .grid import Grid
+from .materials import MaterialMap, tangential_edge_bits
+from .sources import GaussianPulse, port_sigma, sample_times
+
+F32 = np.float32
+PEC_SIGMA_THRESHOLD = 1e6  # rfx/api/_compile.py:118
+
+
+@dataclass
+class _Material:
+    eps_r: float = 1.0
+    sigma: float = 0.0
+    mu_r: float = 1.0
+
+
+@dataclass
+class _PortEntry:
+    position: tuple
+    component: str
+    impedance: float
+    waveform: GaussianPulse | None
+    extent: float | None = None
+    excite: bool = True
+
+
+@dataclass
+class _ProbeEntry:
+    position: tuple
+    component: str
+
+
+@dataclass
+class _DFTPlaneEntry:
+    name: str
+    axis: str
+    coordinate: float
+    component: str
+    freqs: np.ndarray | None
+    n_freqs: int
+
+
+class FDTDState(NamedTuple):
+    ex: torch.Tensor
+    ey: torch.Tensor
+    ez: torch.Tensor
+    hx: torch.Tensor
+    hy: torch.Tensor
+    hz: torch.Tensor
+    step: int
+
+
+@dataclass
+class DFTPlaneProbe:
+    """Frequency-domain plane accumulator (rfx/probes/probes.py:379-473)."""
+    axis: int
+    index: int
+    component: str
+    freqs: np.ndarray
+    accumulator: np.ndarray   # complex128 (n_freqs, n1, n2)
+
+
+@dataclass
+class Result:
+    """Run output (rfx/api/_spec.py:649-692 subset)."""
+    time_series: np.ndarray
+    state: FDTDState | None = None
+    dft_planes: dict = field(default_factory=dict)
+    s_params: np.ndarray | None = None
+    freqs: np.ndarray | None = None
+    grid: object = None
+    dt: float = 0.0
+
+
+@dataclass
+class ForwardResult:
+    """Differentiable forward output (rfx/api/_spec.py:997)."""
+    time_series: torch.Tensor
+    grid: object = None
+    s_params: np.ndarray | None = None
+    freqs: np.ndarray | None = None
+    dft_planes: dict | None = None
+
+
+def _cell_cb_f32(dt, eps_r, sigma):
+    """Cb at a cell in float32 (rfx/simulation.py:181-194)."""
+    eps = F32(eps_r) * F32(EPS_0)
+    loss = F32(sigma) * F32(dt) / (F32(2.0) * eps)
+    return (F32(dt) / eps) / (F32(1.0) + loss)
+
+
+def port_dft(samples: np.ndarray, freqs, dt: float, t_offset: int = 0):
+    """sum_n x[n] exp(-j 2 pi f (n + t_offset) dt) dt in float64."""
+    n = np.arange(samples.shape[0], dtype=np.float64) + t_offset
+    ph = np.exp(-1j * 2.0 * np.pi * np.outer(np.asarray(freqs, np.float64), n * dt)) * dt
+    return ph @ np.asarray(samples, np.float64)
+
+
+def extract_lumped_s11(v_dft, i_dft, z0=50.0):
+    """S11 = (V + Z0 I)/(V - Z0 I) (rfx/probes/probes.py:1239-1273)."""
+    den = v_dft - z0 * i_dft
+    den = np.where(np.abs(den) > 0.0, den, 1.0)
+    return (v_dft + z0 * i_dft) / den
+
+
+class Simulation:
+    """Builder for one FDTD problem (rfx/api/__init__.py:280)."""
+
+    def __init__(self, freq_max: float, domain, *, boundary: str = "cpml",
+                 cpml_layers: int = 16, cpml_kappa_max: float = 1.0, pec_faces=None,
+                 dx: float | None = None, mode: str = "3d", dz_profile=None,
+                 dx_profile=None, dy_profile=None, precision: str = "float32",
+                 solver: str = "yee", stencil_order: int = 2, device=None):
+        if boundary not in ("pec", "cpml"):
+            raise NotImplementedError(f"boundary={boundary!r} is not supported by the HIP backend "
+                                      "(supported: 'pec', 'cpml')")
+        if freq_max <= 0:
+            raise ValueError(f"freq_max must be positive, got {freq_max}")
+        if precision != "float32":
+            raise NotImplementedError("the HIP backend computes in float32 only")
+        if solver != "yee" or stencil_order != 2 or mode != "3d":
+            raise NotImplementedError("only the 3-D second-order Yee solver is supported")
+        if pec_faces:
+            raise NotImplementedError("per-face PEC boundaries are not supported")
+        if dx_profile is not None or dy_profile is not None:
+            raise NotImplementedError("graded x/y profiles are not supported (graded z is)")
+        self._freq_max = float(freq_max)
+        self._dz_profile = None if dz_profile is None else np.asarray(dz_profile, np.float64)
+        if self._dz_profile is not None:
+            domain = (domain[0], domain[1], float(np.sum(self._dz_profile)))
+        self._domain = tuple(float(d) for d in domain)
+        self._boundary = boundary
+        self._cpml_layers = int(cpml_layers) if boundary == "cpml" else 0
+        self._kappa_max = float(cpml_kappa_max)
+        self._dx = dx
+        self._materials: dict[str, _Material] = {}
+        self._geometry: list[tuple[Box, str]] = []
+        self._ports: list[_PortEntry] = []
+        self._probes: list[_ProbeEntry] = []
+        self._dft_planes: list[_DFTPlaneEntry] = []
+        self._device = device
+
+    # ---- builder --------------------------------------------------------------
+    def add_material(self, name: str, *, eps_r: float = 1.0, sigma: float = 0.0,
+                     mu_r: float = 1.0, debye_poles=None, lorentz_poles=None,
+                     chi3: float = 0.0):
+        if debye_poles or lorentz_poles or chi3:
+            raise NotImplementedError("dispersive / nonlinear materials are not supported")
+        self._materials[name] = _Material(float(eps_r), float(sigma), float(mu_r))
+        return self
+
+    def add(self, shape, *, material: str):
+        if not isinstance(shape, Box):
+            raise NotImplementedError("only Box shapes are supported")
+        if material not in self._materials:
+            raise KeyError(f"Unknown material {material!r}; register it with add_material()")
+        self._geometry.append((shape, material))
+        return self
+
+    def add_source(self, position, component: str = "ez", *, waveform=None,
+                   amplitude_kind: str | None = None):
+        if component not in ("ex", "ey", "ez"):
+            raise ValueError(f"component must be ex/ey/ez, got {component!r}")
+        if amplitude_kind is not None:
+            raise NotImplementedError("amplitude_kind is not supported (legacy meaning only)")
+        if waveform is None:
+            waveform = GaussianPulse(f0=self._freq_max / 2, bandwidth=0.8)
+        self._ports.append(_PortEntry(tuple(position), component, 0.0, waveform))
+        return self
+
+    def add_port(self, position, component: str = "ez", *, impedance: float = 50.0,
+                 waveform=None, extent: float | None = None, excite: bool = True,
+                 direction=None, reference_plane_cells=None):
+        if component not in ("ex", "ey", "ez"):
+            raise ValueError(f"component must be ex/ey/ez, got {component!r}")
+        if reference_plane_cells is not None:
+            raise NotImplementedError("reference-plane port waves are not supported")
+        if waveform is None and excite:
+            waveform = GaussianPulse(f0=self._freq_max / 2, bandwidth=0.8)
+        self._ports.append(_PortEntry(tuple(position), component, float(impedance), waveform,
+                                      extent, excite))
+        return self
+
+    def add_probe(self, position, component: str = "ez"):
+        if component not in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            raise ValueError(f"component must be a field name, got {component!r}")
+        self._probes.append(_ProbeEntry(tuple(position), component))
+        return self
+
+    def add_vector_probe(self, position):
+        for c in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            self.add_probe(position, c)
+        return self
+
+    def add_dft_plane_probe(self, *, axis: str, coordinate: float, component: str = "ez",
+                            freqs=None, n_freqs: int = 50, name: str | None = None):
+        if axis not in ("x", "y", "z"):
+            raise ValueError(f"axis must be 'x', 'y', or 'z', got {axis!r}")
+        if component not in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            raise ValueError(f"component must be a field name, got {component!r}")
+        a = "xyz".index(axis)
+        if coordinate < 0 or coordinate > self._domain[a]:
+            raise ValueError(f"coordinate {coordinate} m is outside the {axis}-domain")
+        if name is None:
+            name = f"{component}_{axis}_{len(self._dft_planes)}"
+        self._dft_planes.append(_DFTPlaneEntry(name, axis, float(coordinate), component,
+                                               None if freqs is None else np.asarray(freqs),
+                                               int(n_freqs)))
+        return self
+
+    # ---- set-up ---------------------------------------------------------------
+    def _build_grid(self):
+        if self._dz_profile is not None:
+            from .nonuniform import make_nonuniform_grid
+            return make_nonuniform_grid(self._domain[:2], self._dz_profile,
+                                        self._dx if self._dx is not None else
+                                        float(self._dz_profile.max()),
+                                        self._cpml_layers)
+        return Grid(self._freq_max, self._domain, self._dx, self._cpml_layers)
+
+    def _build_problem(self, n_steps: int, lane: str, eps_override=None,
+                       sigma_override=None, mu_r_override=None, s11_freqs=None):
+        grid = self._build_grid()
+        graded = self._dz_profile is not None
+        shape = grid.shape
+        mat = MaterialMap(shape)
+        coords = grid.node_coords()
+        pec = None
+        for box, name in self._geometry:
+            m = self._materials[name]
+            masks = box.axis_masks(coords)
+            if m.sigma >= PEC_SIGMA_THRESHOLD:
+                if pec is None:
+                    pec = np.zeros(shape, bool)
+                pec[np.ix_(*[np.nonzero(x)[0] for x in masks])] = True
+            else:
+                mat.fill_box(masks, m.eps_r, m.sigma, m.mu_r)
+        if self._cpml_layers > 0:
+            mat.extend_pads(grid.face_pads)
+        if lane == "forward" and (eps_override is not None or sigma_override is not None
+                                  or mu_r_override is not None):
+            mat.override(eps_override, sigma_override, mu_r_override)
+        dt = grid.dt
+        times = sample_times(n_steps, dt)
+        sources, ports = [], []
+        for pe in self._ports:
+            idx = grid.position_to_index(pe.position)
+            if pe.impedance == 0.0:
+                wave = pe.waveform(times)
+                if self._boundary == "cpml" or lane == "forward" or graded:
+                    if graded:
+                        from .nonuniform import current_source_scale
+                        scale = current_source_scale(grid, mat, idx, pe.component, dt)
+                    else:
+                        e, s, _, _ = mat.cell_props(idx)
+                        scale = _cell_cb_f32(dt, e, s)
+                    wave = (scale * wave).astype(F32)
+                sources.append((idx, pe.component, wave))
+                continue
+            if pe.extent is not None:
+                raise NotImplementedError("wire ports (extent=...) are not supported")
+            if graded:
+                raise NotImplementedError("single-cell lumped ports on a graded mesh")
+            sizes = (grid.dx, grid.dx, grid.dx)
+            sp = port_sigma(sizes, pe.component, pe.impedance)
+            e, s, mu, fl = mat.cell_props(idx)
+            mat.set_cell(idx, sig=F32(s + F32(sp)))
+            if pec is not None:
+                pec[idx] = False
+            if pe.excite:
+                e, s, _, _ = mat.cell_props(idx)
+                cb = _cell_cb_f32(dt, e, s)
+                wave = ((cb / F32(grid.dx)) * pe.waveform(times)).astype(F32)
+                sources.append((idx, pe.component, wave))
+            if s11_freqs is not None:
+                ports.append((idx, pe.component, float(grid.dx), (grid.dx,) * 3, pe.impedance))
+        if pec is not None:
+            mat.add_flags(tangential_edge_bits(pec))
+        probes = [(grid.position_to_index(p.position), p.component) for p in self._probes]
+        if not probes and lane == "forward" and self._ports:
+            probes = [(grid.position_to_index(p.position), p.component) for p in self._ports]
+        planes = []
+        for pe in self._dft_planes:
+            a = "xyz".index(pe.axis)
+            pos = [0.0, 0.0, 0.0]
+            pos[a] = pe.coordinate
+            index = grid.position_to_index(tuple(pos))[a]
+            freqs = pe.freqs if pe.freqs is not None else np.linspace(
+                self._freq_max / 10, self._freq_max, pe.n_freqs).astype(F32)
+            planes.append((a, index, pe.component, np.asarray(freqs, np.float64)))
+        inv_e, inv_h = grid.inverse_spacings()
+        faces = face_tables(self._cpml_layers, dt, grid.face_cell_sizes(), self._kappa_max,
+                            "xyz" if self._cpml_layers > 0 else "")
+        # run(): outer-face PEC on every axis (runners/uniform.py:321 passes
+        # pec_axes=None, simulation.py:684-691).  forward(): pec_axes = axes
+        # outside grid.cpml_axes (_execute.py:1420-1421); Grid keeps
+        # cpml_axes="xyz" even with zero layers, so forward() applies none.
+        # The graded lane always applies it (nonuniform.py:1319).
+        pec_axes = "xyz" if (graded or lane == "run") else ""
+        prob = Problem(shape=shape, dt=dt, graded=graded, inv_e=inv_e, inv_h=inv_h, mat=mat,
+                       faces=faces, pec_axes=pec_axes, sources=sources, probes=probes,
+                       ports=[p[:4] for p in ports], planes=planes,
+                       record_order="graded" if graded else "uniform")
+        return grid, prob, ports
+
+    # ---- run / forward -----------------------------------------------------------
+    def run(self, n_steps: int | None = None, *, num_periods: float = 20.0,
+            use_graph: bool = True, **unsupported):
+        """Run the time loop (rfx/api/_execute.py:2862 -> runners/uniform.py:140)."""
+        _reject(unsupported, "run")
+        grid = self._build_grid()
+        if n_steps is None:
+            n_steps = grid.num_timesteps(num_periods)
+        if any(p.impedance != 0.0 for p in self._ports):
+            warnings.warn("run(): lumped-port S-parameters come from forward("
+                          "port_s11_freqs=...); run() returns no s_params", stacklevel=2)
+        grid, prob, _ = self._build_problem(int(n_steps), "run")
+        dr = DeviceRun(prob, int(n_steps), self._device)
+        dr.run(0, int(n_steps), use_graph=use_graph)
+        torch.cuda.synchronize(dr.device)
+        ts = dr.time_series().cpu().numpy().astype(np.float32)
+        planes = {}
+        for entry, (a, idx, comp, freqs), acc in zip(self._dft_planes, prob.planes, dr.plane_acc):
+            planes[entry.name] = DFTPlaneProbe(a, int(idx), comp, np.asarray(freqs),
+                                               acc.cpu().numpy())
+        st = FDTDState(*[f for f in dr.fields], step=int(n_steps))
+        _warn_if_nonfinite(ts, "run")
+        return Result(time_series=ts, state=st, dft_planes=planes, grid=grid, dt=grid.dt)
+
+    def forward(self, *, eps_override=None, sigma_override=None, mu_r_override=None,
+                n_steps: int | None = None, num_periods: float = 20.0, checkpoint: bool = True,
+                checkpoint_segments: int | None = None, port_s11_freqs=None,
+                skip_preflight: bool = False, **unsupported):
+        """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
+        PEC on axes covered by CPML (_execute.py:1420-1421); the returned
+        time series carries gradients w.r.t. ``eps_override`` through the
+        hand-written adjoint."""
+        _reject(unsupported, "forward")
+        grid = self._build_grid()
+        if n_steps is None:
+            n_steps = grid.num_timesteps(num_periods)
+        n_steps = int(n_steps)
+        s11 = None if port_s11_freqs is None else np.asarray(port_s11_freqs, F32)
+        needs_grad = isinstance(eps_override, torch.Tensor) and eps_override.requires_grad
+        eps_np = _to_np(eps_override)
+        grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
+                                                _to_np(sigma_override), _to_np(mu_r_override),
+                                                s11)
+        if needs_grad:
+            from .adjoint import forward_with_grad
+            ts, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
+                                       checkpoint_segments)
+        else:
+            dr = DeviceRun(prob, n_steps, self._device)
+            dr.run(0, n_steps)
+            ts = dr.time_series().clone()
+        s_params = freqs = None
+        if ports:
+            torch.cuda.synchronize(dr.device)
+            samp = dr.port_samples().cpu().numpy().astype(np.float64)
+            s_list = []
+            for q, p in enumerate(ports):
+                v = port_dft(samp[:, q, 0], s11, grid.dt)
+                i = port_dft(samp[:, q, 1], s11, grid.dt)
+                s_list.append(extract_lumped_s11(v, i, p[4]))
+            s_params = s_list[0] if len(s_list) == 1 else np.stack(s_list)
+            freqs = s11
+        planes = None
+        if self._dft_planes:
+            planes = {e.name: DFTPlaneProbe(a, int(idx), comp, np.asarray(fr), acc.cpu().numpy())
+                      for e, (a, idx, comp, fr), acc in zip(self._dft_planes, prob.planes,
+                                                            dr.plane_acc)}
+        return ForwardResult(time_series=ts, grid=grid, s_params=s_params, freqs=freqs,
+                             dft_planes=planes)
+
+
+def _to_np(x):
+    if x is None:
+        return None
+    if isinstance(x, torch.Tensor):
+        return x.detach().to("cpu", torch.float32).numpy()
+    return np.asarray(x, F32)
+
+
+def _reject(kw, where):
+    if kw:
+        raise NotImplementedError(f"{where}(): unsupported arguments {sorted(kw)}")
+
+
+def _warn_if_nonfinite(ts, context):
+    if ts.size and not np.all(np.isfinite(ts)):
+        warnings.warn(f"{context}: non-finite values in the probe time series", stacklevel=3)

@@ -1,0 +1,171 @@
+"""Per-cell material ids and the coefficient table the kernels read.
+
+The reference keeps three float32 arrays (eps_r, sigma, mu_r) per cell and
+recomputes ca/cb every step (rfx/core/yee.py:386-393).  Here every cell
+holds a uint16 id into a table of distinct (eps_r, sigma, mu_r, flags)
+entries; the table carries the precomputed float32 coefficients
+  ca = (1 - L)/(1 + L), cb = (dt/eps)/(1 + L), L = sigma dt / (2 eps)
+  ce = dt/(eps_r eps0)   (CPML E correction, rfx/boundaries/cpml.py:538-546)
+  ch = dt/(mu_r mu0)     (H update and CPML H correction, yee.py:272, cpml.py:817)
+computed with the reference's float32 operation order.  Flags mark cells
+whose tangential E is zeroed by the PEC mask (rfx/boundaries/pec.py:90-208)
+and cells that carry point operations (sources, probes, ports).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .constants import EPS_0, MU_0
+
+F32 = np.float32
+MAX_IDS = 65536
+
+
+class MaterialMap:
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.ids = np.zeros(self.shape, np.uint16)
+        self.table = []           # (eps_r, sigma, mu_r, flags)
+        self._lookup = {}
+        self.get_id(1.0, 0.0, 1.0, 0)
+
+    # -- table -------------------------------------------------------------
+    def get_id(self, eps, sig, mu, flags=0) -> int:
+        key = (F32(eps).tobytes(), F32(sig).tobytes(), F32(mu).tobytes(), int(flags))
+        mid = self._lookup.get(key)
+        if mid is None:
+            mid = len(self.table)
+            if mid >= MAX_IDS:
+                raise ValueError(
+                    f"more than {MAX_IDS} distinct (eps_r, sigma, mu_r) combinations; "
+                    "the material-id layout cannot represent this problem")
+            self.table.append((F32(eps), F32(sig), F32(mu), int(flags)))
+            self._lookup[key] = mid
+        return mid
+
+    def props(self):
+        t = self.table
+        eps = np.array([e[0] for e in t], F32)
+        sig = np.array([e[1] for e in t], F32)
+        mu = np.array([e[2] for e in t], F32)
+        flg = np.array([e[3] for e in t], np.uint32)
+        return eps, sig, mu, flg
+
+    # -- geometry ------------------------------------------------------------
+    def fill_box(self, axis_masks, eps, sig, mu):
+        """Insertion-order fill: later shapes win (rfx/api/_compile.py:206-224)."""
+        idx = [np.nonzero(m)[0] for m in axis_masks]
+        if any(len(a) == 0 for a in idx):
+            return
+        mid = self.get_id(eps, sig, mu, 0)
+        self.ids[np.ix_(*idx)] = mid
+
+    def extend_pads(self, pads):
+        """rfx/geometry/rasterize_grid.py:237-414 on ids.
+
+        pads = (lo_x, hi_x, lo_y, hi_y, lo_z, hi_z).  Lo faces replicate the
+        first interior slice outward; hi faces replicate the last interior
+        slice, except where that slice is vacuum and the one inside it is
+        not: there the inner slice is used and also written onto the last
+        interior slice (the node the half-open box rule dropped).
+        """
+        eps, sig, mu, _ = self.props()
+        for ax in range(3):
+            plo, phi = pads[2 * ax], pads[2 * ax + 1]
+            n = self.shape[ax]
+            ids = np.moveaxis(self.ids, ax, 0)  # view
+            if plo > 0:
+                ids[:plo] = ids[plo:plo + 1]
+            if phi > 0:
+                outer = ids[n - phi - 1].copy()
+                src = outer
+                if n - plo - phi >= 2:
+                    inner = ids[n - phi - 2]
+                    vac_o = (eps[outer] == 1.0) & (sig[outer] == 0.0) & (mu[outer] == 1.0)
+                    vac_i = (eps[inner] == 1.0) & (sig[inner] == 0.0) & (mu[inner] == 1.0)
+                    src = np.where(vac_o & ~vac_i, inner, outer)
+                    ids[n - phi - 1] = src
+                ids[n - phi:] = src
+
+    # -- per-cell edits ------------------------------------------------------
+    def cell_props(self, idx):
+        e, s, m, f = self.table[int(self.ids[idx])]
+        return e, s, m, f
+
+    def set_cell(self, idx, eps=None, sig=None, mu=None, add_flags=0):
+        e, s, m, f = self.cell_props(idx)
+        self.ids[idx] = self.get_id(e if eps is None else eps, s if sig is None else sig,
+                                    m if mu is None else mu, f | add_flags)
+
+    def add_flags(self, cell_mask_bits: np.ndarray):
+        """OR per-cell flag bits (uint8 array of the grid shape) into the ids."""
+        where = np.nonzero(cell_mask_bits)
+        if len(where[0]) == 0:
+            return
+        cur = self.ids[where].astype(np.int64)
+        bits = cell_mask_bits[where].astype(np.int64)
+        key = cur * 256 + bits
+        uniq, inv = np.unique(key, return_inverse=True)
+        new = np.empty(len(uniq), np.uint16)
+        for q, kv in enumerate(uniq):
+            e, s, m, f = self.table[int(kv // 256)]
+            new[q] = self.get_id(e, s, m, f | int(kv % 256))
+        self.ids[where] = new[inv]
+
+    def override(self, eps_r=None, sigma=None, mu_r=None):
+        """Replace per-cell material values wholesale (forward(eps_override=...),
+        rfx/api/_execute.py:2800-2805), keeping each cell's flags."""
+        eps, sig, mu, flg = self.props()
+        cur = self.ids
+        E = eps[cur] if eps_r is None else np.asarray(eps_r, F32).reshape(self.shape)
+        S = sig[cur] if sigma is None else np.asarray(sigma, F32).reshape(self.shape)
+        M = mu[cur] if mu_r is None else np.asarray(mu_r, F32).reshape(self.shape)
+        Fl = flg[cur]
+        key = np.stack([E.view(np.uint32).ravel(), S.view(np.uint32).ravel(),
+                        M.view(np.uint32).ravel(), Fl.ravel()], axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        if len(uniq) > MAX_IDS:
+            raise ValueError(
+                f"override produces {len(uniq)} distinct materials (max {MAX_IDS})")
+        self.table, self._lookup = [], {}
+        self.get_id(1.0, 0.0, 1.0, 0)
+        new = np.empty(len(uniq), np.uint16)
+        for q, row in enumerate(uniq):
+            new[q] = self.get_id(np.uint32(row[0]).view(F32), np.uint32(row[1]).view(F32),
+                                 np.uint32(row[2]).view(F32), int(row[3]))
+        self.ids = new[inv.ravel()].reshape(self.shape)
+
+    def cell_arrays(self):
+        """Dense float32 eps_r, sigma, mu_r (for small grids and tests)."""
+        eps, sig, mu, _ = self.props()
+        return eps[self.ids], sig[self.ids], mu[self.ids]
+
+    # -- kernel table ----------------------------------------------------------
+    def coefficient_table(self, dt: float):
+        eps_r, sig, mu_r, flg = self.props()
+        dt32 = F32(dt)
+        eps = eps_r * F32(EPS_0)
+        loss = sig * dt32 / (F32(2.0) * eps)
+        ca = (F32(1.0) - loss) / (F32(1.0) + loss)
+        cb = (dt32 / eps) / (F32(1.0) + loss)
+        ce = dt32 / (eps_r * F32(EPS_0))
+        ch = dt32 / (mu_r * F32(MU_0))
+        coef = np.stack([ca, cb, ce, ch], axis=1).astype(F32)
+        return np.ascontiguousarray(coef), np.ascontiguousarray(flg)
+
+
+def tangential_edge_bits(pec_mask: np.ndarray) -> np.ndarray:
+    """uint8 bits 1/2/4 = Ex/Ey/Ez zeroed (rfx/boundaries/pec.py:90-176):
+    a component is zeroed at a PEC cell iff a neighbour along that
+    component's axis is PEC too (zero padding past the domain)."""
+    bits = np.zeros(pec_mask.shape, np.uint8)
+    for ax in range(3):
+        nb = np.zeros_like(pec_mask)
+        sl_a = [slice(None)] * 3
+        sl_b = [slice(None)] * 3
+        sl_a[ax] = slice(1, None)
+        sl_b[ax] = slice(0, -1)
+        nb[tuple(sl_a)] |= pec_mask[tuple(sl_b)]
+        nb[tuple(sl_b)] |= pec_mask[tuple(sl_a)]
+        bits |= ((pec_mask & nb).astype(np.uint8) << ax)
+    return bits

@@ -78,3 +78,38 @@ def mixed_case(lane="run"):
                         ((0.003, 0.006, 0.005), "ez"), ((0.006, 0.007, 0.002), "hz")],
                 dft_planes=[("z", 0.005, "ez", None, 4), ("x", 0.008, "hy", None, 3)],
                 s11_freqs=np.array([5e9, 7e9, 9e9]), lane=lane)
+
+
+def to_sim(spec, device=None):
+    """Build an rfx_amd.Simulation from an oracle spec (same registration order)."""
+    import rfx_amd as R
+    kw = {}
+    if device is not None:
+        kw["device"] = device
+    sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec.get("dx"),
+                       boundary=spec.get("boundary", "cpml"),
+                       cpml_layers=spec.get("cpml_layers", 16),
+                       cpml_kappa_max=spec.get("kappa_max", 1.0), **kw)
+    for name, (e, s, m) in spec.get("materials", {}).items():
+        sim.add_material(name, eps_r=e, sigma=s, mu_r=m)
+    for lo, hi, name in spec.get("geometry", []):
+        sim.add(R.Box(lo, hi), material=name)
+    for pos, comp, z0, pulse, excite in spec.get("ports", []):
+        wf = None if pulse is None else R.GaussianPulse(pulse.f0, pulse.bandwidth,
+                                                         pulse.amplitude, pulse.cutoff)
+        if z0 == 0.0:
+            sim.add_source(pos, comp, waveform=wf)
+        else:
+            sim.add_port(pos, comp, impedance=z0, waveform=wf, excite=excite)
+    for pos, comp in spec.get("probes", []):
+        sim.add_probe(pos, comp)
+    for axis, coord, comp, freqs, nf in spec.get("dft_planes", []):
+        sim.add_dft_plane_probe(axis=axis, coordinate=coord, component=comp, freqs=freqs,
+                                n_freqs=nf)
+    return sim
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))

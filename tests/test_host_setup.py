@@ -1,0 +1,74 @@
+"""CPU checks of the product's host-side set-up against the oracle's:
+material assembly (ids -> eps/sigma/mu), PEC tangential bits, source
+tables, probe cells and CPML tables.  No device calls."""
+import numpy as np
+import pytest
+
+from oracle import fdtd_np as o
+from tests import cases
+
+
+def _setups(spec, n, lane):
+    sim = cases.to_sim(spec)
+    grid, prob, ports = sim._build_problem(n, lane, spec.get("eps_override"), None, None,
+                                           spec.get("s11_freqs"))
+    g, cfg = o.build_uniform(spec, n)
+    return grid, prob, ports, g, cfg
+
+
+@pytest.mark.parametrize("name,lane", [("quickstart", "run"), ("mixed", "run"),
+                                       ("mixed", "forward"), ("s11", "forward")])
+def test_materials_sources_match_oracle(name, lane):
+    spec = {"quickstart": cases.quickstart_case, "mixed": lambda: cases.mixed_case(lane),
+            "s11": lambda: dict(cases.S11_CASE)}[name]()
+    n = 64
+    grid, prob, ports, g, cfg = _setups(spec, n, lane)
+    assert grid.shape == g.shape and abs(grid.dt - g.dt) == 0.0
+    eps, sig, mu = prob.mat.cell_arrays()
+    np.testing.assert_array_equal(eps, cfg.eps_r)
+    np.testing.assert_array_equal(sig, cfg.sigma)
+    np.testing.assert_array_equal(mu, cfg.mu_r)
+    # PEC tangential bits
+    _, _, _, flg = prob.mat.props()
+    bits = flg[prob.mat.ids] & 7
+    if cfg.pec_mask is None:
+        assert not bits.any()
+    else:
+        mx, my, mz = o.tangential_edge_masks(cfg.pec_mask)
+        np.testing.assert_array_equal(bits & 1, mx.astype(np.uint32))
+        np.testing.assert_array_equal((bits >> 1) & 1, my.astype(np.uint32))
+        np.testing.assert_array_equal((bits >> 2) & 1, mz.astype(np.uint32))
+    assert prob.pec_axes == cfg.pec_axes
+    assert len(prob.sources) == len(cfg.sources)
+    for (i1, c1, w1), (i2, c2, w2) in zip(prob.sources, cfg.sources):
+        assert tuple(i1) == tuple(i2) and c1 == c2
+        np.testing.assert_array_equal(w1[:n], w2[:n])
+    assert [tuple(p[0]) for p in prob.probes] == [tuple(p[0]) for p in cfg.probes]
+    assert [(p[0], p[1]) for p in prob.planes] == [(p[0], p[1]) for p in cfg.dft_planes]
+    for f, (n_l, b, c, km1, inv_d) in enumerate(prob.faces):
+        ax, side = f // 2, f % 2
+        if cfg.cpml_n == 0:
+            assert n_l == 0
+            continue
+        ob, oc, ok = cfg.cpml_profiles[(ax, side)]
+        np.testing.assert_array_equal(b, ob)
+        np.testing.assert_array_equal(c, oc)
+        np.testing.assert_allclose(km1, 1.0 / ok - 1.0, atol=1e-7)
+
+
+def test_coefficient_table_float32_order():
+    """ca, cb, ce, ch follow yee.py:386-393 and cpml.py:538-546/817 in float32."""
+    from rfx_amd.materials import MaterialMap
+    m = MaterialMap((4, 4, 4))
+    m.get_id(2.2, 0.01, 1.0)
+    m.get_id(4.4, 0.0, 3.0)
+    coef, flg = m.coefficient_table(3.8131e-12)
+    eps_r = np.array([1.0, 2.2, 4.4], np.float32)
+    sig = np.array([0.0, 0.01, 0.0], np.float32)
+    mu = np.array([1.0, 1.0, 3.0], np.float32)
+    dt = np.float32(3.8131e-12)
+    eps = eps_r * np.float32(o.EPS_0)
+    L = sig * dt / (np.float32(2) * eps)
+    np.testing.assert_array_equal(coef[:, 0], (1 - L) / (1 + L))
+    np.testing.assert_array_equal(coef[:, 1], (dt / eps) / (1 + L))
+    np.testing.assert_array_equal(coef[:, 3], dt / (mu * np.float32(o.MU_0)))
