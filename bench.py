@@ -122,6 +122,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--time-steps", type=int, default=None,
+                    help="time steps per run (default: the workload's own count); "
+                         "for short profiled runs")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 512 MiB device copy five times, the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -130,7 +136,7 @@ def main():
     import torch
     from rfx_amd.engine import DeviceRun
     spec = c2_spec() if args.config == "c2" else c1_spec()
-    n_total = N_STEPS[args.config]
+    n_total = args.time_steps or N_STEPS[args.config]
     t_setup = time.perf_counter()
     sim = build_sim(spec)
     grid, prob, _ = sim._build_problem(n_total, "run")
@@ -139,6 +145,15 @@ def main():
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s, grid {grid.shape}, "
           f"materials {len(prob.mat.table)}", file=sys.stderr, flush=True)
     cells = int(np.prod(grid.shape))
+    if args.copy_calib:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print("[bench] copy calibration: 5 x 512 MiB device copies", file=sys.stderr,
+              flush=True)
     for w in range(args.warmup):
         dr.reset()
         dr.run(0, n_total, use_graph=not args.no_graph)

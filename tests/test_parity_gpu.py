@@ -74,9 +74,17 @@ def test_mixed_case_vs_oracle(R, lane):
             e = cases.rel_l2(acc.view(np.float64), ref.dft_planes[q].view(np.float64))
             print("plane", name, e)
             assert e <= TOL
+        # final fields after 100 steps, while the pulse is still propagating:
+        # by 400 steps the remaining field is the static charge left by the
+        # sources plus an H residue ~1e-4 of E, where float32 itself (the
+        # C oracle in float32) sits at 2e-4 (E) and 1e-2 (H) of float64
+        n_f = 100
+        res_f = cases.to_sim(spec).run(n_steps=n_f)
+        g_f, cfg_f = o.build_uniform(spec, n_f)
+        ref_f = cstep.run_c(cfg_f, n_f)
         for c in o.COMPS:
-            got = getattr(res.state, c).cpu().numpy()
-            e = cases.rel_l2(got, ref.fields[c])
+            got = getattr(res_f.state, c).cpu().numpy()
+            e = cases.rel_l2(got, ref_f.fields[c])
             print("field", c, e)
             assert e <= TOL
     else:
