@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,7 +48,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kBX = 64;  // lanes along z (one wavefront)
-constexpr int kBY = 4;   // rows along y per workgroup
+constexpr int kTY = 8;   // rows along y per workgroup (one wavefront per row)
 constexpr int kTimingSamples = 64;  // sampled steps per timed run
 
 struct KParams {
@@ -78,6 +79,8 @@ struct KParams {
   float* port_out;
   int n_port;
   long long* step;               // current absolute step (device)
+  int ichunk;                    // planes per workgroup march
+  int order;                     // cell kernels: 0 flat chunks, 1 column tiles
 };
 
 struct PlaneParams {
@@ -104,24 +107,270 @@ __device__ __forceinline__ long long psi_idx(int face, int l, int i, int j, int 
 }
 
 // ---------------------------------------------------------------------------
-// H pass
+// Tile order.  Each workgroup owns a 64 (z) x kTY (y) tile of columns and
+// marches along x over a chunk [i0, i1) of planes, so the x neighbour of every
+// field comes from registers (loaded once), the y neighbour from the tile row
+// above/below through LDS, and the z neighbour from the adjacent lane.
+// Workgroups are dispatched round-robin over the 8 XCDs; xcd_remap hands each
+// XCD a contiguous range of logical tiles so the halo rows two neighbouring
+// tiles share are fetched into the same L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8;
+  const int x = b % 8, s = b / 8;
+  return x < rem ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s;
+}
+
+struct TileIdx {
+  int i0, i1, j0, k0;
+};
+
+__device__ __forceinline__ TileIdx tile_of(const KParams& p) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int nkb = (p.nz + kBX - 1) / kBX, njb = (p.ny + kTY - 1) / kTY;
+  const int kb = L % nkb;
+  const int t = L / nkb;
+  const int jb = t % njb;
+  const int ic = t / njb;
+  TileIdx r;
+  r.i0 = ic * p.ichunk;
+  r.i1 = min(p.nx, r.i0 + p.ichunk);
+  r.j0 = jb * kTY;
+  r.k0 = kb * kBX;
+  return r;
+}
+
+__device__ __forceinline__ float pick3(int c, float a, float b, float d) {
+  return c == 0 ? a : (c == 1 ? b : d);
+}
+
+// ---------------------------------------------------------------------------
+// H pass: H(i) = H(i) - ch curl_fwd E (yee.py:272-289) + CPML-H (cpml.py:875-1043)
 // ---------------------------------------------------------------------------
 template <bool GRADED>
-__global__ __launch_bounds__(kBX* kBY) void k_update_h(KParams p) {
-  const int nkb = (p.nz + kBX - 1) / kBX;
-  const int njb = (p.ny + kBY - 1) / kBY;
-  const int b = blockIdx.x;
-  const int kb = b % nkb;
-  const int t = b / nkb;
-  const int jb = t % njb;
-  const int i = t / njb;
-  if (b == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+__global__ __launch_bounds__(kBX* kTY) void k_h_march(KParams p) {
+  __shared__ float sEx[2][kTY + 1][kBX];
+  __shared__ float sEz[2][kTY + 1][kBX];
+  const TileIdx T = tile_of(p);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     // advance the step counter; the E pass and DFT kernels of this step read it
     *p.step = *p.step + 1;
   }
-  const int k = kb * kBX + threadIdx.x;
-  const int j = jb * kBY + threadIdx.y;
-  if (k >= p.nz || j >= p.ny || i >= p.nx) return;
+  const int lane = threadIdx.x, ty = threadIdx.y;
+  const int k = T.k0 + lane, j = T.j0 + ty;
+  const bool in = (k < p.nz) && (j < p.ny);
+  const long long P = p.plane;
+  const float* __restrict__ Ex = p.f[0];
+  const float* __restrict__ Ey = p.f[1];
+  const float* __restrict__ Ez = p.f[2];
+  float* __restrict__ Hx = p.f[3];
+  float* __restrict__ Hy = p.f[4];
+  float* __restrict__ Hz = p.f[5];
+  long long c = ((long long)T.i0 * p.ny + (in ? j : 0)) * p.nz + (in ? k : 0);
+  float ex0 = 0.f, ey0 = 0.f, ez0 = 0.f;
+  if (in) {
+    ex0 = Ex[c];
+    ey0 = Ey[c];
+    ez0 = Ez[c];
+  }
+  float iy = p.inv_h0, iz = p.inv_h0;
+  if (GRADED && in) {
+    iy = p.inv_h[1][j];
+    iz = p.inv_h[2][k];
+  }
+  const bool j_edge = (ty == kTY - 1) && in && (j + 1 < p.ny);
+  const bool k_edge = (lane == kBX - 1) && in && (k + 1 < p.nz);
+  for (int i = T.i0; i < T.i1; ++i, c += P) {
+    const int buf = (i - T.i0) & 1;
+    float ex1 = 0.f, ey1 = 0.f, ez1 = 0.f;
+    if (in && i + 1 < p.nx) {
+      ex1 = Ex[c + P];
+      ey1 = Ey[c + P];
+      ez1 = Ez[c + P];
+    }
+    sEx[buf][ty][lane] = ex0;
+    sEz[buf][ty][lane] = ez0;
+    if (ty == kTY - 1) {
+      float a = 0.f, b = 0.f;
+      if (j_edge) {
+        a = Ex[c + p.nz];
+        b = Ez[c + p.nz];
+      }
+      sEx[buf][kTY][lane] = a;
+      sEz[buf][kTY][lane] = b;
+    }
+    float ex_k = __shfl_down(ex0, 1), ey_k = __shfl_down(ey0, 1);
+    if (lane == kBX - 1) {
+      ex_k = k_edge ? Ex[c + 1] : 0.f;
+      ey_k = k_edge ? Ey[c + 1] : 0.f;
+    } else if (k + 1 >= p.nz) {
+      ex_k = ey_k = 0.f;
+    }
+    __syncthreads();
+    if (in) {
+      const float ex_j = sEx[buf][ty + 1][lane];
+      const float ez_j = sEz[buf][ty + 1][lane];
+      const float ix = GRADED ? p.inv_h[0][i] : p.inv_h0;
+      const float ch = p.coef[p.mat[c]].w;
+      float hx = Hx[c], hy = Hy[c], hz = Hz[c];
+      const float cx = (ez_j - ez0) * iy - (ey_k - ey0) * iz;
+      const float cy = (ex_k - ex0) * iz - (ez1 - ez0) * ix;
+      const float cz = (ey1 - ey0) * ix - (ex_j - ex0) * iy;
+      hx = hx - ch * cx;
+      hy = hy - ch * cy;
+      hz = hz - ch * cz;
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        const int n = p.cn[face];
+        if (n == 0) continue;
+        const int ax = face >> 1;
+        const int pos = ax == 0 ? i : ax == 1 ? j : k;
+        const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
+        const int l = pos - ((face & 1) ? N - n : 0);
+        if (l < 0 || l >= n) continue;
+        const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
+        const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+        const float inv = p.cinv[face];
+        float g0, g1;
+        if (ax == 0) {
+          g0 = (ez1 - ez0) * inv;   // Hy += ...
+          g1 = (ey1 - ey0) * inv;   // Hz -= ...
+        } else if (ax == 1) {
+          g0 = (ez_j - ez0) * inv;  // Hx -= ...
+          g1 = (ex_j - ex0) * inv;  // Hz += ...
+        } else {
+          g0 = (ey_k - ey0) * inv;  // Hx += ...
+          g1 = (ex_k - ex0) * inv;  // Hy -= ...
+        }
+        float* ps0 = p.psi_h[face][0];
+        float* ps1 = p.psi_h[face][1];
+        const float s0 = bb * ps0[q] + cc * g0;
+        const float s1 = bb * ps1[q] + cc * g1;
+        ps0[q] = s0;
+        ps1[q] = s1;
+        const float d0 = ch * s0 + ch * km1 * g0;
+        const float d1 = ch * s1 + ch * km1 * g1;
+        if (ax == 0) {
+          hy += d0;
+          hz -= d1;
+        } else if (ax == 1) {
+          hx -= d0;
+          hz += d1;
+        } else {
+          hx += d0;
+          hy -= d1;
+        }
+      }
+      Hx[c] = hx;
+      Hy[c] = hy;
+      Hz[c] = hz;
+    }
+    ex0 = ex1;
+    ey0 = ey1;
+    ez0 = ez1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// E pass: E = ca E + cb curl_bwd H (yee.py:379-404), CPML-E (cpml.py:608-777),
+// outer-face PEC (pec.py:24-46), PEC mask (pec.py:179-208), point operations
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void point_ops(const KParams& p, long long cell, float& e0, float& e1,
+                                          float& e2, float hx, float hy, float hz, float hx_j,
+                                          float hx_k, float hy_i, float hy_k, float hz_i,
+                                          float hz_j) {
+  // binary search the first op of this cell (ops sorted by cell, order)
+  int lo = 0, hi = p.n_ops;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (p.ops[mid].cell < cell) lo = mid + 1;
+    else hi = mid;
+  }
+  const long long n = *p.step;
+  for (int q = lo; q < p.n_ops && p.ops[q].cell == cell; ++q) {
+    const rfxh_point_op op = p.ops[q];
+    if (op.kind == RFXH_OP_PORT) {
+      const float v = -pick3(op.comp, e0, e1, e2) * op.d_par;
+      float cur;
+      if (op.comp == 2)
+        cur = (hy - hy_i) * op.loop[1] - (hx - hx_j) * op.loop[0];
+      else if (op.comp == 0)
+        cur = (hz - hz_j) * op.loop[2] - (hy - hy_k) * op.loop[1];
+      else
+        cur = (hx - hx_k) * op.loop[0] - (hz - hz_i) * op.loop[2];
+      float* o = p.port_out + (n * p.n_port + op.slot) * 2;
+      o[0] = v;
+      o[1] = cur;
+    } else if (op.kind == RFXH_OP_INJECT) {
+      const float w = p.src_wave[n * p.n_src + op.slot];
+      if (op.comp == 0) e0 += w;
+      else if (op.comp == 1) e1 += w;
+      else e2 += w;
+    } else {
+      p.probe_out[n * p.n_probe + op.slot] =
+          op.comp < 3 ? pick3(op.comp, e0, e1, e2) : pick3(op.comp - 3, hx, hy, hz);
+    }
+  }
+}
+
+// Cell-per-thread tile order: logical workgroup L covers plane i of column tile
+// (jb, kb) with i fastest, and each XCD receives a contiguous run of logical
+// ids, so the workgroups of planes i and i+1 of one column tile run back to
+// back on the same XCD and the x neighbours one loads are still in its L2.
+__device__ __forceinline__ bool cell_of(const KParams& p, int& i, int& j, int& k) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const long long pl = (long long)p.ny * p.nz;
+  if (p.order == 2) {
+    // bands: the (y, z) plane is split into 8 flat bands, one per XCD; the
+    // XCD sweeps its band plane by plane (512-cell chunks fastest), so the
+    // band's x neighbours stay in that XCD's L2 between consecutive planes
+    const long long bw = (pl + 7) / 8;
+    const int ncb = (int)((bw + kBX * kTY - 1) / (kBX * kTY));
+    const int b = L / (p.nx * ncb);
+    const int r = L % (p.nx * ncb);
+    i = r / ncb;
+    const long long f = (long long)b * bw + (long long)(r % ncb) * (kBX * kTY) +
+                        threadIdx.y * kBX + threadIdx.x;
+    if (f >= pl || f >= (long long)(b + 1) * bw) return false;
+    j = (int)(f / p.nz);
+    k = (int)(f % p.nz);
+    return true;
+  }
+  if (p.order == 0) {
+    // flat chunks: workgroup L covers 512 consecutive cells of the (y, z)
+    // plane of plane i (chunk index fastest), so neighbouring workgroups
+    // share only the cache line at their common boundary
+    const int nch = (int)((pl + kBX * kTY - 1) / (kBX * kTY));
+    const int ch = L % nch;
+    i = L / nch;
+    const long long f = (long long)ch * (kBX * kTY) + threadIdx.y * kBX + threadIdx.x;
+    if (f >= pl) return false;
+    j = (int)(f / p.nz);
+    k = (int)(f % p.nz);
+    return true;
+  }
+  const int nkb = (p.nz + kBX - 1) / kBX;
+  i = L % p.nx;
+  const int t = L / p.nx;
+  const int kb = t % nkb;
+  const int jb = t / nkb;
+  k = kb * kBX + threadIdx.x;
+  j = jb * kTY + threadIdx.y;
+  return k < p.nz && j < p.ny;
+}
+
+// ---------------------------------------------------------------------------
+// Cell-per-thread passes (one plane per workgroup)
+// ---------------------------------------------------------------------------
+template <bool GRADED>
+__global__ __launch_bounds__(kBX* kTY) void k_update_h(KParams p) {
+  int i, j, k;
+  const bool ok = cell_of(p, i, j, k);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    // advance the step counter; the E pass and DFT kernels of this step read it
+    *p.step = *p.step + 1;
+  }
+  if (!ok) return;
   const long long P = p.plane;
   const long long c = ((long long)i * p.ny + j) * p.nz + k;
   const float* __restrict__ Ex = p.f[0];
@@ -205,51 +454,10 @@ __global__ __launch_bounds__(kBX* kBY) void k_update_h(KParams p) {
 // ---------------------------------------------------------------------------
 // E pass
 // ---------------------------------------------------------------------------
-__device__ void point_ops(const KParams& p, long long c, float e[3], const float h[3],
-                          float hx_j, float hx_k, float hy_i, float hy_k, float hz_i,
-                          float hz_j) {
-  // binary search the first op of this cell (ops sorted by cell, order)
-  int lo = 0, hi = p.n_ops;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (p.ops[mid].cell < c) lo = mid + 1;
-    else hi = mid;
-  }
-  const long long n = *p.step;
-  for (int q = lo; q < p.n_ops && p.ops[q].cell == c; ++q) {
-    const rfxh_point_op op = p.ops[q];
-    if (op.kind == RFXH_OP_PORT) {
-      const float v = -e[op.comp] * op.d_par;
-      float cur;
-      if (op.comp == 2)
-        cur = (h[1] - hy_i) * op.loop[1] - (h[0] - hx_j) * op.loop[0];
-      else if (op.comp == 0)
-        cur = (h[2] - hz_j) * op.loop[2] - (h[1] - hy_k) * op.loop[1];
-      else
-        cur = (h[0] - hx_k) * op.loop[0] - (h[2] - hz_i) * op.loop[2];
-      float* o = p.port_out + (n * p.n_port + op.slot) * 2;
-      o[0] = v;
-      o[1] = cur;
-    } else if (op.kind == RFXH_OP_INJECT) {
-      e[op.comp] += p.src_wave[n * p.n_src + op.slot];
-    } else {
-      p.probe_out[n * p.n_probe + op.slot] = op.comp < 3 ? e[op.comp] : h[op.comp - 3];
-    }
-  }
-}
-
 template <bool GRADED>
-__global__ __launch_bounds__(kBX* kBY) void k_update_e(KParams p) {
-  const int nkb = (p.nz + kBX - 1) / kBX;
-  const int njb = (p.ny + kBY - 1) / kBY;
-  const int b = blockIdx.x;
-  const int kb = b % nkb;
-  const int t = b / nkb;
-  const int jb = t % njb;
-  const int i = t / njb;
-  const int k = kb * kBX + threadIdx.x;
-  const int j = jb * kBY + threadIdx.y;
-  if (k >= p.nz || j >= p.ny || i >= p.nx) return;
+__global__ __launch_bounds__(kBX* kTY) void k_update_e(KParams p) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
   const long long P = p.plane;
   const long long c = ((long long)i * p.ny + j) * p.nz + k;
   const float* __restrict__ Hx = p.f[3];
@@ -273,14 +481,14 @@ __global__ __launch_bounds__(kBX* kBY) void k_update_e(KParams p) {
   const unsigned m = p.mat[c];
   const float4 co = p.coef[m];
   const float ca = co.x, cb = co.y, ce = co.z;
-  float e[3] = {p.f[0][c], p.f[1][c], p.f[2][c]};
+  float e0 = p.f[0][c], e1 = p.f[1][c], e2 = p.f[2][c];
   // yee.py:379-404: E = ca E + cb curl H (backward differences)
   const float cx = (hz - hz_j) * iy - (hy - hy_k) * iz;
   const float cy = (hx - hx_k) * iz - (hz - hz_i) * ix;
   const float cz = (hy - hy_i) * ix - (hx - hx_j) * iy;
-  e[0] = ca * e[0] + cb * cx;
-  e[1] = ca * e[1] + cb * cy;
-  e[2] = ca * e[2] + cb * cz;
+  e0 = ca * e0 + cb * cx;
+  e1 = ca * e1 + cb * cy;
+  e2 = ca * e2 + cb * cz;
   // CPML-E (cpml.py:608-777) with ce = dt/(eps_r eps0) (cpml.py:538-546)
 #pragma unroll
   for (int face = 0; face < 6; ++face) {
@@ -315,38 +523,388 @@ __global__ __launch_bounds__(kBX* kBY) void k_update_e(KParams p) {
     const float d0 = ce * s0 + ce * km1 * g0;
     const float d1 = ce * s1 + ce * km1 * g1;
     if (ax == 0) {
-      e[1] -= d0;
-      e[2] += d1;
+      e1 -= d0;
+      e2 += d1;
     } else if (ax == 1) {
-      e[0] += d0;
-      e[2] -= d1;
+      e0 += d0;
+      e2 -= d1;
     } else {
-      e[0] -= d0;
-      e[1] += d1;
+      e0 -= d0;
+      e1 += d1;
     }
   }
   // outer-face PEC (pec.py:24-46)
   if (p.pec_axes & 1) {
-    if (i == 0 || i == p.nx - 1) e[1] = e[2] = 0.f;
+    if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
   }
   if (p.pec_axes & 2) {
-    if (j == 0 || j == p.ny - 1) e[0] = e[2] = 0.f;
+    if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
   }
   if (p.pec_axes & 4) {
-    if (k == 0 || k == p.nz - 1) e[0] = e[1] = 0.f;
-    if (k == p.nz - 1) e[2] = 0.f;
+    if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
+    if (k == p.nz - 1) e2 = 0.f;
   }
   const uint32_t fl = p.flags[m];
-  if (fl & RFXH_ZERO_EX) e[0] = 0.f;
-  if (fl & RFXH_ZERO_EY) e[1] = 0.f;
-  if (fl & RFXH_ZERO_EZ) e[2] = 0.f;
-  if (fl & RFXH_POINT) {
-    const float h[3] = {hx, hy, hz};
-    point_ops(p, c, e, h, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
+  if (fl & RFXH_ZERO_EX) e0 = 0.f;
+  if (fl & RFXH_ZERO_EY) e1 = 0.f;
+  if (fl & RFXH_ZERO_EZ) e2 = 0.f;
+  if (fl & RFXH_POINT) point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
+  p.f[0][c] = e0;
+  p.f[1][c] = e1;
+  p.f[2][c] = e2;
+}
+
+// ---------------------------------------------------------------------------
+// Flat-chunk marching passes (default): workgroup = 512 consecutive cells of
+// the (y, z) plane, marching over a chunk of x planes.  The x neighbour is
+// carried in registers from the previous plane; y and z neighbours are plain
+// loads of cells the neighbouring lanes / workgroups fetch in the same
+// iteration (L1/L2 hits).  No LDS, no barriers: waves run independently.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool fm_cell(const KParams& p, int& i0, int& i1, int& j, int& k) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const long long pl = (long long)p.ny * p.nz;
+  const int nch = (int)((pl + kBX * kTY - 1) / (kBX * kTY));
+  const int ch = L % nch;
+  const int ic = L / nch;
+  i0 = ic * p.ichunk;
+  i1 = min(p.nx, i0 + p.ichunk);
+  const long long f = (long long)ch * (kBX * kTY) + threadIdx.y * kBX + threadIdx.x;
+  if (f >= pl) return false;
+  j = (int)(f / p.nz);
+  k = (int)(f % p.nz);
+  return true;
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kBX* kTY) void k_h_fm(KParams p) {
+  int i0, i1, j, k;
+  const bool ok = fm_cell(p, i0, i1, j, k);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    // advance the step counter; the E pass and DFT kernels of this step read it
+    *p.step = *p.step + 1;
   }
-  p.f[0][c] = e[0];
-  p.f[1][c] = e[1];
-  p.f[2][c] = e[2];
+  if (!ok) return;
+  const long long P = p.plane;
+  const float* __restrict__ Ex = p.f[0];
+  const float* __restrict__ Ey = p.f[1];
+  const float* __restrict__ Ez = p.f[2];
+  float* __restrict__ Hx = p.f[3];
+  float* __restrict__ Hy = p.f[4];
+  float* __restrict__ Hz = p.f[5];
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
+  float iy = p.inv_h0, iz = p.inv_h0;
+  if (GRADED) {
+    iy = p.inv_h[1][j];
+    iz = p.inv_h[2][k];
+  }
+  long long c = ((long long)i0 * p.ny + j) * p.nz + k;
+  float ex = Ex[c], ey = Ey[c], ez = Ez[c];
+  for (int i = i0; i < i1; ++i, c += P) {
+    const bool hi = i + 1 < p.nx;
+    const float ex_j = hj ? Ex[c + p.nz] : 0.f;
+    const float ez_j = hj ? Ez[c + p.nz] : 0.f;
+    const float ex_k = hk ? Ex[c + 1] : 0.f;
+    const float ey_k = hk ? Ey[c + 1] : 0.f;
+    const float ex_i = hi ? Ex[c + P] : 0.f;
+    const float ey_i = hi ? Ey[c + P] : 0.f;
+    const float ez_i = hi ? Ez[c + P] : 0.f;
+    const float ix = GRADED ? p.inv_h[0][i] : p.inv_h0;
+    const float ch = p.coef[p.mat[c]].w;
+    float hx = Hx[c], hy = Hy[c], hz = Hz[c];
+    const float cx = (ez_j - ez) * iy - (ey_k - ey) * iz;
+    const float cy = (ex_k - ex) * iz - (ez_i - ez) * ix;
+    const float cz = (ey_i - ey) * ix - (ex_j - ex) * iy;
+    hx = hx - ch * cx;
+    hy = hy - ch * cy;
+    hz = hz - ch * cz;
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int n = p.cn[face];
+      if (n == 0) continue;
+      const int ax = face >> 1;
+      const int pos = ax == 0 ? i : ax == 1 ? j : k;
+      const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
+      const int l = pos - ((face & 1) ? N - n : 0);
+      if (l < 0 || l >= n) continue;
+      const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
+      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      const float inv = p.cinv[face];
+      float g0, g1;
+      if (ax == 0) {
+        g0 = (ez_i - ez) * inv;   // Hy += ...
+        g1 = (ey_i - ey) * inv;   // Hz -= ...
+      } else if (ax == 1) {
+        g0 = (ez_j - ez) * inv;   // Hx -= ...
+        g1 = (ex_j - ex) * inv;   // Hz += ...
+      } else {
+        g0 = (ey_k - ey) * inv;   // Hx += ...
+        g1 = (ex_k - ex) * inv;   // Hy -= ...
+      }
+      float* ps0 = p.psi_h[face][0];
+      float* ps1 = p.psi_h[face][1];
+      const float s0 = bb * ps0[q] + cc * g0;
+      const float s1 = bb * ps1[q] + cc * g1;
+      ps0[q] = s0;
+      ps1[q] = s1;
+      const float d0 = ch * s0 + ch * km1 * g0;
+      const float d1 = ch * s1 + ch * km1 * g1;
+      if (ax == 0) {
+        hy += d0;
+        hz -= d1;
+      } else if (ax == 1) {
+        hx -= d0;
+        hz += d1;
+      } else {
+        hx += d0;
+        hy -= d1;
+      }
+    }
+    Hx[c] = hx;
+    Hy[c] = hy;
+    Hz[c] = hz;
+    ex = ex_i;
+    ey = ey_i;
+    ez = ez_i;
+  }
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kBX* kTY) void k_e_fm(KParams p) {
+  int i0, i1, j, k;
+  if (!fm_cell(p, i0, i1, j, k)) return;
+  const long long P = p.plane;
+  float* __restrict__ Ex = p.f[0];
+  float* __restrict__ Ey = p.f[1];
+  float* __restrict__ Ez = p.f[2];
+  const float* __restrict__ Hx = p.f[3];
+  const float* __restrict__ Hy = p.f[4];
+  const float* __restrict__ Hz = p.f[5];
+  const bool hj = j > 0, hk = k > 0;
+  float iy = p.inv_e0, iz = p.inv_e0;
+  if (GRADED) {
+    iy = p.inv_e[1][j];
+    iz = p.inv_e[2][k];
+  }
+  long long c = ((long long)i0 * p.ny + j) * p.nz + k;
+  float hy_i = 0.f, hz_i = 0.f;  // H at plane i-1
+  if (i0 > 0) {
+    hy_i = Hy[c - P];
+    hz_i = Hz[c - P];
+  }
+  for (int i = i0; i < i1; ++i, c += P) {
+    const float hx = Hx[c], hy = Hy[c], hz = Hz[c];
+    const float hx_j = hj ? Hx[c - p.nz] : 0.f;
+    const float hz_j = hj ? Hz[c - p.nz] : 0.f;
+    const float hx_k = hk ? Hx[c - 1] : 0.f;
+    const float hy_k = hk ? Hy[c - 1] : 0.f;
+    const float ix = GRADED ? p.inv_e[0][i] : p.inv_e0;
+    const unsigned m = p.mat[c];
+    const float4 co = p.coef[m];
+    const float ca = co.x, cb = co.y, ce = co.z;
+    float e0 = Ex[c], e1 = Ey[c], e2 = Ez[c];
+    const float cx = (hz - hz_j) * iy - (hy - hy_k) * iz;
+    const float cy = (hx - hx_k) * iz - (hz - hz_i) * ix;
+    const float cz = (hy - hy_i) * ix - (hx - hx_j) * iy;
+    e0 = ca * e0 + cb * cx;
+    e1 = ca * e1 + cb * cy;
+    e2 = ca * e2 + cb * cz;
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int n = p.cn[face];
+      if (n == 0) continue;
+      const int ax = face >> 1;
+      const int pos = ax == 0 ? i : ax == 1 ? j : k;
+      const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
+      const int l = pos - ((face & 1) ? N - n : 0);
+      if (l < 0 || l >= n) continue;
+      const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
+      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      const float inv = p.cinv[face];
+      float g0, g1;
+      if (ax == 0) {
+        g0 = (hz - hz_i) * inv;  // Ey -= ...
+        g1 = (hy - hy_i) * inv;  // Ez += ...
+      } else if (ax == 1) {
+        g0 = (hz - hz_j) * inv;  // Ex += ...
+        g1 = (hx - hx_j) * inv;  // Ez -= ...
+      } else {
+        g0 = (hy - hy_k) * inv;  // Ex -= ...
+        g1 = (hx - hx_k) * inv;  // Ey += ...
+      }
+      float* ps0 = p.psi_e[face][0];
+      float* ps1 = p.psi_e[face][1];
+      const float s0 = bb * ps0[q] + cc * g0;
+      const float s1 = bb * ps1[q] + cc * g1;
+      ps0[q] = s0;
+      ps1[q] = s1;
+      const float d0 = ce * s0 + ce * km1 * g0;
+      const float d1 = ce * s1 + ce * km1 * g1;
+      if (ax == 0) {
+        e1 -= d0;
+        e2 += d1;
+      } else if (ax == 1) {
+        e0 += d0;
+        e2 -= d1;
+      } else {
+        e0 -= d0;
+        e1 += d1;
+      }
+    }
+    if (p.pec_axes & 1) {
+      if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
+    }
+    if (p.pec_axes & 2) {
+      if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
+    }
+    if (p.pec_axes & 4) {
+      if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
+      if (k == p.nz - 1) e2 = 0.f;
+    }
+    const uint32_t fl = p.flags[m];
+    if (fl & RFXH_ZERO_EX) e0 = 0.f;
+    if (fl & RFXH_ZERO_EY) e1 = 0.f;
+    if (fl & RFXH_ZERO_EZ) e2 = 0.f;
+    if (fl & RFXH_POINT) point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
+    Ex[c] = e0;
+    Ey[c] = e1;
+    Ez[c] = e2;
+    hy_i = hy;
+    hz_i = hz;
+  }
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kBX* kTY) void k_e_march(KParams p) {
+  __shared__ float sHx[2][kTY + 1][kBX];
+  __shared__ float sHz[2][kTY + 1][kBX];
+  const TileIdx T = tile_of(p);
+  const int lane = threadIdx.x, ty = threadIdx.y;
+  const int k = T.k0 + lane, j = T.j0 + ty;
+  const bool in = (k < p.nz) && (j < p.ny);
+  const long long P = p.plane;
+  float* __restrict__ Ex = p.f[0];
+  float* __restrict__ Ey = p.f[1];
+  float* __restrict__ Ez = p.f[2];
+  const float* __restrict__ Hx = p.f[3];
+  const float* __restrict__ Hy = p.f[4];
+  const float* __restrict__ Hz = p.f[5];
+  long long c = ((long long)T.i0 * p.ny + (in ? j : 0)) * p.nz + (in ? k : 0);
+  float hy_p = 0.f, hz_p = 0.f;  // H at plane i-1
+  if (in && T.i0 > 0) {
+    hy_p = Hy[c - P];
+    hz_p = Hz[c - P];
+  }
+  float iy = p.inv_e0, iz = p.inv_e0;
+  if (GRADED && in) {
+    iy = p.inv_e[1][j];
+    iz = p.inv_e[2][k];
+  }
+  const bool j_edge = (ty == 0) && in && (j > 0);
+  const bool k_edge = (lane == 0) && in && (k > 0);
+  for (int i = T.i0; i < T.i1; ++i, c += P) {
+    const int buf = (i - T.i0) & 1;
+    float hx = 0.f, hy = 0.f, hz = 0.f;
+    if (in) {
+      hx = Hx[c];
+      hy = Hy[c];
+      hz = Hz[c];
+    }
+    sHx[buf][ty + 1][lane] = hx;
+    sHz[buf][ty + 1][lane] = hz;
+    if (ty == 0) {
+      float a = 0.f, b = 0.f;
+      if (j_edge) {
+        a = Hx[c - p.nz];
+        b = Hz[c - p.nz];
+      }
+      sHx[buf][0][lane] = a;
+      sHz[buf][0][lane] = b;
+    }
+    float hx_k = __shfl_up(hx, 1), hy_k = __shfl_up(hy, 1);
+    if (lane == 0) {
+      hx_k = k_edge ? Hx[c - 1] : 0.f;
+      hy_k = k_edge ? Hy[c - 1] : 0.f;
+    }
+    __syncthreads();
+    if (in) {
+      const float hx_j = sHx[buf][ty][lane];
+      const float hz_j = sHz[buf][ty][lane];
+      const float ix = GRADED ? p.inv_e[0][i] : p.inv_e0;
+      const unsigned m = p.mat[c];
+      const float4 co = p.coef[m];
+      const float ca = co.x, cb = co.y, ce = co.z;
+      float e0 = Ex[c], e1 = Ey[c], e2 = Ez[c];
+      const float cx = (hz - hz_j) * iy - (hy - hy_k) * iz;
+      const float cy = (hx - hx_k) * iz - (hz - hz_p) * ix;
+      const float cz = (hy - hy_p) * ix - (hx - hx_j) * iy;
+      e0 = ca * e0 + cb * cx;
+      e1 = ca * e1 + cb * cy;
+      e2 = ca * e2 + cb * cz;
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        const int n = p.cn[face];
+        if (n == 0) continue;
+        const int ax = face >> 1;
+        const int pos = ax == 0 ? i : ax == 1 ? j : k;
+        const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
+        const int l = pos - ((face & 1) ? N - n : 0);
+        if (l < 0 || l >= n) continue;
+        const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
+        const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+        const float inv = p.cinv[face];
+        float g0, g1;
+        if (ax == 0) {
+          g0 = (hz - hz_p) * inv;  // Ey -= ...
+          g1 = (hy - hy_p) * inv;  // Ez += ...
+        } else if (ax == 1) {
+          g0 = (hz - hz_j) * inv;  // Ex += ...
+          g1 = (hx - hx_j) * inv;  // Ez -= ...
+        } else {
+          g0 = (hy - hy_k) * inv;  // Ex -= ...
+          g1 = (hx - hx_k) * inv;  // Ey += ...
+        }
+        float* ps0 = p.psi_e[face][0];
+        float* ps1 = p.psi_e[face][1];
+        const float s0 = bb * ps0[q] + cc * g0;
+        const float s1 = bb * ps1[q] + cc * g1;
+        ps0[q] = s0;
+        ps1[q] = s1;
+        const float d0 = ce * s0 + ce * km1 * g0;
+        const float d1 = ce * s1 + ce * km1 * g1;
+        if (ax == 0) {
+          e1 -= d0;
+          e2 += d1;
+        } else if (ax == 1) {
+          e0 += d0;
+          e2 -= d1;
+        } else {
+          e0 -= d0;
+          e1 += d1;
+        }
+      }
+      if (p.pec_axes & 1) {
+        if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
+      }
+      if (p.pec_axes & 2) {
+        if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
+      }
+      if (p.pec_axes & 4) {
+        if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
+        if (k == p.nz - 1) e2 = 0.f;
+      }
+      const uint32_t fl = p.flags[m];
+      if (fl & RFXH_ZERO_EX) e0 = 0.f;
+      if (fl & RFXH_ZERO_EY) e1 = 0.f;
+      if (fl & RFXH_ZERO_EZ) e2 = 0.f;
+      if (fl & RFXH_POINT) point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_p, hy_k, hz_p, hz_j);
+      Ex[c] = e0;
+      Ey[c] = e1;
+      Ez[c] = e2;
+    }
+    hy_p = hy;
+    hz_p = hz;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -385,6 +943,7 @@ __global__ void k_set_step(long long* step, long long value) {
 struct rfxh_plan {
   KParams kp;
   dim3 grid, block;
+  int kind = 0;  // 0 flat-chunk march (default), 1 LDS tile march, 2 one plane per workgroup
   std::vector<PlaneParams> planes;
   std::vector<void*> owned;  // device allocations owned by the plan
   long long max_steps = 0;
@@ -428,15 +987,39 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev) {
   // ev (optional): four events bracketing the H pass, the E pass and the DFT
   // kernels of this step, recorded on the stream the kernels run on.
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
-  if (pl->kp.graded)
-    hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, pl->kp);
-  else
-    hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, pl->kp);
+  if (pl->kind == 0) {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_h_fm<true>, pl->grid, pl->block, 0, s, pl->kp);
+    else
+      hipLaunchKernelGGL(k_h_fm<false>, pl->grid, pl->block, 0, s, pl->kp);
+  } else if (pl->kind == 1) {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_h_march<true>, pl->grid, pl->block, 0, s, pl->kp);
+    else
+      hipLaunchKernelGGL(k_h_march<false>, pl->grid, pl->block, 0, s, pl->kp);
+  } else {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, pl->kp);
+    else
+      hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, pl->kp);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
-  if (pl->kp.graded)
-    hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, pl->kp);
-  else
-    hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, pl->kp);
+  if (pl->kind == 0) {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_e_fm<true>, pl->grid, pl->block, 0, s, pl->kp);
+    else
+      hipLaunchKernelGGL(k_e_fm<false>, pl->grid, pl->block, 0, s, pl->kp);
+  } else if (pl->kind == 1) {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_e_march<true>, pl->grid, pl->block, 0, s, pl->kp);
+    else
+      hipLaunchKernelGGL(k_e_march<false>, pl->grid, pl->block, 0, s, pl->kp);
+  } else {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, pl->kp);
+    else
+      hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, pl->kp);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
   for (const PlaneParams& q : pl->planes) {
     const long long npts = (long long)q.n1 * q.n2;
@@ -615,11 +1198,39 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pp.tw = dtw;
     pl->planes.push_back(pp);
   }
-  const long long nkb = (d->nz + kBX - 1) / kBX, njb = (d->ny + kBY - 1) / kBY;
-  const long long nblocks = nkb * njb * d->nx;
+  // kernel design (RFXH_KERNEL = fm | march | cell; fm is the default) and
+  // planes per march (RFXH_ICHUNK, for tuning sweeps)
+  const char* kern = std::getenv("RFXH_KERNEL");
+  pl->kind = 2;
+  if (kern && std::string(kern) == "march") pl->kind = 1;
+  if (kern && std::string(kern) == "fm") pl->kind = 0;
+  const char* ord = std::getenv("RFXH_ORDER");
+  kp.order = 2;
+  if (ord && std::string(ord) == "flat") kp.order = 0;
+  if (ord && std::string(ord) == "tile") kp.order = 1;
+  const long long flat = (long long)d->ny * d->nz;
+  const long long nch = (flat + kBX * kTY - 1) / (kBX * kTY);
+  const long long nkb = (d->nz + kBX - 1) / kBX, njb = (d->ny + kTY - 1) / kTY;
+  const long long tiles = pl->kind == 1 ? nkb * njb : nch;
+  // default march length: enough workgroups to fill 256 CUs several times
+  long long ichunk = pl->kind == 2 ? 1 : std::max<long long>(1, d->nx * tiles / 8192);
+  if (const char* env = std::getenv("RFXH_ICHUNK")) {
+    const long long v = std::atoll(env);
+    if (v >= 1) ichunk = v;
+  }
+  ichunk = std::min<long long>(ichunk, d->nx);
+  const long long nic = (d->nx + ichunk - 1) / ichunk;
+  long long nblocks = tiles * nic;
+  if (pl->kind == 2) {
+    const long long ncb = ((flat + 7) / 8 + kBX * kTY - 1) / (kBX * kTY);
+    nblocks = kp.order == 0   ? (long long)d->nx * nch
+              : kp.order == 1 ? (long long)d->nx * nkb * njb
+                              : 8LL * d->nx * ncb;
+  }
   if (nblocks > 0x7fffffffLL) return bail(fail(RFXH_EINVAL, "grid too large"));
+  kp.ichunk = (int)ichunk;
   pl->grid = dim3((unsigned)nblocks);
-  pl->block = dim3(kBX, kBY);
+  pl->block = dim3(kBX, kTY);
   pl->tev.resize(4 * kTimingSamples);
   for (auto& e : pl->tev) {
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RFXH_EHIP, "hipEventCreate"));
