@@ -88,6 +88,10 @@ typedef struct {
   /* state (device, caller-owned) */
   float* d_f[6];           /* ex, ey, ez, hx, hy, hz */
   float* d_psi;            /* CPML memory, rfxh_psi_floats() floats, zeroed by the caller */
+  float* d_f2[6];          /* optional second field set; when given (with d_psi2 if there is
+                              CPML) each step reads one set and writes the other, which lets
+                              one fused kernel do the whole step.  NULL: in-place two-pass step */
+  float* d_psi2;           /* second CPML memory set (same size as d_psi) */
   /* point operations */
   int32_t n_ops;
   const rfxh_point_op* ops;/* host, any order (sorted internally by cell, order) */
@@ -122,9 +126,17 @@ int rfxh_plan_destroy(rfxh_plan* plan);
  * from a captured hipGraph. */
 int rfxh_run(rfxh_plan* plan, int64_t step0, int64_t n_steps, void* stream, int use_graph);
 
-/* Average duration in milliseconds of each kernel of the last rfxh_run made
- * with timing enabled (rfxh_set_timing(plan, 1)), measured with HIP events on
- * the stream the kernels run on.  which: 0 = H pass, 1 = E pass, 2 = DFT. */
+/* With two field sets: which set (0 = d_f/d_psi, 1 = d_f2/d_psi2) holds the
+ * current state; rfxh_set_state_set tells the plan after the caller wrote a
+ * state (initial conditions, a restored checkpoint) into a set. */
+int rfxh_state_set(rfxh_plan* plan);
+int rfxh_set_state_set(rfxh_plan* plan, int which);
+
+/* Average duration in milliseconds of the kernels of the sampled steps of
+ * runs made with timing enabled (rfxh_set_timing(plan, 1), which also resets
+ * the sums), measured with HIP events on the stream the kernels run on.
+ * which: 0 = fused step kernel (two-pass: H pass), 1 = E pass (two-pass
+ * only), 2 = DFT-plane kernels. */
 int rfxh_set_timing(rfxh_plan* plan, int enable);
 int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* count);
 

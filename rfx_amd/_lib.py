@@ -50,7 +50,7 @@ class Desc(C.Structure):
         ("cpml_b", _FP * 6), ("cpml_c", _FP * 6), ("cpml_km1", _FP * 6),
         ("cpml_inv_d", C.c_float * 6),
         ("pec_axes", C.c_int32),
-        ("d_f", _P * 6), ("d_psi", _P),
+        ("d_f", _P * 6), ("d_psi", _P), ("d_f2", _P * 6), ("d_psi2", _P),
         ("n_ops", C.c_int32), ("ops", C.POINTER(PointOp)),
         ("n_src", C.c_int32), ("d_src_wave", _P),
         ("n_probe", C.c_int32), ("d_probe_out", _P),
@@ -64,7 +64,8 @@ class Desc(C.Structure):
 _lib = None
 
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
-           "rfxh_set_timing", "rfxh_kernel_time_ms", "rfxh_last_error", "rfxh_version")
+           "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
+           "rfxh_last_error", "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -89,6 +90,10 @@ def load():
     lib.rfxh_plan_destroy.restype = C.c_int
     lib.rfxh_run.argtypes = [_P, C.c_int64, C.c_int64, _P, C.c_int]
     lib.rfxh_run.restype = C.c_int
+    lib.rfxh_state_set.argtypes = [_P]
+    lib.rfxh_state_set.restype = C.c_int
+    lib.rfxh_set_state_set.argtypes = [_P, C.c_int]
+    lib.rfxh_set_state_set.restype = C.c_int
     lib.rfxh_set_timing.argtypes = [_P, C.c_int]
     lib.rfxh_set_timing.restype = C.c_int
     lib.rfxh_kernel_time_ms.argtypes = [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]

@@ -1,23 +1,33 @@
 // rfxhip.hip — MI355X (gfx950) FDTD time-stepping kernels and their C ABI.
 //
-// One time step (rfx/simulation.py:1256-1681 uniform, rfx/nonuniform.py:
-// 1256-1565 graded) is three kinds of launch on one stream:
-//   k_update_h  H update (rfx/core/yee.py:225-291 / :411-451) with the CPML
-//               H correction (rfx/boundaries/cpml.py:804-1065) fused in;
-//   k_update_e  E update (yee.py:346-404 / :456-492) with the CPML E
-//               correction (cpml.py:522-801), outer-face PEC (rfx/boundaries/
-//               pec.py:13-48), the tangential PEC mask (pec.py:179-208) and
-//               the per-cell point operations (port V/I sampling, source
-//               injection, probes) fused in;
-//   k_dft_plane one per DFT plane: float64 running sums (rfx/simulation.py:
-//               1574-1588, rfx/nonuniform.py:1458-1476).
-// Fields are float32 structure-of-arrays [nx][ny][nz] with z fastest; a
-// 64-lane wavefront covers 64 consecutive z cells of one (i, j) row, so every
-// field access of a wavefront is one contiguous 256-byte segment.  Material
-// coefficients are read through a uint16 id per cell and a small table that
-// stays cache-resident (2 bytes per cell per pass instead of 12-16).  The
-// step index lives in device memory (advanced by k_update_h) so that one
-// captured hipGraph of a chunk of steps can be replayed for every chunk.
+// One time step follows rfx/simulation.py:1256-1681 (uniform) or
+// rfx/nonuniform.py:1256-1565 (graded):
+//   H update (rfx/core/yee.py:225-291 / :411-451) + CPML-H (rfx/boundaries/
+//   cpml.py:804-1065) -> E update (yee.py:346-404 / :456-492) + CPML-E
+//   (cpml.py:522-801) -> outer-face PEC (rfx/boundaries/pec.py:13-48) ->
+//   tangential PEC mask (pec.py:179-208) -> port V/I sampling, source
+//   injection, probes -> DFT planes.
+//
+// Two kernel designs (no MFMA: ~33 flops per cell-step):
+//   two-pass (default): an H launch and an E launch per step, in place; 12
+//     components read + 6 written per cell-step.
+//   fused (RFXH_KERNEL=fused, needs two field sets): one launch per step.  Fields and CPML memory are
+//     double-buffered (read set A, write set B, swap), so a thread can
+//     compute H^{n+1/2} at its own cell and, redundantly, at its three
+//     backward neighbours, then E^{n+1} at its cell, without waiting for
+//     other workgroups.  HBM traffic ~ 6 components read + 6 written +
+//     a 2-byte material id per cell-step (the other loads hit L1/L2), but
+//     four H evaluations per cell make it VALU-bound on MI355X.
+// Fields are float32 [nx][ny][nz], z fastest.  Each workgroup covers 512
+// consecutive cells of one (y, z) plane (flat chunks: neighbouring
+// workgroups share only the cache line at their boundary; chunk index
+// fastest, XCD-contiguous).  RFXH_ORDER=bands cuts the plane into 8 bands,
+// one per XCD, swept plane after plane (lowest L2 miss traffic, measured
+// slightly slower).
+// Material coefficients come through a uint16 id per cell and a small
+// cache-resident table.  The step index lives in device memory and is
+// advanced by a one-thread kernel at the end of each step, so one captured
+// hipGraph of a chunk of steps is replayed for every chunk.
 
 #include <hip/hip_runtime.h>
 
@@ -47,23 +57,24 @@ int fail(int code, const std::string& msg) {
       return fail(RFXH_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
-constexpr int kBX = 64;  // lanes along z (one wavefront)
-constexpr int kTY = 8;   // rows along y per workgroup (one wavefront per row)
+constexpr int kBX = 64;             // lanes (one wavefront)
+constexpr int kTY = 8;              // wavefronts per workgroup
+constexpr int kWG = kBX * kTY;      // 512 cells per workgroup
 constexpr int kTimingSamples = 64;  // sampled steps per timed run
 
 struct KParams {
   int nx, ny, nz;
   int graded;
-  long long plane;  // ny * nz
+  long long plane;               // ny * nz
   float inv_e0, inv_h0;          // uniform spacing
   const float* inv_e[3];         // graded spacings (device)
   const float* inv_h[3];
   const uint16_t* mat;
-  const float4* coef;            // ca, cb, ce, ch
+  const float4* coef;            // ca, cb, ce, ch per material id
   const uint32_t* flags;
-  float* f[6];
+  float* f[6];                   // two-pass kernels: the field set updated in place
   int cn[6];                     // CPML slab depth per face
-  float* psi_h[6][2];
+  float* psi_h[6][2];            // CPML memory of set 0 (set 1 at + psi_set)
   float* psi_e[6][2];
   const float* pb[6];
   const float* pc[6];
@@ -79,8 +90,24 @@ struct KParams {
   float* port_out;
   int n_port;
   long long* step;               // current absolute step (device)
-  int ichunk;                    // planes per workgroup march
-  int order;                     // cell kernels: 0 flat chunks, 1 column tiles
+  int order;                     // workgroup order: 0 bands, 1 flat chunks
+  float inv_nz;                  // 1/nz for the flat-index split
+  int slo[3], shi[3];            // CPML slab extents: cells i < slo or i >= shi per axis
+};
+
+struct FusedIO {
+  const float* eo[3];
+  const float* ho[3];
+  float* en[3];
+  float* hn[3];
+};
+
+// CPML memory pointers of one read set and one write set, per face and term.
+struct PsiIO {
+  const float* hr[6][2];
+  float* hw[6][2];
+  const float* er[6][2];
+  float* ew[6][2];
 };
 
 struct PlaneParams {
@@ -95,199 +122,107 @@ struct PlaneParams {
   int ny, nz;
 };
 
-__device__ __forceinline__ float ld(const float* __restrict__ a, long long c) { return a[c]; }
-
-// x-face slab index for psi: [l][j][k]; y-face: [i][l][k]; z-face: [i][j][l]
-__device__ __forceinline__ long long psi_idx(int face, int l, int i, int j, int k, int n,
-                                             int ny, int nz) {
-  int ax = face >> 1;
-  if (ax == 0) return ((long long)l * ny + j) * nz + k;
-  if (ax == 1) return ((long long)i * n + l) * nz + k;
-  return ((long long)i * ny + j) * n + l;
+// Loads/stores at a 32-bit byte offset from a (wave-uniform) base pointer: the
+// base stays in SGPRs and the address is one VGPR offset, no 64-bit math.
+// Every array indexed this way is smaller than 4 GiB (checked at plan
+// creation).
+__device__ __forceinline__ float ldf(const float* base, unsigned cell) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (cell << 2));
+}
+__device__ __forceinline__ void stf(float* base, unsigned cell, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (cell << 2)) = v;
+}
+__device__ __forceinline__ unsigned ldu16(const uint16_t* base, unsigned cell) {
+  return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(base) + (cell << 1));
 }
 
-// ---------------------------------------------------------------------------
-// Tile order.  Each workgroup owns a 64 (z) x kTY (y) tile of columns and
-// marches along x over a chunk [i0, i1) of planes, so the x neighbour of every
-// field comes from registers (loaded once), the y neighbour from the tile row
-// above/below through LDS, and the z neighbour from the adjacent lane.
-// Workgroups are dispatched round-robin over the 8 XCDs; xcd_remap hands each
-// XCD a contiguous range of logical tiles so the halo rows two neighbouring
-// tiles share are fetched into the same L2.
-// ---------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
+// receives a contiguous range of logical workgroup ids.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int per = nb / 8, rem = nb % 8;
   const int x = b % 8, s = b / 8;
   return x < rem ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s;
 }
 
-struct TileIdx {
-  int i0, i1, j0, k0;
-};
-
-__device__ __forceinline__ TileIdx tile_of(const KParams& p) {
+__device__ __forceinline__ bool cell_of(const KParams& p, int& i, int& j, int& k) {
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const int nkb = (p.nz + kBX - 1) / kBX, njb = (p.ny + kTY - 1) / kTY;
-  const int kb = L % nkb;
-  const int t = L / nkb;
-  const int jb = t % njb;
-  const int ic = t / njb;
-  TileIdx r;
-  r.i0 = ic * p.ichunk;
-  r.i1 = min(p.nx, r.i0 + p.ichunk);
-  r.j0 = jb * kTY;
-  r.k0 = kb * kBX;
-  return r;
+  const long long pl = p.plane;
+  long long f;
+  if (p.order == 0) {
+    // bands: the (y, z) plane is cut into 8 flat bands, one per XCD; the XCD
+    // sweeps its band plane by plane with 512-cell chunks fastest
+    const long long bw = (pl + 7) / 8;
+    const int ncb = (int)((bw + kWG - 1) / kWG);
+    const int b = L / (p.nx * ncb);
+    const int r = L % (p.nx * ncb);
+    i = r / ncb;
+    f = (long long)b * bw + (long long)(r % ncb) * kWG + threadIdx.y * kBX + threadIdx.x;
+    if (f >= (long long)(b + 1) * bw) return false;
+  } else {
+    // flat chunks of 512 cells, chunk index fastest, then plane
+    const int nch = (int)((pl + kWG - 1) / kWG);
+    i = L / nch;
+    f = (long long)(L % nch) * kWG + threadIdx.y * kBX + threadIdx.x;
+  }
+  if (f >= pl) return false;
+  // f < ny*nz is exact in float; the product is within one of the quotient
+  int jj = (int)((float)f * p.inv_nz);
+  int kk = (int)f - jj * p.nz;
+  if (kk < 0) {
+    --jj;
+    kk += p.nz;
+  } else if (kk >= p.nz) {
+    ++jj;
+    kk -= p.nz;
+  }
+  j = jj;
+  k = kk;
+  return true;
+}
+
+__device__ __forceinline__ bool in_slab(const KParams& p, int i, int j, int k) {
+  return (i < p.slo[0]) | (i >= p.shi[0]) | (j < p.slo[1]) | (j >= p.shi[1]) |
+         (k < p.slo[2]) | (k >= p.shi[2]);
+}
+
+// x-face slab index for psi: [l][j][k]; y-face: [i][l][k]; z-face: [i][j][l]
+__device__ __forceinline__ long long psi_idx(int face, int l, int i, int j, int k, int n,
+                                             int ny, int nz) {
+  const int ax = face >> 1;
+  if (ax == 0) return ((long long)l * ny + j) * nz + k;
+  if (ax == 1) return ((long long)i * n + l) * nz + k;
+  return ((long long)i * ny + j) * n + l;
 }
 
 __device__ __forceinline__ float pick3(int c, float a, float b, float d) {
   return c == 0 ? a : (c == 1 ? b : d);
 }
 
-// ---------------------------------------------------------------------------
-// H pass: H(i) = H(i) - ch curl_fwd E (yee.py:272-289) + CPML-H (cpml.py:875-1043)
-// ---------------------------------------------------------------------------
-template <bool GRADED>
-__global__ __launch_bounds__(kBX* kTY) void k_h_march(KParams p) {
-  __shared__ float sEx[2][kTY + 1][kBX];
-  __shared__ float sEz[2][kTY + 1][kBX];
-  const TileIdx T = tile_of(p);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-    // advance the step counter; the E pass and DFT kernels of this step read it
-    *p.step = *p.step + 1;
-  }
-  const int lane = threadIdx.x, ty = threadIdx.y;
-  const int k = T.k0 + lane, j = T.j0 + ty;
-  const bool in = (k < p.nz) && (j < p.ny);
-  const long long P = p.plane;
-  const float* __restrict__ Ex = p.f[0];
-  const float* __restrict__ Ey = p.f[1];
-  const float* __restrict__ Ez = p.f[2];
-  float* __restrict__ Hx = p.f[3];
-  float* __restrict__ Hy = p.f[4];
-  float* __restrict__ Hz = p.f[5];
-  long long c = ((long long)T.i0 * p.ny + (in ? j : 0)) * p.nz + (in ? k : 0);
-  float ex0 = 0.f, ey0 = 0.f, ez0 = 0.f;
-  if (in) {
-    ex0 = Ex[c];
-    ey0 = Ey[c];
-    ez0 = Ez[c];
-  }
-  float iy = p.inv_h0, iz = p.inv_h0;
-  if (GRADED && in) {
-    iy = p.inv_h[1][j];
-    iz = p.inv_h[2][k];
-  }
-  const bool j_edge = (ty == kTY - 1) && in && (j + 1 < p.ny);
-  const bool k_edge = (lane == kBX - 1) && in && (k + 1 < p.nz);
-  for (int i = T.i0; i < T.i1; ++i, c += P) {
-    const int buf = (i - T.i0) & 1;
-    float ex1 = 0.f, ey1 = 0.f, ez1 = 0.f;
-    if (in && i + 1 < p.nx) {
-      ex1 = Ex[c + P];
-      ey1 = Ey[c + P];
-      ez1 = Ez[c + P];
-    }
-    sEx[buf][ty][lane] = ex0;
-    sEz[buf][ty][lane] = ez0;
-    if (ty == kTY - 1) {
-      float a = 0.f, b = 0.f;
-      if (j_edge) {
-        a = Ex[c + p.nz];
-        b = Ez[c + p.nz];
-      }
-      sEx[buf][kTY][lane] = a;
-      sEz[buf][kTY][lane] = b;
-    }
-    float ex_k = __shfl_down(ex0, 1), ey_k = __shfl_down(ey0, 1);
-    if (lane == kBX - 1) {
-      ex_k = k_edge ? Ex[c + 1] : 0.f;
-      ey_k = k_edge ? Ey[c + 1] : 0.f;
-    } else if (k + 1 >= p.nz) {
-      ex_k = ey_k = 0.f;
-    }
-    __syncthreads();
-    if (in) {
-      const float ex_j = sEx[buf][ty + 1][lane];
-      const float ez_j = sEz[buf][ty + 1][lane];
-      const float ix = GRADED ? p.inv_h[0][i] : p.inv_h0;
-      const float ch = p.coef[p.mat[c]].w;
-      float hx = Hx[c], hy = Hy[c], hz = Hz[c];
-      const float cx = (ez_j - ez0) * iy - (ey_k - ey0) * iz;
-      const float cy = (ex_k - ex0) * iz - (ez1 - ez0) * ix;
-      const float cz = (ey1 - ey0) * ix - (ex_j - ex0) * iy;
-      hx = hx - ch * cx;
-      hy = hy - ch * cy;
-      hz = hz - ch * cz;
-#pragma unroll
-      for (int face = 0; face < 6; ++face) {
-        const int n = p.cn[face];
-        if (n == 0) continue;
-        const int ax = face >> 1;
-        const int pos = ax == 0 ? i : ax == 1 ? j : k;
-        const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
-        const int l = pos - ((face & 1) ? N - n : 0);
-        if (l < 0 || l >= n) continue;
-        const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
-        const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
-        const float inv = p.cinv[face];
-        float g0, g1;
-        if (ax == 0) {
-          g0 = (ez1 - ez0) * inv;   // Hy += ...
-          g1 = (ey1 - ey0) * inv;   // Hz -= ...
-        } else if (ax == 1) {
-          g0 = (ez_j - ez0) * inv;  // Hx -= ...
-          g1 = (ex_j - ex0) * inv;  // Hz += ...
-        } else {
-          g0 = (ey_k - ey0) * inv;  // Hx += ...
-          g1 = (ex_k - ex0) * inv;  // Hy -= ...
-        }
-        float* ps0 = p.psi_h[face][0];
-        float* ps1 = p.psi_h[face][1];
-        const float s0 = bb * ps0[q] + cc * g0;
-        const float s1 = bb * ps1[q] + cc * g1;
-        ps0[q] = s0;
-        ps1[q] = s1;
-        const float d0 = ch * s0 + ch * km1 * g0;
-        const float d1 = ch * s1 + ch * km1 * g1;
-        if (ax == 0) {
-          hy += d0;
-          hz -= d1;
-        } else if (ax == 1) {
-          hx -= d0;
-          hz += d1;
-        } else {
-          hx += d0;
-          hy -= d1;
-        }
-      }
-      Hx[c] = hx;
-      Hy[c] = hy;
-      Hz[c] = hz;
-    }
-    ex0 = ex1;
-    ey0 = ey1;
-    ez0 = ez1;
-  }
+// CPML slab lookup: local layer index of (i, j, k) in face's slab, or -1.
+__device__ __forceinline__ int slab_layer(const KParams& p, int face, int i, int j, int k) {
+  const int n = p.cn[face];
+  if (n == 0) return -1;
+  const int ax = face >> 1;
+  const int pos = ax == 0 ? i : ax == 1 ? j : k;
+  const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
+  const int l = pos - ((face & 1) ? N - n : 0);
+  return (l < 0 || l >= n) ? -1 : l;
 }
 
-// ---------------------------------------------------------------------------
-// E pass: E = ca E + cb curl_bwd H (yee.py:379-404), CPML-E (cpml.py:608-777),
-// outer-face PEC (pec.py:24-46), PEC mask (pec.py:179-208), point operations
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void point_ops(const KParams& p, long long cell, float& e0, float& e1,
+// Port V/I sampling, source injection and probes at one cell, in the order
+// the host gave (rfx/simulation.py:1449-1567; rfx/nonuniform.py:1353-1542).
+__device__ __forceinline__ void point_ops(const KParams& p, unsigned cell, float& e0, float& e1,
                                           float& e2, float hx, float hy, float hz, float hx_j,
                                           float hx_k, float hy_i, float hy_k, float hz_i,
                                           float hz_j) {
-  // binary search the first op of this cell (ops sorted by cell, order)
-  int lo = 0, hi = p.n_ops;
+  int lo = 0, hi = p.n_ops;  // first op of this cell (ops sorted by cell, order)
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (p.ops[mid].cell < cell) lo = mid + 1;
+    if (p.ops[mid].cell < (long long)cell) lo = mid + 1;
     else hi = mid;
   }
   const long long n = *p.step;
-  for (int q = lo; q < p.n_ops && p.ops[q].cell == cell; ++q) {
+  for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)cell; ++q) {
     const rfxh_point_op op = p.ops[q];
     if (op.kind == RFXH_OP_PORT) {
       const float v = -pick3(op.comp, e0, e1, e2) * op.d_par;
@@ -313,77 +248,35 @@ __device__ __forceinline__ void point_ops(const KParams& p, long long cell, floa
   }
 }
 
-// Cell-per-thread tile order: logical workgroup L covers plane i of column tile
-// (jb, kb) with i fastest, and each XCD receives a contiguous run of logical
-// ids, so the workgroups of planes i and i+1 of one column tile run back to
-// back on the same XCD and the x neighbours one loads are still in its L2.
-__device__ __forceinline__ bool cell_of(const KParams& p, int& i, int& j, int& k) {
-  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const long long pl = (long long)p.ny * p.nz;
-  if (p.order == 2) {
-    // bands: the (y, z) plane is split into 8 flat bands, one per XCD; the
-    // XCD sweeps its band plane by plane (512-cell chunks fastest), so the
-    // band's x neighbours stay in that XCD's L2 between consecutive planes
-    const long long bw = (pl + 7) / 8;
-    const int ncb = (int)((bw + kBX * kTY - 1) / (kBX * kTY));
-    const int b = L / (p.nx * ncb);
-    const int r = L % (p.nx * ncb);
-    i = r / ncb;
-    const long long f = (long long)b * bw + (long long)(r % ncb) * (kBX * kTY) +
-                        threadIdx.y * kBX + threadIdx.x;
-    if (f >= pl || f >= (long long)(b + 1) * bw) return false;
-    j = (int)(f / p.nz);
-    k = (int)(f % p.nz);
-    return true;
-  }
-  if (p.order == 0) {
-    // flat chunks: workgroup L covers 512 consecutive cells of the (y, z)
-    // plane of plane i (chunk index fastest), so neighbouring workgroups
-    // share only the cache line at their common boundary
-    const int nch = (int)((pl + kBX * kTY - 1) / (kBX * kTY));
-    const int ch = L % nch;
-    i = L / nch;
-    const long long f = (long long)ch * (kBX * kTY) + threadIdx.y * kBX + threadIdx.x;
-    if (f >= pl) return false;
-    j = (int)(f / p.nz);
-    k = (int)(f % p.nz);
-    return true;
-  }
-  const int nkb = (p.nz + kBX - 1) / kBX;
-  i = L % p.nx;
-  const int t = L / p.nx;
-  const int kb = t % nkb;
-  const int jb = t / nkb;
-  k = kb * kBX + threadIdx.x;
-  j = jb * kTY + threadIdx.y;
-  return k < p.nz && j < p.ny;
-}
-
 // ---------------------------------------------------------------------------
-// Cell-per-thread passes (one plane per workgroup)
+// H at one cell: H^{n+1/2} = H^{n-1/2} - ch curl_fwd E^n (yee.py:272-289,
+// graded :429-451) plus the CPML-H correction (cpml.py:875-1043), reading
+// CPML memory ps.hr and (WRITE) storing ps.hw and the result into Hn.
 // ---------------------------------------------------------------------------
-template <bool GRADED>
-__global__ __launch_bounds__(kBX* kTY) void k_update_h(KParams p) {
-  int i, j, k;
-  const bool ok = cell_of(p, i, j, k);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-    // advance the step counter; the E pass and DFT kernels of this step read it
-    *p.step = *p.step + 1;
-  }
-  if (!ok) return;
-  const long long P = p.plane;
-  const long long c = ((long long)i * p.ny + j) * p.nz + k;
-  const float* __restrict__ Ex = p.f[0];
-  const float* __restrict__ Ey = p.f[1];
-  const float* __restrict__ Ez = p.f[2];
-  const float ex = ld(Ex, c), ey = ld(Ey, c), ez = ld(Ez, c);
+template <bool GRADED, bool WRITE>
+__device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo[3],
+                                         const float* const Ho[3], float* const Hn[3],
+                                         const PsiIO& ps, int i, int j, int k) {
+  const unsigned P = (unsigned)p.plane;
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned m = ldu16(p.mat, c);
+  // neighbour loads are issued unconditionally from an in-range index and
+  // zeroed afterwards (zero padding past the last cell, yee.py:121-128), so
+  // no load sits behind a branch
   const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
-  const float ex_j = hj ? ld(Ex, c + p.nz) : 0.f;
-  const float ez_j = hj ? ld(Ez, c + p.nz) : 0.f;
-  const float ex_k = hk ? ld(Ex, c + 1) : 0.f;
-  const float ey_k = hk ? ld(Ey, c + 1) : 0.f;
-  const float ey_i = hi ? ld(Ey, c + P) : 0.f;
-  const float ez_i = hi ? ld(Ez, c + P) : 0.f;
+  const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
+  const float ex = ldf(Eo[0], c), ey = ldf(Eo[1], c), ez = ldf(Eo[2], c);
+  float ex_j = ldf(Eo[0], cj), ez_j = ldf(Eo[2], cj);
+  float ex_k = ldf(Eo[0], ck), ey_k = ldf(Eo[1], ck);
+  float ey_i = ldf(Eo[1], ci), ez_i = ldf(Eo[2], ci);
+  float hx = ldf(Ho[0], c), hy = ldf(Ho[1], c), hz = ldf(Ho[2], c);
+  const float ch = p.coef[m].w;
+  ex_j = hj ? ex_j : 0.f;
+  ez_j = hj ? ez_j : 0.f;
+  ex_k = hk ? ex_k : 0.f;
+  ey_k = hk ? ey_k : 0.f;
+  ey_i = hi ? ey_i : 0.f;
+  ez_i = hi ? ez_i : 0.f;
   float ix, iy, iz;
   if (GRADED) {
     ix = p.inv_h[0][i];
@@ -392,261 +285,35 @@ __global__ __launch_bounds__(kBX* kTY) void k_update_h(KParams p) {
   } else {
     ix = iy = iz = p.inv_h0;
   }
-  const float4 co = p.coef[p.mat[c]];
-  const float ch = co.w;
-  float hx = p.f[3][c], hy = p.f[4][c], hz = p.f[5][c];
-  // yee.py:272-289: H -= (dt/mu) * curl E (forward differences)
-  const float cx = (ez_j - ez) * iy - (ey_k - ey) * iz;
-  const float cy = (ex_k - ex) * iz - (ez_i - ez) * ix;
-  const float cz = (ey_i - ey) * ix - (ex_j - ex) * iy;
-  hx = hx - ch * cx;
-  hy = hy - ch * cy;
-  hz = hz - ch * cz;
-  // CPML-H (cpml.py:875-1043); ch equals dt/(mu_r mu0) of cpml.py:817
-#pragma unroll
-  for (int face = 0; face < 6; ++face) {
-    const int n = p.cn[face];
-    if (n == 0) continue;
-    const int ax = face >> 1;
-    const int pos = ax == 0 ? i : ax == 1 ? j : k;
-    const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
-    const int lo = (face & 1) ? N - n : 0;
-    const int l = pos - lo;
-    if (l < 0 || l >= n) continue;
-    const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
-    const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
-    const float inv = p.cinv[face];
-    float g0, g1;
-    if (ax == 0) {
-      g0 = (ez_i - ez) * inv;  // Hy += ...
-      g1 = (ey_i - ey) * inv;  // Hz -= ...
-    } else if (ax == 1) {
-      g0 = (ez_j - ez) * inv;  // Hx -= ...
-      g1 = (ex_j - ex) * inv;  // Hz += ...
-    } else {
-      g0 = (ey_k - ey) * inv;  // Hx += ...
-      g1 = (ex_k - ex) * inv;  // Hy -= ...
-    }
-    float* ps0 = p.psi_h[face][0];
-    float* ps1 = p.psi_h[face][1];
-    const float s0 = bb * ps0[q] + cc * g0;
-    const float s1 = bb * ps1[q] + cc * g1;
-    ps0[q] = s0;
-    ps1[q] = s1;
-    const float d0 = ch * s0 + ch * km1 * g0;
-    const float d1 = ch * s1 + ch * km1 * g1;
-    if (ax == 0) {
-      hy += d0;
-      hz -= d1;
-    } else if (ax == 1) {
-      hx -= d0;
-      hz += d1;
-    } else {
-      hx += d0;
-      hy -= d1;
-    }
-  }
-  p.f[3][c] = hx;
-  p.f[4][c] = hy;
-  p.f[5][c] = hz;
-}
-
-// ---------------------------------------------------------------------------
-// E pass
-// ---------------------------------------------------------------------------
-template <bool GRADED>
-__global__ __launch_bounds__(kBX* kTY) void k_update_e(KParams p) {
-  int i, j, k;
-  if (!cell_of(p, i, j, k)) return;
-  const long long P = p.plane;
-  const long long c = ((long long)i * p.ny + j) * p.nz + k;
-  const float* __restrict__ Hx = p.f[3];
-  const float* __restrict__ Hy = p.f[4];
-  const float* __restrict__ Hz = p.f[5];
-  const float hx = ld(Hx, c), hy = ld(Hy, c), hz = ld(Hz, c);
-  const float hx_j = j > 0 ? ld(Hx, c - p.nz) : 0.f;
-  const float hz_j = j > 0 ? ld(Hz, c - p.nz) : 0.f;
-  const float hx_k = k > 0 ? ld(Hx, c - 1) : 0.f;
-  const float hy_k = k > 0 ? ld(Hy, c - 1) : 0.f;
-  const float hy_i = i > 0 ? ld(Hy, c - P) : 0.f;
-  const float hz_i = i > 0 ? ld(Hz, c - P) : 0.f;
-  float ix, iy, iz;
-  if (GRADED) {
-    ix = p.inv_e[0][i];
-    iy = p.inv_e[1][j];
-    iz = p.inv_e[2][k];
-  } else {
-    ix = iy = iz = p.inv_e0;
-  }
-  const unsigned m = p.mat[c];
-  const float4 co = p.coef[m];
-  const float ca = co.x, cb = co.y, ce = co.z;
-  float e0 = p.f[0][c], e1 = p.f[1][c], e2 = p.f[2][c];
-  // yee.py:379-404: E = ca E + cb curl H (backward differences)
-  const float cx = (hz - hz_j) * iy - (hy - hy_k) * iz;
-  const float cy = (hx - hx_k) * iz - (hz - hz_i) * ix;
-  const float cz = (hy - hy_i) * ix - (hx - hx_j) * iy;
-  e0 = ca * e0 + cb * cx;
-  e1 = ca * e1 + cb * cy;
-  e2 = ca * e2 + cb * cz;
-  // CPML-E (cpml.py:608-777) with ce = dt/(eps_r eps0) (cpml.py:538-546)
-#pragma unroll
-  for (int face = 0; face < 6; ++face) {
-    const int n = p.cn[face];
-    if (n == 0) continue;
-    const int ax = face >> 1;
-    const int pos = ax == 0 ? i : ax == 1 ? j : k;
-    const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
-    const int lo = (face & 1) ? N - n : 0;
-    const int l = pos - lo;
-    if (l < 0 || l >= n) continue;
-    const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
-    const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
-    const float inv = p.cinv[face];
-    float g0, g1;
-    if (ax == 0) {
-      g0 = (hz - hz_i) * inv;  // Ey -= ...
-      g1 = (hy - hy_i) * inv;  // Ez += ...
-    } else if (ax == 1) {
-      g0 = (hz - hz_j) * inv;  // Ex += ...
-      g1 = (hx - hx_j) * inv;  // Ez -= ...
-    } else {
-      g0 = (hy - hy_k) * inv;  // Ex -= ...
-      g1 = (hx - hx_k) * inv;  // Ey += ...
-    }
-    float* ps0 = p.psi_e[face][0];
-    float* ps1 = p.psi_e[face][1];
-    const float s0 = bb * ps0[q] + cc * g0;
-    const float s1 = bb * ps1[q] + cc * g1;
-    ps0[q] = s0;
-    ps1[q] = s1;
-    const float d0 = ce * s0 + ce * km1 * g0;
-    const float d1 = ce * s1 + ce * km1 * g1;
-    if (ax == 0) {
-      e1 -= d0;
-      e2 += d1;
-    } else if (ax == 1) {
-      e0 += d0;
-      e2 -= d1;
-    } else {
-      e0 -= d0;
-      e1 += d1;
-    }
-  }
-  // outer-face PEC (pec.py:24-46)
-  if (p.pec_axes & 1) {
-    if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
-  }
-  if (p.pec_axes & 2) {
-    if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
-  }
-  if (p.pec_axes & 4) {
-    if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
-    if (k == p.nz - 1) e2 = 0.f;
-  }
-  const uint32_t fl = p.flags[m];
-  if (fl & RFXH_ZERO_EX) e0 = 0.f;
-  if (fl & RFXH_ZERO_EY) e1 = 0.f;
-  if (fl & RFXH_ZERO_EZ) e2 = 0.f;
-  if (fl & RFXH_POINT) point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
-  p.f[0][c] = e0;
-  p.f[1][c] = e1;
-  p.f[2][c] = e2;
-}
-
-// ---------------------------------------------------------------------------
-// Flat-chunk marching passes (default): workgroup = 512 consecutive cells of
-// the (y, z) plane, marching over a chunk of x planes.  The x neighbour is
-// carried in registers from the previous plane; y and z neighbours are plain
-// loads of cells the neighbouring lanes / workgroups fetch in the same
-// iteration (L1/L2 hits).  No LDS, no barriers: waves run independently.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool fm_cell(const KParams& p, int& i0, int& i1, int& j, int& k) {
-  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const long long pl = (long long)p.ny * p.nz;
-  const int nch = (int)((pl + kBX * kTY - 1) / (kBX * kTY));
-  const int ch = L % nch;
-  const int ic = L / nch;
-  i0 = ic * p.ichunk;
-  i1 = min(p.nx, i0 + p.ichunk);
-  const long long f = (long long)ch * (kBX * kTY) + threadIdx.y * kBX + threadIdx.x;
-  if (f >= pl) return false;
-  j = (int)(f / p.nz);
-  k = (int)(f % p.nz);
-  return true;
-}
-
-template <bool GRADED>
-__global__ __launch_bounds__(kBX* kTY) void k_h_fm(KParams p) {
-  int i0, i1, j, k;
-  const bool ok = fm_cell(p, i0, i1, j, k);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-    // advance the step counter; the E pass and DFT kernels of this step read it
-    *p.step = *p.step + 1;
-  }
-  if (!ok) return;
-  const long long P = p.plane;
-  const float* __restrict__ Ex = p.f[0];
-  const float* __restrict__ Ey = p.f[1];
-  const float* __restrict__ Ez = p.f[2];
-  float* __restrict__ Hx = p.f[3];
-  float* __restrict__ Hy = p.f[4];
-  float* __restrict__ Hz = p.f[5];
-  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
-  float iy = p.inv_h0, iz = p.inv_h0;
-  if (GRADED) {
-    iy = p.inv_h[1][j];
-    iz = p.inv_h[2][k];
-  }
-  long long c = ((long long)i0 * p.ny + j) * p.nz + k;
-  float ex = Ex[c], ey = Ey[c], ez = Ez[c];
-  for (int i = i0; i < i1; ++i, c += P) {
-    const bool hi = i + 1 < p.nx;
-    const float ex_j = hj ? Ex[c + p.nz] : 0.f;
-    const float ez_j = hj ? Ez[c + p.nz] : 0.f;
-    const float ex_k = hk ? Ex[c + 1] : 0.f;
-    const float ey_k = hk ? Ey[c + 1] : 0.f;
-    const float ex_i = hi ? Ex[c + P] : 0.f;
-    const float ey_i = hi ? Ey[c + P] : 0.f;
-    const float ez_i = hi ? Ez[c + P] : 0.f;
-    const float ix = GRADED ? p.inv_h[0][i] : p.inv_h0;
-    const float ch = p.coef[p.mat[c]].w;
-    float hx = Hx[c], hy = Hy[c], hz = Hz[c];
-    const float cx = (ez_j - ez) * iy - (ey_k - ey) * iz;
-    const float cy = (ex_k - ex) * iz - (ez_i - ez) * ix;
-    const float cz = (ey_i - ey) * ix - (ex_j - ex) * iy;
-    hx = hx - ch * cx;
-    hy = hy - ch * cy;
-    hz = hz - ch * cz;
+  hx = hx - ch * ((ez_j - ez) * iy - (ey_k - ey) * iz);
+  hy = hy - ch * ((ex_k - ex) * iz - (ez_i - ez) * ix);
+  hz = hz - ch * ((ey_i - ey) * ix - (ex_j - ex) * iy);
+  if (in_slab(p, i, j, k)) {
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
-      const int n = p.cn[face];
-      if (n == 0) continue;
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
       const int ax = face >> 1;
-      const int pos = ax == 0 ? i : ax == 1 ? j : k;
-      const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
-      const int l = pos - ((face & 1) ? N - n : 0);
-      if (l < 0 || l >= n) continue;
-      const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
+      const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
       const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
       const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
-        g0 = (ez_i - ez) * inv;   // Hy += ...
-        g1 = (ey_i - ey) * inv;   // Hz -= ...
+        g0 = (ez_i - ez) * inv;  // Hy += ...
+        g1 = (ey_i - ey) * inv;  // Hz -= ...
       } else if (ax == 1) {
-        g0 = (ez_j - ez) * inv;   // Hx -= ...
-        g1 = (ex_j - ex) * inv;   // Hz += ...
+        g0 = (ez_j - ez) * inv;  // Hx -= ...
+        g1 = (ex_j - ex) * inv;  // Hz += ...
       } else {
-        g0 = (ey_k - ey) * inv;   // Hx += ...
-        g1 = (ex_k - ex) * inv;   // Hy -= ...
+        g0 = (ey_k - ey) * inv;  // Hx += ...
+        g1 = (ex_k - ex) * inv;  // Hy -= ...
       }
-      float* ps0 = p.psi_h[face][0];
-      float* ps1 = p.psi_h[face][1];
-      const float s0 = bb * ps0[q] + cc * g0;
-      const float s1 = bb * ps1[q] + cc * g1;
-      ps0[q] = s0;
-      ps1[q] = s1;
+      const float s0 = bb * ldf(ps.hr[face][0], q) + cc * g0;
+      const float s1 = bb * ldf(ps.hr[face][1], q) + cc * g1;
+      if (WRITE) {
+        stf(ps.hw[face][0], q, s0);
+        stf(ps.hw[face][1], q, s1);
+      }
       const float d0 = ch * s0 + ch * km1 * g0;
       const float d1 = ch * s1 + ch * km1 * g1;
       if (ax == 0) {
@@ -660,65 +327,46 @@ __global__ __launch_bounds__(kBX* kTY) void k_h_fm(KParams p) {
         hy -= d1;
       }
     }
-    Hx[c] = hx;
-    Hy[c] = hy;
-    Hz[c] = hz;
-    ex = ex_i;
-    ey = ey_i;
-    ez = ez_i;
   }
+  if (WRITE) {
+    stf(Hn[0], c, hx);
+    stf(Hn[1], c, hy);
+    stf(Hn[2], c, hz);
+  }
+  return make_float3(hx, hy, hz);
 }
 
+// E at one cell from H^{n+1/2} at the cell and its backward neighbours:
+// E^{n+1} = ca E^n + cb curl_bwd H (yee.py:379-404), CPML-E (cpml.py:608-777),
+// PEC faces and mask, point operations; stores into En.
 template <bool GRADED>
-__global__ __launch_bounds__(kBX* kTY) void k_e_fm(KParams p) {
-  int i0, i1, j, k;
-  if (!fm_cell(p, i0, i1, j, k)) return;
-  const long long P = p.plane;
-  float* __restrict__ Ex = p.f[0];
-  float* __restrict__ Ey = p.f[1];
-  float* __restrict__ Ez = p.f[2];
-  const float* __restrict__ Hx = p.f[3];
-  const float* __restrict__ Hy = p.f[4];
-  const float* __restrict__ Hz = p.f[5];
-  const bool hj = j > 0, hk = k > 0;
-  float iy = p.inv_e0, iz = p.inv_e0;
+__device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3],
+                                       float* const En[3], const PsiIO& ps, int i, int j, int k,
+                                       float hx, float hy, float hz, float hy_i, float hz_i,
+                                       float hx_j, float hz_j, float hx_k, float hy_k) {
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  float ix, iy, iz;
   if (GRADED) {
+    ix = p.inv_e[0][i];
     iy = p.inv_e[1][j];
     iz = p.inv_e[2][k];
+  } else {
+    ix = iy = iz = p.inv_e0;
   }
-  long long c = ((long long)i0 * p.ny + j) * p.nz + k;
-  float hy_i = 0.f, hz_i = 0.f;  // H at plane i-1
-  if (i0 > 0) {
-    hy_i = Hy[c - P];
-    hz_i = Hz[c - P];
-  }
-  for (int i = i0; i < i1; ++i, c += P) {
-    const float hx = Hx[c], hy = Hy[c], hz = Hz[c];
-    const float hx_j = hj ? Hx[c - p.nz] : 0.f;
-    const float hz_j = hj ? Hz[c - p.nz] : 0.f;
-    const float hx_k = hk ? Hx[c - 1] : 0.f;
-    const float hy_k = hk ? Hy[c - 1] : 0.f;
-    const float ix = GRADED ? p.inv_e[0][i] : p.inv_e0;
-    const unsigned m = p.mat[c];
-    const float4 co = p.coef[m];
-    const float ca = co.x, cb = co.y, ce = co.z;
-    float e0 = Ex[c], e1 = Ey[c], e2 = Ez[c];
-    const float cx = (hz - hz_j) * iy - (hy - hy_k) * iz;
-    const float cy = (hx - hx_k) * iz - (hz - hz_i) * ix;
-    const float cz = (hy - hy_i) * ix - (hx - hx_j) * iy;
-    e0 = ca * e0 + cb * cx;
-    e1 = ca * e1 + cb * cy;
-    e2 = ca * e2 + cb * cz;
+  const unsigned m = ldu16(p.mat, c);
+  float e0 = ldf(Eo[0], c), e1 = ldf(Eo[1], c), e2 = ldf(Eo[2], c);
+  const float4 co = p.coef[m];
+  const float ca = co.x, cb = co.y, ce = co.z;
+  e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
+  e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
+  e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
+  if (in_slab(p, i, j, k)) {
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
-      const int n = p.cn[face];
-      if (n == 0) continue;
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
       const int ax = face >> 1;
-      const int pos = ax == 0 ? i : ax == 1 ? j : k;
-      const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
-      const int l = pos - ((face & 1) ? N - n : 0);
-      if (l < 0 || l >= n) continue;
-      const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
+      const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
       const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
       const float inv = p.cinv[face];
       float g0, g1;
@@ -732,12 +380,10 @@ __global__ __launch_bounds__(kBX* kTY) void k_e_fm(KParams p) {
         g0 = (hy - hy_k) * inv;  // Ex -= ...
         g1 = (hx - hx_k) * inv;  // Ey += ...
       }
-      float* ps0 = p.psi_e[face][0];
-      float* ps1 = p.psi_e[face][1];
-      const float s0 = bb * ps0[q] + cc * g0;
-      const float s1 = bb * ps1[q] + cc * g1;
-      ps0[q] = s0;
-      ps1[q] = s1;
+      const float s0 = bb * ldf(ps.er[face][0], q) + cc * g0;
+      const float s1 = bb * ldf(ps.er[face][1], q) + cc * g1;
+      stf(ps.ew[face][0], q, s0);
+      stf(ps.ew[face][1], q, s1);
       const float d0 = ce * s0 + ce * km1 * g0;
       const float d1 = ce * s1 + ce * km1 * g1;
       if (ax == 0) {
@@ -751,160 +397,97 @@ __global__ __launch_bounds__(kBX* kTY) void k_e_fm(KParams p) {
         e1 += d1;
       }
     }
-    if (p.pec_axes & 1) {
-      if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
-    }
-    if (p.pec_axes & 2) {
-      if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
-    }
-    if (p.pec_axes & 4) {
-      if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
-      if (k == p.nz - 1) e2 = 0.f;
-    }
-    const uint32_t fl = p.flags[m];
+  }
+  if (p.pec_axes & 1) {
+    if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
+  }
+  if (p.pec_axes & 2) {
+    if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
+  }
+  if (p.pec_axes & 4) {
+    if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
+    if (k == p.nz - 1) e2 = 0.f;
+  }
+  const uint32_t fl = p.flags[m];
+  if (fl) {
     if (fl & RFXH_ZERO_EX) e0 = 0.f;
     if (fl & RFXH_ZERO_EY) e1 = 0.f;
     if (fl & RFXH_ZERO_EZ) e2 = 0.f;
-    if (fl & RFXH_POINT) point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
-    Ex[c] = e0;
-    Ey[c] = e1;
-    Ez[c] = e2;
-    hy_i = hy;
-    hz_i = hz;
+    if (fl & RFXH_POINT)
+      point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
   }
+  stf(En[0], c, e0);
+  stf(En[1], c, e1);
+  stf(En[2], c, e2);
+}
+
+// ---------------------------------------------------------------------------
+// Fused step: read set A (E^n, H^{n-1/2}, psi), write set B.
+// ---------------------------------------------------------------------------
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_step_fused(KParams p, FusedIO io, PsiIO ps) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  const float* const Eo[3] = {io.eo[0], io.eo[1], io.eo[2]};
+  const float* const Ho[3] = {io.ho[0], io.ho[1], io.ho[2]};
+  float* const Hn[3] = {io.hn[0], io.hn[1], io.hn[2]};
+  float* const En[3] = {io.en[0], io.en[1], io.en[2]};
+  const float3 h = h_cell<GRADED, true>(p, Eo, Ho, Hn, ps, i, j, k);
+  float hy_i = 0.f, hz_i = 0.f, hx_j = 0.f, hz_j = 0.f, hx_k = 0.f, hy_k = 0.f;
+  if (i > 0) {
+    const float3 t = h_cell<GRADED, false>(p, Eo, Ho, Hn, ps, i - 1, j, k);
+    hy_i = t.y;
+    hz_i = t.z;
+  }
+  if (j > 0) {
+    const float3 t = h_cell<GRADED, false>(p, Eo, Ho, Hn, ps, i, j - 1, k);
+    hx_j = t.x;
+    hz_j = t.z;
+  }
+  if (k > 0) {
+    const float3 t = h_cell<GRADED, false>(p, Eo, Ho, Hn, ps, i, j, k - 1);
+    hx_k = t.x;
+    hy_k = t.y;
+  }
+  e_cell<GRADED>(p, Eo, En, ps, i, j, k, h.x, h.y, h.z, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass step, in place on p.f.
+// ---------------------------------------------------------------------------
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  const float* const Eo[3] = {p.f[0], p.f[1], p.f[2]};
+  const float* const Ho[3] = {p.f[3], p.f[4], p.f[5]};
+  float* const Hn[3] = {p.f[3], p.f[4], p.f[5]};
+  h_cell<GRADED, true>(p, Eo, Ho, Hn, ps, i, j, k);
 }
 
 template <bool GRADED>
-__global__ __launch_bounds__(kBX* kTY) void k_e_march(KParams p) {
-  __shared__ float sHx[2][kTY + 1][kBX];
-  __shared__ float sHz[2][kTY + 1][kBX];
-  const TileIdx T = tile_of(p);
-  const int lane = threadIdx.x, ty = threadIdx.y;
-  const int k = T.k0 + lane, j = T.j0 + ty;
-  const bool in = (k < p.nz) && (j < p.ny);
-  const long long P = p.plane;
-  float* __restrict__ Ex = p.f[0];
-  float* __restrict__ Ey = p.f[1];
-  float* __restrict__ Ez = p.f[2];
-  const float* __restrict__ Hx = p.f[3];
-  const float* __restrict__ Hy = p.f[4];
-  const float* __restrict__ Hz = p.f[5];
-  long long c = ((long long)T.i0 * p.ny + (in ? j : 0)) * p.nz + (in ? k : 0);
-  float hy_p = 0.f, hz_p = 0.f;  // H at plane i-1
-  if (in && T.i0 > 0) {
-    hy_p = Hy[c - P];
-    hz_p = Hz[c - P];
-  }
-  float iy = p.inv_e0, iz = p.inv_e0;
-  if (GRADED && in) {
-    iy = p.inv_e[1][j];
-    iz = p.inv_e[2][k];
-  }
-  const bool j_edge = (ty == 0) && in && (j > 0);
-  const bool k_edge = (lane == 0) && in && (k > 0);
-  for (int i = T.i0; i < T.i1; ++i, c += P) {
-    const int buf = (i - T.i0) & 1;
-    float hx = 0.f, hy = 0.f, hz = 0.f;
-    if (in) {
-      hx = Hx[c];
-      hy = Hy[c];
-      hz = Hz[c];
-    }
-    sHx[buf][ty + 1][lane] = hx;
-    sHz[buf][ty + 1][lane] = hz;
-    if (ty == 0) {
-      float a = 0.f, b = 0.f;
-      if (j_edge) {
-        a = Hx[c - p.nz];
-        b = Hz[c - p.nz];
-      }
-      sHx[buf][0][lane] = a;
-      sHz[buf][0][lane] = b;
-    }
-    float hx_k = __shfl_up(hx, 1), hy_k = __shfl_up(hy, 1);
-    if (lane == 0) {
-      hx_k = k_edge ? Hx[c - 1] : 0.f;
-      hy_k = k_edge ? Hy[c - 1] : 0.f;
-    }
-    __syncthreads();
-    if (in) {
-      const float hx_j = sHx[buf][ty][lane];
-      const float hz_j = sHz[buf][ty][lane];
-      const float ix = GRADED ? p.inv_e[0][i] : p.inv_e0;
-      const unsigned m = p.mat[c];
-      const float4 co = p.coef[m];
-      const float ca = co.x, cb = co.y, ce = co.z;
-      float e0 = Ex[c], e1 = Ey[c], e2 = Ez[c];
-      const float cx = (hz - hz_j) * iy - (hy - hy_k) * iz;
-      const float cy = (hx - hx_k) * iz - (hz - hz_p) * ix;
-      const float cz = (hy - hy_p) * ix - (hx - hx_j) * iy;
-      e0 = ca * e0 + cb * cx;
-      e1 = ca * e1 + cb * cy;
-      e2 = ca * e2 + cb * cz;
-#pragma unroll
-      for (int face = 0; face < 6; ++face) {
-        const int n = p.cn[face];
-        if (n == 0) continue;
-        const int ax = face >> 1;
-        const int pos = ax == 0 ? i : ax == 1 ? j : k;
-        const int N = ax == 0 ? p.nx : ax == 1 ? p.ny : p.nz;
-        const int l = pos - ((face & 1) ? N - n : 0);
-        if (l < 0 || l >= n) continue;
-        const long long q = psi_idx(face, l, i, j, k, n, p.ny, p.nz);
-        const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
-        const float inv = p.cinv[face];
-        float g0, g1;
-        if (ax == 0) {
-          g0 = (hz - hz_p) * inv;  // Ey -= ...
-          g1 = (hy - hy_p) * inv;  // Ez += ...
-        } else if (ax == 1) {
-          g0 = (hz - hz_j) * inv;  // Ex += ...
-          g1 = (hx - hx_j) * inv;  // Ez -= ...
-        } else {
-          g0 = (hy - hy_k) * inv;  // Ex -= ...
-          g1 = (hx - hx_k) * inv;  // Ey += ...
-        }
-        float* ps0 = p.psi_e[face][0];
-        float* ps1 = p.psi_e[face][1];
-        const float s0 = bb * ps0[q] + cc * g0;
-        const float s1 = bb * ps1[q] + cc * g1;
-        ps0[q] = s0;
-        ps1[q] = s1;
-        const float d0 = ce * s0 + ce * km1 * g0;
-        const float d1 = ce * s1 + ce * km1 * g1;
-        if (ax == 0) {
-          e1 -= d0;
-          e2 += d1;
-        } else if (ax == 1) {
-          e0 += d0;
-          e2 -= d1;
-        } else {
-          e0 -= d0;
-          e1 += d1;
-        }
-      }
-      if (p.pec_axes & 1) {
-        if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
-      }
-      if (p.pec_axes & 2) {
-        if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
-      }
-      if (p.pec_axes & 4) {
-        if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
-        if (k == p.nz - 1) e2 = 0.f;
-      }
-      const uint32_t fl = p.flags[m];
-      if (fl & RFXH_ZERO_EX) e0 = 0.f;
-      if (fl & RFXH_ZERO_EY) e1 = 0.f;
-      if (fl & RFXH_ZERO_EZ) e2 = 0.f;
-      if (fl & RFXH_POINT) point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_p, hy_k, hz_p, hz_j);
-      Ex[c] = e0;
-      Ey[c] = e1;
-      Ez[c] = e2;
-    }
-    hy_p = hy;
-    hz_p = hz;
-  }
+__global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const float* Hx = p.f[3];
+  const float* Hy = p.f[4];
+  const float* Hz = p.f[5];
+  const bool hj = j > 0, hk = k > 0, hi = i > 0;
+  const unsigned cj = hj ? c - p.nz : c, ck = hk ? c - 1 : c, ci = hi ? c - (unsigned)p.plane : c;
+  const float hx = ldf(Hx, c), hy = ldf(Hy, c), hz = ldf(Hz, c);
+  float hx_j = ldf(Hx, cj), hz_j = ldf(Hz, cj);
+  float hx_k = ldf(Hx, ck), hy_k = ldf(Hy, ck);
+  float hy_i = ldf(Hy, ci), hz_i = ldf(Hz, ci);
+  hx_j = hj ? hx_j : 0.f;
+  hz_j = hj ? hz_j : 0.f;
+  hx_k = hk ? hx_k : 0.f;
+  hy_k = hk ? hy_k : 0.f;
+  hy_i = hi ? hy_i : 0.f;
+  hz_i = hi ? hz_i : 0.f;
+  const float* const Eo[3] = {p.f[0], p.f[1], p.f[2]};
+  float* const En[3] = {p.f[0], p.f[1], p.f[2]};
+  e_cell<GRADED>(p, Eo, En, ps, i, j, k, hx, hy, hz, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
 }
 
 // ---------------------------------------------------------------------------
@@ -938,19 +521,27 @@ __global__ void k_set_step(long long* step, long long value) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *step = value;
 }
 
+__global__ void k_advance(long long* step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *step = *step + 1;
+}
+
 }  // namespace
 
 struct rfxh_plan {
   KParams kp;
   dim3 grid, block;
-  int kind = 0;  // 0 flat-chunk march (default), 1 LDS tile march, 2 one plane per workgroup
+  bool fused = false;           // second field set given: fused step
+  float* set[2][6];             // field sets (set[1] null in two-pass mode)
+  long long psi_set = 0;        // float offset of CPML set 1 from set 0
+  int cur = 0;                  // which set holds the current state
   std::vector<PlaneParams> planes;
-  std::vector<void*> owned;  // device allocations owned by the plan
+  std::vector<int> plane_comp;
+  std::vector<void*> owned;     // device allocations owned by the plan
   long long max_steps = 0;
   hipStream_t cap_stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  int graph_chunk = 0;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  int graph_chunk = 16;         // even: a chunk returns the state to its starting set
   int timing = 0;
   double t_sum[3] = {0, 0, 0};
   long long t_cnt[3] = {0, 0, 0};
@@ -983,50 +574,58 @@ long long face_slab_floats(const rfxh_desc* d, int face) {
   return s;
 }
 
-int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev) {
-  // ev (optional): four events bracketing the H pass, the E pass and the DFT
+// One time step starting from field set `from` (fused: writes set 1-from).
+int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
+  // ev (optional): four events bracketing the main kernel(s) and the DFT
   // kernels of this step, recorded on the stream the kernels run on.
+  const int to = pl->fused ? 1 - from : from;
+  PsiIO ps;
+  for (int f = 0; f < 6; ++f)
+    for (int t = 0; t < 2; ++t) {
+      const long long ro = from * pl->psi_set, wo = to * pl->psi_set;
+      float* h0 = pl->kp.psi_h[f][t];
+      float* e0 = pl->kp.psi_e[f][t];
+      ps.hr[f][t] = h0 ? h0 + ro : nullptr;
+      ps.hw[f][t] = h0 ? h0 + wo : nullptr;
+      ps.er[f][t] = e0 ? e0 + ro : nullptr;
+      ps.ew[f][t] = e0 ? e0 + wo : nullptr;
+    }
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
-  if (pl->kind == 0) {
+  if (pl->fused) {
+    FusedIO io;
+    for (int a = 0; a < 3; ++a) {
+      io.eo[a] = pl->set[from][a];
+      io.ho[a] = pl->set[from][3 + a];
+      io.en[a] = pl->set[to][a];
+      io.hn[a] = pl->set[to][3 + a];
+    }
     if (pl->kp.graded)
-      hipLaunchKernelGGL(k_h_fm<true>, pl->grid, pl->block, 0, s, pl->kp);
+      hipLaunchKernelGGL(k_step_fused<true>, pl->grid, pl->block, 0, s, pl->kp, io, ps);
     else
-      hipLaunchKernelGGL(k_h_fm<false>, pl->grid, pl->block, 0, s, pl->kp);
-  } else if (pl->kind == 1) {
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_h_march<true>, pl->grid, pl->block, 0, s, pl->kp);
-    else
-      hipLaunchKernelGGL(k_h_march<false>, pl->grid, pl->block, 0, s, pl->kp);
+      hipLaunchKernelGGL(k_step_fused<false>, pl->grid, pl->block, 0, s, pl->kp, io, ps);
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
   } else {
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, pl->kp);
+    KParams kp = pl->kp;
+    for (int c = 0; c < 6; ++c) kp.f[c] = pl->set[from][c];
+    if (kp.graded)
+      hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, kp, ps);
     else
-      hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, pl->kp);
-  }
-  if (ev) HIPCHK(hipEventRecord(ev[1], s));
-  if (pl->kind == 0) {
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_e_fm<true>, pl->grid, pl->block, 0, s, pl->kp);
+      hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, kp, ps);
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
+    if (kp.graded)
+      hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, kp, ps);
     else
-      hipLaunchKernelGGL(k_e_fm<false>, pl->grid, pl->block, 0, s, pl->kp);
-  } else if (pl->kind == 1) {
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_e_march<true>, pl->grid, pl->block, 0, s, pl->kp);
-    else
-      hipLaunchKernelGGL(k_e_march<false>, pl->grid, pl->block, 0, s, pl->kp);
-  } else {
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, pl->kp);
-    else
-      hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, pl->kp);
+      hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, kp, ps);
   }
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
-  for (const PlaneParams& q : pl->planes) {
-    const long long npts = (long long)q.n1 * q.n2;
-    const unsigned nb = (unsigned)((npts + 255) / 256);
-    hipLaunchKernelGGL(k_dft_plane, dim3(nb), dim3(256), 0, s, q);
+  for (size_t q = 0; q < pl->planes.size(); ++q) {
+    PlaneParams pp = pl->planes[q];
+    pp.fld = pl->set[to][pl->plane_comp[q]];
+    const long long npts = (long long)pp.n1 * pp.n2;
+    hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
   }
   if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1065,22 +664,29 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     return fail(RFXH_EINVAL, "material table missing or larger than 65536 entries");
   for (int c = 0; c < 6; ++c)
     if (!d->d_f[c]) return fail(RFXH_EINVAL, "field pointer missing");
+  bool has2 = true;
+  for (int c = 0; c < 6; ++c) has2 = has2 && d->d_f2[c];
   const int N[3] = {d->nx, d->ny, d->nz};
+  bool any_cpml = false;
   for (int f = 0; f < 6; ++f) {
     if (d->cpml_n[f] < 0 || d->cpml_n[f] > N[f >> 1]) return fail(RFXH_EINVAL, "CPML depth outside the axis");
     if (d->cpml_n[f] > 0 && (!d->cpml_b[f] || !d->cpml_c[f] || !d->cpml_km1[f] || !d->d_psi))
       return fail(RFXH_EINVAL, "CPML profile or psi buffer missing");
+    any_cpml = any_cpml || d->cpml_n[f] > 0;
   }
+  if (has2 && any_cpml && !d->d_psi2) return fail(RFXH_EINVAL, "second CPML set missing");
   if (d->max_steps < 0) return fail(RFXH_EINVAL, "max_steps < 0");
   if ((d->n_src > 0 && !d->d_src_wave) || (d->n_probe > 0 && !d->d_probe_out) ||
       (d->n_port > 0 && !d->d_port_out))
     return fail(RFXH_EINVAL, "point-operation table missing");
   const long long ncell = (long long)d->nx * d->ny * d->nz;
+  if (ncell >= (1LL << 30))
+    return fail(RFXH_EINVAL, "grid larger than 2^30 cells (field arrays must stay below 4 GiB)");
   std::vector<rfxh_point_op> ops(d->ops, d->ops + d->n_ops);
   for (const rfxh_point_op& op : ops) {
     if (op.cell < 0 || op.cell >= ncell) return fail(RFXH_EINVAL, "point operation outside the grid");
     if (op.comp < 0 || op.comp > 5) return fail(RFXH_EINVAL, "bad point-operation component");
-    int lim = op.kind == RFXH_OP_PORT ? d->n_port : op.kind == RFXH_OP_INJECT ? d->n_src : d->n_probe;
+    const int lim = op.kind == RFXH_OP_PORT ? d->n_port : op.kind == RFXH_OP_INJECT ? d->n_src : d->n_probe;
     if (op.kind < 0 || op.kind > 2 || op.slot < 0 || op.slot >= lim)
       return fail(RFXH_EINVAL, "bad point-operation slot/kind");
     if (op.kind != RFXH_OP_PROBE && op.comp > 2) return fail(RFXH_EINVAL, "ports/sources drive E only");
@@ -1097,6 +703,11 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   kp.nz = d->nz;
   kp.graded = d->graded ? 1 : 0;
   kp.plane = (long long)d->ny * d->nz;
+  kp.inv_nz = 1.0f / (float)d->nz;
+  for (int a = 0; a < 3; ++a) {
+    kp.slo[a] = d->cpml_n[2 * a];
+    kp.shi[a] = N[a] - d->cpml_n[2 * a + 1];
+  }
   int rc = 0;
   auto bail = [&](int r) {
     rfxh_plan_destroy(pl);
@@ -1127,7 +738,16 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     if ((rc = dev_copy(pl, d->mat_flags, (size_t)d->n_mat, &fl))) return bail(rc);
     kp.flags = fl;
   }
-  for (int c = 0; c < 6; ++c) kp.f[c] = d->d_f[c];
+  // default: the two-pass step (measured faster than the fused step on
+  // MI355X at 533^3: the fused kernel's three redundant H evaluations make it
+  // VALU-bound); RFXH_KERNEL=fused selects the fused step when two sets exist
+  const char* kern = std::getenv("RFXH_KERNEL");
+  pl->fused = has2 && kern && std::string(kern) == "fused";
+  for (int c = 0; c < 6; ++c) {
+    pl->set[0][c] = d->d_f[c];
+    pl->set[1][c] = has2 ? d->d_f2[c] : nullptr;
+    kp.f[c] = d->d_f[c];
+  }
   long long off = 0;
   for (int f = 0; f < 6; ++f) {
     kp.cn[f] = d->cpml_n[f];
@@ -1148,6 +768,8 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     }
     off += 4 * s;
   }
+  // CPML set 1 is addressed as set-0 pointer + psi_set floats
+  pl->psi_set = (has2 && any_cpml) ? (long long)(d->d_psi2 - d->d_psi) : 0;
   kp.pec_axes = d->pec_axes;
   {
     rfxh_point_op* dops;
@@ -1163,7 +785,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   kp.n_port = d->n_port;
   pl->max_steps = d->max_steps;
   {
-    long long init = -1;
+    const long long init = -1;
     long long* st;
     if ((rc = dev_copy(pl, &init, 1, &st))) return bail(rc);
     kp.step = st;
@@ -1197,38 +819,15 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     if ((rc = dev_copy(pl, tw.data(), tw.size(), &dtw))) return bail(rc);
     pp.tw = dtw;
     pl->planes.push_back(pp);
+    pl->plane_comp.push_back(pd.comp);
   }
-  // kernel design (RFXH_KERNEL = fm | march | cell; fm is the default) and
-  // planes per march (RFXH_ICHUNK, for tuning sweeps)
-  const char* kern = std::getenv("RFXH_KERNEL");
-  pl->kind = 2;
-  if (kern && std::string(kern) == "march") pl->kind = 1;
-  if (kern && std::string(kern) == "fm") pl->kind = 0;
+  // workgroup order (RFXH_ORDER = flat (default) | bands, for comparisons)
   const char* ord = std::getenv("RFXH_ORDER");
-  kp.order = 2;
-  if (ord && std::string(ord) == "flat") kp.order = 0;
-  if (ord && std::string(ord) == "tile") kp.order = 1;
-  const long long flat = (long long)d->ny * d->nz;
-  const long long nch = (flat + kBX * kTY - 1) / (kBX * kTY);
-  const long long nkb = (d->nz + kBX - 1) / kBX, njb = (d->ny + kTY - 1) / kTY;
-  const long long tiles = pl->kind == 1 ? nkb * njb : nch;
-  // default march length: enough workgroups to fill 256 CUs several times
-  long long ichunk = pl->kind == 2 ? 1 : std::max<long long>(1, d->nx * tiles / 8192);
-  if (const char* env = std::getenv("RFXH_ICHUNK")) {
-    const long long v = std::atoll(env);
-    if (v >= 1) ichunk = v;
-  }
-  ichunk = std::min<long long>(ichunk, d->nx);
-  const long long nic = (d->nx + ichunk - 1) / ichunk;
-  long long nblocks = tiles * nic;
-  if (pl->kind == 2) {
-    const long long ncb = ((flat + 7) / 8 + kBX * kTY - 1) / (kBX * kTY);
-    nblocks = kp.order == 0   ? (long long)d->nx * nch
-              : kp.order == 1 ? (long long)d->nx * nkb * njb
-                              : 8LL * d->nx * ncb;
-  }
+  kp.order = (ord && std::string(ord) == "bands") ? 0 : 1;
+  const long long flat = kp.plane;
+  const long long nblocks = kp.order == 0 ? 8LL * d->nx * (((flat + 7) / 8 + kWG - 1) / kWG)
+                                          : (long long)d->nx * ((flat + kWG - 1) / kWG);
   if (nblocks > 0x7fffffffLL) return bail(fail(RFXH_EINVAL, "grid too large"));
-  kp.ichunk = (int)ichunk;
   pl->grid = dim3((unsigned)nblocks);
   pl->block = dim3(kBX, kTY);
   pl->tev.resize(4 * kTimingSamples);
@@ -1241,7 +840,8 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
 
 int rfxh_plan_destroy(rfxh_plan* pl) {
   if (!pl) return RFXH_OK;
-  if (pl->graph_exec) (void)hipGraphExecDestroy(pl->graph_exec);
+  for (int g = 0; g < 2; ++g)
+    if (pl->graph_exec[g]) (void)hipGraphExecDestroy(pl->graph_exec[g]);
   if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
   if (pl->ev_in) (void)hipEventDestroy(pl->ev_in);
   if (pl->ev_out) (void)hipEventDestroy(pl->ev_out);
@@ -1249,6 +849,18 @@ int rfxh_plan_destroy(rfxh_plan* pl) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : pl->owned) (void)hipFree(p);
   delete pl;
+  return RFXH_OK;
+}
+
+int rfxh_state_set(rfxh_plan* pl) {
+  if (!pl) return fail(RFXH_EINVAL, "null plan");
+  return pl->cur;
+}
+
+int rfxh_set_state_set(rfxh_plan* pl, int which) {
+  if (!pl || which < 0 || which > 1 || (which == 1 && !pl->set[1][0]))
+    return fail(RFXH_EINVAL, "bad state set");
+  pl->cur = which;
   return RFXH_OK;
 }
 
@@ -1274,22 +886,23 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
   if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
     return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, pl->kp.step, (long long)(step0 - 1));
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, pl->kp.step, (long long)step0);
   HIPCHK(hipGetLastError());
   if (n_steps == 0) return RFXH_OK;
-  const int chunk = 16;
-  if (!use_graph || pl->timing || n_steps < chunk) {
+  const int flip = pl->fused ? 1 : 0;
+  if (!use_graph || pl->timing || n_steps < pl->graph_chunk) {
     // timing: sample up to kTimingSamples evenly spaced steps; the host does
     // not wait inside the loop, so the queue stays full and each event pair
-    // brackets only its kernel
+    // brackets only its kernels
     const int64_t every = std::max<int64_t>(1, n_steps / kTimingSamples);
     int n_samples = 0;
     for (int64_t n = 0; n < n_steps; ++n) {
       hipEvent_t* ev = nullptr;
       if (pl->timing && n % every == every - 1 && n_samples < kTimingSamples)
         ev = pl->tev.data() + 4 * (n_samples++);
-      int rc = launch_step(pl, s, ev);
+      const int rc = launch_step(pl, s, ev, pl->cur);
       if (rc) return rc;
+      pl->cur ^= flip;
     }
     if (n_samples) return collect_timing(pl, n_samples);
     return RFXH_OK;
@@ -1299,31 +912,34 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
     HIPCHK(hipEventCreateWithFlags(&pl->ev_in, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&pl->ev_out, hipEventDisableTiming));
   }
-  if (!pl->graph_exec) {
-    hipGraph_t g;
+  const int g = pl->cur;
+  if (!pl->graph_exec[g]) {
+    hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(pl->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (int n = 0; n < chunk; ++n) {
-      int rc = launch_step(pl, pl->cap_stream, nullptr);
+    int from = g;
+    for (int n = 0; n < pl->graph_chunk; ++n) {
+      const int rc = launch_step(pl, pl->cap_stream, nullptr, from);
       if (rc) {
-        hipStreamEndCapture(pl->cap_stream, &g);
+        (void)hipStreamEndCapture(pl->cap_stream, &graph);
         return rc;
       }
+      from ^= flip;
     }
-    HIPCHK(hipStreamEndCapture(pl->cap_stream, &g));
-    HIPCHK(hipGraphInstantiate(&pl->graph_exec, g, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(g));
-    pl->graph_chunk = chunk;
+    HIPCHK(hipStreamEndCapture(pl->cap_stream, &graph));
+    HIPCHK(hipGraphInstantiate(&pl->graph_exec[g], graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
   }
   HIPCHK(hipEventRecord(pl->ev_in, s));
   HIPCHK(hipStreamWaitEvent(pl->cap_stream, pl->ev_in, 0));
   int64_t done = 0;
   while (n_steps - done >= pl->graph_chunk) {
-    HIPCHK(hipGraphLaunch(pl->graph_exec, pl->cap_stream));
-    done += pl->graph_chunk;
+    HIPCHK(hipGraphLaunch(pl->graph_exec[g], pl->cap_stream));
+    done += pl->graph_chunk;  // even chunk: the state is back in set g
   }
   for (; done < n_steps; ++done) {
-    int rc = launch_step(pl, pl->cap_stream, nullptr);
+    const int rc = launch_step(pl, pl->cap_stream, nullptr, pl->cur);
     if (rc) return rc;
+    pl->cur ^= flip;
   }
   HIPCHK(hipEventRecord(pl->ev_out, pl->cap_stream));
   HIPCHK(hipStreamWaitEvent(s, pl->ev_out, 0));
@@ -1332,6 +948,6 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
 
 const char* rfxh_last_error(void) { return g_err.c_str(); }
 
-int rfxh_version(void) { return 1; }
+int rfxh_version(void) { return 2; }
 
 }  // extern "C"

@@ -53,7 +53,7 @@ def _f32p(a):
 class DeviceRun:
     """Device buffers + rfxhip plan for ``n_steps`` steps of one problem."""
 
-    def __init__(self, prob: Problem, n_steps: int, device=None):
+    def __init__(self, prob: Problem, n_steps: int, device=None, two_sets: bool = True):
         _lib.require_device()
         self.lib = _lib.load()
         self.prob = prob
@@ -88,7 +88,12 @@ class DeviceRun:
 
         # ---- device buffers -------------------------------------------------
         self.ids = torch.from_numpy(mat.ids.view(np.int16)).to(dev)
-        self.fields = [torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(6)]
+        # two field sets: each step reads one and writes the other (one fused
+        # kernel per step); one set: in-place two-pass step
+        self.sets = [[torch.zeros(shape, dtype=torch.float32, device=dev) for _ in range(6)]]
+        if two_sets:
+            self.sets.append([torch.zeros(shape, dtype=torch.float32, device=dev)
+                              for _ in range(6)])
         n_src, n_probe, n_port = len(prob.sources), len(prob.probes), len(prob.ports)
         T = self.n_steps
         if n_src:
@@ -141,10 +146,15 @@ class DeviceRun:
                 d.cpml_inv_d[f] = inv_d
         d.pec_axes = sum(1 << a for a in range(3) if "xyz"[a] in prob.pec_axes)
         for c in range(6):
-            d.d_f[c] = self.fields[c].data_ptr()
+            d.d_f[c] = self.sets[0][c].data_ptr()
+            if two_sets:
+                d.d_f2[c] = self.sets[1][c].data_ptr()
         npsi = self.lib.rfxh_psi_floats(C.byref(d))
-        self.psi = torch.zeros(max(int(npsi), 1), dtype=torch.float32, device=dev)
-        d.d_psi = self.psi.data_ptr()
+        self.psis = [torch.zeros(max(int(npsi), 1), dtype=torch.float32, device=dev)
+                     for _ in range(len(self.sets))]
+        d.d_psi = self.psis[0].data_ptr()
+        if two_sets:
+            d.d_psi2 = self.psis[1].data_ptr()
         d.n_ops = n_ops
         d.ops = self._ops
         d.n_src = n_src
@@ -172,11 +182,32 @@ class DeviceRun:
             _lib.check(self.lib.rfxh_run(self.plan, int(step0), n, self.stream_ptr(),
                                          1 if use_graph else 0))
 
+    def state_set(self) -> int:
+        return int(self.lib.rfxh_state_set(self.plan))
+
+    @property
+    def fields(self):
+        """The six field tensors (ex, ey, ez, hx, hy, hz) holding the current state."""
+        return self.sets[self.state_set()]
+
+    @property
+    def psi(self):
+        return self.psis[self.state_set()]
+
+    def load_state(self, fields, psi):
+        """Write a state (six field tensors and the CPML memory) into set 0
+        and make it current."""
+        for dst, src in zip(self.sets[0], fields):
+            dst.copy_(src)
+        self.psis[0].copy_(psi)
+        _lib.check(self.lib.rfxh_set_state_set(self.plan, 0))
+
     def reset(self):
         """Zero the state and outputs (a fresh run over the same problem)."""
-        for f in self.fields:
+        for f in self.sets[0]:
             f.zero_()
-        self.psi.zero_()
+        self.psis[0].zero_()
+        _lib.check(self.lib.rfxh_set_state_set(self.plan, 0))
         self.probe_out.zero_()
         self.port_out.zero_()
         for a in self.plane_acc:

@@ -151,3 +151,27 @@ def test_zero_source_stays_zero_gpu(R):
     res = sim.run(n_steps=50)
     for c in o.COMPS:
         assert float(getattr(res.state, c).abs().max()) == 0.0
+
+
+def test_fused_step_matches_two_pass(R):
+    """The fused single-launch step (two field sets, redundant H at the
+    backward neighbours) and the in-place two-pass step compute the same
+    fields; they may differ only by float32 contraction/rounding."""
+    _say("fused vs two-pass kernels")
+    import torch
+    from rfx_amd.engine import DeviceRun
+    spec = cases.mixed_case("run")
+    sim = cases.to_sim(spec)
+    n = 150
+    _, prob, _ = sim._build_problem(n, "run")
+    out = []
+    for two in (True, False):
+        dr = DeviceRun(prob, n, two_sets=two)
+        dr.run(0, n)
+        torch.cuda.synchronize()
+        out.append(([f.cpu().numpy().copy() for f in dr.fields],
+                    dr.time_series().cpu().numpy().copy()))
+        dr.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert cases.rel_l2(a, b) < 1e-6
+    assert cases.rel_l2(out[0][1], out[1][1]) < 1e-6
