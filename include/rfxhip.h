@@ -140,6 +140,42 @@ int rfxh_set_state_set(rfxh_plan* plan, int which);
 int rfxh_set_timing(rfxh_plan* plan, int enable);
 int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* count);
 
+/* ---- reverse mode (gradient of a probe-series objective w.r.t. eps_r) ----
+ * Replaces jax.grad through the scan (rfx/api/_execute.py:2447-2858 with
+ * rfx/simulation.py:1975, 2011-2055 rematerialisation): the forward run
+ * records, per step and per cell of a box region, E^n, curl H^{n+1/2} and
+ * the CPML-E term of the E update (9 floats); the reverse sweep runs the
+ * transposed step backwards and accumulates dJ/d eps_r on the region. */
+typedef struct {
+  int32_t i0, j0, k0;      /* region origin (cells) */
+  int32_t ni, nj, nk;      /* region extent */
+} rfxh_region;
+
+/* Forward steps step0 .. step0+n_steps-1 (two-pass sub-steps on the current
+ * field set) recording 9 x region cells floats per step into
+ * d_rec[n][9][region] (row n = step step0 + n). */
+int rfxh_run_record(rfxh_plan* plan, int64_t step0, int64_t n_steps, const rfxh_region* region,
+                    float* d_rec, void* stream);
+
+typedef struct {
+  float* d_adj[6];           /* adjoint fields E-bar x,y,z, H-bar x,y,z; zeroed before the first call */
+  float* d_adj_psi;          /* adjoint CPML memory, rfxh_psi_floats() floats, zeroed */
+  float* d_gbar;             /* scratch, rfxh_psi_floats() floats */
+  const float* d_probe_adj;  /* [max_steps][n_probe] dJ/d(probe sample) */
+  float* d_src_adj;          /* [max_steps][n_src] out: dJ/d(source table entry); may be NULL */
+  const float* d_rec;        /* forward record rows rec_step0 .. rec_step0+rec_rows-1 */
+  int64_t rec_step0, rec_rows;
+  rfxh_region region;
+  float* d_grad;             /* [region cells] dJ/d eps_r, accumulated (+=); NULL: no gradient */
+  const float* dcoef;        /* host [n_mat][3]: d ca, d cb, d ce / d eps_r per material id */
+} rfxh_adjoint_desc;
+
+/* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries
+ * across calls, so a run can be swept segment by segment (latest first),
+ * each segment with its own forward record. */
+int rfxh_adjoint(rfxh_plan* plan, const rfxh_adjoint_desc* desc, int64_t lo, int64_t hi,
+                 void* stream);
+
 const char* rfxh_last_error(void);
 int rfxh_version(void);
 

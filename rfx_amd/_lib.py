@@ -61,11 +61,25 @@ class Desc(C.Structure):
     ]
 
 
+class Region(C.Structure):
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("k0", C.c_int32),
+                ("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32)]
+
+
+class AdjointDesc(C.Structure):
+    _fields_ = [
+        ("d_adj", _P * 6), ("d_adj_psi", _P), ("d_gbar", _P),
+        ("d_probe_adj", _P), ("d_src_adj", _P),
+        ("d_rec", _P), ("rec_step0", C.c_int64), ("rec_rows", C.c_int64),
+        ("region", Region), ("d_grad", _P), ("dcoef", _FP),
+    ]
+
+
 _lib = None
 
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
-           "rfxh_last_error", "rfxh_version")
+           "rfxh_run_record", "rfxh_adjoint", "rfxh_last_error", "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -98,6 +112,10 @@ def load():
     lib.rfxh_set_timing.restype = C.c_int
     lib.rfxh_kernel_time_ms.argtypes = [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.rfxh_kernel_time_ms.restype = C.c_int
+    lib.rfxh_run_record.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(Region), _P, _P]
+    lib.rfxh_run_record.restype = C.c_int
+    lib.rfxh_adjoint.argtypes = [_P, C.POINTER(AdjointDesc), C.c_int64, C.c_int64, _P]
+    lib.rfxh_adjoint.restype = C.c_int
     lib.rfxh_last_error.argtypes = []
     lib.rfxh_last_error.restype = C.c_char_p
     lib.rfxh_version.argtypes = []

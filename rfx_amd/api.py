@@ -352,11 +352,13 @@ class Simulation:
     def forward(self, *, eps_override=None, sigma_override=None, mu_r_override=None,
                 n_steps: int | None = None, num_periods: float = 20.0, checkpoint: bool = True,
                 checkpoint_segments: int | None = None, port_s11_freqs=None,
-                skip_preflight: bool = False, **unsupported):
+                skip_preflight: bool = False, grad_region=None, **unsupported):
         """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
         PEC on axes covered by CPML (_execute.py:1420-1421); the returned
         time series carries gradients w.r.t. ``eps_override`` through the
-        hand-written adjoint."""
+        hand-written adjoint.  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
+        limits where dJ/d eps_r is evaluated (default: the whole grid);
+        ``checkpoint_segments`` = K forces K recompute segments."""
         _reject(unsupported, "forward")
         grid = self._build_grid()
         if n_steps is None:
@@ -371,7 +373,7 @@ class Simulation:
         if needs_grad:
             from .adjoint import forward_with_grad
             ts, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
-                                       checkpoint_segments)
+                                       checkpoint_segments, grad_region)
         else:
             dr = DeviceRun(prob, n_steps, self._device)
             dr.run(0, n_steps)

@@ -517,6 +517,398 @@ __global__ __launch_bounds__(256) void k_dft_plane(PlaneParams q) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Reverse mode.  The forward step is linear in the state (E, H, CPML memory)
+// for fixed materials, so its adjoint is the transpose of each sub-step run
+// in reverse order.  A reverse step is two gather kernels:
+//   k_adj_h: H-bar += (E update)^T E-bar + (CPML-E)^T g-bar_E, then the H-probe
+//            adjoint and the CPML-H memory adjoint (writes g-bar_H);
+//   k_adj_e: E-bar <- ca E-bar + (H update)^T H-bar + (CPML-H)^T g-bar_H, then
+//            the prologue of the previous step: E-probe adjoint, source
+//            adjoint record, PEC projection, CPML-E memory adjoint (writes
+//            g-bar_E) and the eps_r gradient from the forward record.
+// k_record stores, for the gradient region, E^n, curl_bwd H^{n+1/2} and the
+// CPML-E term of the forward E update (9 floats per cell per step), so that
+// dE'/d(ca, cb, ce) is known without keeping whole forward states.
+// ---------------------------------------------------------------------------
+struct RegionParams {
+  int i0, j0, k0;   // box origin
+  int ni, nj, nk;   // box extent
+  long long n;      // ni * nj * nk
+};
+
+struct AdjParams {
+  float* a[6];            // adjoint fields: E-bar x, y, z, H-bar x, y, z
+  float* ph[6][2];        // adjoint CPML-H memory (psi layout)
+  float* pe[6][2];        // adjoint CPML-E memory
+  float* gh[6][2];        // g-bar of the CPML-H terms (scratch, psi layout)
+  float* ge[6][2];        // g-bar of the CPML-E terms
+  const float* probe_adj; // [max_steps][n_probe] dJ/dy
+  float* src_adj;         // [max_steps][n_src]   dJ/dw
+  const float* rec;       // [rec_rows][9][region n]
+  long long rec_step0, rec_rows;
+  RegionParams reg;
+  float* grad;            // [region n] dJ/d eps_r
+  const float4* dcoef;    // per material id: d ca, d cb, d ce / d eps_r
+};
+
+__device__ __forceinline__ long long region_index(const RegionParams& r, int i, int j, int k) {
+  const int a = i - r.i0, b = j - r.j0, c = k - r.k0;
+  if (a < 0 || b < 0 || c < 0 || a >= r.ni || b >= r.nj || c >= r.nk) return -1;
+  return ((long long)a * r.nj + b) * r.nk + c;
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParams r, float* rec) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= r.n) return;
+  const int c3 = (int)(q % r.nk);
+  const long long t = q / r.nk;
+  const int i = r.i0 + (int)(t / r.nj), j = r.j0 + (int)(t % r.nj), k = r.k0 + c3;
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const float* Hx = p.f[3];
+  const float* Hy = p.f[4];
+  const float* Hz = p.f[5];
+  const float hx = ldf(Hx, c), hy = ldf(Hy, c), hz = ldf(Hz, c);
+  const float hx_j = j > 0 ? ldf(Hx, c - p.nz) : 0.f, hz_j = j > 0 ? ldf(Hz, c - p.nz) : 0.f;
+  const float hx_k = k > 0 ? ldf(Hx, c - 1) : 0.f, hy_k = k > 0 ? ldf(Hy, c - 1) : 0.f;
+  const float hy_i = i > 0 ? ldf(Hy, c - (unsigned)p.plane) : 0.f;
+  const float hz_i = i > 0 ? ldf(Hz, c - (unsigned)p.plane) : 0.f;
+  float ix, iy, iz;
+  if (GRADED) {
+    ix = p.inv_e[0][i];
+    iy = p.inv_e[1][j];
+    iz = p.inv_e[2][k];
+  } else {
+    ix = iy = iz = p.inv_e0;
+  }
+  float cp[3] = {0.f, 0.f, 0.f};
+  if (in_slab(p, i, j, k)) {
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const unsigned pq = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      const float inv = p.cinv[face];
+      float g0, g1;
+      if (ax == 0) {
+        g0 = (hz - hz_i) * inv;
+        g1 = (hy - hy_i) * inv;
+      } else if (ax == 1) {
+        g0 = (hz - hz_j) * inv;
+        g1 = (hx - hx_j) * inv;
+      } else {
+        g0 = (hy - hy_k) * inv;
+        g1 = (hx - hx_k) * inv;
+      }
+      const float t0 = bb * ldf(ps.er[face][0], pq) + cc * g0 + km1 * g0;
+      const float t1 = bb * ldf(ps.er[face][1], pq) + cc * g1 + km1 * g1;
+      if (ax == 0) {
+        cp[1] -= t0;
+        cp[2] += t1;
+      } else if (ax == 1) {
+        cp[0] += t0;
+        cp[2] -= t1;
+      } else {
+        cp[0] -= t0;
+        cp[1] += t1;
+      }
+    }
+  }
+  const float vals[9] = {ldf(p.f[0], c), ldf(p.f[1], c), ldf(p.f[2], c),
+                         (hz - hz_j) * iy - (hy - hy_k) * iz,
+                         (hx - hx_k) * iz - (hz - hz_i) * ix,
+                         (hy - hy_i) * ix - (hx - hx_j) * iy,
+                         cp[0], cp[1], cp[2]};
+#pragma unroll
+  for (int v = 0; v < 9; ++v) rec[(long long)v * r.n + q] = vals[v];
+}
+
+// ch or cb of the cell's material (w = ch, y = cb)
+__device__ __forceinline__ float4 coef_at(const KParams& p, unsigned c) {
+  return p.coef[ldu16(p.mat, c)];
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long n) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned P = (unsigned)p.plane;
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
+  float ix, iy, iz, ixn, iyn, izn;  // E-update spacings at the cell and at its forward neighbour
+  if (GRADED) {
+    ix = p.inv_e[0][i];
+    iy = p.inv_e[1][j];
+    iz = p.inv_e[2][k];
+    ixn = hi ? p.inv_e[0][i + 1] : 0.f;
+    iyn = hj ? p.inv_e[1][j + 1] : 0.f;
+    izn = hk ? p.inv_e[2][k + 1] : 0.f;
+  } else {
+    ix = iy = iz = p.inv_e0;
+    ixn = hi ? p.inv_e0 : 0.f;
+    iyn = hj ? p.inv_e0 : 0.f;
+    izn = hk ? p.inv_e0 : 0.f;
+  }
+  // v = cb * E-bar at the cell and its forward neighbours
+  const float cb0 = coef_at(p, c).y;
+  const float vx = cb0 * ldf(a.a[0], c), vy = cb0 * ldf(a.a[1], c), vz = cb0 * ldf(a.a[2], c);
+  const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
+  const float cbj = coef_at(p, cj).y, cbk = coef_at(p, ck).y, cbi = coef_at(p, ci).y;
+  const float vx_j = cbj * ldf(a.a[0], cj), vz_j = cbj * ldf(a.a[2], cj);
+  const float vx_k = cbk * ldf(a.a[0], ck), vy_k = cbk * ldf(a.a[1], ck);
+  const float vy_i = cbi * ldf(a.a[1], ci), vz_i = cbi * ldf(a.a[2], ci);
+  float hx = ldf(a.a[3], c), hy = ldf(a.a[4], c), hz = ldf(a.a[5], c);
+  // transpose of E += cb curl_bwd H (yee.py:379-404)
+  hz += vx * iy - vx_j * iyn - vy * ix + vy_i * ixn;
+  hy += -vx * iz + vx_k * izn + vz * ix - vz_i * ixn;
+  hx += vy * iz - vy_k * izn - vz * iy + vz_j * iyn;
+  // transpose of the CPML-E differences g = (H(x) - H(x - e)) / d
+  if (in_slab(p, i, j, k) || in_slab(p, hi ? i + 1 : i, hj ? j + 1 : j, hk ? k + 1 : k)) {
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      if (p.cn[face] == 0) continue;
+      const int ax = face >> 1;
+      const float inv = p.cinv[face];
+      const int l0 = slab_layer(p, face, i, j, k);
+      const bool nb = ax == 0 ? hi : ax == 1 ? hj : hk;
+      const int ii = i + (ax == 0), jj = j + (ax == 1), kk = k + (ax == 2);
+      const int l1 = nb ? slab_layer(p, face, ii, jj, kk) : -1;
+      float d0 = 0.f, d1 = 0.f;
+      if (l0 >= 0) {
+        const unsigned q = (unsigned)psi_idx(face, l0, i, j, k, p.cn[face], p.ny, p.nz);
+        d0 += ldf(a.ge[face][0], q);
+        d1 += ldf(a.ge[face][1], q);
+      }
+      if (l1 >= 0) {
+        const unsigned q = (unsigned)psi_idx(face, l1, ii, jj, kk, p.cn[face], p.ny, p.nz);
+        d0 -= ldf(a.ge[face][0], q);
+        d1 -= ldf(a.ge[face][1], q);
+      }
+      d0 *= inv;
+      d1 *= inv;
+      if (ax == 0) {
+        hz += d0;
+        hy += d1;
+      } else if (ax == 1) {
+        hz += d0;
+        hx += d1;
+      } else {
+        hy += d0;
+        hx += d1;
+      }
+    }
+  }
+  // H-probe adjoints of step n
+  const uint32_t fl = p.flags[ldu16(p.mat, c)];
+  if ((fl & RFXH_POINT) && a.probe_adj) {
+    int lo = 0, hi2 = p.n_ops;
+    while (lo < hi2) {
+      const int mid = (lo + hi2) >> 1;
+      if (p.ops[mid].cell < (long long)c) lo = mid + 1;
+      else hi2 = mid;
+    }
+    for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+      const rfxh_point_op op = p.ops[q];
+      if (op.kind == RFXH_OP_PROBE && op.comp >= 3) {
+        const float g = a.probe_adj[n * p.n_probe + op.slot];
+        if (op.comp == 3) hx += g;
+        else if (op.comp == 4) hy += g;
+        else hz += g;
+      }
+    }
+  }
+  // CPML-H memory adjoint (cpml.py:875-1043 transposed)
+  if (in_slab(p, i, j, k)) {
+    const float ch = coef_at(p, c).w;
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      float s0, s1, h0, h1;  // signs and adjoints of the two H components the face corrects
+      if (ax == 0) {
+        s0 = 1.f; h0 = hy; s1 = -1.f; h1 = hz;
+      } else if (ax == 1) {
+        s0 = -1.f; h0 = hx; s1 = 1.f; h1 = hz;
+      } else {
+        s0 = 1.f; h0 = hx; s1 = -1.f; h1 = hy;
+      }
+      const float t0 = ldf(a.ph[face][0], q) + s0 * ch * h0;
+      const float t1 = ldf(a.ph[face][1], q) + s1 * ch * h1;
+      stf(a.gh[face][0], q, cc * t0 + s0 * ch * km1 * h0);
+      stf(a.gh[face][1], q, cc * t1 + s1 * ch * km1 * h1);
+      stf(a.ph[face][0], q, bb * t0);
+      stf(a.ph[face][1], q, bb * t1);
+    }
+  }
+  stf(a.a[3], c, hx);
+  stf(a.a[4], c, hy);
+  stf(a.a[5], c, hz);
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gather, long long m) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned P = (unsigned)p.plane;
+  const unsigned mid0 = ldu16(p.mat, c);
+  const float4 co = p.coef[mid0];
+  float ex = ldf(a.a[0], c), ey = ldf(a.a[1], c), ez = ldf(a.a[2], c);
+  if (gather) {
+    const bool hj = j > 0, hk = k > 0, hi = i > 0;
+    float ix, iy, iz, ixp, iyp, izp;  // H-update spacings at the cell and its backward neighbour
+    if (GRADED) {
+      ix = p.inv_h[0][i];
+      iy = p.inv_h[1][j];
+      iz = p.inv_h[2][k];
+      ixp = hi ? p.inv_h[0][i - 1] : 0.f;
+      iyp = hj ? p.inv_h[1][j - 1] : 0.f;
+      izp = hk ? p.inv_h[2][k - 1] : 0.f;
+    } else {
+      ix = iy = iz = p.inv_h0;
+      ixp = hi ? p.inv_h0 : 0.f;
+      iyp = hj ? p.inv_h0 : 0.f;
+      izp = hk ? p.inv_h0 : 0.f;
+    }
+    const float ch0 = co.w;
+    const float ux = ch0 * ldf(a.a[3], c), uy = ch0 * ldf(a.a[4], c), uz = ch0 * ldf(a.a[5], c);
+    const unsigned cj = hj ? c - p.nz : c, ck = hk ? c - 1 : c, ci = hi ? c - P : c;
+    const float chj = coef_at(p, cj).w, chk = coef_at(p, ck).w, chi = coef_at(p, ci).w;
+    const float ux_j = chj * ldf(a.a[3], cj), uz_j = chj * ldf(a.a[5], cj);
+    const float ux_k = chk * ldf(a.a[3], ck), uy_k = chk * ldf(a.a[4], ck);
+    const float uy_i = chi * ldf(a.a[4], ci), uz_i = chi * ldf(a.a[5], ci);
+    // transpose of H -= ch curl_fwd E (yee.py:272-289), plus ca E-bar
+    ex = co.x * ex + uy * iz - uy_k * izp - uz * iy + uz_j * iyp;
+    ey = co.x * ey - ux * iz + ux_k * izp + uz * ix - uz_i * ixp;
+    ez = co.x * ez + ux * iy - ux_j * iyp - uy * ix + uy_i * ixp;
+    // transpose of the CPML-H differences g = (E(x + e) - E(x)) / d
+    if (in_slab(p, i, j, k) || in_slab(p, hi ? i - 1 : i, hj ? j - 1 : j, hk ? k - 1 : k)) {
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        if (p.cn[face] == 0) continue;
+        const int ax = face >> 1;
+        const float inv = p.cinv[face];
+        const int l0 = slab_layer(p, face, i, j, k);
+        const bool nb = ax == 0 ? hi : ax == 1 ? hj : hk;
+        const int ii = i - (ax == 0), jj = j - (ax == 1), kk = k - (ax == 2);
+        const int l1 = nb ? slab_layer(p, face, ii, jj, kk) : -1;
+        float d0 = 0.f, d1 = 0.f;
+        if (l1 >= 0) {
+          const unsigned q = (unsigned)psi_idx(face, l1, ii, jj, kk, p.cn[face], p.ny, p.nz);
+          d0 += ldf(a.gh[face][0], q);
+          d1 += ldf(a.gh[face][1], q);
+        }
+        if (l0 >= 0) {
+          const unsigned q = (unsigned)psi_idx(face, l0, i, j, k, p.cn[face], p.ny, p.nz);
+          d0 -= ldf(a.gh[face][0], q);
+          d1 -= ldf(a.gh[face][1], q);
+        }
+        d0 *= inv;
+        d1 *= inv;
+        if (ax == 0) {
+          ez += d0;
+          ey += d1;
+        } else if (ax == 1) {
+          ez += d0;
+          ex += d1;
+        } else {
+          ey += d0;
+          ex += d1;
+        }
+      }
+    }
+  }
+  if (m >= 0) {
+    // prologue of step m: probes read E after injection, injection follows
+    // the PEC projection (rfx/simulation.py:1392-1567)
+    const uint32_t fl = p.flags[mid0];
+    if (fl & RFXH_POINT) {
+      int lo = 0, hi2 = p.n_ops;
+      while (lo < hi2) {
+        const int md = (lo + hi2) >> 1;
+        if (p.ops[md].cell < (long long)c) lo = md + 1;
+        else hi2 = md;
+      }
+      for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+        const rfxh_point_op op = p.ops[q];
+        if (op.kind == RFXH_OP_PROBE && op.comp < 3 && a.probe_adj) {
+          const float g = a.probe_adj[m * p.n_probe + op.slot];
+          if (op.comp == 0) ex += g;
+          else if (op.comp == 1) ey += g;
+          else ez += g;
+        }
+      }
+      for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+        const rfxh_point_op op = p.ops[q];
+        if (op.kind == RFXH_OP_INJECT && a.src_adj)
+          a.src_adj[m * p.n_src + op.slot] = pick3(op.comp, ex, ey, ez);
+      }
+    }
+    if (p.pec_axes & 1) {
+      if (i == 0 || i == p.nx - 1) ey = ez = 0.f;
+    }
+    if (p.pec_axes & 2) {
+      if (j == 0 || j == p.ny - 1) ex = ez = 0.f;
+    }
+    if (p.pec_axes & 4) {
+      if (k == 0 || k == p.nz - 1) ex = ey = 0.f;
+      if (k == p.nz - 1) ez = 0.f;
+    }
+    if (fl & RFXH_ZERO_EX) ex = 0.f;
+    if (fl & RFXH_ZERO_EY) ey = 0.f;
+    if (fl & RFXH_ZERO_EZ) ez = 0.f;
+    // CPML-E memory adjoint (cpml.py:608-777 transposed)
+    if (in_slab(p, i, j, k)) {
+      const float ce = co.z;
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        const int l = slab_layer(p, face, i, j, k);
+        if (l < 0) continue;
+        const int ax = face >> 1;
+        const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+        const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+        float s0, s1, e0v, e1v;
+        if (ax == 0) {
+          s0 = -1.f; e0v = ey; s1 = 1.f; e1v = ez;
+        } else if (ax == 1) {
+          s0 = 1.f; e0v = ex; s1 = -1.f; e1v = ez;
+        } else {
+          s0 = -1.f; e0v = ex; s1 = 1.f; e1v = ey;
+        }
+        const float t0 = ldf(a.pe[face][0], q) + s0 * ce * e0v;
+        const float t1 = ldf(a.pe[face][1], q) + s1 * ce * e1v;
+        stf(a.ge[face][0], q, cc * t0 + s0 * ce * km1 * e0v);
+        stf(a.ge[face][1], q, cc * t1 + s1 * ce * km1 * e1v);
+        stf(a.pe[face][0], q, bb * t0);
+        stf(a.pe[face][1], q, bb * t1);
+      }
+    }
+    // eps_r gradient from the forward record of step m
+    if (a.grad) {
+      const long long r = region_index(a.reg, i, j, k);
+      const long long row = m - a.rec_step0;
+      if (r >= 0 && row >= 0 && row < a.rec_rows) {
+        const float* rr = a.rec + row * 9 * a.reg.n + r;
+        const long long N = a.reg.n;
+        const float A = ex * rr[0] + ey * rr[N] + ez * rr[2 * N];
+        const float B = ex * rr[3 * N] + ey * rr[4 * N] + ez * rr[5 * N];
+        const float C = ex * rr[6 * N] + ey * rr[7 * N] + ez * rr[8 * N];
+        const float4 dc = a.dcoef[mid0];
+        a.grad[r] += dc.x * A + dc.y * B + dc.z * C;
+      }
+    }
+  }
+  stf(a.a[0], c, ex);
+  stf(a.a[1], c, ey);
+  stf(a.a[2], c, ez);
+}
+
 __global__ void k_set_step(long long* step, long long value) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *step = value;
 }
@@ -546,6 +938,8 @@ struct rfxh_plan {
   double t_sum[3] = {0, 0, 0};
   long long t_cnt[3] = {0, 0, 0};
   std::vector<hipEvent_t> tev;
+  float4* dcoef = nullptr;      // per material d(ca, cb, ce)/d eps_r (reverse mode)
+  int n_mat = 0;
 };
 
 namespace {
@@ -784,6 +1178,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   kp.port_out = d->d_port_out;
   kp.n_port = d->n_port;
   pl->max_steps = d->max_steps;
+  pl->n_mat = d->n_mat;
   {
     const long long init = -1;
     long long* st;
@@ -943,6 +1338,158 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
   }
   HIPCHK(hipEventRecord(pl->ev_out, pl->cap_stream));
   HIPCHK(hipStreamWaitEvent(s, pl->ev_out, 0));
+  return RFXH_OK;
+}
+
+static int check_region(const rfxh_plan* pl, const rfxh_region* r) {
+  if (!r || r->ni < 1 || r->nj < 1 || r->nk < 1 || r->i0 < 0 || r->j0 < 0 || r->k0 < 0 ||
+      r->i0 + r->ni > pl->kp.nx || r->j0 + r->nj > pl->kp.ny || r->k0 + r->nk > pl->kp.nz)
+    return fail(RFXH_EINVAL, "gradient region outside the grid");
+  return RFXH_OK;
+}
+
+static RegionParams region_params(const rfxh_region* r) {
+  RegionParams rp;
+  rp.i0 = r->i0;
+  rp.j0 = r->j0;
+  rp.k0 = r->k0;
+  rp.ni = r->ni;
+  rp.nj = r->nj;
+  rp.nk = r->nk;
+  rp.n = (long long)r->ni * r->nj * r->nk;
+  return rp;
+}
+
+static PsiIO psi_io(const rfxh_plan* pl, int set) {
+  PsiIO ps;
+  for (int f = 0; f < 6; ++f)
+    for (int t = 0; t < 2; ++t) {
+      const long long o = set * pl->psi_set;
+      float* h0 = pl->kp.psi_h[f][t];
+      float* e0 = pl->kp.psi_e[f][t];
+      ps.hr[f][t] = h0 ? h0 + o : nullptr;
+      ps.hw[f][t] = h0 ? h0 + o : nullptr;
+      ps.er[f][t] = e0 ? e0 + o : nullptr;
+      ps.ew[f][t] = e0 ? e0 + o : nullptr;
+    }
+  return ps;
+}
+
+int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
+                    float* d_rec, void* stream) {
+  if (!pl) return fail(RFXH_EINVAL, "null plan");
+  if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
+    return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
+  int rc = check_region(pl, region);
+  if (rc) return rc;
+  if (!d_rec) return fail(RFXH_EINVAL, "record buffer missing");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const RegionParams rp = region_params(region);
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, pl->kp.step, (long long)step0);
+  // two-pass sub-steps in place on the current set, with the record taken
+  // between the H and the E update (E^n and H^{n+1/2} both in memory)
+  KParams kp = pl->kp;
+  for (int c = 0; c < 6; ++c) kp.f[c] = pl->set[pl->cur][c];
+  const PsiIO ps = psi_io(pl, pl->cur);
+  const unsigned rb = (unsigned)((rp.n + 255) / 256);
+  for (int64_t n = 0; n < n_steps; ++n) {
+    if (kp.graded) {
+      hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, kp, ps);
+      hipLaunchKernelGGL(k_record<true>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
+      hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, kp, ps);
+    } else {
+      hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, kp, ps);
+      hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
+      hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, kp, ps);
+    }
+    for (size_t q = 0; q < pl->planes.size(); ++q) {
+      PlaneParams pp = pl->planes[q];
+      pp.fld = pl->set[pl->cur][pl->plane_comp[q]];
+      const long long npts = (long long)pp.n1 * pp.n2;
+      hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
+    }
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
+  }
+  HIPCHK(hipGetLastError());
+  return RFXH_OK;
+}
+
+int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
+  if (!pl || !ad) return fail(RFXH_EINVAL, "null plan or adjoint descriptor");
+  if (lo < 0 || hi < lo || hi > pl->max_steps) return fail(RFXH_EINVAL, "bad step range");
+  for (int c = 0; c < 6; ++c)
+    if (!ad->d_adj[c]) return fail(RFXH_EINVAL, "adjoint field pointer missing");
+  bool any_cpml = false;
+  for (int f = 0; f < 6; ++f) any_cpml = any_cpml || pl->kp.cn[f] > 0;
+  if (any_cpml && (!ad->d_adj_psi || !ad->d_gbar)) return fail(RFXH_EINVAL, "adjoint CPML buffers missing");
+  if (pl->kp.n_probe > 0 && !ad->d_probe_adj) return fail(RFXH_EINVAL, "probe adjoint table missing");
+  if (ad->d_grad) {
+    int rc = check_region(pl, &ad->region);
+    if (rc) return rc;
+    if (!ad->d_rec || !ad->dcoef || ad->rec_rows < 0) return fail(RFXH_EINVAL, "forward record missing");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  AdjParams a;
+  std::memset(&a, 0, sizeof(a));
+  for (int c = 0; c < 6; ++c) a.a[c] = ad->d_adj[c];
+  // adjoint CPML memory and g-bar share the psi layout
+  {
+    // offsets of each face/term inside one psi-sized buffer (as in the forward)
+    long long off = 0;
+    for (int f = 0; f < 6; ++f) {
+      const int n = pl->kp.cn[f];
+      const int ax = f >> 1;
+      long long sz = n;
+      if (ax != 0) sz *= pl->kp.nx;
+      if (ax != 1) sz *= pl->kp.ny;
+      if (ax != 2) sz *= pl->kp.nz;
+      if (n > 0) {
+        a.ph[f][0] = ad->d_adj_psi + off;
+        a.ph[f][1] = ad->d_adj_psi + off + sz;
+        a.pe[f][0] = ad->d_adj_psi + off + 2 * sz;
+        a.pe[f][1] = ad->d_adj_psi + off + 3 * sz;
+        a.gh[f][0] = ad->d_gbar + off;
+        a.gh[f][1] = ad->d_gbar + off + sz;
+        a.ge[f][0] = ad->d_gbar + off + 2 * sz;
+        a.ge[f][1] = ad->d_gbar + off + 3 * sz;
+      }
+      off += 4 * sz;
+    }
+  }
+  a.probe_adj = ad->d_probe_adj;
+  a.src_adj = ad->d_src_adj;
+  if (ad->d_grad) {
+    if (!pl->dcoef) {
+      std::vector<float4> tmp(pl->n_mat);
+      for (int m = 0; m < pl->n_mat; ++m)
+        tmp[m] = make_float4(ad->dcoef[3 * m], ad->dcoef[3 * m + 1], ad->dcoef[3 * m + 2], 0.f);
+      int rc = dev_copy(pl, tmp.data(), tmp.size(), &pl->dcoef);
+      if (rc) return rc;
+    }
+    a.rec = ad->d_rec;
+    a.rec_step0 = ad->rec_step0;
+    a.rec_rows = ad->rec_rows;
+    a.reg = region_params(&ad->region);
+    a.grad = ad->d_grad;
+    a.dcoef = pl->dcoef;
+  }
+  if (hi == lo) return RFXH_OK;
+  // prologue of step hi-1 (E-bar holds the adjoint of E^{hi})
+  if (pl->kp.graded)
+    hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, 0, (long long)(hi - 1));
+  else
+    hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, 0, (long long)(hi - 1));
+  for (int64_t n = hi - 1; n >= lo; --n) {
+    const long long m = n - 1 >= lo ? (long long)(n - 1) : -1;
+    if (pl->kp.graded) {
+      hipLaunchKernelGGL(k_adj_h<true>, pl->grid, pl->block, 0, s, pl->kp, a, (long long)n);
+      hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, 1, m);
+    } else {
+      hipLaunchKernelGGL(k_adj_h<false>, pl->grid, pl->block, 0, s, pl->kp, a, (long long)n);
+      hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, 1, m);
+    }
+  }
+  HIPCHK(hipGetLastError());
   return RFXH_OK;
 }
 

@@ -1,0 +1,165 @@
+"""Reverse mode on an MI355X: transpose identity, finite differences, the
+gradient-ladder fixture, segmented recomputation."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cstep
+from oracle import fdtd_np as o
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import rfx_amd  # noqa: F401
+    return torch
+
+
+@pytest.mark.parametrize("lane", ["forward", "run"])
+def test_adjoint_is_transpose_of_step(torch_mod, lane):
+    """<F x0, g> == <x0, F^T g> for random initial fields/CPML memory and
+    random probe weights, no sources (the step is then linear)."""
+    torch = torch_mod
+    _say(f"adjoint dot-product test ({lane} lane)")
+    import ctypes as C
+    from rfx_amd import _lib
+    from rfx_amd.adjoint import AdjointRunner
+    spec = cases.mixed_case(lane)
+    spec["ports"] = []
+    spec["s11_freqs"] = None
+    spec["probes"] = spec["probes"] + [((0.012, 0.003, 0.007), "ey"), ((0.002, 0.010, 0.002), "hy")]
+    sim = cases.to_sim(spec)
+    n = 60
+    _, prob, _ = sim._build_problem(n, lane)
+    run = AdjointRunner(prob, n, "cuda")
+    dr = run.dr
+    g = torch.Generator(device="cpu").manual_seed(7)
+    x0 = [torch.randn(prob.shape, generator=g).cuda() for _ in range(6)]
+    psi0 = torch.randn(dr.psis[0].numel(), generator=g).cuda()
+    dr.load_state(x0, psi0)
+    ts = run.forward()
+    w = torch.randn(ts.shape, generator=g).cuda()
+    lhs = float((ts.double() * w.double()).sum())
+    # reverse sweep with the probe weights as dJ/dy and no gradient output
+    T = n
+    probe_adj = w.float().contiguous()
+    npsi = psi0.numel()
+    adj = [torch.zeros(prob.shape, device="cuda") for _ in range(6)]
+    adj_psi = torch.zeros(npsi, device="cuda")
+    gbar = torch.zeros(npsi, device="cuda")
+    d = _lib.AdjointDesc()
+    for c in range(6):
+        d.d_adj[c] = adj[c].data_ptr()
+    d.d_adj_psi = adj_psi.data_ptr()
+    d.d_gbar = gbar.data_ptr()
+    d.d_probe_adj = probe_adj.data_ptr()
+    _lib.check(dr.lib.rfxh_adjoint(dr.plan, C.byref(d), 0, T, dr.stream_ptr()))
+    torch.cuda.synchronize()
+    rhs = sum(float((a.double() * x.double()).sum()) for a, x in zip(adj, x0))
+    rhs += float((adj_psi.double() * psi0.double()).sum())
+    # float32 rounding scales with |F x0| |g|, not with the inner product
+    scale = float(ts.double().norm() * w.double().norm())
+    print("dot test", lhs, rhs, abs(lhs - rhs) / scale)
+    assert abs(lhs - rhs) <= 1e-5 * scale
+
+
+def _ladder_J(torch, sim, eps, n, dt, region=None, segments=None):
+    res = sim.forward(eps_override=eps, n_steps=n, grad_region=region,
+                      checkpoint_segments=segments)
+    ts = res.time_series[:, 0].double()
+    t = torch.arange(n, dtype=torch.float64, device=ts.device) * dt
+    ph = torch.exp(-1j * 2 * np.pi * 8e9 * t)
+    acc = (ts.to(torch.complex128) * ph).sum()
+    return (acc.real ** 2 + acc.imag ** 2), res
+
+
+def test_ladder_gradient_vs_fixture_and_fd(torch_mod):
+    """Slab-eps gradient on the lambda/20 ladder rung vs the reference's
+    jax.grad and the float64 oracle's central difference (<= 1e-3)."""
+    torch = torch_mod
+    _say("ladder rung lambda/20 gradient")
+    lad = cases.ladder()["rungs"][0]
+    spec, n, m1d = cases.ladder_case(20, 4.0)
+    sim = cases.to_sim(spec)
+    eps = torch.tensor(spec["eps_override"], requires_grad=True)
+    J, res = _ladder_J(torch, sim, eps, n, sim._build_grid().dt)
+    J.backward()
+    g = eps.grad.numpy()
+    d_slab = float(g[m1d].sum())
+    # float64 oracle central difference, step 1e-3
+    h = 1e-3
+    vals = []
+    for e in (4.0 + h, 4.0 - h):
+        sp, _, _ = cases.ladder_case(20, e)
+        gg, cfg = o.build_uniform(sp, n)
+        out = cstep.run_c(cfg, n, want_fields=False)
+        vals.append(cases.ladder_objective(out.time_series[:, 0], gg.dt))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    print("grad slab", d_slab, "oracle FD", fd, "jax.grad", lad["grad_ad"])
+    assert abs(d_slab - fd) / abs(fd) < 1e-3
+    assert abs(d_slab - lad["grad_ad"]) / abs(lad["grad_ad"]) < 1e-3
+
+
+def test_gradient_directions_vs_oracle_fd(torch_mod):
+    """Mixed lossy/PEC/CPML case, gradient region inside and across the CPML:
+    directional derivatives vs float64 oracle central differences."""
+    torch = torch_mod
+    _say("mixed-case directional FD")
+    spec = cases.mixed_case("forward")
+    spec["ports"] = [p for p in spec["ports"] if p[2] == 0.0]
+    spec["s11_freqs"] = None
+    n = 160
+    sim = cases.to_sim(spec)
+    g0, cfg0 = o.build_uniform(spec, n)
+    base = cfg0.eps_r.copy()
+    region = (2, 14, 3, 17, 2, 10)   # crosses the 4-layer CPML on x_lo, y_lo, z_lo
+    eps = torch.tensor(base, requires_grad=True)
+    res = sim.forward(eps_override=eps, n_steps=n, grad_region=region)
+    ts = res.time_series.double()
+    wts = torch.linspace(0.5, 1.5, ts.shape[1], dtype=torch.float64, device=ts.device)
+    J = (ts ** 2 * wts).sum()
+    J.backward()
+    grad = eps.grad.numpy().astype(np.float64)
+    rng = np.random.default_rng(3)
+    i0, i1, j0, j1, k0, k1 = region
+    for trial in range(3):
+        d = np.zeros_like(base, dtype=np.float64)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        h = 1e-3
+        Js = []
+        for sgn in (1, -1):
+            sp = dict(spec)
+            sp["eps_override"] = (base + sgn * h * d).astype(np.float32)
+            gg, cfg = o.build_uniform(sp, n)
+            out = cstep.run_c(cfg, n, want_fields=False)
+            Js.append(float((out.time_series ** 2 * np.linspace(0.5, 1.5, ts.shape[1])).sum()))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float((grad * d).sum())
+        print("direction", trial, ad, fd, abs(ad - fd) / abs(fd))
+        assert abs(ad - fd) <= 2e-3 * abs(fd)
+
+
+def test_segmented_gradient_equals_single(torch_mod):
+    torch = torch_mod
+    _say("checkpoint segments == single segment")
+    spec, n, m1d = cases.ladder_case(20, 4.0)
+    sim = cases.to_sim(spec)
+    grads = []
+    for segs in (None, 7):
+        eps = torch.tensor(spec["eps_override"], requires_grad=True)
+        J, _ = _ladder_J(torch, sim, eps, n, sim._build_grid().dt, segments=segs)
+        J.backward()
+        grads.append(eps.grad.numpy().copy())
+    np.testing.assert_allclose(grads[1], grads[0], rtol=1e-5,
+                               atol=1e-6 * np.abs(grads[0]).max())

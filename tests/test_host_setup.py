@@ -72,3 +72,27 @@ def test_coefficient_table_float32_order():
     np.testing.assert_array_equal(coef[:, 0], (1 - L) / (1 + L))
     np.testing.assert_array_equal(coef[:, 1], (dt / eps) / (1 + L))
     np.testing.assert_array_equal(coef[:, 3], dt / (mu * np.float32(o.MU_0)))
+
+
+def test_eps_derivative_table_matches_finite_difference():
+    """d(ca, cb, ce)/d eps_r of the reverse mode vs a float64 central
+    difference of the coefficient formulas."""
+    from rfx_amd.adjoint import eps_derivative_table
+    from rfx_amd.materials import MaterialMap
+    dt = 1.9e-12
+    m = MaterialMap((2, 2, 2))
+    for e, s in ((2.2, 0.01), (4.4, 0.0), (9.0, 0.05), (1.0, 20.0)):
+        m.get_id(e, s, 1.0)
+    tab = eps_derivative_table(m, dt)
+    eps_r, sig, _, _ = m.props()
+
+    def coefs(e, s):
+        eps = e * o.EPS_0
+        L = s * dt / (2 * eps)
+        return np.array([(1 - L) / (1 + L), (dt / eps) / (1 + L), dt / eps])
+
+    for q in range(len(eps_r)):
+        e, s = float(eps_r[q]), float(sig[q])
+        h = 1e-6 * e
+        fd = (coefs(e + h, s) - coefs(e - h, s)) / (2 * h)
+        np.testing.assert_allclose(tab[q], fd, rtol=1e-5, atol=1e-30)
