@@ -97,7 +97,11 @@ class GradedGrid:
     Profiles are the PHYSICAL interior cell sizes; each axis is padded by
     ``cpml_layers`` boundary-size cells per face (_pad_profile :98-128)
     and gets one duplicated bounding cell (_append_bounding_node :130-155).
-    dt = 0.99 / (c sqrt(sum 1/dmin^2)) with c = 1/sqrt(eps0 mu0) (:386-401).
+    dt = 0.99 / (c sqrt(sum 1/dmin^2)) with c = 1/sqrt(eps0 mu0) (:386-401),
+    from the float64 padded profiles.  Everything downstream of the grid
+    reads the float32 cell-size arrays (``dx_arr``/``dy_arr``/``dz`` are
+    jnp float32, :403-405): inverse spacings, node coordinates, index
+    lookup, source volumes and port metrics.
     """
 
     def __init__(self, dz_profile, dx, domain_xy, cpml_layers=8,
@@ -118,6 +122,7 @@ class GradedGrid:
         if dy_profile is None:
             dy_profile = np.full(int(round(domain_xy[1] / dx)), float(dx))
         self.d_full = [_full(dx_profile), _full(dy_profile), _full(dz_profile)]
+        self.d32 = [d.astype(F32) for d in self.d_full]
         self.shape = tuple(len(d) for d in self.d_full)
         dmins = [float(np.min(d)) for d in self.d_full]
         self.dt = float(0.99 / (C0_NU * np.sqrt(sum(1.0 / m ** 2 for m in dmins))))
@@ -125,27 +130,55 @@ class GradedGrid:
         self.dy = float(np.asarray(dy_profile)[0])
         # rfx/nonuniform.py:212-224 (float32 arrays)
         self.inv_e, self.inv_h = [], []
-        for d in self.d_full:
-            a = d.astype(F32)
+        for a in self.d32:
             inv_local = F32(1.0) / a
             inv_mean = F32(2.0) / (a[:-1] + a[1:])
             self.inv_h.append(np.concatenate([inv_local[:-1], np.zeros(1, F32)]))
             self.inv_e.append(np.concatenate([inv_local[:1], inv_mean]))
-        # CPML face cell sizes (rfx/boundaries/cpml.py:174-197)
-        dz = self.d_full[2]
+        # CPML face cell sizes (rfx/boundaries/cpml.py:174-197): the scalar
+        # boundary dx / dy and the float32 first/last dz entries
+        dz = self.d32[2]
         self.face_d = {"x": (self.dx, self.dx), "y": (self.dy, self.dy),
                        "z": (float(dz[0]), float(dz[-1]))}
 
-    def node_positions(self, axis):
-        """rfx/geometry/rasterize_grid.py:84 node coordinates (float32).
+    def num_timesteps(self, freq_max, num_periods=20.0):
+        return int(np.ceil(num_periods * (1.0 / freq_max) / self.dt))
 
-        Node 0 sits at -(sum of the lo pad cells); node k at the cumulative
-        sum of the first k cells, offset so the first interior node is 0.
-        """
-        d = self.d_full[axis]
-        edges = np.concatenate([[0.0], np.cumsum(d[:-1])])
-        off = float(np.sum(d[:self.pad]))
-        return (edges - off).astype(F32)
+    def node_positions(self, axis):
+        """rfx/geometry/rasterize_grid.py:60-116: float64 cumulative sums of
+        the float32 cell sizes, origin at the first interior node, cast to
+        float32."""
+        d = self.d32[axis].astype(np.float64)
+        edges = np.concatenate([[0.0], np.cumsum(d)])
+        return (edges[:-1] - edges[self.pad]).astype(F32)
+
+    def node_coords_f32(self):
+        return [self.node_positions(a) for a in range(3)]
+
+    def position_to_index(self, pos):
+        """rfx/nonuniform.py:426-517: nearest interior cell edge, with edges
+        the float32 running sum of the float32 interior cells (numpy cumsum
+        keeps float32) and the distance taken in float32; first minimum."""
+        idx = []
+        for a in range(3):
+            d = self.d32[a]
+            interior = d[self.pad:len(d) - self.pad - 1]
+            edges = np.insert(np.cumsum(interior), 0, 0.0)
+            dist = np.abs(edges - F32(pos[a]))
+            idx.append(int(np.argmin(dist)) + self.pad)
+        return tuple(idx)
+
+    def dual64(self, axis, k):
+        """e_node_dual_spacing_at on the float64 view of the float32 profile
+        (rfx/nonuniform.py:253-271 with np.float64 input)."""
+        d = self.d32[axis].astype(np.float64)
+        return float(d[0]) if k <= 0 else float(0.5 * (d[k - 1] + d[k]))
+
+    def dual_z32(self, k):
+        """The same on the jnp float32 ``grid.dz`` (float32 arithmetic), as
+        the port metrics read it (rfx/runners/nonuniform.py:773, 854)."""
+        d = self.d32[2]
+        return float(d[0]) if k <= 0 else float(F32(0.5) * (d[k - 1] + d[k]))
 
     def spacing(self):
         return dict(kind="graded", inv_e=self.inv_e, inv_h=self.inv_h)
@@ -762,3 +795,162 @@ def simulate_uniform(spec, n_steps, dtype=np.float64):
     g, cfg = build_uniform(spec, n_steps)
     out = run_loop(cfg, n_steps, dtype)
     return g, cfg, out
+
+
+# ---------------------------------------------------------------------------
+# Graded-z lane set-up (rfx/runners/nonuniform.py:465-925, rfx/nonuniform.py)
+# ---------------------------------------------------------------------------
+
+def current_source_f32(g: GradedGrid, eps_r, sigma, idx, comp, dt, wave_f32):
+    """rfx/nonuniform.py:588-691 make_current_source: w = f32(Cb / dV) * s(t),
+    Cb from the cell's (float64-promoted) eps_r and sigma, dV = primal width
+    on the component's axis x dual spacings on the two transverse axes."""
+    eps = float(eps_r[idx]) * EPS_0
+    sig = float(sigma[idx])
+    loss = sig * dt / (2.0 * eps)
+    cb = (dt / eps) / (1.0 + loss)
+    a = {"ex": 0, "ey": 1, "ez": 2}[comp]
+    w = []
+    for ax in range(3):
+        if ax == a:
+            w.append(float(g.d32[ax][idx[ax]]))
+        else:
+            w.append(g.dual64(ax, idx[ax]))
+    dV = w[0] * w[1] * w[2]
+    return (F32(cb / dV) * wave_f32).astype(F32)
+
+
+def _graded_port_metrics(g: GradedGrid, cell, axis):
+    """d_parallel (primal) and the two transverse dual spacings of a port
+    cell (rfx/runners/nonuniform.py:766-796, 846-868)."""
+    ci, cj, ck = cell
+    dxi = float(g.d32[0][ci])
+    dyj = float(g.d32[1][cj])
+    dual_x = g.dual64(0, ci)
+    dual_y = g.dual64(1, cj)
+    dual_z = g.dual_z32(ck)
+    if axis == 2:
+        return float(g.d32[2][ck]), dual_x, dual_y, (dual_x, dual_y, dual_z)
+    if axis == 1:
+        return dyj, dual_x, dual_z, (dual_x, dual_y, dual_z)
+    return dxi, dual_y, dual_z, (dual_x, dual_y, dual_z)
+
+
+def build_graded(spec, n_steps):
+    """StepConfig for the graded-z lane (Simulation(dz_profile=...).run()).
+
+    spec keys as build_uniform plus ``dz_profile``; ``domain`` is (Lx, Ly)
+    (z comes from the profile); a port entry may carry a sixth element
+    ``extent`` (wire port).  Order of set-up (rfx/runners/nonuniform.py):
+    grid (build_nonuniform_grid :25-60, dx default C0/freq_max/20),
+    materials on node coordinates + CPML pad extension
+    (assemble_materials_nu :63-140), optional eps/sigma override
+    (:630-643), ports in registration order (:694-887): soft sources are
+    current sources normalised by the CONCRETE (pre-override, pre-fold)
+    materials; a single-cell port folds sigma = d_par/(Z0 dual1 dual2) and
+    injects a current source; a wire port folds n_live d/(Z0 dual1 dual2)
+    on every live cell and injects source/n_live on each, and records V/I
+    at its middle cell.  Outer-face PEC always (rfx/nonuniform.py:1319);
+    CPML kappa_max 1 (init_cpml reads the grid's kappa_max, which the
+    graded grid does not carry, rfx/boundaries/cpml.py:391-392).
+    """
+    boundary = spec.get("boundary", "cpml")
+    layers = spec.get("cpml_layers", 16) if boundary == "cpml" else 0
+    dx = spec.get("dx")
+    if dx is None:
+        dx = C0 / spec["freq_max"] / 20.0
+    g = GradedGrid(spec["dz_profile"], dx, spec["domain"][:2], layers)
+    shape = g.shape
+    coords = g.node_coords_f32()
+    eps_r, sigma, mu_r, pec = assemble_materials(
+        shape, coords, spec.get("geometry", []), spec.get("materials", {}), layers)
+    eps_c, sig_c = eps_r.copy(), sigma.copy()
+    if spec.get("eps_override") is not None:
+        eps_r = np.asarray(spec["eps_override"], F32).reshape(shape).copy()
+    if spec.get("sigma_override") is not None:
+        sigma = np.asarray(spec["sigma_override"], F32).reshape(shape).copy()
+    dt = g.dt
+    tms = times_f32(n_steps, dt)
+    sources, ports_vi = [], []
+    s11_freqs = spec.get("s11_freqs")
+    for entry in spec.get("ports", []):
+        pos, comp, z0, pulse, excite = entry[:5]
+        extent = entry[5] if len(entry) > 5 else None
+        idx = g.position_to_index(pos)
+        if pulse is None and excite:
+            pulse = Pulse(spec["freq_max"] / 2, 0.8)
+        wave = None
+        if pulse is not None:
+            wave = gaussian_pulse_f32(tms, pulse.f0, pulse.bandwidth, pulse.amplitude,
+                                      pulse.cutoff)
+        axis = {"ex": 0, "ey": 1, "ez": 2}[comp]
+        if z0 == 0.0:
+            sources.append((idx, comp, current_source_f32(g, eps_c, sig_c, idx, comp, dt, wave)))
+            continue
+        if extent is None:
+            d_par, p1, p2, _ = _graded_port_metrics(g, idx, axis)
+            sigma = sigma.copy()
+            sigma[idx] = F32(sigma[idx] + F32(d_par / (z0 * p1 * p2)))
+            if pec is not None:
+                pec = pec.copy()
+                pec[idx] = False
+            if excite:
+                sources.append((idx, comp, current_source_f32(g, eps_c, sig_c, idx, comp, dt,
+                                                              wave)))
+            continue
+        end = list(pos)
+        end[axis] += extent
+        idx_end = g.position_to_index(tuple(end))
+        lo, hi = sorted((idx[axis], idx_end[axis]))
+        cells = []
+        for k in range(lo, hi + 1):
+            c = list(idx)
+            c[axis] = k
+            cells.append(tuple(c))
+        live = [True if pec is None else not bool(pec[c]) for c in cells]
+        n_live = sum(live)
+        if n_live == 0:
+            raise ValueError("wire port: every extent cell lies inside PEC")
+        sigma = sigma.copy()
+        for c, lv in zip(cells, live):
+            if not lv:
+                continue
+            d_par, p1, p2, _ = _graded_port_metrics(g, c, axis)
+            sigma[c] = F32(sigma[c] + F32(n_live * d_par / (z0 * p1 * p2)))
+            if pec is not None:
+                pec = pec.copy()
+                pec[c] = False
+        if excite:
+            for c, lv in zip(cells, live):
+                if lv:
+                    w = current_source_f32(g, eps_c, sig_c, c, comp, dt, wave)
+                    sources.append((c, comp, (w / n_live).astype(F32)))
+        mid = cells[len(cells) // 2]
+        if s11_freqs is not None:
+            d_par, _, _, loop = _graded_port_metrics(g, mid, axis)
+            ports_vi.append((mid, comp, np.asarray(s11_freqs, F32), d_par, loop))
+    probes = [(g.position_to_index(p), c) for p, c in spec.get("probes", [])]
+    planes = []
+    for axis_name, coord, comp, freqs, nf in spec.get("dft_planes", []):
+        a = "xyz".index(axis_name)
+        pos = [0.0, 0.0, 0.0]
+        pos[a] = coord
+        index = g.position_to_index(tuple(pos))[a]
+        if freqs is None:
+            freqs = np.linspace(spec["freq_max"] / 10, spec["freq_max"], nf).astype(F32)
+        planes.append((a, index, comp, np.asarray(freqs, F32)))
+    profiles, face_d = {}, {}
+    if layers > 0:
+        for ax, name in enumerate("xyz"):
+            d_lo, d_hi = g.face_d[name]
+            b, c, k = cpml_profile(layers, dt, d_lo, 1.0)
+            profiles[(ax, 0)] = (b, c, k)
+            b, c, k = cpml_profile(layers, dt, d_hi, 1.0)
+            profiles[(ax, 1)] = (b[::-1].copy(), c[::-1].copy(), k[::-1].copy())
+            face_d[ax] = (d_lo, d_hi)
+    cfg = StepConfig(shape=shape, dt=dt, spacing=g.spacing(), eps_r=eps_r, sigma=sigma,
+                     mu_r=mu_r, cpml_n=layers, cpml_axes="xyz" if layers > 0 else "",
+                     cpml_profiles=profiles, cpml_face_d=face_d, pec_axes="xyz",
+                     pec_mask=pec, sources=sources, probes=probes, dft_planes=planes,
+                     ports_vi=ports_vi, record_order="graded")
+    return g, cfg

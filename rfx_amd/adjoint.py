@@ -142,9 +142,11 @@ class AdjointRunner:
         i0, i1, j0, j1, k0, k1 = self.box
         full = torch.zeros(self.prob.shape, dtype=torch.float32, device=dev)
         full[i0:i1, j0:j1, k0:k1] = grad.reshape(i1 - i0, j1 - j0, k1 - k0)
-        # source tables depend on eps_r through Cb at the source cell
-        # (rfx/simulation.py:181-194, 275-284): w[n] = Cb * s[n]
-        if n_src:
+        # uniform lane: source tables depend on eps_r through Cb at the
+        # source cell (rfx/simulation.py:181-194, 275-284): w[n] = Cb * s[n].
+        # The graded lane normalises sources with the materials before the
+        # override (rfx/runners/nonuniform.py:630-643): no dependence.
+        if n_src and not self.prob.graded:
             sa = src_adj[:, :n_src].double().cpu().numpy()
             eps_r, sig, _, _ = self.prob.mat.props()
             for q, (idx, comp, wave) in enumerate(self.prob.sources):

@@ -234,10 +234,8 @@ class Simulation:
     def _build_grid(self):
         if self._dz_profile is not None:
             from .nonuniform import make_nonuniform_grid
-            return make_nonuniform_grid(self._domain[:2], self._dz_profile,
-                                        self._dx if self._dx is not None else
-                                        float(self._dz_profile.max()),
-                                        self._cpml_layers)
+            return make_nonuniform_grid(self._freq_max, self._domain[:2], self._dz_profile,
+                                        self._dx, self._cpml_layers)
         return Grid(self._freq_max, self._domain, self._dx, self._cpml_layers)
 
     def _build_problem(self, n_steps: int, lane: str, eps_override=None,
@@ -259,30 +257,35 @@ class Simulation:
                 mat.fill_box(masks, m.eps_r, m.sigma, m.mu_r)
         if self._cpml_layers > 0:
             mat.extend_pads(grid.face_pads)
-        if lane == "forward" and (eps_override is not None or sigma_override is not None
-                                  or mu_r_override is not None):
+        # graded lane: sources are normalised by the materials before any
+        # override or port fold (rfx/runners/nonuniform.py:630-643)
+        concrete = {}
+        if graded:
+            for pe in self._ports:
+                for c in self._port_cells(grid, pe):
+                    concrete[c] = mat.cell_props(c)[:2]
+        if (eps_override is not None or sigma_override is not None
+                or mu_r_override is not None) and (lane == "forward" or graded):
             mat.override(eps_override, sigma_override, mu_r_override)
         dt = grid.dt
         times = sample_times(n_steps, dt)
         sources, ports = [], []
         for pe in self._ports:
+            if graded:
+                self._graded_port(grid, mat, pe, times, dt, concrete, pec, sources, ports,
+                                  s11_freqs)
+                continue
             idx = grid.position_to_index(pe.position)
             if pe.impedance == 0.0:
                 wave = pe.waveform(times)
-                if self._boundary == "cpml" or lane == "forward" or graded:
-                    if graded:
-                        from .nonuniform import current_source_scale
-                        scale = current_source_scale(grid, mat, idx, pe.component, dt)
-                    else:
-                        e, s, _, _ = mat.cell_props(idx)
-                        scale = _cell_cb_f32(dt, e, s)
-                    wave = (scale * wave).astype(F32)
+                if self._boundary == "cpml" or lane == "forward":
+                    e, s, _, _ = mat.cell_props(idx)
+                    wave = (_cell_cb_f32(dt, e, s) * wave).astype(F32)
                 sources.append((idx, pe.component, wave))
                 continue
             if pe.extent is not None:
-                raise NotImplementedError("wire ports (extent=...) are not supported")
-            if graded:
-                raise NotImplementedError("single-cell lumped ports on a graded mesh")
+                raise NotImplementedError("wire ports (extent=...) are supported on the graded "
+                                          "lane only")
             sizes = (grid.dx, grid.dx, grid.dx)
             sp = port_sigma(sizes, pe.component, pe.impedance)
             e, s, mu, fl = mat.cell_props(idx)
@@ -295,7 +298,8 @@ class Simulation:
                 wave = ((cb / F32(grid.dx)) * pe.waveform(times)).astype(F32)
                 sources.append((idx, pe.component, wave))
             if s11_freqs is not None:
-                ports.append((idx, pe.component, float(grid.dx), (grid.dx,) * 3, pe.impedance))
+                ports.append((idx, pe.component, float(grid.dx), (grid.dx,) * 3, pe.impedance,
+                              pe.excite))
         if pec is not None:
             mat.add_flags(tangential_edge_bits(pec))
         probes = [(grid.position_to_index(p.position), p.component) for p in self._probes]
@@ -311,7 +315,10 @@ class Simulation:
                 self._freq_max / 10, self._freq_max, pe.n_freqs).astype(F32)
             planes.append((a, index, pe.component, np.asarray(freqs, np.float64)))
         inv_e, inv_h = grid.inverse_spacings()
-        faces = face_tables(self._cpml_layers, dt, grid.face_cell_sizes(), self._kappa_max,
+        # the graded grid carries no kappa_max, so its CPML uses 1
+        # (rfx/boundaries/cpml.py:391-392)
+        kappa = 1.0 if graded else self._kappa_max
+        faces = face_tables(self._cpml_layers, dt, grid.face_cell_sizes(), kappa,
                             "xyz" if self._cpml_layers > 0 else "")
         # run(): outer-face PEC on every axis (runners/uniform.py:321 passes
         # pec_axes=None, simulation.py:684-691).  forward(): pec_axes = axes
@@ -325,29 +332,112 @@ class Simulation:
                        record_order="graded" if graded else "uniform")
         return grid, prob, ports
 
+    def _port_cells(self, grid, pe):
+        """Cells a port entry touches (wire ports: every extent cell)."""
+        idx = grid.position_to_index(pe.position)
+        if pe.impedance == 0.0 or pe.extent is None:
+            return [idx]
+        axis = "xyz".index(pe.component[1])
+        end = list(pe.position)
+        end[axis] += pe.extent
+        k_end = grid.position_to_index(tuple(end))[axis]
+        lo, hi = sorted((idx[axis], k_end))
+        cells = []
+        for k in range(lo, hi + 1):
+            c = list(idx)
+            c[axis] = k
+            cells.append(tuple(c))
+        return cells
+
+    def _graded_port(self, grid, mat, pe, times, dt, concrete, pec, sources, ports, s11_freqs):
+        """One port entry on the graded lane (rfx/runners/nonuniform.py:
+        694-887): soft source -> current source; single-cell port -> sigma
+        fold d_par/(Z0 dual1 dual2) + current source; wire port -> fold
+        n_live d/(Z0 dual1 dual2) on every live extent cell, source/n_live on
+        each, V/I recorded at the middle cell."""
+        from .nonuniform import current_source_scale
+        axis = "xyz".index(pe.component[1])
+        cells = self._port_cells(grid, pe)
+
+        def source(c):
+            e, s = concrete[c]
+            return (current_source_scale(grid, e, s, c, pe.component, dt)
+                    * pe.waveform(times)).astype(F32)
+
+        if pe.impedance == 0.0:
+            sources.append((cells[0], pe.component, source(cells[0])))
+            return
+        live = [pec is None or not bool(pec[c]) for c in cells]
+        n_live = sum(live)
+        if n_live == 0:
+            raise ValueError(f"port at {pe.position}: every extent cell lies inside PEC "
+                             "geometry, so it has no live cell to terminate or drive")
+        wire = pe.extent is not None
+        scale = n_live if wire else 1
+        for c, lv in zip(cells, live):
+            if not lv:
+                continue
+            d_par, p1, p2, _ = grid.port_metrics(c, axis)
+            _, s, _, _ = mat.cell_props(c)
+            mat.set_cell(c, sig=F32(s + F32(scale * d_par / (pe.impedance * p1 * p2))))
+            if pec is not None:
+                pec[c] = False
+        if pe.excite:
+            for c, lv in zip(cells, live):
+                if lv:
+                    w = source(c)
+                    sources.append((c, pe.component, (w / n_live).astype(F32) if wire else w))
+        if wire and s11_freqs is not None:
+            mid = cells[len(cells) // 2]
+            d_par, _, _, loop = grid.port_metrics(mid, axis)
+            ports.append((mid, pe.component, d_par, loop, pe.impedance, pe.excite))
+
     # ---- run / forward -----------------------------------------------------------
     def run(self, n_steps: int | None = None, *, num_periods: float = 20.0,
-            use_graph: bool = True, **unsupported):
-        """Run the time loop (rfx/api/_execute.py:2862 -> runners/uniform.py:140)."""
+            use_graph: bool = True, s_param_freqs=None, **unsupported):
+        """Run the time loop (rfx/api/_execute.py:2862 -> runners/uniform.py:140,
+        graded z -> runners/nonuniform.py:465).  On the graded lane
+        ``s_param_freqs`` gives the wire-port S matrix S[j, k] = b_j / a_k
+        (rfx/nonuniform.py:1900-1935)."""
         _reject(unsupported, "run")
+        graded = self._dz_profile is not None
         grid = self._build_grid()
         if n_steps is None:
             n_steps = grid.num_timesteps(num_periods)
-        if any(p.impedance != 0.0 for p in self._ports):
+        if not graded and any(p.impedance != 0.0 for p in self._ports):
             warnings.warn("run(): lumped-port S-parameters come from forward("
                           "port_s11_freqs=...); run() returns no s_params", stacklevel=2)
-        grid, prob, _ = self._build_problem(int(n_steps), "run")
+        sf = None
+        if graded and s_param_freqs is not None:
+            sf = np.asarray(s_param_freqs, F32)
+        grid, prob, ports = self._build_problem(int(n_steps), "run", s11_freqs=sf)
         dr = DeviceRun(prob, int(n_steps), self._device)
         dr.run(0, int(n_steps), use_graph=use_graph)
         torch.cuda.synchronize(dr.device)
         ts = dr.time_series().cpu().numpy().astype(np.float32)
+        s_params = None
+        if ports:
+            samp = dr.port_samples().cpu().numpy().astype(np.float64)
+            ab = []
+            for q, p in enumerate(ports):
+                v = port_dft(samp[:, q, 0], sf, grid.dt)
+                i = port_dft(samp[:, q, 1], sf, grid.dt)
+                zi = p[4] * i
+                ab.append(((-v + zi) / 2.0, (-v - zi) / 2.0))
+            excited = [q for q, p in enumerate(ports) if p[5]] or list(range(len(ports)))
+            s_params = np.zeros((len(ports), len(ports), len(sf)), np.complex128)
+            for k in excited:
+                a_k = np.where(np.abs(ab[k][0]) > 0, ab[k][0], 1.0)
+                for j in range(len(ports)):
+                    s_params[j, k] = ab[j][1] / a_k
         planes = {}
         for entry, (a, idx, comp, freqs), acc in zip(self._dft_planes, prob.planes, dr.plane_acc):
             planes[entry.name] = DFTPlaneProbe(a, int(idx), comp, np.asarray(freqs),
                                                acc.cpu().numpy())
         st = FDTDState(*[f for f in dr.fields], step=int(n_steps))
         _warn_if_nonfinite(ts, "run")
-        return Result(time_series=ts, state=st, dft_planes=planes, grid=grid, dt=grid.dt)
+        return Result(time_series=ts, state=st, dft_planes=planes, s_params=s_params,
+                      freqs=sf if s_params is not None else None, grid=grid, dt=grid.dt)
 
     def forward(self, *, eps_override=None, sigma_override=None, mu_r_override=None,
                 n_steps: int | None = None, num_periods: float = 20.0, checkpoint: bool = True,

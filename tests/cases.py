@@ -86,6 +86,8 @@ def to_sim(spec, device=None):
     kw = {}
     if device is not None:
         kw["device"] = device
+    if spec.get("dz_profile") is not None:
+        kw["dz_profile"] = spec["dz_profile"]
     sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec.get("dx"),
                        boundary=spec.get("boundary", "cpml"),
                        cpml_layers=spec.get("cpml_layers", 16),
@@ -94,13 +96,15 @@ def to_sim(spec, device=None):
         sim.add_material(name, eps_r=e, sigma=s, mu_r=m)
     for lo, hi, name in spec.get("geometry", []):
         sim.add(R.Box(lo, hi), material=name)
-    for pos, comp, z0, pulse, excite in spec.get("ports", []):
+    for entry in spec.get("ports", []):
+        pos, comp, z0, pulse, excite = entry[:5]
+        extent = entry[5] if len(entry) > 5 else None
         wf = None if pulse is None else R.GaussianPulse(pulse.f0, pulse.bandwidth,
                                                          pulse.amplitude, pulse.cutoff)
         if z0 == 0.0:
             sim.add_source(pos, comp, waveform=wf)
         else:
-            sim.add_port(pos, comp, impedance=z0, waveform=wf, excite=excite)
+            sim.add_port(pos, comp, impedance=z0, waveform=wf, excite=excite, extent=extent)
     for pos, comp in spec.get("probes", []):
         sim.add_probe(pos, comp)
     for axis, coord, comp, freqs, nf in spec.get("dft_planes", []):
@@ -113,3 +117,61 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def smooth_grading(cells, max_ratio=1.3):
+    """Geometric transition cells wherever adjacent cells differ by more
+    than max_ratio (rfx/auto_config.py:656-723, no preserved regions)."""
+    cells = [float(c) for c in cells]
+    out = [cells[0]]
+    for target in cells[1:]:
+        prev = out[-1]
+        while target / prev > max_ratio + 1e-12:
+            prev = prev * max_ratio
+            out.append(prev)
+        while prev / target > max_ratio + 1e-12:
+            prev = prev / max_ratio
+            out.append(prev)
+        out.append(target)
+    return np.array(out)
+
+
+# Air-filled PEC cavity on a 4:1 graded z mesh
+# (tests/test_nonuniform_cavity_accuracy.py:80-116 of the reference)
+CAVITY_A, CAVITY_B, CAVITY_DX = 40e-3, 35e-3, 1e-3
+
+
+def graded_cavity_case(n_steps=8000):
+    fine = 0.25e-3
+    dz_raw = [CAVITY_DX] * 17 + [fine] * 8 + [CAVITY_DX] * 17
+    dz = smooth_grading(dz_raw, 1.3)
+    d = float(np.sum(dz))
+    f111 = (o.C0 / 2) * np.sqrt((1 / CAVITY_A) ** 2 + (1 / CAVITY_B) ** 2 + (1 / d) ** 2)
+    a, b = CAVITY_A, CAVITY_B
+    spec = dict(freq_max=2 * f111, domain=(a, b), dx=CAVITY_DX, dz_profile=dz,
+                boundary="pec",
+                ports=[((a / 3, b / 3, d / 4), "ez", 0.0, o.Pulse(f111, 0.8), True)],
+                probes=[((2 * a / 3, 2 * b / 3, d / 4), "ez")])
+    return spec, n_steps, f111, d
+
+
+def mixed_graded_case():
+    """Graded-z case touching every feature of the graded lane: CPML on all
+    faces, lossy / magnetic boxes, a PEC sheet, a soft source, a single-cell
+    port, a wire port with V/I, E and H probes, DFT planes."""
+    dz = smooth_grading([1e-3] * 4 + [0.4e-3] * 4 + [1e-3] * 4, 1.3)
+    d = float(np.sum(dz))
+    return dict(freq_max=10e9, domain=(0.016, 0.012), dx=1e-3, dz_profile=dz,
+                boundary="cpml", cpml_layers=4,
+                materials={"sub": (4.4, 0.02, 1.0), "pec": (1.0, 1e10, 1.0),
+                           "mag": (1.5, 0.0, 2.0)},
+                geometry=[((0.003, 0.002, 0.0), (0.013, 0.010, 0.004), "sub"),
+                          ((0.004, 0.003, 0.004), (0.009, 0.007, 0.004), "pec"),
+                          ((0.011, 0.008, 0.007), (0.014, 0.011, 0.010), "mag")],
+                ports=[((0.006, 0.005, 0.002), "ez", 0.0, o.Pulse(8e9, 0.8), True),
+                       ((0.012, 0.004, 0.003), "ez", 50.0, o.Pulse(6e9, 0.7), True, 0.0022),
+                       ((0.003, 0.009, 0.008), "ex", 50.0, None, False)],
+                probes=[((0.010, 0.006, 0.004), "ez"), ((0.005, 0.008, 0.006), "hy"),
+                        ((0.012, 0.004, 0.003), "ez")],
+                dft_planes=[("z", 0.004, "ez", None, 3), ("x", 0.008, "hx", None, 2)],
+                s11_freqs=np.array([5e9, 7e9]))

@@ -96,3 +96,57 @@ def test_eps_derivative_table_matches_finite_difference():
         h = 1e-6 * e
         fd = (coefs(e + h, s) - coefs(e - h, s)) / (2 * h)
         np.testing.assert_allclose(tab[q], fd, rtol=1e-5, atol=1e-30)
+
+
+@pytest.mark.parametrize("name", ["mixed_graded", "cavity"])
+def test_graded_setup_matches_oracle(name):
+    """Graded-z lane: grid metrics, materials on the graded node
+    coordinates, current-source tables, port folds / metrics, probes,
+    planes and CPML tables equal the oracle's build_graded."""
+    spec = cases.mixed_graded_case() if name == "mixed_graded" else cases.graded_cavity_case()[0]
+    n = 48
+    sim = cases.to_sim(spec)
+    s11 = None if spec.get("s11_freqs") is None else np.asarray(spec["s11_freqs"], np.float32)
+    grid, prob, ports = sim._build_problem(n, "run", s11_freqs=s11)
+    g, cfg = o.build_graded(spec, n)
+    assert grid.shape == g.shape and grid.dt == g.dt and prob.graded
+    inv_e, inv_h = grid.inverse_spacings()
+    for a in range(3):
+        np.testing.assert_array_equal(inv_e[a], g.inv_e[a])
+        np.testing.assert_array_equal(inv_h[a], g.inv_h[a])
+        np.testing.assert_array_equal(grid.node_coords()[a], g.node_positions(a))
+    eps, sig, mu = prob.mat.cell_arrays()
+    np.testing.assert_array_equal(eps, cfg.eps_r)
+    np.testing.assert_array_equal(sig, cfg.sigma)
+    np.testing.assert_array_equal(mu, cfg.mu_r)
+    _, _, _, flg = prob.mat.props()
+    bits = flg[prob.mat.ids] & 7
+    if cfg.pec_mask is None:
+        assert not bits.any()
+    else:
+        assert cfg.pec_mask.any()
+        mx, my, mz = o.tangential_edge_masks(cfg.pec_mask)
+        np.testing.assert_array_equal(bits & 1, mx.astype(np.uint32))
+        np.testing.assert_array_equal((bits >> 1) & 1, my.astype(np.uint32))
+        np.testing.assert_array_equal((bits >> 2) & 1, mz.astype(np.uint32))
+    assert prob.pec_axes == "xyz" and prob.record_order == "graded"
+    assert len(prob.sources) == len(cfg.sources) and len(prob.sources) > 0
+    for (i1, c1, w1), (i2, c2, w2) in zip(prob.sources, cfg.sources):
+        assert tuple(i1) == tuple(i2) and c1 == c2
+        np.testing.assert_array_equal(w1[:n], w2[:n])
+    assert [tuple(p[0]) for p in prob.probes] == [tuple(p[0]) for p in cfg.probes]
+    assert [(p[0], p[1]) for p in prob.planes] == [(p[0], p[1]) for p in cfg.dft_planes]
+    assert len(ports) == len(cfg.ports_vi)
+    for p, q in zip(ports, cfg.ports_vi):
+        assert tuple(p[0]) == tuple(q[0]) and p[1] == q[1]
+        assert p[2] == q[3] and tuple(p[3]) == tuple(q[4])
+    for f, (n_l, b, c, km1, inv_d) in enumerate(prob.faces):
+        ax, side = f // 2, f % 2
+        if cfg.cpml_n == 0:
+            assert n_l == 0
+            continue
+        ob, oc, ok = cfg.cpml_profiles[(ax, side)]
+        np.testing.assert_array_equal(b, ob)
+        np.testing.assert_array_equal(c, oc)
+        assert inv_d == np.float32(1.0 / cfg.cpml_face_d[ax][side]) or abs(
+            inv_d * cfg.cpml_face_d[ax][side] - 1) < 1e-7
