@@ -176,6 +176,24 @@ typedef struct {
 int rfxh_adjoint(rfxh_plan* plan, const rfxh_adjoint_desc* desc, int64_t lo, int64_t hi,
                  void* stream);
 
+/* ---- batched sweeps ----
+ * Many independent problems (uneven grid sizes and step counts) advanced
+ * together: every time step is one H launch and one E launch covering all
+ * instances that still have steps left, instead of two launches per
+ * instance.  Replaces the per-instance loop of rfx/batch.py:530-559
+ * (run_batch) and the vmap program of rfx/vmap_sweep.py:434-643, whose
+ * uneven-shape case runs instances one after another.  Each instance keeps
+ * its own plan (fields, tables, outputs); a batch only groups them. */
+typedef struct rfxh_batch rfxh_batch;
+int rfxh_batch_create(rfxh_plan* const* plans, int n_plans, rfxh_batch** out);
+int rfxh_batch_destroy(rfxh_batch* batch);
+/* Instance q advances n_steps[q] steps from absolute step step0[q]
+ * (two-pass sub-steps on its current field set).  use_graph != 0 replays
+ * captured chunks of steps wherever the set of active instances is
+ * constant. */
+int rfxh_batch_run(rfxh_batch* batch, const int64_t* step0, const int64_t* n_steps, void* stream,
+                   int use_graph);
+
 const char* rfxh_last_error(void);
 int rfxh_version(void);
 

@@ -79,7 +79,8 @@ _lib = None
 
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
-           "rfxh_run_record", "rfxh_adjoint", "rfxh_last_error", "rfxh_version")
+           "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
+           "rfxh_batch_run", "rfxh_last_error", "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -116,6 +117,12 @@ def load():
     lib.rfxh_run_record.restype = C.c_int
     lib.rfxh_adjoint.argtypes = [_P, C.POINTER(AdjointDesc), C.c_int64, C.c_int64, _P]
     lib.rfxh_adjoint.restype = C.c_int
+    lib.rfxh_batch_create.argtypes = [C.POINTER(_P), C.c_int, C.POINTER(_P)]
+    lib.rfxh_batch_create.restype = C.c_int
+    lib.rfxh_batch_destroy.argtypes = [_P]
+    lib.rfxh_batch_destroy.restype = C.c_int
+    lib.rfxh_batch_run.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, C.c_int]
+    lib.rfxh_batch_run.restype = C.c_int
     lib.rfxh_last_error.argtypes = []
     lib.rfxh_last_error.restype = C.c_char_p
     lib.rfxh_version.argtypes = []

@@ -37,6 +37,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "rfxhip.h"
@@ -144,8 +145,8 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < rem ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s;
 }
 
-__device__ __forceinline__ bool cell_of(const KParams& p, int& i, int& j, int& k) {
-  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+// Cell of this thread in logical workgroup L of the problem p.
+__device__ __forceinline__ bool cell_at(const KParams& p, int L, int& i, int& j, int& k) {
   const long long pl = p.plane;
   long long f;
   if (p.order == 0) {
@@ -178,6 +179,10 @@ __device__ __forceinline__ bool cell_of(const KParams& p, int& i, int& j, int& k
   j = jj;
   k = kk;
   return true;
+}
+
+__device__ __forceinline__ bool cell_of(const KParams& p, int& i, int& j, int& k) {
+  return cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), i, j, k);
 }
 
 __device__ __forceinline__ bool in_slab(const KParams& p, int i, int j, int k) {
@@ -456,9 +461,7 @@ __global__ __launch_bounds__(kWG) void k_step_fused(KParams p, FusedIO io, PsiIO
 // Two-pass step, in place on p.f.
 // ---------------------------------------------------------------------------
 template <bool GRADED>
-__global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
-  int i, j, k;
-  if (!cell_of(p, i, j, k)) return;
+__device__ __forceinline__ void two_pass_h(const KParams& p, const PsiIO& ps, int i, int j, int k) {
   const float* const Eo[3] = {p.f[0], p.f[1], p.f[2]};
   const float* const Ho[3] = {p.f[3], p.f[4], p.f[5]};
   float* const Hn[3] = {p.f[3], p.f[4], p.f[5]};
@@ -466,9 +469,7 @@ __global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
 }
 
 template <bool GRADED>
-__global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
-  int i, j, k;
-  if (!cell_of(p, i, j, k)) return;
+__device__ __forceinline__ void two_pass_e(const KParams& p, const PsiIO& ps, int i, int j, int k) {
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
   const float* Hx = p.f[3];
   const float* Hy = p.f[4];
@@ -488,6 +489,72 @@ __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
   const float* const Eo[3] = {p.f[0], p.f[1], p.f[2]};
   float* const En[3] = {p.f[0], p.f[1], p.f[2]};
   e_cell<GRADED>(p, Eo, En, ps, i, j, k, hx, hy, hz, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  two_pass_h<GRADED>(p, ps, i, j, k);
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  two_pass_e<GRADED>(p, ps, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// Batched two-pass step over many independent problems: one launch per
+// half-step covers every active instance.  Instance q owns logical
+// workgroups [start[q], start[q+1]); the instance table lives in device
+// memory, so every parameter read below is wave-uniform (scalar loads).
+// ---------------------------------------------------------------------------
+struct BatchEntry {
+  KParams p;   // p.f = the instance's current field set
+  PsiIO ps;
+};
+
+__device__ __forceinline__ int batch_instance(const int* __restrict__ start, int n, int L) {
+  int lo = 0, hi = n - 1;  // largest q with start[q] <= L
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= L) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int PASS>
+__global__ __launch_bounds__(kWG) void k_batch_pass(const BatchEntry* __restrict__ ents,
+                                                   const int* __restrict__ start, int n_act) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int q = batch_instance(start, n_act, L);
+  const KParams& p = ents[q].p;
+  int i, j, k;
+  if (!cell_at(p, L - start[q], i, j, k)) return;
+  if (PASS == 0) {
+    if (p.graded) two_pass_h<true>(p, ents[q].ps, i, j, k);
+    else two_pass_h<false>(p, ents[q].ps, i, j, k);
+  } else {
+    if (p.graded) two_pass_e<true>(p, ents[q].ps, i, j, k);
+    else two_pass_e<false>(p, ents[q].ps, i, j, k);
+  }
+}
+
+__global__ void k_batch_advance(const BatchEntry* __restrict__ ents, int n_act) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q < n_act) {
+    long long* st = ents[q].p.step;
+    *st = *st + 1;
+  }
+}
+
+__global__ void k_batch_set_steps(const BatchEntry* __restrict__ ents, const long long* __restrict__ v,
+                                  int n) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q < n) *ents[q].p.step = v[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1490,6 +1557,184 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
     }
   }
   HIPCHK(hipGetLastError());
+  return RFXH_OK;
+}
+
+// ---- batched sweeps --------------------------------------------------------
+}  // extern "C"
+
+struct rfxh_batch {
+  std::vector<rfxh_plan*> plans;
+  BatchEntry* d_ents = nullptr;   // instance table in run order
+  int* d_start = nullptr;         // first logical workgroup per instance (+ total)
+  long long* d_steps = nullptr;   // step0 per instance, run order
+  std::vector<int> order;         // run order -> plan index
+  std::vector<int> start;         // host copy of d_start
+  hipStream_t cap_stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  std::map<int, hipGraphExec_t> graphs;  // keyed by active count
+  int graph_chunk = 16;
+};
+
+namespace {
+
+void batch_drop_graphs(rfxh_batch* b) {
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+}
+
+// One step of the first n_act instances (run order).
+int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
+  const unsigned nb = (unsigned)b->start[n_act];
+  hipLaunchKernelGGL(k_batch_pass<0>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
+  hipLaunchKernelGGL(k_batch_pass<1>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
+  for (int q = 0; q < n_act; ++q) {
+    rfxh_plan* pl = b->plans[b->order[q]];
+    for (size_t r = 0; r < pl->planes.size(); ++r) {
+      PlaneParams pp = pl->planes[r];
+      pp.fld = pl->set[pl->cur][pl->plane_comp[r]];
+      const long long npts = (long long)pp.n1 * pp.n2;
+      hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
+    }
+  }
+  hipLaunchKernelGGL(k_batch_advance, dim3((unsigned)((n_act + 63) / 64)), dim3(64), 0, s, b->d_ents,
+                     n_act);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rfxh_batch_create(rfxh_plan* const* plans, int n_plans, rfxh_batch** out) {
+  if (!plans || n_plans < 1 || !out) return fail(RFXH_EINVAL, "no plans");
+  *out = nullptr;
+  for (int q = 0; q < n_plans; ++q)
+    if (!plans[q]) return fail(RFXH_EINVAL, "null plan in batch");
+  rfxh_batch* b = new rfxh_batch();
+  b->plans.assign(plans, plans + n_plans);
+  auto bail = [&](int r) {
+    rfxh_batch_destroy(b);
+    return r;
+  };
+  if (hipMalloc(&b->d_ents, sizeof(BatchEntry) * n_plans) != hipSuccess ||
+      hipMalloc(&b->d_start, sizeof(int) * (n_plans + 1)) != hipSuccess ||
+      hipMalloc(&b->d_steps, sizeof(long long) * n_plans) != hipSuccess)
+    return bail(fail(RFXH_ENOMEM, "batch tables"));
+  *out = b;
+  return RFXH_OK;
+}
+
+int rfxh_batch_destroy(rfxh_batch* b) {
+  if (!b) return RFXH_OK;
+  batch_drop_graphs(b);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->d_ents) (void)hipFree(b->d_ents);
+  if (b->d_start) (void)hipFree(b->d_start);
+  if (b->d_steps) (void)hipFree(b->d_steps);
+  delete b;
+  return RFXH_OK;
+}
+
+int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, void* stream,
+                   int use_graph) {
+  if (!b || !step0 || !n_steps) return fail(RFXH_EINVAL, "bad batch arguments");
+  const int n = (int)b->plans.size();
+  for (int q = 0; q < n; ++q) {
+    const rfxh_plan* pl = b->plans[q];
+    if (step0[q] < 0 || n_steps[q] < 0 || step0[q] + n_steps[q] > pl->max_steps)
+      return fail(RFXH_EINVAL, "batch instance steps outside its tables (max_steps)");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // run order: most steps first, so the active instances are always a prefix
+  std::vector<int> order(n);
+  for (int q = 0; q < n; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return n_steps[a] > n_steps[c]; });
+  std::vector<BatchEntry> ents(n);
+  std::vector<int> start(n + 1, 0);
+  std::vector<long long> st(n);
+  for (int r = 0; r < n; ++r) {
+    rfxh_plan* pl = b->plans[order[r]];
+    ents[r].p = pl->kp;
+    for (int c = 0; c < 6; ++c) ents[r].p.f[c] = pl->set[pl->cur][c];
+    ents[r].ps = psi_io(pl, pl->cur);
+    const long long nb = (long long)start[r] + (long long)pl->grid.x;
+    if (nb > 0x7fffffffLL) return fail(RFXH_EINVAL, "batch too large for one launch");
+    start[r + 1] = (int)nb;
+    st[r] = step0[order[r]];
+  }
+  // the captured graphs launch per-instance plane kernels: keep them only
+  // while the instance table is unchanged
+  bool same = order == b->order && start == b->start;
+  if (same) {
+    std::vector<BatchEntry> old(n);
+    HIPCHK(hipMemcpy(old.data(), b->d_ents, sizeof(BatchEntry) * n, hipMemcpyDeviceToHost));
+    same = std::memcmp(old.data(), ents.data(), sizeof(BatchEntry) * n) == 0;
+  }
+  if (!same) {
+    batch_drop_graphs(b);
+    HIPCHK(hipMemcpyAsync(b->d_ents, ents.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->d_start, start.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    b->order = order;
+    b->start = start;
+  }
+  HIPCHK(hipMemcpyAsync(b->d_steps, st.data(), sizeof(long long) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_batch_set_steps, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, b->d_ents,
+                     b->d_steps, n);
+  HIPCHK(hipGetLastError());
+  if (use_graph && !b->cap_stream) {
+    HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
+  }
+  hipStream_t w = s;
+  if (use_graph) {
+    HIPCHK(hipEventRecord(b->ev_in, s));
+    HIPCHK(hipStreamWaitEvent(b->cap_stream, b->ev_in, 0));
+    w = b->cap_stream;
+  }
+  int64_t done = 0;
+  for (int act = n; act > 0; --act) {
+    const int64_t until = n_steps[order[act - 1]];  // the shortest active instance ends here
+    int64_t len = until - done;
+    if (len <= 0) continue;
+    if (use_graph && len >= b->graph_chunk) {
+      auto it = b->graphs.find(act);
+      if (it == b->graphs.end()) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int c = 0; c < b->graph_chunk; ++c) {
+          const int rc = batch_step(b, b->cap_stream, act);
+          if (rc) {
+            (void)hipStreamEndCapture(b->cap_stream, &graph);
+            return rc;
+          }
+        }
+        HIPCHK(hipStreamEndCapture(b->cap_stream, &graph));
+        hipGraphExec_t ge;
+        HIPCHK(hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        it = b->graphs.emplace(act, ge).first;
+      }
+      while (len >= b->graph_chunk) {
+        HIPCHK(hipGraphLaunch(it->second, w));
+        len -= b->graph_chunk;
+        done += b->graph_chunk;
+      }
+    }
+    for (; len > 0; --len, ++done) {
+      const int rc = batch_step(b, w, act);
+      if (rc) return rc;
+    }
+  }
+  if (use_graph) {
+    HIPCHK(hipEventRecord(b->ev_out, b->cap_stream));
+    HIPCHK(hipStreamWaitEvent(s, b->ev_out, 0));
+  }
   return RFXH_OK;
 }
 

@@ -175,3 +175,26 @@ def mixed_graded_case():
                         ((0.012, 0.004, 0.003), "ez")],
                 dft_planes=[("z", 0.004, "ez", None, 3), ("x", 0.008, "hx", None, 2)],
                 s11_freqs=np.array([5e9, 7e9]))
+
+
+def sweep_instance(rng, shape_cells, dx=1e-3, layers=4):
+    """One instance of a batched sweep: vacuum box with a random dielectric
+    slab normal to a random axis, an Ez pulse source and four Ez probes
+    (the sweep workload, at test sizes)."""
+    nx, ny, nz = shape_cells
+    dom = (nx * dx, ny * dx, nz * dx)
+    ax = int(rng.integers(0, 3))
+    th = int(rng.integers(2, 6))
+    lo = [0.0, 0.0, 0.0]
+    hi = list(dom)
+    c0 = int(rng.integers(2, shape_cells[ax] - th - 1))
+    lo[ax] = (c0 + 0.5) * dx
+    hi[ax] = (c0 + th + 0.5) * dx
+    f0 = float(rng.uniform(2e9, 20e9))
+    src = tuple(0.3 * d for d in dom)
+    probes = [(tuple(float(rng.uniform(0.2, 0.8)) * d for d in dom), "ez") for _ in range(4)]
+    return dict(freq_max=25e9, domain=dom, dx=dx, boundary="cpml", cpml_layers=layers,
+                materials={"slab": (float(rng.uniform(2, 12)), 0.0, 1.0)},
+                geometry=[(tuple(lo), tuple(hi), "slab")],
+                ports=[(src, "ez", 0.0, o.Pulse(f0, 0.5), True)],
+                probes=probes, lane="run")
