@@ -66,6 +66,77 @@ def c1_spec():
                 probes=[((0.11, 0.03, 0.025), "ez")], dft_planes=[])
 
 
+def c4_spec():
+    """Gradient workload: 256x256x128 interior, dx = 1 mm, 10-layer CPML,
+    a centred 64x64x4 design region of eps_r ~ U(1, 6) (seed 4), a line of
+    Ez soft sources along y at (x0, z_c) with a 12 GHz pulse, 3 Ez probes
+    beyond the region; objective sum_p sum_f |DFT_f ts_p|^2 over 5
+    frequencies in [11, 13] GHz; 3000 steps."""
+    dx = 1e-3
+    nx, ny, nz = 256, 256, 128
+    x0, zc = 40 * dx, nz // 2 * dx
+    sources = [((x0, (j + 0.5) * dx, zc), "ez", 12e9, 0.5) for j in range(8, ny - 8)]
+    probes = [((216 * dx, (ny // 2 + o_) * dx, zc), "ez") for o_ in (-40, 0, 40)]
+    return dict(freq_max=20e9, domain=(nx * dx, ny * dx, nz * dx), dx=dx, boundary="cpml",
+                cpml_layers=10, materials={}, geometry=[], sources=sources, probes=probes,
+                dft_planes=[])
+
+
+def grad_cost(n_steps=3000):
+    """Wall time of forward + reverse sweep over the plain forward run, both
+    timed on the device work only (set-up done beforehand)."""
+    import torch
+    from rfx_amd.adjoint import AdjointRunner
+    from rfx_amd.engine import DeviceRun
+    spec = c4_spec()
+    sim = build_sim(spec)
+    grid = sim._build_grid()
+    p = grid.axis_pads[0]
+    shape = grid.shape
+    rng = np.random.default_rng(4)
+    eps = np.ones(shape, np.float32)
+    ci, cj, ck = shape[0] // 2, shape[1] // 2, shape[2] // 2
+    region = (ci - 32, ci + 32, cj - 32, cj + 32, ck - 2, ck + 2)
+    eps[region[0]:region[1], region[2]:region[3], region[4]:region[5]] = \
+        rng.uniform(1.0, 6.0, (64, 64, 4)).astype(np.float32)
+    _, prob, _ = sim._build_problem(n_steps, "forward", eps)
+    dt = grid.dt
+    freqs = torch.linspace(11e9, 13e9, 5, dtype=torch.float64, device="cuda:0")
+    t = torch.arange(n_steps, dtype=torch.float64, device="cuda:0") * dt
+    ph = torch.exp(-2j * np.pi * freqs[:, None] * t[None, :])      # (5, n)
+
+    def objective(ts):
+        acc = ph @ ts.double().to(torch.complex128)                    # (5, probes)
+        return (acc.real ** 2 + acc.imag ** 2).sum()
+
+    dr = DeviceRun(prob, n_steps, "cuda:0", two_sets=False)
+    dr.run(0, n_steps)
+    torch.cuda.synchronize()
+    dr.reset()
+    t0 = time.perf_counter()
+    dr.run(0, n_steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    dr.close()
+    runner = AdjointRunner(prob, n_steps, "cuda:0", region)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ts = runner.forward().detach().requires_grad_(True)
+    J = objective(ts)
+    (g,) = torch.autograd.grad(J, ts)
+    grad = runner.backward(g)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    finite = bool(torch.isfinite(grad).all())
+    del runner
+    torch.cuda.empty_cache()
+    return {"workload": f"{shape[0]}x{shape[1]}x{shape[2]} cells, 64x64x4 design region, "
+                        f"{len(spec['sources'])} line-source cells, 3 probes, 5-freq DFT "
+                        f"objective, {n_steps} time steps, one record segment",
+            "forward_s": round(t_fwd, 4), "forward_adjoint_s": round(t_grad, 4),
+            "ratio": round(t_grad / t_fwd, 3), "gradient_finite": finite}
+
+
 def build_sim(spec):
     import rfx_amd as R
     sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec["dx"],
@@ -121,6 +192,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(N_STEPS))
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad", action="store_true",
+                    help="skip the forward+adjoint / forward timing")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--time-steps", type=int, default=None,
                     help="time steps per run (default: the workload's own count); "
@@ -209,10 +282,15 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic},
         "cpu_baseline": None,
+        "grad_vs_forward": None,
     }
     dr.close()
     del dr
     torch.cuda.empty_cache()
+    if not args.no_grad and args.config == "c2":
+        print("[bench] forward+adjoint vs forward ...", file=sys.stderr, flush=True)
+        out["grad_vs_forward"] = grad_cost()
+        print(f"[bench] grad {out['grad_vs_forward']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(spec, n_total, args.cpu_steps)

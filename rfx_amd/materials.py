@@ -121,18 +121,30 @@ class MaterialMap:
         S = sig[cur] if sigma is None else np.asarray(sigma, F32).reshape(self.shape)
         M = mu[cur] if mu_r is None else np.asarray(mu_r, F32).reshape(self.shape)
         Fl = flg[cur]
-        key = np.stack([E.view(np.uint32).ravel(), S.view(np.uint32).ravel(),
-                        M.view(np.uint32).ravel(), Fl.ravel()], axis=1)
-        uniq, inv = np.unique(key, axis=0, return_inverse=True)
-        if len(uniq) > MAX_IDS:
+        # distinct (eps, sigma, mu, flags) tuples: one 1-D unique per
+        # property (bit patterns), then one over the combined int64 key
+        parts = [np.unique(a.view(np.uint32).ravel() if a.dtype == F32 else a.ravel(),
+                           return_inverse=True) for a in (E, S, M, Fl)]
+        if max(len(v) for v, _ in parts[:3]) > MAX_IDS or len(parts[3][0]) > 256:
+            raise ValueError(f"override produces more than {MAX_IDS} distinct materials")
+        key = np.zeros(E.size, np.int64)       # < 65536^3 * 256 < 2^63
+        for vals, inv in parts:
+            key = key * len(vals) + inv.ravel()
+        ukey, inv = np.unique(key, return_inverse=True)
+        if len(ukey) > MAX_IDS:
             raise ValueError(
-                f"override produces {len(uniq)} distinct materials (max {MAX_IDS})")
+                f"override produces {len(ukey)} distinct materials (max {MAX_IDS})")
         self.table, self._lookup = [], {}
         self.get_id(1.0, 0.0, 1.0, 0)
-        new = np.empty(len(uniq), np.uint16)
-        for q, row in enumerate(uniq):
-            new[q] = self.get_id(np.uint32(row[0]).view(F32), np.uint32(row[1]).view(F32),
-                                 np.uint32(row[2]).view(F32), int(row[3]))
+        new = np.empty(len(ukey), np.uint16)
+        for q, kv in enumerate(ukey):
+            idx = []
+            for vals, _ in reversed(parts):
+                kv, r = divmod(int(kv), len(vals))
+                idx.append(vals[r])
+            fl, m, s, e = idx
+            new[q] = self.get_id(np.uint32(e).view(F32), np.uint32(s).view(F32),
+                                 np.uint32(m).view(F32), int(fl))
         self.ids = new[inv.ravel()].reshape(self.shape)
 
     def cell_arrays(self):
