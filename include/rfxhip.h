@@ -140,6 +140,15 @@ int rfxh_set_state_set(rfxh_plan* plan, int which);
 int rfxh_set_timing(rfxh_plan* plan, int enable);
 int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* count);
 
+/* Kernel variant of a plan: fused != 0 selects the single-launch step (needs
+ * the second field set), 0 the two-pass step; order 1 = flat workgroup
+ * order, 0 = XCD bands.  Variants compute the same step; they differ in
+ * speed only.  The project's autotuner (rfx_amd/autotune.py) times them on
+ * the actual problem.  No counterpart in the reference (its XLA lowering
+ * has no kernel choice). */
+int rfxh_set_variant(rfxh_plan* plan, int fused, int order);
+int rfxh_get_variant(rfxh_plan* plan, int* fused, int* order);
+
 /* ---- reverse mode (gradient of a probe-series objective w.r.t. eps_r) ----
  * Replaces jax.grad through the scan (rfx/api/_execute.py:2447-2858 with
  * rfx/simulation.py:1975, 2011-2055 rematerialisation): the forward run

@@ -80,7 +80,8 @@ _lib = None
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
            "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
-           "rfxh_batch_run", "rfxh_last_error", "rfxh_version")
+           "rfxh_batch_run", "rfxh_set_variant", "rfxh_get_variant", "rfxh_last_error",
+           "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -123,6 +124,10 @@ def load():
     lib.rfxh_batch_destroy.restype = C.c_int
     lib.rfxh_batch_run.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, C.c_int]
     lib.rfxh_batch_run.restype = C.c_int
+    lib.rfxh_set_variant.argtypes = [_P, C.c_int, C.c_int]
+    lib.rfxh_set_variant.restype = C.c_int
+    lib.rfxh_get_variant.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.rfxh_get_variant.restype = C.c_int
     lib.rfxh_last_error.argtypes = []
     lib.rfxh_last_error.restype = C.c_char_p
     lib.rfxh_version.argtypes = []

@@ -1,0 +1,121 @@
+"""Kernel-variant autotuner.
+
+The plan's step can run as the two-pass kernels (H pass, E pass, in place)
+or, with a second field set, as one fused kernel; either with the flat or
+the XCD-band workgroup order (include/rfxhip.h ``rfxh_set_variant``).  All
+variants compute the same step.  Which is fastest depends on the grid (CPML
+share, plane size, cache fit), so the tuner runs a few real steps of the
+actual problem under each variant, timed with events on the stream the run
+uses, keeps the fastest and resets the state.  Results are cached per
+(shape, graded, CPML depths, device) for the process and, optionally, in a
+JSON file.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import torch
+
+from . import _lib
+
+_CACHE: dict = {}
+
+
+def _key(dr) -> str:
+    p = dr.prob
+    dev = torch.cuda.get_device_name(dr.device)
+    faces = tuple(int(f[0]) for f in p.faces)
+    return f"{tuple(p.shape)}|{int(p.graded)}|{faces}|{len(dr.sets)}|{dev}"
+
+
+def variants(dr):
+    fused_ok = len(dr.sets) == 2
+    out = [(0, 1), (0, 0)]
+    if fused_ok:
+        out += [(1, 1), (1, 0)]
+    return out
+
+
+def get_variant(dr):
+    f, o = C.c_int(), C.c_int()
+    _lib.check(dr.lib.rfxh_get_variant(dr.plan, C.byref(f), C.byref(o)))
+    return f.value, o.value
+
+
+def set_variant(dr, fused: int, order: int):
+    _lib.check(dr.lib.rfxh_set_variant(dr.plan, int(fused), int(order)))
+
+
+def autotune(dr, trial_steps: int = 32, repeats: int = 2, cache_file: str | None = None):
+    """Pick the fastest variant for ``dr`` (an engine.DeviceRun); leaves the
+    run reset (zero state) with that variant selected.  Returns
+    {"variant": (fused, order), "ms_per_step": {variant: ms}}."""
+    key = _key(dr)
+    if cache_file and not _CACHE and os.path.exists(cache_file):
+        with open(cache_file) as fh:
+            _CACHE.update({k: tuple(v) for k, v in json.load(fh).items()})
+    if key in _CACHE:
+        set_variant(dr, *_CACHE[key])
+        dr.reset()
+        return {"variant": _CACHE[key], "ms_per_step": None, "cached": True}
+    steps = max(1, min(int(trial_steps), dr.n_steps))
+    stream = torch.cuda.current_stream(dr.device)
+    timings = {}
+    for v in variants(dr):
+        set_variant(dr, *v)
+        dr.reset()
+        dr.run(0, steps)                      # warm-up, builds the graph
+        best = None
+        for _ in range(repeats):
+            dr.reset()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            dr.run(0, steps)
+            b.record(stream)
+            b.synchronize()
+            ms = a.elapsed_time(b) / steps
+            best = ms if best is None else min(best, ms)
+        timings[v] = best
+    choice = min(timings, key=timings.get)
+    set_variant(dr, *choice)
+    dr.reset()
+    _CACHE[key] = choice
+    if cache_file:
+        os.makedirs(os.path.dirname(os.path.abspath(cache_file)), exist_ok=True)
+        with open(cache_file, "w") as fh:
+            json.dump({k: list(v) for k, v in _CACHE.items()}, fh)
+    return {"variant": choice, "ms_per_step": {f"{k[0]},{k[1]}": v for k, v in timings.items()},
+            "cached": False}
+
+
+def autotune_batch(batch, trial_steps: int = 32):
+    """Pick the workgroup order (flat or XCD bands) for every instance of an
+    rfx_amd.batch.Batch by timing ``trial_steps`` batched steps each; the
+    batch step is always two-pass.  Leaves the instances reset."""
+    n = len(batch.runs)
+    step0 = (C.c_int64 * n)(*([0] * n))
+    counts = (C.c_int64 * n)(*[min(trial_steps, s) for s in batch.n_steps])
+    stream = torch.cuda.current_stream(batch.device)
+    timings = {}
+    for order in (1, 0):
+        for r in batch.runs:
+            set_variant(r, 0, order)
+        for rep in range(2):
+            for r in batch.runs:
+                r.reset()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            _lib.check(batch.lib.rfxh_batch_run(batch.handle, step0, counts, stream.cuda_stream, 1))
+            b.record(stream)
+            b.synchronize()
+            if rep == 1:
+                timings[order] = a.elapsed_time(b)
+    choice = min(timings, key=timings.get)
+    for r in batch.runs:
+        set_variant(r, 0, choice)
+        r.reset()
+    return {"order": choice, "ms": timings}

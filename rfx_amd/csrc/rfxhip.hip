@@ -1091,6 +1091,18 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   return 0;
 }
 
+// Launch grid of the plan's workgroup order.
+int plan_grid(rfxh_plan* pl) {
+  const KParams& kp = pl->kp;
+  const long long flat = kp.plane;
+  const long long nblocks = kp.order == 0 ? 8LL * kp.nx * (((flat + 7) / 8 + kWG - 1) / kWG)
+                                          : (long long)kp.nx * ((flat + kWG - 1) / kWG);
+  if (nblocks > 0x7fffffffLL) return fail(RFXH_EINVAL, "grid too large");
+  pl->grid = dim3((unsigned)nblocks);
+  pl->block = dim3(kBX, kTY);
+  return 0;
+}
+
 // Sum the sampled kernel durations once the run has been queued.
 int collect_timing(rfxh_plan* pl, int n_samples) {
   for (int q = 0; q < n_samples; ++q) {
@@ -1283,15 +1295,11 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pl->planes.push_back(pp);
     pl->plane_comp.push_back(pd.comp);
   }
-  // workgroup order (RFXH_ORDER = flat (default) | bands, for comparisons)
+  // workgroup order (RFXH_ORDER = flat (default) | bands, for comparisons;
+  // rfxh_set_variant changes it later)
   const char* ord = std::getenv("RFXH_ORDER");
   kp.order = (ord && std::string(ord) == "bands") ? 0 : 1;
-  const long long flat = kp.plane;
-  const long long nblocks = kp.order == 0 ? 8LL * d->nx * (((flat + 7) / 8 + kWG - 1) / kWG)
-                                          : (long long)d->nx * ((flat + kWG - 1) / kWG);
-  if (nblocks > 0x7fffffffLL) return bail(fail(RFXH_EINVAL, "grid too large"));
-  pl->grid = dim3((unsigned)nblocks);
-  pl->block = dim3(kBX, kTY);
+  if ((rc = plan_grid(pl))) return bail(rc);
   pl->tev.resize(4 * kTimingSamples);
   for (auto& e : pl->tev) {
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RFXH_EHIP, "hipEventCreate"));
@@ -1440,6 +1448,33 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
       ps.ew[f][t] = e0 ? e0 + o : nullptr;
     }
   return ps;
+}
+
+int rfxh_set_variant(rfxh_plan* pl, int fused, int order) {
+  if (!pl || order < 0 || order > 1) return fail(RFXH_EINVAL, "bad variant");
+  if (fused && !pl->set[1][0]) return fail(RFXH_EINVAL, "the fused step needs a second field set");
+  const int old_order = pl->kp.order;
+  pl->kp.order = order;
+  int rc = plan_grid(pl);
+  if (rc) {
+    pl->kp.order = old_order;
+    (void)plan_grid(pl);
+    return rc;
+  }
+  pl->fused = fused != 0;
+  for (int g = 0; g < 2; ++g)
+    if (pl->graph_exec[g]) {
+      (void)hipGraphExecDestroy(pl->graph_exec[g]);
+      pl->graph_exec[g] = nullptr;
+    }
+  return RFXH_OK;
+}
+
+int rfxh_get_variant(rfxh_plan* pl, int* fused, int* order) {
+  if (!pl || !fused || !order) return fail(RFXH_EINVAL, "bad arguments");
+  *fused = pl->fused ? 1 : 0;
+  *order = pl->kp.order;
+  return RFXH_OK;
 }
 
 int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
