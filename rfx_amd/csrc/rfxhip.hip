@@ -145,30 +145,33 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < rem ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s;
 }
 
-// Cell of this thread in logical workgroup L of the problem p.
+// Cell of this thread in logical workgroup L of the problem p.  All index
+// math is 32-bit: a y-z plane holds fewer than 2^24 cells (checked at plan
+// creation), so the in-plane index f is exact in float and the float
+// quotient f/nz is within one of the integer one.
 __device__ __forceinline__ bool cell_at(const KParams& p, int L, int& i, int& j, int& k) {
-  const long long pl = p.plane;
-  long long f;
+  const int pl = (int)p.plane;
+  const int t = (int)(threadIdx.y * kBX + threadIdx.x);
+  int f;
   if (p.order == 0) {
     // bands: the (y, z) plane is cut into 8 flat bands, one per XCD; the XCD
     // sweeps its band plane by plane with 512-cell chunks fastest
-    const long long bw = (pl + 7) / 8;
-    const int ncb = (int)((bw + kWG - 1) / kWG);
+    const int bw = (pl + 7) / 8;
+    const int ncb = (bw + kWG - 1) / kWG;
     const int b = L / (p.nx * ncb);
-    const int r = L % (p.nx * ncb);
+    const int r = L - b * (p.nx * ncb);
     i = r / ncb;
-    f = (long long)b * bw + (long long)(r % ncb) * kWG + threadIdx.y * kBX + threadIdx.x;
-    if (f >= (long long)(b + 1) * bw) return false;
+    f = b * bw + (r - i * ncb) * kWG + t;
+    if (f >= (b + 1) * bw) return false;
   } else {
     // flat chunks of 512 cells, chunk index fastest, then plane
-    const int nch = (int)((pl + kWG - 1) / kWG);
+    const int nch = (pl + kWG - 1) / kWG;
     i = L / nch;
-    f = (long long)(L % nch) * kWG + threadIdx.y * kBX + threadIdx.x;
+    f = (L - i * nch) * kWG + t;
   }
   if (f >= pl) return false;
-  // f < ny*nz is exact in float; the product is within one of the quotient
   int jj = (int)((float)f * p.inv_nz);
-  int kk = (int)f - jj * p.nz;
+  int kk = f - jj * p.nz;
   if (kk < 0) {
     --jj;
     kk += p.nz;
@@ -1155,6 +1158,8 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   const long long ncell = (long long)d->nx * d->ny * d->nz;
   if (ncell >= (1LL << 30))
     return fail(RFXH_EINVAL, "grid larger than 2^30 cells (field arrays must stay below 4 GiB)");
+  if ((long long)d->ny * d->nz >= (1LL << 24))
+    return fail(RFXH_EINVAL, "y-z plane of 2^24 cells or more (the in-plane index must be exact in float32)");
   std::vector<rfxh_point_op> ops(d->ops, d->ops + d->n_ops);
   for (const rfxh_point_op& op : ops) {
     if (op.cell < 0 || op.cell >= ncell) return fail(RFXH_EINVAL, "point operation outside the grid");

@@ -163,3 +163,42 @@ def test_segmented_gradient_equals_single(torch_mod):
         grads.append(eps.grad.numpy().copy())
     np.testing.assert_allclose(grads[1], grads[0], rtol=1e-5,
                                atol=1e-6 * np.abs(grads[0]).max())
+
+
+def test_graded_gradient_directions_vs_oracle_fd(torch_mod):
+    """Graded-z lane (CPML, lossy substrate, PEC sheet): directional
+    derivatives of a probe-energy objective vs float64 oracle central
+    differences."""
+    torch = torch_mod
+    _say("graded directional FD")
+    spec = cases.mixed_graded_case()
+    spec["ports"] = [p for p in spec["ports"] if p[2] == 0.0]
+    spec["s11_freqs"] = None
+    n = 200
+    sim = cases.to_sim(spec)
+    g0, cfg0 = o.build_graded(spec, n)
+    base = cfg0.eps_r.copy()
+    region = (3, 15, 4, 14, 2, 12)
+    eps = torch.tensor(base, requires_grad=True)
+    res = sim.forward(eps_override=eps, n_steps=n, grad_region=region)
+    ts = res.time_series.double()
+    J = (ts ** 2).sum()
+    J.backward()
+    grad = eps.grad.numpy().astype(np.float64)
+    rng = np.random.default_rng(11)
+    i0, i1, j0, j1, k0, k1 = region
+    for trial in range(2):
+        d = np.zeros_like(base, dtype=np.float64)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        h = 1e-3
+        Js = []
+        for sgn in (1, -1):
+            sp = dict(spec)
+            sp["eps_override"] = (base + sgn * h * d).astype(np.float32)
+            gg, cfg = o.build_graded(sp, n)
+            out = cstep.run_c(cfg, n, want_fields=False)
+            Js.append(float((out.time_series ** 2).sum()))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float((grad * d).sum())
+        print("graded direction", trial, ad, fd, abs(ad - fd) / abs(fd))
+        assert abs(ad - fd) <= 2e-3 * abs(fd)
