@@ -198,6 +198,11 @@ def main():
     ap.add_argument("--time-steps", type=int, default=None,
                     help="time steps per run (default: the workload's own count); "
                          "for short profiled runs")
+    ap.add_argument("--variant", default="auto",
+                    choices=["auto", "two-pass-flat", "two-pass-bands", "fused-flat",
+                             "fused-bands"],
+                    help="kernel variant; auto = the project autotuner's choice for this "
+                         "problem (timed on its first steps, state reset afterwards)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 512 MiB device copy five times, the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE")
@@ -217,6 +222,17 @@ def main():
     torch.cuda.synchronize()
     print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s, grid {grid.shape}, "
           f"materials {len(prob.mat.table)}", file=sys.stderr, flush=True)
+    from rfx_amd import autotune as at
+    tuning = None
+    if args.variant == "auto":
+        tuning = at.autotune(dr, trial_steps=32)
+        fused_v, order_v = tuning["variant"]
+    else:
+        fused_v = 1 if args.variant.startswith("fused") else 0
+        order_v = 1 if args.variant.endswith("flat") else 0
+        at.set_variant(dr, fused_v, order_v)
+    variant = f"{'fused' if fused_v else 'two-pass'}-{'flat' if order_v else 'bands'}"
+    print(f"[bench] variant {variant} (autotune: {tuning})", file=sys.stderr, flush=True)
     cells = int(np.prod(grid.shape))
     if args.copy_calib:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
@@ -245,8 +261,7 @@ def main():
     ms_e, n_e = dr.kernel_time_ms(1)
     ms_d, n_d = dr.kernel_time_ms(2)
     value = args.steps * n_total * cells / el / 1e6
-    fused = ms_e < 0.05 * ms_h  # the fused step has no separate E pass
-    if fused:
+    if fused_v:
         # one launch does the whole step: 48 B per cell
         which, ms, unit_bytes = "k_step_fused", ms_h, BYTES_PER_CELL_STEP * cells
     else:
@@ -258,7 +273,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get(f"{args.config}:{which}")
+            traffic = json.load(fh).get(f"{args.config}:{variant}:{which}")
     out = {
         "metric": "Mcell-steps/s fp32 + % HBM roofline; fwd+adjoint grad time vs plain forward",
         "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -271,7 +286,8 @@ def main():
                                 f"16 probes, 32-freq DFT plane, {n_total} time steps")
                    if args.config == "c2" else
                    f"README quick start {grid.shape}, {n_total} time steps",
-                   "cells": cells, "time_steps": n_total,
+                   "cells": cells, "time_steps": n_total, "variant": variant,
+                   "autotune_ms_per_step": tuning["ms_per_step"] if tuning else None,
                    "pct_hbm_roofline_48B": round(100 * value * 1e6 * BYTES_PER_CELL_STEP
                                                  / (HBM_PEAK_GBS * 1e9), 2),
                    "kernel_ms": {"k_update_h": round(ms_h, 4), "k_update_e": round(ms_e, 4),

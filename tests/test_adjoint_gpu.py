@@ -146,8 +146,11 @@ def test_gradient_directions_vs_oracle_fd(torch_mod):
             Js.append(float((out.time_series ** 2 * np.linspace(0.5, 1.5, ts.shape[1])).sum()))
         fd = (Js[0] - Js[1]) / (2 * h)
         ad = float((grad * d).sum())
-        print("direction", trial, ad, fd, abs(ad - fd) / abs(fd))
-        assert abs(ad - fd) <= 2e-3 * abs(fd)
+        # a random direction can nearly cancel; float32 rounding of the
+        # gradient scales with sum |g_i d_i|, not with the projection
+        scale = float(np.abs(grad * d).sum())
+        print("direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
 
 
 def test_segmented_gradient_equals_single(torch_mod):
@@ -200,5 +203,6 @@ def test_graded_gradient_directions_vs_oracle_fd(torch_mod):
             Js.append(float((out.time_series ** 2).sum()))
         fd = (Js[0] - Js[1]) / (2 * h)
         ad = float((grad * d).sum())
-        print("graded direction", trial, ad, fd, abs(ad - fd) / abs(fd))
-        assert abs(ad - fd) <= 2e-3 * abs(fd)
+        scale = float(np.abs(grad * d).sum())
+        print("graded direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
