@@ -137,6 +137,70 @@ def grad_cost(n_steps=3000):
             "ratio": round(t_grad / t_fwd, 3), "gradient_finite": finite}
 
 
+def c5_specs(n_inst=48):
+    """Batched-sweep workload: 48 instances, interior sizes from {64^3, 96^3,
+    128x128x64, 160x96x96, 192x192x48}, dx = 1 mm, 8-layer CPML, one
+    dielectric slab (eps_r ~ U(2,12), 4-20 cells thick, normal to a random
+    axis), an Ez pulse source (f0 ~ U(2,20) GHz), 4 Ez probes, 1500-4000
+    steps; seed 5."""
+    rng = np.random.default_rng(5)
+    sizes = [(64, 64, 64), (96, 96, 96), (128, 128, 64), (160, 96, 96), (192, 192, 48)]
+    dx = 1e-3
+    out = []
+    for q in range(n_inst):
+        n = sizes[int(rng.integers(0, len(sizes)))]
+        ax = int(rng.integers(0, 3))
+        th = int(rng.integers(4, 21))
+        c0 = int(rng.integers(1, n[ax] - th - 1))
+        lo, hi = [0.0, 0.0, 0.0], [n[0] * dx, n[1] * dx, n[2] * dx]
+        lo[ax], hi[ax] = (c0 + 0.5) * dx, (c0 + th + 0.5) * dx
+        eps = float(rng.uniform(2, 12))
+        f0 = float(rng.uniform(2e9, 20e9))
+        steps = int(rng.integers(1500, 4001))
+        src = tuple(0.3 * v * dx for v in n)
+        probes = [(tuple(float(rng.uniform(0.2, 0.8)) * v * dx for v in n), "ez") for _ in range(4)]
+        out.append((dict(freq_max=25e9, domain=tuple(v * dx for v in n), dx=dx, boundary="cpml",
+                         cpml_layers=8, materials={"slab": (eps, 0.0, 1.0)},
+                         geometry=[(tuple(lo), tuple(hi), "slab")],
+                         sources=[(src, "ez", f0, 0.5)], probes=probes, dft_planes=[]), steps))
+    return out
+
+
+def batch_sweep():
+    """Whole-sweep throughput of the batched launch vs the same instances
+    run one after another (each with its own captured graphs)."""
+    import torch
+    from rfx_amd.batch import Batch
+    specs = c5_specs()
+    sims = [build_sim(sp) for sp, _ in specs]
+    steps = [n for _, n in specs]
+    b = Batch(sims, steps, "cuda:0")
+    cell_steps = sum(int(np.prod(r.prob.shape)) * n for r, n in zip(b.runs, steps))
+    b.run()                                   # warm-up: graphs captured
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    for r, n in zip(b.runs, steps):           # warm-up of the per-instance graphs
+        r.reset()
+        r.run(0, min(n, 64))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r, n in zip(b.runs, steps):
+        r.reset()
+        r.run(0, n)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    b.close()
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(specs)} instances, {min(steps)}-{max(steps)} steps, "
+                        f"{cell_steps / 1e9:.1f} G cell-steps",
+            "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
+            "batched_Mcell_steps_per_s": round(cell_steps / t_batch / 1e6, 1),
+            "sequential_Mcell_steps_per_s": round(cell_steps / t_seq / 1e6, 1)}
+
+
 def build_sim(spec):
     import rfx_amd as R
     sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec["dx"],
@@ -192,6 +256,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(N_STEPS))
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-sweep timing")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the forward+adjoint / forward timing")
     ap.add_argument("--no-graph", action="store_true")
@@ -299,6 +365,7 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": None,
         "grad_vs_forward": None,
+        "batch_sweep": None,
     }
     dr.close()
     del dr
@@ -307,6 +374,10 @@ def main():
         print("[bench] forward+adjoint vs forward ...", file=sys.stderr, flush=True)
         out["grad_vs_forward"] = grad_cost()
         print(f"[bench] grad {out['grad_vs_forward']}", file=sys.stderr, flush=True)
+    if not args.no_batch and args.config == "c2":
+        print("[bench] batched sweep ...", file=sys.stderr, flush=True)
+        out["batch_sweep"] = batch_sweep()
+        print(f"[bench] batch {out['batch_sweep']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(spec, n_total, args.cpu_steps)
