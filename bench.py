@@ -265,8 +265,6 @@ def main():
                     help="time steps per run (default: the workload's own count); "
                          "for short profiled runs")
     ap.add_argument("--variant", default="auto",
-                    choices=["auto", "two-pass-flat", "two-pass-bands", "fused-flat",
-                             "fused-bands"],
                     help="kernel variant; auto = the project autotuner's choice for this "
                          "problem (timed on its first steps, state reset afterwards)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -292,12 +290,11 @@ def main():
     tuning = None
     if args.variant == "auto":
         tuning = at.autotune(dr, trial_steps=32)
-        fused_v, order_v = tuning["variant"]
+        kern_v, prm_v = tuning["variant"]
     else:
-        fused_v = 1 if args.variant.startswith("fused") else 0
-        order_v = 1 if args.variant.endswith("flat") else 0
-        at.set_variant(dr, fused_v, order_v)
-    variant = f"{'fused' if fused_v else 'two-pass'}-{'flat' if order_v else 'bands'}"
+        kern_v, prm_v = at.parse_variant(args.variant)
+        at.set_variant(dr, kern_v, prm_v)
+    variant = at.variant_name((kern_v, prm_v))
     print(f"[bench] variant {variant} (autotune: {tuning})", file=sys.stderr, flush=True)
     cells = int(np.prod(grid.shape))
     if args.copy_calib:
@@ -327,9 +324,10 @@ def main():
     ms_e, n_e = dr.kernel_time_ms(1)
     ms_d, n_d = dr.kernel_time_ms(2)
     value = args.steps * n_total * cells / el / 1e6
-    if fused_v:
+    if kern_v:
         # one launch does the whole step: 48 B per cell
-        which, ms, unit_bytes = "k_step_fused", ms_h, BYTES_PER_CELL_STEP * cells
+        which = "k_step_march" if kern_v == 2 else "k_step_fused"
+        ms, unit_bytes = ms_h, BYTES_PER_CELL_STEP * cells
     else:
         # each pass advances half the field components: 24 B per cell
         which, ms = ("k_update_e", ms_e) if ms_e >= ms_h else ("k_update_h", ms_h)

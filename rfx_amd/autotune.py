@@ -1,8 +1,10 @@
 """Kernel-variant autotuner.
 
 The plan's step can run as the two-pass kernels (H pass, E pass, in place)
-or, with a second field set, as one fused kernel; either with the flat or
-the XCD-band workgroup order (include/rfxhip.h ``rfxh_set_variant``).  All
+or, with a second field set, as one fused kernel (flat or XCD-band
+workgroup order) or as the marching kernel (tiles of the y-z plane walking
+along x, in a chosen number of x segments) (include/rfxhip.h
+``rfxh_set_variant``).  All
 variants compute the same step.  Which is fastest depends on the grid (CPML
 share, plane size, cache fit), so the tuner runs a few real steps of the
 actual problem under each variant, timed with events on the stream the run
@@ -31,11 +33,28 @@ def _key(dr) -> str:
 
 
 def variants(dr):
-    fused_ok = len(dr.sets) == 2
+    """(kernel, parameter) candidates: kernel 0 two-pass and 1 fused with
+    workgroup order 1 (flat) or 0 (XCD bands); kernel 2 marching with its
+    number of x segments (0 = automatic)."""
     out = [(0, 1), (0, 0)]
-    if fused_ok:
-        out += [(1, 1), (1, 0)]
+    if len(dr.sets) == 2:
+        out += [(1, 1), (1, 0), (2, 0)]
+        nx = dr.prob.shape[0]
+        out += [(2, s) for s in (2, 4, 8, 16) if s <= max(1, nx // 4)]
     return out
+
+
+def variant_name(v) -> str:
+    k, prm = v
+    if k == 2:
+        return "march" if prm == 0 else f"march-{prm}"
+    return f"{'fused' if k else 'two-pass'}-{'flat' if prm else 'bands'}"
+
+
+def parse_variant(name: str):
+    if name.startswith("march"):
+        return (2, int(name.split("-")[1]) if "-" in name else 0)
+    return (1 if name.startswith("fused") else 0, 1 if name.endswith("flat") else 0)
 
 
 def get_variant(dr):
@@ -87,7 +106,7 @@ def autotune(dr, trial_steps: int = 32, repeats: int = 2, cache_file: str | None
         os.makedirs(os.path.dirname(os.path.abspath(cache_file)), exist_ok=True)
         with open(cache_file, "w") as fh:
             json.dump({k: list(v) for k, v in _CACHE.items()}, fh)
-    return {"variant": choice, "ms_per_step": {f"{k[0]},{k[1]}": v for k, v in timings.items()},
+    return {"variant": choice, "ms_per_step": {variant_name(k): round(v, 4) for k, v in timings.items()},
             "cached": False}
 
 

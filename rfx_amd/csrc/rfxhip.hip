@@ -257,34 +257,17 @@ __device__ __forceinline__ void point_ops(const KParams& p, unsigned cell, float
 }
 
 // ---------------------------------------------------------------------------
-// H at one cell: H^{n+1/2} = H^{n-1/2} - ch curl_fwd E^n (yee.py:272-289,
-// graded :429-451) plus the CPML-H correction (cpml.py:875-1043), reading
-// CPML memory ps.hr and (WRITE) storing ps.hw and the result into Hn.
+// H at one cell from values: H^{n+1/2} = H^{n-1/2} - ch curl_fwd E^n
+// (yee.py:272-289, graded :429-451) plus the CPML-H correction
+// (cpml.py:875-1043).  E at the cell and at its forward neighbours (zero
+// past the last cell, yee.py:121-128) come in as values; CPML memory is
+// read from ps.hr and, when WRITE, the new memory is stored into ps.hw.
 // ---------------------------------------------------------------------------
 template <bool GRADED, bool WRITE>
-__device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo[3],
-                                         const float* const Ho[3], float* const Hn[3],
-                                         const PsiIO& ps, int i, int j, int k) {
-  const unsigned P = (unsigned)p.plane;
-  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
-  const unsigned m = ldu16(p.mat, c);
-  // neighbour loads are issued unconditionally from an in-range index and
-  // zeroed afterwards (zero padding past the last cell, yee.py:121-128), so
-  // no load sits behind a branch
-  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
-  const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
-  const float ex = ldf(Eo[0], c), ey = ldf(Eo[1], c), ez = ldf(Eo[2], c);
-  float ex_j = ldf(Eo[0], cj), ez_j = ldf(Eo[2], cj);
-  float ex_k = ldf(Eo[0], ck), ey_k = ldf(Eo[1], ck);
-  float ey_i = ldf(Eo[1], ci), ez_i = ldf(Eo[2], ci);
-  float hx = ldf(Ho[0], c), hy = ldf(Ho[1], c), hz = ldf(Ho[2], c);
-  const float ch = p.coef[m].w;
-  ex_j = hj ? ex_j : 0.f;
-  ez_j = hj ? ez_j : 0.f;
-  ex_k = hk ? ex_k : 0.f;
-  ey_k = hk ? ey_k : 0.f;
-  ey_i = hi ? ey_i : 0.f;
-  ez_i = hi ? ez_i : 0.f;
+__device__ __forceinline__ float3 h_core(const KParams& p, const PsiIO& ps, int i, int j, int k,
+                                         float ch, float ex, float ey, float ez, float ex_j,
+                                         float ez_j, float ex_k, float ey_k, float ey_i, float ez_i,
+                                         float hx, float hy, float hz, bool wr = true) {
   float ix, iy, iz;
   if (GRADED) {
     ix = p.inv_h[0][i];
@@ -318,7 +301,7 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
       }
       const float s0 = bb * ldf(ps.hr[face][0], q) + cc * g0;
       const float s1 = bb * ldf(ps.hr[face][1], q) + cc * g1;
-      if (WRITE) {
+      if (WRITE && wr) {
         stf(ps.hw[face][0], q, s0);
         stf(ps.hw[face][1], q, s1);
       }
@@ -336,23 +319,52 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
       }
     }
   }
-  if (WRITE) {
-    stf(Hn[0], c, hx);
-    stf(Hn[1], c, hy);
-    stf(Hn[2], c, hz);
-  }
   return make_float3(hx, hy, hz);
 }
 
-// E at one cell from H^{n+1/2} at the cell and its backward neighbours:
-// E^{n+1} = ca E^n + cb curl_bwd H (yee.py:379-404), CPML-E (cpml.py:608-777),
-// PEC faces and mask, point operations; stores into En.
-template <bool GRADED>
-__device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3],
-                                       float* const En[3], const PsiIO& ps, int i, int j, int k,
-                                       float hx, float hy, float hz, float hy_i, float hz_i,
-                                       float hx_j, float hz_j, float hx_k, float hy_k) {
+// H at one cell reading E^n and H^{n-1/2} from Eo / Ho, storing into Hn.
+template <bool GRADED, bool WRITE>
+__device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo[3],
+                                         const float* const Ho[3], float* const Hn[3],
+                                         const PsiIO& ps, int i, int j, int k) {
+  const unsigned P = (unsigned)p.plane;
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned m = ldu16(p.mat, c);
+  // neighbour loads are issued unconditionally from an in-range index and
+  // zeroed afterwards, so no load sits behind a branch
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
+  const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
+  const float ex = ldf(Eo[0], c), ey = ldf(Eo[1], c), ez = ldf(Eo[2], c);
+  float ex_j = ldf(Eo[0], cj), ez_j = ldf(Eo[2], cj);
+  float ex_k = ldf(Eo[0], ck), ey_k = ldf(Eo[1], ck);
+  float ey_i = ldf(Eo[1], ci), ez_i = ldf(Eo[2], ci);
+  const float hx = ldf(Ho[0], c), hy = ldf(Ho[1], c), hz = ldf(Ho[2], c);
+  const float ch = p.coef[m].w;
+  ex_j = hj ? ex_j : 0.f;
+  ez_j = hj ? ez_j : 0.f;
+  ex_k = hk ? ex_k : 0.f;
+  ey_k = hk ? ey_k : 0.f;
+  ey_i = hi ? ey_i : 0.f;
+  ez_i = hi ? ez_i : 0.f;
+  const float3 h = h_core<GRADED, WRITE>(p, ps, i, j, k, ch, ex, ey, ez, ex_j, ez_j, ex_k, ey_k,
+                                         ey_i, ez_i, hx, hy, hz);
+  if (WRITE) {
+    stf(Hn[0], c, h.x);
+    stf(Hn[1], c, h.y);
+    stf(Hn[2], c, h.z);
+  }
+  return h;
+}
+
+// E at one cell from values: E^{n+1} = ca E^n + cb curl_bwd H (yee.py:379-404),
+// CPML-E (cpml.py:608-777), PEC faces and mask, point operations.  H^{n+1/2}
+// at the cell and its backward neighbours (zero before the first cell) and
+// E^n at the cell come in as values; returns E^{n+1}.
+template <bool GRADED>
+__device__ __forceinline__ float3 e_core(const KParams& p, const PsiIO& ps, int i, int j, int k,
+                                         unsigned c, unsigned m, float e0, float e1, float e2,
+                                         float hx, float hy, float hz, float hy_i, float hz_i,
+                                         float hx_j, float hz_j, float hx_k, float hy_k) {
   float ix, iy, iz;
   if (GRADED) {
     ix = p.inv_e[0][i];
@@ -361,8 +373,6 @@ __device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3
   } else {
     ix = iy = iz = p.inv_e0;
   }
-  const unsigned m = ldu16(p.mat, c);
-  float e0 = ldf(Eo[0], c), e1 = ldf(Eo[1], c), e2 = ldf(Eo[2], c);
   const float4 co = p.coef[m];
   const float ca = co.x, cb = co.y, ce = co.z;
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
@@ -424,9 +434,23 @@ __device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3
     if (fl & RFXH_POINT)
       point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j);
   }
-  stf(En[0], c, e0);
-  stf(En[1], c, e1);
-  stf(En[2], c, e2);
+  return make_float3(e0, e1, e2);
+}
+
+// E at one cell reading E^n from Eo, storing E^{n+1} into En.
+template <bool GRADED>
+__device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3],
+                                       float* const En[3], const PsiIO& ps, int i, int j, int k,
+                                       float hx, float hy, float hz, float hy_i, float hz_i,
+                                       float hx_j, float hz_j, float hx_k, float hy_k) {
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned m = ldu16(p.mat, c);
+  const float e0 = ldf(Eo[0], c), e1 = ldf(Eo[1], c), e2 = ldf(Eo[2], c);
+  const float3 e = e_core<GRADED>(p, ps, i, j, k, c, m, e0, e1, e2, hx, hy, hz, hy_i, hz_i, hx_j,
+                                  hz_j, hx_k, hy_k);
+  stf(En[0], c, e.x);
+  stf(En[1], c, e.y);
+  stf(En[2], c, e.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,6 +482,127 @@ __global__ __launch_bounds__(kWG) void k_step_fused(KParams p, FusedIO io, PsiIO
     hy_k = t.y;
   }
   e_cell<GRADED>(p, Eo, En, ps, i, j, k, h.x, h.y, h.z, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
+}
+
+// ---------------------------------------------------------------------------
+// Marching fused step: read set A (E^n, H^{n-1/2}, psi), write set B.
+// A workgroup owns a tile of kMJ x kMK cells of the (y, z) plane and walks
+// along x through its segment of planes.  Wave 0 and lane 0 of every wave
+// are halo threads: they compute H at the tile's j-1 row and k-1 column
+// (never stored), so the E update finds every backward H neighbour in LDS
+// (double-buffered, one barrier per plane).  E and H of the cell's own
+// column stay in registers from one plane to the next: E^n of plane i+1 is
+// loaded once and reused as the cell's own E^n at the next plane, and
+// H^{n+1/2} of plane i is the x-backward neighbour at plane i+1.  Each field
+// value is read from HBM once per step; neighbours along y and z come
+// through the caches from the same workgroup.
+// ---------------------------------------------------------------------------
+constexpr int kMJ = 8;               // interior rows (y) per tile
+constexpr int kMK = kBX - 1;         // interior columns (z) per tile
+constexpr int kMW = kMJ + 1;         // waves per workgroup (incl. the halo row)
+
+struct MarchIO {
+  const float* eo[3];
+  const float* ho[3];
+  float* en[3];
+  float* hn[3];
+  int ntk, ntj, seg_len;
+};
+
+template <bool GRADED>
+__global__ __launch_bounds__(kBX * kMW) void k_step_march(KParams p, MarchIO io, PsiIO ps) {
+  __shared__ float lhx[2][kMW][kBX];
+  __shared__ float lhy[2][kMW][kBX];
+  __shared__ float lhz[2][kMW][kBX];
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int tk = L % io.ntk;
+  const int r = L / io.ntk;
+  const int tj = r % io.ntj;
+  const int seg = r / io.ntj;
+  const int w = (int)threadIdx.y, l = (int)threadIdx.x;
+  const int j = tj * kMJ + w - 1, k = tk * kMK + l - 1;
+  const int i0 = seg * io.seg_len;
+  const int i1 = min(p.nx, i0 + io.seg_len);
+  const bool valid = j >= 0 && j < p.ny && k >= 0 && k < p.nz;
+  const bool inner = valid && w > 0 && l > 0;
+  const unsigned P = (unsigned)p.plane;
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
+  // column base (plane 0) of this thread's cell; clamped for invalid threads
+  const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
+  const unsigned dj = hj ? (unsigned)p.nz : 0u, dk = hk ? 1u : 0u;
+  float e0 = 0.f, e1 = 0.f, e2 = 0.f;  // E^n of the current plane, own cell
+  float hyp = 0.f, hzp = 0.f;          // H^{n+1/2} of the previous plane
+  if (valid) {
+    const unsigned c = (unsigned)i0 * P + c0;
+    e0 = ldf(io.eo[0], c);
+    e1 = ldf(io.eo[1], c);
+    e2 = ldf(io.eo[2], c);
+    if (i0 > 0) {
+      // H of plane i0-1 (owned by the previous segment): recomputed, not stored
+      const unsigned cm = c - P;
+      const unsigned m = ldu16(p.mat, cm);
+      const float ax = ldf(io.eo[0], cm), ay = ldf(io.eo[1], cm), az = ldf(io.eo[2], cm);
+      float ex_j = ldf(io.eo[0], cm + dj), ez_j = ldf(io.eo[2], cm + dj);
+      float ex_k = ldf(io.eo[0], cm + dk), ey_k = ldf(io.eo[1], cm + dk);
+      ex_j = hj ? ex_j : 0.f;
+      ez_j = hj ? ez_j : 0.f;
+      ex_k = hk ? ex_k : 0.f;
+      ey_k = hk ? ey_k : 0.f;
+      const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, p.coef[m].w, ax, ay, az, ex_j,
+                                             ez_j, ex_k, ey_k, e1, e2, ldf(io.ho[0], cm),
+                                             ldf(io.ho[1], cm), ldf(io.ho[2], cm));
+      hyp = h.y;
+      hzp = h.z;
+    }
+  }
+  for (int i = i0; i < i1; ++i) {
+    const int b = i & 1;
+    const unsigned c = (unsigned)i * P + c0;
+    float3 h = make_float3(0.f, 0.f, 0.f);
+    unsigned m = 0;
+    float n0 = 0.f, n1 = 0.f, n2 = 0.f;  // E^n of plane i+1, own cell
+    if (valid) {
+      m = ldu16(p.mat, c);
+      const bool hi = i + 1 < p.nx;
+      const unsigned cn = hi ? c + P : c;
+      n0 = ldf(io.eo[0], cn);
+      n1 = ldf(io.eo[1], cn);
+      n2 = ldf(io.eo[2], cn);
+      float ex_j = ldf(io.eo[0], c + dj), ez_j = ldf(io.eo[2], c + dj);
+      float ex_k = ldf(io.eo[0], c + dk), ey_k = ldf(io.eo[1], c + dk);
+      const float hx = ldf(io.ho[0], c), hy = ldf(io.ho[1], c), hz = ldf(io.ho[2], c);
+      n0 = hi ? n0 : 0.f;
+      n1 = hi ? n1 : 0.f;
+      n2 = hi ? n2 : 0.f;
+      ex_j = hj ? ex_j : 0.f;
+      ez_j = hj ? ez_j : 0.f;
+      ex_k = hk ? ex_k : 0.f;
+      ey_k = hk ? ey_k : 0.f;
+      h = h_core<GRADED, true>(p, ps, i, j, k, p.coef[m].w, e0, e1, e2, ex_j, ez_j, ex_k, ey_k,
+                               n1, n2, hx, hy, hz, inner);
+    }
+    lhx[b][w][l] = h.x;
+    lhy[b][w][l] = h.y;
+    lhz[b][w][l] = h.z;
+    __syncthreads();
+    if (inner) {
+      const float hx_j = lhx[b][w - 1][l], hz_j = lhz[b][w - 1][l];
+      const float hx_k = lhx[b][w][l - 1], hy_k = lhy[b][w][l - 1];
+      const float3 e = e_core<GRADED>(p, ps, i, j, k, c, m, e0, e1, e2, h.x, h.y, h.z, hyp, hzp,
+                                      hx_j, hz_j, hx_k, hy_k);
+      stf(io.hn[0], c, h.x);
+      stf(io.hn[1], c, h.y);
+      stf(io.hn[2], c, h.z);
+      stf(io.en[0], c, e.x);
+      stf(io.en[1], c, e.y);
+      stf(io.en[2], c, e.z);
+    }
+    hyp = h.y;
+    hzp = h.z;
+    e0 = n0;
+    e1 = n1;
+    e2 = n2;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -992,7 +1137,9 @@ __global__ void k_advance(long long* step) {
 struct rfxh_plan {
   KParams kp;
   dim3 grid, block;
-  bool fused = false;           // second field set given: fused step
+  bool fused = false;           // ping-pong step over two field sets (fused or marching kernel)
+  bool march = false;           // the ping-pong step uses the marching kernel
+  int march_seg = 0;            // x segments of the marching kernel (0: automatic)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   int cur = 0;                  // which set holds the current state
@@ -1038,6 +1185,26 @@ long long face_slab_floats(const rfxh_desc* d, int face) {
   return s;
 }
 
+// Tiles and x segments of the marching kernel.  Automatic segments: enough
+// workgroups for about eight per CU, at least four planes per segment.
+long long march_blocks(const KParams& kp, int want_seg, MarchIO* io, int* nseg_out) {
+  const int ntj = (kp.ny + kMJ - 1) / kMJ, ntk = (kp.nz + kMK - 1) / kMK;
+  const long long tiles = (long long)ntj * ntk;
+  int nseg = want_seg;
+  if (nseg <= 0) {
+    nseg = (int)std::max<long long>(1, (2048 + tiles - 1) / tiles);
+    nseg = std::min(nseg, std::max(1, kp.nx / 4));
+  }
+  nseg = std::min(nseg, kp.nx);
+  const int seg_len = (kp.nx + nseg - 1) / nseg;
+  nseg = (kp.nx + seg_len - 1) / seg_len;
+  io->ntj = ntj;
+  io->ntk = ntk;
+  io->seg_len = seg_len;
+  *nseg_out = nseg;
+  return tiles * nseg;
+}
+
 // One time step starting from field set `from` (fused: writes set 1-from).
 int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   // ev (optional): four events bracketing the main kernel(s) and the DFT
@@ -1055,7 +1222,22 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
       ps.ew[f][t] = e0 ? e0 + wo : nullptr;
     }
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
-  if (pl->fused) {
+  if (pl->march) {
+    MarchIO io;
+    for (int a = 0; a < 3; ++a) {
+      io.eo[a] = pl->set[from][a];
+      io.ho[a] = pl->set[from][3 + a];
+      io.en[a] = pl->set[to][a];
+      io.hn[a] = pl->set[to][3 + a];
+    }
+    int nseg = 1;
+    const long long nblk = march_blocks(pl->kp, pl->march_seg, &io, &nseg);
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_step_march<true>, dim3((unsigned)nblk), dim3(kBX, kMW), 0, s, pl->kp, io, ps);
+    else
+      hipLaunchKernelGGL(k_step_march<false>, dim3((unsigned)nblk), dim3(kBX, kMW), 0, s, pl->kp, io, ps);
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
+  } else if (pl->fused) {
     FusedIO io;
     for (int a = 0; a < 3; ++a) {
       io.eo[a] = pl->set[from][a];
@@ -1220,7 +1402,8 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   // MI355X at 533^3: the fused kernel's three redundant H evaluations make it
   // VALU-bound); RFXH_KERNEL=fused selects the fused step when two sets exist
   const char* kern = std::getenv("RFXH_KERNEL");
-  pl->fused = has2 && kern && std::string(kern) == "fused";
+  pl->fused = has2 && kern && (std::string(kern) == "fused" || std::string(kern) == "march");
+  pl->march = has2 && kern && std::string(kern) == "march";
   for (int c = 0; c < 6; ++c) {
     pl->set[0][c] = d->d_f[c];
     pl->set[1][c] = has2 ? d->d_f2[c] : nullptr;
@@ -1455,18 +1638,26 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
   return ps;
 }
 
-int rfxh_set_variant(rfxh_plan* pl, int fused, int order) {
-  if (!pl || order < 0 || order > 1) return fail(RFXH_EINVAL, "bad variant");
-  if (fused && !pl->set[1][0]) return fail(RFXH_EINVAL, "the fused step needs a second field set");
-  const int old_order = pl->kp.order;
-  pl->kp.order = order;
-  int rc = plan_grid(pl);
-  if (rc) {
-    pl->kp.order = old_order;
-    (void)plan_grid(pl);
-    return rc;
+int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
+  if (!pl || kernel < 0 || kernel > 2) return fail(RFXH_EINVAL, "bad variant");
+  if (kernel > 0 && !pl->set[1][0])
+    return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
+  if (kernel == 2) {
+    if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
+    pl->march_seg = param;
+  } else {
+    if (param < 0 || param > 1) return fail(RFXH_EINVAL, "bad workgroup order");
+    const int old_order = pl->kp.order;
+    pl->kp.order = param;
+    int rc = plan_grid(pl);
+    if (rc) {
+      pl->kp.order = old_order;
+      (void)plan_grid(pl);
+      return rc;
+    }
   }
-  pl->fused = fused != 0;
+  pl->fused = kernel != 0;
+  pl->march = kernel == 2;
   for (int g = 0; g < 2; ++g)
     if (pl->graph_exec[g]) {
       (void)hipGraphExecDestroy(pl->graph_exec[g]);
@@ -1475,10 +1666,10 @@ int rfxh_set_variant(rfxh_plan* pl, int fused, int order) {
   return RFXH_OK;
 }
 
-int rfxh_get_variant(rfxh_plan* pl, int* fused, int* order) {
-  if (!pl || !fused || !order) return fail(RFXH_EINVAL, "bad arguments");
-  *fused = pl->fused ? 1 : 0;
-  *order = pl->kp.order;
+int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
+  if (!pl || !kernel || !param) return fail(RFXH_EINVAL, "bad arguments");
+  *kernel = pl->march ? 2 : pl->fused ? 1 : 0;
+  *param = pl->march ? pl->march_seg : pl->kp.order;
   return RFXH_OK;
 }
 

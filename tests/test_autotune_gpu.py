@@ -1,6 +1,7 @@
 """Kernel variants and the autotuner on an MI355X: every variant (two-pass
-or fused step, flat or XCD-band workgroup order) meets the oracle bar on
-the mixed case, and the tuner leaves a valid choice and a zeroed state."""
+or fused step, flat or XCD-band workgroup order, marching step with any
+segment count) meets the oracle bar on the mixed case and the graded case,
+and the tuner leaves a valid choice and a zeroed state."""
 import sys
 
 import numpy as np
@@ -53,6 +54,9 @@ def test_every_variant_matches_oracle_and_tuner_picks_one(R):
         series[v] = ts
     base = series[(0, 1)]
     assert np.array_equal(series[(0, 0)], base)          # order does not change arithmetic
+    for v in series:
+        if v[0] == 2:                                     # marching == fused arithmetic per cell
+            assert cases.rel_l2(series[v], series[(1, 1)]) <= 1e-6
     res = at.autotune(dr, trial_steps=16)
     print("autotune", res)
     assert tuple(res["variant"]) in at.variants(dr)
@@ -60,6 +64,33 @@ def test_every_variant_matches_oracle_and_tuner_picks_one(R):
     dr.run(0, n)
     torch.cuda.synchronize()
     assert cases.rel_l2(dr.time_series().cpu().numpy(), ref.time_series) <= 1e-4
+    dr.close()
+
+
+def test_march_graded_vs_oracle(R):
+    import torch
+    from rfx_amd import autotune as at
+    from rfx_amd.engine import DeviceRun
+    _say("marching kernel on the graded mixed case")
+    spec = cases.mixed_graded_case()
+    n = 300
+    sim = cases.to_sim(spec)
+    _, prob, _ = sim._build_problem(n, "run", s11_freqs=np.asarray(spec["s11_freqs"], np.float32))
+    g, cfg = o.build_graded(spec, n)
+    ref = cstep.run_c(cfg, n, want_fields=False)
+    dr = DeviceRun(prob, n, "cuda")
+    for seg in (0, 1, 3, 7):
+        at.set_variant(dr, 2, seg)
+        dr.reset()
+        dr.run(0, n)
+        torch.cuda.synchronize()
+        ts = dr.time_series().cpu().numpy()
+        for p in range(ts.shape[1]):
+            e = cases.rel_l2(ts[:, p], ref.time_series[:, p])
+            print("march seg", seg, "probe", p, e)
+            assert e <= 1e-4
+        pv = dr.port_samples().cpu().numpy()[:, 0, 0]
+        assert cases.rel_l2(pv, ref.port_v[0]) <= 1e-4
     dr.close()
 
 
