@@ -427,6 +427,8 @@ class StepConfig:
     dft_planes: list = field(default_factory=list)     # (axis, index, comp, freqs)
     ports_vi: list = field(default_factory=list)       # (idx, comp, freqs, d_par, d_loop(dict))
     record_order: str = "uniform"                      # "uniform" | "graded"
+    # flux monitors: (axis, index, lo1, hi1, lo2, hi2, freqs f32, weights f64 [n_steps+1] or None)
+    flux: list = field(default_factory=list)
 
 
 class Stepper:
@@ -595,6 +597,7 @@ class RunOutput:
     port_i_dft: list
     fields: dict
     psi: dict
+    flux: list = field(default_factory=list)   # per monitor (e1, e2, h1, h2) complex128 DFTs
 
 
 def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
@@ -614,6 +617,8 @@ def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
             for ax, _, _, fr in cfg.dft_planes]
     pv = [np.zeros(n_steps) for _ in cfg.ports_vi]
     pi = [np.zeros(n_steps) for _ in cfg.ports_vi]
+    fl = [tuple(np.zeros((len(m[6]), m[3] - m[2], m[5] - m[4]), np.complex128) for _ in range(4))
+          for m in cfg.flux]
     dt = cfg.dt
     for n in range(n_steps):
         st.step_fields()
@@ -622,10 +627,12 @@ def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
             _inject(f, cfg, n)
             _probe(f, cfg, ts, n)
             _planes(f, cfg, accs, (n + 1) * dt, dt)
+            _flux(f, cfg, fl, n + 1, dt)
         else:
             _inject(f, cfg, n)
             _sample_ports(f, cfg, pv, pi, n)
             _planes(f, cfg, accs, n * dt, dt)
+            _flux(f, cfg, fl, n, dt)
             _probe(f, cfg, ts, n)
     tvec = np.arange(n_steps) * dt
     vd, idft = [], []
@@ -633,7 +640,7 @@ def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
         ph = np.exp(-1j * 2.0 * np.pi * np.outer(np.asarray(freqs, np.float64), tvec)) * dt
         vd.append(ph @ pv[p])
         idft.append(ph @ pi[p])
-    return RunOutput(ts, accs, pv, pi, vd, idft, f, st.psi)
+    return RunOutput(ts, accs, pv, pi, vd, idft, f, st.psi, fl)
 
 
 def _plane_shape(shape, ax):
@@ -668,6 +675,69 @@ def _planes(f, cfg, accs, t, dt):
     for q, (ax, index, comp, freqs) in enumerate(cfg.dft_planes):
         ph = np.exp(-1j * 2.0 * np.pi * np.asarray(freqs, np.float64) * t) * dt
         accs[q] += ph[:, None, None] * _plane(f[comp], ax, index)[None].astype(np.float64)
+
+
+# rfx/probes/probes.py:506-510: tangential (E1, E2, H1, H2) per plane normal
+FLUX_COMPONENTS = {0: ("ey", "ez", "hy", "hz"), 1: ("ez", "ex", "hz", "hx"),
+                   2: ("ex", "ey", "hx", "hy")}
+
+
+def _flux(f, cfg, fl, step, dt):
+    """Flux-monitor DFTs (rfx/simulation.py:1590-1644 uniform, step = n+1;
+    rfx/nonuniform.py:1484-1522 graded, step = n): E of the plane at
+    t = step dt, H averaged over the planes index-1 and index (co-located
+    with E) at t - dt/2, times dt and the window weight of `step`."""
+    for q, (ax, idx, lo1, hi1, lo2, hi2, freqs, wts) in enumerate(cfg.flux):
+        e1n, e2n, h1n, h2n = FLUX_COMPONENTS[ax]
+        im1 = max(idx - 1, 0)
+
+        def sl(a, i):
+            p = _plane(a, ax, i)
+            return p[lo1:hi1, lo2:hi2].astype(np.float64)
+
+        w = 1.0 if wts is None else float(wts[step])
+        t = step * dt
+        fr = np.asarray(freqs, np.float64)
+        ke = np.exp(-1j * 2.0 * np.pi * fr * t) * dt * w
+        kh = np.exp(-1j * 2.0 * np.pi * fr * (t - dt * 0.5)) * dt * w
+        hv1 = ((_plane(f[h1n], ax, im1) + _plane(f[h1n], ax, idx)) * 0.5)[lo1:hi1, lo2:hi2]
+        hv2 = ((_plane(f[h2n], ax, im1) + _plane(f[h2n], ax, idx)) * 0.5)[lo1:hi1, lo2:hi2]
+        e1, e2, a1, a2 = fl[q]
+        e1 += ke[:, None, None] * sl(f[e1n], idx)[None]
+        e2 += ke[:, None, None] * sl(f[e2n], idx)[None]
+        a1 += kh[:, None, None] * hv1.astype(np.float64)[None]
+        a2 += kh[:, None, None] * hv2.astype(np.float64)[None]
+
+
+def dft_window_weights(total_steps, window="rect", alpha=0.25):
+    """rfx/core/dft_utils.py:8-49 in float32, for steps 0..total_steps."""
+    if total_steps <= 1 or window == "rect":
+        return None
+    n = np.arange(total_steps + 1, dtype=F32)
+    N = F32(total_steps - 1)
+    x = n / np.maximum(N, F32(1.0))
+    two_pi = F32(2.0 * np.pi)
+    if window == "hann" or (window == "tukey" and alpha >= 1.0):
+        w = F32(0.5) * (F32(1.0) - np.cos(two_pi * x))
+    elif window == "tukey":
+        a = F32(alpha)
+        if alpha <= 0.0:
+            return None
+        left = F32(0.5) * (F32(1.0) + np.cos(F32(np.pi) * (F32(2.0) * x / a - F32(1.0))))
+        right = F32(0.5) * (F32(1.0) + np.cos(F32(np.pi) * (F32(2.0) * x / a - F32(2.0) / a
+                                                             + F32(1.0))))
+        w = np.where(x < a / F32(2.0), left, np.where(x <= F32(1.0) - a / F32(2.0), F32(1.0),
+                                                       right))
+    else:
+        raise ValueError(f"unknown DFT window {window!r}")
+    return w.astype(np.float64)
+
+
+def flux_spectrum(accs, dA):
+    """rfx/probes/probes.py:615-660: sum Re(E1 H2* - E2 H1*) dA (float64)."""
+    e1, e2, h1, h2 = accs
+    return np.real(np.sum((e1 * np.conj(h2) - e2 * np.conj(h1)) * np.asarray(dA, np.float64),
+                          axis=(-2, -1)))
 
 
 def extract_lumped_s11(v_dft, i_dft, z0=50.0):
@@ -781,12 +851,14 @@ def build_uniform(spec, n_steps):
     # cpml_axes = "xyz" even with zero layers (rfx/grid.py:86), so forward()
     # applies no outer-face PEC on either boundary kind.
     pec_axes = "xyz" if lane == "run" else ""
+    flux, areas = _flux_entries(spec, g, n_steps, False)
     cfg = StepConfig(shape=shape, dt=dt, spacing=g.spacing(), eps_r=eps_r,
                      sigma=sigma, mu_r=mu_r, cpml_n=layers,
                      cpml_axes="xyz" if layers > 0 else "",
                      cpml_profiles=profiles, cpml_face_d=face_d,
                      pec_axes=pec_axes, pec_mask=pec, sources=sources,
-                     probes=probes, dft_planes=planes, ports_vi=ports_vi)
+                     probes=probes, dft_planes=planes, ports_vi=ports_vi, flux=flux)
+    cfg.flux_area = areas
     return g, cfg
 
 
@@ -953,4 +1025,44 @@ def build_graded(spec, n_steps):
                      cpml_profiles=profiles, cpml_face_d=face_d, pec_axes="xyz",
                      pec_mask=pec, sources=sources, probes=probes, dft_planes=planes,
                      ports_vi=ports_vi, record_order="graded")
+    cfg.flux, cfg.flux_area = _flux_entries(spec, g, n_steps, True)
     return g, cfg
+
+
+def _flux_entries(spec, g, n_steps, graded):
+    """Flux monitors of a spec: (axis, coord, freqs|None, n_freqs, size|None,
+    center|None, window, alpha) -> StepConfig.flux entries and dA arrays
+    (rfx/runners/uniform.py:66-137; graded rfx/runners/nonuniform.py:924-958,
+    full plane only, per-cell tangential widths)."""
+    out, areas = [], []
+    for ent in spec.get("flux_monitors", []):
+        axis_name, coord, freqs, nf, size, center, window, alpha = ent
+        a = "xyz".index(axis_name)
+        pos = [0.0, 0.0, 0.0]
+        pos[a] = coord
+        index = g.position_to_index(tuple(pos))[a]
+        if freqs is None:
+            freqs = np.linspace(spec["freq_max"] / 10, spec["freq_max"], nf).astype(F32)
+        tang = [t for t in range(3) if t != a]
+        if size is None:
+            (lo1, hi1), (lo2, hi2) = (0, g.shape[tang[0]]), (0, g.shape[tang[1]])
+        else:
+            if graded:
+                raise NotImplementedError("finite flux regions on the graded lane")
+            b = []
+            dom = spec["domain"]
+            for q, t in enumerate(tang):
+                c = center[q] if center is not None else dom[t] / 2.0
+                lo = max(0, int(round(c / g.dx - size[q] / (2.0 * g.dx))) + g.pad)
+                hi = min(g.shape[t], int(round(c / g.dx + size[q] / (2.0 * g.dx))) + g.pad)
+                b.append((lo, hi))
+            (lo1, hi1), (lo2, hi2) = b
+        if graded:
+            w1, w2 = g.d32[tang[0]][lo1:hi1], g.d32[tang[1]][lo2:hi2]
+        else:
+            w1 = np.full(hi1 - lo1, F32(g.dx))
+            w2 = np.full(hi2 - lo2, F32(g.dx))
+        areas.append(w1.reshape(-1, 1) * w2.reshape(1, -1))
+        out.append((a, index, lo1, hi1, lo2, hi2, np.asarray(freqs, F32),
+                    dft_window_weights(n_steps, window, alpha)))
+    return out, areas

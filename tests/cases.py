@@ -198,3 +198,27 @@ def sweep_instance(rng, shape_cells, dx=1e-3, layers=4):
                 geometry=[(tuple(lo), tuple(hi), "slab")],
                 ports=[(src, "ez", 0.0, o.Pulse(f0, 0.5), True)],
                 probes=probes, lane="run")
+
+
+def flux_box(h, c=0.015, freqs=(8e9, 10e9, 12e9)):
+    """Six finite flux monitors forming a closed box of half-size h around
+    (c, c, c): (axis, coord, freqs, n, size, center, window, alpha)."""
+    out = []
+    for name in "xyz":
+        for sgn in (-1, 1):
+            out.append((name, c + sgn * h, np.array(freqs, np.float32), len(freqs),
+                        (2 * h, 2 * h), (c, c), "rect", 0.25))
+    return out
+
+
+def flux_case():
+    return dict(freq_max=20e9, domain=(0.03, 0.03, 0.03), dx=1e-3, boundary="cpml",
+                cpml_layers=8, ports=[((0.015, 0.015, 0.015), "ez", 0.0, o.Pulse(10e9, 0.5), True)],
+                probes=[((0.02, 0.015, 0.015), "ez")], lane="run",
+                flux_monitors=flux_box(0.005) + flux_box(0.009))
+
+
+def box_total(fluxes):
+    """Net outward flux of one box from its six monitor spectra (lo face
+    first, then hi face, per axis)."""
+    return sum((1 if q % 2 else -1) * f for q, f in enumerate(fluxes))
