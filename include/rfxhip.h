@@ -67,6 +67,17 @@ typedef struct {
 } rfxh_plane;
 
 typedef struct {
+  int32_t axis;        /* plane normal 0/1/2 */
+  int32_t index;       /* plane index along the normal */
+  int32_t lo1, n1;     /* region along the plane's first array dimension */
+  int32_t lo2, n2;     /* region along the second */
+  int32_t n_freq;
+  const double* freqs; /* host, n_freq Hz */
+  const float* weight; /* host [max_steps + 1]: DFT window weight per step value; NULL = 1 */
+  double* d_acc;       /* device complex128 [4][n_freq][n1][n2]: tangential E1, E2, H1, H2 */
+} rfxh_flux;
+
+typedef struct {
   int32_t nx, ny, nz;
   int32_t graded;          /* 0: uniform spacing; 1: graded (per-index inverse spacings) */
   const float* inv_e[3];   /* host, length n_axis: E-update inverse spacings
@@ -107,6 +118,13 @@ typedef struct {
   const rfxh_plane* planes;
   int32_t plane_t_offset;  /* t = (n + offset) dt: 1 uniform lane (rfx/simulation.py:1575),
                               0 graded lane (rfx/nonuniform.py:1458-1476) */
+  /* flux monitors (rfx/probes/probes.py:476-613, in-scan rfx/simulation.py:1590-1644,
+     rfx/nonuniform.py:1484-1522): tangential (E1, E2, H1, H2) with
+     (E1, E2, H1, H2) = (Ey, Ez, Hy, Hz) | (Ez, Ex, Hz, Hx) | (Ex, Ey, Hx, Hy) for
+     normal x | y | z; E of the plane at t = (n + offset) dt, H averaged over
+     the planes index-1 and index at t - dt/2; float64 sums */
+  int32_t n_flux;
+  const rfxh_flux* flux;
 } rfxh_desc;
 
 typedef struct rfxh_plan rfxh_plan;

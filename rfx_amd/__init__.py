@@ -6,11 +6,13 @@ differentiable ``Simulation.forward(eps_override=...)``.  The time loop runs
 as hand-written HIP kernels (rfx_amd/csrc/rfxhip.hip) behind the C ABI in
 include/rfxhip.h; torch provides device memory, streams and autograd.
 """
-from .api import DFTPlaneProbe, FDTDState, ForwardResult, Result, Simulation, extract_lumped_s11
+from .api import (DFTPlaneProbe, FDTDState, FluxMonitor, ForwardResult, Result, Simulation,
+                  extract_lumped_s11, flux_spectrum)
 from .geometry import Box
 from .grid import Grid
 from .sources import GaussianPulse, LumpedPort
 
-__all__ = ["Simulation", "Result", "ForwardResult", "FDTDState", "DFTPlaneProbe", "Box",
+__all__ = ["Simulation", "Result", "ForwardResult", "FDTDState", "DFTPlaneProbe", "FluxMonitor",
+           "flux_spectrum", "Box",
            "GaussianPulse", "LumpedPort", "Grid", "extract_lumped_s11"]
 __version__ = "0.1.0"

@@ -38,6 +38,12 @@ class Plane(C.Structure):
                 ("d_acc", _P)]
 
 
+class Flux(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("index", C.c_int32), ("lo1", C.c_int32), ("n1", C.c_int32),
+                ("lo2", C.c_int32), ("n2", C.c_int32), ("n_freq", C.c_int32),
+                ("freqs", C.POINTER(C.c_double)), ("weight", _FP), ("d_acc", _P)]
+
+
 class Desc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
@@ -58,6 +64,7 @@ class Desc(C.Structure):
         ("max_steps", C.c_int64),
         ("n_plane", C.c_int32), ("planes", C.POINTER(Plane)),
         ("plane_t_offset", C.c_int32),
+        ("n_flux", C.c_int32), ("flux", C.POINTER(Flux)),
     ]
 
 

@@ -62,6 +62,66 @@ class _DFTPlaneEntry:
     n_freqs: int
 
 
+@dataclass
+class _FluxMonitorEntry:
+    name: str
+    axis: str
+    coordinate: float
+    freqs: np.ndarray | None
+    n_freqs: int
+    size: tuple | None
+    center: tuple | None
+    window: str
+    alpha: float
+
+
+FLUX_COMPONENTS = {0: ("ey", "ez", "hy", "hz"), 1: ("ez", "ex", "hz", "hx"),
+                   2: ("ex", "ey", "hx", "hy")}
+
+
+@dataclass
+class FluxMonitor:
+    """Frequency-domain tangential E and H on a plane region
+    (rfx/probes/probes.py:476-503); ``flux_spectrum`` integrates the flux."""
+    e1_dft: np.ndarray    # complex128 (n_freqs, n1, n2)
+    e2_dft: np.ndarray
+    h1_dft: np.ndarray
+    h2_dft: np.ndarray
+    freqs: np.ndarray
+    axis: int
+    index: int
+    dA: np.ndarray        # float32 (n1, n2) cell areas
+    lo1: int
+    hi1: int
+    lo2: int
+    hi2: int
+
+
+def flux_spectrum(mon: FluxMonitor) -> np.ndarray:
+    """Integral of Re(E x H*) . n dA per frequency, in float64
+    (rfx/probes/probes.py:615-660); positive = power along +axis."""
+    integrand = mon.e1_dft * np.conj(mon.h2_dft) - mon.e2_dft * np.conj(mon.h1_dft)
+    return np.real(np.sum(integrand * np.asarray(mon.dA, np.float64), axis=(-2, -1)))
+
+
+def dft_window_weights(total_steps: int, window: str = "rect", alpha: float = 0.25):
+    """Streaming DFT window weight per step value 0..total_steps, float32
+    (rfx/core/dft_utils.py:8-49); None for the rectangular window."""
+    if window not in ("rect", "hann", "tukey"):
+        raise ValueError(f"dft_window must be 'rect', 'hann', or 'tukey', got {window!r}")
+    if total_steps <= 1 or window == "rect" or (window == "tukey" and alpha <= 0.0):
+        return None
+    x = np.arange(total_steps + 1, dtype=F32) / np.maximum(F32(total_steps - 1), F32(1.0))
+    tp = F32(2.0 * np.pi)
+    if window == "hann" or alpha >= 1.0:
+        return (F32(0.5) * (F32(1.0) - np.cos(tp * x))).astype(F32)
+    a = F32(alpha)
+    left = F32(0.5) * (F32(1.0) + np.cos(F32(np.pi) * (F32(2.0) * x / a - F32(1.0))))
+    right = F32(0.5) * (F32(1.0) + np.cos(F32(np.pi) * (F32(2.0) * x / a - F32(2.0) / a + F32(1.0))))
+    return np.where(x < a / F32(2.0), left,
+                    np.where(x <= F32(1.0) - a / F32(2.0), F32(1.0), right)).astype(F32)
+
+
 class FDTDState(NamedTuple):
     ex: torch.Tensor
     ey: torch.Tensor
@@ -92,6 +152,7 @@ class Result:
     freqs: np.ndarray | None = None
     grid: object = None
     dt: float = 0.0
+    flux_monitors: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -102,6 +163,7 @@ class ForwardResult:
     s_params: np.ndarray | None = None
     freqs: np.ndarray | None = None
     dft_planes: dict | None = None
+    flux_monitors: dict | None = None
 
 
 def _cell_cb_f32(dt, eps_r, sigma):
@@ -160,6 +222,7 @@ class Simulation:
         self._ports: list[_PortEntry] = []
         self._probes: list[_ProbeEntry] = []
         self._dft_planes: list[_DFTPlaneEntry] = []
+        self._flux_monitors: list[_FluxMonitorEntry] = []
         self._device = device
 
     # ---- builder --------------------------------------------------------------
@@ -229,6 +292,77 @@ class Simulation:
                                                None if freqs is None else np.asarray(freqs),
                                                int(n_freqs)))
         return self
+
+    def add_flux_monitor(self, *, axis: str, coordinate: float, freqs=None, n_freqs: int = 50,
+                         size=None, center=None, name: str | None = None,
+                         dft_window: str = "rect", dft_window_alpha: float = 0.25):
+        """Poynting flux monitor on a plane (rfx/api/__init__.py:2577-2645):
+        ``size`` / ``center`` restrict it to a region of the two tangential
+        axes (uniform lane only, as in the reference's graded runner)."""
+        if axis not in ("x", "y", "z"):
+            raise ValueError(f"axis must be 'x', 'y', or 'z', got {axis!r}")
+        a = "xyz".index(axis)
+        if coordinate < 0 or coordinate > self._domain[a]:
+            raise ValueError(f"coordinate {coordinate} m is outside the {axis}-domain "
+                             f"[0, {self._domain[a]}]")
+        if dft_window not in ("rect", "hann", "tukey"):
+            raise ValueError(f"dft_window must be 'rect', 'hann', or 'tukey', got {dft_window!r}")
+        if size is not None and self._dz_profile is not None:
+            raise NotImplementedError("finite flux regions (size=...) are not supported on the "
+                                      "graded mesh; use a full-plane monitor")
+        if name is None:
+            name = f"flux_{axis}_{len(self._flux_monitors)}"
+        self._flux_monitors.append(_FluxMonitorEntry(
+            name, axis, float(coordinate), None if freqs is None else np.asarray(freqs),
+            int(n_freqs), None if size is None else tuple(size),
+            None if center is None else tuple(center), dft_window, float(dft_window_alpha)))
+        return self
+
+    def _flux_setup(self, grid, n_steps):
+        """(Problem.fluxes entries, dA arrays) for the registered monitors
+        (rfx/runners/uniform.py:66-137; rfx/runners/nonuniform.py:924-958)."""
+        graded = self._dz_profile is not None
+        out, areas = [], []
+        for e in self._flux_monitors:
+            a = "xyz".index(e.axis)
+            pos = [0.0, 0.0, 0.0]
+            pos[a] = e.coordinate
+            index = grid.position_to_index(tuple(pos))[a]
+            freqs = e.freqs if e.freqs is not None else np.linspace(
+                self._freq_max / 10, self._freq_max, e.n_freqs).astype(F32)
+            tang = [t for t in range(3) if t != a]
+            if e.size is None:
+                (lo1, hi1), (lo2, hi2) = (0, grid.shape[tang[0]]), (0, grid.shape[tang[1]])
+            else:
+                b = []
+                for q, t in enumerate(tang):
+                    c = e.center[q] if e.center is not None else self._domain[t] / 2.0
+                    pad = grid.axis_pads[t]
+                    lo = max(0, int(round(c / grid.dx - e.size[q] / (2.0 * grid.dx))) + pad)
+                    hi = min(grid.shape[t], int(round(c / grid.dx + e.size[q] / (2.0 * grid.dx)))
+                             + pad)
+                    b.append((lo, hi))
+                (lo1, hi1), (lo2, hi2) = b
+            if hi1 <= lo1 or hi2 <= lo2:
+                raise ValueError(f"flux monitor {e.name!r}: empty region")
+            if graded:
+                w1, w2 = grid.d32[tang[0]][lo1:hi1], grid.d32[tang[1]][lo2:hi2]
+            else:
+                w1 = np.full(hi1 - lo1, F32(grid.dx))
+                w2 = np.full(hi2 - lo2, F32(grid.dx))
+            areas.append(w1.reshape(-1, 1) * w2.reshape(1, -1))
+            out.append((a, index, lo1, hi1, lo2, hi2, np.asarray(freqs, np.float64),
+                        dft_window_weights(n_steps, e.window, e.alpha)))
+        return out, areas
+
+    def _flux_results(self, prob, dr, areas):
+        res = {}
+        for e, (a, idx, lo1, hi1, lo2, hi2, fr, _), acc, dA in zip(
+                self._flux_monitors, prob.fluxes, dr.flux_acc, areas):
+            x = acc.cpu().numpy()
+            res[e.name] = FluxMonitor(x[0], x[1], x[2], x[3], np.asarray(fr), a, int(idx), dA,
+                                      lo1, hi1, lo2, hi2)
+        return res
 
     # ---- set-up ---------------------------------------------------------------
     def _build_grid(self):
@@ -326,10 +460,12 @@ class Simulation:
         # cpml_axes="xyz" even with zero layers, so forward() applies none.
         # The graded lane always applies it (nonuniform.py:1319).
         pec_axes = "xyz" if (graded or lane == "run") else ""
+        fluxes, areas = self._flux_setup(grid, n_steps)
         prob = Problem(shape=shape, dt=dt, graded=graded, inv_e=inv_e, inv_h=inv_h, mat=mat,
                        faces=faces, pec_axes=pec_axes, sources=sources, probes=probes,
-                       ports=[p[:4] for p in ports], planes=planes,
+                       ports=[p[:4] for p in ports], planes=planes, fluxes=fluxes,
                        record_order="graded" if graded else "uniform")
+        prob.flux_areas = areas
         return grid, prob, ports
 
     def _port_cells(self, grid, pe):
@@ -437,7 +573,8 @@ class Simulation:
         st = FDTDState(*[f for f in dr.fields], step=int(n_steps))
         _warn_if_nonfinite(ts, "run")
         return Result(time_series=ts, state=st, dft_planes=planes, s_params=s_params,
-                      freqs=sf if s_params is not None else None, grid=grid, dt=grid.dt)
+                      freqs=sf if s_params is not None else None, grid=grid, dt=grid.dt,
+                      flux_monitors=self._flux_results(prob, dr, prob.flux_areas))
 
     def forward(self, *, eps_override=None, sigma_override=None, mu_r_override=None,
                 n_steps: int | None = None, num_periods: float = 20.0, checkpoint: bool = True,
@@ -484,8 +621,9 @@ class Simulation:
             planes = {e.name: DFTPlaneProbe(a, int(idx), comp, np.asarray(fr), acc.cpu().numpy())
                       for e, (a, idx, comp, fr), acc in zip(self._dft_planes, prob.planes,
                                                             dr.plane_acc)}
+        fluxes = self._flux_results(prob, dr, prob.flux_areas) if self._flux_monitors else None
         return ForwardResult(time_series=ts, grid=grid, s_params=s_params, freqs=freqs,
-                             dft_planes=planes)
+                             dft_planes=planes, flux_monitors=fluxes)
 
 
 def _to_np(x):

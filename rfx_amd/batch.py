@@ -70,7 +70,8 @@ class Batch:
                                                             r.plane_acc)}
             _warn_if_nonfinite(ts, "run_batch")
             out.append(Result(time_series=ts, state=FDTDState(*r.fields, step=ns),
-                              dft_planes=planes, grid=g, dt=g.dt))
+                              dft_planes=planes, grid=g, dt=g.dt,
+                              flux_monitors=sim._flux_results(prob, r, prob.flux_areas)))
         return out
 
     def close(self):

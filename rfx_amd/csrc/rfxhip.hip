@@ -123,6 +123,18 @@ struct PlaneParams {
   int ny, nz;
 };
 
+struct FluxParams {
+  const float* f[4];     // E1, E2, H1, H2 field arrays (current set)
+  double2* acc;          // [4][nf][n1][n2]
+  const double2* twe;    // [max_steps][nf] E twiddles (dt and window folded in)
+  const double2* twh;    // [max_steps][nf] H twiddles (half a step earlier)
+  long long* step;
+  long long max_steps;
+  int nf, axis, index, index_m1;
+  int lo1, n1, lo2, n2;
+  int ny, nz;
+};
+
 // Loads/stores at a 32-bit byte offset from a (wave-uniform) base pointer: the
 // base stays in SGPRs and the address is one VGPR offset, no 64-bit math.
 // Every array indexed this way is smaller than 4 GiB (checked at plan
@@ -733,6 +745,46 @@ __global__ __launch_bounds__(256) void k_dft_plane(PlaneParams q) {
 }
 
 // ---------------------------------------------------------------------------
+// Flux monitor: tangential E at the plane, tangential H averaged over the
+// planes index-1 and index, accumulated with their own twiddles (float64).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_flux(FluxParams q) {
+  const long long npts = (long long)q.n1 * q.n2;
+  const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= npts) return;
+  const long long n = *q.step;
+  if (n < 0 || n >= q.max_steps) return;
+  const int u = (int)(pidx / q.n2) + q.lo1, v = (int)(pidx % q.n2) + q.lo2;
+  int ii[3], im[3];
+  const int a1 = q.axis == 0 ? 1 : 0, a2 = q.axis == 2 ? 1 : 2;
+  ii[q.axis] = q.index;
+  im[q.axis] = q.index_m1;
+  ii[a1] = im[a1] = u;
+  ii[a2] = im[a2] = v;
+  const long long c = ((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2];
+  const long long cm = ((long long)im[0] * q.ny + im[1]) * q.nz + im[2];
+  double x[4];
+  x[0] = (double)q.f[0][c];
+  x[1] = (double)q.f[1][c];
+  x[2] = (double)((q.f[2][cm] + q.f[2][c]) * 0.5f);
+  x[3] = (double)((q.f[3][cm] + q.f[3][c]) * 0.5f);
+  const double2* te = q.twe + n * q.nf;
+  const double2* th = q.twh + n * q.nf;
+  for (int f = 0; f < q.nf; ++f) {
+    const double2 we = te[f], wh = th[f];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const double2 w = t < 2 ? we : wh;
+      double2* a = q.acc + ((long long)t * q.nf + f) * npts + pidx;
+      double2 cur = *a;
+      cur.x += w.x * x[t];
+      cur.y += w.y * x[t];
+      *a = cur;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Reverse mode.  The forward step is linear in the state (E, H, CPML memory)
 // for fixed materials, so its adjoint is the transpose of each sub-step run
 // in reverse order.  A reverse step is two gather kernels:
@@ -1145,6 +1197,8 @@ struct rfxh_plan {
   int cur = 0;                  // which set holds the current state
   std::vector<PlaneParams> planes;
   std::vector<int> plane_comp;
+  std::vector<FluxParams> fluxes;
+  std::vector<int> flux_comp;     // 4 field components per monitor
   std::vector<void*> owned;     // device allocations owned by the plan
   long long max_steps = 0;
   hipStream_t cap_stream = nullptr;
@@ -1203,6 +1257,22 @@ long long march_blocks(const KParams& kp, int want_seg, MarchIO* io, int* nseg_o
   io->seg_len = seg_len;
   *nseg_out = nseg;
   return tiles * nseg;
+}
+
+// DFT-plane and flux-monitor kernels on field set `set` (after the step).
+void launch_accumulators(rfxh_plan* pl, hipStream_t s, int set) {
+  for (size_t q = 0; q < pl->planes.size(); ++q) {
+    PlaneParams pp = pl->planes[q];
+    pp.fld = pl->set[set][pl->plane_comp[q]];
+    const long long npts = (long long)pp.n1 * pp.n2;
+    hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
+  }
+  for (size_t q = 0; q < pl->fluxes.size(); ++q) {
+    FluxParams fp = pl->fluxes[q];
+    for (int t = 0; t < 4; ++t) fp.f[t] = pl->set[set][pl->flux_comp[4 * q + t]];
+    const long long npts = (long long)fp.n1 * fp.n2;
+    hipLaunchKernelGGL(k_flux, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, fp);
+  }
 }
 
 // One time step starting from field set `from` (fused: writes set 1-from).
@@ -1264,12 +1334,7 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
       hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, kp, ps);
   }
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
-  for (size_t q = 0; q < pl->planes.size(); ++q) {
-    PlaneParams pp = pl->planes[q];
-    pp.fld = pl->set[to][pl->plane_comp[q]];
-    const long long npts = (long long)pp.n1 * pp.n2;
-    hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
-  }
+  launch_accumulators(pl, s, to);
   if (ev) HIPCHK(hipEventRecord(ev[3], s));
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
   HIPCHK(hipGetLastError());
@@ -1482,6 +1547,50 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pp.tw = dtw;
     pl->planes.push_back(pp);
     pl->plane_comp.push_back(pd.comp);
+  }
+  static const int kFluxComp[3][4] = {{1, 2, 4, 5}, {2, 0, 5, 3}, {0, 1, 3, 4}};
+  for (int q = 0; q < d->n_flux; ++q) {
+    const rfxh_flux& fd = d->flux[q];
+    if (fd.axis < 0 || fd.axis > 2 || fd.index < 0 || fd.index >= N[fd.axis] || fd.n_freq < 1 ||
+        !fd.d_acc || !fd.freqs || fd.n1 < 1 || fd.n2 < 1 || fd.lo1 < 0 || fd.lo2 < 0)
+      return bail(fail(RFXH_EINVAL, "bad flux monitor"));
+    const int t1 = fd.axis == 0 ? 1 : 0, t2 = fd.axis == 2 ? 1 : 2;
+    if (fd.lo1 + fd.n1 > N[t1] || fd.lo2 + fd.n2 > N[t2])
+      return bail(fail(RFXH_EINVAL, "flux region outside the plane"));
+    FluxParams fp;
+    std::memset(&fp, 0, sizeof(fp));
+    fp.acc = reinterpret_cast<double2*>(fd.d_acc);
+    fp.step = kp.step;
+    fp.max_steps = d->max_steps;
+    fp.nf = fd.n_freq;
+    fp.axis = fd.axis;
+    fp.index = fd.index;
+    fp.index_m1 = fd.index > 0 ? fd.index - 1 : 0;
+    fp.lo1 = fd.lo1;
+    fp.n1 = fd.n1;
+    fp.lo2 = fd.lo2;
+    fp.n2 = fd.n2;
+    fp.ny = d->ny;
+    fp.nz = d->nz;
+    std::vector<double2> te((size_t)d->max_steps * fd.n_freq), th(te.size());
+    for (long long n = 0; n < d->max_steps; ++n) {
+      const long long stp = n + d->plane_t_offset;
+      const double w = fd.weight ? (double)fd.weight[stp] : 1.0;
+      const double t = (double)stp * d->dt;
+      for (int f = 0; f < fd.n_freq; ++f) {
+        const double ae = -2.0 * M_PI * fd.freqs[f] * t;
+        const double ah = -2.0 * M_PI * fd.freqs[f] * (t - d->dt * 0.5);
+        te[n * fd.n_freq + f] = make_double2(std::cos(ae) * d->dt * w, std::sin(ae) * d->dt * w);
+        th[n * fd.n_freq + f] = make_double2(std::cos(ah) * d->dt * w, std::sin(ah) * d->dt * w);
+      }
+    }
+    double2 *dte, *dth;
+    if ((rc = dev_copy(pl, te.data(), te.size(), &dte))) return bail(rc);
+    if ((rc = dev_copy(pl, th.data(), th.size(), &dth))) return bail(rc);
+    fp.twe = dte;
+    fp.twh = dth;
+    pl->fluxes.push_back(fp);
+    for (int t = 0; t < 4; ++t) pl->flux_comp.push_back(kFluxComp[fd.axis][t]);
   }
   // workgroup order (RFXH_ORDER = flat (default) | bands, for comparisons;
   // rfxh_set_variant changes it later)
@@ -1700,12 +1809,7 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
       hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
       hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, kp, ps);
     }
-    for (size_t q = 0; q < pl->planes.size(); ++q) {
-      PlaneParams pp = pl->planes[q];
-      pp.fld = pl->set[pl->cur][pl->plane_comp[q]];
-      const long long npts = (long long)pp.n1 * pp.n2;
-      hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
-    }
+    launch_accumulators(pl, s, pl->cur);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
   }
   HIPCHK(hipGetLastError());
@@ -1821,12 +1925,7 @@ int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
   hipLaunchKernelGGL(k_batch_pass<1>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
   for (int q = 0; q < n_act; ++q) {
     rfxh_plan* pl = b->plans[b->order[q]];
-    for (size_t r = 0; r < pl->planes.size(); ++r) {
-      PlaneParams pp = pl->planes[r];
-      pp.fld = pl->set[pl->cur][pl->plane_comp[r]];
-      const long long npts = (long long)pp.n1 * pp.n2;
-      hipLaunchKernelGGL(k_dft_plane, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
-    }
+    launch_accumulators(pl, s, pl->cur);
   }
   hipLaunchKernelGGL(k_batch_advance, dim3((unsigned)((n_act + 63) / 64)), dim3(64), 0, s, b->d_ents,
                      n_act);

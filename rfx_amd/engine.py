@@ -34,6 +34,8 @@ class Problem:
     probes: list = field(default_factory=list)    # (idx, comp)
     ports: list = field(default_factory=list)     # (idx, comp, d_par, (lx, ly, lz))
     planes: list = field(default_factory=list)    # (axis, index, comp, freqs f64)
+    # flux monitors: (axis, index, lo1, hi1, lo2, hi2, freqs f64, weights f32 [n+1] | None)
+    fluxes: list = field(default_factory=list)
     record_order: str = "uniform"                 # "uniform" | "graded"
 
     @property
@@ -120,6 +122,26 @@ class DeviceRun:
             pc.freqs = fr.ctypes.data_as(C.POINTER(C.c_double))
             pc.d_acc = acc.data_ptr()
         self._planes_c = planes_c
+        self.flux_acc = []
+        flux_c = (_lib.Flux * max(len(prob.fluxes), 1))()
+        for q, (axis, index, lo1, hi1, lo2, hi2, freqs, wts) in enumerate(prob.fluxes):
+            fr = np.ascontiguousarray(np.asarray(freqs, np.float64))
+            self._plane_freqs.append(fr)
+            acc = torch.zeros((4, len(fr), hi1 - lo1, hi2 - lo2), dtype=torch.complex128, device=dev)
+            self.flux_acc.append(acc)
+            fc = flux_c[q]
+            fc.axis, fc.index = axis, int(index)
+            fc.lo1, fc.n1, fc.lo2, fc.n2 = lo1, hi1 - lo1, lo2, hi2 - lo2
+            fc.n_freq = len(fr)
+            fc.freqs = fr.ctypes.data_as(C.POINTER(C.c_double))
+            if wts is not None:
+                w = np.ascontiguousarray(np.asarray(wts, np.float32))
+                if len(w) < T + 1:
+                    raise ValueError("flux window table shorter than n_steps + 1")
+                self._plane_freqs.append(w)
+                fc.weight = _f32p(w)
+            fc.d_acc = acc.data_ptr()
+        self._flux_c = flux_c
 
         # ---- descriptor -----------------------------------------------------
         d = _lib.Desc()
@@ -167,6 +189,8 @@ class DeviceRun:
         d.n_plane = len(prob.planes)
         d.planes = planes_c
         d.plane_t_offset = prob.plane_t_offset
+        d.n_flux = len(prob.fluxes)
+        d.flux = flux_c
         self.desc = d
         self.plan = C.c_void_p()
         with torch.cuda.device(dev):
@@ -211,6 +235,8 @@ class DeviceRun:
         self.probe_out.zero_()
         self.port_out.zero_()
         for a in self.plane_acc:
+            a.zero_()
+        for a in self.flux_acc:
             a.zero_()
 
     def set_timing(self, enable: bool):

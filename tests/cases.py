@@ -110,6 +110,9 @@ def to_sim(spec, device=None):
     for axis, coord, comp, freqs, nf in spec.get("dft_planes", []):
         sim.add_dft_plane_probe(axis=axis, coordinate=coord, component=comp, freqs=freqs,
                                 n_freqs=nf)
+    for axis, coord, freqs, nf, size, center, window, alpha in spec.get("flux_monitors", []):
+        sim.add_flux_monitor(axis=axis, coordinate=coord, freqs=freqs, n_freqs=nf, size=size,
+                             center=center, dft_window=window, dft_window_alpha=alpha)
     return sim
 
 

@@ -150,3 +150,33 @@ def test_graded_setup_matches_oracle(name):
         np.testing.assert_array_equal(c, oc)
         assert inv_d == np.float32(1.0 / cfg.cpml_face_d[ax][side]) or abs(
             inv_d * cfg.cpml_face_d[ax][side] - 1) < 1e-7
+
+
+@pytest.mark.parametrize("graded", [False, True])
+def test_flux_setup_matches_oracle(graded):
+    """Flux monitors: plane index, tangential region, cell areas and window
+    weights equal the oracle's set-up (uniform: finite boxes and a Tukey
+    window; graded: full planes with per-cell widths)."""
+    if graded:
+        spec = cases.mixed_graded_case()
+        spec["flux_monitors"] = [("z", 0.006, None, 3, None, None, "hann", 0.25),
+                                 ("x", 0.004, None, 2, None, None, "rect", 0.25)]
+        build = o.build_graded
+    else:
+        spec = cases.flux_case()
+        spec["flux_monitors"] = spec["flux_monitors"] + [("y", 0.012, None, 4, None, None,
+                                                          "tukey", 0.3)]
+        build = o.build_uniform
+    n = 64
+    sim = cases.to_sim(spec)
+    grid, prob, _ = sim._build_problem(n, "run")
+    g, cfg = build(spec, n)
+    assert len(prob.fluxes) == len(cfg.flux) == len(prob.flux_areas)
+    for mine, ref, a1, a2 in zip(prob.fluxes, cfg.flux, prob.flux_areas, cfg.flux_area):
+        assert tuple(mine[:6]) == tuple(ref[:6])
+        np.testing.assert_array_equal(np.asarray(mine[6], np.float32), ref[6])
+        np.testing.assert_array_equal(a1, a2)
+        if ref[7] is None:
+            assert mine[7] is None
+        else:
+            np.testing.assert_allclose(mine[7], ref[7], rtol=0, atol=1e-7)
