@@ -192,13 +192,29 @@ def batch_sweep():
         r.run(0, n)
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
+    problems = [r.prob for r in b.runs]
     b.close()
+    torch.cuda.empty_cache()
+    # instances 0, 2, 4, ... also need d(sum of squared probe samples)/d eps_r
+    # over the whole grid: forward with checkpoints + reverse sweep each
+    from rfx_amd.adjoint import AdjointRunner
+    t0 = time.perf_counter()
+    n_adj = 0
+    for q in range(0, len(problems), 2):
+        runner = AdjointRunner(problems[q], steps[q], "cuda:0")
+        ts = runner.forward()
+        grad = runner.backward(2.0 * ts)
+        n_adj += 1
+        del runner, grad
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
     torch.cuda.empty_cache()
     return {"workload": f"{len(specs)} instances, {min(steps)}-{max(steps)} steps, "
                         f"{cell_steps / 1e9:.1f} G cell-steps",
             "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
             "batched_Mcell_steps_per_s": round(cell_steps / t_batch / 1e6, 1),
-            "sequential_Mcell_steps_per_s": round(cell_steps / t_seq / 1e6, 1)}
+            "sequential_Mcell_steps_per_s": round(cell_steps / t_seq / 1e6, 1),
+            "adjoint_instances": n_adj, "adjoint_s_incl_setup": round(t_adj, 3)}
 
 
 def build_sim(spec):

@@ -84,13 +84,17 @@ def _ladder_J(torch, sim, eps, n, dt, region=None, segments=None):
     return (acc.real ** 2 + acc.imag ** 2), res
 
 
-def test_ladder_gradient_vs_fixture_and_fd(torch_mod):
-    """Slab-eps gradient on the lambda/20 ladder rung vs the reference's
-    jax.grad and the float64 oracle's central difference (<= 1e-3)."""
+@pytest.mark.parametrize("rung", [0, 1, 2, 3])
+def test_ladder_gradient_vs_fixture_and_fd(torch_mod, rung):
+    """Slab-eps gradient on every rung of the reference's gradient ladder
+    (lambda/20 .. lambda/60) vs its jax.grad value and the float64 oracle's
+    central difference (<= 1e-3)."""
     torch = torch_mod
-    _say("ladder rung lambda/20 gradient")
-    lad = cases.ladder()["rungs"][0]
-    spec, n, m1d = cases.ladder_case(20, 4.0)
+    lad = cases.ladder()["rungs"][rung]
+    N = lad["lambda_over_dx"]
+    _say(f"ladder rung lambda/{N} gradient")
+    spec, n, m1d = cases.ladder_case(N, 4.0)
+    assert n == lad["n_steps"]
     sim = cases.to_sim(spec)
     eps = torch.tensor(spec["eps_override"], requires_grad=True)
     J, res = _ladder_J(torch, sim, eps, n, sim._build_grid().dt)
@@ -101,7 +105,7 @@ def test_ladder_gradient_vs_fixture_and_fd(torch_mod):
     h = 1e-3
     vals = []
     for e in (4.0 + h, 4.0 - h):
-        sp, _, _ = cases.ladder_case(20, e)
+        sp, _, _ = cases.ladder_case(N, e)
         gg, cfg = o.build_uniform(sp, n)
         out = cstep.run_c(cfg, n, want_fields=False)
         vals.append(cases.ladder_objective(out.time_series[:, 0], gg.dt))
