@@ -27,11 +27,15 @@ def R():
 
 
 def _compare(res, out, cfg):
+    """Each accumulator to 1e-4 of itself plus 1e-6 of the monitor's largest
+    one: a component that vanishes by symmetry (Hz on an x-normal plane
+    around an Ez source) holds only rounding noise on both sides."""
     for q, (name, mon) in enumerate(res.flux_monitors.items()):
         ref = out.flux[q]
+        scale = max(np.linalg.norm(r) for r in ref)
         for t, acc in enumerate((mon.e1_dft, mon.e2_dft, mon.h1_dft, mon.h2_dft)):
-            e = cases.rel_l2(acc.view(np.float64), ref[t].view(np.float64))
-            assert e <= 1e-4, (name, t, e)
+            err = np.linalg.norm(acc - ref[t])
+            assert err <= 1e-4 * np.linalg.norm(ref[t]) + 1e-6 * scale, (name, t, err, scale)
         np.testing.assert_array_equal(mon.dA, cfg.flux_area[q])
 
 
