@@ -158,14 +158,16 @@ int rfxh_set_state_set(rfxh_plan* plan, int which);
 int rfxh_set_timing(rfxh_plan* plan, int enable);
 int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* count);
 
-/* Kernel variant of a plan: fused != 0 selects the single-launch step (needs
- * the second field set), 0 the two-pass step; order 1 = flat workgroup
- * order, 0 = XCD bands.  Variants compute the same step; they differ in
- * speed only.  The project's autotuner (rfx_amd/autotune.py) times them on
- * the actual problem.  No counterpart in the reference (its XLA lowering
- * has no kernel choice). */
-int rfxh_set_variant(rfxh_plan* plan, int fused, int order);
-int rfxh_get_variant(rfxh_plan* plan, int* fused, int* order);
+/* Kernel variant of a plan.  kernel 0: two-pass step (param = workgroup
+ * order, 1 flat, 0 XCD bands); 1: fused single-launch step (same param);
+ * 2 / 3: marching step with 6- / 14-row tiles (param = number of x
+ * segments, 0 = automatic).  Kernels 1-3 need the second field set.
+ * Variants compute the same step; they differ in speed only.  The
+ * project's autotuner (rfx_amd/autotune.py) times them on the actual
+ * problem.  No counterpart in the reference (its XLA lowering has no
+ * kernel choice). */
+int rfxh_set_variant(rfxh_plan* plan, int kernel, int param);
+int rfxh_get_variant(rfxh_plan* plan, int* kernel, int* param);
 
 /* ---- reverse mode (gradient of a probe-series objective w.r.t. eps_r) ----
  * Replaces jax.grad through the scan (rfx/api/_execute.py:2447-2858 with

@@ -2,8 +2,8 @@
 
 The plan's step can run as the two-pass kernels (H pass, E pass, in place)
 or, with a second field set, as one fused kernel (flat or XCD-band
-workgroup order) or as the marching kernel (tiles of the y-z plane walking
-along x, in a chosen number of x segments) (include/rfxhip.h
+workgroup order) or as the marching kernel (6- or 14-row tiles of the y-z
+plane walking along x, in a chosen number of x segments) (include/rfxhip.h
 ``rfxh_set_variant``).  All
 variants compute the same step.  Which is fastest depends on the grid (CPML
 share, plane size, cache fit), so the tuner runs a few real steps of the
@@ -38,22 +38,30 @@ def variants(dr):
     number of x segments (0 = automatic)."""
     out = [(0, 1), (0, 0)]
     if len(dr.sets) == 2:
-        out += [(1, 1), (1, 0), (2, 0)]
+        out += [(1, 1), (1, 0)]
         nx = dr.prob.shape[0]
-        out += [(2, s) for s in (2, 4, 8, 16) if s <= max(1, nx // 4)]
+        for kern in (2, 3):
+            out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32) if s <= max(1, nx // 4)]
     return out
+
+
+MARCH_ROWS = {2: 6, 3: 14}
 
 
 def variant_name(v) -> str:
     k, prm = v
-    if k == 2:
-        return "march" if prm == 0 else f"march-{prm}"
+    if k in MARCH_ROWS:
+        base = f"march{MARCH_ROWS[k]}"
+        return base if prm == 0 else f"{base}-{prm}"
     return f"{'fused' if k else 'two-pass'}-{'flat' if prm else 'bands'}"
 
 
 def parse_variant(name: str):
     if name.startswith("march"):
-        return (2, int(name.split("-")[1]) if "-" in name else 0)
+        head, _, seg = name.partition("-")
+        rows = int(head[5:])
+        kern = {r: k for k, r in MARCH_ROWS.items()}[rows]
+        return (kern, int(seg) if seg else 0)
     return (1 if name.startswith("fused") else 0, 1 if name.endswith("flat") else 0)
 
 

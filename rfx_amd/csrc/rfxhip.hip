@@ -498,20 +498,20 @@ __global__ __launch_bounds__(kWG) void k_step_fused(KParams p, FusedIO io, PsiIO
 
 // ---------------------------------------------------------------------------
 // Marching fused step: read set A (E^n, H^{n-1/2}, psi), write set B.
-// A workgroup owns a tile of kMJ x kMK cells of the (y, z) plane and walks
-// along x through its segment of planes.  Wave 0 and lane 0 of every wave
-// are halo threads: they compute H at the tile's j-1 row and k-1 column
-// (never stored), so the E update finds every backward H neighbour in LDS
-// (double-buffered, one barrier per plane).  E and H of the cell's own
-// column stay in registers from one plane to the next: E^n of plane i+1 is
-// loaded once and reused as the cell's own E^n at the next plane, and
-// H^{n+1/2} of plane i is the x-backward neighbour at plane i+1.  Each field
-// value is read from HBM once per step; neighbours along y and z come
-// through the caches from the same workgroup.
+// A workgroup owns a tile of MJ rows (y) x 64 columns (z) of the y-z plane
+// and walks along x through its segment of planes.  Waves 1..MJ own the
+// tile's rows; wave 0 computes H on the row j0-1 and lanes 0..MJ-1 of the
+// last wave compute H on the column k0-1 (halo values, never stored), so the
+// E update finds every backward H neighbour in LDS (double-buffered, one
+// barrier per plane).  Each thread keeps its own column's E of the next
+// plane and H of the previous plane in registers, so x-neighbours are never
+// re-read, and the inputs of plane i+1 are loaded while plane i is being
+// computed.  Tiles are 64 cells wide in z, so every row of every field is
+// read and written in whole cache lines.
 // ---------------------------------------------------------------------------
-constexpr int kMJ = 8;               // interior rows (y) per tile
-constexpr int kMK = kBX - 1;         // interior columns (z) per tile
-constexpr int kMW = kMJ + 1;         // waves per workgroup (incl. the halo row)
+// Tile heights (interior rows): 6 (8 waves, two workgroups per CU at the
+// kernel's register use) or 14 (16 waves, one 1024-thread workgroup).
+constexpr int kMarchRows[2] = {6, 14};
 
 struct MarchIO {
   const float* eo[3];
@@ -521,48 +521,81 @@ struct MarchIO {
   int ntk, ntj, seg_len;
 };
 
-template <bool GRADED>
-__global__ __launch_bounds__(kBX * kMW) void k_step_march(KParams p, MarchIO io, PsiIO ps) {
-  __shared__ float lhx[2][kMW][kBX];
-  __shared__ float lhy[2][kMW][kBX];
-  __shared__ float lhz[2][kMW][kBX];
+// Inputs of one plane for one thread's cell.
+struct MarchIn {
+  float ex_j, ez_j, ex_k, ey_k;  // E at (j+1) and (k+1), zero past the grid
+  float hx, hy, hz;              // H^{n-1/2} at the cell
+  unsigned m;                    // material id
+};
+
+__device__ __forceinline__ MarchIn march_load(const KParams& p, const MarchIO& io, unsigned c,
+                                              unsigned dj, unsigned dk, bool hj, bool hk) {
+  MarchIn r;
+  r.m = ldu16(p.mat, c);
+  r.ex_j = ldf(io.eo[0], c + dj);
+  r.ez_j = ldf(io.eo[2], c + dj);
+  r.ex_k = ldf(io.eo[0], c + dk);
+  r.ey_k = ldf(io.eo[1], c + dk);
+  r.hx = ldf(io.ho[0], c);
+  r.hy = ldf(io.ho[1], c);
+  r.hz = ldf(io.ho[2], c);
+  r.ex_j = hj ? r.ex_j : 0.f;
+  r.ez_j = hj ? r.ez_j : 0.f;
+  r.ex_k = hk ? r.ex_k : 0.f;
+  r.ey_k = hk ? r.ey_k : 0.f;
+  return r;
+}
+
+template <bool GRADED, int MJ>
+__global__ __launch_bounds__(kBX * (MJ + 2)) void k_step_march(KParams p, MarchIO io, PsiIO ps) {
+  constexpr int kMJ = MJ;
+  __shared__ float lhx[2][kMJ + 1][kBX + 1];
+  __shared__ float lhy[2][kMJ + 1][kBX + 1];
+  __shared__ float lhz[2][kMJ + 1][kBX + 1];
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int tk = L % io.ntk;
   const int r = L / io.ntk;
   const int tj = r % io.ntj;
   const int seg = r / io.ntj;
   const int w = (int)threadIdx.y, l = (int)threadIdx.x;
-  const int j = tj * kMJ + w - 1, k = tk * kMK + l - 1;
+  // role: rows (w <= kMJ) or the halo column (last wave, lanes < kMJ)
+  const bool rows = w <= kMJ;
+  const bool active = rows || l < kMJ;
+  const int j = rows ? tj * kMJ + w - 1 : tj * kMJ + l;
+  const int k = rows ? tk * kBX + l : tk * kBX - 1;
+  const int srow = rows ? w : l + 1, scol = rows ? l + 1 : 0;  // LDS slot
   const int i0 = seg * io.seg_len;
   const int i1 = min(p.nx, i0 + io.seg_len);
-  const bool valid = j >= 0 && j < p.ny && k >= 0 && k < p.nz;
-  const bool inner = valid && w > 0 && l > 0;
+  const bool valid = active && j >= 0 && j < p.ny && k >= 0 && k < p.nz;
+  const bool inner = valid && rows && w > 0;
   const unsigned P = (unsigned)p.plane;
   const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
-  // column base (plane 0) of this thread's cell; clamped for invalid threads
   const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
   const unsigned dj = hj ? (unsigned)p.nz : 0u, dk = hk ? 1u : 0u;
-  float e0 = 0.f, e1 = 0.f, e2 = 0.f;  // E^n of the current plane, own cell
-  float hyp = 0.f, hzp = 0.f;          // H^{n+1/2} of the previous plane
-  if (valid) {
+  float e0 = 0.f, e1 = 0.f, e2 = 0.f;  // E^n of plane i, own cell
+  float n0 = 0.f, n1 = 0.f, n2 = 0.f;  // E^n of plane i+1
+  float hyp = 0.f, hzp = 0.f;          // H^{n+1/2} of plane i-1
+  MarchIn cur;
+  cur.ex_j = cur.ez_j = cur.ex_k = cur.ey_k = cur.hx = cur.hy = cur.hz = 0.f;
+  cur.m = 0;
+  if (valid && i0 < i1) {
     const unsigned c = (unsigned)i0 * P + c0;
     e0 = ldf(io.eo[0], c);
     e1 = ldf(io.eo[1], c);
     e2 = ldf(io.eo[2], c);
+    if (i0 + 1 < p.nx) {
+      n0 = ldf(io.eo[0], c + P);
+      n1 = ldf(io.eo[1], c + P);
+      n2 = ldf(io.eo[2], c + P);
+    }
+    cur = march_load(p, io, c, dj, dk, hj, hk);
     if (i0 > 0) {
       // H of plane i0-1 (owned by the previous segment): recomputed, not stored
       const unsigned cm = c - P;
-      const unsigned m = ldu16(p.mat, cm);
-      const float ax = ldf(io.eo[0], cm), ay = ldf(io.eo[1], cm), az = ldf(io.eo[2], cm);
-      float ex_j = ldf(io.eo[0], cm + dj), ez_j = ldf(io.eo[2], cm + dj);
-      float ex_k = ldf(io.eo[0], cm + dk), ey_k = ldf(io.eo[1], cm + dk);
-      ex_j = hj ? ex_j : 0.f;
-      ez_j = hj ? ez_j : 0.f;
-      ex_k = hk ? ex_k : 0.f;
-      ey_k = hk ? ey_k : 0.f;
-      const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, p.coef[m].w, ax, ay, az, ex_j,
-                                             ez_j, ex_k, ey_k, e1, e2, ldf(io.ho[0], cm),
-                                             ldf(io.ho[1], cm), ldf(io.ho[2], cm));
+      const MarchIn pm = march_load(p, io, cm, dj, dk, hj, hk);
+      const float3 h = h_core<GRADED, false>(
+          p, ps, i0 - 1, j, k, p.coef[pm.m].w, ldf(io.eo[0], cm), ldf(io.eo[1], cm),
+          ldf(io.eo[2], cm), pm.ex_j, pm.ez_j, pm.ex_k, pm.ey_k, e1, e2, pm.hx, pm.hy, pm.hz);
       hyp = h.y;
       hzp = h.z;
     }
@@ -570,38 +603,32 @@ __global__ __launch_bounds__(kBX * kMW) void k_step_march(KParams p, MarchIO io,
   for (int i = i0; i < i1; ++i) {
     const int b = i & 1;
     const unsigned c = (unsigned)i * P + c0;
-    float3 h = make_float3(0.f, 0.f, 0.f);
-    unsigned m = 0;
-    float n0 = 0.f, n1 = 0.f, n2 = 0.f;  // E^n of plane i+1, own cell
-    if (valid) {
-      m = ldu16(p.mat, c);
-      const bool hi = i + 1 < p.nx;
-      const unsigned cn = hi ? c + P : c;
-      n0 = ldf(io.eo[0], cn);
-      n1 = ldf(io.eo[1], cn);
-      n2 = ldf(io.eo[2], cn);
-      float ex_j = ldf(io.eo[0], c + dj), ez_j = ldf(io.eo[2], c + dj);
-      float ex_k = ldf(io.eo[0], c + dk), ey_k = ldf(io.eo[1], c + dk);
-      const float hx = ldf(io.ho[0], c), hy = ldf(io.ho[1], c), hz = ldf(io.ho[2], c);
-      n0 = hi ? n0 : 0.f;
-      n1 = hi ? n1 : 0.f;
-      n2 = hi ? n2 : 0.f;
-      ex_j = hj ? ex_j : 0.f;
-      ez_j = hj ? ez_j : 0.f;
-      ex_k = hk ? ex_k : 0.f;
-      ey_k = hk ? ey_k : 0.f;
-      h = h_core<GRADED, true>(p, ps, i, j, k, p.coef[m].w, e0, e1, e2, ex_j, ez_j, ex_k, ey_k,
-                               n1, n2, hx, hy, hz, inner);
+    // inputs of plane i+1, loaded while plane i is computed
+    MarchIn nxt = cur;
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f;  // E^n of plane i+2
+    if (valid && i + 1 < i1) {
+      nxt = march_load(p, io, c + P, dj, dk, hj, hk);
+      if (i + 2 < p.nx) {
+        q0 = ldf(io.eo[0], c + 2 * P);
+        q1 = ldf(io.eo[1], c + 2 * P);
+        q2 = ldf(io.eo[2], c + 2 * P);
+      }
     }
-    lhx[b][w][l] = h.x;
-    lhy[b][w][l] = h.y;
-    lhz[b][w][l] = h.z;
+    float3 h = make_float3(0.f, 0.f, 0.f);
+    if (valid)
+      h = h_core<GRADED, true>(p, ps, i, j, k, p.coef[cur.m].w, e0, e1, e2, cur.ex_j, cur.ez_j,
+                               cur.ex_k, cur.ey_k, n1, n2, cur.hx, cur.hy, cur.hz, inner);
+    if (active) {
+      lhx[b][srow][scol] = h.x;
+      lhy[b][srow][scol] = h.y;
+      lhz[b][srow][scol] = h.z;
+    }
     __syncthreads();
     if (inner) {
-      const float hx_j = lhx[b][w - 1][l], hz_j = lhz[b][w - 1][l];
-      const float hx_k = lhx[b][w][l - 1], hy_k = lhy[b][w][l - 1];
-      const float3 e = e_core<GRADED>(p, ps, i, j, k, c, m, e0, e1, e2, h.x, h.y, h.z, hyp, hzp,
-                                      hx_j, hz_j, hx_k, hy_k);
+      const float hx_j = lhx[b][w - 1][l + 1], hz_j = lhz[b][w - 1][l + 1];
+      const float hx_k = lhx[b][w][l], hy_k = lhy[b][w][l];
+      const float3 e = e_core<GRADED>(p, ps, i, j, k, c, cur.m, e0, e1, e2, h.x, h.y, h.z, hyp,
+                                      hzp, hx_j, hz_j, hx_k, hy_k);
       stf(io.hn[0], c, h.x);
       stf(io.hn[1], c, h.y);
       stf(io.hn[2], c, h.z);
@@ -614,6 +641,10 @@ __global__ __launch_bounds__(kBX * kMW) void k_step_march(KParams p, MarchIO io,
     e0 = n0;
     e1 = n1;
     e2 = n2;
+    n0 = q0;
+    n1 = q1;
+    n2 = q2;
+    cur = nxt;
   }
 }
 
@@ -1192,6 +1223,7 @@ struct rfxh_plan {
   bool fused = false;           // ping-pong step over two field sets (fused or marching kernel)
   bool march = false;           // the ping-pong step uses the marching kernel
   int march_seg = 0;            // x segments of the marching kernel (0: automatic)
+  int march_tile = 0;           // index into kMarchRows
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   int cur = 0;                  // which set holds the current state
@@ -1241,8 +1273,8 @@ long long face_slab_floats(const rfxh_desc* d, int face) {
 
 // Tiles and x segments of the marching kernel.  Automatic segments: enough
 // workgroups for about eight per CU, at least four planes per segment.
-long long march_blocks(const KParams& kp, int want_seg, MarchIO* io, int* nseg_out) {
-  const int ntj = (kp.ny + kMJ - 1) / kMJ, ntk = (kp.nz + kMK - 1) / kMK;
+long long march_blocks(const KParams& kp, int rows, int want_seg, MarchIO* io, int* nseg_out) {
+  const int ntj = (kp.ny + rows - 1) / rows, ntk = (kp.nz + kBX - 1) / kBX;
   const long long tiles = (long long)ntj * ntk;
   int nseg = want_seg;
   if (nseg <= 0) {
@@ -1301,11 +1333,16 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
       io.hn[a] = pl->set[to][3 + a];
     }
     int nseg = 1;
-    const long long nblk = march_blocks(pl->kp, pl->march_seg, &io, &nseg);
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_step_march<true>, dim3((unsigned)nblk), dim3(kBX, kMW), 0, s, pl->kp, io, ps);
-    else
-      hipLaunchKernelGGL(k_step_march<false>, dim3((unsigned)nblk), dim3(kBX, kMW), 0, s, pl->kp, io, ps);
+    const int rows = kMarchRows[pl->march_tile];
+    const long long nblk = march_blocks(pl->kp, rows, pl->march_seg, &io, &nseg);
+    const dim3 g((unsigned)nblk), blk(kBX, rows + 2);
+    if (pl->march_tile == 0) {
+      if (pl->kp.graded) hipLaunchKernelGGL((k_step_march<true, 6>), g, blk, 0, s, pl->kp, io, ps);
+      else hipLaunchKernelGGL((k_step_march<false, 6>), g, blk, 0, s, pl->kp, io, ps);
+    } else {
+      if (pl->kp.graded) hipLaunchKernelGGL((k_step_march<true, 14>), g, blk, 0, s, pl->kp, io, ps);
+      else hipLaunchKernelGGL((k_step_march<false, 14>), g, blk, 0, s, pl->kp, io, ps);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
   } else if (pl->fused) {
     FusedIO io;
@@ -1748,12 +1785,13 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
 }
 
 int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
-  if (!pl || kernel < 0 || kernel > 2) return fail(RFXH_EINVAL, "bad variant");
+  if (!pl || kernel < 0 || kernel > 3) return fail(RFXH_EINVAL, "bad variant");
   if (kernel > 0 && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
-  if (kernel == 2) {
+  if (kernel >= 2) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     pl->march_seg = param;
+    pl->march_tile = kernel - 2;
   } else {
     if (param < 0 || param > 1) return fail(RFXH_EINVAL, "bad workgroup order");
     const int old_order = pl->kp.order;
@@ -1766,7 +1804,7 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
     }
   }
   pl->fused = kernel != 0;
-  pl->march = kernel == 2;
+  pl->march = kernel >= 2;
   for (int g = 0; g < 2; ++g)
     if (pl->graph_exec[g]) {
       (void)hipGraphExecDestroy(pl->graph_exec[g]);
@@ -1777,7 +1815,7 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
 
 int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
   if (!pl || !kernel || !param) return fail(RFXH_EINVAL, "bad arguments");
-  *kernel = pl->march ? 2 : pl->fused ? 1 : 0;
+  *kernel = pl->march ? 2 + pl->march_tile : pl->fused ? 1 : 0;
   *param = pl->march ? pl->march_seg : pl->kp.order;
   return RFXH_OK;
 }
