@@ -1,7 +1,8 @@
 """Kernel-variant autotuner.
 
-The plan's step can run as the two-pass kernels (H pass, E pass, in place)
-or, with a second field set, as one fused kernel (flat or XCD-band
+The plan's step can run as the two-pass kernels (H pass, E pass, in place;
+one cell per thread or columns of 2 or 4 cells along x) or, with a second
+field set, as one fused kernel (flat or XCD-band
 workgroup order) or as the marching kernel (6- or 14-row tiles of the y-z
 plane walking along x, in a chosen number of x segments) (include/rfxhip.h
 ``rfxh_set_variant``).  All
@@ -36,7 +37,7 @@ def variants(dr):
     """(kernel, parameter) candidates: kernel 0 two-pass and 1 fused with
     workgroup order 1 (flat) or 0 (XCD bands); kernel 2 marching with its
     number of x segments (0 = automatic)."""
-    out = [(0, 1), (0, 0)]
+    out = [(0, 1), (0, 0), (4, 1), (4, 0), (5, 1), (5, 0)]
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
         nx = dr.prob.shape[0]
@@ -46,14 +47,16 @@ def variants(dr):
 
 
 MARCH_ROWS = {2: 6, 3: 14}
+ORDERED = {0: "two-pass", 1: "fused", 4: "col2", 5: "col4"}
 
 
 def variant_name(v) -> str:
+    """two-pass-flat, col2-bands, fused-flat, march14-16, ..."""
     k, prm = v
     if k in MARCH_ROWS:
         base = f"march{MARCH_ROWS[k]}"
         return base if prm == 0 else f"{base}-{prm}"
-    return f"{'fused' if k else 'two-pass'}-{'flat' if prm else 'bands'}"
+    return f"{ORDERED[k]}-{'flat' if prm else 'bands'}"
 
 
 def parse_variant(name: str):
@@ -62,7 +65,9 @@ def parse_variant(name: str):
         rows = int(head[5:])
         kern = {r: k for k, r in MARCH_ROWS.items()}[rows]
         return (kern, int(seg) if seg else 0)
-    return (1 if name.startswith("fused") else 0, 1 if name.endswith("flat") else 0)
+    head, _, order = name.rpartition("-")
+    kern = {v: k for k, v in ORDERED.items()}[head]
+    return (kern, 1 if order == "flat" else 0)
 
 
 def get_variant(dr):

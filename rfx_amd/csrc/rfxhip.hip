@@ -161,7 +161,11 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // math is 32-bit: a y-z plane holds fewer than 2^24 cells (checked at plan
 // creation), so the in-plane index f is exact in float and the float
 // quotient f/nz is within one of the integer one.
-__device__ __forceinline__ bool cell_at(const KParams& p, int L, int& i, int& j, int& k) {
+__device__ __forceinline__ bool cell_at(const KParams& p, int L, int& i, int& j, int& k,
+                                        int nxb = -1) {
+  // nxb: number of x positions the grid walks (x blocks of several planes
+  // for the column kernels; default the planes themselves)
+  if (nxb < 0) nxb = p.nx;
   const int pl = (int)p.plane;
   const int t = (int)(threadIdx.y * kBX + threadIdx.x);
   int f;
@@ -170,8 +174,8 @@ __device__ __forceinline__ bool cell_at(const KParams& p, int L, int& i, int& j,
     // sweeps its band plane by plane with 512-cell chunks fastest
     const int bw = (pl + 7) / 8;
     const int ncb = (bw + kWG - 1) / kWG;
-    const int b = L / (p.nx * ncb);
-    const int r = L - b * (p.nx * ncb);
+    const int b = L / (nxb * ncb);
+    const int r = L - b * (nxb * ncb);
     i = r / ncb;
     f = b * bw + (r - i * ncb) * kWG + t;
     if (f >= (b + 1) * bw) return false;
@@ -694,6 +698,120 @@ __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
   int i, j, k;
   if (!cell_of(p, i, j, k)) return;
   two_pass_e<GRADED>(p, ps, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// Column variants of the two passes: each thread updates NP consecutive
+// cells along x.  The x-neighbour one cell needs is the next (H pass) or
+// previous (E pass) cell's own value, already in registers, so x-neighbours
+// are re-read once per column instead of once per cell, and each thread
+// has NP cells' loads in flight at once.
+// ---------------------------------------------------------------------------
+template <bool GRADED, int NP>
+__global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
+  int ib, j, k;
+  const int nxb = (p.nx + NP - 1) / NP;
+  if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, nxb)) return;
+  const int i0 = ib * NP;
+  const unsigned P = (unsigned)p.plane;
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
+  const unsigned c0 = ((unsigned)i0 * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned dj = hj ? (unsigned)p.nz : 0u, dk = hk ? 1u : 0u;
+  float ex[NP + 1], ey[NP + 1], ez[NP + 1], exj[NP], ezj[NP], exk[NP], eyk[NP];
+  float hx[NP], hy[NP], hz[NP];
+  unsigned m[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const bool in = i0 + q < p.nx;
+    const unsigned c = in ? c0 + (unsigned)q * P : c0;
+    m[q] = ldu16(p.mat, c);
+    ex[q] = ldf(p.f[0], c);
+    ey[q] = ldf(p.f[1], c);
+    ez[q] = ldf(p.f[2], c);
+    exj[q] = ldf(p.f[0], c + dj);
+    ezj[q] = ldf(p.f[2], c + dj);
+    exk[q] = ldf(p.f[0], c + dk);
+    eyk[q] = ldf(p.f[1], c + dk);
+    hx[q] = ldf(p.f[3], c);
+    hy[q] = ldf(p.f[4], c);
+    hz[q] = ldf(p.f[5], c);
+  }
+  {
+    // E of the cell past the column (zero past the last plane)
+    const bool in = i0 + NP < p.nx;
+    const unsigned c = in ? c0 + (unsigned)NP * P : c0;
+    ey[NP] = ldf(p.f[1], c);
+    ez[NP] = ldf(p.f[2], c);
+    ey[NP] = in ? ey[NP] : 0.f;
+    ez[NP] = in ? ez[NP] : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    if (i0 + q >= p.nx) break;
+    const float exj_ = hj ? exj[q] : 0.f, ezj_ = hj ? ezj[q] : 0.f;
+    const float exk_ = hk ? exk[q] : 0.f, eyk_ = hk ? eyk[q] : 0.f;
+    const float eyi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ey[q + 1] : 0.f) : ey[NP];
+    const float ezi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ez[q + 1] : 0.f) : ez[NP];
+    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, p.coef[m[q]].w, ex[q], ey[q], ez[q],
+                                          exj_, ezj_, exk_, eyk_, eyi, ezi, hx[q], hy[q], hz[q]);
+    const unsigned c = c0 + (unsigned)q * P;
+    stf(p.f[3], c, h.x);
+    stf(p.f[4], c, h.y);
+    stf(p.f[5], c, h.z);
+  }
+}
+
+template <bool GRADED, int NP>
+__global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
+  int ib, j, k;
+  const int nxb = (p.nx + NP - 1) / NP;
+  if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, nxb)) return;
+  const int i0 = ib * NP;
+  const unsigned P = (unsigned)p.plane;
+  const bool bj = j > 0, bk = k > 0;
+  const unsigned c0 = ((unsigned)i0 * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned dj = bj ? (unsigned)p.nz : 0u, dk = bk ? 1u : 0u;
+  // index 0 holds the cell before the column (i0-1), 1..NP the column
+  float hx[NP + 1], hy[NP + 1], hz[NP + 1], hxj[NP], hzj[NP], hxk[NP], hyk[NP];
+  float e0[NP], e1[NP], e2[NP];
+  unsigned m[NP];
+  {
+    const bool in = i0 > 0;
+    const unsigned c = in ? c0 - P : c0;
+    hy[0] = ldf(p.f[4], c);
+    hz[0] = ldf(p.f[5], c);
+    hy[0] = in ? hy[0] : 0.f;
+    hz[0] = in ? hz[0] : 0.f;
+    hx[0] = 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const bool in = i0 + q < p.nx;
+    const unsigned c = in ? c0 + (unsigned)q * P : c0;
+    m[q] = ldu16(p.mat, c);
+    hx[q + 1] = ldf(p.f[3], c);
+    hy[q + 1] = ldf(p.f[4], c);
+    hz[q + 1] = ldf(p.f[5], c);
+    hxj[q] = ldf(p.f[3], c - dj);
+    hzj[q] = ldf(p.f[5], c - dj);
+    hxk[q] = ldf(p.f[3], c - dk);
+    hyk[q] = ldf(p.f[4], c - dk);
+    e0[q] = ldf(p.f[0], c);
+    e1[q] = ldf(p.f[1], c);
+    e2[q] = ldf(p.f[2], c);
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    if (i0 + q >= p.nx) break;
+    const unsigned c = c0 + (unsigned)q * P;
+    const float hxj_ = bj ? hxj[q] : 0.f, hzj_ = bj ? hzj[q] : 0.f;
+    const float hxk_ = bk ? hxk[q] : 0.f, hyk_ = bk ? hyk[q] : 0.f;
+    const float3 e = e_core<GRADED>(p, ps, i0 + q, j, k, c, m[q], e0[q], e1[q], e2[q], hx[q + 1],
+                                    hy[q + 1], hz[q + 1], hy[q], hz[q], hxj_, hzj_, hxk_, hyk_);
+    stf(p.f[0], c, e.x);
+    stf(p.f[1], c, e.y);
+    stf(p.f[2], c, e.z);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1220,10 +1338,12 @@ __global__ void k_advance(long long* step) {
 struct rfxh_plan {
   KParams kp;
   dim3 grid, block;
+  dim3 grid_col;                // column kernels (ncol cells per thread)
   bool fused = false;           // ping-pong step over two field sets (fused or marching kernel)
   bool march = false;           // the ping-pong step uses the marching kernel
   int march_seg = 0;            // x segments of the marching kernel (0: automatic)
   int march_tile = 0;           // index into kMarchRows
+  int ncol = 1;                 // two-pass step: cells per thread along x (1, 2, 4)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   int cur = 0;                  // which set holds the current state
@@ -1360,15 +1480,28 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   } else {
     KParams kp = pl->kp;
     for (int c = 0; c < 6; ++c) kp.f[c] = pl->set[from][c];
-    if (kp.graded)
-      hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, kp, ps);
-    else
-      hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, kp, ps);
+    const dim3 g = pl->ncol == 1 ? pl->grid : pl->grid_col, blk = pl->block;
+    if (pl->ncol == 1) {
+      if (kp.graded) hipLaunchKernelGGL(k_update_h<true>, g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL(k_update_h<false>, g, blk, 0, s, kp, ps);
+    } else if (pl->ncol == 2) {
+      if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 2>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_h_col<false, 2>), g, blk, 0, s, kp, ps);
+    } else {
+      if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 4>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_h_col<false, 4>), g, blk, 0, s, kp, ps);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    if (kp.graded)
-      hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, kp, ps);
-    else
-      hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, kp, ps);
+    if (pl->ncol == 1) {
+      if (kp.graded) hipLaunchKernelGGL(k_update_e<true>, g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL(k_update_e<false>, g, blk, 0, s, kp, ps);
+    } else if (pl->ncol == 2) {
+      if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 2>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_e_col<false, 2>), g, blk, 0, s, kp, ps);
+    } else {
+      if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 4>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_e_col<false, 4>), g, blk, 0, s, kp, ps);
+    }
   }
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
   launch_accumulators(pl, s, to);
@@ -1379,13 +1512,19 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
 }
 
 // Launch grid of the plan's workgroup order.
+// pl->grid: one cell per thread (every kernel that maps with cell_of);
+// pl->grid_col: the column kernels' grid (ncol cells per thread along x).
 int plan_grid(rfxh_plan* pl) {
   const KParams& kp = pl->kp;
   const long long flat = kp.plane;
-  const long long nblocks = kp.order == 0 ? 8LL * kp.nx * (((flat + 7) / 8 + kWG - 1) / kWG)
-                                          : (long long)kp.nx * ((flat + kWG - 1) / kWG);
-  if (nblocks > 0x7fffffffLL) return fail(RFXH_EINVAL, "grid too large");
-  pl->grid = dim3((unsigned)nblocks);
+  auto blocks = [&](long long nxb) {
+    return kp.order == 0 ? 8LL * nxb * (((flat + 7) / 8 + kWG - 1) / kWG)
+                         : nxb * ((flat + kWG - 1) / kWG);
+  };
+  const long long n1 = blocks(kp.nx), nc = blocks((kp.nx + pl->ncol - 1) / pl->ncol);
+  if (n1 > 0x7fffffffLL) return fail(RFXH_EINVAL, "grid too large");
+  pl->grid = dim3((unsigned)n1);
+  pl->grid_col = dim3((unsigned)nc);
   pl->block = dim3(kBX, kTY);
   return 0;
 }
@@ -1785,26 +1924,28 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
 }
 
 int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
-  if (!pl || kernel < 0 || kernel > 3) return fail(RFXH_EINVAL, "bad variant");
-  if (kernel > 0 && !pl->set[1][0])
+  if (!pl || kernel < 0 || kernel > 5) return fail(RFXH_EINVAL, "bad variant");
+  if ((kernel == 1 || kernel == 2 || kernel == 3) && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
-  if (kernel >= 2) {
+  if (kernel == 2 || kernel == 3) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     pl->march_seg = param;
     pl->march_tile = kernel - 2;
   } else {
     if (param < 0 || param > 1) return fail(RFXH_EINVAL, "bad workgroup order");
-    const int old_order = pl->kp.order;
+    const int old_order = pl->kp.order, old_ncol = pl->ncol;
     pl->kp.order = param;
+    pl->ncol = kernel == 4 ? 2 : kernel == 5 ? 4 : 1;
     int rc = plan_grid(pl);
     if (rc) {
       pl->kp.order = old_order;
+      pl->ncol = old_ncol;
       (void)plan_grid(pl);
       return rc;
     }
   }
-  pl->fused = kernel != 0;
-  pl->march = kernel >= 2;
+  pl->fused = kernel == 1 || kernel == 2 || kernel == 3;
+  pl->march = kernel == 2 || kernel == 3;
   for (int g = 0; g < 2; ++g)
     if (pl->graph_exec[g]) {
       (void)hipGraphExecDestroy(pl->graph_exec[g]);
@@ -1815,7 +1956,7 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
 
 int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
   if (!pl || !kernel || !param) return fail(RFXH_EINVAL, "bad arguments");
-  *kernel = pl->march ? 2 + pl->march_tile : pl->fused ? 1 : 0;
+  *kernel = pl->march ? 2 + pl->march_tile : pl->fused ? 1 : pl->ncol == 2 ? 4 : pl->ncol == 4 ? 5 : 0;
   *param = pl->march ? pl->march_seg : pl->kp.order;
   return RFXH_OK;
 }
