@@ -340,13 +340,14 @@ def main():
     ms_e, n_e = dr.kernel_time_ms(1)
     ms_d, n_d = dr.kernel_time_ms(2)
     value = args.steps * n_total * cells / el / 1e6
-    if kern_v:
+    if kern_v in (1, 2, 3):
         # one launch does the whole step: 48 B per cell
-        which = "k_step_march" if kern_v == 2 else "k_step_fused"
+        which = "k_step_march" if kern_v in (2, 3) else "k_step_fused"
         ms, unit_bytes = ms_h, BYTES_PER_CELL_STEP * cells
     else:
-        # each pass advances half the field components: 24 B per cell
-        which, ms = ("k_update_e", ms_e) if ms_e >= ms_h else ("k_update_h", ms_h)
+        # two passes, each advancing half the field components: 24 B per cell
+        sfx = "_col" if kern_v in (4, 5) else ""
+        which, ms = (f"k_update_e{sfx}", ms_e) if ms_e >= ms_h else (f"k_update_h{sfx}", ms_h)
         unit_bytes = 0.5 * BYTES_PER_CELL_STEP * cells
     achieved = unit_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
     traffic = None
@@ -370,7 +371,7 @@ def main():
                    "autotune_ms_per_step": tuning["ms_per_step"] if tuning else None,
                    "pct_hbm_roofline_48B": round(100 * value * 1e6 * BYTES_PER_CELL_STEP
                                                  / (HBM_PEAK_GBS * 1e9), 2),
-                   "kernel_ms": {"k_update_h": round(ms_h, 4), "k_update_e": round(ms_e, 4),
+                   "kernel_ms": {"h_pass_or_step": round(ms_h, 4), "e_pass": round(ms_e, 4),
                                  "k_dft_plane": round(ms_d, 4), "samples": n_e}},
         "roofline": {"bound": "hbm", "kernel": which,
                      "achieved": round(achieved, 1) if achieved else None,
