@@ -1,8 +1,8 @@
 """Kernel-variant autotuner.
 
 The plan's step can run as the two-pass kernels (H pass, E pass, in place;
-one cell per thread or columns of 2 or 4 cells along x) or, with a second
-field set, as one fused kernel (flat or XCD-band
+one cell per thread or columns of 2, 3 or 4 cells along x, optionally with
+streaming stores) or, with a second field set, as one fused kernel (flat or XCD-band
 workgroup order) or as the marching kernel (6- or 14-row tiles of the y-z
 plane walking along x, in a chosen number of x segments) (include/rfxhip.h
 ``rfxh_set_variant``).  All
@@ -37,7 +37,9 @@ def variants(dr):
     """(kernel, parameter) candidates: kernel 0 two-pass and 1 fused with
     workgroup order 1 (flat) or 0 (XCD bands); kernel 2 marching with its
     number of x segments (0 = automatic)."""
-    out = [(0, 1), (0, 0), (4, 1), (4, 0), (5, 1), (5, 0)]
+    out = [(0, 1), (0, 0)]
+    for kern in (4, 6, 5):                  # columns of 2, 3, 4 cells
+        out += [(kern, order | nt) for nt in (0, 2) for order in (1, 0)]
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
         nx = dr.prob.shape[0]
@@ -47,16 +49,17 @@ def variants(dr):
 
 
 MARCH_ROWS = {2: 6, 3: 14}
-ORDERED = {0: "two-pass", 1: "fused", 4: "col2", 5: "col4"}
+ORDERED = {0: "two-pass", 1: "fused", 4: "col2", 5: "col4", 6: "col3"}
 
 
 def variant_name(v) -> str:
-    """two-pass-flat, col2-bands, fused-flat, march14-16, ..."""
+    """two-pass-flat, col2-bands, col3-flat-nt, fused-flat, march14-16, ..."""
     k, prm = v
     if k in MARCH_ROWS:
         base = f"march{MARCH_ROWS[k]}"
         return base if prm == 0 else f"{base}-{prm}"
-    return f"{ORDERED[k]}-{'flat' if prm else 'bands'}"
+    name = f"{ORDERED[k]}-{'flat' if prm & 1 else 'bands'}"
+    return name + "-nt" if prm & 2 else name
 
 
 def parse_variant(name: str):
@@ -65,9 +68,12 @@ def parse_variant(name: str):
         rows = int(head[5:])
         kern = {r: k for k, r in MARCH_ROWS.items()}[rows]
         return (kern, int(seg) if seg else 0)
+    nt = 0
+    if name.endswith("-nt"):
+        name, nt = name[:-3], 2
     head, _, order = name.rpartition("-")
     kern = {v: k for k, v in ORDERED.items()}[head]
-    return (kern, 1 if order == "flat" else 0)
+    return (kern, (1 if order == "flat" else 0) | nt)
 
 
 def get_variant(dr):

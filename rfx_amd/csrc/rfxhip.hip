@@ -92,6 +92,7 @@ struct KParams {
   int n_port;
   long long* step;               // current absolute step (device)
   int order;                     // workgroup order: 0 bands, 1 flat chunks
+  int nt;                        // column kernels: field stores bypass the caches (streaming)
   float inv_nz;                  // 1/nz for the flat-index split
   int slo[3], shi[3];            // CPML slab extents: cells i < slo or i >= shi per axis
 };
@@ -144,6 +145,14 @@ __device__ __forceinline__ float ldf(const float* base, unsigned cell) {
 }
 __device__ __forceinline__ void stf(float* base, unsigned cell, float v) {
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (cell << 2)) = v;
+}
+// Store that marks the line as streaming (non-temporal).
+__device__ __forceinline__ void stf_nt(float* base, unsigned cell, float v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (cell << 2)));
+}
+__device__ __forceinline__ void stf_sel(float* base, unsigned cell, float v, bool nt) {
+  if (nt) stf_nt(base, cell, v);
+  else stf(base, cell, v);
 }
 __device__ __forceinline__ unsigned ldu16(const uint16_t* base, unsigned cell) {
   return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(base) + (cell << 1));
@@ -755,9 +764,9 @@ __global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
     const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, p.coef[m[q]].w, ex[q], ey[q], ez[q],
                                           exj_, ezj_, exk_, eyk_, eyi, ezi, hx[q], hy[q], hz[q]);
     const unsigned c = c0 + (unsigned)q * P;
-    stf(p.f[3], c, h.x);
-    stf(p.f[4], c, h.y);
-    stf(p.f[5], c, h.z);
+    stf_sel(p.f[3], c, h.x, p.nt);
+    stf_sel(p.f[4], c, h.y, p.nt);
+    stf_sel(p.f[5], c, h.z, p.nt);
   }
 }
 
@@ -808,9 +817,9 @@ __global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
     const float hxk_ = bk ? hxk[q] : 0.f, hyk_ = bk ? hyk[q] : 0.f;
     const float3 e = e_core<GRADED>(p, ps, i0 + q, j, k, c, m[q], e0[q], e1[q], e2[q], hx[q + 1],
                                     hy[q + 1], hz[q + 1], hy[q], hz[q], hxj_, hzj_, hxk_, hyk_);
-    stf(p.f[0], c, e.x);
-    stf(p.f[1], c, e.y);
-    stf(p.f[2], c, e.z);
+    stf_sel(p.f[0], c, e.x, p.nt);
+    stf_sel(p.f[1], c, e.y, p.nt);
+    stf_sel(p.f[2], c, e.z, p.nt);
   }
 }
 
@@ -1487,6 +1496,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     } else if (pl->ncol == 2) {
       if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 2>), g, blk, 0, s, kp, ps);
       else hipLaunchKernelGGL((k_update_h_col<false, 2>), g, blk, 0, s, kp, ps);
+    } else if (pl->ncol == 3) {
+      if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 3>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_h_col<false, 3>), g, blk, 0, s, kp, ps);
     } else {
       if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 4>), g, blk, 0, s, kp, ps);
       else hipLaunchKernelGGL((k_update_h_col<false, 4>), g, blk, 0, s, kp, ps);
@@ -1498,6 +1510,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     } else if (pl->ncol == 2) {
       if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 2>), g, blk, 0, s, kp, ps);
       else hipLaunchKernelGGL((k_update_e_col<false, 2>), g, blk, 0, s, kp, ps);
+    } else if (pl->ncol == 3) {
+      if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 3>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_e_col<false, 3>), g, blk, 0, s, kp, ps);
     } else {
       if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 4>), g, blk, 0, s, kp, ps);
       else hipLaunchKernelGGL((k_update_e_col<false, 4>), g, blk, 0, s, kp, ps);
@@ -1924,7 +1939,7 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
 }
 
 int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
-  if (!pl || kernel < 0 || kernel > 5) return fail(RFXH_EINVAL, "bad variant");
+  if (!pl || kernel < 0 || kernel > 6) return fail(RFXH_EINVAL, "bad variant");
   if ((kernel == 1 || kernel == 2 || kernel == 3) && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
   if (kernel == 2 || kernel == 3) {
@@ -1932,10 +1947,12 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
     pl->march_seg = param;
     pl->march_tile = kernel - 2;
   } else {
-    if (param < 0 || param > 1) return fail(RFXH_EINVAL, "bad workgroup order");
+    const bool col = kernel >= 4;
+    if (param < 0 || param > (col ? 3 : 1)) return fail(RFXH_EINVAL, "bad workgroup order");
     const int old_order = pl->kp.order, old_ncol = pl->ncol;
-    pl->kp.order = param;
-    pl->ncol = kernel == 4 ? 2 : kernel == 5 ? 4 : 1;
+    pl->kp.order = param & 1;
+    pl->kp.nt = col ? (param >> 1) & 1 : 0;
+    pl->ncol = kernel == 4 ? 2 : kernel == 5 ? 4 : kernel == 6 ? 3 : 1;
     int rc = plan_grid(pl);
     if (rc) {
       pl->kp.order = old_order;
@@ -1956,8 +1973,10 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
 
 int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
   if (!pl || !kernel || !param) return fail(RFXH_EINVAL, "bad arguments");
-  *kernel = pl->march ? 2 + pl->march_tile : pl->fused ? 1 : pl->ncol == 2 ? 4 : pl->ncol == 4 ? 5 : 0;
-  *param = pl->march ? pl->march_seg : pl->kp.order;
+  *kernel = pl->march ? 2 + pl->march_tile
+          : pl->fused ? 1
+          : pl->ncol == 2 ? 4 : pl->ncol == 4 ? 5 : pl->ncol == 3 ? 6 : 0;
+  *param = pl->march ? pl->march_seg : pl->kp.order | (pl->kp.nt << 1);
   return RFXH_OK;
 }
 
