@@ -2,7 +2,8 @@
 
 The plan's step can run as the two-pass kernels (H pass, E pass, in place;
 one cell per thread or columns of 2, 3 or 4 cells along x, optionally with
-streaming stores) or, with a second field set, as one fused kernel (flat or XCD-band
+the material table staged in LDS or streaming stores) or, with a second
+field set, as one fused kernel (flat or XCD-band
 workgroup order) or as the marching kernel (6- or 14-row tiles of the y-z
 plane walking along x, in a chosen number of x segments) (include/rfxhip.h
 ``rfxh_set_variant``).  All
@@ -38,8 +39,9 @@ def variants(dr):
     workgroup order 1 (flat) or 0 (XCD bands); kernel 2 marching with its
     number of x segments (0 = automatic)."""
     out = [(0, 1), (0, 0)]
+    lds = (0, 4) if len(dr.flags) <= 512 else (0,)
     for kern in (4, 6, 5):                  # columns of 2, 3, 4 cells
-        out += [(kern, order | nt) for nt in (0, 2) for order in (1, 0)]
+        out += [(kern, order | t) for t in lds for order in (1, 0)]
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
         nx = dr.prob.shape[0]
@@ -59,6 +61,8 @@ def variant_name(v) -> str:
         base = f"march{MARCH_ROWS[k]}"
         return base if prm == 0 else f"{base}-{prm}"
     name = f"{ORDERED[k]}-{'flat' if prm & 1 else 'bands'}"
+    if prm & 4:
+        name += "-ldstab"
     return name + "-nt" if prm & 2 else name
 
 
@@ -68,12 +72,14 @@ def parse_variant(name: str):
         rows = int(head[5:])
         kern = {r: k for k, r in MARCH_ROWS.items()}[rows]
         return (kern, int(seg) if seg else 0)
-    nt = 0
+    bits = 0
     if name.endswith("-nt"):
-        name, nt = name[:-3], 2
+        name, bits = name[:-3], bits | 2
+    if name.endswith("-ldstab"):
+        name, bits = name[:-7], bits | 4
     head, _, order = name.rpartition("-")
     kern = {v: k for k, v in ORDERED.items()}[head]
-    return (kern, (1 if order == "flat" else 0) | nt)
+    return (kern, (1 if order == "flat" else 0) | bits)
 
 
 def get_variant(dr):
