@@ -1,9 +1,8 @@
 """Kernel-variant autotuner.
 
 The plan's step can run as the two-pass kernels (H pass, E pass, in place;
-one cell per thread or columns of 2, 3 or 4 cells along x, optionally with
-the material table staged in LDS or streaming stores) or, with a second
-field set, as one fused kernel (flat or XCD-band
+one cell per thread or columns of 2, 3 or 4 cells along x) or, with a
+second field set, as one fused kernel (flat or XCD-band
 workgroup order) or as the marching kernel (6- or 14-row tiles of the y-z
 plane walking along x, in a chosen number of x segments) (include/rfxhip.h
 ``rfxh_set_variant``).  All
@@ -39,9 +38,8 @@ def variants(dr):
     workgroup order 1 (flat) or 0 (XCD bands); kernel 2 marching with its
     number of x segments (0 = automatic)."""
     out = [(0, 1), (0, 0)]
-    lds = (0, 4) if len(dr.flags) <= 512 else (0,)
     for kern in (4, 6, 5):                  # columns of 2, 3, 4 cells
-        out += [(kern, order | t) for t in lds for order in (1, 0)]
+        out += [(kern, order) for order in (1, 0)]
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
         nx = dr.prob.shape[0]
@@ -55,15 +53,12 @@ ORDERED = {0: "two-pass", 1: "fused", 4: "col2", 5: "col4", 6: "col3"}
 
 
 def variant_name(v) -> str:
-    """two-pass-flat, col2-bands, col3-flat-nt, fused-flat, march14-16, ..."""
+    """two-pass-flat, col2-bands, col3-flat, fused-flat, march14-16, ..."""
     k, prm = v
     if k in MARCH_ROWS:
         base = f"march{MARCH_ROWS[k]}"
         return base if prm == 0 else f"{base}-{prm}"
-    name = f"{ORDERED[k]}-{'flat' if prm & 1 else 'bands'}"
-    if prm & 4:
-        name += "-ldstab"
-    return name + "-nt" if prm & 2 else name
+    return f"{ORDERED[k]}-{'flat' if prm else 'bands'}"
 
 
 def parse_variant(name: str):
@@ -72,14 +67,9 @@ def parse_variant(name: str):
         rows = int(head[5:])
         kern = {r: k for k, r in MARCH_ROWS.items()}[rows]
         return (kern, int(seg) if seg else 0)
-    bits = 0
-    if name.endswith("-nt"):
-        name, bits = name[:-3], bits | 2
-    if name.endswith("-ldstab"):
-        name, bits = name[:-7], bits | 4
     head, _, order = name.rpartition("-")
     kern = {v: k for k, v in ORDERED.items()}[head]
-    return (kern, (1 if order == "flat" else 0) | bits)
+    return (kern, 1 if order == "flat" else 0)
 
 
 def get_variant(dr):

@@ -92,9 +92,6 @@ struct KParams {
   int n_port;
   long long* step;               // current absolute step (device)
   int order;                     // workgroup order: 0 bands, 1 flat chunks
-  int nt;                        // column kernels: field stores bypass the caches (streaming)
-  int lds_tab;                   // column kernels: coefficient table staged in LDS (n_mat <= kTabLds)
-  int n_mat;                     // material table entries
   float inv_nz;                  // 1/nz for the flat-index split
   int slo[3], shi[3];            // CPML slab extents: cells i < slo or i >= shi per axis
 };
@@ -147,14 +144,6 @@ __device__ __forceinline__ float ldf(const float* base, unsigned cell) {
 }
 __device__ __forceinline__ void stf(float* base, unsigned cell, float v) {
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (cell << 2)) = v;
-}
-// Store that marks the line as streaming (non-temporal).
-__device__ __forceinline__ void stf_nt(float* base, unsigned cell, float v) {
-  __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (cell << 2)));
-}
-__device__ __forceinline__ void stf_sel(float* base, unsigned cell, float v, bool nt) {
-  if (nt) stf_nt(base, cell, v);
-  else stf(base, cell, v);
 }
 __device__ __forceinline__ unsigned ldu16(const uint16_t* base, unsigned cell) {
   return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(base) + (cell << 1));
@@ -726,22 +715,8 @@ __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
 // are re-read once per column instead of once per cell, and each thread
 // has NP cells' loads in flight at once.
 // ---------------------------------------------------------------------------
-constexpr int kTabLds = 512;  // largest material table staged in LDS
-
-// Copy the material table into LDS (whole workgroup, before any early exit).
-__device__ __forceinline__ void stage_table(const KParams& p, float4* tab, uint32_t* fl) {
-  const int t = (int)(threadIdx.y * kBX + threadIdx.x);
-  for (int q = t; q < p.n_mat; q += kWG) {
-    tab[q] = p.coef[q];
-    if (fl) fl[q] = p.flags[q];
-  }
-  __syncthreads();
-}
-
 template <bool GRADED, int NP>
 __global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
-  __shared__ float4 tab[kTabLds];
-  if (p.lds_tab) stage_table(p, tab, nullptr);
   int ib, j, k;
   const int nxb = (p.nx + NP - 1) / NP;
   if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, nxb)) return;
@@ -785,21 +760,17 @@ __global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
     const float exk_ = hk ? exk[q] : 0.f, eyk_ = hk ? eyk[q] : 0.f;
     const float eyi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ey[q + 1] : 0.f) : ey[NP];
     const float ezi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ez[q + 1] : 0.f) : ez[NP];
-    const float ch = p.lds_tab ? tab[m[q]].w : p.coef[m[q]].w;
-    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, ch, ex[q], ey[q], ez[q],
+    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, p.coef[m[q]].w, ex[q], ey[q], ez[q],
                                           exj_, ezj_, exk_, eyk_, eyi, ezi, hx[q], hy[q], hz[q]);
     const unsigned c = c0 + (unsigned)q * P;
-    stf_sel(p.f[3], c, h.x, p.nt);
-    stf_sel(p.f[4], c, h.y, p.nt);
-    stf_sel(p.f[5], c, h.z, p.nt);
+    stf(p.f[3], c, h.x);
+    stf(p.f[4], c, h.y);
+    stf(p.f[5], c, h.z);
   }
 }
 
 template <bool GRADED, int NP>
 __global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
-  __shared__ float4 tab[kTabLds];
-  __shared__ uint32_t tfl[kTabLds];
-  if (p.lds_tab) stage_table(p, tab, tfl);
   int ib, j, k;
   const int nxb = (p.nx + NP - 1) / NP;
   if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, nxb)) return;
@@ -843,14 +814,11 @@ __global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
     const unsigned c = c0 + (unsigned)q * P;
     const float hxj_ = bj ? hxj[q] : 0.f, hzj_ = bj ? hzj[q] : 0.f;
     const float hxk_ = bk ? hxk[q] : 0.f, hyk_ = bk ? hyk[q] : 0.f;
-    const float4 co = p.lds_tab ? tab[m[q]] : p.coef[m[q]];
-    const uint32_t fl = p.lds_tab ? tfl[m[q]] : p.flags[m[q]];
-    const float3 e = e_core_co<GRADED>(p, ps, i0 + q, j, k, c, co, fl, e0[q], e1[q], e2[q],
-                                       hx[q + 1], hy[q + 1], hz[q + 1], hy[q], hz[q], hxj_, hzj_,
-                                       hxk_, hyk_);
-    stf_sel(p.f[0], c, e.x, p.nt);
-    stf_sel(p.f[1], c, e.y, p.nt);
-    stf_sel(p.f[2], c, e.z, p.nt);
+    const float3 e = e_core<GRADED>(p, ps, i0 + q, j, k, c, m[q], e0[q], e1[q], e2[q], hx[q + 1],
+                                    hy[q + 1], hz[q + 1], hy[q], hz[q], hxj_, hzj_, hxk_, hyk_);
+    stf(p.f[0], c, e.x);
+    stf(p.f[1], c, e.y);
+    stf(p.f[2], c, e.z);
   }
 }
 
@@ -1733,7 +1701,6 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   kp.n_port = d->n_port;
   pl->max_steps = d->max_steps;
   pl->n_mat = d->n_mat;
-  kp.n_mat = d->n_mat;
   {
     const long long init = -1;
     long long* st;
@@ -1979,14 +1946,9 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
     pl->march_seg = param;
     pl->march_tile = kernel - 2;
   } else {
-    const bool col = kernel >= 4;
-    if (param < 0 || param > (col ? 7 : 1)) return fail(RFXH_EINVAL, "bad workgroup order");
-    if ((param & 4) && pl->n_mat > kTabLds)
-      return fail(RFXH_EINVAL, "material table too large to stage in LDS");
+    if (param < 0 || param > 1) return fail(RFXH_EINVAL, "bad workgroup order");
     const int old_order = pl->kp.order, old_ncol = pl->ncol;
-    pl->kp.order = param & 1;
-    pl->kp.nt = col ? (param >> 1) & 1 : 0;
-    pl->kp.lds_tab = col ? (param >> 2) & 1 : 0;
+    pl->kp.order = param;
     pl->ncol = kernel == 4 ? 2 : kernel == 5 ? 4 : kernel == 6 ? 3 : 1;
     int rc = plan_grid(pl);
     if (rc) {
@@ -2011,7 +1973,7 @@ int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
   *kernel = pl->march ? 2 + pl->march_tile
           : pl->fused ? 1
           : pl->ncol == 2 ? 4 : pl->ncol == 4 ? 5 : pl->ncol == 3 ? 6 : 0;
-  *param = pl->march ? pl->march_seg : pl->kp.order | (pl->kp.nt << 1) | (pl->kp.lds_tab << 2);
+  *param = pl->march ? pl->march_seg : pl->kp.order;
   return RFXH_OK;
 }
 
