@@ -931,7 +931,12 @@ def build_graded(spec, n_steps):
     dx = spec.get("dx")
     if dx is None:
         dx = C0 / spec["freq_max"] / 20.0
-    g = GradedGrid(spec["dz_profile"], dx, spec["domain"][:2], layers)
+    dz = spec.get("dz_profile")
+    if dz is None:
+        # rfx/runners/nonuniform.py:47-55: uniform z from the domain
+        dz = np.full(max(1, int(round(spec["domain"][2] / dx))), float(dx))
+    g = GradedGrid(dz, dx, spec["domain"][:2], layers, dx_profile=spec.get("dx_profile"),
+                   dy_profile=spec.get("dy_profile"))
     shape = g.shape
     coords = g.node_coords_f32()
     eps_r, sigma, mu_r, pec = assemble_materials(

@@ -206,13 +206,25 @@ class Simulation:
             raise NotImplementedError("only the 3-D second-order Yee solver is supported")
         if pec_faces:
             raise NotImplementedError("per-face PEC boundaries are not supported")
-        if dx_profile is not None or dy_profile is not None:
-            raise NotImplementedError("graded x/y profiles are not supported (graded z is)")
         self._freq_max = float(freq_max)
-        self._dz_profile = None if dz_profile is None else np.asarray(dz_profile, np.float64)
-        if self._dz_profile is not None:
-            domain = (domain[0], domain[1], float(np.sum(self._dz_profile)))
-        self._domain = tuple(float(d) for d in domain)
+        # profiles set the domain extents of their axes; dz only where the
+        # domain gives no z extent (rfx/api/__init__.py:434-476)
+        prof = [None if q is None else np.asarray(q, np.float64)
+                for q in (dx_profile, dy_profile, dz_profile)]
+        dom = list(domain) + [0.0] * (3 - len(domain))
+        if prof[0] is not None:
+            dom[0] = float(np.sum(prof[0]))
+        if prof[1] is not None:
+            dom[1] = float(np.sum(prof[1]))
+        if prof[2] is not None and dom[2] <= 0:
+            dom[2] = float(np.sum(prof[2]))
+        if (prof[0] is not None or prof[1] is not None) and dx is None:
+            raise ValueError("dx_profile / dy_profile require an explicit dx (boundary cell size)")
+        if any(d <= 0 for d in dom):
+            raise ValueError(f"domain dimensions must be positive, got {tuple(dom)}")
+        self._dx_profile, self._dy_profile, self._dz_profile = prof
+        self._graded = any(q is not None for q in prof)
+        self._domain = tuple(float(d) for d in dom)
         self._boundary = boundary
         self._cpml_layers = int(cpml_layers) if boundary == "cpml" else 0
         self._kappa_max = float(cpml_kappa_max)
@@ -307,7 +319,7 @@ class Simulation:
                              f"[0, {self._domain[a]}]")
         if dft_window not in ("rect", "hann", "tukey"):
             raise ValueError(f"dft_window must be 'rect', 'hann', or 'tukey', got {dft_window!r}")
-        if size is not None and self._dz_profile is not None:
+        if size is not None and self._graded:
             raise NotImplementedError("finite flux regions (size=...) are not supported on the "
                                       "graded mesh; use a full-plane monitor")
         if name is None:
@@ -321,7 +333,7 @@ class Simulation:
     def _flux_setup(self, grid, n_steps):
         """(Problem.fluxes entries, dA arrays) for the registered monitors
         (rfx/runners/uniform.py:66-137; rfx/runners/nonuniform.py:924-958)."""
-        graded = self._dz_profile is not None
+        graded = self._graded
         out, areas = [], []
         for e in self._flux_monitors:
             a = "xyz".index(e.axis)
@@ -366,16 +378,17 @@ class Simulation:
 
     # ---- set-up ---------------------------------------------------------------
     def _build_grid(self):
-        if self._dz_profile is not None:
+        if self._graded:
             from .nonuniform import make_nonuniform_grid
-            return make_nonuniform_grid(self._freq_max, self._domain[:2], self._dz_profile,
-                                        self._dx, self._cpml_layers)
+            return make_nonuniform_grid(self._freq_max, self._domain, self._dz_profile,
+                                        self._dx, self._cpml_layers, self._dx_profile,
+                                        self._dy_profile)
         return Grid(self._freq_max, self._domain, self._dx, self._cpml_layers)
 
     def _build_problem(self, n_steps: int, lane: str, eps_override=None,
                        sigma_override=None, mu_r_override=None, s11_freqs=None):
         grid = self._build_grid()
-        graded = self._dz_profile is not None
+        graded = self._graded
         shape = grid.shape
         mat = MaterialMap(shape)
         coords = grid.node_coords()
@@ -536,7 +549,7 @@ class Simulation:
         ``s_param_freqs`` gives the wire-port S matrix S[j, k] = b_j / a_k
         (rfx/nonuniform.py:1900-1935)."""
         _reject(unsupported, "run")
-        graded = self._dz_profile is not None
+        graded = self._graded
         grid = self._build_grid()
         if n_steps is None:
             n_steps = grid.num_timesteps(num_periods)

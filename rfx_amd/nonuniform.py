@@ -1,4 +1,4 @@
-"""Graded-z Yee grid: host set-up of the non-uniform lane.
+"""Graded Yee grid: host set-up of the non-uniform lane.
 
 Restates rfx/nonuniform.py:37-422 (NonUniformGrid, make_nonuniform_grid,
 _profile_to_inv_arrays, position_to_index) and the source / port metrics
@@ -25,20 +25,36 @@ AXIS = {"ex": 0, "ey": 1, "ez": 2}
 
 
 class GradedGrid:
-    """Uniform x/y (spacing dx), graded z (``dz_profile``, physical cells)."""
+    """Graded Yee grid: per-axis physical cell-size profiles; an axis given
+    no profile is uniform with spacing dx over its domain extent
+    (rfx/runners/nonuniform.py:25-60, rfx/nonuniform.py:274-422)."""
 
-    def __init__(self, freq_max: float, domain_xy, dz_profile, dx: float | None = None,
-                 cpml_layers: int = 8):
+    def __init__(self, freq_max: float, domain, dz_profile=None, dx: float | None = None,
+                 cpml_layers: int = 8, dx_profile=None, dy_profile=None):
         self.freq_max = float(freq_max)
+        if (dx_profile is not None or dy_profile is not None) and dx is None:
+            raise ValueError("dx_profile / dy_profile require an explicit dx (boundary cell size)")
         self.dx = float(dx) if dx is not None else C0 / self.freq_max / 20.0
-        self.dy = self.dx
         p = int(cpml_layers)
         self.cpml_layers = p
-        prof_xy = [np.full(int(round(float(domain_xy[a]) / self.dx)), self.dx) for a in range(2)]
-        dz = np.asarray(dz_profile, np.float64)
-        if dz.ndim != 1 or dz.size < 1 or np.any(dz <= 0):
-            raise ValueError("dz_profile must be a non-empty 1-D array of positive cell sizes")
-        self.d64 = [self._pad(a, p) for a in (prof_xy[0], prof_xy[1], dz)]
+
+        def prof(given, axis):
+            if given is None:
+                return np.full(max(1, int(round(float(domain[axis]) / self.dx))), self.dx)
+            a = np.asarray(given, np.float64)
+            if a.ndim != 1 or a.size < 1 or np.any(a <= 0):
+                raise ValueError(f"{'xyz'[axis]} profile must be a non-empty 1-D array of "
+                                 "positive cell sizes")
+            return a
+
+        px, py, pz = prof(dx_profile, 0), prof(dy_profile, 1), prof(dz_profile, 2)
+        # the CPML uses the boundary cell sizes (rfx/nonuniform.py:323-360)
+        if dx_profile is not None and (abs(px[0] - self.dx) > 1e-12 or abs(px[-1] - self.dx) > 1e-12):
+            raise ValueError(f"dx_profile must start and end with the boundary dx={self.dx}")
+        self.dy = float(py[0]) if dy_profile is not None else self.dx
+        if dy_profile is not None and abs(py[-1] - self.dy) > 1e-12:
+            raise ValueError("dy_profile boundary cells must match each other")
+        self.d64 = [self._pad(a, p) for a in (px, py, pz)]
         self.d32 = [d.astype(F32) for d in self.d64]
         self.shape = tuple(len(d) for d in self.d64)
         self.nx, self.ny, self.nz = self.shape
@@ -131,8 +147,9 @@ class GradedGrid:
                 f"dt={self.dt:.4e} s)")
 
 
-def make_nonuniform_grid(freq_max, domain_xy, dz_profile, dx=None, cpml_layers=8) -> GradedGrid:
-    return GradedGrid(freq_max, domain_xy, dz_profile, dx, cpml_layers)
+def make_nonuniform_grid(freq_max, domain, dz_profile=None, dx=None, cpml_layers=8,
+                         dx_profile=None, dy_profile=None) -> GradedGrid:
+    return GradedGrid(freq_max, domain, dz_profile, dx, cpml_layers, dx_profile, dy_profile)
 
 
 def current_source_scale(grid: GradedGrid, eps_r: float, sigma: float, idx, component: str,

@@ -86,8 +86,9 @@ def to_sim(spec, device=None):
     kw = {}
     if device is not None:
         kw["device"] = device
-    if spec.get("dz_profile") is not None:
-        kw["dz_profile"] = spec["dz_profile"]
+    for key in ("dx_profile", "dy_profile", "dz_profile"):
+        if spec.get(key) is not None:
+            kw[key] = spec[key]
     sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec.get("dx"),
                        boundary=spec.get("boundary", "cpml"),
                        cpml_layers=spec.get("cpml_layers", 16),
@@ -225,3 +226,27 @@ def box_total(fluxes):
     """Net outward flux of one box from its six monitor spectra (lo face
     first, then hi face, per axis)."""
     return sum((1 if q % 2 else -1) * f for q, f in enumerate(fluxes))
+
+
+def graded_xy_cavity_case(n_steps=8000):
+    """Air PEC cavity graded in x (4:1) and y (2:1), uniform z: TM110 depends
+    on both graded extents (tests/test_nonuniform_xy_cavity_accuracy.py of
+    the reference)."""
+    dx = 1e-3
+    px = smooth_grading([dx] * 17 + [0.25e-3] * 8 + [dx] * 17, 1.3)
+    py = smooth_grading([dx] * 15 + [0.5e-3] * 6 + [dx] * 15, 1.3)
+    a, b, d = float(px.sum()), float(py.sum()), 30e-3
+    f110 = (o.C0 / 2) * np.sqrt((1 / a) ** 2 + (1 / b) ** 2)
+    spec = dict(freq_max=2 * f110, domain=(a, b, d), dx=dx, dx_profile=px, dy_profile=py,
+                boundary="pec",
+                ports=[((a / 3, b / 3, d / 2), "ez", 0.0, o.Pulse(f110, 0.8), True)],
+                probes=[((2 * a / 3, 2 * b / 3, d / 2), "ez")])
+    return spec, n_steps, f110
+
+
+def mixed_graded_xy_case():
+    """The mixed graded case with x and y graded too (and z graded)."""
+    spec = mixed_graded_case()
+    spec["dx_profile"] = smooth_grading([1e-3] * 5 + [0.5e-3] * 4 + [1e-3] * 7, 1.3)
+    spec["dy_profile"] = smooth_grading([1e-3] * 4 + [0.4e-3] * 5 + [1e-3] * 5, 1.3)
+    return spec

@@ -70,3 +70,40 @@ def test_graded_cavity_vs_oracle_and_tm111(R):
     err = np.min(np.abs(fs - f111)) / f111
     print("TM111 rel err (float32 run)", err)
     assert err < 4e-4
+
+
+def test_mixed_graded_xy_vs_oracle(R):
+    _say("mixed case graded in x, y and z")
+    spec = cases.mixed_graded_xy_case()
+    n = 500
+    sim = cases.to_sim(spec)
+    res = sim.run(n_steps=n, s_param_freqs=spec["s11_freqs"])
+    g, cfg = o.build_graded(spec, n)
+    ref = cstep.run_c(cfg, n)
+    for p in range(ref.time_series.shape[1]):
+        e = cases.rel_l2(res.time_series[:, p], ref.time_series[:, p])
+        print("probe", p, e)
+        assert e <= TOL
+    for q, name in enumerate(res.dft_planes):
+        e = cases.rel_l2(res.dft_planes[name].accumulator.view(np.float64),
+                         ref.dft_planes[q].view(np.float64))
+        assert e <= TOL
+    s_ref = o.extract_lumped_s11(ref.port_v_dft[0], ref.port_i_dft[0], 50.0)
+    np.testing.assert_allclose(res.s_params[0, 0], s_ref, rtol=TOL)
+
+
+def test_graded_xy_cavity_tm110(R):
+    _say("x/y-graded PEC cavity x 8000 (TM110)")
+    spec, n, f110 = cases.graded_xy_cavity_case()
+    sim = cases.to_sim(spec)
+    res = sim.run(n_steps=n)
+    g, cfg = o.build_graded(spec, n)
+    ref = cstep.run_c(cfg, 2000, want_fields=False)
+    assert cases.rel_l2(res.time_series[:2000], ref.time_series) <= TOL
+    ts = res.time_series[:, 0].astype(np.float64)
+    n0 = int(np.ceil(2 * 3.0 / (f110 * 0.8 * np.pi) / g.dt))
+    f, _ = matrix_pencil_freqs(ts[n0::16], g.dt * 16, rel_tol=1e-6)
+    fs = f[(f > 0.6 * f110) & (f < 1.5 * f110)]
+    err = np.min(np.abs(fs - f110)) / f110
+    print("TM110 rel err (float32 run)", err)
+    assert err < 5e-4

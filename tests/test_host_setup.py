@@ -98,12 +98,14 @@ def test_eps_derivative_table_matches_finite_difference():
         np.testing.assert_allclose(tab[q], fd, rtol=1e-5, atol=1e-30)
 
 
-@pytest.mark.parametrize("name", ["mixed_graded", "cavity"])
+@pytest.mark.parametrize("name", ["mixed_graded", "cavity", "mixed_graded_xy", "xy_cavity"])
 def test_graded_setup_matches_oracle(name):
     """Graded-z lane: grid metrics, materials on the graded node
     coordinates, current-source tables, port folds / metrics, probes,
     planes and CPML tables equal the oracle's build_graded."""
-    spec = cases.mixed_graded_case() if name == "mixed_graded" else cases.graded_cavity_case()[0]
+    spec = {"mixed_graded": cases.mixed_graded_case, "mixed_graded_xy": cases.mixed_graded_xy_case,
+            "cavity": lambda: cases.graded_cavity_case()[0],
+            "xy_cavity": lambda: cases.graded_xy_cavity_case()[0]}[name]()
     n = 48
     sim = cases.to_sim(spec)
     s11 = None if spec.get("s11_freqs") is None else np.asarray(spec["s11_freqs"], np.float32)

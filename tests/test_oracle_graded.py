@@ -238,3 +238,42 @@ def test_uniform_profile_reduces_to_uniform_lane():
     ts_g = og.time_series * dx ** 3
     rel = np.linalg.norm(ts_g - ou.time_series) / np.linalg.norm(ou.time_series)
     assert rel < 1e-5
+
+
+def _tm110_error(g, cfg, n, f):
+    out = cstep.run_c(cfg, n, want_fields=False)
+    ts = out.time_series[:, 0]
+    n0 = int(np.ceil(2 * 3.0 / (f * 0.8 * np.pi) / g.dt))
+    fr, _ = matrix_pencil_freqs(ts[n0::16], g.dt * 16)
+    fs = fr[(fr > 0.6 * f) & (fr < 1.5 * f)]
+    by = sorted(fs, key=lambda x: abs(x - f))
+    return abs(by[0] - f) / f, abs(by[1] - f)
+
+
+def test_graded_xy_cavity_tm110_matches_closed_form():
+    """x- and y-graded air PEC cavity: TM110 within 0.05 % of
+    (c/2) sqrt((1/a)^2 + (1/b)^2), the reference's gate
+    (tests/test_nonuniform_xy_cavity_accuracy.py:100-125); measured 0.031 %."""
+    spec, n, f110 = cases.graded_xy_cavity_case()
+    g, cfg = o.build_graded(spec, n)
+    err, second = _tm110_error(g, cfg, n, f110)
+    print("TM110 rel err", err)
+    assert err < 5e-4 and second > 0.3e9
+
+
+def test_xy_cavity_gate_catches_a_missing_bounding_node_on_x():
+    """Planted mistake: the x axis without its duplicated bounding cell."""
+    spec, n, f110 = cases.graded_xy_cavity_case()
+    g, cfg = o.build_graded(spec, n)
+    bad = copy.copy(cfg)
+    bad.shape = (cfg.shape[0] - 1,) + cfg.shape[1:]
+    for k in ("eps_r", "sigma", "mu_r"):
+        setattr(bad, k, getattr(cfg, k)[:-1].copy())
+    dxa = g.d32[0][:-1]
+    inv_h = np.concatenate([(F32(1) / dxa)[:-1], np.zeros(1, F32)])
+    inv_e = np.concatenate([(F32(1) / dxa)[:1], F32(2) / (dxa[:-1] + dxa[1:])])
+    bad.spacing = dict(kind="graded", inv_e=[inv_e, g.inv_e[1], g.inv_e[2]],
+                       inv_h=[inv_h, g.inv_h[1], g.inv_h[2]])
+    bad.sources = [(s[0], s[1], s[2]) for s in cfg.sources]
+    err, _ = _tm110_error(g, bad, n, f110)
+    assert err > 5e-4 * 5
