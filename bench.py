@@ -137,6 +137,82 @@ def grad_cost(n_steps=3000):
             "ratio": round(t_grad / t_fwd, 3), "gradient_finite": finite}
 
 
+def c3_profile():
+    """dz (m) of the graded patch workload, 120 cells: 10 x 1.0 mm air,
+    13 cells geometric 1.0 -> 0.1 mm (ratio 1.194), 16 x 0.1 mm substrate,
+    13 cells geometric 0.1 -> 1.0 mm, 68 x 1.0 mm air."""
+    r = 0.1 ** (1.0 / 13.0)
+    down = [1e-3 * r ** k for k in range(1, 14)]
+    up = [1e-4 / r ** k for k in range(1, 14)]
+    dz = [1e-3] * 10 + down + [1e-4] * 16 + up + [1e-3] * 68
+    return np.asarray(dz)
+
+
+def c3_spec(n_xy=300):
+    """Graded patch antenna: dx = dy = 0.5 mm, n_xy x n_xy interior, graded
+    z (c3_profile), 12-layer CPML; FR4-like substrate (eps_r 4.4, sigma
+    0.002) between a PEC ground sheet and a 30 x 38 mm PEC patch sheet
+    (sigma >= 1e6 -> mask); a 50-ohm wire port from ground to patch
+    (1.6 mm), 1-4 GHz pulse (f0 2.5 GHz, bandwidth 1.2); S11 at 201
+    frequencies; 8000 steps."""
+    dx = 5e-4
+    dz = c3_profile()
+    L = n_xy * dx
+    z0 = float(np.sum(dz[:23]))           # substrate bottom node
+    z1 = z0 + 16 * 1e-4                   # substrate top node
+    c = L / 2
+    return dict(freq_max=4e9, domain=(L, L), dx=dx, dz_profile=dz, cpml_layers=12,
+                materials={"fr4": (4.4, 0.002, 1.0), "pec": (1.0, 1e10, 1.0)},
+                geometry=[((0.0, 0.0, z0), (L, L, z1), "fr4"),
+                          ((0.0, 0.0, z0), (L, L, z0), "pec"),
+                          ((c - 0.015, c - 0.019, z1), (c + 0.015, c + 0.019, z1), "pec")],
+                port=((c - 0.005, c, z0), 50.0, 1.6e-3, 2.5e9, 1.2),
+                s_freqs=np.linspace(1e9, 4e9, 201), probes=[((c + 0.02, c, z1 + 2e-3), "ez")])
+
+
+def build_c3_sim(spec):
+    import rfx_amd as R
+    sim = R.Simulation(spec["freq_max"], spec["domain"], dx=spec["dx"], boundary="cpml",
+                       cpml_layers=spec["cpml_layers"], dz_profile=spec["dz_profile"])
+    for name, (e, s_, m) in spec["materials"].items():
+        sim.add_material(name, eps_r=e, sigma=s_, mu_r=m)
+    for lo, hi, name in spec["geometry"]:
+        sim.add(R.Box(lo, hi), material=name)
+    pos, z0, ext, f0, bw = spec["port"]
+    sim.add_port(pos, "ez", impedance=z0, extent=ext,
+                 waveform=R.GaussianPulse(f0=f0, bandwidth=bw))
+    for pos, comp in spec["probes"]:
+        sim.add_probe(pos, comp)
+    return sim
+
+
+def graded_run(n_steps=8000):
+    """Throughput of the graded lane on the patch workload (whole run incl.
+    wire-port sampling; S11 from the port DFT)."""
+    import torch
+    from rfx_amd.engine import DeviceRun
+    spec = c3_spec()
+    sim = build_c3_sim(spec)
+    grid, prob, ports = sim._build_problem(n_steps, "run",
+                                           s11_freqs=np.asarray(spec["s_freqs"], np.float32))
+    dr = DeviceRun(prob, n_steps, "cuda:0", two_sets=False)
+    dr.run(0, min(n_steps, 64))
+    torch.cuda.synchronize()
+    dr.reset()
+    t0 = time.perf_counter()
+    dr.run(0, n_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    cells = int(np.prod(grid.shape))
+    finite = bool(torch.isfinite(dr.port_samples()).all())
+    dr.close()
+    return {"workload": f"graded patch {grid.shape[0]}x{grid.shape[1]}x{grid.shape[2]} cells, "
+                        f"dz {float(grid.d64[2].min()) * 1e3:.2f}-{float(grid.d64[2].max()) * 1e3:.2f}"
+                        f" mm, wire port, {n_steps} steps",
+            "Mcell_steps_per_s": round(n_steps * cells / el / 1e6, 1), "seconds": round(el, 3),
+            "port_samples_finite": finite}
+
+
 def c5_specs(n_inst=48):
     """Batched-sweep workload: 48 instances, interior sizes from {64^3, 96^3,
     128x128x64, 160x96x96, 192x192x48}, dx = 1 mm, 8-layer CPML, one
@@ -272,6 +348,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(N_STEPS))
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graded", action="store_true",
+                    help="skip the graded-lane timing")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-sweep timing")
     ap.add_argument("--no-grad", action="store_true",
@@ -381,6 +459,7 @@ def main():
         "cpu_baseline": None,
         "grad_vs_forward": None,
         "batch_sweep": None,
+        "graded_lane": None,
     }
     dr.close()
     del dr
@@ -389,6 +468,10 @@ def main():
         print("[bench] forward+adjoint vs forward ...", file=sys.stderr, flush=True)
         out["grad_vs_forward"] = grad_cost()
         print(f"[bench] grad {out['grad_vs_forward']}", file=sys.stderr, flush=True)
+    if not args.no_graded and args.config == "c2":
+        print("[bench] graded lane ...", file=sys.stderr, flush=True)
+        out["graded_lane"] = graded_run()
+        print(f"[bench] graded {out['graded_lane']}", file=sys.stderr, flush=True)
     if not args.no_batch and args.config == "c2":
         print("[bench] batched sweep ...", file=sys.stderr, flush=True)
         out["batch_sweep"] = batch_sweep()

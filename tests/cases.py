@@ -250,3 +250,30 @@ def mixed_graded_xy_case():
     spec["dx_profile"] = smooth_grading([1e-3] * 5 + [0.5e-3] * 4 + [1e-3] * 7, 1.3)
     spec["dy_profile"] = smooth_grading([1e-3] * 4 + [0.4e-3] * 5 + [1e-3] * 5, 1.3)
     return spec
+
+
+def patch_dz():
+    """Graded dz of the patch workload: 10 x 1 mm, 13 cells 1 -> 0.1 mm
+    (ratio 1.194), 16 x 0.1 mm substrate, 13 cells 0.1 -> 1 mm, 68 x 1 mm."""
+    r = 0.1 ** (1.0 / 13.0)
+    return np.asarray([1e-3] * 10 + [1e-3 * r ** k for k in range(1, 14)] + [1e-4] * 16
+                      + [1e-4 / r ** k for k in range(1, 14)] + [1e-3] * 68)
+
+
+def patch_case(n_xy=60):
+    """The graded patch workload on a smaller x/y extent (oracle spec)."""
+    dx = 5e-4
+    dz = patch_dz()
+    L = n_xy * dx
+    z0 = float(np.sum(dz[:23]))
+    z1 = z0 + 16 * 1e-4
+    c = L / 2
+    return dict(freq_max=4e9, domain=(L, L), dx=dx, dz_profile=dz, boundary="cpml",
+                cpml_layers=12,
+                materials={"fr4": (4.4, 0.002, 1.0), "pec": (1.0, 1e10, 1.0)},
+                geometry=[((0.0, 0.0, z0), (L, L, z1), "fr4"),
+                          ((0.0, 0.0, z0), (L, L, z0), "pec"),
+                          ((c - 0.006, c - 0.007, z1), (c + 0.006, c + 0.007, z1), "pec")],
+                ports=[((c - 0.003, c, z0), "ez", 50.0, o.Pulse(2.5e9, 1.2), True, 1.6e-3)],
+                probes=[((c + 0.008, c, z1 + 2e-3), "ez")],
+                s11_freqs=np.linspace(1e9, 4e9, 21))

@@ -107,3 +107,22 @@ def test_graded_xy_cavity_tm110(R):
     err = np.min(np.abs(fs - f110)) / f110
     print("TM110 rel err (float32 run)", err)
     assert err < 5e-4
+
+
+def test_patch_workload_vs_oracle(R):
+    """The graded patch workload (PEC ground and patch sheets, FR4, wire
+    port through the substrate) at 60 x 60 interior cells."""
+    _say("graded patch 85x85x145 x 600")
+    spec = cases.patch_case(60)
+    n = 600
+    sim = cases.to_sim(spec)
+    res = sim.run(n_steps=n, s_param_freqs=spec["s11_freqs"])
+    g, cfg = o.build_graded(spec, n)
+    ref = cstep.run_c(cfg, n, want_fields=False)
+    e = cases.rel_l2(res.time_series, ref.time_series)
+    print("patch probe rel-L2", e)
+    assert e <= TOL
+    s_ref = o.extract_lumped_s11(ref.port_v_dft[0], ref.port_i_dft[0], 50.0)
+    err = np.abs(res.s_params[0, 0] - s_ref) / np.abs(s_ref)
+    print("patch S11 max rel err", err.max())
+    assert err.max() <= 1e-3
