@@ -543,11 +543,14 @@ class Simulation:
 
     # ---- run / forward -----------------------------------------------------------
     def run(self, n_steps: int | None = None, *, num_periods: float = 20.0,
-            use_graph: bool = True, s_param_freqs=None, **unsupported):
+            use_graph: bool = True, s_param_freqs=None, autotune: bool = False,
+            **unsupported):
         """Run the time loop (rfx/api/_execute.py:2862 -> runners/uniform.py:140,
         graded z -> runners/nonuniform.py:465).  On the graded lane
         ``s_param_freqs`` gives the wire-port S matrix S[j, k] = b_j / a_k
-        (rfx/nonuniform.py:1900-1935)."""
+        (rfx/nonuniform.py:1900-1935).  ``autotune=True`` first picks the
+        fastest kernel variant for this problem (a few timed steps, then a
+        restart from zero)."""
         _reject(unsupported, "run")
         graded = self._graded
         grid = self._build_grid()
@@ -561,6 +564,11 @@ class Simulation:
             sf = np.asarray(s_param_freqs, F32)
         grid, prob, ports = self._build_problem(int(n_steps), "run", s11_freqs=sf)
         dr = DeviceRun(prob, int(n_steps), self._device)
+        if autotune:
+            # time the kernel variants on this problem's first steps, keep
+            # the fastest, restart from a zero state (rfx_amd/autotune.py)
+            from .autotune import autotune as _tune
+            _tune(dr, trial_steps=min(32, int(n_steps)))
         dr.run(0, int(n_steps), use_graph=use_graph)
         torch.cuda.synchronize(dr.device)
         ts = dr.time_series().cpu().numpy().astype(np.float32)

@@ -1351,7 +1351,7 @@ struct rfxh_plan {
   bool march = false;           // the ping-pong step uses the marching kernel
   int march_seg = 0;            // x segments of the marching kernel (0: automatic)
   int march_tile = 0;           // index into kMarchRows
-  int ncol = 1;                 // two-pass step: cells per thread along x (1, 2, 4)
+  int ncol = 1;                 // two-pass step: cells per thread along x (1, 2, 3, 4)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   int cur = 0;                  // which set holds the current state
@@ -1653,12 +1653,15 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     if ((rc = dev_copy(pl, d->mat_flags, (size_t)d->n_mat, &fl))) return bail(rc);
     kp.flags = fl;
   }
-  // default: the two-pass step (measured faster than the fused step on
-  // MI355X at 533^3: the fused kernel's three redundant H evaluations make it
-  // VALU-bound); RFXH_KERNEL=fused selects the fused step when two sets exist
+  // default: the two-pass step with 2-cell columns along x, the fastest
+  // variant measured on MI355X at 533^3 and on the graded workloads;
+  // RFXH_KERNEL = onecell | fused | march selects another (fused and march
+  // need the second field set); rfxh_set_variant changes it later
   const char* kern = std::getenv("RFXH_KERNEL");
-  pl->fused = has2 && kern && (std::string(kern) == "fused" || std::string(kern) == "march");
-  pl->march = has2 && kern && std::string(kern) == "march";
+  const std::string kname = kern ? kern : "";
+  pl->fused = has2 && (kname == "fused" || kname == "march");
+  pl->march = has2 && kname == "march";
+  pl->ncol = (pl->fused || kname == "onecell") ? 1 : 2;
   for (int c = 0; c < 6; ++c) {
     pl->set[0][c] = d->d_f[c];
     pl->set[1][c] = has2 ? d->d_f2[c] : nullptr;

@@ -164,9 +164,12 @@ def test_fused_step_matches_two_pass(R):
     sim = cases.to_sim(spec)
     n = 150
     _, prob, _ = sim._build_problem(n, "run")
+    from rfx_amd import autotune as at
     out = []
     for two in (True, False):
         dr = DeviceRun(prob, n, two_sets=two)
+        if two:
+            at.set_variant(dr, 1, 1)                      # fused single-launch step
         dr.run(0, n)
         torch.cuda.synchronize()
         out.append(([f.cpu().numpy().copy() for f in dr.fields],
