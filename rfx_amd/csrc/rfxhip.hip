@@ -1435,6 +1435,40 @@ void launch_accumulators(rfxh_plan* pl, hipStream_t s, int set) {
   }
 }
 
+template <int PASS, bool GRADED, int NP>
+void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps) {
+  const dim3 g = NP == 1 ? pl->grid : pl->grid_col, blk = pl->block;
+  if (NP == 1) {
+    if (PASS == 0) hipLaunchKernelGGL(k_update_h<GRADED>, g, blk, 0, s, kp, ps);
+    else hipLaunchKernelGGL(k_update_e<GRADED>, g, blk, 0, s, kp, ps);
+  } else {
+    if (PASS == 0) hipLaunchKernelGGL((k_update_h_col<GRADED, NP>), g, blk, 0, s, kp, ps);
+    else hipLaunchKernelGGL((k_update_e_col<GRADED, NP>), g, blk, 0, s, kp, ps);
+  }
+}
+
+template <int PASS, bool GRADED>
+void launch_pass_g(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps) {
+  switch (pl->ncol) {
+    case 2: launch_pass_n<PASS, GRADED, 2>(pl, s, kp, ps); break;
+    case 3: launch_pass_n<PASS, GRADED, 3>(pl, s, kp, ps); break;
+    case 4: launch_pass_n<PASS, GRADED, 4>(pl, s, kp, ps); break;
+    default: launch_pass_n<PASS, GRADED, 1>(pl, s, kp, ps); break;
+  }
+}
+
+// One two-pass half step (pass 0: H, pass 1: E) in place on kp.f, with the
+// plan's column width.
+void launch_pass(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps, int pass) {
+  if (pass == 0) {
+    if (kp.graded) launch_pass_g<0, true>(pl, s, kp, ps);
+    else launch_pass_g<0, false>(pl, s, kp, ps);
+  } else {
+    if (kp.graded) launch_pass_g<1, true>(pl, s, kp, ps);
+    else launch_pass_g<1, false>(pl, s, kp, ps);
+  }
+}
+
 // One time step starting from field set `from` (fused: writes set 1-from).
 int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   // ev (optional): four events bracketing the main kernel(s) and the DFT
@@ -1488,34 +1522,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   } else {
     KParams kp = pl->kp;
     for (int c = 0; c < 6; ++c) kp.f[c] = pl->set[from][c];
-    const dim3 g = pl->ncol == 1 ? pl->grid : pl->grid_col, blk = pl->block;
-    if (pl->ncol == 1) {
-      if (kp.graded) hipLaunchKernelGGL(k_update_h<true>, g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL(k_update_h<false>, g, blk, 0, s, kp, ps);
-    } else if (pl->ncol == 2) {
-      if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 2>), g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL((k_update_h_col<false, 2>), g, blk, 0, s, kp, ps);
-    } else if (pl->ncol == 3) {
-      if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 3>), g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL((k_update_h_col<false, 3>), g, blk, 0, s, kp, ps);
-    } else {
-      if (kp.graded) hipLaunchKernelGGL((k_update_h_col<true, 4>), g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL((k_update_h_col<false, 4>), g, blk, 0, s, kp, ps);
-    }
+    launch_pass(pl, s, kp, ps, 0);
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    if (pl->ncol == 1) {
-      if (kp.graded) hipLaunchKernelGGL(k_update_e<true>, g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL(k_update_e<false>, g, blk, 0, s, kp, ps);
-    } else if (pl->ncol == 2) {
-      if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 2>), g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL((k_update_e_col<false, 2>), g, blk, 0, s, kp, ps);
-    } else if (pl->ncol == 3) {
-      if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 3>), g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL((k_update_e_col<false, 3>), g, blk, 0, s, kp, ps);
-    } else {
-      if (kp.graded) hipLaunchKernelGGL((k_update_e_col<true, 4>), g, blk, 0, s, kp, ps);
-      else hipLaunchKernelGGL((k_update_e_col<false, 4>), g, blk, 0, s, kp, ps);
-    }
+    launch_pass(pl, s, kp, ps, 1);
   }
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
   launch_accumulators(pl, s, to);
@@ -1998,15 +2007,12 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
   const PsiIO ps = psi_io(pl, pl->cur);
   const unsigned rb = (unsigned)((rp.n + 255) / 256);
   for (int64_t n = 0; n < n_steps; ++n) {
-    if (kp.graded) {
-      hipLaunchKernelGGL(k_update_h<true>, pl->grid, pl->block, 0, s, kp, ps);
+    launch_pass(pl, s, kp, ps, 0);
+    if (kp.graded)
       hipLaunchKernelGGL(k_record<true>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
-      hipLaunchKernelGGL(k_update_e<true>, pl->grid, pl->block, 0, s, kp, ps);
-    } else {
-      hipLaunchKernelGGL(k_update_h<false>, pl->grid, pl->block, 0, s, kp, ps);
+    else
       hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
-      hipLaunchKernelGGL(k_update_e<false>, pl->grid, pl->block, 0, s, kp, ps);
-    }
+    launch_pass(pl, s, kp, ps, 1);
     launch_accumulators(pl, s, pl->cur);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
   }
