@@ -162,8 +162,9 @@ int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* cou
  * order, 1 flat, 0 XCD bands); 1: fused single-launch step (same param);
  * 2 / 3: marching step with 6- / 14-row tiles (param = number of x
  * segments, 0 = automatic); 4 / 6 / 5: two-pass step with columns of
- * 2 / 3 / 4 cells along x per thread (param = workgroup order).  Kernels 1-3
- * need the second field set.
+ * 2 / 3 / 4 cells along x per thread (param = workgroup order); 7: wave-march
+ * step, each wavefront walking one row along x with no barriers (param = x
+ * segments, 0 = automatic).  Kernels 1-3 and 7 need the second field set.
  * Variants compute the same step; they differ in speed only.  The
  * project's autotuner (rfx_amd/autotune.py) times them on the actual
  * problem.  No counterpart in the reference (its XLA lowering has no

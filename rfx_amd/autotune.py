@@ -3,7 +3,8 @@
 The plan's step can run as the two-pass kernels (H pass, E pass, in place;
 one cell per thread or columns of 2, 3 or 4 cells along x) or, with a
 second field set, as one fused kernel (flat or XCD-band
-workgroup order) or as the marching kernel (6- or 14-row tiles of the y-z
+workgroup order), as the barrier-free wave-march kernel (each wavefront
+walks one row along x) or as the marching kernel (6- or 14-row tiles of the y-z
 plane walking along x, in a chosen number of x segments) (include/rfxhip.h
 ``rfxh_set_variant``).  All
 variants compute the same step.  Which is fastest depends on the grid (CPML
@@ -43,7 +44,7 @@ def variants(dr):
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
         nx = dr.prob.shape[0]
-        for kern in (2, 3):
+        for kern in (2, 3, 7):
             out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32) if s <= max(1, nx // 4)]
     return out
 
@@ -53,8 +54,10 @@ ORDERED = {0: "two-pass", 1: "fused", 4: "col2", 5: "col4", 6: "col3"}
 
 
 def variant_name(v) -> str:
-    """two-pass-flat, col2-bands, col3-flat, fused-flat, march14-16, ..."""
+    """two-pass-flat, col2-bands, col3-flat, fused-flat, march14-16, wmarch-8, ..."""
     k, prm = v
+    if k == 7:
+        return "wmarch" if prm == 0 else f"wmarch-{prm}"
     if k in MARCH_ROWS:
         base = f"march{MARCH_ROWS[k]}"
         return base if prm == 0 else f"{base}-{prm}"
@@ -62,6 +65,9 @@ def variant_name(v) -> str:
 
 
 def parse_variant(name: str):
+    if name.startswith("wmarch"):
+        _, _, seg = name.partition("-")
+        return (7, int(seg) if seg else 0)
     if name.startswith("march"):
         head, _, seg = name.partition("-")
         rows = int(head[5:])

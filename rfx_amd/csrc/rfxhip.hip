@@ -661,6 +661,151 @@ __global__ __launch_bounds__(kBX * (MJ + 2)) void k_step_march(KParams p, MarchI
 }
 
 // ---------------------------------------------------------------------------
+// Wave-march step: read set A, write set B, no barriers.  Each wavefront owns
+// one row (y) of 63 cells (z) plus a halo lane at k0-1 and walks along x
+// through its segment of planes.  Per plane it computes H on its own row and,
+// again, on the row below (j-1, the E update's backward-y neighbour), takes
+// the backward-z neighbour H from the next lower lane (cross-lane shuffle)
+// and the backward-x neighbour from its registers, then updates E.  Waves
+// never wait for each other.
+// ---------------------------------------------------------------------------
+constexpr int kWMW = 4;              // independent wavefronts per workgroup
+constexpr int kWMK = kBX - 1;        // interior columns per wave
+
+struct WMarchIO {
+  const float* eo[3];
+  const float* ho[3];
+  float* en[3];
+  float* hn[3];
+  int ntk, ntj, seg_len;
+};
+
+template <bool GRADED>
+__global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO io, PsiIO ps) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int tk = L % io.ntk;
+  const int r = L / io.ntk;
+  const int tj = r % io.ntj;
+  const int seg = r / io.ntj;
+  const int l = (int)threadIdx.x;
+  const int j = tj * kWMW + (int)threadIdx.y;
+  const int k = tk * kWMK + l - 1;
+  const int i0 = seg * io.seg_len;
+  const int i1 = min(p.nx, i0 + io.seg_len);
+  const bool vrow = j < p.ny;
+  const bool valid = vrow && k >= 0 && k < p.nz;
+  const bool valid_m = valid && j > 0;          // the row below exists
+  const bool inner = valid && l > 0;
+  const unsigned P = (unsigned)p.plane;
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
+  const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
+  const unsigned Z = (unsigned)p.nz;
+  const unsigned dj = hj ? Z : 0u, dk = hk ? 1u : 0u;
+  // E^n of plane i at the own row (a*) and the row below (b*)
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f;
+  float hyp = 0.f, hzp = 0.f;                   // own-row H^{n+1/2} of plane i-1
+  if (valid && i0 < i1) {
+    const unsigned c = (unsigned)i0 * P + c0;
+    a0 = ldf(io.eo[0], c);
+    a1 = ldf(io.eo[1], c);
+    a2 = ldf(io.eo[2], c);
+    if (valid_m) {
+      b0 = ldf(io.eo[0], c - Z);
+      b1 = ldf(io.eo[1], c - Z);
+      b2 = ldf(io.eo[2], c - Z);
+    }
+  }
+  if (valid && i0 > 0 && i0 < i1) {
+    // own-row H of plane i0-1 (owned by the previous segment): recomputed
+    const unsigned cm = (unsigned)(i0 - 1) * P + c0;
+    const unsigned m = ldu16(p.mat, cm);
+    float ex_j = ldf(io.eo[0], cm + dj), ez_j = ldf(io.eo[2], cm + dj);
+    float ex_k = ldf(io.eo[0], cm + dk), ey_k = ldf(io.eo[1], cm + dk);
+    ex_j = hj ? ex_j : 0.f;
+    ez_j = hj ? ez_j : 0.f;
+    ex_k = hk ? ex_k : 0.f;
+    ey_k = hk ? ey_k : 0.f;
+    const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, p.coef[m].w, ldf(io.eo[0], cm),
+                                           ldf(io.eo[1], cm), ldf(io.eo[2], cm), ex_j, ez_j, ex_k,
+                                           ey_k, a1, a2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
+                                           ldf(io.ho[2], cm));
+    hyp = h.y;
+    hzp = h.z;
+  }
+  for (int i = i0; i < i1; ++i) {
+    const unsigned c = (unsigned)i * P + c0;
+    const bool hi = i + 1 < p.nx;
+    // E^n of plane i+1 (own row, row below), next to-be-current values
+    float n0 = 0.f, n1 = 0.f, n2 = 0.f, q0 = 0.f, q1 = 0.f, q2 = 0.f;
+    float3 h = make_float3(0.f, 0.f, 0.f), hm = make_float3(0.f, 0.f, 0.f);
+    unsigned m = 0;
+    // backward-z H comes from lane l-1 and forward-z E from lane l+1; lanes
+    // past the grid carry zeros and lane 63 loads its forward-z E itself
+    const float up_ex = __shfl_down(a0, 1), up_ey = __shfl_down(a1, 1);
+    const float upm_ex = __shfl_down(b0, 1), upm_ey = __shfl_down(b1, 1);
+    if (valid) {
+      m = ldu16(p.mat, c);
+      if (hi) {
+        n0 = ldf(io.eo[0], c + P);
+        n1 = ldf(io.eo[1], c + P);
+        n2 = ldf(io.eo[2], c + P);
+      }
+      float ex_j = ldf(io.eo[0], c + dj), ez_j = ldf(io.eo[2], c + dj);
+      ex_j = hj ? ex_j : 0.f;
+      ez_j = hj ? ez_j : 0.f;
+      float ex_k = up_ex, ey_k = up_ey;
+      if (l == kBX - 1) {
+        ex_k = ldf(io.eo[0], c + dk);
+        ey_k = ldf(io.eo[1], c + dk);
+      }
+      ex_k = hk ? ex_k : 0.f;
+      ey_k = hk ? ey_k : 0.f;
+      h = h_core<GRADED, true>(p, ps, i, j, k, p.coef[m].w, a0, a1, a2, ex_j, ez_j, ex_k, ey_k, n1,
+                               n2, ldf(io.ho[0], c), ldf(io.ho[1], c), ldf(io.ho[2], c), inner);
+      if (valid_m) {
+        const unsigned cm = c - Z;
+        const unsigned mm = ldu16(p.mat, cm);
+        if (hi) {
+          q0 = ldf(io.eo[0], cm + P);
+          q1 = ldf(io.eo[1], cm + P);
+          q2 = ldf(io.eo[2], cm + P);
+        }
+        float bx_k = upm_ex, by_k = upm_ey;
+        if (l == kBX - 1) {
+          bx_k = ldf(io.eo[0], cm + dk);
+          by_k = ldf(io.eo[1], cm + dk);
+        }
+        bx_k = hk ? bx_k : 0.f;
+        by_k = hk ? by_k : 0.f;
+        // row below: its forward-y neighbour is the own row
+        hm = h_core<GRADED, false>(p, ps, i, j - 1, k, p.coef[mm].w, b0, b1, b2, a0, a2, bx_k,
+                                   by_k, q1, q2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
+                                   ldf(io.ho[2], cm));
+      }
+    }
+    const float hx_k = __shfl_up(h.x, 1), hy_k = __shfl_up(h.y, 1);
+    if (inner) {
+      const float3 e = e_core<GRADED>(p, ps, i, j, k, c, m, a0, a1, a2, h.x, h.y, h.z, hyp, hzp,
+                                      hm.x, hm.z, hx_k, hy_k);
+      stf(io.hn[0], c, h.x);
+      stf(io.hn[1], c, h.y);
+      stf(io.hn[2], c, h.z);
+      stf(io.en[0], c, e.x);
+      stf(io.en[1], c, e.y);
+      stf(io.en[2], c, e.z);
+    }
+    hyp = h.y;
+    hzp = h.z;
+    a0 = n0;
+    a1 = n1;
+    a2 = n2;
+    b0 = q0;
+    b1 = q1;
+    b2 = q2;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Two-pass step, in place on p.f.
 // ---------------------------------------------------------------------------
 template <bool GRADED>
@@ -1352,6 +1497,8 @@ struct rfxh_plan {
   int march_seg = 0;            // x segments of the marching kernel (0: automatic)
   int march_tile = 0;           // index into kMarchRows
   int ncol = 1;                 // two-pass step: cells per thread along x (1, 2, 3, 4)
+  bool wmarch = false;          // the ping-pong step uses the wave-march kernel
+  int wmarch_seg = 0;           // its x segments (0: automatic)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   int cur = 0;                  // which set holds the current state
@@ -1486,7 +1633,32 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
       ps.ew[f][t] = e0 ? e0 + wo : nullptr;
     }
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
-  if (pl->march) {
+  if (pl->wmarch) {
+    WMarchIO io;
+    for (int a = 0; a < 3; ++a) {
+      io.eo[a] = pl->set[from][a];
+      io.ho[a] = pl->set[from][3 + a];
+      io.en[a] = pl->set[to][a];
+      io.hn[a] = pl->set[to][3 + a];
+    }
+    const KParams& kp = pl->kp;
+    io.ntj = (kp.ny + kWMW - 1) / kWMW;
+    io.ntk = (kp.nz + kWMK - 1) / kWMK;
+    const long long tiles = (long long)io.ntj * io.ntk;
+    int nseg = pl->wmarch_seg;
+    if (nseg <= 0) {
+      // about 32 K wavefronts, at least four planes per segment
+      nseg = (int)std::max<long long>(1, (8192 + tiles - 1) / tiles);
+      nseg = std::min(nseg, std::max(1, kp.nx / 4));
+    }
+    nseg = std::min(nseg, kp.nx);
+    io.seg_len = (kp.nx + nseg - 1) / nseg;
+    nseg = (kp.nx + io.seg_len - 1) / io.seg_len;
+    const dim3 g((unsigned)(tiles * nseg)), blk(kBX, kWMW);
+    if (kp.graded) hipLaunchKernelGGL(k_step_wmarch<true>, g, blk, 0, s, kp, io, ps);
+    else hipLaunchKernelGGL(k_step_wmarch<false>, g, blk, 0, s, kp, io, ps);
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
+  } else if (pl->march) {
     MarchIO io;
     for (int a = 0; a < 3; ++a) {
       io.eo[a] = pl->set[from][a];
@@ -1950,13 +2122,18 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
 }
 
 int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
-  if (!pl || kernel < 0 || kernel > 6) return fail(RFXH_EINVAL, "bad variant");
-  if ((kernel == 1 || kernel == 2 || kernel == 3) && !pl->set[1][0])
+  if (!pl || kernel < 0 || kernel > 7) return fail(RFXH_EINVAL, "bad variant");
+  const bool pingpong = kernel == 1 || kernel == 2 || kernel == 3 || kernel == 7;
+  if (pingpong && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
-  if (kernel == 2 || kernel == 3) {
+  if (kernel == 2 || kernel == 3 || kernel == 7) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
-    pl->march_seg = param;
-    pl->march_tile = kernel - 2;
+    if (kernel == 7) {
+      pl->wmarch_seg = param;
+    } else {
+      pl->march_seg = param;
+      pl->march_tile = kernel - 2;
+    }
   } else {
     if (param < 0 || param > 1) return fail(RFXH_EINVAL, "bad workgroup order");
     const int old_order = pl->kp.order, old_ncol = pl->ncol;
@@ -1970,8 +2147,9 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
       return rc;
     }
   }
-  pl->fused = kernel == 1 || kernel == 2 || kernel == 3;
+  pl->fused = pingpong;
   pl->march = kernel == 2 || kernel == 3;
+  pl->wmarch = kernel == 7;
   for (int g = 0; g < 2; ++g)
     if (pl->graph_exec[g]) {
       (void)hipGraphExecDestroy(pl->graph_exec[g]);
@@ -1982,10 +2160,11 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
 
 int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
   if (!pl || !kernel || !param) return fail(RFXH_EINVAL, "bad arguments");
-  *kernel = pl->march ? 2 + pl->march_tile
+  *kernel = pl->wmarch ? 7
+          : pl->march ? 2 + pl->march_tile
           : pl->fused ? 1
           : pl->ncol == 2 ? 4 : pl->ncol == 4 ? 5 : pl->ncol == 3 ? 6 : 0;
-  *param = pl->march ? pl->march_seg : pl->kp.order;
+  *param = pl->wmarch ? pl->wmarch_seg : pl->march ? pl->march_seg : pl->kp.order;
   return RFXH_OK;
 }
 
