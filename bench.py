@@ -252,6 +252,8 @@ def batch_sweep():
     steps = [n for _, n in specs]
     b = Batch(sims, steps, "cuda:0")
     cell_steps = sum(int(np.prod(r.prob.shape)) * n for r, n in zip(b.runs, steps))
+    from rfx_amd import autotune as at
+    tune = at.autotune_batch(b, trial_steps=64)
     b.run()                                   # warm-up: graphs captured
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -259,6 +261,7 @@ def batch_sweep():
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     for r, n in zip(b.runs, steps):           # warm-up of the per-instance graphs
+        at.set_variant(r, 4, 1)               # each instance on its own default step
         r.reset()
         r.run(0, min(n, 64))
     torch.cuda.synchronize()
@@ -287,6 +290,7 @@ def batch_sweep():
     torch.cuda.empty_cache()
     return {"workload": f"{len(specs)} instances, {min(steps)}-{max(steps)} steps, "
                         f"{cell_steps / 1e9:.1f} G cell-steps",
+            "batched_variant": tune["variant"],
             "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
             "batched_Mcell_steps_per_s": round(cell_steps / t_batch / 1e6, 1),
             "sequential_Mcell_steps_per_s": round(cell_steps / t_seq / 1e6, 1),
@@ -418,13 +422,13 @@ def main():
     ms_e, n_e = dr.kernel_time_ms(1)
     ms_d, n_d = dr.kernel_time_ms(2)
     value = args.steps * n_total * cells / el / 1e6
-    if kern_v in (1, 2, 3):
+    if kern_v in (1, 2, 3, 7):
         # one launch does the whole step: 48 B per cell
-        which = "k_step_march" if kern_v in (2, 3) else "k_step_fused"
+        which = {1: "k_step_fused", 2: "k_step_march", 3: "k_step_march", 7: "k_step_wmarch"}[kern_v]
         ms, unit_bytes = ms_h, BYTES_PER_CELL_STEP * cells
     else:
         # two passes, each advancing half the field components: 24 B per cell
-        sfx = "_col" if kern_v in (4, 5) else ""
+        sfx = "_col" if kern_v in (4, 5, 6) else ""
         which, ms = (f"k_update_e{sfx}", ms_e) if ms_e >= ms_h else (f"k_update_h{sfx}", ms_h)
         unit_bytes = 0.5 * BYTES_PER_CELL_STEP * cells
     achieved = unit_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None

@@ -153,8 +153,8 @@ int rfxh_set_state_set(rfxh_plan* plan, int which);
 /* Average duration in milliseconds of the kernels of the sampled steps of
  * runs made with timing enabled (rfxh_set_timing(plan, 1), which also resets
  * the sums), measured with HIP events on the stream the kernels run on.
- * which: 0 = fused step kernel (two-pass: H pass), 1 = E pass (two-pass
- * only), 2 = DFT-plane kernels. */
+ * which: 0 = single-launch step kernel (two-pass: H pass), 1 = E pass
+ * (two-pass only), 2 = DFT-plane kernels. */
 int rfxh_set_timing(rfxh_plan* plan, int enable);
 int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* count);
 
@@ -220,9 +220,11 @@ typedef struct rfxh_batch rfxh_batch;
 int rfxh_batch_create(rfxh_plan* const* plans, int n_plans, rfxh_batch** out);
 int rfxh_batch_destroy(rfxh_batch* batch);
 /* Instance q advances n_steps[q] steps from absolute step step0[q]
- * (two-pass sub-steps on its current field set).  use_graph != 0 replays
- * captured chunks of steps wherever the set of active instances is
- * constant. */
+ * (two-pass sub-steps on its current field set, each instance in its
+ * plan's workgroup order; 2 cells per thread along x when every instance's
+ * plan variant is the 2-cell column step, kernel 4, else one cell per
+ * thread).  use_graph != 0 replays captured chunks of steps wherever the
+ * set of active instances is constant. */
 int rfxh_batch_run(rfxh_batch* batch, const int64_t* step0, const int64_t* n_steps, void* stream,
                    int use_graph);
 

@@ -35,15 +35,17 @@ def _key(dr) -> str:
 
 
 def variants(dr):
-    """(kernel, parameter) candidates: kernel 0 two-pass and 1 fused with
-    workgroup order 1 (flat) or 0 (XCD bands); kernel 2 marching with its
-    number of x segments (0 = automatic)."""
+    """(kernel, parameter) candidates: two-pass kernels 0 (one cell per
+    thread) and 4 / 6 / 5 (columns of 2 / 3 / 4 cells) with workgroup order
+    1 (flat) or 0 (XCD bands); with two field sets also the fused step 1
+    (same orders) and the marching steps 2 / 3 (6- / 14-row tiles) and the
+    wave-march 7 with their number of x segments (0 = automatic)."""
     out = [(0, 1), (0, 0)]
     for kern in (4, 6, 5):                  # columns of 2, 3, 4 cells
         out += [(kern, order) for order in (1, 0)]
+    nx = dr.prob.shape[0]
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
-        nx = dr.prob.shape[0]
         for kern in (2, 3, 7):
             out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32) if s <= max(1, nx // 4)]
     return out
@@ -132,17 +134,18 @@ def autotune(dr, trial_steps: int = 32, repeats: int = 2, cache_file: str | None
 
 
 def autotune_batch(batch, trial_steps: int = 32):
-    """Pick the workgroup order (flat or XCD bands) for every instance of an
-    rfx_amd.batch.Batch by timing ``trial_steps`` batched steps each; the
-    batch step is always two-pass.  Leaves the instances reset."""
+    """Pick the batched step's layout for every instance of an
+    rfx_amd.batch.Batch -- one or 2 cells per thread along x (two-pass
+    variants 0 and 4), flat or XCD-band workgroup order -- by timing
+    ``trial_steps`` batched steps of each.  Leaves the instances reset."""
     n = len(batch.runs)
     step0 = (C.c_int64 * n)(*([0] * n))
     counts = (C.c_int64 * n)(*[min(trial_steps, s) for s in batch.n_steps])
     stream = torch.cuda.current_stream(batch.device)
     timings = {}
-    for order in (1, 0):
+    for cand in ((0, 1), (0, 0), (4, 1), (4, 0)):
         for r in batch.runs:
-            set_variant(r, 0, order)
+            set_variant(r, *cand)
         for rep in range(2):
             for r in batch.runs:
                 r.reset()
@@ -153,9 +156,10 @@ def autotune_batch(batch, trial_steps: int = 32):
             b.record(stream)
             b.synchronize()
             if rep == 1:
-                timings[order] = a.elapsed_time(b)
-    choice = min(timings, key=timings.get)
+                timings[variant_name(cand)] = a.elapsed_time(b)
+    best = min(timings, key=timings.get)
+    choice = parse_variant(best)
     for r in batch.runs:
-        set_variant(r, 0, choice)
+        set_variant(r, *choice)
         r.reset()
-    return {"order": choice, "ms": timings}
+    return {"variant": best, "ms": timings}

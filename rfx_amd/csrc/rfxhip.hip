@@ -139,14 +139,30 @@ struct FluxParams {
 // base stays in SGPRs and the address is one VGPR offset, no 64-bit math.
 // Every array indexed this way is smaller than 4 GiB (checked at plan
 // creation).
+// The pointers are cast to the global address space: where a base pointer
+// comes from device memory (the batched step's instance table) rather than
+// a kernel argument, the compiler cannot infer it and would emit flat
+// accesses, which also count against the scalar-load counter and stall the
+// scalar parameter loads behind them.
+#define RFX_G __attribute__((address_space(1)))
 __device__ __forceinline__ float ldf(const float* base, unsigned cell) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (cell << 2));
+  return *(const RFX_G float*)((const RFX_G char*)base + (cell << 2));
 }
 __device__ __forceinline__ void stf(float* base, unsigned cell, float v) {
-  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (cell << 2)) = v;
+  *(RFX_G float*)((RFX_G char*)base + (cell << 2)) = v;
 }
 __device__ __forceinline__ unsigned ldu16(const uint16_t* base, unsigned cell) {
-  return *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(base) + (cell << 1));
+  return *(const RFX_G uint16_t*)((const RFX_G char*)base + (cell << 1));
+}
+// Element idx of a small global table (coefficients, profiles, spacings).
+template <class T>
+__device__ __forceinline__ T gld(const T* base, int idx) {
+  return ((const RFX_G T*)base)[idx];
+}
+__device__ __forceinline__ float4 gld(const float4* base, int idx) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = ((const RFX_G v4*)base)[idx];
+  return make_float4(v.x, v.y, v.z, v.w);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
@@ -286,9 +302,9 @@ __device__ __forceinline__ float3 h_core(const KParams& p, const PsiIO& ps, int 
                                          float hx, float hy, float hz, bool wr = true) {
   float ix, iy, iz;
   if (GRADED) {
-    ix = p.inv_h[0][i];
-    iy = p.inv_h[1][j];
-    iz = p.inv_h[2][k];
+    ix = gld(p.inv_h[0], i);
+    iy = gld(p.inv_h[1], j);
+    iz = gld(p.inv_h[2], k);
   } else {
     ix = iy = iz = p.inv_h0;
   }
@@ -302,7 +318,7 @@ __device__ __forceinline__ float3 h_core(const KParams& p, const PsiIO& ps, int 
       if (l < 0) continue;
       const int ax = face >> 1;
       const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
-      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      const float bb = gld(p.pb[face], l), cc = gld(p.pc[face], l), km1 = gld(p.pk[face], l);
       const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
@@ -355,7 +371,7 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
   float ex_k = ldf(Eo[0], ck), ey_k = ldf(Eo[1], ck);
   float ey_i = ldf(Eo[1], ci), ez_i = ldf(Eo[2], ci);
   const float hx = ldf(Ho[0], c), hy = ldf(Ho[1], c), hz = ldf(Ho[2], c);
-  const float ch = p.coef[m].w;
+  const float ch = gld(p.coef, m).w;
   ex_j = hj ? ex_j : 0.f;
   ez_j = hj ? ez_j : 0.f;
   ex_k = hk ? ex_k : 0.f;
@@ -384,9 +400,9 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
                                             float hy_k) {
   float ix, iy, iz;
   if (GRADED) {
-    ix = p.inv_e[0][i];
-    iy = p.inv_e[1][j];
-    iz = p.inv_e[2][k];
+    ix = gld(p.inv_e[0], i);
+    iy = gld(p.inv_e[1], j);
+    iz = gld(p.inv_e[2], k);
   } else {
     ix = iy = iz = p.inv_e0;
   }
@@ -401,7 +417,7 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
       if (l < 0) continue;
       const int ax = face >> 1;
       const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
-      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      const float bb = gld(p.pb[face], l), cc = gld(p.pc[face], l), km1 = gld(p.pk[face], l);
       const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
@@ -457,7 +473,7 @@ __device__ __forceinline__ float3 e_core(const KParams& p, const PsiIO& ps, int 
                                          unsigned c, unsigned m, float e0, float e1, float e2,
                                          float hx, float hy, float hz, float hy_i, float hz_i,
                                          float hx_j, float hz_j, float hx_k, float hy_k) {
-  return e_core_co<GRADED>(p, ps, i, j, k, c, p.coef[m], p.flags[m], e0, e1, e2, hx, hy, hz, hy_i,
+  return e_core_co<GRADED>(p, ps, i, j, k, c, gld(p.coef, m), gld(p.flags, m), e0, e1, e2, hx, hy, hz, hy_i,
                            hz_i, hx_j, hz_j, hx_k, hy_k);
 }
 
@@ -725,7 +741,7 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
     ez_j = hj ? ez_j : 0.f;
     ex_k = hk ? ex_k : 0.f;
     ey_k = hk ? ey_k : 0.f;
-    const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, p.coef[m].w, ldf(io.eo[0], cm),
+    const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, gld(p.coef, m).w, ldf(io.eo[0], cm),
                                            ldf(io.eo[1], cm), ldf(io.eo[2], cm), ex_j, ez_j, ex_k,
                                            ey_k, a1, a2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
                                            ldf(io.ho[2], cm));
@@ -760,7 +776,7 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
       }
       ex_k = hk ? ex_k : 0.f;
       ey_k = hk ? ey_k : 0.f;
-      h = h_core<GRADED, true>(p, ps, i, j, k, p.coef[m].w, a0, a1, a2, ex_j, ez_j, ex_k, ey_k, n1,
+      h = h_core<GRADED, true>(p, ps, i, j, k, gld(p.coef, m).w, a0, a1, a2, ex_j, ez_j, ex_k, ey_k, n1,
                                n2, ldf(io.ho[0], c), ldf(io.ho[1], c), ldf(io.ho[2], c), inner);
       if (valid_m) {
         const unsigned cm = c - Z;
@@ -778,7 +794,7 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
         bx_k = hk ? bx_k : 0.f;
         by_k = hk ? by_k : 0.f;
         // row below: its forward-y neighbour is the own row
-        hm = h_core<GRADED, false>(p, ps, i, j - 1, k, p.coef[mm].w, b0, b1, b2, a0, a2, bx_k,
+        hm = h_core<GRADED, false>(p, ps, i, j - 1, k, gld(p.coef, mm).w, b0, b1, b2, a0, a2, bx_k,
                                    by_k, q1, q2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
                                    ldf(io.ho[2], cm));
       }
@@ -860,12 +876,11 @@ __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
 // are re-read once per column instead of once per cell, and each thread
 // has NP cells' loads in flight at once.
 // ---------------------------------------------------------------------------
+// Cells i0 .. i0+NP-1 of the column, updating only those below iend
+// (iend <= nx; the others are read as neighbours only).
 template <bool GRADED, int NP>
-__global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
-  int ib, j, k;
-  const int nxb = (p.nx + NP - 1) / NP;
-  if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, nxb)) return;
-  const int i0 = ib * NP;
+__device__ __forceinline__ void col_h(const KParams& p, const PsiIO& ps, int i0, int j, int k,
+                                      int iend) {
   const unsigned P = (unsigned)p.plane;
   const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
   const unsigned c0 = ((unsigned)i0 * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
@@ -900,12 +915,12 @@ __global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
   }
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
-    if (i0 + q >= p.nx) break;
+    if (i0 + q >= iend) break;
     const float exj_ = hj ? exj[q] : 0.f, ezj_ = hj ? ezj[q] : 0.f;
     const float exk_ = hk ? exk[q] : 0.f, eyk_ = hk ? eyk[q] : 0.f;
     const float eyi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ey[q + 1] : 0.f) : ey[NP];
     const float ezi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ez[q + 1] : 0.f) : ez[NP];
-    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, p.coef[m[q]].w, ex[q], ey[q], ez[q],
+    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, gld(p.coef, (int)m[q]).w, ex[q], ey[q], ez[q],
                                           exj_, ezj_, exk_, eyk_, eyi, ezi, hx[q], hy[q], hz[q]);
     const unsigned c = c0 + (unsigned)q * P;
     stf(p.f[3], c, h.x);
@@ -915,11 +930,8 @@ __global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
 }
 
 template <bool GRADED, int NP>
-__global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
-  int ib, j, k;
-  const int nxb = (p.nx + NP - 1) / NP;
-  if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, nxb)) return;
-  const int i0 = ib * NP;
+__device__ __forceinline__ void col_e(const KParams& p, const PsiIO& ps, int i0, int j, int k,
+                                      int iend) {
   const unsigned P = (unsigned)p.plane;
   const bool bj = j > 0, bk = k > 0;
   const unsigned c0 = ((unsigned)i0 * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
@@ -955,7 +967,7 @@ __global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
   }
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
-    if (i0 + q >= p.nx) break;
+    if (i0 + q >= iend) break;
     const unsigned c = c0 + (unsigned)q * P;
     const float hxj_ = bj ? hxj[q] : 0.f, hzj_ = bj ? hzj[q] : 0.f;
     const float hxk_ = bk ? hxk[q] : 0.f, hyk_ = bk ? hyk[q] : 0.f;
@@ -967,11 +979,30 @@ __global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
   }
 }
 
+template <bool GRADED, int NP>
+__global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
+  int ib, j, k;
+  if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, (p.nx + NP - 1) / NP))
+    return;
+  col_h<GRADED, NP>(p, ps, ib * NP, j, k, p.nx);
+}
+
+template <bool GRADED, int NP>
+__global__ __launch_bounds__(kWG) void k_update_e_col(KParams p, PsiIO ps) {
+  int ib, j, k;
+  if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, (p.nx + NP - 1) / NP))
+    return;
+  col_e<GRADED, NP>(p, ps, ib * NP, j, k, p.nx);
+}
+
 // ---------------------------------------------------------------------------
 // Batched two-pass step over many independent problems: one launch per
 // half-step covers every active instance.  Instance q owns logical
 // workgroups [start[q], start[q+1]); the instance table lives in device
 // memory, so every parameter read below is wave-uniform (scalar loads).
+// The instance of a workgroup is a binary search over the few range starts
+// (they stay in the scalar cache); a per-workgroup owner table was measured
+// slower, its one load per workgroup missing that cache.
 // ---------------------------------------------------------------------------
 struct BatchEntry {
   KParams p;   // p.f = the instance's current field set
@@ -982,26 +1013,28 @@ __device__ __forceinline__ int batch_instance(const int* __restrict__ start, int
   int lo = 0, hi = n - 1;  // largest q with start[q] <= L
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (start[mid] <= L) lo = mid;
+    if (gld(start, mid) <= L) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
 
-template <int PASS>
+// NP cells per thread along x (the column kernels' body; NP = 1 is the
+// one-cell step).
+template <int PASS, int NP>
 __global__ __launch_bounds__(kWG) void k_batch_pass(const BatchEntry* __restrict__ ents,
                                                    const int* __restrict__ start, int n_act) {
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int q = batch_instance(start, n_act, L);
   const KParams& p = ents[q].p;
-  int i, j, k;
-  if (!cell_at(p, L - start[q], i, j, k)) return;
+  int ib, j, k;
+  if (!cell_at(p, L - gld(start, q), ib, j, k, (p.nx + NP - 1) / NP)) return;
   if (PASS == 0) {
-    if (p.graded) two_pass_h<true>(p, ents[q].ps, i, j, k);
-    else two_pass_h<false>(p, ents[q].ps, i, j, k);
+    if (p.graded) col_h<true, NP>(p, ents[q].ps, ib * NP, j, k, p.nx);
+    else col_h<false, NP>(p, ents[q].ps, ib * NP, j, k, p.nx);
   } else {
-    if (p.graded) two_pass_e<true>(p, ents[q].ps, i, j, k);
-    else two_pass_e<false>(p, ents[q].ps, i, j, k);
+    if (p.graded) col_e<true, NP>(p, ents[q].ps, ib * NP, j, k, p.nx);
+    else col_e<false, NP>(p, ents[q].ps, ib * NP, j, k, p.nx);
   }
 }
 
@@ -2285,9 +2318,10 @@ struct rfxh_batch {
   std::vector<rfxh_plan*> plans;
   BatchEntry* d_ents = nullptr;   // instance table in run order
   int* d_start = nullptr;         // first logical workgroup per instance (+ total)
+  int ncol = 1;                   // cells per thread along x of the batched passes
   long long* d_steps = nullptr;   // step0 per instance, run order
   std::vector<int> order;         // run order -> plan index
-  std::vector<int> start;         // host copy of d_start
+  std::vector<int> start;         // first logical workgroup per instance (+ total)
   hipStream_t cap_stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::map<int, hipGraphExec_t> graphs;  // keyed by active count
@@ -2304,8 +2338,17 @@ void batch_drop_graphs(rfxh_batch* b) {
 // One step of the first n_act instances (run order).
 int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
   const unsigned nb = (unsigned)b->start[n_act];
-  hipLaunchKernelGGL(k_batch_pass<0>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
-  hipLaunchKernelGGL(k_batch_pass<1>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
+  if (b->ncol == 2) {
+    hipLaunchKernelGGL((k_batch_pass<0, 2>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start,
+                       n_act);
+    hipLaunchKernelGGL((k_batch_pass<1, 2>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start,
+                       n_act);
+  } else {
+    hipLaunchKernelGGL((k_batch_pass<0, 1>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start,
+                       n_act);
+    hipLaunchKernelGGL((k_batch_pass<1, 1>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start,
+                       n_act);
+  }
   for (int q = 0; q < n_act; ++q) {
     rfxh_plan* pl = b->plans[b->order[q]];
     launch_accumulators(pl, s, pl->cur);
@@ -2367,6 +2410,10 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
   for (int q = 0; q < n; ++q) order[q] = q;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int c) { return n_steps[a] > n_steps[c]; });
+  // 2-cell columns when every instance's plan runs them, else one cell per thread
+  int ncol = 2;
+  for (int q = 0; q < n; ++q)
+    if (b->plans[q]->ncol != 2) ncol = 1;
   std::vector<BatchEntry> ents(n);
   std::vector<int> start(n + 1, 0);
   std::vector<long long> st(n);
@@ -2375,14 +2422,18 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
     ents[r].p = pl->kp;
     for (int c = 0; c < 6; ++c) ents[r].p.f[c] = pl->set[pl->cur][c];
     ents[r].ps = psi_io(pl, pl->cur);
-    const long long nb = (long long)start[r] + (long long)pl->grid.x;
+    const KParams& kq = pl->kp;
+    const long long flat = kq.plane, nxb = (kq.nx + ncol - 1) / ncol;
+    const long long blk = kq.order == 0 ? 8LL * nxb * (((flat + 7) / 8 + kWG - 1) / kWG)
+                                        : nxb * ((flat + kWG - 1) / kWG);
+    const long long nb = (long long)start[r] + blk;
     if (nb > 0x7fffffffLL) return fail(RFXH_EINVAL, "batch too large for one launch");
     start[r + 1] = (int)nb;
     st[r] = step0[order[r]];
   }
   // the captured graphs launch per-instance plane kernels: keep them only
   // while the instance table is unchanged
-  bool same = order == b->order && start == b->start;
+  bool same = order == b->order && start == b->start && ncol == b->ncol;
   if (same) {
     std::vector<BatchEntry> old(n);
     HIPCHK(hipMemcpy(old.data(), b->d_ents, sizeof(BatchEntry) * n, hipMemcpyDeviceToHost));
@@ -2392,6 +2443,7 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
     batch_drop_graphs(b);
     HIPCHK(hipMemcpyAsync(b->d_ents, ents.data(), sizeof(BatchEntry) * n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->d_start, start.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    b->ncol = ncol;
     b->order = order;
     b->start = start;
   }

@@ -54,3 +54,31 @@ def test_batch_equals_independent_runs(R, use_graph):
     e = cases.rel_l2(res[2].time_series, ref.time_series)
     print("instance 2 vs oracle", e)
     assert e <= 1e-4
+
+
+def test_batch_mixed_layouts(R):
+    """Instances on one cell per thread and on 2-cell columns, in flat and
+    XCD-band workgroup order, share one batched launch; each equals its own
+    run."""
+    _say("batch with mixed per-instance layouts")
+    from rfx_amd import autotune as at
+    from rfx_amd.batch import Batch
+    rng = np.random.default_rng(11)
+    shapes = [(24, 24, 24), (33, 20, 16), (15, 28, 30), (40, 16, 12)]
+    specs = [cases.sweep_instance(rng, s) for s in shapes]
+    specs.append(cases.mixed_graded_case())
+    steps = [200, 120, 151, 77, 160]
+    layouts = [(0, 1), (4, 0), (4, 1), (0, 0), (4, 1)]
+    sims = [cases.to_sim(sp) for sp in specs]
+    b = Batch(sims, steps)
+    try:
+        for r, v in zip(b.runs, layouts):
+            at.set_variant(r, *v)
+        res = b.run()
+    finally:
+        b.close()
+    for q, (sim, n) in enumerate(zip(sims, steps)):
+        ind = sim.run(n_steps=n)
+        e = cases.rel_l2(res[q].time_series, ind.time_series)
+        print("instance", q, at.variant_name(layouts[q]), n, e)
+        assert e <= 1e-6
