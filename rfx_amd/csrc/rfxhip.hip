@@ -677,34 +677,53 @@ __global__ __launch_bounds__(kBX * (MJ + 2)) void k_step_march(KParams p, MarchI
 }
 
 // ---------------------------------------------------------------------------
-// Wave-march step: read set A, write set B, no barriers.  Each wavefront owns
-// one row (y) of 63 cells (z) plus a halo lane at k0-1 and walks along x
-// through its segment of planes.  Per plane it computes H on its own row and,
-// again, on the row below (j-1, the E update's backward-y neighbour), takes
-// the backward-z neighbour H from the next lower lane (cross-lane shuffle)
-// and the backward-x neighbour from its registers, then updates E.  Waves
-// never wait for each other.
+// Wave-march step: read set A, write set B, no barriers in the march.  Each
+// wavefront owns one row (y) of 63 cells (z) plus a halo lane at k0-1 and
+// walks along x through its segment of planes; the wavefronts of a
+// workgroup take consecutive z tiles of one row.  Per plane a wavefront
+// computes H on its own row and, again, on the row below (j-1, the E
+// update's backward-y neighbour), takes the backward-z neighbour H from the
+// next lower lane (cross-lane shuffle) and the backward-x neighbour from its
+// registers, then updates E.  Waves never wait for each other.
+//
+// Software pipeline: every global load a plane needs is issued one plane
+// ahead (E two planes ahead, since the H update of plane i reads E of plane
+// i+1), and the material coefficients come from a table staged in LDS, so
+// no load is consumed in the iteration that issues it.  Vector-memory
+// loads complete in order, so a load issued and waited for inside the
+// compute phase would also wait for every prefetch issued before it.
+// CPML memory and point operations still load inline; only the wavefronts
+// whose cells have them pay that wait.
 // ---------------------------------------------------------------------------
 constexpr int kWMW = 4;              // independent wavefronts per workgroup
 constexpr int kWMK = kBX - 1;        // interior columns per wave
+constexpr int kWMMat = 1024;         // material table entries staged in LDS (the variant's limit)
 
 struct WMarchIO {
   const float* eo[3];
   const float* ho[3];
   float* en[3];
   float* hn[3];
-  int ntk, ntj, seg_len;
+  int ntk, ntj, seg_len;   // workgroups along z (kWMW tiles each), rows, planes per segment
+  int n_mat;
 };
 
 template <bool GRADED>
 __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO io, PsiIO ps) {
+  __shared__ float4 s_coef[kWMMat];
+  __shared__ uint32_t s_flag[kWMMat];
+  // the plan rejects this variant for more than kWMMat materials
+  for (int t = (int)(threadIdx.y * kBX + threadIdx.x); t < io.n_mat; t += kBX * kWMW) {
+    s_coef[t] = gld(p.coef, t);
+    s_flag[t] = gld(p.flags, t);
+  }
+  __syncthreads();
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const int tk = L % io.ntk;
+  const int tk = (L % io.ntk) * kWMW + (int)threadIdx.y;
   const int r = L / io.ntk;
-  const int tj = r % io.ntj;
+  const int j = r % io.ntj;
   const int seg = r / io.ntj;
   const int l = (int)threadIdx.x;
-  const int j = tj * kWMW + (int)threadIdx.y;
   const int k = tk * kWMK + l - 1;
   const int i0 = seg * io.seg_len;
   const int i1 = min(p.nx, i0 + io.seg_len);
@@ -712,36 +731,68 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
   const bool valid = vrow && k >= 0 && k < p.nz;
   const bool valid_m = valid && j > 0;          // the row below exists
   const bool inner = valid && l > 0;
+  const bool last = l == kBX - 1;               // loads its forward-z E itself
   const unsigned P = (unsigned)p.plane;
   const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
-  const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
   const unsigned Z = (unsigned)p.nz;
+  // in-plane offsets of the own row and the row below (clamped to a valid
+  // cell where the lane has none; such values are never used)
+  const unsigned c0 = valid ? (unsigned)j * Z + (unsigned)k : 0u;
+  const unsigned cb0 = valid_m ? c0 - Z : c0;
   const unsigned dj = hj ? Z : 0u, dk = hk ? 1u : 0u;
-  // E^n of plane i at the own row (a*) and the row below (b*)
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f;
-  float hyp = 0.f, hzp = 0.f;                   // own-row H^{n+1/2} of plane i-1
-  if (valid && i0 < i1) {
-    const unsigned c = (unsigned)i0 * P + c0;
-    a0 = ldf(io.eo[0], c);
-    a1 = ldf(io.eo[1], c);
-    a2 = ldf(io.eo[2], c);
-    if (valid_m) {
-      b0 = ldf(io.eo[0], c - Z);
-      b1 = ldf(io.eo[1], c - Z);
-      b2 = ldf(io.eo[2], c - Z);
+  if (i0 >= i1 || tk * kWMK - 1 >= p.nz) return;   // no planes, or a tile past the row's end
+  auto coef = [&](unsigned m) { return s_coef[m]; };
+  auto flag = [&](unsigned m) { return s_flag[m]; };
+  auto plane_off = [&](int i) { return (unsigned)min(i, p.nx - 1) * P; };
+
+  // plane i values: E (own a*, below b*), E of plane i+1 (n*, q*), old H
+  // (own h*, below x/z g*), material ids, forward-y E of the own row (ej*),
+  // forward-z E of lane 63 (own ek*, below fk*)
+  float a0, a1, a2, b0, b1, b2, n0, n1, n2, q0, q1, q2;
+  float h0, h1, h2, g0, g2, ej0, ej2, ek0 = 0.f, ek1 = 0.f, fk0 = 0.f, fk1 = 0.f;
+  unsigned m, mb;
+  {
+    const unsigned o = plane_off(i0), on = plane_off(i0 + 1);
+    a0 = ldf(io.eo[0], o + c0);
+    a1 = ldf(io.eo[1], o + c0);
+    a2 = ldf(io.eo[2], o + c0);
+    b0 = ldf(io.eo[0], o + cb0);
+    b1 = ldf(io.eo[1], o + cb0);
+    b2 = ldf(io.eo[2], o + cb0);
+    n0 = ldf(io.eo[0], on + c0);
+    n1 = ldf(io.eo[1], on + c0);
+    n2 = ldf(io.eo[2], on + c0);
+    q0 = ldf(io.eo[0], on + cb0);
+    q1 = ldf(io.eo[1], on + cb0);
+    q2 = ldf(io.eo[2], on + cb0);
+    h0 = ldf(io.ho[0], o + c0);
+    h1 = ldf(io.ho[1], o + c0);
+    h2 = ldf(io.ho[2], o + c0);
+    g0 = ldf(io.ho[0], o + cb0);
+    g2 = ldf(io.ho[2], o + cb0);
+    m = ldu16(p.mat, o + c0);
+    mb = ldu16(p.mat, o + cb0);
+    ej0 = ldf(io.eo[0], o + c0 + dj);
+    ej2 = ldf(io.eo[2], o + c0 + dj);
+    if (last) {
+      ek0 = ldf(io.eo[0], o + c0 + dk);
+      ek1 = ldf(io.eo[1], o + c0 + dk);
+      fk0 = ldf(io.eo[0], o + cb0 + dk);
+      fk1 = ldf(io.eo[1], o + cb0 + dk);
     }
   }
-  if (valid && i0 > 0 && i0 < i1) {
+  float hyp = 0.f, hzp = 0.f;                   // own-row H^{n+1/2} of plane i-1
+  if (valid && i0 > 0) {
     // own-row H of plane i0-1 (owned by the previous segment): recomputed
     const unsigned cm = (unsigned)(i0 - 1) * P + c0;
-    const unsigned m = ldu16(p.mat, cm);
+    const unsigned mm = ldu16(p.mat, cm);
     float ex_j = ldf(io.eo[0], cm + dj), ez_j = ldf(io.eo[2], cm + dj);
     float ex_k = ldf(io.eo[0], cm + dk), ey_k = ldf(io.eo[1], cm + dk);
     ex_j = hj ? ex_j : 0.f;
     ez_j = hj ? ez_j : 0.f;
     ex_k = hk ? ex_k : 0.f;
     ey_k = hk ? ey_k : 0.f;
-    const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, gld(p.coef, m).w, ldf(io.eo[0], cm),
+    const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, coef(mm).w, ldf(io.eo[0], cm),
                                            ldf(io.eo[1], cm), ldf(io.eo[2], cm), ex_j, ez_j, ex_k,
                                            ey_k, a1, a2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
                                            ldf(io.ho[2], cm));
@@ -749,60 +800,58 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
     hzp = h.z;
   }
   for (int i = i0; i < i1; ++i) {
+    // ---- prefetch: plane i+1 (old H, ids, forward-y/z E) and E of plane i+2
+    const unsigned o1 = plane_off(i + 1), o2 = plane_off(i + 2);
+    const float N0 = ldf(io.eo[0], o2 + c0), N1 = ldf(io.eo[1], o2 + c0),
+                N2 = ldf(io.eo[2], o2 + c0);
+    const float Q0 = ldf(io.eo[0], o2 + cb0), Q1 = ldf(io.eo[1], o2 + cb0),
+                Q2 = ldf(io.eo[2], o2 + cb0);
+    const float H0 = ldf(io.ho[0], o1 + c0), H1 = ldf(io.ho[1], o1 + c0),
+                H2 = ldf(io.ho[2], o1 + c0);
+    const float G0 = ldf(io.ho[0], o1 + cb0), G2 = ldf(io.ho[2], o1 + cb0);
+    const unsigned M = ldu16(p.mat, o1 + c0), MB = ldu16(p.mat, o1 + cb0);
+    const float EJ0 = ldf(io.eo[0], o1 + c0 + dj), EJ2 = ldf(io.eo[2], o1 + c0 + dj);
+    float EK0 = 0.f, EK1 = 0.f, FK0 = 0.f, FK1 = 0.f;
+    if (last) {
+      EK0 = ldf(io.eo[0], o1 + c0 + dk);
+      EK1 = ldf(io.eo[1], o1 + c0 + dk);
+      FK0 = ldf(io.eo[0], o1 + cb0 + dk);
+      FK1 = ldf(io.eo[1], o1 + cb0 + dk);
+    }
+    // ---- compute plane i from the values loaded one iteration earlier
     const unsigned c = (unsigned)i * P + c0;
     const bool hi = i + 1 < p.nx;
-    // E^n of plane i+1 (own row, row below), next to-be-current values
-    float n0 = 0.f, n1 = 0.f, n2 = 0.f, q0 = 0.f, q1 = 0.f, q2 = 0.f;
-    float3 h = make_float3(0.f, 0.f, 0.f), hm = make_float3(0.f, 0.f, 0.f);
-    unsigned m = 0;
+    const float x1 = hi ? n1 : 0.f, x2 = hi ? n2 : 0.f;      // E of plane i+1
+    const float y1 = hi ? q1 : 0.f, y2 = hi ? q2 : 0.f;
     // backward-z H comes from lane l-1 and forward-z E from lane l+1; lanes
-    // past the grid carry zeros and lane 63 loads its forward-z E itself
-    const float up_ex = __shfl_down(a0, 1), up_ey = __shfl_down(a1, 1);
-    const float upm_ex = __shfl_down(b0, 1), upm_ey = __shfl_down(b1, 1);
+    // past the grid carry zeros and lane 63 has loaded its forward-z E
+    float ex_k = __shfl_down(a0, 1), ey_k = __shfl_down(a1, 1);
+    float bx_k = __shfl_down(b0, 1), by_k = __shfl_down(b1, 1);
+    if (last) {
+      ex_k = ek0;
+      ey_k = ek1;
+      bx_k = fk0;
+      by_k = fk1;
+    }
+    ex_k = hk ? ex_k : 0.f;
+    ey_k = hk ? ey_k : 0.f;
+    bx_k = hk ? bx_k : 0.f;
+    by_k = hk ? by_k : 0.f;
+    float3 h = make_float3(0.f, 0.f, 0.f), hm = make_float3(0.f, 0.f, 0.f);
     if (valid) {
-      m = ldu16(p.mat, c);
-      if (hi) {
-        n0 = ldf(io.eo[0], c + P);
-        n1 = ldf(io.eo[1], c + P);
-        n2 = ldf(io.eo[2], c + P);
-      }
-      float ex_j = ldf(io.eo[0], c + dj), ez_j = ldf(io.eo[2], c + dj);
-      ex_j = hj ? ex_j : 0.f;
-      ez_j = hj ? ez_j : 0.f;
-      float ex_k = up_ex, ey_k = up_ey;
-      if (l == kBX - 1) {
-        ex_k = ldf(io.eo[0], c + dk);
-        ey_k = ldf(io.eo[1], c + dk);
-      }
-      ex_k = hk ? ex_k : 0.f;
-      ey_k = hk ? ey_k : 0.f;
-      h = h_core<GRADED, true>(p, ps, i, j, k, gld(p.coef, m).w, a0, a1, a2, ex_j, ez_j, ex_k, ey_k, n1,
-                               n2, ldf(io.ho[0], c), ldf(io.ho[1], c), ldf(io.ho[2], c), inner);
+      h = h_core<GRADED, true>(p, ps, i, j, k, coef(m).w, a0, a1, a2, hj ? ej0 : 0.f,
+                               hj ? ej2 : 0.f, ex_k, ey_k, x1, x2, h0, h1, h2, inner);
       if (valid_m) {
-        const unsigned cm = c - Z;
-        const unsigned mm = ldu16(p.mat, cm);
-        if (hi) {
-          q0 = ldf(io.eo[0], cm + P);
-          q1 = ldf(io.eo[1], cm + P);
-          q2 = ldf(io.eo[2], cm + P);
-        }
-        float bx_k = upm_ex, by_k = upm_ey;
-        if (l == kBX - 1) {
-          bx_k = ldf(io.eo[0], cm + dk);
-          by_k = ldf(io.eo[1], cm + dk);
-        }
-        bx_k = hk ? bx_k : 0.f;
-        by_k = hk ? by_k : 0.f;
-        // row below: its forward-y neighbour is the own row
-        hm = h_core<GRADED, false>(p, ps, i, j - 1, k, gld(p.coef, mm).w, b0, b1, b2, a0, a2, bx_k,
-                                   by_k, q1, q2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
-                                   ldf(io.ho[2], cm));
+        // row below: its forward-y neighbour is the own row; its old Hy is
+        // not needed (only Hx, Hz of the row below enter the E update)
+        hm = h_core<GRADED, false>(p, ps, i, j - 1, k, coef(mb).w, b0, b1, b2, a0, a2, bx_k,
+                                   by_k, y1, y2, g0, 0.f, g2);
       }
     }
     const float hx_k = __shfl_up(h.x, 1), hy_k = __shfl_up(h.y, 1);
     if (inner) {
-      const float3 e = e_core<GRADED>(p, ps, i, j, k, c, m, a0, a1, a2, h.x, h.y, h.z, hyp, hzp,
-                                      hm.x, hm.z, hx_k, hy_k);
+      const float3 e = e_core_co<GRADED>(p, ps, i, j, k, c, coef(m), flag(m), a0, a1, a2, h.x,
+                                         h.y, h.z, hyp, hzp, hm.x, hm.z, hx_k, hy_k);
       stf(io.hn[0], c, h.x);
       stf(io.hn[1], c, h.y);
       stf(io.hn[2], c, h.z);
@@ -812,12 +861,32 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
     }
     hyp = h.y;
     hzp = h.z;
+    // ---- rotate the pipeline
     a0 = n0;
     a1 = n1;
     a2 = n2;
     b0 = q0;
     b1 = q1;
     b2 = q2;
+    n0 = N0;
+    n1 = N1;
+    n2 = N2;
+    q0 = Q0;
+    q1 = Q1;
+    q2 = Q2;
+    h0 = H0;
+    h1 = H1;
+    h2 = H2;
+    g0 = G0;
+    g2 = G2;
+    m = M;
+    mb = MB;
+    ej0 = EJ0;
+    ej2 = EJ2;
+    ek0 = EK0;
+    ek1 = EK1;
+    fk0 = FK0;
+    fk1 = FK1;
   }
 }
 
@@ -1675,8 +1744,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
       io.hn[a] = pl->set[to][3 + a];
     }
     const KParams& kp = pl->kp;
-    io.ntj = (kp.ny + kWMW - 1) / kWMW;
-    io.ntk = (kp.nz + kWMK - 1) / kWMK;
+    io.n_mat = pl->n_mat;
+    io.ntj = kp.ny;                                          // one row per workgroup
+    io.ntk = ((kp.nz + kWMK - 1) / kWMK + kWMW - 1) / kWMW;  // groups of kWMW z tiles
     const long long tiles = (long long)io.ntj * io.ntk;
     int nseg = pl->wmarch_seg;
     if (nseg <= 0) {
@@ -2162,6 +2232,8 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
   if (kernel == 2 || kernel == 3 || kernel == 7) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     if (kernel == 7) {
+      if (pl->n_mat > kWMMat)
+        return fail(RFXH_EINVAL, "the wave-march step takes at most 1024 materials");
       pl->wmarch_seg = param;
     } else {
       pl->march_seg = param;
