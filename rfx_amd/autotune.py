@@ -48,7 +48,7 @@ def variants(dr):
         out += [(1, 1), (1, 0)]
         # the wave-march step stages the material table in LDS (at most 1024 ids)
         for kern in (2, 3) + ((7,) if len(dr.flags) <= 1024 else ()):
-            out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32) if s <= max(1, nx // 4)]
+            out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32, 64) if s <= max(1, nx // 4)]
     return out
 
 

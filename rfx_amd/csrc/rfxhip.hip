@@ -678,13 +678,14 @@ __global__ __launch_bounds__(kBX * (MJ + 2)) void k_step_march(KParams p, MarchI
 
 // ---------------------------------------------------------------------------
 // Wave-march step: read set A, write set B, no barriers in the march.  Each
-// wavefront owns one row (y) of 63 cells (z) plus a halo lane at k0-1 and
-// walks along x through its segment of planes; the wavefronts of a
-// workgroup take consecutive z tiles of one row.  Per plane a wavefront
-// computes H on its own row and, again, on the row below (j-1, the E
-// update's backward-y neighbour), takes the backward-z neighbour H from the
-// next lower lane (cross-lane shuffle) and the backward-x neighbour from its
-// registers, then updates E.  Waves never wait for each other.
+// wavefront owns two rows (y = j, j+1) of a run of 63 cells (z) plus a halo
+// lane at k0-1 and walks along x through its segment of planes; the
+// wavefronts of a workgroup take consecutive z runs of one row pair.  Per
+// plane a wavefront computes H on its two rows and, again, on the row below
+// (j-1, the E update's backward-y neighbour of row j; row j+1 has row j),
+// takes the backward-z neighbour H from the next lower lane (cross-lane
+// shuffle) and the backward-x neighbour from its registers, then updates E
+// on both rows.  Waves never wait for each other.
 //
 // Software pipeline: every global load a plane needs is issued one plane
 // ahead (E two planes ahead, since the H update of plane i reads E of plane
@@ -704,8 +705,17 @@ struct WMarchIO {
   const float* ho[3];
   float* en[3];
   float* hn[3];
-  int ntk, ntj, seg_len;   // workgroups along z (kWMW tiles each), rows, planes per segment
+  int ntk, ntj, seg_len;   // workgroups along z (kWMW runs each), row pairs, planes per segment
   int n_mat;
+};
+
+// The values of one plane a wavefront needs: E of the two own rows (e) and
+// of the row below (eb), old H of the own rows (h) and of the row below (x,
+// z: hb), material ids, forward-y E of row j+2 (ey2: x, z) and lane 63's
+// forward-z E (x, y) of rows j, j+1 and j-1 (ek).
+struct WMPlane {
+  float e[2][3], eb[3], h[2][3], hb[2], ey2[2], ek[3][2];
+  unsigned m[2], mb;
 };
 
 template <bool GRADED>
@@ -721,172 +731,161 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int tk = (L % io.ntk) * kWMW + (int)threadIdx.y;
   const int r = L / io.ntk;
-  const int j = r % io.ntj;
+  const int j = 2 * (r % io.ntj);
   const int seg = r / io.ntj;
   const int l = (int)threadIdx.x;
   const int k = tk * kWMK + l - 1;
   const int i0 = seg * io.seg_len;
   const int i1 = min(p.nx, i0 + io.seg_len);
-  const bool vrow = j < p.ny;
-  const bool valid = vrow && k >= 0 && k < p.nz;
-  const bool valid_m = valid && j > 0;          // the row below exists
-  const bool inner = valid && l > 0;
+  const bool kin = k >= 0 && k < p.nz;
+  const bool v0 = kin && j < p.ny, v1 = kin && j + 1 < p.ny;   // own rows exist
+  const bool vb = v0 && j > 0;                                 // the row below exists
+  const bool h2 = j + 2 < p.ny;                                // row j+2 exists
+  const bool inner = l > 0;
   const bool last = l == kBX - 1;               // loads its forward-z E itself
   const unsigned P = (unsigned)p.plane;
-  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz;
+  const bool hk = k + 1 < p.nz;
   const unsigned Z = (unsigned)p.nz;
-  // in-plane offsets of the own row and the row below (clamped to a valid
-  // cell where the lane has none; such values are never used)
-  const unsigned c0 = valid ? (unsigned)j * Z + (unsigned)k : 0u;
-  const unsigned cb0 = valid_m ? c0 - Z : c0;
-  const unsigned dj = hj ? Z : 0u, dk = hk ? 1u : 0u;
-  if (i0 >= i1 || tk * kWMK - 1 >= p.nz) return;   // no planes, or a tile past the row's end
-  auto coef = [&](unsigned m) { return s_coef[m]; };
-  auto flag = [&](unsigned m) { return s_flag[m]; };
+  // in-plane offsets (clamped to a valid cell where a lane has none; such
+  // values are never used)
+  const unsigned c0 = v0 ? (unsigned)j * Z + (unsigned)k : 0u;
+  const unsigned c1 = v1 ? c0 + Z : c0;
+  const unsigned cb = vb ? c0 - Z : c0;
+  const unsigned c2 = (v1 && h2) ? c0 + 2 * Z : c0;
+  const unsigned dk = hk ? 1u : 0u;
+  if (i0 >= i1 || tk * kWMK - 1 >= p.nz || j >= p.ny) return;
   auto plane_off = [&](int i) { return (unsigned)min(i, p.nx - 1) * P; };
-
-  // plane i values: E (own a*, below b*), E of plane i+1 (n*, q*), old H
-  // (own h*, below x/z g*), material ids, forward-y E of the own row (ej*),
-  // forward-z E of lane 63 (own ek*, below fk*)
-  float a0, a1, a2, b0, b1, b2, n0, n1, n2, q0, q1, q2;
-  float h0, h1, h2, g0, g2, ej0, ej2, ek0 = 0.f, ek1 = 0.f, fk0 = 0.f, fk1 = 0.f;
-  unsigned m, mb;
-  {
-    const unsigned o = plane_off(i0), on = plane_off(i0 + 1);
-    a0 = ldf(io.eo[0], o + c0);
-    a1 = ldf(io.eo[1], o + c0);
-    a2 = ldf(io.eo[2], o + c0);
-    b0 = ldf(io.eo[0], o + cb0);
-    b1 = ldf(io.eo[1], o + cb0);
-    b2 = ldf(io.eo[2], o + cb0);
-    n0 = ldf(io.eo[0], on + c0);
-    n1 = ldf(io.eo[1], on + c0);
-    n2 = ldf(io.eo[2], on + c0);
-    q0 = ldf(io.eo[0], on + cb0);
-    q1 = ldf(io.eo[1], on + cb0);
-    q2 = ldf(io.eo[2], on + cb0);
-    h0 = ldf(io.ho[0], o + c0);
-    h1 = ldf(io.ho[1], o + c0);
-    h2 = ldf(io.ho[2], o + c0);
-    g0 = ldf(io.ho[0], o + cb0);
-    g2 = ldf(io.ho[2], o + cb0);
-    m = ldu16(p.mat, o + c0);
-    mb = ldu16(p.mat, o + cb0);
-    ej0 = ldf(io.eo[0], o + c0 + dj);
-    ej2 = ldf(io.eo[2], o + c0 + dj);
-    if (last) {
-      ek0 = ldf(io.eo[0], o + c0 + dk);
-      ek1 = ldf(io.eo[1], o + c0 + dk);
-      fk0 = ldf(io.eo[0], o + cb0 + dk);
-      fk1 = ldf(io.eo[1], o + cb0 + dk);
+  // all of a plane's loads except E (E comes two planes ahead, see below)
+  auto load_rest = [&](WMPlane& w, unsigned o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      w.h[0][c] = ldf(io.ho[c], o + c0);
+      w.h[1][c] = ldf(io.ho[c], o + c1);
     }
-  }
-  float hyp = 0.f, hzp = 0.f;                   // own-row H^{n+1/2} of plane i-1
-  if (valid && i0 > 0) {
-    // own-row H of plane i0-1 (owned by the previous segment): recomputed
-    const unsigned cm = (unsigned)(i0 - 1) * P + c0;
-    const unsigned mm = ldu16(p.mat, cm);
-    float ex_j = ldf(io.eo[0], cm + dj), ez_j = ldf(io.eo[2], cm + dj);
-    float ex_k = ldf(io.eo[0], cm + dk), ey_k = ldf(io.eo[1], cm + dk);
-    ex_j = hj ? ex_j : 0.f;
-    ez_j = hj ? ez_j : 0.f;
-    ex_k = hk ? ex_k : 0.f;
-    ey_k = hk ? ey_k : 0.f;
-    const float3 h = h_core<GRADED, false>(p, ps, i0 - 1, j, k, coef(mm).w, ldf(io.eo[0], cm),
-                                           ldf(io.eo[1], cm), ldf(io.eo[2], cm), ex_j, ez_j, ex_k,
-                                           ey_k, a1, a2, ldf(io.ho[0], cm), ldf(io.ho[1], cm),
-                                           ldf(io.ho[2], cm));
-    hyp = h.y;
-    hzp = h.z;
+    w.hb[0] = ldf(io.ho[0], o + cb);
+    w.hb[1] = ldf(io.ho[2], o + cb);
+    w.m[0] = ldu16(p.mat, o + c0);
+    w.m[1] = ldu16(p.mat, o + c1);
+    w.mb = ldu16(p.mat, o + cb);
+    w.ey2[0] = ldf(io.eo[0], o + c2);
+    w.ey2[1] = ldf(io.eo[2], o + c2);
+    if (last) {
+      w.ek[0][0] = ldf(io.eo[0], o + c0 + dk);
+      w.ek[0][1] = ldf(io.eo[1], o + c0 + dk);
+      w.ek[1][0] = ldf(io.eo[0], o + c1 + dk);
+      w.ek[1][1] = ldf(io.eo[1], o + c1 + dk);
+      w.ek[2][0] = ldf(io.eo[0], o + cb + dk);
+      w.ek[2][1] = ldf(io.eo[1], o + cb + dk);
+    }
+  };
+  auto load_e = [&](float (&e)[2][3], float (&eb)[3], unsigned o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      e[0][c] = ldf(io.eo[c], o + c0);
+      e[1][c] = ldf(io.eo[c], o + c1);
+      eb[c] = ldf(io.eo[c], o + cb);
+    }
+  };
+  WMPlane w;            // plane i
+  float en[2][3], enb[3];   // E of plane i+1
+#pragma unroll
+  for (int q = 0; q < 3; ++q) w.ek[q][0] = w.ek[q][1] = 0.f;
+  load_e(w.e, w.eb, plane_off(i0));
+  load_e(en, enb, plane_off(i0 + 1));
+  load_rest(w, plane_off(i0));
+  // own-row H of plane i-1 (y, z), for the backward-x neighbours
+  float hyp[2] = {0.f, 0.f}, hzp[2] = {0.f, 0.f};
+  if (i0 > 0) {
+    // owned by the previous segment: recomputed
+    const unsigned o = (unsigned)(i0 - 1) * P;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const unsigned cq = q ? c1 : c0;
+      const unsigned cy = q ? c2 : c1;     // forward-y neighbour row
+      const bool hy = q ? h2 : j + 1 < p.ny;
+      const unsigned mm = ldu16(p.mat, o + cq);
+      float ex_j = ldf(io.eo[0], o + cy), ez_j = ldf(io.eo[2], o + cy);
+      float ex_k = ldf(io.eo[0], o + cq + dk), ey_k = ldf(io.eo[1], o + cq + dk);
+      const float3 h = h_core<GRADED, false>(
+          p, ps, i0 - 1, j + q, k, s_coef[mm].w, ldf(io.eo[0], o + cq), ldf(io.eo[1], o + cq),
+          ldf(io.eo[2], o + cq), hy ? ex_j : 0.f, hy ? ez_j : 0.f, hk ? ex_k : 0.f,
+          hk ? ey_k : 0.f, w.e[q][1], w.e[q][2], ldf(io.ho[0], o + cq), ldf(io.ho[1], o + cq),
+          ldf(io.ho[2], o + cq));
+      hyp[q] = h.y;
+      hzp[q] = h.z;
+    }
   }
   for (int i = i0; i < i1; ++i) {
-    // ---- prefetch: plane i+1 (old H, ids, forward-y/z E) and E of plane i+2
-    const unsigned o1 = plane_off(i + 1), o2 = plane_off(i + 2);
-    const float N0 = ldf(io.eo[0], o2 + c0), N1 = ldf(io.eo[1], o2 + c0),
-                N2 = ldf(io.eo[2], o2 + c0);
-    const float Q0 = ldf(io.eo[0], o2 + cb0), Q1 = ldf(io.eo[1], o2 + cb0),
-                Q2 = ldf(io.eo[2], o2 + cb0);
-    const float H0 = ldf(io.ho[0], o1 + c0), H1 = ldf(io.ho[1], o1 + c0),
-                H2 = ldf(io.ho[2], o1 + c0);
-    const float G0 = ldf(io.ho[0], o1 + cb0), G2 = ldf(io.ho[2], o1 + cb0);
-    const unsigned M = ldu16(p.mat, o1 + c0), MB = ldu16(p.mat, o1 + cb0);
-    const float EJ0 = ldf(io.eo[0], o1 + c0 + dj), EJ2 = ldf(io.eo[2], o1 + c0 + dj);
-    float EK0 = 0.f, EK1 = 0.f, FK0 = 0.f, FK1 = 0.f;
-    if (last) {
-      EK0 = ldf(io.eo[0], o1 + c0 + dk);
-      EK1 = ldf(io.eo[1], o1 + c0 + dk);
-      FK0 = ldf(io.eo[0], o1 + cb0 + dk);
-      FK1 = ldf(io.eo[1], o1 + cb0 + dk);
-    }
+    // ---- prefetch: E of plane i+2, everything else of plane i+1
+    WMPlane nw;
+    float nn[2][3], nnb[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) nw.ek[q][0] = nw.ek[q][1] = 0.f;
+    load_e(nn, nnb, plane_off(i + 2));
+    load_rest(nw, plane_off(i + 1));
     // ---- compute plane i from the values loaded one iteration earlier
-    const unsigned c = (unsigned)i * P + c0;
     const bool hi = i + 1 < p.nx;
-    const float x1 = hi ? n1 : 0.f, x2 = hi ? n2 : 0.f;      // E of plane i+1
-    const float y1 = hi ? q1 : 0.f, y2 = hi ? q2 : 0.f;
-    // backward-z H comes from lane l-1 and forward-z E from lane l+1; lanes
-    // past the grid carry zeros and lane 63 has loaded its forward-z E
-    float ex_k = __shfl_down(a0, 1), ey_k = __shfl_down(a1, 1);
-    float bx_k = __shfl_down(b0, 1), by_k = __shfl_down(b1, 1);
-    if (last) {
-      ex_k = ek0;
-      ey_k = ek1;
-      bx_k = fk0;
-      by_k = fk1;
+    // forward-z E from lane l+1 (lane 63 has loaded its own)
+    float fk[3][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float* src = q < 2 ? w.e[q] : w.eb;
+      fk[q][0] = last ? w.ek[q][0] : __shfl_down(src[0], 1);
+      fk[q][1] = last ? w.ek[q][1] : __shfl_down(src[1], 1);
+      fk[q][0] = hk ? fk[q][0] : 0.f;
+      fk[q][1] = hk ? fk[q][1] : 0.f;
     }
-    ex_k = hk ? ex_k : 0.f;
-    ey_k = hk ? ey_k : 0.f;
-    bx_k = hk ? bx_k : 0.f;
-    by_k = hk ? by_k : 0.f;
-    float3 h = make_float3(0.f, 0.f, 0.f), hm = make_float3(0.f, 0.f, 0.f);
-    if (valid) {
-      h = h_core<GRADED, true>(p, ps, i, j, k, coef(m).w, a0, a1, a2, hj ? ej0 : 0.f,
-                               hj ? ej2 : 0.f, ex_k, ey_k, x1, x2, h0, h1, h2, inner);
-      if (valid_m) {
-        // row below: its forward-y neighbour is the own row; its old Hy is
-        // not needed (only Hx, Hz of the row below enter the E update)
-        hm = h_core<GRADED, false>(p, ps, i, j - 1, k, coef(mb).w, b0, b1, b2, a0, a2, bx_k,
-                                   by_k, y1, y2, g0, 0.f, g2);
+    float3 hm = make_float3(0.f, 0.f, 0.f);
+    if (vb) {
+      // row below: forward-y neighbour is row j; its old Hy is not needed
+      hm = h_core<GRADED, false>(p, ps, i, j - 1, k, s_coef[w.mb].w, w.eb[0], w.eb[1], w.eb[2],
+                                 w.e[0][0], w.e[0][2], fk[2][0], fk[2][1], hi ? enb[1] : 0.f,
+                                 hi ? enb[2] : 0.f, w.hb[0], 0.f, w.hb[1]);
+    }
+    float3 h[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const bool vq = q ? v1 : v0;
+      const bool hy = q ? h2 : v1;                       // forward-y row exists
+      const float eyx = q ? w.ey2[0] : w.e[1][0], eyz = q ? w.ey2[1] : w.e[1][2];
+      h[q] = make_float3(0.f, 0.f, 0.f);
+      if (vq)
+        h[q] = h_core<GRADED, true>(p, ps, i, j + q, k, s_coef[w.m[q]].w, w.e[q][0], w.e[q][1],
+                                    w.e[q][2], hy ? eyx : 0.f, hy ? eyz : 0.f, fk[q][0], fk[q][1],
+                                    hi ? en[q][1] : 0.f, hi ? en[q][2] : 0.f, w.h[q][0],
+                                    w.h[q][1], w.h[q][2], inner);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float hx_k = __shfl_up(h[q].x, 1), hy_k = __shfl_up(h[q].y, 1);
+      const bool vq = q ? v1 : v0;
+      if (vq && inner) {
+        const unsigned c = (unsigned)i * P + (q ? c1 : c0);
+        const float3 hj = q ? h[0] : hm;                 // backward-y neighbour
+        const float3 e = e_core_co<GRADED>(p, ps, i, j + q, k, c, s_coef[w.m[q]], s_flag[w.m[q]],
+                                           w.e[q][0], w.e[q][1], w.e[q][2], h[q].x, h[q].y,
+                                           h[q].z, hyp[q], hzp[q], hj.x, hj.z, hx_k, hy_k);
+        stf(io.hn[0], c, h[q].x);
+        stf(io.hn[1], c, h[q].y);
+        stf(io.hn[2], c, h[q].z);
+        stf(io.en[0], c, e.x);
+        stf(io.en[1], c, e.y);
+        stf(io.en[2], c, e.z);
       }
+      hyp[q] = h[q].y;
+      hzp[q] = h[q].z;
     }
-    const float hx_k = __shfl_up(h.x, 1), hy_k = __shfl_up(h.y, 1);
-    if (inner) {
-      const float3 e = e_core_co<GRADED>(p, ps, i, j, k, c, coef(m), flag(m), a0, a1, a2, h.x,
-                                         h.y, h.z, hyp, hzp, hm.x, hm.z, hx_k, hy_k);
-      stf(io.hn[0], c, h.x);
-      stf(io.hn[1], c, h.y);
-      stf(io.hn[2], c, h.z);
-      stf(io.en[0], c, e.x);
-      stf(io.en[1], c, e.y);
-      stf(io.en[2], c, e.z);
-    }
-    hyp = h.y;
-    hzp = h.z;
     // ---- rotate the pipeline
-    a0 = n0;
-    a1 = n1;
-    a2 = n2;
-    b0 = q0;
-    b1 = q1;
-    b2 = q2;
-    n0 = N0;
-    n1 = N1;
-    n2 = N2;
-    q0 = Q0;
-    q1 = Q1;
-    q2 = Q2;
-    h0 = H0;
-    h1 = H1;
-    h2 = H2;
-    g0 = G0;
-    g2 = G2;
-    m = M;
-    mb = MB;
-    ej0 = EJ0;
-    ej2 = EJ2;
-    ek0 = EK0;
-    ek1 = EK1;
-    fk0 = FK0;
-    fk1 = FK1;
+    w = nw;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      w.e[0][c] = en[0][c];
+      w.e[1][c] = en[1][c];
+      w.eb[c] = enb[c];
+      en[0][c] = nn[0][c];
+      en[1][c] = nn[1][c];
+      enb[c] = nnb[c];
+    }
   }
 }
 
@@ -1745,7 +1744,7 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     }
     const KParams& kp = pl->kp;
     io.n_mat = pl->n_mat;
-    io.ntj = kp.ny;                                          // one row per workgroup
+    io.ntj = (kp.ny + 1) / 2;                                // one row pair per workgroup
     io.ntk = ((kp.nz + kWMK - 1) / kWMK + kWMW - 1) / kWMW;  // groups of kWMW z tiles
     const long long tiles = (long long)io.ntj * io.ntk;
     int nseg = pl->wmarch_seg;
