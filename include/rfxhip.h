@@ -220,11 +220,9 @@ typedef struct rfxh_batch rfxh_batch;
 int rfxh_batch_create(rfxh_plan* const* plans, int n_plans, rfxh_batch** out);
 int rfxh_batch_destroy(rfxh_batch* batch);
 /* Instance q advances n_steps[q] steps from absolute step step0[q]
- * (two-pass sub-steps on its current field set, each instance in its
- * plan's workgroup order; 2 cells per thread along x when every instance's
- * plan variant is the 2-cell column step, kernel 4, else one cell per
- * thread).  use_graph != 0 replays captured chunks of steps wherever the
- * set of active instances is constant. */
+ * (two-pass sub-steps on its current field set, one cell per thread, each
+ * instance in its plan's workgroup order).  use_graph != 0 replays captured
+ * chunks of steps wherever the set of active instances is constant. */
 int rfxh_batch_run(rfxh_batch* batch, const int64_t* step0, const int64_t* n_steps, void* stream,
                    int use_graph);
 

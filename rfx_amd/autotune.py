@@ -135,16 +135,17 @@ def autotune(dr, trial_steps: int = 32, repeats: int = 2, cache_file: str | None
 
 
 def autotune_batch(batch, trial_steps: int = 32):
-    """Pick the batched step's layout for every instance of an
-    rfx_amd.batch.Batch -- one or 2 cells per thread along x (two-pass
-    variants 0 and 4), flat or XCD-band workgroup order -- by timing
-    ``trial_steps`` batched steps of each.  Leaves the instances reset."""
+    """Pick the workgroup order (flat or XCD bands) of the batched step for
+    every instance of an rfx_amd.batch.Batch by timing ``trial_steps``
+    batched steps of each; the batched step runs one cell per thread, so the
+    instances are set to the one-cell two-pass variant in that order.
+    Leaves the instances reset."""
     n = len(batch.runs)
     step0 = (C.c_int64 * n)(*([0] * n))
     counts = (C.c_int64 * n)(*[min(trial_steps, s) for s in batch.n_steps])
     stream = torch.cuda.current_stream(batch.device)
     timings = {}
-    for cand in ((0, 1), (0, 0), (4, 1), (4, 0)):
+    for cand in ((0, 1), (0, 0)):
         for r in batch.runs:
             set_variant(r, *cand)
         for rep in range(2):
