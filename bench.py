@@ -196,6 +196,8 @@ def graded_run(n_steps=8000):
     grid, prob, ports = sim._build_problem(n_steps, "run",
                                            s11_freqs=np.asarray(spec["s_freqs"], np.float32))
     dr = DeviceRun(prob, n_steps, "cuda:0", two_sets=False)
+    from rfx_amd import autotune as at
+    tune = at.autotune(dr, trial_steps=32)
     dr.run(0, min(n_steps, 64))
     torch.cuda.synchronize()
     dr.reset()
@@ -209,6 +211,7 @@ def graded_run(n_steps=8000):
     return {"workload": f"graded patch {grid.shape[0]}x{grid.shape[1]}x{grid.shape[2]} cells, "
                         f"dz {float(grid.d64[2].min()) * 1e3:.2f}-{float(grid.d64[2].max()) * 1e3:.2f}"
                         f" mm, wire port, {n_steps} steps",
+            "variant": at.variant_name(tuple(tune["variant"])),
             "Mcell_steps_per_s": round(n_steps * cells / el / 1e6, 1), "seconds": round(el, 3),
             "port_samples_finite": finite}
 
