@@ -73,6 +73,7 @@ struct KParams {
   const uint16_t* mat;
   const float4* coef;            // ca, cb, ce, ch per material id
   const uint32_t* flags;
+  const float4* ecoef;           // ca, cb, ce, flag bits (as float bits): the E update's row
   float* f[6];                   // two-pass kernels: the field set updated in place
   int cn[6];                     // CPML slab depth per face
   float* psi_h[6][2];            // CPML memory of set 0 (set 1 at + psi_set)
@@ -158,6 +159,11 @@ __device__ __forceinline__ unsigned ldu16(const uint16_t* base, unsigned cell) {
 template <class T>
 __device__ __forceinline__ T gld(const T* base, int idx) {
   return ((const RFX_G T*)base)[idx];
+}
+// ch of material m alone (one dword instead of the 16-byte row: the H
+// update needs nothing else; measured 1.5 % off the H pass)
+__device__ __forceinline__ float ld_ch(const float4* coef, unsigned m) {
+  return gld(reinterpret_cast<const float*>(coef), 4 * (int)m + 3);
 }
 __device__ __forceinline__ float4 gld(const float4* base, int idx) {
   typedef float v4 __attribute__((ext_vector_type(4)));
@@ -371,7 +377,7 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
   float ex_k = ldf(Eo[0], ck), ey_k = ldf(Eo[1], ck);
   float ey_i = ldf(Eo[1], ci), ez_i = ldf(Eo[2], ci);
   const float hx = ldf(Ho[0], c), hy = ldf(Ho[1], c), hz = ldf(Ho[2], c);
-  const float ch = gld(p.coef, m).w;
+  const float ch = ld_ch(p.coef, m);
   ex_j = hj ? ex_j : 0.f;
   ez_j = hj ? ez_j : 0.f;
   ex_k = hk ? ex_k : 0.f;
@@ -473,8 +479,10 @@ __device__ __forceinline__ float3 e_core(const KParams& p, const PsiIO& ps, int 
                                          unsigned c, unsigned m, float e0, float e1, float e2,
                                          float hx, float hy, float hz, float hy_i, float hz_i,
                                          float hx_j, float hz_j, float hx_k, float hy_k) {
-  return e_core_co<GRADED>(p, ps, i, j, k, c, gld(p.coef, m), gld(p.flags, m), e0, e1, e2, hx, hy, hz, hy_i,
-                           hz_i, hx_j, hz_j, hx_k, hy_k);
+  // one 16-byte row carries everything the E update needs
+  const float4 ec = gld(p.ecoef, (int)m);
+  return e_core_co<GRADED>(p, ps, i, j, k, c, ec, __float_as_uint(ec.w), e0, e1, e2, hx, hy, hz,
+                           hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
 }
 
 // E at one cell reading E^n from Eo, storing E^{n+1} into En.
@@ -988,7 +996,7 @@ __device__ __forceinline__ void col_h(const KParams& p, const PsiIO& ps, int i0,
     const float exk_ = hk ? exk[q] : 0.f, eyk_ = hk ? eyk[q] : 0.f;
     const float eyi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ey[q + 1] : 0.f) : ey[NP];
     const float ezi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ez[q + 1] : 0.f) : ez[NP];
-    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, gld(p.coef, (int)m[q]).w, ex[q], ey[q], ez[q],
+    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, ld_ch(p.coef, m[q]), ex[q], ey[q], ez[q],
                                           exj_, ezj_, exk_, eyk_, eyi, ezi, hx[q], hy[q], hz[q]);
     const unsigned c = c0 + (unsigned)q * P;
     stf(p.f[3], c, h.x);
@@ -1938,6 +1946,14 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     uint32_t* fl;
     if ((rc = dev_copy(pl, d->mat_flags, (size_t)d->n_mat, &fl))) return bail(rc);
     kp.flags = fl;
+    for (int m = 0; m < d->n_mat; ++m) {
+      float bits;
+      std::memcpy(&bits, &d->mat_flags[m], sizeof(bits));
+      tmp[m].w = bits;
+    }
+    float4* ec;
+    if ((rc = dev_copy(pl, tmp.data(), tmp.size(), &ec))) return bail(rc);
+    kp.ecoef = ec;
   }
   // default: the two-pass step with 2-cell columns along x, the fastest
   // variant measured on MI355X at 533^3 and on the graded workloads;
