@@ -174,9 +174,28 @@ __device__ __forceinline__ float4 gld(const float4* base, int idx) {
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so each XCD
 // receives a contiguous range of logical workgroup ids.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
-  const int per = nb / 8, rem = nb % 8;
-  const int x = b % 8, s = b / 8;
-  return x < rem ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s;
+  const unsigned ub = (unsigned)b, un = (unsigned)nb;
+  const unsigned per = un >> 3, rem = un & 7u;
+  const unsigned x = ub & 7u, s = ub >> 3;
+  return (int)(x < rem ? x * (per + 1) + s : rem * (per + 1) + (x - rem) * per + s);
+}
+
+// q = n / d, r = n % d for wave-uniform 0 <= n < 2^31, d >= 1: the product
+// with the hardware double reciprocal (relative error near 2^-50, so the
+// estimate is within one of the quotient) corrected once in integers --
+// a few instructions instead of the generic integer-division sequence,
+// which every workgroup of the step kernels runs before its first load.
+__device__ __forceinline__ int div_fast(int n, int d, int& r) {
+  int q = (int)((double)n * __builtin_amdgcn_rcp((double)d));
+  r = n - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  return q;
 }
 
 // Cell of this thread in logical workgroup L of the problem p.  All index
@@ -194,18 +213,19 @@ __device__ __forceinline__ bool cell_at(const KParams& p, int L, int& i, int& j,
   if (p.order == 0) {
     // bands: the (y, z) plane is cut into 8 flat bands, one per XCD; the XCD
     // sweeps its band plane by plane with 512-cell chunks fastest
-    const int bw = (pl + 7) / 8;
-    const int ncb = (bw + kWG - 1) / kWG;
-    const int b = L / (nxb * ncb);
-    const int r = L - b * (nxb * ncb);
-    i = r / ncb;
-    f = b * bw + (r - i * ncb) * kWG + t;
+    const int bw = (int)(((unsigned)pl + 7u) >> 3);
+    const int ncb = (int)(((unsigned)bw + kWG - 1) / kWG);
+    int r, c;
+    const int b = div_fast(L, nxb * ncb, r);
+    i = div_fast(r, ncb, c);
+    f = b * bw + c * kWG + t;
     if (f >= (b + 1) * bw) return false;
   } else {
     // flat chunks of 512 cells, chunk index fastest, then plane
-    const int nch = (pl + kWG - 1) / kWG;
-    i = L / nch;
-    f = (L - i * nch) * kWG + t;
+    const int nch = (int)(((unsigned)pl + kWG - 1) / kWG);
+    int c;
+    i = div_fast(L, nch, c);
+    f = c * kWG + t;
   }
   if (f >= pl) return false;
   int jj = (int)((float)f * p.inv_nz);
