@@ -338,6 +338,10 @@ __device__ __forceinline__ float3 h_core(const KParams& p, const PsiIO& ps, int 
   hy = hy - ch * ((ex_k - ex) * iz - (ez_i - ez) * ix);
   hz = hz - ch * ((ey_i - ey) * ix - (ex_j - ex) * iy);
   if (in_slab(p, i, j, k)) {
+    // keep the CPML pointer and profile loads inside the slab branch: the
+    // compiler would otherwise hoist them into every wavefront (and spill
+    // them to VGPR lanes), slab cells or not
+    __asm__ volatile("" ::: "memory");
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
       const int l = slab_layer(p, face, i, j, k);
@@ -437,6 +441,10 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
   e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
   e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
   if (in_slab(p, i, j, k)) {
+    // keep the CPML pointer and profile loads inside the slab branch: the
+    // compiler would otherwise hoist them into every wavefront (and spill
+    // them to VGPR lanes), slab cells or not
+    __asm__ volatile("" ::: "memory");
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
       const int l = slab_layer(p, face, i, j, k);
@@ -1075,8 +1083,14 @@ __device__ __forceinline__ void col_e(const KParams& p, const PsiIO& ps, int i0,
   }
 }
 
+// The CPML pointers come through device memory (psp) rather than as
+// kernel arguments: as arguments the compiler loads all 48 of them up front
+// and spills them to VGPR lanes in every wavefront, although only the
+// wavefronts with slab cells use them.
 template <bool GRADED, int NP>
-__global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 8 : 1)))
+void k_update_h_col(KParams p, const PsiIO* __restrict__ psp) {
+  const PsiIO& ps = *psp;
   int ib, j, k;
   if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, (p.nx + NP - 1) / NP))
     return;
@@ -1087,7 +1101,8 @@ __global__ __launch_bounds__(kWG) void k_update_h_col(KParams p, PsiIO ps) {
 // compiler's own choice, 67, allows 7) -- measured 3.7 % off the E pass.
 template <bool GRADED, int NP>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 8 : 1)))
-void k_update_e_col(KParams p, PsiIO ps) {
+void k_update_e_col(KParams p, const PsiIO* __restrict__ psp) {
+  const PsiIO& ps = *psp;
   int ib, j, k;
   if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, (p.nx + NP - 1) / NP))
     return;
@@ -1636,6 +1651,7 @@ struct rfxh_plan {
   int wmarch_seg = 0;           // its x segments (0: automatic)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
+  PsiIO* d_psio = nullptr;      // in-place CPML read/write pointers of set 0 and set 1 (device)
   int cur = 0;                  // which set holds the current state
   std::vector<PlaneParams> planes;
   std::vector<int> plane_comp;
@@ -1718,36 +1734,40 @@ void launch_accumulators(rfxh_plan* pl, hipStream_t s, int set) {
 }
 
 template <int PASS, bool GRADED, int NP>
-void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps) {
+void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps,
+                   int set) {
   const dim3 g = NP == 1 ? pl->grid : pl->grid_col, blk = pl->block;
   if (NP == 1) {
     if (PASS == 0) hipLaunchKernelGGL(k_update_h<GRADED>, g, blk, 0, s, kp, ps);
     else hipLaunchKernelGGL(k_update_e<GRADED>, g, blk, 0, s, kp, ps);
   } else {
-    if (PASS == 0) hipLaunchKernelGGL((k_update_h_col<GRADED, NP>), g, blk, 0, s, kp, ps);
-    else hipLaunchKernelGGL((k_update_e_col<GRADED, NP>), g, blk, 0, s, kp, ps);
+    const PsiIO* psp = pl->d_psio + set;
+    if (PASS == 0) hipLaunchKernelGGL((k_update_h_col<GRADED, NP>), g, blk, 0, s, kp, psp);
+    else hipLaunchKernelGGL((k_update_e_col<GRADED, NP>), g, blk, 0, s, kp, psp);
   }
 }
 
 template <int PASS, bool GRADED>
-void launch_pass_g(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps) {
+void launch_pass_g(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps,
+                   int set) {
   switch (pl->ncol) {
-    case 2: launch_pass_n<PASS, GRADED, 2>(pl, s, kp, ps); break;
-    case 3: launch_pass_n<PASS, GRADED, 3>(pl, s, kp, ps); break;
-    case 4: launch_pass_n<PASS, GRADED, 4>(pl, s, kp, ps); break;
-    default: launch_pass_n<PASS, GRADED, 1>(pl, s, kp, ps); break;
+    case 2: launch_pass_n<PASS, GRADED, 2>(pl, s, kp, ps, set); break;
+    case 3: launch_pass_n<PASS, GRADED, 3>(pl, s, kp, ps, set); break;
+    case 4: launch_pass_n<PASS, GRADED, 4>(pl, s, kp, ps, set); break;
+    default: launch_pass_n<PASS, GRADED, 1>(pl, s, kp, ps, set); break;
   }
 }
 
-// One two-pass half step (pass 0: H, pass 1: E) in place on kp.f, with the
-// plan's column width.
-void launch_pass(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps, int pass) {
+// One two-pass half step (pass 0: H, pass 1: E) in place on field set `set`
+// (kp.f and ps point at it), with the plan's column width.
+void launch_pass(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const PsiIO& ps, int pass,
+                 int set) {
   if (pass == 0) {
-    if (kp.graded) launch_pass_g<0, true>(pl, s, kp, ps);
-    else launch_pass_g<0, false>(pl, s, kp, ps);
+    if (kp.graded) launch_pass_g<0, true>(pl, s, kp, ps, set);
+    else launch_pass_g<0, false>(pl, s, kp, ps, set);
   } else {
-    if (kp.graded) launch_pass_g<1, true>(pl, s, kp, ps);
-    else launch_pass_g<1, false>(pl, s, kp, ps);
+    if (kp.graded) launch_pass_g<1, true>(pl, s, kp, ps, set);
+    else launch_pass_g<1, false>(pl, s, kp, ps, set);
   }
 }
 
@@ -1830,9 +1850,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   } else {
     KParams kp = pl->kp;
     for (int c = 0; c < 6; ++c) kp.f[c] = pl->set[from][c];
-    launch_pass(pl, s, kp, ps, 0);
+    launch_pass(pl, s, kp, ps, 0, from);
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    launch_pass(pl, s, kp, ps, 1);
+    launch_pass(pl, s, kp, ps, 1, from);
   }
   if (ev) HIPCHK(hipEventRecord(ev[2], s));
   launch_accumulators(pl, s, to);
@@ -1885,6 +1905,8 @@ int64_t rfxh_psi_floats(const rfxh_desc* d) {
   for (int f = 0; f < 6; ++f) tot += 4 * face_slab_floats(d, f);
   return tot;
 }
+
+static PsiIO psi_io(const rfxh_plan* pl, int set);
 
 int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   if (!d || !out) return fail(RFXH_EINVAL, "null descriptor or output");
@@ -2014,6 +2036,10 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   }
   // CPML set 1 is addressed as set-0 pointer + psi_set floats
   pl->psi_set = (has2 && any_cpml) ? (long long)(d->d_psi2 - d->d_psi) : 0;
+  {
+    const PsiIO both[2] = {psi_io(pl, 0), psi_io(pl, has2 ? 1 : 0)};
+    if ((rc = dev_copy(pl, both, 2, &pl->d_psio))) return bail(rc);
+  }
   kp.pec_axes = d->pec_axes;
   {
     rfxh_point_op* dops;
@@ -2332,12 +2358,12 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
   const PsiIO ps = psi_io(pl, pl->cur);
   const unsigned rb = (unsigned)((rp.n + 255) / 256);
   for (int64_t n = 0; n < n_steps; ++n) {
-    launch_pass(pl, s, kp, ps, 0);
+    launch_pass(pl, s, kp, ps, 0, pl->cur);
     if (kp.graded)
       hipLaunchKernelGGL(k_record<true>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
     else
       hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
-    launch_pass(pl, s, kp, ps, 1);
+    launch_pass(pl, s, kp, ps, 1, pl->cur);
     launch_accumulators(pl, s, pl->cur);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
   }
