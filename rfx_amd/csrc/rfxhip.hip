@@ -112,11 +112,15 @@ struct PsiIO {
   float* ew[6][2];
 };
 
+constexpr int kDftK = 16;   // steps staged per fold of the DFT-plane accumulators
+
 struct PlaneParams {
   const float* fld;
   double2* acc;
   const double2* tw;  // [max_steps][nf] twiddles exp(-j 2 pi f t) dt
   long long* step;
+  float* snap;        // [kDftK][n1*n2] ring of plane values not yet folded in
+  const long long* acc_from;  // first step not yet folded into acc (device)
   long long max_steps;
   int nf;
   int axis, index;
@@ -1172,6 +1176,38 @@ __global__ void k_batch_set_steps(const BatchEntry* __restrict__ ents, const lon
 // ---------------------------------------------------------------------------
 // DFT plane: acc[f][u][v] += tw[n][f] * F(plane)  (float64 sums)
 // ---------------------------------------------------------------------------
+// The plane values of kDftK consecutive steps are staged in a ring and
+// folded into the float64 accumulators together, so the accumulators (16 B
+// per frequency and plane cell) are read and written once per kDftK steps
+// instead of every step.  The additions happen in the same order as one
+// step at a time (acc, then step lo, lo+1, ...), so the sums are unchanged.
+__device__ __forceinline__ void dft_fold(const PlaneParams& q, long long pidx, long long npts,
+                                         long long lo, long long hi, bool hi_in_reg, float xhi) {
+  float xs[kDftK];
+  const int cnt = (int)(hi - lo + 1);
+#pragma unroll
+  for (int m = 0; m < kDftK; ++m) {
+    if (m < cnt) {
+      const long long st = lo + m;
+      xs[m] = (hi_in_reg && st == hi) ? xhi : q.snap[(st % kDftK) * npts + pidx];
+    }
+  }
+  for (int f = 0; f < q.nf; ++f) {
+    double2* a = q.acc + (long long)f * npts + pidx;
+    double2 cur = *a;
+#pragma unroll
+    for (int m = 0; m < kDftK; ++m) {
+      if (m < cnt) {
+        const double2 w = q.tw[(lo + m) * q.nf + f];
+        const double x = (double)xs[m];
+        cur.x += w.x * x;
+        cur.y += w.y * x;
+      }
+    }
+    *a = cur;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dft_plane(PlaneParams q) {
   const long long npts = (long long)q.n1 * q.n2;
   const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1184,16 +1220,32 @@ __global__ __launch_bounds__(256) void k_dft_plane(PlaneParams q) {
   ii[q.axis] = q.index;
   ii[a1] = u;
   ii[a2] = v;
-  const double x = (double)q.fld[((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2]];
-  const double2* tw = q.tw + n * q.nf;
-  for (int f = 0; f < q.nf; ++f) {
-    const double2 w = tw[f];
-    double2* a = q.acc + (long long)f * npts + pidx;
-    double2 cur = *a;
-    cur.x += w.x * x;
-    cur.y += w.y * x;
-    *a = cur;
+  const float x = q.fld[((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2]];
+  const int slot = (int)(n % kDftK);
+  if (slot == kDftK - 1) {
+    const long long lo = max(*q.acc_from, n - (kDftK - 1));
+    dft_fold(q, pidx, npts, lo, n, true, x);
+  } else {
+    q.snap[slot * npts + pidx] = x;
   }
+}
+
+// Folds the staged steps of the last, incomplete group (end of a run).
+__global__ __launch_bounds__(256) void k_dft_flush(PlaneParams q) {
+  const long long npts = (long long)q.n1 * q.n2;
+  const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= npts) return;
+  const long long last = *q.step - 1;
+  if (last < 0) return;
+  const int slot = (int)(last % kDftK);
+  if (slot == kDftK - 1) return;   // folded by k_dft_plane
+  const long long lo = max(*q.acc_from, last - slot);
+  if (lo > last) return;
+  dft_fold(q, pidx, npts, lo, last, false, 0.f);
+}
+
+__global__ void k_set_acc_from(long long* acc_from, const long long* step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *acc_from = *step;
 }
 
 // ---------------------------------------------------------------------------
@@ -1651,6 +1703,7 @@ struct rfxh_plan {
   int wmarch_seg = 0;           // its x segments (0: automatic)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
+  long long* acc_from = nullptr;  // first step not yet folded into the DFT planes (device)
   PsiIO* d_psio = nullptr;      // in-place CPML read/write pointers of set 0 and set 1 (device)
   int cur = 0;                  // which set holds the current state
   std::vector<PlaneParams> planes;
@@ -1731,6 +1784,23 @@ void launch_accumulators(rfxh_plan* pl, hipStream_t s, int set) {
     const long long npts = (long long)fp.n1 * fp.n2;
     hipLaunchKernelGGL(k_flux, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, fp);
   }
+}
+
+// Start of a run at step0: steps before it are already folded into the
+// DFT-plane accumulators.
+void dft_begin(rfxh_plan* pl, hipStream_t s, long long step0) {
+  if (pl->acc_from) hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, pl->acc_from, step0);
+}
+
+// End of a run: fold the staged steps of the last incomplete group.
+void dft_flush(rfxh_plan* pl, hipStream_t s) {
+  if (pl->planes.empty()) return;
+  for (size_t q = 0; q < pl->planes.size(); ++q) {
+    const PlaneParams& pp = pl->planes[q];
+    const long long npts = (long long)pp.n1 * pp.n2;
+    hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
+  }
+  hipLaunchKernelGGL(k_set_acc_from, dim3(1), dim3(64), 0, s, pl->acc_from, pl->kp.step);
 }
 
 template <int PASS, bool GRADED, int NP>
@@ -2070,6 +2140,11 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pp.fld = d->d_f[pd.comp];
     pp.acc = reinterpret_cast<double2*>(pd.d_acc);
     pp.step = kp.step;
+    if (!pl->acc_from) {
+      const long long zero = 0;
+      if ((rc = dev_copy(pl, &zero, 1, &pl->acc_from))) return bail(rc);
+    }
+    pp.acc_from = pl->acc_from;
     pp.max_steps = d->max_steps;
     pp.nf = pd.n_freq;
     pp.axis = pd.axis;
@@ -2078,6 +2153,10 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pp.n2 = N[pd.axis == 2 ? 1 : 2];
     pp.ny = d->ny;
     pp.nz = d->nz;
+    {
+      const std::vector<float> ring((size_t)kDftK * pp.n1 * pp.n2, 0.f);
+      if ((rc = dev_copy(pl, ring.data(), ring.size(), &pp.snap))) return bail(rc);
+    }
     std::vector<double2> tw((size_t)d->max_steps * pd.n_freq);
     for (long long n = 0; n < d->max_steps; ++n) {
       const double t = (double)(n + d->plane_t_offset) * d->dt;
@@ -2198,6 +2277,7 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
     return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, pl->kp.step, (long long)step0);
+  dft_begin(pl, s, step0);
   HIPCHK(hipGetLastError());
   if (n_steps == 0) return RFXH_OK;
   const int flip = pl->fused ? 1 : 0;
@@ -2215,6 +2295,8 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
       if (rc) return rc;
       pl->cur ^= flip;
     }
+    dft_flush(pl, s);
+    HIPCHK(hipGetLastError());
     if (n_samples) return collect_timing(pl, n_samples);
     return RFXH_OK;
   }
@@ -2252,6 +2334,8 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
     if (rc) return rc;
     pl->cur ^= flip;
   }
+  dft_flush(pl, pl->cap_stream);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(pl->ev_out, pl->cap_stream));
   HIPCHK(hipStreamWaitEvent(s, pl->ev_out, 0));
   return RFXH_OK;
@@ -2351,6 +2435,7 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
   hipStream_t s = static_cast<hipStream_t>(stream);
   const RegionParams rp = region_params(region);
   hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, pl->kp.step, (long long)step0);
+  dft_begin(pl, s, step0);
   // two-pass sub-steps in place on the current set, with the record taken
   // between the H and the E update (E^n and H^{n+1/2} both in memory)
   KParams kp = pl->kp;
@@ -2367,6 +2452,7 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
     launch_accumulators(pl, s, pl->cur);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
   }
+  dft_flush(pl, s);
   HIPCHK(hipGetLastError());
   return RFXH_OK;
 }
@@ -2574,6 +2660,7 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
   HIPCHK(hipMemcpyAsync(b->d_steps, st.data(), sizeof(long long) * n, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_batch_set_steps, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, b->d_ents,
                      b->d_steps, n);
+  for (int q = 0; q < n; ++q) dft_begin(b->plans[q], s, step0[q]);
   HIPCHK(hipGetLastError());
   if (use_graph && !b->cap_stream) {
     HIPCHK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
@@ -2620,6 +2707,8 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
       if (rc) return rc;
     }
   }
+  for (int q = 0; q < n; ++q) dft_flush(b->plans[q], w);
+  HIPCHK(hipGetLastError());
   if (use_graph) {
     HIPCHK(hipEventRecord(b->ev_out, b->cap_stream));
     HIPCHK(hipStreamWaitEvent(s, b->ev_out, 0));
