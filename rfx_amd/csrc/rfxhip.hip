@@ -759,7 +759,11 @@ struct WMPlane {
 };
 
 template <bool GRADED>
-__global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO io, PsiIO ps) {
+__global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO io,
+                                                            const PsiIO* __restrict__ psp) {
+  // CPML pointers through device memory: as kernel arguments they are
+  // hoisted out of the march loop and spilled to VGPR lanes
+  const PsiIO& ps = *psp;
   __shared__ float4 s_coef[kWMMat];
   __shared__ uint32_t s_flag[kWMMat];
   // the plan rejects this variant for more than kWMMat materials
@@ -1881,8 +1885,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     io.seg_len = (kp.nx + nseg - 1) / nseg;
     nseg = (kp.nx + io.seg_len - 1) / io.seg_len;
     const dim3 g((unsigned)(tiles * nseg)), blk(kBX, kWMW);
-    if (kp.graded) hipLaunchKernelGGL(k_step_wmarch<true>, g, blk, 0, s, kp, io, ps);
-    else hipLaunchKernelGGL(k_step_wmarch<false>, g, blk, 0, s, kp, io, ps);
+    const PsiIO* psp = pl->d_psio + 2 + from;   // read set `from`, write set `to`
+    if (kp.graded) hipLaunchKernelGGL(k_step_wmarch<true>, g, blk, 0, s, kp, io, psp);
+    else hipLaunchKernelGGL(k_step_wmarch<false>, g, blk, 0, s, kp, io, psp);
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
   } else if (pl->march) {
     MarchIO io;
@@ -2107,8 +2112,18 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   // CPML set 1 is addressed as set-0 pointer + psi_set floats
   pl->psi_set = (has2 && any_cpml) ? (long long)(d->d_psi2 - d->d_psi) : 0;
   {
-    const PsiIO both[2] = {psi_io(pl, 0), psi_io(pl, has2 ? 1 : 0)};
-    if ((rc = dev_copy(pl, both, 2, &pl->d_psio))) return bail(rc);
+    // [0], [1]: in place on set 0 / 1; [2], [3]: read set 0 / 1, write the
+    // other (the ping-pong steps)
+    const PsiIO s0 = psi_io(pl, 0), s1 = psi_io(pl, has2 ? 1 : 0);
+    PsiIO tab[4] = {s0, s1, s0, s1};
+    for (int f = 0; f < 6; ++f)
+      for (int t = 0; t < 2; ++t) {
+        tab[2].hw[f][t] = s1.hw[f][t];
+        tab[2].ew[f][t] = s1.ew[f][t];
+        tab[3].hw[f][t] = s0.hw[f][t];
+        tab[3].ew[f][t] = s0.ew[f][t];
+      }
+    if ((rc = dev_copy(pl, tab, 4, &pl->d_psio))) return bail(rc);
   }
   kp.pec_axes = d->pec_axes;
   {
