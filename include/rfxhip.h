@@ -77,6 +77,26 @@ typedef struct {
   double* d_acc;       /* device complex128 [4][n_freq][n1][n2]: tangential E1, E2, H1, H2 */
 } rfxh_flux;
 
+/* Near-to-far-field (Huygens) box: DFTs of the tangential fields on its six
+ * faces (rfx/farfield.py:32-80 NTFFBox, :208-313 accumulate_ntff; in-scan
+ * rfx/simulation.py:1570-1572, rfx/nonuniform.py:1524-1527).  Faces lie on
+ * the index planes lo[a] and hi[a] of each axis and cover the half-open
+ * ranges [lo, hi) of the other two axes.  Per face, in this order,
+ *   x faces (Ey, Ez, Hy, Hz) over [lo1, hi1) x [lo2, hi2) = j x k,
+ *   y faces (Ex, Ez, Hx, Hz) over i x k,
+ *   z faces (Ex, Ey, Hx, Hy) over i x j,
+ * E at t = n dt and H at t = (n + 1/2) dt for step n (scan index), the
+ * fields of the plane itself (no averaging), float64 sums (the reference
+ * uses complex64 with Kahan compensation). */
+typedef struct {
+  int32_t lo[3];       /* i_lo, j_lo, k_lo */
+  int32_t hi[3];       /* i_hi, j_hi, k_hi (hi > lo, hi < n_axis) */
+  int32_t n_freq;
+  const double* freqs; /* host, n_freq Hz */
+  double* d_acc[6];    /* device complex128 [4][n_freq][n1][n2] per face x_lo, x_hi, y_lo, y_hi,
+                          z_lo, z_hi */
+} rfxh_ntff;
+
 typedef struct {
   int32_t nx, ny, nz;
   int32_t graded;          /* 0: uniform spacing; 1: graded (per-index inverse spacings) */
@@ -125,6 +145,9 @@ typedef struct {
      the planes index-1 and index at t - dt/2; float64 sums */
   int32_t n_flux;
   const rfxh_flux* flux;
+  /* near-to-far-field boxes */
+  int32_t n_ntff;
+  const rfxh_ntff* ntff;
 } rfxh_desc;
 
 typedef struct rfxh_plan rfxh_plan;
@@ -163,8 +186,9 @@ int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* cou
  * 2 / 3: marching step with 6- / 14-row tiles (param = number of x
  * segments, 0 = automatic); 4 / 6 / 5: two-pass step with columns of
  * 2 / 3 / 4 cells along x per thread (param = workgroup order); 7: wave-march
- * step, each wavefront walking one row along x with no barriers (param = x
- * segments, 0 = automatic).  Kernels 1-3 and 7 need the second field set.
+ * step, each wavefront walking a pair of rows along x with no barriers
+ * (param = x segments, 0 = automatic; at most 1024 material ids).  Kernels
+ * 1-3 and 7 need the second field set.
  * Variants compute the same step; they differ in speed only.  The
  * project's autotuner (rfx_amd/autotune.py) times them on the actual
  * problem.  No counterpart in the reference (its XLA lowering has no
@@ -225,6 +249,37 @@ int rfxh_batch_destroy(rfxh_batch* batch);
  * chunks of steps wherever the set of active instances is constant. */
 int rfxh_batch_run(rfxh_batch* batch, const int64_t* step0, const int64_t* n_steps, void* stream,
                    int use_graph);
+
+/* ---- near-to-far-field transform ----
+ * Far-field radiation of an NTFF box from its accumulated face DFTs
+ * (rfx/farfield.py:393-556 compute_far_field, post-processing in NumPy in
+ * the reference).  Per face with outward normal n (sign s = -1 on lo faces,
+ * +1 on hi faces): J = n x H, M = -n x E; per frequency k = 2 pi f / c0 and
+ * direction r(theta, phi):
+ *   N = sum J e^{j k r.p} dS,  L = sum M e^{j k r.p} dS,
+ *   E_theta = -j k / (4 pi) (L_phi + eta0 N_theta),
+ *   E_phi   =  j k / (4 pi) (L_theta - eta0 N_phi)
+ * with p the position of the face cell (pos[a][index]) and dS the product
+ * of the widths (width[a][index]) of its two tangential indices. */
+typedef struct {
+  int32_t lo[3], hi[3];      /* the box, as in rfxh_ntff */
+  int32_t n_freq;
+  const double* freqs;       /* host, n_freq Hz */
+  const double* d_acc[6];    /* device, the rfxh_ntff accumulators (read only) */
+  int32_t n_axis[3];         /* lengths of pos[a] / width[a] (the grid shape) */
+  const double* pos[3];      /* host: position (m) of each index along each axis */
+  const double* width[3];    /* host: cell width (m) of each index along each axis */
+  double c0;                 /* speed of light (m/s) */
+  double eta0;               /* free-space impedance (ohm) */
+} rfxh_farfield_desc;
+
+/* e_theta, e_phi: device complex128 [n_freq][n_theta][n_phi] (re, im
+ * interleaved), caller-owned.  theta, phi: host arrays (radians).  Copies
+ * the small host tables to the device, runs one kernel on `stream` and waits
+ * for it (the call returns with the outputs written). */
+int rfxh_ntff_far_field(const rfxh_farfield_desc* desc, int32_t n_theta, const double* theta,
+                        int32_t n_phi, const double* phi, double* e_theta, double* e_phi,
+                        void* stream);
 
 const char* rfxh_last_error(void);
 int rfxh_version(void);

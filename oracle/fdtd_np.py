@@ -28,6 +28,10 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   * graded mesh: a uniform profile must reproduce the uniform lane after the
     documented source-normalisation factor; PEC-cavity resonance closed
     form.
+  * NTFF box (_ntff, far_field): faces equal time-shifted DFT planes;
+    single-cell current elements (hand-derived n x H, -n x E); the
+    short-dipole directivity the reference's own test records (1.84 dBi,
+    4.78 dBi); far-field power against closed flux-monitor boxes.
 """
 from __future__ import annotations
 
@@ -429,6 +433,8 @@ class StepConfig:
     record_order: str = "uniform"                      # "uniform" | "graded"
     # flux monitors: (axis, index, lo1, hi1, lo2, hi2, freqs f32, weights f64 [n_steps+1] or None)
     flux: list = field(default_factory=list)
+    # near-to-far-field boxes: (lo (i, j, k), hi (i, j, k), freqs f32)
+    ntff: list = field(default_factory=list)
 
 
 class Stepper:
@@ -598,6 +604,7 @@ class RunOutput:
     fields: dict
     psi: dict
     flux: list = field(default_factory=list)   # per monitor (e1, e2, h1, h2) complex128 DFTs
+    ntff: list = field(default_factory=list)   # per box {face: (n_freqs, n1, n2, 4) complex128}
 
 
 def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
@@ -619,6 +626,8 @@ def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
     pi = [np.zeros(n_steps) for _ in cfg.ports_vi]
     fl = [tuple(np.zeros((len(m[6]), m[3] - m[2], m[5] - m[4]), np.complex128) for _ in range(4))
           for m in cfg.flux]
+    nt = [{name: np.zeros((len(fr),) + shp + (4,), np.complex128)
+           for name, shp in zip(NTFF_FACES, ntff_face_shapes(lo, hi))} for lo, hi, fr in cfg.ntff]
     dt = cfg.dt
     for n in range(n_steps):
         st.step_fields()
@@ -634,13 +643,14 @@ def run_loop(cfg: StepConfig, n_steps: int, dtype=np.float64, stepper=None):
             _planes(f, cfg, accs, n * dt, dt)
             _flux(f, cfg, fl, n, dt)
             _probe(f, cfg, ts, n)
+        _ntff(f, cfg, nt, n, dt)
     tvec = np.arange(n_steps) * dt
     vd, idft = [], []
     for p, (idx, comp, freqs, d_par, loop) in enumerate(cfg.ports_vi):
         ph = np.exp(-1j * 2.0 * np.pi * np.outer(np.asarray(freqs, np.float64), tvec)) * dt
         vd.append(ph @ pv[p])
         idft.append(ph @ pi[p])
-    return RunOutput(ts, accs, pv, pi, vd, idft, f, st.psi, fl)
+    return RunOutput(ts, accs, pv, pi, vd, idft, f, st.psi, fl, nt)
 
 
 def _plane_shape(shape, ax):
@@ -707,6 +717,115 @@ def _flux(f, cfg, fl, step, dt):
         e2 += ke[:, None, None] * sl(f[e2n], idx)[None]
         a1 += kh[:, None, None] * hv1.astype(np.float64)[None]
         a2 += kh[:, None, None] * hv2.astype(np.float64)[None]
+
+
+# rfx/farfield.py:81-84: tangential components stored per face normal
+NTFF_COMPONENTS = {0: ("ey", "ez", "hy", "hz"), 1: ("ex", "ez", "hx", "hz"),
+                   2: ("ex", "ey", "hx", "hy")}
+NTFF_FACES = ("x_lo", "x_hi", "y_lo", "y_hi", "z_lo", "z_hi")
+
+
+def ntff_face_shapes(lo, hi):
+    ni, nj, nk = (hi[a] - lo[a] for a in range(3))
+    return [(nj, nk), (nj, nk), (ni, nk), (ni, nk), (ni, nj), (ni, nj)]
+
+
+def _ntff(f, cfg, nt, n, dt):
+    """rfx/farfield.py:208-313 (accumulate_ntff) with float64 sums: per
+    face the tangential (E1, E2, H1, H2) of the plane itself over the box's
+    half-open ranges; E at t = n dt, H at t = n dt + dt/2 (n = scan index,
+    on both lanes: rfx/simulation.py:1570-1572, rfx/nonuniform.py:1527)."""
+    for q, (lo, hi, freqs) in enumerate(cfg.ntff):
+        fr = np.asarray(freqs, np.float64)
+        t = n * dt
+        pe = np.exp(-1j * 2.0 * np.pi * fr * t) * dt
+        ph = np.exp(-1j * 2.0 * np.pi * fr * (t + dt * 0.5)) * dt
+        kern = np.stack([pe, pe, ph, ph], axis=-1)[:, None, None, :]
+        rng = [slice(lo[a], hi[a]) for a in range(3)]
+        for face, name in enumerate(NTFF_FACES):
+            ax = face // 2
+            idx = hi[ax] if face & 1 else lo[ax]
+            sl = list(rng)
+            sl[ax] = idx
+            vals = np.stack([f[c][tuple(sl)].astype(np.float64) for c in NTFF_COMPONENTS[ax]],
+                            axis=-1)
+            nt[q][name] += kern * vals[None]
+
+
+def ntff_axis_tables(shape, dx, dy=None, dz=None, cpml_lo=(0, 0, 0)):
+    """Face positions and widths per axis (rfx/farfield.py:344-390 and
+    :456-462): (index - cpml_lo) * d on a uniform axis, the cumulative z
+    edges (float64 sums of the float32 dz profile, origin at cpml_lo) on a
+    graded z axis; widths d, or dz[k]."""
+    dy = dx if dy is None else dy
+    pos = [(np.arange(shape[0]) - cpml_lo[0]) * float(dx),
+           (np.arange(shape[1]) - cpml_lo[1]) * float(dy)]
+    width = [np.full(shape[0], float(dx)), np.full(shape[1], float(dy))]
+    if dz is None:
+        pos.append((np.arange(shape[2]) - cpml_lo[2]) * float(dx))
+        width.append(np.full(shape[2], float(dx)))
+    else:
+        d64 = np.asarray(dz, np.float64)
+        edges = np.concatenate([[0.0], np.cumsum(d64)])
+        pos.append((edges - edges[cpml_lo[2]])[:shape[2]])
+        width.append(d64)
+    return pos, width
+
+
+def far_field(faces, lo, hi, freqs, pos, width, theta, phi, c0=C0, eta0=None):
+    """rfx/farfield.py:393-556 (compute_far_field) in float64: surface
+    currents J = n x H, M = -n x E per face (outward normal n = -/+ axis on
+    the lo / hi face), N = sum J e^{j k r.p} dS, L = sum M e^{j k r.p} dS,
+    E_theta = -j k/(4 pi) (L_phi + eta0 N_theta),
+    E_phi = j k/(4 pi) (L_theta - eta0 N_phi).  Returns (E_theta, E_phi),
+    each (n_freqs, n_theta, n_phi)."""
+    if eta0 is None:
+        eta0 = float(np.sqrt(MU_0 / EPS_0))
+    theta = np.asarray(theta, np.float64)
+    phi = np.asarray(phi, np.float64)
+    fr = np.asarray(freqs, np.float64)
+    k = 2 * np.pi * fr / c0
+    TH, PH = np.meshgrid(theta, phi, indexing="ij")
+    st, ct, sp, cp = np.sin(TH), np.cos(TH), np.sin(PH), np.cos(PH)
+    r_hat = np.stack([st * cp, st * sp, ct], -1).reshape(-1, 3)
+    th_hat = np.stack([ct * cp, ct * sp, -st], -1).reshape(-1, 3)
+    ph_hat = np.stack([-sp, cp, np.zeros_like(st)], -1).reshape(-1, 3)
+    N = np.zeros((len(fr), r_hat.shape[0], 3), np.complex128)
+    L = np.zeros_like(N)
+    for face, name in enumerate(NTFF_FACES):
+        ax = face // 2
+        normal = np.zeros(3)
+        normal[ax] = 1.0 if face & 1 else -1.0
+        t1, t2 = [a for a in range(3) if a != ax]
+        data = np.asarray(faces[name], np.complex128)           # (nf, n1, n2, 4)
+        i1 = np.arange(lo[t1], hi[t1])
+        i2 = np.arange(lo[t2], hi[t2])
+        P = np.zeros((len(i1), len(i2), 3))
+        P[..., ax] = pos[ax][hi[ax] if face & 1 else lo[ax]]
+        P[..., t1] = pos[t1][i1][:, None]
+        P[..., t2] = pos[t2][i2][None, :]
+        dS = width[t1][i1][:, None] * width[t2][i2][None, :]
+        # tangential E and H as 3-vectors, then the cross products
+        Ev = np.zeros(data.shape[:3] + (3,), np.complex128)
+        Hv = np.zeros_like(Ev)
+        Ev[..., t1], Ev[..., t2] = data[..., 0], data[..., 1]
+        Hv[..., t1], Hv[..., t2] = data[..., 2], data[..., 3]
+        J = np.cross(normal, Hv)
+        M = -np.cross(normal, Ev)
+        dot = P.reshape(-1, 3) @ r_hat.T                        # (nc, n_dir)
+        for fi in range(len(fr)):
+            e = np.exp(1j * k[fi] * dot)
+            N[fi] += e.T @ (J[fi] * dS[..., None]).reshape(-1, 3)
+            L[fi] += e.T @ (M[fi] * dS[..., None]).reshape(-1, 3)
+    n_th = np.einsum("fdc,dc->fd", N, th_hat)
+    n_ph = np.einsum("fdc,dc->fd", N, ph_hat)
+    l_th = np.einsum("fdc,dc->fd", L, th_hat)
+    l_ph = np.einsum("fdc,dc->fd", L, ph_hat)
+    g = (1j * k / (4 * np.pi))[:, None]
+    e_th = -g * (l_ph + eta0 * n_th)
+    e_ph = g * (l_th - eta0 * n_ph)
+    shp = (len(fr), len(theta), len(phi))
+    return e_th.reshape(shp), e_ph.reshape(shp)
 
 
 def dft_window_weights(total_steps, window="rect", alpha=0.25):
@@ -859,6 +978,7 @@ def build_uniform(spec, n_steps):
                      pec_axes=pec_axes, pec_mask=pec, sources=sources,
                      probes=probes, dft_planes=planes, ports_vi=ports_vi, flux=flux)
     cfg.flux_area = areas
+    cfg.ntff = _ntff_entries(spec, g)
     return g, cfg
 
 
@@ -1031,7 +1151,23 @@ def build_graded(spec, n_steps):
                      pec_mask=pec, sources=sources, probes=probes, dft_planes=planes,
                      ports_vi=ports_vi, record_order="graded")
     cfg.flux, cfg.flux_area = _flux_entries(spec, g, n_steps, True)
+    cfg.ntff = _ntff_entries(spec, g)
     return g, cfg
+
+
+def _ntff_entries(spec, g):
+    """NTFF box of a spec: (corner_lo, corner_hi, freqs|None, n_freqs) ->
+    [(lo, hi, freqs f32)] through the lane's position-to-index rule
+    (rfx/farfield.py:127-141, rfx/runners/nonuniform.py:1086-1097); default
+    frequencies as rfx/api/__init__.py:2666-2667, stored in float32."""
+    ent = spec.get("ntff")
+    if ent is None:
+        return []
+    corner_lo, corner_hi, freqs, nf = ent
+    if freqs is None:
+        freqs = np.linspace(spec["freq_max"] / 10, spec["freq_max"], nf)
+    return [(g.position_to_index(corner_lo), g.position_to_index(corner_hi),
+             np.asarray(freqs, F32))]
 
 
 def _flux_entries(spec, g, n_steps, graded):

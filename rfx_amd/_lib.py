@@ -44,6 +44,18 @@ class Flux(C.Structure):
                 ("freqs", C.POINTER(C.c_double)), ("weight", _FP), ("d_acc", _P)]
 
 
+class Ntff(C.Structure):
+    _fields_ = [("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3), ("n_freq", C.c_int32),
+                ("freqs", C.POINTER(C.c_double)), ("d_acc", _P * 6)]
+
+
+class FarFieldDesc(C.Structure):
+    _fields_ = [("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3), ("n_freq", C.c_int32),
+                ("freqs", C.POINTER(C.c_double)), ("d_acc", _P * 6),
+                ("n_axis", C.c_int32 * 3), ("pos", C.POINTER(C.c_double) * 3),
+                ("width", C.POINTER(C.c_double) * 3), ("c0", C.c_double), ("eta0", C.c_double)]
+
+
 class Desc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
@@ -65,6 +77,7 @@ class Desc(C.Structure):
         ("n_plane", C.c_int32), ("planes", C.POINTER(Plane)),
         ("plane_t_offset", C.c_int32),
         ("n_flux", C.c_int32), ("flux", C.POINTER(Flux)),
+        ("n_ntff", C.c_int32), ("ntff", C.POINTER(Ntff)),
     ]
 
 
@@ -87,8 +100,8 @@ _lib = None
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
            "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
-           "rfxh_batch_run", "rfxh_set_variant", "rfxh_get_variant", "rfxh_last_error",
-           "rfxh_version")
+           "rfxh_batch_run", "rfxh_set_variant", "rfxh_get_variant", "rfxh_ntff_far_field",
+           "rfxh_last_error", "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -135,6 +148,10 @@ def load():
     lib.rfxh_set_variant.restype = C.c_int
     lib.rfxh_get_variant.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.rfxh_get_variant.restype = C.c_int
+    lib.rfxh_ntff_far_field.argtypes = [C.POINTER(FarFieldDesc), C.c_int32,
+                                        C.POINTER(C.c_double), C.c_int32,
+                                        C.POINTER(C.c_double), _P, _P, _P]
+    lib.rfxh_ntff_far_field.restype = C.c_int
     lib.rfxh_last_error.argtypes = []
     lib.rfxh_last_error.restype = C.c_char_p
     lib.rfxh_version.argtypes = []

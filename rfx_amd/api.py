@@ -153,6 +153,8 @@ class Result:
     grid: object = None
     dt: float = 0.0
     flux_monitors: dict = field(default_factory=dict)
+    ntff_data: object = None      # rfx_amd.farfield.NTFFData
+    ntff_box: object = None       # rfx_amd.farfield.NTFFBox
 
 
 @dataclass
@@ -164,6 +166,8 @@ class ForwardResult:
     freqs: np.ndarray | None = None
     dft_planes: dict | None = None
     flux_monitors: dict | None = None
+    ntff_data: object = None
+    ntff_box: object = None
 
 
 def _cell_cb_f32(dt, eps_r, sigma):
@@ -235,6 +239,7 @@ class Simulation:
         self._probes: list[_ProbeEntry] = []
         self._dft_planes: list[_DFTPlaneEntry] = []
         self._flux_monitors: list[_FluxMonitorEntry] = []
+        self._ntff = None
         self._device = device
 
     # ---- builder --------------------------------------------------------------
@@ -329,6 +334,34 @@ class Simulation:
             int(n_freqs), None if size is None else tuple(size),
             None if center is None else tuple(center), dft_window, float(dft_window_alpha)))
         return self
+
+    def add_ntff_box(self, corner_lo, corner_hi, freqs=None, n_freqs: int = 50):
+        """Near-to-far-field box (rfx/api/__init__.py:2647-2669): the time
+        loop accumulates the tangential-field DFTs on its six faces, returned
+        as ``result.ntff_data`` / ``result.ntff_box`` for
+        ``rfx_amd.farfield.compute_far_field``."""
+        if freqs is None:
+            freqs = np.linspace(self._freq_max / 10, self._freq_max, n_freqs)
+        self._ntff = (tuple(corner_lo), tuple(corner_hi), np.asarray(freqs, F32))
+        return self
+
+    def _ntff_setup(self, grid):
+        """NTFFBox for the registered box (rfx/runners/uniform.py:648-651,
+        rfx/runners/nonuniform.py:1079-1097): corners through the lane's
+        position-to-index rule, frequencies in float32."""
+        if self._ntff is None:
+            return None
+        from .farfield import check_box, make_ntff_box
+        lo, hi, freqs = self._ntff
+        box = make_ntff_box(grid, lo, hi, freqs)
+        check_box(box, grid.shape)
+        return box
+
+    def _ntff_results(self, box, dr):
+        if box is None:
+            return None
+        from .farfield import data_from_device
+        return data_from_device(dr.ntff_acc[0])
 
     def _flux_setup(self, grid, n_steps):
         """(Problem.fluxes entries, dA arrays) for the registered monitors
@@ -474,11 +507,14 @@ class Simulation:
         # The graded lane always applies it (nonuniform.py:1319).
         pec_axes = "xyz" if (graded or lane == "run") else ""
         fluxes, areas = self._flux_setup(grid, n_steps)
+        box = self._ntff_setup(grid)
+        ntff = [] if box is None else [(box.lo, box.hi, np.asarray(box.freqs, np.float64))]
         prob = Problem(shape=shape, dt=dt, graded=graded, inv_e=inv_e, inv_h=inv_h, mat=mat,
                        faces=faces, pec_axes=pec_axes, sources=sources, probes=probes,
                        ports=[p[:4] for p in ports], planes=planes, fluxes=fluxes,
-                       record_order="graded" if graded else "uniform")
+                       record_order="graded" if graded else "uniform", ntff=ntff)
         prob.flux_areas = areas
+        prob.ntff_box = box
         return grid, prob, ports
 
     def _port_cells(self, grid, pe):
@@ -595,7 +631,8 @@ class Simulation:
         _warn_if_nonfinite(ts, "run")
         return Result(time_series=ts, state=st, dft_planes=planes, s_params=s_params,
                       freqs=sf if s_params is not None else None, grid=grid, dt=grid.dt,
-                      flux_monitors=self._flux_results(prob, dr, prob.flux_areas))
+                      flux_monitors=self._flux_results(prob, dr, prob.flux_areas),
+                      ntff_data=self._ntff_results(prob.ntff_box, dr), ntff_box=prob.ntff_box)
 
     def forward(self, *, eps_override=None, sigma_override=None, mu_r_override=None,
                 n_steps: int | None = None, num_periods: float = 20.0, checkpoint: bool = True,
@@ -644,7 +681,9 @@ class Simulation:
                                                             dr.plane_acc)}
         fluxes = self._flux_results(prob, dr, prob.flux_areas) if self._flux_monitors else None
         return ForwardResult(time_series=ts, grid=grid, s_params=s_params, freqs=freqs,
-                             dft_planes=planes, flux_monitors=fluxes)
+                             dft_planes=planes, flux_monitors=fluxes,
+                             ntff_data=self._ntff_results(prob.ntff_box, dr),
+                             ntff_box=prob.ntff_box)
 
 
 def _to_np(x):

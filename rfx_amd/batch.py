@@ -71,7 +71,9 @@ class Batch:
             _warn_if_nonfinite(ts, "run_batch")
             out.append(Result(time_series=ts, state=FDTDState(*r.fields, step=ns),
                               dft_planes=planes, grid=g, dt=g.dt,
-                              flux_monitors=sim._flux_results(prob, r, prob.flux_areas)))
+                              flux_monitors=sim._flux_results(prob, r, prob.flux_areas),
+                              ntff_data=sim._ntff_results(prob.ntff_box, r),
+                              ntff_box=prob.ntff_box))
         return out
 
     def close(self):

@@ -1293,6 +1293,119 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams q) {
 }
 
 // ---------------------------------------------------------------------------
+// Near-to-far-field transform: one thread per (direction, frequency).  The
+// face cells stream through LDS in chunks of kFFBlock, each staged once per
+// workgroup: its position and its four tangential DFT values at the
+// workgroup's frequency times s dS (s = -1 on lo faces, +1 on hi faces).
+// Per axis a the four sums S_t = sum s dS f_t e^{j k r.p} over both faces
+// then fold into N = sum J dS e^{..} and L = sum M dS e^{..} through the
+// face's cross products (J = n x H, M = -n x E).  Float64 throughout: the
+// phase k r.p reaches a few hundred radians on large boxes.
+// ---------------------------------------------------------------------------
+constexpr int kFFBlock = 128;
+
+struct FarFieldParams {
+  const double2* acc[6];   // [4][nf][n1][n2] per face
+  const double* pos[3];    // per-axis positions (device copies)
+  const double* width[3];  // per-axis cell widths
+  const double* dirs;      // [n_dir][9]: r-hat, theta-hat, phi-hat
+  const double* kwave;     // [nf] 2 pi f / c0
+  double2* e_theta;        // [nf][n_dir]
+  double2* e_phi;
+  int lo[3], hi[3];
+  int nf, n_dir;
+  double eta0;
+};
+
+__global__ __launch_bounds__(kFFBlock) void k_far_field(FarFieldParams q) {
+  __shared__ double s_p[3][kFFBlock];
+  __shared__ double2 s_v[4][kFFBlock];
+  const int tid = threadIdx.x;
+  const int f = blockIdx.y;
+  const int d = blockIdx.x * kFFBlock + tid;
+  const bool live = d < q.n_dir;
+  const double k = q.kwave[f];
+  double r[3] = {0.0, 0.0, 0.0};
+  if (live)
+    for (int c = 0; c < 3; ++c) r[c] = q.dirs[9 * (long long)d + c];
+  double2 Nv[3], Lv[3];
+  for (int c = 0; c < 3; ++c) Nv[c] = Lv[c] = make_double2(0.0, 0.0);
+  for (int ax = 0; ax < 3; ++ax) {
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    const int n2 = q.hi[t2] - q.lo[t2];
+    const long long npts = (long long)(q.hi[t1] - q.lo[t1]) * n2;
+    double2 S[4];
+    for (int t = 0; t < 4; ++t) S[t] = make_double2(0.0, 0.0);
+    for (int side = 0; side < 2; ++side) {
+      const double2* acc = q.acc[2 * ax + side];
+      const double sgn = side ? 1.0 : -1.0;
+      const double pn = q.pos[ax][side ? q.hi[ax] : q.lo[ax]];
+      for (long long base = 0; base < npts; base += kFFBlock) {
+        __syncthreads();   // the previous chunk has been consumed
+        const long long c = base + tid;
+        if (c < npts) {
+          const int i1 = q.lo[t1] + (int)(c / n2), i2 = q.lo[t2] + (int)(c % n2);
+          s_p[ax][tid] = pn;
+          s_p[t1][tid] = q.pos[t1][i1];
+          s_p[t2][tid] = q.pos[t2][i2];
+          const double ds = sgn * (q.width[t1][i1] * q.width[t2][i2]);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const double2 a = acc[((long long)t * q.nf + f) * npts + c];
+            s_v[t][tid] = make_double2(a.x * ds, a.y * ds);
+          }
+        }
+        __syncthreads();
+        const int m = (int)min((long long)kFFBlock, npts - base);
+        for (int e = 0; e < m; ++e) {
+          const double ph = k * (r[0] * s_p[0][e] + r[1] * s_p[1][e] + r[2] * s_p[2][e]);
+          double sn, cs;
+          sincos(ph, &sn, &cs);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const double2 v = s_v[t][e];
+            S[t].x += cs * v.x - sn * v.y;
+            S[t].y += cs * v.y + sn * v.x;
+          }
+        }
+      }
+    }
+    // (f0, f1, f2, f3) = x: (Ey, Ez, Hy, Hz); y: (Ex, Ez, Hx, Hz); z: (Ex, Ey, Hx, Hy)
+    auto add = [](double2& a, const double2& b, double s) {
+      a.x += s * b.x;
+      a.y += s * b.y;
+    };
+    if (ax == 0) {          // J = (0, -f3, f2), M = (0, f1, -f0)
+      add(Nv[1], S[3], -1.0); add(Nv[2], S[2], 1.0);
+      add(Lv[1], S[1], 1.0);  add(Lv[2], S[0], -1.0);
+    } else if (ax == 1) {   // J = (f3, 0, -f2), M = (-f1, 0, f0)
+      add(Nv[0], S[3], 1.0);  add(Nv[2], S[2], -1.0);
+      add(Lv[0], S[1], -1.0); add(Lv[2], S[0], 1.0);
+    } else {                // J = (-f3, f2, 0), M = (f1, -f0, 0)
+      add(Nv[0], S[3], -1.0); add(Nv[1], S[2], 1.0);
+      add(Lv[0], S[1], 1.0);  add(Lv[1], S[0], -1.0);
+    }
+  }
+  if (!live) return;
+  const double* th = q.dirs + 9 * (long long)d + 3;
+  const double* ph = th + 3;
+  double2 nth = make_double2(0.0, 0.0), nph = nth, lth = nth, lph = nth;
+  for (int c = 0; c < 3; ++c) {
+    nth.x += Nv[c].x * th[c]; nth.y += Nv[c].y * th[c];
+    nph.x += Nv[c].x * ph[c]; nph.y += Nv[c].y * ph[c];
+    lth.x += Lv[c].x * th[c]; lth.y += Lv[c].y * th[c];
+    lph.x += Lv[c].x * ph[c]; lph.y += Lv[c].y * ph[c];
+  }
+  // E_theta = -j k/(4 pi) (L_phi + eta0 N_theta); E_phi = j k/(4 pi) (L_theta - eta0 N_phi)
+  const double g = k / (4.0 * M_PI);
+  const double2 a = make_double2(lph.x + q.eta0 * nth.x, lph.y + q.eta0 * nth.y);
+  const double2 b = make_double2(lth.x - q.eta0 * nph.x, lth.y - q.eta0 * nph.y);
+  const long long o = (long long)f * q.n_dir + d;
+  q.e_theta[o] = make_double2(g * a.y, -g * a.x);   // -j g a
+  q.e_phi[o] = make_double2(-g * b.y, g * b.x);     //  j g b
+}
+
+// ---------------------------------------------------------------------------
 // Reverse mode.  The forward step is linear in the state (E, H, CPML memory)
 // for fixed materials, so its adjoint is the transpose of each sub-step run
 // in reverse order.  A reverse step is two gather kernels:
@@ -2230,6 +2343,55 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pl->fluxes.push_back(fp);
     for (int t = 0; t < 4; ++t) pl->flux_comp.push_back(kFluxComp[fd.axis][t]);
   }
+  // NTFF boxes: six faces each, run by the flux kernel with the H planes
+  // not averaged (index_m1 = index: (h + h) * 0.5f is h exactly) and the
+  // box's own time base, t = n dt for E and (n + 1/2) dt for H on both
+  // lanes (rfx/farfield.py:234-245, called with the scan index).
+  static const int kNtffComp[3][4] = {{1, 2, 4, 5}, {0, 2, 3, 5}, {0, 1, 3, 4}};
+  for (int q = 0; q < d->n_ntff; ++q) {
+    const rfxh_ntff& nb = d->ntff[q];
+    if (nb.n_freq < 1 || !nb.freqs)
+      return bail(fail(RFXH_EINVAL, "bad NTFF box: no frequencies"));
+    for (int a = 0; a < 3; ++a)
+      if (nb.lo[a] < 0 || nb.hi[a] <= nb.lo[a] || nb.hi[a] >= N[a])
+        return bail(fail(RFXH_EINVAL, "bad NTFF box: need 0 <= lo < hi < n on every axis"));
+    std::vector<double2> te((size_t)d->max_steps * nb.n_freq), th(te.size());
+    for (long long n = 0; n < d->max_steps; ++n) {
+      const double t = (double)n * d->dt;
+      for (int f = 0; f < nb.n_freq; ++f) {
+        const double ae = -2.0 * M_PI * nb.freqs[f] * t;
+        const double ah = -2.0 * M_PI * nb.freqs[f] * (t + d->dt * 0.5);
+        te[n * nb.n_freq + f] = make_double2(std::cos(ae) * d->dt, std::sin(ae) * d->dt);
+        th[n * nb.n_freq + f] = make_double2(std::cos(ah) * d->dt, std::sin(ah) * d->dt);
+      }
+    }
+    double2 *dte, *dth;
+    if ((rc = dev_copy(pl, te.data(), te.size(), &dte))) return bail(rc);
+    if ((rc = dev_copy(pl, th.data(), th.size(), &dth))) return bail(rc);
+    for (int face = 0; face < 6; ++face) {
+      const int ax = face / 2;
+      if (!nb.d_acc[face]) return bail(fail(RFXH_EINVAL, "NTFF face accumulator is NULL"));
+      const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+      FluxParams fp;
+      std::memset(&fp, 0, sizeof(fp));
+      fp.acc = reinterpret_cast<double2*>(nb.d_acc[face]);
+      fp.step = kp.step;
+      fp.max_steps = d->max_steps;
+      fp.nf = nb.n_freq;
+      fp.axis = ax;
+      fp.index = fp.index_m1 = (face & 1) ? nb.hi[ax] : nb.lo[ax];
+      fp.lo1 = nb.lo[t1];
+      fp.n1 = nb.hi[t1] - nb.lo[t1];
+      fp.lo2 = nb.lo[t2];
+      fp.n2 = nb.hi[t2] - nb.lo[t2];
+      fp.ny = d->ny;
+      fp.nz = d->nz;
+      fp.twe = dte;
+      fp.twh = dth;
+      pl->fluxes.push_back(fp);
+      for (int t = 0; t < 4; ++t) pl->flux_comp.push_back(kNtffComp[ax][t]);
+    }
+  }
   // workgroup order (RFXH_ORDER = flat (default) | bands, for comparisons;
   // rfxh_set_variant changes it later)
   const char* ord = std::getenv("RFXH_ORDER");
@@ -2729,6 +2891,86 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
     HIPCHK(hipStreamWaitEvent(s, b->ev_out, 0));
   }
   return RFXH_OK;
+}
+
+int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const double* theta,
+                        int32_t n_phi, const double* phi, double* e_theta, double* e_phi,
+                        void* stream) {
+  if (!dsc || !theta || !phi || !e_theta || !e_phi || n_theta < 1 || n_phi < 1)
+    return fail(RFXH_EINVAL, "far field: missing arguments");
+  if (dsc->n_freq < 1 || dsc->n_freq > 65535 || !dsc->freqs)
+    return fail(RFXH_EINVAL, "far field: need 1..65535 frequencies");
+  if ((long long)n_theta * n_phi > (1LL << 30))
+    return fail(RFXH_EINVAL, "far field: too many directions");
+  for (int a = 0; a < 3; ++a) {
+    if (!dsc->pos[a] || !dsc->width[a] || dsc->n_axis[a] < 1)
+      return fail(RFXH_EINVAL, "far field: missing position / width table");
+    if (dsc->lo[a] < 0 || dsc->hi[a] <= dsc->lo[a] || dsc->hi[a] >= dsc->n_axis[a])
+      return fail(RFXH_EINVAL, "far field: need 0 <= lo < hi < n on every axis");
+  }
+  for (int face = 0; face < 6; ++face)
+    if (!dsc->d_acc[face]) return fail(RFXH_EINVAL, "far field: face accumulator is NULL");
+  const int n_dir = n_theta * n_phi;
+  std::vector<double> dirs((size_t)n_dir * 9);
+  for (int it = 0; it < n_theta; ++it) {
+    const double st = std::sin(theta[it]), ct = std::cos(theta[it]);
+    for (int ip = 0; ip < n_phi; ++ip) {
+      const double sp = std::sin(phi[ip]), cp = std::cos(phi[ip]);
+      double* o = dirs.data() + 9 * ((size_t)it * n_phi + ip);
+      o[0] = st * cp; o[1] = st * sp; o[2] = ct;     // r-hat
+      o[3] = ct * cp; o[4] = ct * sp; o[5] = -st;    // theta-hat
+      o[6] = -sp;     o[7] = cp;      o[8] = 0.0;    // phi-hat
+    }
+  }
+  std::vector<double> kw(dsc->n_freq);
+  for (int f = 0; f < dsc->n_freq; ++f) kw[f] = 2.0 * M_PI * dsc->freqs[f] / dsc->c0;
+  // one device block for every table: positions and widths per axis, the
+  // directions and the wave numbers
+  size_t total = dirs.size() + kw.size();
+  for (int a = 0; a < 3; ++a) total += 2 * (size_t)dsc->n_axis[a];
+  double* blk = nullptr;
+  if (hipMalloc(&blk, total * sizeof(double)) != hipSuccess)
+    return fail(RFXH_ENOMEM, "far field: device tables");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FarFieldParams q;
+  std::memset(&q, 0, sizeof(q));
+  size_t off = 0;
+  auto put = [&](const double* src, size_t n) -> const double* {
+    double* dst = blk + off;
+    off += n;
+    return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess
+               ? dst : nullptr;
+  };
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    ok = ok && (q.pos[a] = put(dsc->pos[a], dsc->n_axis[a])) != nullptr;
+    ok = ok && (q.width[a] = put(dsc->width[a], dsc->n_axis[a])) != nullptr;
+  }
+  ok = ok && (q.dirs = put(dirs.data(), dirs.size())) != nullptr;
+  ok = ok && (q.kwave = put(kw.data(), kw.size())) != nullptr;
+  int rc = RFXH_OK;
+  if (!ok) {
+    rc = fail(RFXH_EHIP, "far field: table upload");
+  } else {
+    for (int face = 0; face < 6; ++face) q.acc[face] = reinterpret_cast<const double2*>(dsc->d_acc[face]);
+    for (int a = 0; a < 3; ++a) {
+      q.lo[a] = dsc->lo[a];
+      q.hi[a] = dsc->hi[a];
+    }
+    q.nf = dsc->n_freq;
+    q.n_dir = n_dir;
+    q.eta0 = dsc->eta0;
+    q.e_theta = reinterpret_cast<double2*>(e_theta);
+    q.e_phi = reinterpret_cast<double2*>(e_phi);
+    hipLaunchKernelGGL(k_far_field, dim3((unsigned)((n_dir + kFFBlock - 1) / kFFBlock),
+                                         (unsigned)dsc->n_freq), dim3(kFFBlock), 0, s, q);
+    if (hipGetLastError() != hipSuccess) rc = fail(RFXH_EHIP, "far field: launch");
+  }
+  // the uploads and the kernel must finish before the tables are freed
+  if (hipStreamSynchronize(s) != hipSuccess && rc == RFXH_OK)
+    rc = fail(RFXH_EHIP, "far field: kernel");
+  (void)hipFree(blk);
+  return rc;
 }
 
 const char* rfxh_last_error(void) { return g_err.c_str(); }

@@ -37,6 +37,8 @@ class Problem:
     # flux monitors: (axis, index, lo1, hi1, lo2, hi2, freqs f64, weights f32 [n+1] | None)
     fluxes: list = field(default_factory=list)
     record_order: str = "uniform"                 # "uniform" | "graded"
+    # near-to-far-field boxes: (lo (i, j, k), hi (i, j, k), freqs f64)
+    ntff: list = field(default_factory=list)
 
     @property
     def plane_t_offset(self):
@@ -142,6 +144,25 @@ class DeviceRun:
                 fc.weight = _f32p(w)
             fc.d_acc = acc.data_ptr()
         self._flux_c = flux_c
+        # NTFF boxes: six face accumulators [4][n_freqs][n1][n2] each
+        self.ntff_acc = []
+        ntff_c = (_lib.Ntff * max(len(prob.ntff), 1))()
+        for q, (lo, hi, freqs) in enumerate(prob.ntff):
+            fr = np.ascontiguousarray(np.asarray(freqs, np.float64))
+            self._plane_freqs.append(fr)
+            ni, nj, nk = (hi[a] - lo[a] for a in range(3))
+            faces = []
+            for n1, n2 in ((nj, nk), (nj, nk), (ni, nk), (ni, nk), (ni, nj), (ni, nj)):
+                faces.append(torch.zeros((4, len(fr), n1, n2), dtype=torch.complex128, device=dev))
+            self.ntff_acc.append(faces)
+            nc = ntff_c[q]
+            for a in range(3):
+                nc.lo[a], nc.hi[a] = int(lo[a]), int(hi[a])
+            nc.n_freq = len(fr)
+            nc.freqs = fr.ctypes.data_as(C.POINTER(C.c_double))
+            for f in range(6):
+                nc.d_acc[f] = faces[f].data_ptr()
+        self._ntff_c = ntff_c
 
         # ---- descriptor -----------------------------------------------------
         d = _lib.Desc()
@@ -191,6 +212,8 @@ class DeviceRun:
         d.plane_t_offset = prob.plane_t_offset
         d.n_flux = len(prob.fluxes)
         d.flux = flux_c
+        d.n_ntff = len(prob.ntff)
+        d.ntff = ntff_c
         self.desc = d
         self.plan = C.c_void_p()
         with torch.cuda.device(dev):
@@ -238,6 +261,9 @@ class DeviceRun:
             a.zero_()
         for a in self.flux_acc:
             a.zero_()
+        for faces in self.ntff_acc:
+            for a in faces:
+                a.zero_()
 
     def set_timing(self, enable: bool):
         _lib.check(self.lib.rfxh_set_timing(self.plan, 1 if enable else 0))

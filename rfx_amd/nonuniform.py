@@ -86,6 +86,12 @@ class GradedGrid:
             inv_e.append(np.concatenate([local[:1], F32(2.0) / (d[:-1] + d[1:])]))
         return inv_e, inv_h
 
+    @property
+    def dz(self):
+        """Padded float32 z cell sizes, bounding cell included
+        (NonUniformGrid.dz, rfx/nonuniform.py:58)."""
+        return self.d32[2]
+
     def face_cell_sizes(self):
         """CPML face cell sizes (rfx/boundaries/cpml.py:174-197): the scalar
         dx, dy and the float32 first / last dz entries."""

@@ -114,6 +114,9 @@ def to_sim(spec, device=None):
     for axis, coord, freqs, nf, size, center, window, alpha in spec.get("flux_monitors", []):
         sim.add_flux_monitor(axis=axis, coordinate=coord, freqs=freqs, n_freqs=nf, size=size,
                              center=center, dft_window=window, dft_window_alpha=alpha)
+    if spec.get("ntff") is not None:
+        lo, hi, freqs, nf = spec["ntff"]
+        sim.add_ntff_box(lo, hi, freqs=freqs, n_freqs=nf)
     return sim
 
 
@@ -220,6 +223,27 @@ def flux_case():
                 cpml_layers=8, ports=[((0.015, 0.015, 0.015), "ez", 0.0, o.Pulse(10e9, 0.5), True)],
                 probes=[((0.02, 0.015, 0.015), "ez")], lane="run",
                 flux_monitors=flux_box(0.005) + flux_box(0.009))
+
+
+def ntff_case():
+    """flux_case with an NTFF box of half-size 7 mm between its two flux
+    boxes, at the flux monitors' frequencies."""
+    spec = flux_case()
+    spec["ntff"] = ((0.008, 0.008, 0.008), (0.022, 0.022, 0.022),
+                    np.array((8e9, 10e9, 12e9), np.float32), 3)
+    return spec
+
+
+def dipole_ntff_case():
+    """The reference's short-dipole NTFF set-up (tests/test_farfield.py:
+    24-56 of the reference): 30 mm cube, 8-layer CPML, default dx at
+    5 GHz, Ez pulse at 3 GHz in the centre, NTFF box 4 mm inside the domain
+    edges, one frequency; 200 steps."""
+    return dict(freq_max=5e9, domain=(0.03, 0.03, 0.03), boundary="cpml", cpml_layers=8,
+                ports=[((0.015, 0.015, 0.015), "ez", 0.0, o.Pulse(3e9, 0.5), True)],
+                probes=[((0.015, 0.015, 0.015), "ez")], lane="run",
+                ntff=((0.004, 0.004, 0.004), (0.026, 0.026, 0.026),
+                      np.array([3e9], np.float32), 1)), 200
 
 
 def box_total(fluxes):
