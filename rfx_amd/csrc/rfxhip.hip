@@ -113,6 +113,7 @@ struct PsiIO {
 };
 
 constexpr int kDftK = 16;   // steps staged per fold of the DFT-plane accumulators
+constexpr int kFoldF = 4;    // accumulators a fold keeps in flight
 
 struct PlaneParams {
   const float* fld;
@@ -132,7 +133,9 @@ struct FluxParams {
   const float* f[4];     // E1, E2, H1, H2 field arrays (current set)
   double2* acc;          // [4][nf][n1][n2]
   const double2* twe;    // [max_steps][nf] E twiddles (dt and window folded in)
-  const double2* twh;    // [max_steps][nf] H twiddles (half a step earlier)
+  const double2* twh;    // [max_steps][nf] H twiddles (half a step off E)
+  float* snap;           // [kDftK][4][n1*n2] ring of values not yet folded in
+  const long long* acc_from;  // first step not yet folded into acc (device)
   long long* step;
   long long max_steps;
   int nf, axis, index, index_m1;
@@ -1196,19 +1199,27 @@ __device__ __forceinline__ void dft_fold(const PlaneParams& q, long long pidx, l
       xs[m] = (hi_in_reg && st == hi) ? xhi : q.snap[(st % kDftK) * npts + pidx];
     }
   }
-  for (int f = 0; f < q.nf; ++f) {
-    double2* a = q.acc + (long long)f * npts + pidx;
-    double2 cur = *a;
+  for (int f0 = 0; f0 < q.nf; f0 += kFoldF) {
+    double2 cur[kFoldF];
 #pragma unroll
-    for (int m = 0; m < kDftK; ++m) {
-      if (m < cnt) {
-        const double2 w = q.tw[(lo + m) * q.nf + f];
-        const double x = (double)xs[m];
-        cur.x += w.x * x;
-        cur.y += w.y * x;
+    for (int u = 0; u < kFoldF; ++u)
+      if (f0 + u < q.nf) cur[u] = q.acc[(long long)(f0 + u) * npts + pidx];
+#pragma unroll
+    for (int u = 0; u < kFoldF; ++u) {
+      if (f0 + u >= q.nf) break;
+#pragma unroll
+      for (int m = 0; m < kDftK; ++m) {
+        if (m < cnt) {
+          const double2 w = q.tw[(lo + m) * q.nf + f0 + u];
+          const double x = (double)xs[m];
+          cur[u].x += w.x * x;
+          cur[u].y += w.y * x;
+        }
       }
     }
-    *a = cur;
+#pragma unroll
+    for (int u = 0; u < kFoldF; ++u)
+      if (f0 + u < q.nf) q.acc[(long long)(f0 + u) * npts + pidx] = cur[u];
   }
 }
 
@@ -1256,6 +1267,50 @@ __global__ void k_set_acc_from(long long* acc_from, const long long* step) {
 // Flux monitor: tangential E at the plane, tangential H averaged over the
 // planes index-1 and index, accumulated with their own twiddles (float64).
 // ---------------------------------------------------------------------------
+// Folds steps lo..hi of the ring (step hi from registers when hi_in_reg)
+// into the four accumulators, in step order per frequency: the additions a
+// per-step accumulation makes, in the same order.
+__device__ __forceinline__ void flux_fold(const FluxParams& q, long long pidx, long long npts,
+                                          long long lo, long long hi, bool hi_in_reg,
+                                          const float* xhi) {
+  const int cnt = (int)(hi - lo + 1);
+  for (int t = 0; t < 4; ++t) {
+    float xs[kDftK];
+#pragma unroll
+    for (int m = 0; m < kDftK; ++m) {
+      if (m < cnt) {
+        const long long st = lo + m;
+        xs[m] = (hi_in_reg && st == hi) ? xhi[t] : q.snap[((st % kDftK) * 4 + t) * npts + pidx];
+      }
+    }
+    const double2* tw = t < 2 ? q.twe : q.twh;
+    // kFoldF frequencies at a time: their accumulator loads are in flight
+    // together instead of one round trip per frequency
+    for (int f0 = 0; f0 < q.nf; f0 += kFoldF) {
+      double2 cur[kFoldF];
+#pragma unroll
+      for (int u = 0; u < kFoldF; ++u)
+        if (f0 + u < q.nf) cur[u] = q.acc[((long long)t * q.nf + f0 + u) * npts + pidx];
+#pragma unroll
+      for (int u = 0; u < kFoldF; ++u) {
+        if (f0 + u >= q.nf) break;
+#pragma unroll
+        for (int m = 0; m < kDftK; ++m) {
+          if (m < cnt) {
+            const double2 w = tw[(lo + m) * q.nf + f0 + u];
+            const double x = (double)xs[m];
+            cur[u].x += w.x * x;
+            cur[u].y += w.y * x;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kFoldF; ++u)
+        if (f0 + u < q.nf) q.acc[((long long)t * q.nf + f0 + u) * npts + pidx] = cur[u];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_flux(FluxParams q) {
   const long long npts = (long long)q.n1 * q.n2;
   const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1271,25 +1326,33 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams q) {
   ii[a2] = im[a2] = v;
   const long long c = ((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2];
   const long long cm = ((long long)im[0] * q.ny + im[1]) * q.nz + im[2];
-  double x[4];
-  x[0] = (double)q.f[0][c];
-  x[1] = (double)q.f[1][c];
-  x[2] = (double)((q.f[2][cm] + q.f[2][c]) * 0.5f);
-  x[3] = (double)((q.f[3][cm] + q.f[3][c]) * 0.5f);
-  const double2* te = q.twe + n * q.nf;
-  const double2* th = q.twh + n * q.nf;
-  for (int f = 0; f < q.nf; ++f) {
-    const double2 we = te[f], wh = th[f];
+  float x[4];
+  x[0] = q.f[0][c];
+  x[1] = q.f[1][c];
+  x[2] = (q.f[2][cm] + q.f[2][c]) * 0.5f;
+  x[3] = (q.f[3][cm] + q.f[3][c]) * 0.5f;
+  const int slot = (int)(n % kDftK);
+  if (slot == kDftK - 1) {
+    const long long lo = max(*q.acc_from, n - (kDftK - 1));
+    flux_fold(q, pidx, npts, lo, n, true, x);
+  } else {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const double2 w = t < 2 ? we : wh;
-      double2* a = q.acc + ((long long)t * q.nf + f) * npts + pidx;
-      double2 cur = *a;
-      cur.x += w.x * x[t];
-      cur.y += w.y * x[t];
-      *a = cur;
-    }
+    for (int t = 0; t < 4; ++t) q.snap[((long long)slot * 4 + t) * npts + pidx] = x[t];
   }
+}
+
+// Folds the staged steps of the last, incomplete group (end of a run).
+__global__ __launch_bounds__(256) void k_flux_flush(FluxParams q) {
+  const long long npts = (long long)q.n1 * q.n2;
+  const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= npts) return;
+  const long long last = *q.step - 1;
+  if (last < 0) return;
+  const int slot = (int)(last % kDftK);
+  if (slot == kDftK - 1) return;   // folded by k_flux
+  const long long lo = max(*q.acc_from, last - slot);
+  if (lo > last) return;
+  flux_fold(q, pidx, npts, lo, last, false, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1911,11 +1974,16 @@ void dft_begin(rfxh_plan* pl, hipStream_t s, long long step0) {
 
 // End of a run: fold the staged steps of the last incomplete group.
 void dft_flush(rfxh_plan* pl, hipStream_t s) {
-  if (pl->planes.empty()) return;
+  if (pl->planes.empty() && pl->fluxes.empty()) return;
   for (size_t q = 0; q < pl->planes.size(); ++q) {
     const PlaneParams& pp = pl->planes[q];
     const long long npts = (long long)pp.n1 * pp.n2;
     hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
+  }
+  for (size_t q = 0; q < pl->fluxes.size(); ++q) {
+    const FluxParams& fp = pl->fluxes[q];   // the flush reads only the ring
+    const long long npts = (long long)fp.n1 * fp.n2;
+    hipLaunchKernelGGL(k_flux_flush, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, fp);
   }
   hipLaunchKernelGGL(k_set_acc_from, dim3(1), dim3(64), 0, s, pl->acc_from, pl->kp.step);
 }
@@ -2081,6 +2149,18 @@ int collect_timing(rfxh_plan* pl, int n_samples) {
     }
   }
   return 0;
+}
+
+// Snapshot ring of a flux monitor / NTFF face and the plan's fold marker.
+int flux_ring(rfxh_plan* pl, FluxParams& fp) {
+  int rc;
+  if (!pl->acc_from) {
+    const long long zero = 0;
+    if ((rc = dev_copy(pl, &zero, 1, &pl->acc_from))) return rc;
+  }
+  fp.acc_from = pl->acc_from;
+  const std::vector<float> ring((size_t)kDftK * 4 * fp.n1 * fp.n2, 0.f);
+  return dev_copy(pl, ring.data(), ring.size(), &fp.snap);
 }
 
 }  // namespace
@@ -2340,6 +2420,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     if ((rc = dev_copy(pl, th.data(), th.size(), &dth))) return bail(rc);
     fp.twe = dte;
     fp.twh = dth;
+    if ((rc = flux_ring(pl, fp))) return bail(rc);
     pl->fluxes.push_back(fp);
     for (int t = 0; t < 4; ++t) pl->flux_comp.push_back(kFluxComp[fd.axis][t]);
   }
@@ -2388,6 +2469,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
       fp.nz = d->nz;
       fp.twe = dte;
       fp.twh = dth;
+      if ((rc = flux_ring(pl, fp))) return bail(rc);
       pl->fluxes.push_back(fp);
       for (int t = 0; t < 4; ++t) pl->flux_comp.push_back(kNtffComp[ax][t]);
     }

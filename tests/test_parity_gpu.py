@@ -117,15 +117,18 @@ def test_graph_and_chunked_runs_bit_identical(R):
     import torch
     from rfx_amd.engine import DeviceRun
     spec = cases.mixed_case("run")
+    spec["flux_monitors"] = [("x", 0.010, None, 3, None, None, "hann", 0.25),
+                             ("z", 0.004, None, 2, (0.008, 0.006), None, "rect", 0.25)]
+    spec["ntff"] = ((0.002, 0.002, 0.002), (0.014, 0.010, 0.008), None, 3)
     sim = cases.to_sim(spec)
     n = 100
     _, prob, _ = sim._build_problem(n, "run")
     outs = []
-    # the DFT planes fold 16 staged steps at a time: runs that stop and
-    # start inside a group must give the same sums
+    # DFT planes, flux monitors and NTFF faces fold 16 staged steps at a
+    # time: runs that stop and start inside a group must give the same sums
     for mode in ("direct", "graph", "chunked", "chunked3"):
         dr = DeviceRun(prob, n)
-        assert dr.plane_acc, "the case needs a DFT plane"
+        assert dr.plane_acc and dr.flux_acc and dr.ntff_acc, "the case needs every accumulator"
         if mode == "direct":
             dr.run(0, n, use_graph=False)
         elif mode == "graph":
@@ -141,7 +144,9 @@ def test_graph_and_chunked_runs_bit_identical(R):
         torch.cuda.synchronize()
         outs.append((dr.time_series().cpu().numpy().copy(),
                      [f.cpu().numpy().copy() for f in dr.fields],
-                     [a.cpu().numpy().copy() for a in dr.plane_acc]))
+                     [a.cpu().numpy().copy() for a in dr.plane_acc]
+                     + [a.cpu().numpy().copy() for a in dr.flux_acc]
+                     + [a.cpu().numpy().copy() for a in dr.ntff_acc[0]]))
         dr.close()
     for other in outs[1:]:
         np.testing.assert_array_equal(outs[0][0], other[0])
