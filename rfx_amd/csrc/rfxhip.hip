@@ -95,6 +95,8 @@ struct KParams {
   int order;                     // workgroup order: 0 bands, 1 flat chunks
   float inv_nz;                  // 1/nz for the flat-index split
   int slo[3], shi[3];            // CPML slab extents: cells i < slo or i >= shi per axis
+  float ch_all;                  // > 0: every material has this ch (one mu_r), and the
+                                 // column H pass reads no material id
 };
 
 struct FusedIO {
@@ -993,7 +995,7 @@ __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
 // ---------------------------------------------------------------------------
 // Cells i0 .. i0+NP-1 of the column, updating only those below iend
 // (iend <= nx; the others are read as neighbours only).
-template <bool GRADED, int NP>
+template <bool GRADED, int NP, bool ONE_MU>
 __device__ __forceinline__ void col_h(const KParams& p, const PsiIO& ps, int i0, int j, int k,
                                       int iend) {
   const unsigned P = (unsigned)p.plane;
@@ -1007,7 +1009,7 @@ __device__ __forceinline__ void col_h(const KParams& p, const PsiIO& ps, int i0,
   for (int q = 0; q < NP; ++q) {
     const bool in = i0 + q < p.nx;
     const unsigned c = in ? c0 + (unsigned)q * P : c0;
-    m[q] = ldu16(p.mat, c);
+    m[q] = ONE_MU ? 0u : ldu16(p.mat, c);
     ex[q] = ldf(p.f[0], c);
     ey[q] = ldf(p.f[1], c);
     ez[q] = ldf(p.f[2], c);
@@ -1035,7 +1037,8 @@ __device__ __forceinline__ void col_h(const KParams& p, const PsiIO& ps, int i0,
     const float exk_ = hk ? exk[q] : 0.f, eyk_ = hk ? eyk[q] : 0.f;
     const float eyi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ey[q + 1] : 0.f) : ey[NP];
     const float ezi = (q + 1 < NP) ? ((i0 + q + 1 < p.nx) ? ez[q + 1] : 0.f) : ez[NP];
-    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k, ld_ch(p.coef, m[q]), ex[q], ey[q], ez[q],
+    const float3 h = h_core<GRADED, true>(p, ps, i0 + q, j, k,
+                                          ONE_MU ? p.ch_all : ld_ch(p.coef, m[q]), ex[q], ey[q], ez[q],
                                           exj_, ezj_, exk_, eyk_, eyi, ezi, hx[q], hy[q], hz[q]);
     const unsigned c = c0 + (unsigned)q * P;
     stf(p.f[3], c, h.x);
@@ -1098,14 +1101,16 @@ __device__ __forceinline__ void col_e(const KParams& p, const PsiIO& ps, int i0,
 // kernel arguments: as arguments the compiler loads all 48 of them up front
 // and spills them to VGPR lanes in every wavefront, although only the
 // wavefronts with slab cells use them.
-template <bool GRADED, int NP>
+// ONE_MU: every material has the same ch (p.ch_all), so the H pass reads
+// no material id.
+template <bool GRADED, int NP, bool ONE_MU>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 8 : 1)))
 void k_update_h_col(KParams p, const PsiIO* __restrict__ psp) {
   const PsiIO& ps = *psp;
   int ib, j, k;
   if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, (p.nx + NP - 1) / NP))
     return;
-  col_h<GRADED, NP>(p, ps, ib * NP, j, k, p.nx);
+  col_h<GRADED, NP, ONE_MU>(p, ps, ib * NP, j, k, p.nx);
 }
 
 // 2-cell columns: held to 64 VGPRs (8 wavefronts per SIMD, no spills; the
@@ -1997,8 +2002,14 @@ void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const 
     else hipLaunchKernelGGL(k_update_e<GRADED>, g, blk, 0, s, kp, ps);
   } else {
     const PsiIO* psp = pl->d_psio + set;
-    if (PASS == 0) hipLaunchKernelGGL((k_update_h_col<GRADED, NP>), g, blk, 0, s, kp, psp);
-    else hipLaunchKernelGGL((k_update_e_col<GRADED, NP>), g, blk, 0, s, kp, psp);
+    if (PASS == 0) {
+      if (kp.ch_all > 0.f)
+        hipLaunchKernelGGL((k_update_h_col<GRADED, NP, true>), g, blk, 0, s, kp, psp);
+      else
+        hipLaunchKernelGGL((k_update_h_col<GRADED, NP, false>), g, blk, 0, s, kp, psp);
+    } else {
+      hipLaunchKernelGGL((k_update_e_col<GRADED, NP>), g, blk, 0, s, kp, psp);
+    }
   }
 }
 
@@ -2256,6 +2267,13 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
                            d->mat_coef[4 * m + 3]);
     if ((rc = dev_copy(pl, tmp.data(), tmp.size(), &co))) return bail(rc);
     kp.coef = co;
+    // one mu_r everywhere (the usual case): ch is a constant and the H
+    // update need not read the material id (2 B per cell of its traffic)
+    kp.ch_all = tmp[0].w > 0.f ? tmp[0].w : 0.f;
+    for (int m = 1; m < d->n_mat; ++m)
+      if (tmp[m].w != tmp[0].w) kp.ch_all = 0.f;
+    const char* chm = std::getenv("RFXH_CH_PER_CELL");
+    if (chm && std::string(chm) == "1") kp.ch_all = 0.f;
     uint32_t* fl;
     if ((rc = dev_copy(pl, d->mat_flags, (size_t)d->n_mat, &fl))) return bail(rc);
     kp.flags = fl;
