@@ -4,9 +4,9 @@ The plan's step can run as the two-pass kernels (H pass, E pass, in place;
 one cell per thread or columns of 2, 3 or 4 cells along x) or, with a
 second field set, as one fused kernel (flat or XCD-band
 workgroup order), as the barrier-free wave-march kernel (each wavefront
-walks one row along x) or as the marching kernel (6- or 14-row tiles of the y-z
-plane walking along x, in a chosen number of x segments) (include/rfxhip.h
-``rfxh_set_variant``).  All
+walks a pair of rows along x) or as the marching kernel (6- or 14-row tiles
+of the y-z plane walking along x), the last two in a chosen number of x
+segments (include/rfxhip.h ``rfxh_set_variant``).  All
 variants compute the same step.  Which is fastest depends on the grid (CPML
 share, plane size, cache fit), so the tuner runs a few real steps of the
 actual problem under each variant, timed with events on the stream the run
