@@ -394,13 +394,13 @@ __device__ __forceinline__ float3 h_core(const KParams& p, const PsiIO& ps, int 
 }
 
 // H at one cell reading E^n and H^{n-1/2} from Eo / Ho, storing into Hn.
-template <bool GRADED, bool WRITE>
+template <bool GRADED, bool WRITE, bool ONE_MU = false>
 __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo[3],
                                          const float* const Ho[3], float* const Hn[3],
                                          const PsiIO& ps, int i, int j, int k) {
   const unsigned P = (unsigned)p.plane;
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
-  const unsigned m = ldu16(p.mat, c);
+  const unsigned m = ONE_MU ? 0u : ldu16(p.mat, c);
   // neighbour loads are issued unconditionally from an in-range index and
   // zeroed afterwards, so no load sits behind a branch
   const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
@@ -410,7 +410,7 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
   float ex_k = ldf(Eo[0], ck), ey_k = ldf(Eo[1], ck);
   float ey_i = ldf(Eo[1], ci), ez_i = ldf(Eo[2], ci);
   const float hx = ldf(Ho[0], c), hy = ldf(Ho[1], c), hz = ldf(Ho[2], c);
-  const float ch = ld_ch(p.coef, m);
+  const float ch = ONE_MU ? p.ch_all : ld_ch(p.coef, m);
   ex_j = hj ? ex_j : 0.f;
   ez_j = hj ? ez_j : 0.f;
   ex_k = hk ? ex_k : 0.f;
@@ -941,12 +941,12 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
 // ---------------------------------------------------------------------------
 // Two-pass step, in place on p.f.
 // ---------------------------------------------------------------------------
-template <bool GRADED>
+template <bool GRADED, bool ONE_MU = false>
 __device__ __forceinline__ void two_pass_h(const KParams& p, const PsiIO& ps, int i, int j, int k) {
   const float* const Eo[3] = {p.f[0], p.f[1], p.f[2]};
   const float* const Ho[3] = {p.f[3], p.f[4], p.f[5]};
   float* const Hn[3] = {p.f[3], p.f[4], p.f[5]};
-  h_cell<GRADED, true>(p, Eo, Ho, Hn, ps, i, j, k);
+  h_cell<GRADED, true, ONE_MU>(p, Eo, Ho, Hn, ps, i, j, k);
 }
 
 template <bool GRADED>
@@ -972,11 +972,11 @@ __device__ __forceinline__ void two_pass_e(const KParams& p, const PsiIO& ps, in
   e_cell<GRADED>(p, Eo, En, ps, i, j, k, hx, hy, hz, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
 }
 
-template <bool GRADED>
+template <bool GRADED, bool ONE_MU>
 __global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
   int i, j, k;
   if (!cell_of(p, i, j, k)) return;
-  two_pass_h<GRADED>(p, ps, i, j, k);
+  two_pass_h<GRADED, ONE_MU>(p, ps, i, j, k);
 }
 
 template <bool GRADED>
@@ -1154,7 +1154,9 @@ __device__ __forceinline__ int batch_instance(const int* __restrict__ start, int
 // bodies and 35 with the column bodies at one cell; the instance
 // parameters stay behind a reference into the table (a by-value copy
 // measured as slow as the column bodies).
-template <int PASS>
+// ONE_MU: every instance has one mu_r (its ch_all), so the H pass reads no
+// material id.
+template <int PASS, bool ONE_MU>
 __global__ __launch_bounds__(kWG) void k_batch_pass(const BatchEntry* __restrict__ ents,
                                                    const int* __restrict__ start, int n_act) {
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -1163,8 +1165,8 @@ __global__ __launch_bounds__(kWG) void k_batch_pass(const BatchEntry* __restrict
   int i, j, k;
   if (!cell_at(p, L - gld(start, q), i, j, k)) return;
   if (PASS == 0) {
-    if (p.graded) two_pass_h<true>(p, ents[q].ps, i, j, k);
-    else two_pass_h<false>(p, ents[q].ps, i, j, k);
+    if (p.graded) two_pass_h<true, ONE_MU>(p, ents[q].ps, i, j, k);
+    else two_pass_h<false, ONE_MU>(p, ents[q].ps, i, j, k);
   } else {
     if (p.graded) two_pass_e<true>(p, ents[q].ps, i, j, k);
     else two_pass_e<false>(p, ents[q].ps, i, j, k);
@@ -1998,7 +2000,10 @@ void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const 
                    int set) {
   const dim3 g = NP == 1 ? pl->grid : pl->grid_col, blk = pl->block;
   if (NP == 1) {
-    if (PASS == 0) hipLaunchKernelGGL(k_update_h<GRADED>, g, blk, 0, s, kp, ps);
+    if (PASS == 0) {
+      if (kp.ch_all > 0.f) hipLaunchKernelGGL((k_update_h<GRADED, true>), g, blk, 0, s, kp, ps);
+      else hipLaunchKernelGGL((k_update_h<GRADED, false>), g, blk, 0, s, kp, ps);
+    }
     else hipLaunchKernelGGL(k_update_e<GRADED>, g, blk, 0, s, kp, ps);
   } else {
     const PsiIO* psp = pl->d_psio + set;
@@ -2839,8 +2844,16 @@ void batch_drop_graphs(rfxh_batch* b) {
 // One step of the first n_act instances (run order).
 int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
   const unsigned nb = (unsigned)b->start[n_act];
-  hipLaunchKernelGGL(k_batch_pass<0>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
-  hipLaunchKernelGGL(k_batch_pass<1>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents, b->d_start, n_act);
+  bool one_mu = true;
+  for (int q = 0; q < n_act; ++q) one_mu = one_mu && b->plans[b->order[q]]->kp.ch_all > 0.f;
+  if (one_mu)
+    hipLaunchKernelGGL((k_batch_pass<0, true>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+                       b->d_start, n_act);
+  else
+    hipLaunchKernelGGL((k_batch_pass<0, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+                       b->d_start, n_act);
+  hipLaunchKernelGGL((k_batch_pass<1, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+                     b->d_start, n_act);
   for (int q = 0; q < n_act; ++q) {
     rfxh_plan* pl = b->plans[b->order[q]];
     launch_accumulators(pl, s, pl->cur);
