@@ -41,6 +41,8 @@ extern "C" {
 #define RFXH_ZERO_EY 2u
 #define RFXH_ZERO_EZ 4u
 #define RFXH_POINT 8u     /* cell carries point operations (source/probe/port) */
+#define RFXH_POLES 0xFFFF0000u /* bit 16 + q: Debye pole q acts at the cell
+                                   (rfx/materials/debye.py:76-179 per-pole masks) */
 
 /* Point operation kinds. */
 #define RFXH_OP_PORT 0    /* record V = -E_c * d_par and the Ampere-loop current */
@@ -148,6 +150,16 @@ typedef struct {
   /* near-to-far-field boxes */
   int32_t n_ntff;
   const rfxh_ntff* ntff;
+  /* Debye dispersion (rfx/materials/debye.py, E update rfx/simulation.py:
+     406-412): E^{n+1} = ca E^n + cb curl H + sum_p cc_p P_p^n and
+     P_p^{n+1} = alpha_p P_p^n + beta_p (E^{n+1} + E^n) at the cells whose
+     material flags carry pole bits; ca, cb of every material are then the
+     Debye ones, (eps_inf - sum beta - sigma dt/2) / gamma and dt / gamma */
+  int32_t n_pole;          /* 0..16 */
+  const float* pole_coef;  /* host [n_mat][n_pole][3]: alpha, beta, cc (0 where absent) */
+  int32_t pol_lo[3];       /* box holding every cell with poles */
+  int32_t pol_n[3];
+  float* d_pol;            /* device [n_pole][3][box cells] polarisation, zeroed by the caller */
 } rfxh_desc;
 
 typedef struct rfxh_plan rfxh_plan;

@@ -298,6 +298,52 @@ def assemble_materials(shape, coords, geometry, materials, cpml_pad):
     return eps_r, sigma, mu_r, (pec if pec.any() else None)
 
 
+def debye_pole_masks(shape, coords, geometry, debye):
+    """rfx/api/_compile.py:203-232, 352: per-pole masks, OR-ed over every
+    shape whose material lists the pole, keyed by the pole's value in order
+    of first appearance (equal poles of different materials merge), not
+    replaced by later shapes and not extended into the CPML pads.
+    debye: material name -> [(delta_eps, tau), ...].  Returns
+    (poles, masks)."""
+    poles, masks = [], []
+    for lo, hi, name in geometry:
+        for pole in debye.get(name, []):
+            pole = (float(pole[0]), float(pole[1]))
+            if pole not in poles:
+                poles.append(pole)
+                masks.append(np.zeros(shape, bool))
+            sel = np.ix_(*[np.nonzero(_axis_mask(coords[a], lo[a], hi[a]))[0]
+                           for a in range(3)])
+            masks[poles.index(pole)][sel] = True
+    return poles, masks
+
+
+def debye_coefficients(poles, masks, eps_r, sigma, dt, dtype=np.float64):
+    """rfx/materials/debye.py:76-179 (init_debye): per pole
+    alpha = (2 tau - dt)/(2 tau + dt), beta = eps0 d_eps dt/(2 tau + dt)
+    inside its mask (0 elsewhere); gamma = eps0 eps_inf + sum beta +
+    sigma dt/2 (floored at eps0 1e-10); ca = (eps0 eps_inf - sum beta -
+    sigma dt/2)/gamma, cb = dt/gamma, cc_p = (1 - alpha_p)/gamma."""
+    dt_ = dtype(dt)
+    eps_inf = eps_r.astype(dtype) * dtype(EPS_0)
+    sig = sigma.astype(dtype)
+    alpha, beta = [], []
+    for (de, tau), m in zip(poles, masks):
+        a = (2.0 * tau - dt) / (2.0 * tau + dt)
+        b = EPS_0 * de * dt / (2.0 * tau + dt)
+        alpha.append(np.where(m, dtype(a), dtype(0.0)))
+        beta.append(np.where(m, dtype(b), dtype(0.0)))
+    beta_sum = np.zeros_like(eps_inf)
+    for b in beta:
+        beta_sum = beta_sum + b
+    gamma = eps_inf + beta_sum + sig * dt_ / dtype(2.0)
+    gamma = np.maximum(gamma, dtype(EPS_0 * 1e-10))
+    ca = (eps_inf - beta_sum - sig * dt_ / dtype(2.0)) / gamma
+    cb = dt_ / gamma
+    cc = [(dtype(1.0) - a) / gamma for a in alpha]
+    return ca, cb, alpha, beta, cc
+
+
 def tangential_edge_masks(cell_mask):
     """rfx/boundaries/pec.py:90-176 (non-periodic axes: zero-padded shifts)."""
     out = []
@@ -435,6 +481,9 @@ class StepConfig:
     flux: list = field(default_factory=list)
     # near-to-far-field boxes: (lo (i, j, k), hi (i, j, k), freqs f32)
     ntff: list = field(default_factory=list)
+    # Debye poles [(delta_eps, tau)] and their per-cell masks
+    debye_poles: list = field(default_factory=list)
+    debye_masks: list = field(default_factory=list)
 
 
 class Stepper:
@@ -457,6 +506,15 @@ class Stepper:
         self.ce_cpml = dt / (cfg.eps_r.astype(dtype) * dtype(EPS_0))
         self.ch_cpml = dt / (cfg.mu_r.astype(dtype) * dtype(MU_0))
         self.f = {c: np.zeros(cfg.shape, dtype) for c in COMPS}
+        self.debye = None
+        if cfg.debye_poles:
+            # with any pole every cell takes the Debye ca, cb (rfx/simulation.py:406-412)
+            ca, cb, alpha, beta, cc = debye_coefficients(
+                cfg.debye_poles, cfg.debye_masks, cfg.eps_r, cfg.sigma, cfg.dt, dtype)
+            self.ca, self.cb = ca, cb
+            pol = [{c: np.zeros(cfg.shape, dtype) for c in ("ex", "ey", "ez")}
+                   for _ in cfg.debye_poles]
+            self.debye = (alpha, beta, cc, pol)
         self.psi = {}
         self.step_count = 0
         if cfg.pec_mask is not None:
@@ -498,9 +556,24 @@ class Stepper:
         cx = self._dbwd(f["hz"], 1) - self._dbwd(f["hy"], 2)
         cy = self._dbwd(f["hx"], 2) - self._dbwd(f["hz"], 0)
         cz = self._dbwd(f["hy"], 0) - self._dbwd(f["hx"], 1)
-        f["ex"] = self.ca * f["ex"] + self.cb * cx
-        f["ey"] = self.ca * f["ey"] + self.cb * cy
-        f["ez"] = self.ca * f["ez"] + self.cb * cz
+        if self.debye is None:
+            f["ex"] = self.ca * f["ex"] + self.cb * cx
+            f["ey"] = self.ca * f["ey"] + self.cb * cy
+            f["ez"] = self.ca * f["ez"] + self.cb * cz
+            return
+        # rfx/materials/debye.py:182-250: E^{n+1} = ca E + cb curl + sum cc P^n,
+        # P^{n+1} = alpha P^n + beta (E^{n+1} + E^n)
+        alpha, beta, cc, pol = self.debye
+        for comp, curl in (("ex", cx), ("ey", cy), ("ez", cz)):
+            old = f[comp]
+            new = self.ca * old + self.cb * curl
+            acc = np.zeros_like(new)
+            for q in range(len(pol)):
+                acc = acc + cc[q] * pol[q][comp]
+            new = new + acc
+            for q in range(len(pol)):
+                pol[q][comp] = alpha[q] * pol[q][comp] + beta[q] * (new + old)
+            f[comp] = new
 
     def _slab(self, ax, side):
         n = self.cfg.cpml_n
@@ -979,6 +1052,9 @@ def build_uniform(spec, n_steps):
                      probes=probes, dft_planes=planes, ports_vi=ports_vi, flux=flux)
     cfg.flux_area = areas
     cfg.ntff = _ntff_entries(spec, g)
+    if spec.get("debye"):
+        cfg.debye_poles, cfg.debye_masks = debye_pole_masks(
+            shape, coords, spec.get("geometry", []), spec["debye"])
     return g, cfg
 
 
@@ -1152,6 +1228,9 @@ def build_graded(spec, n_steps):
                      ports_vi=ports_vi, record_order="graded")
     cfg.flux, cfg.flux_area = _flux_entries(spec, g, n_steps, True)
     cfg.ntff = _ntff_entries(spec, g)
+    if spec.get("debye"):
+        cfg.debye_poles, cfg.debye_masks = debye_pole_masks(
+            shape, coords, spec.get("geometry", []), spec["debye"])
     return g, cfg
 
 

@@ -20,6 +20,7 @@ RFXH_ZERO_EX = 1
 RFXH_ZERO_EY = 2
 RFXH_ZERO_EZ = 4
 RFXH_POINT = 8
+RFXH_POLES = 0xFFFF0000
 OP_PORT, OP_INJECT, OP_PROBE = 0, 1, 2
 
 _P = C.c_void_p
@@ -78,6 +79,8 @@ class Desc(C.Structure):
         ("plane_t_offset", C.c_int32),
         ("n_flux", C.c_int32), ("flux", C.POINTER(Flux)),
         ("n_ntff", C.c_int32), ("ntff", C.POINTER(Ntff)),
+        ("n_pole", C.c_int32), ("pole_coef", _FP), ("pol_lo", C.c_int32 * 3),
+        ("pol_n", C.c_int32 * 3), ("d_pol", _P),
     ]
 
 

@@ -22,7 +22,7 @@ from .cpml import face_tables
 from .engine import DeviceRun, Problem
 from .geometry import Box
 from .grid import Grid
-from .materials import MaterialMap, tangential_edge_bits
+from .materials import MAX_POLES, POLE_SHIFT, DebyePole, MaterialMap, tangential_edge_bits
 from .sources import GaussianPulse, port_sigma, sample_times
 
 F32 = np.float32
@@ -34,6 +34,7 @@ class _Material:
     eps_r: float = 1.0
     sigma: float = 0.0
     mu_r: float = 1.0
+    debye: tuple = ()          # DebyePole entries
 
 
 @dataclass
@@ -246,9 +247,17 @@ class Simulation:
     def add_material(self, name: str, *, eps_r: float = 1.0, sigma: float = 0.0,
                      mu_r: float = 1.0, debye_poles=None, lorentz_poles=None,
                      chi3: float = 0.0):
-        if debye_poles or lorentz_poles or chi3:
-            raise NotImplementedError("dispersive / nonlinear materials are not supported")
-        self._materials[name] = _Material(float(eps_r), float(sigma), float(mu_r))
+        """Material with optional Debye poles (rfx/api/__init__.py:812-853):
+        ``debye_poles`` is a list of DebyePole(delta_eps, tau) (or pairs);
+        eps_r is then eps_inf.  Lorentz poles and Kerr chi3 are not
+        supported."""
+        if lorentz_poles or chi3:
+            raise NotImplementedError("Lorentz poles and Kerr nonlinearity are not supported")
+        poles = tuple(DebyePole(float(p[0]), float(p[1])) for p in (debye_poles or ()))
+        for p in poles:
+            if not p.tau > 0:
+                raise ValueError(f"Debye pole {p}: tau must be positive")
+        self._materials[name] = _Material(float(eps_r), float(sigma), float(mu_r), poles)
         return self
 
     def add(self, shape, *, material: str):
@@ -437,6 +446,24 @@ class Simulation:
                 mat.fill_box(masks, m.eps_r, m.sigma, m.mu_r)
         if self._cpml_layers > 0:
             mat.extend_pads(grid.face_pads)
+        # Debye pole masks (rfx/api/_compile.py:203-232, 352): per pole, keyed by
+        # its value in order of first appearance, OR-ed over every shape of a
+        # material listing it (a later shape does not remove it), not extended
+        # into the CPML pads; carried as flag bits 16 + q of the material ids
+        poles = []
+        pole_bits = None
+        for box, name in self._geometry:
+            for pole in self._materials[name].debye:
+                if pole not in poles:
+                    poles.append(pole)
+                    if len(poles) > MAX_POLES:
+                        raise NotImplementedError(f"more than {MAX_POLES} distinct Debye poles")
+                if pole_bits is None:
+                    pole_bits = np.zeros(shape, np.uint32)
+                sel = np.ix_(*[np.nonzero(x)[0] for x in box.axis_masks(coords)])
+                pole_bits[sel] |= np.uint32(1 << (POLE_SHIFT + poles.index(pole)))
+        if pole_bits is not None:
+            mat.add_flags(pole_bits)
         # graded lane: sources are normalised by the materials before any
         # override or port fold (rfx/runners/nonuniform.py:630-643)
         concrete = {}
@@ -512,7 +539,8 @@ class Simulation:
         prob = Problem(shape=shape, dt=dt, graded=graded, inv_e=inv_e, inv_h=inv_h, mat=mat,
                        faces=faces, pec_axes=pec_axes, sources=sources, probes=probes,
                        ports=[p[:4] for p in ports], planes=planes, fluxes=fluxes,
-                       record_order="graded" if graded else "uniform", ntff=ntff)
+                       record_order="graded" if graded else "uniform", ntff=ntff,
+                       poles=[(p.delta_eps, p.tau) for p in poles])
         prob.flux_areas = areas
         prob.ntff_box = box
         return grid, prob, ports
@@ -655,6 +683,9 @@ class Simulation:
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),
                                                 s11)
+        if needs_grad and prob.poles:
+            raise NotImplementedError("gradients through Debye materials are not supported "
+                                      "(the reverse mode covers non-dispersive problems)")
         if needs_grad:
             from .adjoint import forward_with_grad
             ts, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,

@@ -107,7 +107,10 @@ def autotune(dr, trial_steps: int = 32, repeats: int = 2, cache_file: str | None
     stream = torch.cuda.current_stream(dr.device)
     timings = {}
     for v in variants(dr):
-        set_variant(dr, *v)
+        try:
+            set_variant(dr, *v)
+        except _lib.RfxHipError:
+            continue                      # not available for this plan (e.g. Debye poles)
         dr.reset()
         dr.run(0, steps)                      # warm-up, builds the graph
         best = None

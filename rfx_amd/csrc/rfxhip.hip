@@ -112,6 +112,12 @@ struct PsiIO {
   float* hw[6][2];
   const float* er[6][2];
   float* ew[6][2];
+  // Debye polarisation (in place, shared by both field sets): P[pole][comp]
+  // over the box pol_lo + [0, pol_n); per material and pole alpha, beta, cc
+  float* pol;
+  const float* pcoef;
+  int n_pole;
+  int pol_lo[3], pol_n[3];
 };
 
 constexpr int kDftK = 16;   // steps staged per fold of the DFT-plane accumulators
@@ -431,7 +437,47 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
 // CPML-E (cpml.py:608-777), PEC faces and mask, point operations.  H^{n+1/2}
 // at the cell and its backward neighbours (zero before the first cell) and
 // E^n at the cell come in as values; returns E^{n+1}.
-template <bool GRADED>
+// Debye ADE at one cell (rfx/materials/debye.py:182-250): with E the plain
+// update ca E^n + cb curl H,
+//   E^{n+1} = E + sum_p cc_p P_p^n,
+//   P_p^{n+1} = alpha_p P_p^n + beta_p (E^{n+1} + E^n),
+// before the CPML correction, PEC and point operations (the reference
+// applies those to the Debye-updated E).  ca, cb of the material table are
+// the Debye ones.  A cell outside the state box is left alone (the host
+// puts every cell with poles inside it).
+__device__ __noinline__ void debye_cell(const KParams& p, const PsiIO& ps, int i, int j, int k,
+                                        unsigned c, float o0, float o1, float o2, float& e0,
+                                        float& e1, float& e2) {
+  const int li = i - ps.pol_lo[0], lj = j - ps.pol_lo[1], lk = k - ps.pol_lo[2];
+  if (!ps.pol || li < 0 || lj < 0 || lk < 0 || li >= ps.pol_n[0] || lj >= ps.pol_n[1] ||
+      lk >= ps.pol_n[2])
+    return;
+  const unsigned R = (unsigned)ps.pol_n[0] * (unsigned)ps.pol_n[1] * (unsigned)ps.pol_n[2];
+  const unsigned r = ((unsigned)li * (unsigned)ps.pol_n[1] + (unsigned)lj) * (unsigned)ps.pol_n[2] +
+                     (unsigned)lk;
+  const unsigned m = ldu16(p.mat, c);
+  const float* pc = ps.pcoef + (size_t)m * ps.n_pole * 3;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int q = 0; q < ps.n_pole; ++q) {
+    const float cc = pc[3 * q + 2];
+    const float* P = ps.pol + (size_t)q * 3 * R;
+    s0 += cc * P[r];
+    s1 += cc * P[R + r];
+    s2 += cc * P[2 * R + r];
+  }
+  e0 += s0;
+  e1 += s1;
+  e2 += s2;
+  for (int q = 0; q < ps.n_pole; ++q) {
+    const float a = pc[3 * q], b = pc[3 * q + 1];
+    float* P = ps.pol + (size_t)q * 3 * R;
+    P[r] = a * P[r] + b * (e0 + o0);
+    P[R + r] = a * P[R + r] + b * (e1 + o1);
+    P[2 * R + r] = a * P[2 * R + r] + b * (e2 + o2);
+  }
+}
+
+template <bool GRADED, bool DISP = false>
 __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, int i, int j, int k,
                                             unsigned c, float4 co, uint32_t fl, float e0, float e1,
                                             float e2, float hx, float hy, float hz, float hy_i,
@@ -446,9 +492,11 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
     ix = iy = iz = p.inv_e0;
   }
   const float ca = co.x, cb = co.y, ce = co.z;
+  const float o0 = e0, o1 = e1, o2 = e2;
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
   e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
   e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
+  if (DISP && (fl & RFXH_POLES)) debye_cell(p, ps, i, j, k, c, o0, o1, o2, e0, e1, e2);
   if (in_slab(p, i, j, k)) {
     // keep the CPML pointer and profile loads inside the slab branch: the
     // compiler would otherwise hoist them into every wavefront (and spill
@@ -511,19 +559,19 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
   return make_float3(e0, e1, e2);
 }
 
-template <bool GRADED>
+template <bool GRADED, bool DISP = false>
 __device__ __forceinline__ float3 e_core(const KParams& p, const PsiIO& ps, int i, int j, int k,
                                          unsigned c, unsigned m, float e0, float e1, float e2,
                                          float hx, float hy, float hz, float hy_i, float hz_i,
                                          float hx_j, float hz_j, float hx_k, float hy_k) {
   // one 16-byte row carries everything the E update needs
   const float4 ec = gld(p.ecoef, (int)m);
-  return e_core_co<GRADED>(p, ps, i, j, k, c, ec, __float_as_uint(ec.w), e0, e1, e2, hx, hy, hz,
+  return e_core_co<GRADED, DISP>(p, ps, i, j, k, c, ec, __float_as_uint(ec.w), e0, e1, e2, hx, hy, hz,
                            hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
 }
 
 // E at one cell reading E^n from Eo, storing E^{n+1} into En.
-template <bool GRADED>
+template <bool GRADED, bool DISP = false>
 __device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3],
                                        float* const En[3], const PsiIO& ps, int i, int j, int k,
                                        float hx, float hy, float hz, float hy_i, float hz_i,
@@ -531,8 +579,8 @@ __device__ __forceinline__ void e_cell(const KParams& p, const float* const Eo[3
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
   const unsigned m = ldu16(p.mat, c);
   const float e0 = ldf(Eo[0], c), e1 = ldf(Eo[1], c), e2 = ldf(Eo[2], c);
-  const float3 e = e_core<GRADED>(p, ps, i, j, k, c, m, e0, e1, e2, hx, hy, hz, hy_i, hz_i, hx_j,
-                                  hz_j, hx_k, hy_k);
+  const float3 e = e_core<GRADED, DISP>(p, ps, i, j, k, c, m, e0, e1, e2, hx, hy, hz, hy_i, hz_i,
+                                        hx_j, hz_j, hx_k, hy_k);
   stf(En[0], c, e.x);
   stf(En[1], c, e.y);
   stf(En[2], c, e.z);
@@ -949,7 +997,7 @@ __device__ __forceinline__ void two_pass_h(const KParams& p, const PsiIO& ps, in
   h_cell<GRADED, true, ONE_MU>(p, Eo, Ho, Hn, ps, i, j, k);
 }
 
-template <bool GRADED>
+template <bool GRADED, bool DISP = false>
 __device__ __forceinline__ void two_pass_e(const KParams& p, const PsiIO& ps, int i, int j, int k) {
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
   const float* Hx = p.f[3];
@@ -969,7 +1017,7 @@ __device__ __forceinline__ void two_pass_e(const KParams& p, const PsiIO& ps, in
   hz_i = hi ? hz_i : 0.f;
   const float* const Eo[3] = {p.f[0], p.f[1], p.f[2]};
   float* const En[3] = {p.f[0], p.f[1], p.f[2]};
-  e_cell<GRADED>(p, Eo, En, ps, i, j, k, hx, hy, hz, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
+  e_cell<GRADED, DISP>(p, Eo, En, ps, i, j, k, hx, hy, hz, hy_i, hz_i, hx_j, hz_j, hx_k, hy_k);
 }
 
 template <bool GRADED, bool ONE_MU>
@@ -979,11 +1027,13 @@ __global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
   two_pass_h<GRADED, ONE_MU>(p, ps, i, j, k);
 }
 
-template <bool GRADED>
+// DISP: the plan has Debye poles (a separate instance: the ADE branch would
+// cost the plain kernels registers)
+template <bool GRADED, bool DISP>
 __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
   int i, j, k;
   if (!cell_of(p, i, j, k)) return;
-  two_pass_e<GRADED>(p, ps, i, j, k);
+  two_pass_e<GRADED, DISP>(p, ps, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1047,7 +1097,7 @@ __device__ __forceinline__ void col_h(const KParams& p, const PsiIO& ps, int i0,
   }
 }
 
-template <bool GRADED, int NP>
+template <bool GRADED, int NP, bool DISP>
 __device__ __forceinline__ void col_e(const KParams& p, const PsiIO& ps, int i0, int j, int k,
                                       int iend) {
   const unsigned P = (unsigned)p.plane;
@@ -1089,8 +1139,9 @@ __device__ __forceinline__ void col_e(const KParams& p, const PsiIO& ps, int i0,
     const unsigned c = c0 + (unsigned)q * P;
     const float hxj_ = bj ? hxj[q] : 0.f, hzj_ = bj ? hzj[q] : 0.f;
     const float hxk_ = bk ? hxk[q] : 0.f, hyk_ = bk ? hyk[q] : 0.f;
-    const float3 e = e_core<GRADED>(p, ps, i0 + q, j, k, c, m[q], e0[q], e1[q], e2[q], hx[q + 1],
-                                    hy[q + 1], hz[q + 1], hy[q], hz[q], hxj_, hzj_, hxk_, hyk_);
+    const float3 e = e_core<GRADED, DISP>(p, ps, i0 + q, j, k, c, m[q], e0[q], e1[q], e2[q],
+                                          hx[q + 1], hy[q + 1], hz[q + 1], hy[q], hz[q], hxj_, hzj_,
+                                          hxk_, hyk_);
     stf(p.f[0], c, e.x);
     stf(p.f[1], c, e.y);
     stf(p.f[2], c, e.z);
@@ -1115,14 +1166,14 @@ void k_update_h_col(KParams p, const PsiIO* __restrict__ psp) {
 
 // 2-cell columns: held to 64 VGPRs (8 wavefronts per SIMD, no spills; the
 // compiler's own choice, 67, allows 7) -- measured 3.7 % off the E pass.
-template <bool GRADED, int NP>
+template <bool GRADED, int NP, bool DISP>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 8 : 1)))
 void k_update_e_col(KParams p, const PsiIO* __restrict__ psp) {
   const PsiIO& ps = *psp;
   int ib, j, k;
   if (!cell_at(p, xcd_remap((int)blockIdx.x, (int)gridDim.x), ib, j, k, (p.nx + NP - 1) / NP))
     return;
-  col_e<GRADED, NP>(p, ps, ib * NP, j, k, p.nx);
+  col_e<GRADED, NP, DISP>(p, ps, ib * NP, j, k, p.nx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,7 +1207,7 @@ __device__ __forceinline__ int batch_instance(const int* __restrict__ start, int
 // measured as slow as the column bodies).
 // ONE_MU: every instance has one mu_r (its ch_all), so the H pass reads no
 // material id.
-template <int PASS, bool ONE_MU>
+template <int PASS, bool ONE_MU, bool DISP>
 __global__ __launch_bounds__(kWG) void k_batch_pass(const BatchEntry* __restrict__ ents,
                                                    const int* __restrict__ start, int n_act) {
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -1168,8 +1219,8 @@ __global__ __launch_bounds__(kWG) void k_batch_pass(const BatchEntry* __restrict
     if (p.graded) two_pass_h<true, ONE_MU>(p, ents[q].ps, i, j, k);
     else two_pass_h<false, ONE_MU>(p, ents[q].ps, i, j, k);
   } else {
-    if (p.graded) two_pass_e<true>(p, ents[q].ps, i, j, k);
-    else two_pass_e<false>(p, ents[q].ps, i, j, k);
+    if (p.graded) two_pass_e<true, DISP>(p, ents[q].ps, i, j, k);
+    else two_pass_e<false, DISP>(p, ents[q].ps, i, j, k);
   }
 }
 
@@ -1879,6 +1930,11 @@ __global__ void k_advance(long long* step) {
 
 struct rfxh_plan {
   KParams kp;
+  // Debye polarisation state (caller-owned) and per-material pole coefficients
+  float* pol = nullptr;
+  const float* pcoef = nullptr;
+  int n_pole = 0;
+  int pol_lo[3] = {0, 0, 0}, pol_n[3] = {0, 0, 0};
   dim3 grid, block;
   dim3 grid_col;                // column kernels (ncol cells per thread)
   bool fused = false;           // ping-pong step over two field sets (fused or marching kernel)
@@ -2004,7 +2060,8 @@ void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const 
       if (kp.ch_all > 0.f) hipLaunchKernelGGL((k_update_h<GRADED, true>), g, blk, 0, s, kp, ps);
       else hipLaunchKernelGGL((k_update_h<GRADED, false>), g, blk, 0, s, kp, ps);
     }
-    else hipLaunchKernelGGL(k_update_e<GRADED>, g, blk, 0, s, kp, ps);
+    else if (pl->n_pole > 0) hipLaunchKernelGGL((k_update_e<GRADED, true>), g, blk, 0, s, kp, ps);
+    else hipLaunchKernelGGL((k_update_e<GRADED, false>), g, blk, 0, s, kp, ps);
   } else {
     const PsiIO* psp = pl->d_psio + set;
     if (PASS == 0) {
@@ -2013,7 +2070,10 @@ void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const 
       else
         hipLaunchKernelGGL((k_update_h_col<GRADED, NP, false>), g, blk, 0, s, kp, psp);
     } else {
-      hipLaunchKernelGGL((k_update_e_col<GRADED, NP>), g, blk, 0, s, kp, psp);
+      if (pl->n_pole > 0)
+        hipLaunchKernelGGL((k_update_e_col<GRADED, NP, true>), g, blk, 0, s, kp, psp);
+      else
+        hipLaunchKernelGGL((k_update_e_col<GRADED, NP, false>), g, blk, 0, s, kp, psp);
     }
   }
 }
@@ -2048,6 +2108,13 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   // kernels of this step, recorded on the stream the kernels run on.
   const int to = pl->fused ? 1 - from : from;
   PsiIO ps;
+  ps.pol = pl->pol;
+  ps.pcoef = pl->pcoef;
+  ps.n_pole = pl->n_pole;
+  for (int a = 0; a < 3; ++a) {
+    ps.pol_lo[a] = pl->pol_lo[a];
+    ps.pol_n[a] = pl->pol_n[a];
+  }
   for (int f = 0; f < 6; ++f)
     for (int t = 0; t < 2; ++t) {
       const long long ro = from * pl->psi_set, wo = to * pl->psi_set;
@@ -2297,7 +2364,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   // need the second field set); rfxh_set_variant changes it later
   const char* kern = std::getenv("RFXH_KERNEL");
   const std::string kname = kern ? kern : "";
-  pl->fused = has2 && (kname == "fused" || kname == "march");
+  pl->fused = has2 && d->n_pole == 0 && (kname == "fused" || kname == "march");
   pl->march = has2 && kname == "march";
   pl->ncol = (pl->fused || kname == "onecell") ? 1 : 2;
   for (int c = 0; c < 6; ++c) {
@@ -2324,6 +2391,33 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
       kp.pk[f] = k;
     }
     off += 4 * s;
+  }
+  // Debye poles: cells whose material flags carry pole bits update their
+  // polarisation in the box pol_lo + [0, pol_n)
+  if (d->n_pole < 0 || d->n_pole > 16) return bail(fail(RFXH_EINVAL, "n_pole must be 0..16"));
+  if (d->n_pole > 0) {
+    if (!d->pole_coef || !d->d_pol) return bail(fail(RFXH_EINVAL, "Debye tables missing"));
+    for (int a = 0; a < 3; ++a)
+      if (d->pol_lo[a] < 0 || d->pol_n[a] < 1 || d->pol_lo[a] + d->pol_n[a] > N[a])
+        return bail(fail(RFXH_EINVAL, "Debye state box outside the grid"));
+    for (int m = 0; m < d->n_mat; ++m)
+      if ((d->mat_flags[m] & RFXH_POLES) >> 16 >= (1u << d->n_pole))
+        return bail(fail(RFXH_EINVAL, "material flags name a pole past n_pole"));
+    if ((long long)d->pol_n[0] * d->pol_n[1] * d->pol_n[2] * 3 * d->n_pole >= (1LL << 32))
+      return bail(fail(RFXH_EINVAL, "Debye state larger than 2^32 floats"));
+    float* pc;
+    if ((rc = dev_copy(pl, d->pole_coef, (size_t)d->n_mat * d->n_pole * 3, &pc))) return bail(rc);
+    pl->pcoef = pc;
+    pl->pol = d->d_pol;
+    pl->n_pole = d->n_pole;
+    for (int a = 0; a < 3; ++a) {
+      pl->pol_lo[a] = d->pol_lo[a];
+      pl->pol_n[a] = d->pol_n[a];
+    }
+  } else {
+    for (int m = 0; m < d->n_mat; ++m)
+      if (d->mat_flags[m] & RFXH_POLES)
+        return bail(fail(RFXH_EINVAL, "material flags name Debye poles but n_pole is 0"));
   }
   // CPML set 1 is addressed as set-0 pointer + psi_set floats
   pl->psi_set = (has2 && any_cpml) ? (long long)(d->d_psi2 - d->d_psi) : 0;
@@ -2644,6 +2738,13 @@ static RegionParams region_params(const rfxh_region* r) {
 
 static PsiIO psi_io(const rfxh_plan* pl, int set) {
   PsiIO ps;
+  ps.pol = pl->pol;
+  ps.pcoef = pl->pcoef;
+  ps.n_pole = pl->n_pole;
+  for (int a = 0; a < 3; ++a) {
+    ps.pol_lo[a] = pl->pol_lo[a];
+    ps.pol_n[a] = pl->pol_n[a];
+  }
   for (int f = 0; f < 6; ++f)
     for (int t = 0; t < 2; ++t) {
       const long long o = set * pl->psi_set;
@@ -2662,6 +2763,8 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
   const bool pingpong = kernel == 1 || kernel == 2 || kernel == 3 || kernel == 7;
   if (pingpong && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
+  if (pingpong && pl->n_pole > 0)
+    return fail(RFXH_EINVAL, "Debye materials run on the two-pass kernels (0, 4, 5, 6)");
   if (kernel == 2 || kernel == 3 || kernel == 7) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     if (kernel == 7) {
@@ -2709,6 +2812,8 @@ int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
 int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
                     float* d_rec, void* stream) {
   if (!pl) return fail(RFXH_EINVAL, "null plan");
+  if (pl->n_pole > 0)
+    return fail(RFXH_EINVAL, "Debye materials: the reverse mode covers non-dispersive plans only");
   if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
     return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
   int rc = check_region(pl, region);
@@ -2741,6 +2846,8 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
 
 int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
   if (!pl || !ad) return fail(RFXH_EINVAL, "null plan or adjoint descriptor");
+  if (pl->n_pole > 0)
+    return fail(RFXH_EINVAL, "Debye materials: the reverse mode covers non-dispersive plans only");
   if (lo < 0 || hi < lo || hi > pl->max_steps) return fail(RFXH_EINVAL, "bad step range");
   for (int c = 0; c < 6; ++c)
     if (!ad->d_adj[c]) return fail(RFXH_EINVAL, "adjoint field pointer missing");
@@ -2844,16 +2951,23 @@ void batch_drop_graphs(rfxh_batch* b) {
 // One step of the first n_act instances (run order).
 int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
   const unsigned nb = (unsigned)b->start[n_act];
-  bool one_mu = true;
-  for (int q = 0; q < n_act; ++q) one_mu = one_mu && b->plans[b->order[q]]->kp.ch_all > 0.f;
+  bool one_mu = true, disp = false;
+  for (int q = 0; q < n_act; ++q) {
+    one_mu = one_mu && b->plans[b->order[q]]->kp.ch_all > 0.f;
+    disp = disp || b->plans[b->order[q]]->n_pole > 0;
+  }
   if (one_mu)
-    hipLaunchKernelGGL((k_batch_pass<0, true>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+    hipLaunchKernelGGL((k_batch_pass<0, true, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
                        b->d_start, n_act);
   else
-    hipLaunchKernelGGL((k_batch_pass<0, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+    hipLaunchKernelGGL((k_batch_pass<0, false, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
                        b->d_start, n_act);
-  hipLaunchKernelGGL((k_batch_pass<1, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
-                     b->d_start, n_act);
+  if (disp)
+    hipLaunchKernelGGL((k_batch_pass<1, false, true>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+                       b->d_start, n_act);
+  else
+    hipLaunchKernelGGL((k_batch_pass<1, false, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
+                       b->d_start, n_act);
   for (int q = 0; q < n_act; ++q) {
     rfxh_plan* pl = b->plans[b->order[q]];
     launch_accumulators(pl, s, pl->cur);

@@ -39,6 +39,8 @@ class Problem:
     record_order: str = "uniform"                 # "uniform" | "graded"
     # near-to-far-field boxes: (lo (i, j, k), hi (i, j, k), freqs f64)
     ntff: list = field(default_factory=list)
+    # Debye poles (delta_eps, tau); pole q acts where the material flags carry bit 16 + q
+    poles: list = field(default_factory=list)
 
     @property
     def plane_t_offset(self):
@@ -80,7 +82,22 @@ class DeviceRun:
         mat = prob.mat
         for op in ops:
             mat.set_cell(tuple(op[6]), add_flags=_lib.RFXH_POINT)
-        self.coef, self.flags = mat.coefficient_table(prob.dt)
+        if prob.poles:
+            from .materials import POLE_SHIFT, debye_tables
+            self.coef, self.flags, self.pole_rows = debye_tables(mat, prob.dt, prob.poles)
+            # the box holding every cell with poles carries the polarisation
+            disp = np.nonzero((self.flags >> POLE_SHIFT) != 0)[0]
+            cells = np.isin(mat.ids, disp.astype(np.uint16))
+            lo, n = [], []
+            for a in range(3):
+                any_a = np.any(cells, axis=tuple(b for b in range(3) if b != a))
+                idx = np.nonzero(any_a)[0]
+                lo.append(int(idx[0]))
+                n.append(int(idx[-1]) - int(idx[0]) + 1)
+            self.pol_box = (tuple(lo), tuple(n))
+        else:
+            self.coef, self.flags = mat.coefficient_table(prob.dt)
+            self.pole_rows, self.pol_box = None, None
         self._ops = (_lib.PointOp * max(len(ops), 1))()
         for q, (cell, kind, comp, slot, dpar, loop, _) in enumerate(ops):
             o = self._ops[q]
@@ -108,6 +125,10 @@ class DeviceRun:
         else:
             self.src_wave = None
         self.probe_out = torch.zeros((T, max(n_probe, 1)), dtype=torch.float32, device=dev)
+        self.pol = None
+        if prob.poles:
+            nbox = int(np.prod(self.pol_box[1]))
+            self.pol = torch.zeros(len(prob.poles) * 3 * nbox, dtype=torch.float32, device=dev)
         self.port_out = torch.zeros((T, max(n_port, 1), 2), dtype=torch.float32, device=dev)
         self.plane_acc = []
         planes_c = (_lib.Plane * max(len(prob.planes), 1))()
@@ -214,6 +235,13 @@ class DeviceRun:
         d.flux = flux_c
         d.n_ntff = len(prob.ntff)
         d.ntff = ntff_c
+        if prob.poles:
+            d.n_pole = len(prob.poles)
+            d.pole_coef = _f32p(self.pole_rows)
+            for a in range(3):
+                d.pol_lo[a] = self.pol_box[0][a]
+                d.pol_n[a] = self.pol_box[1][a]
+            d.d_pol = self.pol.data_ptr()
         self.desc = d
         self.plan = C.c_void_p()
         with torch.cuda.device(dev):
@@ -264,6 +292,8 @@ class DeviceRun:
         for faces in self.ntff_acc:
             for a in faces:
                 a.zero_()
+        if self.pol is not None:
+            self.pol.zero_()
 
     def set_timing(self, enable: bool):
         _lib.check(self.lib.rfxh_set_timing(self.plan, 1 if enable else 0))

@@ -13,12 +13,22 @@ and cells that carry point operations (sources, probes, ports).
 """
 from __future__ import annotations
 
+from typing import NamedTuple
+
 import numpy as np
 
 from .constants import EPS_0, MU_0
 
 F32 = np.float32
 MAX_IDS = 65536
+POLE_SHIFT = 16      # flag bit 16 + q: Debye pole q acts at the cell
+MAX_POLES = 16
+
+
+class DebyePole(NamedTuple):
+    """One Debye relaxation term d_eps / (1 + j w tau) (rfx/materials/debye.py:38-48)."""
+    delta_eps: float
+    tau: float
 
 
 class MaterialMap:
@@ -98,18 +108,19 @@ class MaterialMap:
                                     m if mu is None else mu, f | add_flags)
 
     def add_flags(self, cell_mask_bits: np.ndarray):
-        """OR per-cell flag bits (uint8 array of the grid shape) into the ids."""
+        """OR per-cell flag bits (integer array of the grid shape, up to 32
+        bits) into the ids."""
         where = np.nonzero(cell_mask_bits)
         if len(where[0]) == 0:
             return
         cur = self.ids[where].astype(np.int64)
         bits = cell_mask_bits[where].astype(np.int64)
-        key = cur * 256 + bits
+        key = cur * (1 << 32) + bits
         uniq, inv = np.unique(key, return_inverse=True)
         new = np.empty(len(uniq), np.uint16)
         for q, kv in enumerate(uniq):
-            e, s, m, f = self.table[int(kv // 256)]
-            new[q] = self.get_id(e, s, m, f | int(kv % 256))
+            e, s, m, f = self.table[int(kv >> 32)]
+            new[q] = self.get_id(e, s, m, f | int(kv & 0xFFFFFFFF))
         self.ids[where] = new[inv]
 
     def override(self, eps_r=None, sigma=None, mu_r=None):
@@ -164,6 +175,43 @@ class MaterialMap:
         ch = dt32 / (mu_r * F32(MU_0))
         coef = np.stack([ca, cb, ce, ch], axis=1).astype(F32)
         return np.ascontiguousarray(coef), np.ascontiguousarray(flg)
+
+
+def debye_tables(mat: MaterialMap, dt: float, poles):
+    """Coefficient table and per-pole rows when Debye poles exist
+    (rfx/materials/debye.py:76-179, init_debye, in its float32 operation
+    order): per pole alpha = (2 tau - dt)/(2 tau + dt) and beta = eps0 d_eps
+    dt/(2 tau + dt) where the material's flags carry the pole, else 0;
+    gamma = eps_inf + sum beta + sigma dt/2 (floored at eps0 1e-10),
+    ca = (eps_inf - sum beta - sigma dt/2)/gamma, cb = dt/gamma for every
+    material, cc = (1 - alpha)/gamma.  Returns (coef [n_mat][4] with the
+    Debye ca, cb and the usual ce, ch; flags; pole rows [n_mat][n_pole][3] =
+    alpha, beta, cc)."""
+    coef, flg = mat.coefficient_table(dt)
+    eps_r, sig, _, _ = mat.props()
+    n_mat, n_pole = len(flg), len(poles)
+    dt32 = F32(dt)
+    alpha = np.zeros((n_pole, n_mat), F32)
+    beta = np.zeros((n_pole, n_mat), F32)
+    for q, (de, tau) in enumerate(poles):
+        on = ((flg >> (POLE_SHIFT + q)) & 1).astype(bool)
+        alpha[q] = np.where(on, F32((2.0 * tau - dt) / (2.0 * tau + dt)), F32(0.0))
+        beta[q] = np.where(on, F32(EPS_0 * de * dt / (2.0 * tau + dt)), F32(0.0))
+    eps_inf = eps_r * F32(EPS_0)
+    beta_sum = np.zeros(n_mat, F32)
+    for q in range(n_pole):
+        beta_sum = beta_sum + beta[q]
+    half_loss = sig * dt32 / F32(2.0)
+    gamma = np.maximum(eps_inf + beta_sum + half_loss, F32(EPS_0 * 1e-10))
+    coef = coef.copy()
+    coef[:, 0] = (eps_inf - beta_sum - half_loss) / gamma
+    coef[:, 1] = dt32 / gamma
+    rows = np.zeros((n_mat, n_pole, 3), F32)
+    for q in range(n_pole):
+        rows[:, q, 0] = alpha[q]
+        rows[:, q, 1] = beta[q]
+        rows[:, q, 2] = (F32(1.0) - alpha[q]) / gamma
+    return np.ascontiguousarray(coef), flg, np.ascontiguousarray(rows)
 
 
 def tangential_edge_bits(pec_mask: np.ndarray) -> np.ndarray:

@@ -94,7 +94,8 @@ def to_sim(spec, device=None):
                        cpml_layers=spec.get("cpml_layers", 16),
                        cpml_kappa_max=spec.get("kappa_max", 1.0), **kw)
     for name, (e, s, m) in spec.get("materials", {}).items():
-        sim.add_material(name, eps_r=e, sigma=s, mu_r=m)
+        poles = [R.DebyePole(de, tau) for de, tau in spec.get("debye", {}).get(name, [])]
+        sim.add_material(name, eps_r=e, sigma=s, mu_r=m, debye_poles=poles or None)
     for lo, hi, name in spec.get("geometry", []):
         sim.add(R.Box(lo, hi), material=name)
     for entry in spec.get("ports", []):
@@ -244,6 +245,26 @@ def dipole_ntff_case():
                 probes=[((0.015, 0.015, 0.015), "ez")], lane="run",
                 ntff=((0.004, 0.004, 0.004), (0.026, 0.026, 0.026),
                       np.array([3e9], np.float32), 1)), 200
+
+
+def debye_case(tau, delta_eps=3.0, eps_inf=2.0, lane="run", overlap=True):
+    """Small CPML case with a Debye box (eps_inf, one pole delta_eps, tau),
+    optionally a second box of another material overlapping it (the pole
+    stays where the later box sets the permittivity), a lumped port and a
+    soft source outside the Debye box, and E / H probes."""
+    geometry = [((0.004, 0.003, 0.002), (0.011, 0.009, 0.007), "deb")]
+    if overlap:
+        geometry.append(((0.009, 0.002, 0.001), (0.013, 0.006, 0.005), "d"))
+    return dict(freq_max=10e9, domain=(0.016, 0.012, 0.010), dx=1e-3, boundary="cpml",
+                cpml_layers=4,
+                materials={"deb": (eps_inf, 0.01, 1.0), "d": (3.0, 0.0, 1.0)},
+                debye={"deb": [(delta_eps, tau)]},
+                geometry=geometry,
+                ports=[((0.003, 0.006, 0.005), "ez", 0.0, o.Pulse(8e9, 0.8), True),
+                       ((0.002, 0.003, 0.003), "ex", 50.0, o.Pulse(7e9, 0.9), True)],
+                probes=[((0.013, 0.006, 0.005), "ez"), ((0.007, 0.006, 0.004), "ey"),
+                        ((0.010, 0.005, 0.004), "hx")],
+                lane=lane)
 
 
 def box_total(fluxes):
