@@ -41,8 +41,9 @@ extern "C" {
 #define RFXH_ZERO_EY 2u
 #define RFXH_ZERO_EZ 4u
 #define RFXH_POINT 8u     /* cell carries point operations (source/probe/port) */
-#define RFXH_POLES 0xFFFF0000u /* bit 16 + q: Debye pole q acts at the cell
-                                   (rfx/materials/debye.py:76-179 per-pole masks) */
+#define RFXH_POLES 0xFFFF0000u /* bit 16 + q: dispersion pole q acts at the cell
+                                   (per-pole masks, rfx/materials/debye.py:76-179,
+                                   rfx/materials/lorentz.py:103-198) */
 
 /* Point operation kinds. */
 #define RFXH_OP_PORT 0    /* record V = -E_c * d_par and the Ampere-loop current */
@@ -150,16 +151,20 @@ typedef struct {
   /* near-to-far-field boxes */
   int32_t n_ntff;
   const rfxh_ntff* ntff;
-  /* Debye dispersion (rfx/materials/debye.py, E update rfx/simulation.py:
-     406-412): E^{n+1} = ca E^n + cb curl H + sum_p cc_p P_p^n and
-     P_p^{n+1} = alpha_p P_p^n + beta_p (E^{n+1} + E^n) at the cells whose
-     material flags carry pole bits; ca, cb of every material are then the
-     Debye ones, (eps_inf - sum beta - sigma dt/2) / gamma and dt / gamma */
-  int32_t n_pole;          /* 0..16 */
-  const float* pole_coef;  /* host [n_mat][n_pole][3]: alpha, beta, cc (0 where absent) */
+  /* dispersion (E update rfx/simulation.py:361-500): at the cells whose
+     material flags carry pole bits, with E the plain update ca E^n + cb curl H,
+       Lorentz poles (rfx/materials/lorentz.py):  P_l^{n+1} = a P_l^n + b P_l^{n-1} + c E^n,
+       E^{n+1} = E + sum_d cc_d P_d^n - cc_l sum_l (P_l^{n+1} - P_l^n),
+       Debye poles (rfx/materials/debye.py):  P_d^{n+1} = alpha P_d^n + beta (E^{n+1} + E^n);
+     ca, cb of every material are then the dispersive ones */
+  int32_t n_pole;          /* 0..16: poles [0, n_debye) Debye, [n_debye, n_pole) Lorentz */
+  int32_t n_debye;
+  const float* pole_coef;  /* host [n_mat][n_pole][4]: Debye (alpha, beta, cc, 0), Lorentz
+                              (a, b, c, cc_l); zero where the material lacks the pole */
   int32_t pol_lo[3];       /* box holding every cell with poles */
   int32_t pol_n[3];
-  float* d_pol;            /* device [n_pole][3][box cells] polarisation, zeroed by the caller */
+  float* d_pol;            /* device, zeroed by the caller: per Debye pole P[3][box cells],
+                              then per Lorentz pole P[3][box cells], P^{n-1}[3][box cells] */
 } rfxh_desc;
 
 typedef struct rfxh_plan rfxh_plan;

@@ -79,8 +79,8 @@ def _cell(shape, idx):
 
 def run_c(cfg: o.StepConfig, n_steps: int, double: bool = True,
           init_fields=None, want_fields: bool = True):
-    if cfg.debye_poles:
-        raise NotImplementedError("the C stepper has no Debye ADE: use fdtd_np.run_loop")
+    if cfg.debye_poles or cfg.lorentz_poles:
+        raise NotImplementedError("the C stepper has no dispersion ADE: use fdtd_np.run_loop")
     lib = _load()
     keep = []
 

@@ -298,17 +298,37 @@ def assemble_materials(shape, coords, geometry, materials, cpml_pad):
     return eps_r, sigma, mu_r, (pec if pec.any() else None)
 
 
+def lorentz_coefficients(poles, masks, eps_r, sigma, dt, dtype=np.float64):
+    """rfx/materials/lorentz.py:103-198 (init_lorentz): per pole, inside its
+    mask, a = (2 - w0^2 dt^2)/(1 + delta dt), b = -(1 - delta dt)/(1 +
+    delta dt), c = eps0 kappa dt^2/(1 + delta dt); gamma = eps0 eps_inf +
+    sigma dt/2, ca = (eps0 eps_inf - sigma dt/2)/gamma, cb = dt/gamma,
+    cc = 1/gamma."""
+    dt_ = dtype(dt)
+    eps_inf = eps_r.astype(dtype) * dtype(EPS_0)
+    sig = sigma.astype(dtype)
+    a_l, b_l, c_l = [], [], []
+    for (w0, delta, kappa), m in zip(poles, masks):
+        denom = 1.0 + delta * dt
+        a_l.append(np.where(m, dtype((2.0 - w0 ** 2 * dt ** 2) / denom), dtype(0.0)))
+        b_l.append(np.where(m, dtype(-(1.0 - delta * dt) / denom), dtype(0.0)))
+        c_l.append(np.where(m, dtype(EPS_0 * kappa * dt ** 2 / denom), dtype(0.0)))
+    gamma = np.maximum(eps_inf + sig * dt_ / dtype(2.0), dtype(EPS_0 * 1e-10))
+    ca = (eps_inf - sig * dt_ / dtype(2.0)) / gamma
+    return ca, dt_ / gamma, a_l, b_l, c_l, dtype(1.0) / gamma
+
+
 def debye_pole_masks(shape, coords, geometry, debye):
     """rfx/api/_compile.py:203-232, 352: per-pole masks, OR-ed over every
     shape whose material lists the pole, keyed by the pole's value in order
     of first appearance (equal poles of different materials merge), not
     replaced by later shapes and not extended into the CPML pads.
-    debye: material name -> [(delta_eps, tau), ...].  Returns
-    (poles, masks)."""
+    debye: material name -> [pole tuple, ...] (Debye (delta_eps, tau) or
+    Lorentz (omega_0, delta, kappa)).  Returns (poles, masks)."""
     poles, masks = [], []
     for lo, hi, name in geometry:
         for pole in debye.get(name, []):
-            pole = (float(pole[0]), float(pole[1]))
+            pole = tuple(float(v) for v in pole)
             if pole not in poles:
                 poles.append(pole)
                 masks.append(np.zeros(shape, bool))
@@ -481,9 +501,12 @@ class StepConfig:
     flux: list = field(default_factory=list)
     # near-to-far-field boxes: (lo (i, j, k), hi (i, j, k), freqs f32)
     ntff: list = field(default_factory=list)
-    # Debye poles [(delta_eps, tau)] and their per-cell masks
+    # Debye poles [(delta_eps, tau)], Lorentz poles [(omega_0, delta, kappa)]
+    # and their per-cell masks
     debye_poles: list = field(default_factory=list)
     debye_masks: list = field(default_factory=list)
+    lorentz_poles: list = field(default_factory=list)
+    lorentz_masks: list = field(default_factory=list)
 
 
 class Stepper:
@@ -506,15 +529,34 @@ class Stepper:
         self.ce_cpml = dt / (cfg.eps_r.astype(dtype) * dtype(EPS_0))
         self.ch_cpml = dt / (cfg.mu_r.astype(dtype) * dtype(MU_0))
         self.f = {c: np.zeros(cfg.shape, dtype) for c in COMPS}
-        self.debye = None
+        self.debye = self.lorentz = None
+        z3 = lambda: {c: np.zeros(cfg.shape, dtype) for c in ("ex", "ey", "ez")}  # noqa: E731
         if cfg.debye_poles:
-            # with any pole every cell takes the Debye ca, cb (rfx/simulation.py:406-412)
+            # with any pole every cell takes the dispersive ca, cb
+            # (rfx/simulation.py:361-500)
             ca, cb, alpha, beta, cc = debye_coefficients(
                 cfg.debye_poles, cfg.debye_masks, cfg.eps_r, cfg.sigma, cfg.dt, dtype)
             self.ca, self.cb = ca, cb
-            pol = [{c: np.zeros(cfg.shape, dtype) for c in ("ex", "ey", "ez")}
-                   for _ in cfg.debye_poles]
-            self.debye = (alpha, beta, cc, pol)
+            self.debye = [alpha, beta, cc, [z3() for _ in cfg.debye_poles]]
+        if cfg.lorentz_poles:
+            ca_l, cb_l, a_l, b_l, c_l, cc_l = lorentz_coefficients(
+                cfg.lorentz_poles, cfg.lorentz_masks, cfg.eps_r, cfg.sigma, cfg.dt, dtype)
+            if self.debye is None:
+                self.ca, self.cb = ca_l, cb_l
+            else:
+                # rfx/simulation.py:470-478
+                alpha, beta, _, pol = self.debye
+                beta_sum = np.zeros_like(ca_l)
+                for b in beta:
+                    beta_sum = beta_sum + b
+                g_base = dtype(1.0) / cc_l
+                g_tot = np.maximum(g_base + beta_sum, dtype(EPS_0 * 1e-10))
+                self.ca = (ca_l * g_base - beta_sum) / g_tot
+                self.cb = dtype(cfg.dt) / g_tot
+                self.debye[2] = [(dtype(1.0) - a) / g_tot for a in alpha]
+                cc_l = dtype(1.0) / g_tot
+            self.lorentz = (a_l, b_l, c_l, cc_l, [z3() for _ in cfg.lorentz_poles],
+                            [z3() for _ in cfg.lorentz_poles])
         self.psi = {}
         self.step_count = 0
         if cfg.pec_mask is not None:
@@ -556,23 +598,37 @@ class Stepper:
         cx = self._dbwd(f["hz"], 1) - self._dbwd(f["hy"], 2)
         cy = self._dbwd(f["hx"], 2) - self._dbwd(f["hz"], 0)
         cz = self._dbwd(f["hy"], 0) - self._dbwd(f["hx"], 1)
-        if self.debye is None:
+        if self.debye is None and self.lorentz is None:
             f["ex"] = self.ca * f["ex"] + self.cb * cx
             f["ey"] = self.ca * f["ey"] + self.cb * cy
             f["ez"] = self.ca * f["ez"] + self.cb * cz
             return
-        # rfx/materials/debye.py:182-250: E^{n+1} = ca E + cb curl + sum cc P^n,
-        # P^{n+1} = alpha P^n + beta (E^{n+1} + E^n)
-        alpha, beta, cc, pol = self.debye
+        # rfx/materials/lorentz.py:200-275, rfx/simulation.py:423-500,
+        # rfx/materials/debye.py:182-250: Lorentz P first (explicit in E^n),
+        # E^{n+1} = ca E + cb curl + sum cc_d P_d^n - cc_l sum (P_l^{n+1} - P_l^n),
+        # then P_d^{n+1} = alpha P_d^n + beta (E^{n+1} + E^n)
         for comp, curl in (("ex", cx), ("ey", cy), ("ez", cz)):
             old = f[comp]
             new = self.ca * old + self.cb * curl
-            acc = np.zeros_like(new)
-            for q in range(len(pol)):
-                acc = acc + cc[q] * pol[q][comp]
-            new = new + acc
-            for q in range(len(pol)):
-                pol[q][comp] = alpha[q] * pol[q][comp] + beta[q] * (new + old)
+            if self.lorentz is not None:
+                a_l, b_l, c_l, cc_l, pl, pl_prev = self.lorentz
+                dp = np.zeros_like(new)
+                for q in range(len(pl)):
+                    pn = a_l[q] * pl[q][comp] + b_l[q] * pl_prev[q][comp] + c_l[q] * old
+                    dp = dp + (pn - pl[q][comp])
+                    pl_prev[q][comp] = pl[q][comp]
+                    pl[q][comp] = pn
+            if self.debye is not None:
+                alpha, beta, cc, pol = self.debye
+                acc = np.zeros_like(new)
+                for q in range(len(pol)):
+                    acc = acc + cc[q] * pol[q][comp]
+                new = new + acc
+            if self.lorentz is not None:
+                new = new - cc_l * dp
+            if self.debye is not None:
+                for q in range(len(pol)):
+                    pol[q][comp] = alpha[q] * pol[q][comp] + beta[q] * (new + old)
             f[comp] = new
 
     def _slab(self, ax, side):
@@ -1055,6 +1111,9 @@ def build_uniform(spec, n_steps):
     if spec.get("debye"):
         cfg.debye_poles, cfg.debye_masks = debye_pole_masks(
             shape, coords, spec.get("geometry", []), spec["debye"])
+    if spec.get("lorentz"):
+        cfg.lorentz_poles, cfg.lorentz_masks = debye_pole_masks(
+            shape, coords, spec.get("geometry", []), spec["lorentz"])
     return g, cfg
 
 
@@ -1231,6 +1290,9 @@ def build_graded(spec, n_steps):
     if spec.get("debye"):
         cfg.debye_poles, cfg.debye_masks = debye_pole_masks(
             shape, coords, spec.get("geometry", []), spec["debye"])
+    if spec.get("lorentz"):
+        cfg.lorentz_poles, cfg.lorentz_masks = debye_pole_masks(
+            shape, coords, spec.get("geometry", []), spec["lorentz"])
     return g, cfg
 
 

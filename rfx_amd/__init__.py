@@ -10,10 +10,11 @@ from .api import (DFTPlaneProbe, FDTDState, FluxMonitor, ForwardResult, Result, 
                   extract_lumped_s11, flux_spectrum)
 from .geometry import Box
 from .grid import Grid
-from .materials import DebyePole
+from .materials import DebyePole, LorentzPole, drude_pole, lorentz_pole
 from .sources import GaussianPulse, LumpedPort
 
 __all__ = ["Simulation", "Result", "ForwardResult", "FDTDState", "DFTPlaneProbe", "FluxMonitor",
            "flux_spectrum", "Box",
-           "GaussianPulse", "LumpedPort", "Grid", "extract_lumped_s11", "DebyePole"]
+           "GaussianPulse", "LumpedPort", "Grid", "extract_lumped_s11", "DebyePole",
+           "LorentzPole", "lorentz_pole", "drude_pole"]
 __version__ = "0.1.0"

@@ -79,7 +79,7 @@ class Desc(C.Structure):
         ("plane_t_offset", C.c_int32),
         ("n_flux", C.c_int32), ("flux", C.POINTER(Flux)),
         ("n_ntff", C.c_int32), ("ntff", C.POINTER(Ntff)),
-        ("n_pole", C.c_int32), ("pole_coef", _FP), ("pol_lo", C.c_int32 * 3),
+        ("n_pole", C.c_int32), ("n_debye", C.c_int32), ("pole_coef", _FP), ("pol_lo", C.c_int32 * 3),
         ("pol_n", C.c_int32 * 3), ("d_pol", _P),
     ]
 

@@ -22,7 +22,8 @@ from .cpml import face_tables
 from .engine import DeviceRun, Problem
 from .geometry import Box
 from .grid import Grid
-from .materials import MAX_POLES, POLE_SHIFT, DebyePole, MaterialMap, tangential_edge_bits
+from .materials import (MAX_POLES, POLE_SHIFT, DebyePole, LorentzPole, MaterialMap,
+                        tangential_edge_bits)
 from .sources import GaussianPulse, port_sigma, sample_times
 
 F32 = np.float32
@@ -35,6 +36,7 @@ class _Material:
     sigma: float = 0.0
     mu_r: float = 1.0
     debye: tuple = ()          # DebyePole entries
+    lorentz: tuple = ()        # LorentzPole entries
 
 
 @dataclass
@@ -247,17 +249,20 @@ class Simulation:
     def add_material(self, name: str, *, eps_r: float = 1.0, sigma: float = 0.0,
                      mu_r: float = 1.0, debye_poles=None, lorentz_poles=None,
                      chi3: float = 0.0):
-        """Material with optional Debye poles (rfx/api/__init__.py:812-853):
-        ``debye_poles`` is a list of DebyePole(delta_eps, tau) (or pairs);
-        eps_r is then eps_inf.  Lorentz poles and Kerr chi3 are not
+        """Material with optional dispersion (rfx/api/__init__.py:812-853):
+        ``debye_poles`` a list of DebyePole(delta_eps, tau), ``lorentz_poles``
+        a list of LorentzPole(omega_0, delta, kappa) (see ``lorentz_pole`` /
+        ``drude_pole``); eps_r is then eps_inf.  Kerr chi3 is not
         supported."""
-        if lorentz_poles or chi3:
-            raise NotImplementedError("Lorentz poles and Kerr nonlinearity are not supported")
+        if chi3:
+            raise NotImplementedError("Kerr nonlinearity is not supported")
         poles = tuple(DebyePole(float(p[0]), float(p[1])) for p in (debye_poles or ()))
         for p in poles:
             if not p.tau > 0:
                 raise ValueError(f"Debye pole {p}: tau must be positive")
-        self._materials[name] = _Material(float(eps_r), float(sigma), float(mu_r), poles)
+        lor = tuple(LorentzPole(float(p[0]), float(p[1]), float(p[2]))
+                    for p in (lorentz_poles or ()))
+        self._materials[name] = _Material(float(eps_r), float(sigma), float(mu_r), poles, lor)
         return self
 
     def add(self, shape, *, material: str):
@@ -450,19 +455,24 @@ class Simulation:
         # its value in order of first appearance, OR-ed over every shape of a
         # material listing it (a later shape does not remove it), not extended
         # into the CPML pads; carried as flag bits 16 + q of the material ids
-        poles = []
-        pole_bits = None
+        # (Debye poles take bits 16.., Lorentz poles the bits after them)
+        poles, lorentz = [], []
         for box, name in self._geometry:
-            for pole in self._materials[name].debye:
-                if pole not in poles:
-                    poles.append(pole)
-                    if len(poles) > MAX_POLES:
-                        raise NotImplementedError(f"more than {MAX_POLES} distinct Debye poles")
-                if pole_bits is None:
-                    pole_bits = np.zeros(shape, np.uint32)
-                sel = np.ix_(*[np.nonzero(x)[0] for x in box.axis_masks(coords)])
-                pole_bits[sel] |= np.uint32(1 << (POLE_SHIFT + poles.index(pole)))
-        if pole_bits is not None:
+            m = self._materials[name]
+            poles += [p for p in m.debye if p not in poles]
+            lorentz += [p for p in m.lorentz if p not in lorentz]
+        if len(poles) + len(lorentz) > MAX_POLES:
+            raise NotImplementedError(f"more than {MAX_POLES} distinct dispersion poles")
+        if poles or lorentz:
+            pole_bits = np.zeros(shape, np.uint32)
+            for box, name in self._geometry:
+                m = self._materials[name]
+                bits = sum(1 << (POLE_SHIFT + poles.index(p)) for p in set(m.debye))
+                bits |= sum(1 << (POLE_SHIFT + len(poles) + lorentz.index(p))
+                            for p in set(m.lorentz))
+                if bits:
+                    sel = np.ix_(*[np.nonzero(x)[0] for x in box.axis_masks(coords)])
+                    pole_bits[sel] |= np.uint32(bits)
             mat.add_flags(pole_bits)
         # graded lane: sources are normalised by the materials before any
         # override or port fold (rfx/runners/nonuniform.py:630-643)
@@ -540,7 +550,8 @@ class Simulation:
                        faces=faces, pec_axes=pec_axes, sources=sources, probes=probes,
                        ports=[p[:4] for p in ports], planes=planes, fluxes=fluxes,
                        record_order="graded" if graded else "uniform", ntff=ntff,
-                       poles=[(p.delta_eps, p.tau) for p in poles])
+                       poles=[(p.delta_eps, p.tau) for p in poles],
+                       lorentz=[(p.omega_0, p.delta, p.kappa) for p in lorentz])
         prob.flux_areas = areas
         prob.ntff_box = box
         return grid, prob, ports
@@ -683,8 +694,8 @@ class Simulation:
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),
                                                 s11)
-        if needs_grad and prob.poles:
-            raise NotImplementedError("gradients through Debye materials are not supported "
+        if needs_grad and (prob.poles or prob.lorentz):
+            raise NotImplementedError("gradients through dispersive materials are not supported "
                                       "(the reverse mode covers non-dispersive problems)")
         if needs_grad:
             from .adjoint import forward_with_grad

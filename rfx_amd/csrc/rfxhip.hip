@@ -112,11 +112,13 @@ struct PsiIO {
   float* hw[6][2];
   const float* er[6][2];
   float* ew[6][2];
-  // Debye polarisation (in place, shared by both field sets): P[pole][comp]
-  // over the box pol_lo + [0, pol_n); per material and pole alpha, beta, cc
+  // dispersive polarisation (in place, shared by both field sets) over the
+  // box pol_lo + [0, pol_n): poles [0, n_debye) are Debye (P[3] each),
+  // the rest Lorentz (P[3] then P^{n-1}[3]); per material and pole a row of
+  // 4 floats, Debye (alpha, beta, cc, 0), Lorentz (a, b, c, cc)
   float* pol;
   const float* pcoef;
-  int n_pole;
+  int n_pole, n_debye;
   int pol_lo[3], pol_n[3];
 };
 
@@ -437,14 +439,15 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
 // CPML-E (cpml.py:608-777), PEC faces and mask, point operations.  H^{n+1/2}
 // at the cell and its backward neighbours (zero before the first cell) and
 // E^n at the cell come in as values; returns E^{n+1}.
-// Debye ADE at one cell (rfx/materials/debye.py:182-250): with E the plain
-// update ca E^n + cb curl H,
-//   E^{n+1} = E + sum_p cc_p P_p^n,
-//   P_p^{n+1} = alpha_p P_p^n + beta_p (E^{n+1} + E^n),
-// before the CPML correction, PEC and point operations (the reference
-// applies those to the Debye-updated E).  ca, cb of the material table are
-// the Debye ones.  A cell outside the state box is left alone (the host
-// puts every cell with poles inside it).
+// Dispersive ADE at one cell, with E the plain update ca E^n + cb curl H
+// (ca, cb of the material table are the dispersive ones):
+//   Lorentz poles first, explicit in E^n (rfx/materials/lorentz.py:200-275):
+//     P_l^{n+1} = a P_l^n + b P_l^{n-1} + c E^n,  dP = sum_l (P_l^{n+1} - P_l^n)
+//   E^{n+1} = E + sum_d cc_d P_d^n - cc_l dP    (rfx/simulation.py:423-500)
+//   Debye poles (rfx/materials/debye.py:182-250):
+//     P_d^{n+1} = alpha P_d^n + beta (E^{n+1} + E^n)
+// before the CPML correction, PEC and point operations.  A cell outside the
+// state box is left alone (the host puts every cell with poles inside it).
 __device__ __noinline__ void debye_cell(const KParams& p, const PsiIO& ps, int i, int j, int k,
                                         unsigned c, float o0, float o1, float o2, float& e0,
                                         float& e1, float& e2) {
@@ -456,20 +459,47 @@ __device__ __noinline__ void debye_cell(const KParams& p, const PsiIO& ps, int i
   const unsigned r = ((unsigned)li * (unsigned)ps.pol_n[1] + (unsigned)lj) * (unsigned)ps.pol_n[2] +
                      (unsigned)lk;
   const unsigned m = ldu16(p.mat, c);
-  const float* pc = ps.pcoef + (size_t)m * ps.n_pole * 3;
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-  for (int q = 0; q < ps.n_pole; ++q) {
-    const float cc = pc[3 * q + 2];
-    const float* P = ps.pol + (size_t)q * 3 * R;
-    s0 += cc * P[r];
-    s1 += cc * P[R + r];
-    s2 += cc * P[2 * R + r];
+  const float* pc = ps.pcoef + (size_t)m * ps.n_pole * 4;
+  const int nd = ps.n_debye;
+  float d0 = 0.f, d1 = 0.f, d2 = 0.f, ccl = 0.f;
+  for (int q = nd; q < ps.n_pole; ++q) {
+    const float a = pc[4 * q], b = pc[4 * q + 1], cl = pc[4 * q + 2];
+    ccl = pc[4 * q + 3];
+    float* P = ps.pol + (size_t)nd * 3 * R + (size_t)(q - nd) * 6 * R;
+    const float p0 = P[r], p1 = P[R + r], p2 = P[2 * R + r];
+    const float n0 = a * p0 + b * P[3 * R + r] + cl * o0;
+    const float n1 = a * p1 + b * P[4 * R + r] + cl * o1;
+    const float n2 = a * p2 + b * P[5 * R + r] + cl * o2;
+    d0 += n0 - p0;
+    d1 += n1 - p1;
+    d2 += n2 - p2;
+    P[3 * R + r] = p0;
+    P[4 * R + r] = p1;
+    P[5 * R + r] = p2;
+    P[r] = n0;
+    P[R + r] = n1;
+    P[2 * R + r] = n2;
   }
-  e0 += s0;
-  e1 += s1;
-  e2 += s2;
-  for (int q = 0; q < ps.n_pole; ++q) {
-    const float a = pc[3 * q], b = pc[3 * q + 1];
+  if (nd > 0) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int q = 0; q < nd; ++q) {
+      const float cc = pc[4 * q + 2];
+      const float* P = ps.pol + (size_t)q * 3 * R;
+      s0 += cc * P[r];
+      s1 += cc * P[R + r];
+      s2 += cc * P[2 * R + r];
+    }
+    e0 += s0;
+    e1 += s1;
+    e2 += s2;
+  }
+  if (nd < ps.n_pole) {
+    e0 -= ccl * d0;
+    e1 -= ccl * d1;
+    e2 -= ccl * d2;
+  }
+  for (int q = 0; q < nd; ++q) {
+    const float a = pc[4 * q], b = pc[4 * q + 1];
     float* P = ps.pol + (size_t)q * 3 * R;
     P[r] = a * P[r] + b * (e0 + o0);
     P[R + r] = a * P[R + r] + b * (e1 + o1);
@@ -1933,7 +1963,7 @@ struct rfxh_plan {
   // Debye polarisation state (caller-owned) and per-material pole coefficients
   float* pol = nullptr;
   const float* pcoef = nullptr;
-  int n_pole = 0;
+  int n_pole = 0, n_debye = 0;
   int pol_lo[3] = {0, 0, 0}, pol_n[3] = {0, 0, 0};
   dim3 grid, block;
   dim3 grid_col;                // column kernels (ncol cells per thread)
@@ -2111,6 +2141,7 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   ps.pol = pl->pol;
   ps.pcoef = pl->pcoef;
   ps.n_pole = pl->n_pole;
+  ps.n_debye = pl->n_debye;
   for (int a = 0; a < 3; ++a) {
     ps.pol_lo[a] = pl->pol_lo[a];
     ps.pol_n[a] = pl->pol_n[a];
@@ -2395,21 +2426,25 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   // Debye poles: cells whose material flags carry pole bits update their
   // polarisation in the box pol_lo + [0, pol_n)
   if (d->n_pole < 0 || d->n_pole > 16) return bail(fail(RFXH_EINVAL, "n_pole must be 0..16"));
+  if (d->n_debye < 0 || d->n_debye > d->n_pole)
+    return bail(fail(RFXH_EINVAL, "n_debye must be 0..n_pole"));
   if (d->n_pole > 0) {
-    if (!d->pole_coef || !d->d_pol) return bail(fail(RFXH_EINVAL, "Debye tables missing"));
+    if (!d->pole_coef || !d->d_pol) return bail(fail(RFXH_EINVAL, "dispersion tables missing"));
     for (int a = 0; a < 3; ++a)
       if (d->pol_lo[a] < 0 || d->pol_n[a] < 1 || d->pol_lo[a] + d->pol_n[a] > N[a])
         return bail(fail(RFXH_EINVAL, "Debye state box outside the grid"));
     for (int m = 0; m < d->n_mat; ++m)
       if ((d->mat_flags[m] & RFXH_POLES) >> 16 >= (1u << d->n_pole))
         return bail(fail(RFXH_EINVAL, "material flags name a pole past n_pole"));
-    if ((long long)d->pol_n[0] * d->pol_n[1] * d->pol_n[2] * 3 * d->n_pole >= (1LL << 32))
-      return bail(fail(RFXH_EINVAL, "Debye state larger than 2^32 floats"));
+    if ((long long)d->pol_n[0] * d->pol_n[1] * d->pol_n[2] * (6LL * d->n_pole - 3LL * d->n_debye) >=
+        (1LL << 32))
+      return bail(fail(RFXH_EINVAL, "dispersion state larger than 2^32 floats"));
     float* pc;
-    if ((rc = dev_copy(pl, d->pole_coef, (size_t)d->n_mat * d->n_pole * 3, &pc))) return bail(rc);
+    if ((rc = dev_copy(pl, d->pole_coef, (size_t)d->n_mat * d->n_pole * 4, &pc))) return bail(rc);
     pl->pcoef = pc;
     pl->pol = d->d_pol;
     pl->n_pole = d->n_pole;
+    pl->n_debye = d->n_debye;
     for (int a = 0; a < 3; ++a) {
       pl->pol_lo[a] = d->pol_lo[a];
       pl->pol_n[a] = d->pol_n[a];
@@ -2417,7 +2452,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   } else {
     for (int m = 0; m < d->n_mat; ++m)
       if (d->mat_flags[m] & RFXH_POLES)
-        return bail(fail(RFXH_EINVAL, "material flags name Debye poles but n_pole is 0"));
+        return bail(fail(RFXH_EINVAL, "material flags name poles but n_pole is 0"));
   }
   // CPML set 1 is addressed as set-0 pointer + psi_set floats
   pl->psi_set = (has2 && any_cpml) ? (long long)(d->d_psi2 - d->d_psi) : 0;
@@ -2741,6 +2776,7 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
   ps.pol = pl->pol;
   ps.pcoef = pl->pcoef;
   ps.n_pole = pl->n_pole;
+  ps.n_debye = pl->n_debye;
   for (int a = 0; a < 3; ++a) {
     ps.pol_lo[a] = pl->pol_lo[a];
     ps.pol_n[a] = pl->pol_n[a];

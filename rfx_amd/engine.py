@@ -39,8 +39,10 @@ class Problem:
     record_order: str = "uniform"                 # "uniform" | "graded"
     # near-to-far-field boxes: (lo (i, j, k), hi (i, j, k), freqs f64)
     ntff: list = field(default_factory=list)
-    # Debye poles (delta_eps, tau); pole q acts where the material flags carry bit 16 + q
+    # dispersion poles: Debye (delta_eps, tau) as poles 0.., then Lorentz
+    # (omega_0, delta, kappa); pole q acts where the material flags carry bit 16 + q
     poles: list = field(default_factory=list)
+    lorentz: list = field(default_factory=list)
 
     @property
     def plane_t_offset(self):
@@ -82,9 +84,10 @@ class DeviceRun:
         mat = prob.mat
         for op in ops:
             mat.set_cell(tuple(op[6]), add_flags=_lib.RFXH_POINT)
-        if prob.poles:
-            from .materials import POLE_SHIFT, debye_tables
-            self.coef, self.flags, self.pole_rows = debye_tables(mat, prob.dt, prob.poles)
+        if prob.poles or prob.lorentz:
+            from .materials import POLE_SHIFT, dispersion_tables
+            self.coef, self.flags, self.pole_rows = dispersion_tables(mat, prob.dt, prob.poles,
+                                                                      prob.lorentz)
             # the box holding every cell with poles carries the polarisation
             disp = np.nonzero((self.flags >> POLE_SHIFT) != 0)[0]
             cells = np.isin(mat.ids, disp.astype(np.uint16))
@@ -126,9 +129,10 @@ class DeviceRun:
             self.src_wave = None
         self.probe_out = torch.zeros((T, max(n_probe, 1)), dtype=torch.float32, device=dev)
         self.pol = None
-        if prob.poles:
+        if prob.poles or prob.lorentz:
             nbox = int(np.prod(self.pol_box[1]))
-            self.pol = torch.zeros(len(prob.poles) * 3 * nbox, dtype=torch.float32, device=dev)
+            n_state = 3 * len(prob.poles) + 6 * len(prob.lorentz)
+            self.pol = torch.zeros(n_state * nbox, dtype=torch.float32, device=dev)
         self.port_out = torch.zeros((T, max(n_port, 1), 2), dtype=torch.float32, device=dev)
         self.plane_acc = []
         planes_c = (_lib.Plane * max(len(prob.planes), 1))()
@@ -235,8 +239,9 @@ class DeviceRun:
         d.flux = flux_c
         d.n_ntff = len(prob.ntff)
         d.ntff = ntff_c
-        if prob.poles:
-            d.n_pole = len(prob.poles)
+        if prob.poles or prob.lorentz:
+            d.n_pole = len(prob.poles) + len(prob.lorentz)
+            d.n_debye = len(prob.poles)
             d.pole_coef = _f32p(self.pole_rows)
             for a in range(3):
                 d.pol_lo[a] = self.pol_box[0][a]

@@ -31,6 +31,26 @@ class DebyePole(NamedTuple):
     tau: float
 
 
+class LorentzPole(NamedTuple):
+    """One Lorentz oscillator kappa / (w0^2 - w^2 + 2 j delta w); Drude with
+    omega_0 = 0 (rfx/materials/lorentz.py:31-43)."""
+    omega_0: float
+    delta: float
+    kappa: float
+
+
+def drude_pole(omega_p: float, gamma: float) -> LorentzPole:
+    """Drude term from the plasma frequency and collision rate (rad/s)
+    (rfx/materials/lorentz.py:46-56)."""
+    return LorentzPole(omega_0=0.0, delta=gamma / 2.0, kappa=omega_p ** 2)
+
+
+def lorentz_pole(delta_eps: float, omega_0: float, delta: float) -> LorentzPole:
+    """Lorentz term from d_eps, resonance and damping (rad/s)
+    (rfx/materials/lorentz.py:59-71)."""
+    return LorentzPole(omega_0=omega_0, delta=delta, kappa=delta_eps * omega_0 ** 2)
+
+
 class MaterialMap:
     def __init__(self, shape):
         self.shape = tuple(shape)
@@ -177,37 +197,72 @@ class MaterialMap:
         return np.ascontiguousarray(coef), np.ascontiguousarray(flg)
 
 
-def debye_tables(mat: MaterialMap, dt: float, poles):
-    """Coefficient table and per-pole rows when Debye poles exist
-    (rfx/materials/debye.py:76-179, init_debye, in its float32 operation
-    order): per pole alpha = (2 tau - dt)/(2 tau + dt) and beta = eps0 d_eps
-    dt/(2 tau + dt) where the material's flags carry the pole, else 0;
-    gamma = eps_inf + sum beta + sigma dt/2 (floored at eps0 1e-10),
-    ca = (eps_inf - sum beta - sigma dt/2)/gamma, cb = dt/gamma for every
-    material, cc = (1 - alpha)/gamma.  Returns (coef [n_mat][4] with the
-    Debye ca, cb and the usual ce, ch; flags; pole rows [n_mat][n_pole][3] =
-    alpha, beta, cc)."""
+def dispersion_tables(mat: MaterialMap, dt: float, debye_poles, lorentz_poles=()):
+    """Coefficient table and per-pole rows when dispersion poles exist, in
+    the reference's float32 operation order.  Pole q (Debye first, then
+    Lorentz) acts where the material's flags carry bit 16 + q.
+
+    Debye (rfx/materials/debye.py:76-179): alpha = (2 tau - dt)/(2 tau + dt),
+    beta = eps0 d_eps dt/(2 tau + dt); gamma = eps_inf + sum beta +
+    sigma dt/2, ca = (eps_inf - sum beta - sigma dt/2)/gamma, cb = dt/gamma,
+    cc = (1 - alpha)/gamma.  Lorentz (rfx/materials/lorentz.py:103-198):
+    a = (2 - w0^2 dt^2)/(1 + delta dt), b = -(1 - delta dt)/(1 + delta dt),
+    c = eps0 kappa dt^2/(1 + delta dt); gamma = eps_inf + sigma dt/2,
+    ca = (eps_inf - sigma dt/2)/gamma, cb = dt/gamma, cc = 1/gamma.  Both
+    (rfx/simulation.py:470-478): gamma_t = 1/cc_L + sum beta, ca = (ca_L/cc_L -
+    sum beta)/gamma_t, cb = dt/gamma_t, cc_D = (1 - alpha)/gamma_t,
+    cc_L = 1/gamma_t.  Gammas are floored at eps0 1e-10.
+
+    Returns (coef [n_mat][4] with the dispersive ca, cb and the usual ce,
+    ch; flags; rows [n_mat][n_pole][4] = Debye (alpha, beta, cc, 0),
+    Lorentz (a, b, c, cc))."""
     coef, flg = mat.coefficient_table(dt)
     eps_r, sig, _, _ = mat.props()
-    n_mat, n_pole = len(flg), len(poles)
+    n_mat, nd, nl = len(flg), len(debye_poles), len(lorentz_poles)
     dt32 = F32(dt)
-    alpha = np.zeros((n_pole, n_mat), F32)
-    beta = np.zeros((n_pole, n_mat), F32)
-    for q, (de, tau) in enumerate(poles):
-        on = ((flg >> (POLE_SHIFT + q)) & 1).astype(bool)
-        alpha[q] = np.where(on, F32((2.0 * tau - dt) / (2.0 * tau + dt)), F32(0.0))
-        beta[q] = np.where(on, F32(EPS_0 * de * dt / (2.0 * tau + dt)), F32(0.0))
+    floor = F32(EPS_0 * 1e-10)
+
+    def on(q):
+        return ((flg >> (POLE_SHIFT + q)) & 1).astype(bool)
+
     eps_inf = eps_r * F32(EPS_0)
-    beta_sum = np.zeros(n_mat, F32)
-    for q in range(n_pole):
-        beta_sum = beta_sum + beta[q]
     half_loss = sig * dt32 / F32(2.0)
-    gamma = np.maximum(eps_inf + beta_sum + half_loss, F32(EPS_0 * 1e-10))
+    rows = np.zeros((n_mat, nd + nl, 4), F32)
+    alpha, beta = [], []
+    for q, (de, tau) in enumerate(debye_poles):
+        alpha.append(np.where(on(q), F32((2.0 * tau - dt) / (2.0 * tau + dt)), F32(0.0)))
+        beta.append(np.where(on(q), F32(EPS_0 * de * dt / (2.0 * tau + dt)), F32(0.0)))
+    beta_sum = np.zeros(n_mat, F32)
+    for b in beta:
+        beta_sum = beta_sum + b
+    for l, (w0, delta, kappa) in enumerate(lorentz_poles):
+        denom = 1.0 + delta * dt
+        m = on(nd + l)
+        rows[:, nd + l, 0] = np.where(m, F32((2.0 - w0 ** 2 * dt ** 2) / denom), F32(0.0))
+        rows[:, nd + l, 1] = np.where(m, F32(-(1.0 - delta * dt) / denom), F32(0.0))
+        rows[:, nd + l, 2] = np.where(m, F32(EPS_0 * kappa * dt ** 2 / denom), F32(0.0))
     coef = coef.copy()
-    coef[:, 0] = (eps_inf - beta_sum - half_loss) / gamma
-    coef[:, 1] = dt32 / gamma
-    rows = np.zeros((n_mat, n_pole, 3), F32)
-    for q in range(n_pole):
+    if nl == 0:
+        gamma = np.maximum(eps_inf + beta_sum + half_loss, floor)
+        coef[:, 0] = (eps_inf - beta_sum - half_loss) / gamma
+        coef[:, 1] = dt32 / gamma
+    else:
+        g_l = np.maximum(eps_inf + half_loss, floor)
+        ca_l = (eps_inf - half_loss) / g_l
+        cc_l = F32(1.0) / g_l
+        if nd == 0:
+            gamma = g_l
+            coef[:, 0] = ca_l
+            coef[:, 1] = dt32 / g_l
+        else:
+            g_base = F32(1.0) / cc_l
+            gamma = np.maximum(g_base + beta_sum, floor)
+            coef[:, 0] = (ca_l * g_base - beta_sum) / gamma
+            coef[:, 1] = dt32 / gamma
+            cc_l = F32(1.0) / gamma
+        for l in range(nl):
+            rows[:, nd + l, 3] = cc_l
+    for q in range(nd):
         rows[:, q, 0] = alpha[q]
         rows[:, q, 1] = beta[q]
         rows[:, q, 2] = (F32(1.0) - alpha[q]) / gamma

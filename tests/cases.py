@@ -95,7 +95,9 @@ def to_sim(spec, device=None):
                        cpml_kappa_max=spec.get("kappa_max", 1.0), **kw)
     for name, (e, s, m) in spec.get("materials", {}).items():
         poles = [R.DebyePole(de, tau) for de, tau in spec.get("debye", {}).get(name, [])]
-        sim.add_material(name, eps_r=e, sigma=s, mu_r=m, debye_poles=poles or None)
+        lor = [R.LorentzPole(*p) for p in spec.get("lorentz", {}).get(name, [])]
+        sim.add_material(name, eps_r=e, sigma=s, mu_r=m, debye_poles=poles or None,
+                         lorentz_poles=lor or None)
     for lo, hi, name in spec.get("geometry", []):
         sim.add(R.Box(lo, hi), material=name)
     for entry in spec.get("ports", []):
@@ -265,6 +267,24 @@ def debye_case(tau, delta_eps=3.0, eps_inf=2.0, lane="run", overlap=True):
                 probes=[((0.013, 0.006, 0.005), "ez"), ((0.007, 0.006, 0.004), "ey"),
                         ((0.010, 0.005, 0.004), "hx")],
                 lane=lane)
+
+
+def lorentz_cavity_case(n_steps=6000):
+    """PEC cavity 20 x 16 x 3 mm (dx 1 mm) filled with a lossless Lorentz
+    medium (eps_inf 2, d_eps 1.5, f0 6 GHz, no damping), an Ez pulse off
+    centre and an Ez probe; returns (spec, n_steps, the two TM110 polariton
+    frequencies of the discrete dispersion relation)."""
+    from tests.test_oracle_lorentz import polariton_freqs
+    a, b, c, dx = 0.020, 0.016, 0.003, 1e-3
+    w0 = 2 * np.pi * 6e9
+    kappa = 1.5 * w0 ** 2
+    spec = dict(freq_max=20e9, domain=(a, b, c), dx=dx, boundary="pec",
+                materials={"lor": (2.0, 0.0, 1.0)}, lorentz={"lor": [(w0, 0.0, kappa)]},
+                geometry=[((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), "lor")],
+                ports=[((0.006, 0.005, 0.001), "ez", 0.0, o.Pulse(7e9, 1.0), True)],
+                probes=[((0.013, 0.011, 0.001), "ez")], lane="run")
+    g = o.UniformGrid(spec["freq_max"], spec["domain"], dx, 0)
+    return spec, n_steps, polariton_freqs(2.0, w0, kappa, a, b, dx, g.dt)
 
 
 def box_total(fluxes):

@@ -115,3 +115,34 @@ def test_debye_gradient_refused(R):
     eps = torch.ones(g.shape, dtype=torch.float32, requires_grad=True)
     with pytest.raises(NotImplementedError):
         sim.forward(eps_override=eps, n_steps=4)
+
+
+def test_lorentz_cavity_polaritons_gpu(R):
+    _say("Lorentz-filled PEC cavity 21x17x4 x 6000")
+    from tests.spectral import matrix_pencil_freqs
+    spec, n, pred = cases.lorentz_cavity_case()
+    res = cases.to_sim(spec).run(n_steps=n)
+    g, cfg = o.build_uniform(spec, 1000)
+    ref = o.run_loop(cfg, 1000)
+    e = cases.rel_l2(res.time_series[:1000, 0], ref.time_series[:, 0])
+    print("series (1000 steps)", e)
+    assert e <= TOL
+    f, _ = matrix_pencil_freqs(res.time_series[:, 0].astype(np.float64), res.dt, max_order=60)
+    err = [np.min(np.abs(f - p)) / p for p in pred]
+    print("polaritons", pred, err)
+    assert max(err) < 1e-5
+
+
+def test_debye_and_lorentz_vs_oracle(R):
+    _say("Debye + Lorentz + Drude case x 400")
+    spec = cases.debye_case(20e-12)
+    w0 = 2 * np.pi * 9e9
+    spec["lorentz"] = {"deb": [(w0, 1e9, 1.2 * w0 ** 2)], "d": [(0.0, 5e9, 4e20)]}
+    n = 400
+    res = cases.to_sim(spec).run(n_steps=n)
+    g, cfg = o.build_uniform(spec, n)
+    ref = o.run_loop(cfg, n)
+    for p in range(res.time_series.shape[1]):
+        e = cases.rel_l2(res.time_series[:, p], ref.time_series[:, p])
+        print("probe", p, e)
+        assert e <= TOL
