@@ -28,6 +28,12 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   * graded mesh: a uniform profile must reproduce the uniform lane after the
     documented source-normalisation factor; PEC-cavity resonance closed
     form.
+  * Debye / Lorentz ADE (debye_coefficients, lorentz_coefficients, the
+    dispersive Stepper.update_e): the reference's hand-computed
+    coefficients; tau -> 0 / tau -> inf limits (a plain dielectric of
+    eps_inf + d_eps / eps_inf); the exact polariton frequencies of a
+    Lorentz-filled PEC cavity; kappa = 0 reducing the mixed update to the
+    Debye one.
   * NTFF box (_ntff, far_field): faces equal time-shifted DFT planes;
     single-cell current elements (hand-derived n x H, -n x E); the
     short-dipole directivity the reference's own test records (1.84 dBi,
