@@ -183,3 +183,22 @@ def test_far_field_power_equals_box_flux():
     print("box flux", small, big, "far field", p_ref / ref, "re-timed", p_fix / ref)
     assert np.all(np.abs(p_fix - ref) <= 0.015 * ref)
     assert np.all((p_ref < ref) & (p_ref > 0.85 * ref))
+
+
+def test_graded_positions_reduce_to_uniform():
+    """A graded z axis of equal cells gives the uniform positions and
+    widths (the cumulative-edge rule of rfx/farfield.py:436-441 against the
+    (k - cpml_lo) dx rule), and a 2:1 step moves the later nodes by the
+    extra width."""
+    shape, dx, p = (12, 10, 9), 1e-3, 3
+    u_pos, u_w = o.ntff_axis_tables(shape, dx, cpml_lo=(p, p, p))
+    g_pos, g_w = o.ntff_axis_tables(shape, dx, dz=np.full(shape[2], dx, np.float32),
+                                    cpml_lo=(p, p, p))
+    for a in range(3):
+        np.testing.assert_allclose(g_pos[a], u_pos[a], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(g_w[a], u_w[a], rtol=1e-7)
+    dz = np.full(shape[2], dx)
+    dz[5] = 2 * dx
+    g_pos, _ = o.ntff_axis_tables(shape, dx, dz=dz, cpml_lo=(p, p, p))
+    np.testing.assert_allclose(g_pos[2][:6], u_pos[2][:6], atol=1e-12)
+    np.testing.assert_allclose(g_pos[2][6:] - u_pos[2][6:], dx, atol=1e-12)
