@@ -1355,53 +1355,54 @@ __global__ void k_set_acc_from(long long* acc_from, const long long* step) {
 // Flux monitor: tangential E at the plane, tangential H averaged over the
 // planes index-1 and index, accumulated with their own twiddles (float64).
 // ---------------------------------------------------------------------------
-// Folds steps lo..hi of the ring (step hi from registers when hi_in_reg)
-// into the four accumulators, in step order per frequency: the additions a
-// per-step accumulation makes, in the same order.
-__device__ __forceinline__ void flux_fold(const FluxParams& q, long long pidx, long long npts,
-                                          long long lo, long long hi, bool hi_in_reg,
-                                          const float* xhi) {
+// Folds steps lo..hi of component t's ring (step hi from a register when
+// hi_in_reg) into its accumulators, in step order per frequency: the
+// additions a per-step accumulation makes, in the same order.  One thread
+// per (point, component) (blockIdx.y = component): four times the threads
+// of one per point, so the accumulator round trips overlap more.
+__device__ __forceinline__ void flux_fold(const FluxParams& q, int t, long long pidx,
+                                          long long npts, long long lo, long long hi,
+                                          bool hi_in_reg, float xhi) {
   const int cnt = (int)(hi - lo + 1);
-  for (int t = 0; t < 4; ++t) {
-    float xs[kDftK];
+  float xs[kDftK];
 #pragma unroll
-    for (int m = 0; m < kDftK; ++m) {
-      if (m < cnt) {
-        const long long st = lo + m;
-        xs[m] = (hi_in_reg && st == hi) ? xhi[t] : q.snap[((st % kDftK) * 4 + t) * npts + pidx];
-      }
+  for (int m = 0; m < kDftK; ++m) {
+    if (m < cnt) {
+      const long long st = lo + m;
+      xs[m] = (hi_in_reg && st == hi) ? xhi : q.snap[((st % kDftK) * 4 + t) * npts + pidx];
     }
-    const double2* tw = t < 2 ? q.twe : q.twh;
-    // kFoldF frequencies at a time: their accumulator loads are in flight
-    // together instead of one round trip per frequency
-    for (int f0 = 0; f0 < q.nf; f0 += kFoldF) {
-      double2 cur[kFoldF];
+  }
+  const double2* tw = t < 2 ? q.twe : q.twh;
+  // kFoldF frequencies at a time: their accumulator loads are in flight
+  // together instead of one round trip per frequency
+  for (int f0 = 0; f0 < q.nf; f0 += kFoldF) {
+    double2 cur[kFoldF];
 #pragma unroll
-      for (int u = 0; u < kFoldF; ++u)
-        if (f0 + u < q.nf) cur[u] = q.acc[((long long)t * q.nf + f0 + u) * npts + pidx];
+    for (int u = 0; u < kFoldF; ++u)
+      if (f0 + u < q.nf) cur[u] = q.acc[((long long)t * q.nf + f0 + u) * npts + pidx];
 #pragma unroll
-      for (int u = 0; u < kFoldF; ++u) {
-        if (f0 + u >= q.nf) break;
+    for (int u = 0; u < kFoldF; ++u) {
+      if (f0 + u >= q.nf) break;
 #pragma unroll
-        for (int m = 0; m < kDftK; ++m) {
-          if (m < cnt) {
-            const double2 w = tw[(lo + m) * q.nf + f0 + u];
-            const double x = (double)xs[m];
-            cur[u].x += w.x * x;
-            cur[u].y += w.y * x;
-          }
+      for (int m = 0; m < kDftK; ++m) {
+        if (m < cnt) {
+          const double2 w = tw[(lo + m) * q.nf + f0 + u];
+          const double x = (double)xs[m];
+          cur[u].x += w.x * x;
+          cur[u].y += w.y * x;
         }
       }
-#pragma unroll
-      for (int u = 0; u < kFoldF; ++u)
-        if (f0 + u < q.nf) q.acc[((long long)t * q.nf + f0 + u) * npts + pidx] = cur[u];
     }
+#pragma unroll
+    for (int u = 0; u < kFoldF; ++u)
+      if (f0 + u < q.nf) q.acc[((long long)t * q.nf + f0 + u) * npts + pidx] = cur[u];
   }
 }
 
 __global__ __launch_bounds__(256) void k_flux(FluxParams q) {
   const long long npts = (long long)q.n1 * q.n2;
   const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = (int)blockIdx.y;   // component: E1, E2, H1, H2
   if (pidx >= npts) return;
   const long long n = *q.step;
   if (n < 0 || n >= q.max_steps) return;
@@ -1414,18 +1415,14 @@ __global__ __launch_bounds__(256) void k_flux(FluxParams q) {
   ii[a2] = im[a2] = v;
   const long long c = ((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2];
   const long long cm = ((long long)im[0] * q.ny + im[1]) * q.nz + im[2];
-  float x[4];
-  x[0] = q.f[0][c];
-  x[1] = q.f[1][c];
-  x[2] = (q.f[2][cm] + q.f[2][c]) * 0.5f;
-  x[3] = (q.f[3][cm] + q.f[3][c]) * 0.5f;
+  const float* f = q.f[t];
+  const float x = t < 2 ? f[c] : (f[cm] + f[c]) * 0.5f;
   const int slot = (int)(n % kDftK);
   if (slot == kDftK - 1) {
     const long long lo = max(*q.acc_from, n - (kDftK - 1));
-    flux_fold(q, pidx, npts, lo, n, true, x);
+    flux_fold(q, t, pidx, npts, lo, n, true, x);
   } else {
-#pragma unroll
-    for (int t = 0; t < 4; ++t) q.snap[((long long)slot * 4 + t) * npts + pidx] = x[t];
+    q.snap[((long long)slot * 4 + t) * npts + pidx] = x;
   }
 }
 
@@ -1440,7 +1437,7 @@ __global__ __launch_bounds__(256) void k_flux_flush(FluxParams q) {
   if (slot == kDftK - 1) return;   // folded by k_flux
   const long long lo = max(*q.acc_from, last - slot);
   if (lo > last) return;
-  flux_fold(q, pidx, npts, lo, last, false, nullptr);
+  flux_fold(q, (int)blockIdx.y, pidx, npts, lo, last, false, 0.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -2055,7 +2052,7 @@ void launch_accumulators(rfxh_plan* pl, hipStream_t s, int set) {
     FluxParams fp = pl->fluxes[q];
     for (int t = 0; t < 4; ++t) fp.f[t] = pl->set[set][pl->flux_comp[4 * q + t]];
     const long long npts = (long long)fp.n1 * fp.n2;
-    hipLaunchKernelGGL(k_flux, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_flux, dim3((unsigned)((npts + 255) / 256), 4), dim3(256), 0, s, fp);
   }
 }
 
@@ -2076,7 +2073,8 @@ void dft_flush(rfxh_plan* pl, hipStream_t s) {
   for (size_t q = 0; q < pl->fluxes.size(); ++q) {
     const FluxParams& fp = pl->fluxes[q];   // the flush reads only the ring
     const long long npts = (long long)fp.n1 * fp.n2;
-    hipLaunchKernelGGL(k_flux_flush, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, fp);
+    hipLaunchKernelGGL(k_flux_flush, dim3((unsigned)((npts + 255) / 256), 4), dim3(256), 0, s,
+                       fp);
   }
   hipLaunchKernelGGL(k_set_acc_from, dim3(1), dim3(64), 0, s, pl->acc_from, pl->kp.step);
 }
