@@ -448,9 +448,9 @@ __device__ __forceinline__ float3 h_cell(const KParams& p, const float* const Eo
 //     P_d^{n+1} = alpha P_d^n + beta (E^{n+1} + E^n)
 // before the CPML correction, PEC and point operations.  A cell outside the
 // state box is left alone (the host puts every cell with poles inside it).
-__device__ __noinline__ void debye_cell(const KParams& p, const PsiIO& ps, int i, int j, int k,
-                                        unsigned c, float o0, float o1, float o2, float& e0,
-                                        float& e1, float& e2) {
+__device__ __noinline__ void dispersion_cell(const KParams& p, const PsiIO& ps, int i, int j,
+                                             int k, unsigned c, float o0, float o1, float o2,
+                                             float& e0, float& e1, float& e2) {
   const int li = i - ps.pol_lo[0], lj = j - ps.pol_lo[1], lk = k - ps.pol_lo[2];
   if (!ps.pol || li < 0 || lj < 0 || lk < 0 || li >= ps.pol_n[0] || lj >= ps.pol_n[1] ||
       lk >= ps.pol_n[2])
@@ -526,7 +526,7 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
   e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
   e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
-  if (DISP && (fl & RFXH_POLES)) debye_cell(p, ps, i, j, k, c, o0, o1, o2, e0, e1, e2);
+  if (DISP && (fl & RFXH_POLES)) dispersion_cell(p, ps, i, j, k, c, o0, o1, o2, e0, e1, e2);
   if (in_slab(p, i, j, k)) {
     // keep the CPML pointer and profile loads inside the slab branch: the
     // compiler would otherwise hoist them into every wavefront (and spill
