@@ -111,3 +111,17 @@ def test_far_field_rejects_bad_box(R):
     bad = res.ntff_box._replace(i_hi=res.grid.shape[0])
     with pytest.raises(ValueError):
         F.compute_far_field(res.ntff_data, bad, res.grid, [0.5], [0.0])
+
+
+def test_batch_carries_the_ntff_sums(R):
+    """A batched run returns the same NTFF face sums as the instance run on
+    its own (the batched step and the column step give bit-identical
+    fields)."""
+    _say("NTFF in a batch of two")
+    from rfx_amd.batch import run_batch
+    spec, n = cases.dipole_ntff_case()
+    single = cases.to_sim(spec).run(n_steps=n)
+    res = run_batch([cases.to_sim(spec), cases.to_sim(cases.mixed_case("run"))], [n, 60])
+    for f in o.NTFF_FACES:
+        np.testing.assert_array_equal(getattr(res[0].ntff_data, f), getattr(single.ntff_data, f))
+    assert res[1].ntff_data is None
