@@ -111,7 +111,7 @@ typedef struct {
   const uint16_t* d_mat;   /* device [nx*ny*nz] */
   int32_t n_mat;
   const float* mat_coef;   /* host [n_mat][4] = ca, cb, ce (= dt/(eps_r eps0)), ch (= dt/(mu_r mu0)) */
-  const uint32_t* mat_flags; /* host [n_mat] RFXH_ZERO_* | RFXH_POINT */
+  const uint32_t* mat_flags; /* host [n_mat] RFXH_ZERO_* | RFXH_POINT | pole bits (RFXH_POLES) */
   /* CPML, faces ordered x_lo, x_hi, y_lo, y_hi, z_lo, z_hi */
   int32_t cpml_n[6];       /* slab depth per face (0 = no CPML on that face) */
   const float* cpml_b[6];  /* host [cpml_n[f]] in slab-local order (index 0 = slab's first cell) */
