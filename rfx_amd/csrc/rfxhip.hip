@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(kWG) void k_update_h(KParams p, PsiIO ps) {
   two_pass_h<GRADED, ONE_MU>(p, ps, i, j, k);
 }
 
-// DISP: the plan has Debye poles (a separate instance: the ADE branch would
+// DISP: the plan has dispersive poles (a separate instance: the ADE branch would
 // cost the plain kernels registers)
 template <bool GRADED, bool DISP>
 __global__ __launch_bounds__(kWG) void k_update_e(KParams p, PsiIO ps) {
@@ -1960,7 +1960,7 @@ __global__ void k_advance(long long* step) {
 
 struct rfxh_plan {
   KParams kp;
-  // Debye polarisation state (caller-owned) and per-material pole coefficients
+  // Debye / Lorentz polarisation state (caller-owned) and per-material pole coefficients
   float* pol = nullptr;
   const float* pcoef = nullptr;
   int n_pole = 0, n_debye = 0;
@@ -2424,7 +2424,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     }
     off += 4 * s;
   }
-  // Debye poles: cells whose material flags carry pole bits update their
+  // Dispersive poles: cells whose material flags carry pole bits update their
   // polarisation in the box pol_lo + [0, pol_n)
   if (d->n_pole < 0 || d->n_pole > 16) return bail(fail(RFXH_EINVAL, "n_pole must be 0..16"));
   if (d->n_debye < 0 || d->n_debye > d->n_pole)
@@ -2433,7 +2433,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     if (!d->pole_coef || !d->d_pol) return bail(fail(RFXH_EINVAL, "dispersion tables missing"));
     for (int a = 0; a < 3; ++a)
       if (d->pol_lo[a] < 0 || d->pol_n[a] < 1 || d->pol_lo[a] + d->pol_n[a] > N[a])
-        return bail(fail(RFXH_EINVAL, "Debye state box outside the grid"));
+        return bail(fail(RFXH_EINVAL, "polarisation state box outside the grid"));
     for (int m = 0; m < d->n_mat; ++m)
       if ((d->mat_flags[m] & RFXH_POLES) >> 16 >= (1u << d->n_pole))
         return bail(fail(RFXH_EINVAL, "material flags name a pole past n_pole"));
@@ -2801,7 +2801,7 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
   if (pingpong && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
   if (pingpong && pl->n_pole > 0)
-    return fail(RFXH_EINVAL, "Debye materials run on the two-pass kernels (0, 4, 5, 6)");
+    return fail(RFXH_EINVAL, "dispersive (Debye / Lorentz) materials run on the two-pass kernels (0, 4, 5, 6)");
   if (kernel == 2 || kernel == 3 || kernel == 7) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     if (kernel == 7) {
@@ -2850,7 +2850,7 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
                     float* d_rec, void* stream) {
   if (!pl) return fail(RFXH_EINVAL, "null plan");
   if (pl->n_pole > 0)
-    return fail(RFXH_EINVAL, "Debye materials: the reverse mode covers non-dispersive plans only");
+    return fail(RFXH_EINVAL, "dispersive materials: the reverse mode covers non-dispersive plans only");
   if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
     return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
   int rc = check_region(pl, region);
@@ -2884,7 +2884,7 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
 int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
   if (!pl || !ad) return fail(RFXH_EINVAL, "null plan or adjoint descriptor");
   if (pl->n_pole > 0)
-    return fail(RFXH_EINVAL, "Debye materials: the reverse mode covers non-dispersive plans only");
+    return fail(RFXH_EINVAL, "dispersive materials: the reverse mode covers non-dispersive plans only");
   if (lo < 0 || hi < lo || hi > pl->max_steps) return fail(RFXH_EINVAL, "bad step range");
   for (int c = 0; c < 6; ++c)
     if (!ad->d_adj[c]) return fail(RFXH_EINVAL, "adjoint field pointer missing");
