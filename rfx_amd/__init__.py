@@ -8,6 +8,7 @@ include/rfxhip.h; torch provides device memory, streams and autograd.
 """
 from .api import (DFTPlaneProbe, FDTDState, FluxMonitor, ForwardResult, Result, Simulation,
                   extract_lumped_s11, flux_spectrum)
+from .boundary import Boundary, BoundarySpec
 from .geometry import Box
 from .grid import Grid
 from .materials import DebyePole, LorentzPole, drude_pole, lorentz_pole
@@ -16,5 +17,5 @@ from .sources import GaussianPulse, LumpedPort
 __all__ = ["Simulation", "Result", "ForwardResult", "FDTDState", "DFTPlaneProbe", "FluxMonitor",
            "flux_spectrum", "Box",
            "GaussianPulse", "LumpedPort", "Grid", "extract_lumped_s11", "DebyePole",
-           "LorentzPole", "lorentz_pole", "drude_pole"]
+           "LorentzPole", "lorentz_pole", "drude_pole", "Boundary", "BoundarySpec"]
 __version__ = "0.1.0"

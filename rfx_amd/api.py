@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from .constants import EPS_0
+from .boundary import BoundarySpec
 from .cpml import face_tables
 from .engine import DeviceRun, Problem
 from .geometry import Box
@@ -197,11 +198,14 @@ def extract_lumped_s11(v_dft, i_dft, z0=50.0):
 class Simulation:
     """Builder for one FDTD problem (rfx/api/__init__.py:280)."""
 
-    def __init__(self, freq_max: float, domain, *, boundary: str = "cpml",
+    def __init__(self, freq_max: float, domain, *, boundary: str | BoundarySpec = "cpml",
                  cpml_layers: int = 16, cpml_kappa_max: float = 1.0, pec_faces=None,
                  dx: float | None = None, mode: str = "3d", dz_profile=None,
                  dx_profile=None, dy_profile=None, precision: str = "float32",
                  solver: str = "yee", stencil_order: int = 2, device=None):
+        if isinstance(boundary, BoundarySpec):
+            # one kind on all six faces (rfx/api/__init__.py takes str or BoundarySpec)
+            boundary, cpml_layers = boundary.uniform(int(cpml_layers))
         if boundary not in ("pec", "cpml"):
             raise NotImplementedError(f"boundary={boundary!r} is not supported by the HIP backend "
                                       "(supported: 'pec', 'cpml')")
