@@ -1197,10 +1197,12 @@ void k_update_h_col(KParams p, const PsiIO* __restrict__ psp) {
   col_h<GRADED, NP, ONE_MU>(p, ps, ib * NP, j, k, p.nx);
 }
 
-// 2-cell columns: held to 64 VGPRs (8 wavefronts per SIMD, no spills; the
-// compiler's own choice, 67, allows 7) -- measured 3.7 % off the E pass.
+// 2-cell columns: an occupancy target of 7 wavefronts per SIMD.  With
+// 512-thread workgroups, holding the kernel to 64 VGPRs (8 wavefronts) was
+// 3.7 % faster than the compiler's own choice; with 256-thread workgroups
+// a target of 7 is 0.7-1 % faster than 8 and the unconstrained build.
 template <bool GRADED, int NP, bool DISP>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 8 : 1)))
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 7 : 1)))
 void k_update_e_col(KParams p, const PsiIO* __restrict__ psp) {
   const PsiIO& ps = *psp;
   int ib, j, k;
