@@ -34,22 +34,27 @@ HBM_PEAK_GBS = 8000.0
 
 
 def c2_spec(n_int=512):
-    """The 512^3 workload as high-level inputs (shared by the GPU run and the
-    CPU baseline)."""
+    """The 512^3 workload as high-level inputs (shared by the GPU run, the
+    parity tests and the CPU baseline).  n_int < 512 scales the same seeded
+    draws (box corners and sizes, probe cells) to an n_int^3 interior."""
     rng = np.random.default_rng(2)
     dx = 1e-3
     L = n_int * dx
+    f = n_int / 512
     mats, geom = {}, []
     for b in range(64):
         size = rng.integers(8, 65, size=3)
-        lo = np.array([rng.integers(0, n_int - s + 1) for s in size])
+        lo = np.array([rng.integers(0, 512 - s + 1) for s in size])
         eps = float(rng.uniform(1.0, 10.0))
         sig = float(rng.uniform(0.0, 0.05))
+        if n_int != 512:
+            size = np.maximum(2, np.round(size * f)).astype(np.int64)
+            lo = np.minimum(np.round(lo * f).astype(np.int64), n_int - size)
         name = f"m{b}"
         mats[name] = (eps, sig, 1.0)
         geom.append((tuple((lo + 0.5) * dx), tuple((lo + size + 0.5) * dx), name))
     c = n_int // 2 * dx
-    probes = [(tuple((rng.integers(0, n_int + 1, size=3)) * dx), "ez") for _ in range(16)]
+    probes = [(tuple(np.round(rng.integers(0, 513, size=3) * f) * dx), "ez") for _ in range(16)]
     return dict(freq_max=20e9, domain=(L, L, L), dx=dx, boundary="cpml", cpml_layers=10,
                 materials=mats, geometry=geom,
                 sources=[((c, c, c), "ez", 10e9, 0.5)],
@@ -66,20 +71,48 @@ def c1_spec():
                 probes=[((0.11, 0.03, 0.025), "ez")], dft_planes=[])
 
 
-def c4_spec():
+def c4_spec(n=(256, 256, 128), cpml_layers=10):
     """Gradient workload: 256x256x128 interior, dx = 1 mm, 10-layer CPML,
-    a centred 64x64x4 design region of eps_r ~ U(1, 6) (seed 4), a line of
-    Ez soft sources along y at (x0, z_c) with a 12 GHz pulse, 3 Ez probes
-    beyond the region; objective sum_p sum_f |DFT_f ts_p|^2 over 5
-    frequencies in [11, 13] GHz; 3000 steps."""
+    a line of Ez soft sources along y at (x0, z_c) with a 12 GHz pulse, 3
+    Ez probes beyond the design region; objective sum_p sum_f |DFT_f
+    ts_p|^2 over 5 frequencies in [11, 13] GHz; 3000 steps.  n scales the
+    interior (positions scale with it)."""
     dx = 1e-3
-    nx, ny, nz = 256, 256, 128
-    x0, zc = 40 * dx, nz // 2 * dx
-    sources = [((x0, (j + 0.5) * dx, zc), "ez", 12e9, 0.5) for j in range(8, ny - 8)]
-    probes = [((216 * dx, (ny // 2 + o_) * dx, zc), "ez") for o_ in (-40, 0, 40)]
+    nx, ny, nz = n
+    x0, zc = round(40 * nx / 256) * dx, nz // 2 * dx
+    edge = max(2, round(8 * ny / 256))
+    sources = [((x0, (j + 0.5) * dx, zc), "ez", 12e9, 0.5) for j in range(edge, ny - edge)]
+    off = round(40 * ny / 256)
+    probes = [((round(216 * nx / 256) * dx, (ny // 2 + o_) * dx, zc), "ez")
+              for o_ in (-off, 0, off)]
     return dict(freq_max=20e9, domain=(nx * dx, ny * dx, nz * dx), dx=dx, boundary="cpml",
-                cpml_layers=10, materials={}, geometry=[], sources=sources, probes=probes,
-                dft_planes=[])
+                cpml_layers=cpml_layers, materials={}, geometry=[], sources=sources,
+                probes=probes, dft_planes=[])
+
+
+def c4_design(shape, region_n=(64, 64, 4), seed=4):
+    """The centred design region (i0, i1, j0, j1, k0, k1) of region_n voxels
+    and the eps_r array over the whole grid: 1 outside, U(1, 6) inside."""
+    rng = np.random.default_rng(seed)
+    eps = np.ones(shape, np.float32)
+    c = [s // 2 for s in shape]
+    h = [r // 2 for r in region_n]
+    region = (c[0] - h[0], c[0] - h[0] + region_n[0], c[1] - h[1], c[1] - h[1] + region_n[1],
+              c[2] - h[2], c[2] - h[2] + region_n[2])
+    eps[region[0]:region[1], region[2]:region[3], region[4]:region[5]] = \
+        rng.uniform(1.0, 6.0, region_n).astype(np.float32)
+    return eps, region
+
+
+C4_FREQS = np.linspace(11e9, 13e9, 5)
+
+
+def c4_objective_np(ts, dt):
+    """sum_p sum_f |sum_n ts[n, p] e^{-j 2 pi f n dt}|^2 in float64."""
+    t = np.arange(ts.shape[0]) * dt
+    ph = np.exp(-2j * np.pi * C4_FREQS[:, None] * t[None, :])
+    acc = ph @ np.asarray(ts, np.float64)
+    return float(np.sum(acc.real ** 2 + acc.imag ** 2))
 
 
 def grad_cost(n_steps=3000):
@@ -91,17 +124,11 @@ def grad_cost(n_steps=3000):
     spec = c4_spec()
     sim = build_sim(spec)
     grid = sim._build_grid()
-    p = grid.axis_pads[0]
     shape = grid.shape
-    rng = np.random.default_rng(4)
-    eps = np.ones(shape, np.float32)
-    ci, cj, ck = shape[0] // 2, shape[1] // 2, shape[2] // 2
-    region = (ci - 32, ci + 32, cj - 32, cj + 32, ck - 2, ck + 2)
-    eps[region[0]:region[1], region[2]:region[3], region[4]:region[5]] = \
-        rng.uniform(1.0, 6.0, (64, 64, 4)).astype(np.float32)
+    eps, region = c4_design(shape)
     _, prob, _ = sim._build_problem(n_steps, "forward", eps)
     dt = grid.dt
-    freqs = torch.linspace(11e9, 13e9, 5, dtype=torch.float64, device="cuda:0")
+    freqs = torch.tensor(C4_FREQS, dtype=torch.float64, device="cuda:0")
     t = torch.arange(n_steps, dtype=torch.float64, device="cuda:0") * dt
     ph = torch.exp(-2j * np.pi * freqs[:, None] * t[None, :])      # (5, n)
 

@@ -26,7 +26,45 @@ from . import _lib
 from .constants import EPS_0, MU_0
 from .engine import DeviceRun, Problem
 
-DEFAULT_RECORD_BUDGET = 24 << 30  # bytes of device memory for the forward record
+DEFAULT_RECORD_BUDGET = 24 << 30  # bytes of device memory for the record and checkpoints
+
+
+def segment_count(n_steps: int, per_step: int, state: int, budget: int,
+                  forced: int | None = None) -> int:
+    """Number of recompute segments of a reverse-mode run.
+
+    One segment (no checkpoints, no recompute) when the record of every
+    step fits the budget.  Otherwise K segments cost a record of
+    ceil(T/K) steps plus K state checkpoints (fields + CPML memory); the
+    total is smallest near K = sqrt(T per_step / state), the reference's
+    sqrt(n) split (rfx/simulation.py:2011-2055).  The run's cost does not
+    depend on K beyond K = 1 (every step is recomputed once), so the
+    smallest K whose memory fits is taken.  Raises MemoryError when even
+    the best K does not fit (a smaller grad_region shrinks the record)."""
+    T = int(n_steps)
+    if forced is not None and int(forced) > 1:
+        return min(int(forced), max(T, 1))
+    if T * per_step <= budget:
+        return 1
+
+    def need(k):
+        return math.ceil(T / k) * per_step + k * state
+
+    k_best = max(2, min(T, int(round(math.sqrt(T * per_step / max(state, 1))))))
+    k_best = min((k for k in (k_best - 1, k_best, k_best + 1) if 2 <= k <= T), key=need)
+    if need(k_best) > budget:
+        raise MemoryError(
+            f"reverse mode needs at least {need(k_best) / 2**30:.1f} GiB ({k_best} segments of "
+            f"{math.ceil(T / k_best)} steps: record {per_step} B/step, checkpoint {state} B) "
+            f"but the budget is {budget / 2**30:.1f} GiB; pass a smaller grad_region")
+    lo, hi = 2, k_best                    # need() falls from 2 to k_best: bisect the first fit
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if need(mid) <= budget:
+            hi = mid
+        else:
+            lo = mid + 1
+    return lo
 
 
 def eps_derivative_table(mat, dt: float) -> np.ndarray:
@@ -70,10 +108,12 @@ class AdjointRunner:
         self.region = _lib.Region(i0, j0, k0, i1 - i0, j1 - j0, k1 - k0)
         self.nreg = (i1 - i0) * (j1 - j0) * (k1 - k0)
         per_step = 9 * 4 * self.nreg
-        if checkpoint_segments is not None and int(checkpoint_segments) > 1:
-            nseg = int(checkpoint_segments)
-        else:
-            nseg = max(1, math.ceil(self.n_steps * per_step / record_budget))
+        state = 4 * (6 * int(np.prod(shape)) + int(self.dr.psis[0].numel()))
+        budget = int(record_budget)
+        if self.dr.device.type == "cuda":
+            free, _ = torch.cuda.mem_get_info(self.dr.device)
+            budget = min(budget, int(0.8 * free))
+        nseg = segment_count(self.n_steps, per_step, state, budget, checkpoint_segments)
         self.seg_len = math.ceil(self.n_steps / nseg)
         self.bounds = [(lo, min(self.n_steps, lo + self.seg_len))
                        for lo in range(0, self.n_steps, self.seg_len)]

@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from .constants import EPS_0
-from .boundary import BoundarySpec
+from .boundary import BoundarySpec, run_kind
 from .cpml import face_tables
 from .engine import DeviceRun, Problem
 from .geometry import Box
@@ -205,7 +205,7 @@ class Simulation:
                  solver: str = "yee", stencil_order: int = 2, device=None):
         if isinstance(boundary, BoundarySpec):
             # one kind on all six faces (rfx/api/__init__.py takes str or BoundarySpec)
-            boundary, cpml_layers = boundary.uniform(int(cpml_layers))
+            boundary, cpml_layers = run_kind(boundary, int(cpml_layers))
         if boundary not in ("pec", "cpml"):
             raise NotImplementedError(f"boundary={boundary!r} is not supported by the HIP backend "
                                       "(supported: 'pec', 'cpml')")
@@ -694,6 +694,11 @@ class Simulation:
         n_steps = int(n_steps)
         s11 = None if port_s11_freqs is None else np.asarray(port_s11_freqs, F32)
         needs_grad = isinstance(eps_override, torch.Tensor) and eps_override.requires_grad
+        for nm, ov in (("sigma_override", sigma_override), ("mu_r_override", mu_r_override)):
+            if isinstance(ov, torch.Tensor) and ov.requires_grad:
+                raise NotImplementedError(
+                    f"gradients w.r.t. {nm} are not supported by the reverse mode (it "
+                    "differentiates eps_override); pass a tensor without requires_grad")
         eps_np = _to_np(eps_override)
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),

@@ -28,10 +28,16 @@ _CACHE: dict = {}
 
 
 def _key(dr) -> str:
+    """Everything that decides which variants a plan accepts and how fast
+    they run: shape, lane, CPML depths, field sets, dispersion poles, whether
+    the material table fits the LDS-staged variants, and the device."""
     p = dr.prob
     dev = torch.cuda.get_device_name(dr.device)
     faces = tuple(int(f[0]) for f in p.faces)
-    return f"{tuple(p.shape)}|{int(p.graded)}|{faces}|{len(dr.sets)}|{dev}"
+    n_pole = len(p.poles) + len(p.lorentz)
+    big = int(len(dr.flags) > 1024)
+    extras = f"{len(p.planes)}|{len(p.fluxes)}|{len(p.ntff)}"
+    return f"{tuple(p.shape)}|{int(p.graded)}|{faces}|{len(dr.sets)}|{n_pole}|{big}|{extras}|{dev}"
 
 
 def variants(dr):
@@ -100,9 +106,12 @@ def autotune(dr, trial_steps: int = 32, repeats: int = 2, cache_file: str | None
         with open(cache_file) as fh:
             _CACHE.update({k: tuple(v) for k, v in json.load(fh).items()})
     if key in _CACHE:
-        set_variant(dr, *_CACHE[key])
-        dr.reset()
-        return {"variant": _CACHE[key], "ms_per_step": None, "cached": True}
+        try:
+            set_variant(dr, *_CACHE[key])
+            dr.reset()
+            return {"variant": _CACHE[key], "ms_per_step": None, "cached": True}
+        except _lib.RfxHipError:
+            del _CACHE[key]               # a stale entry this plan refuses: tune again
     steps = max(1, min(int(trial_steps), dr.n_steps))
     stream = torch.cuda.current_stream(dr.device)
     timings = {}

@@ -1,0 +1,58 @@
+"""Host rules of the reverse mode (CPU): the recompute-segment count keeps
+record + checkpoints inside the memory budget, follows the sqrt(n) split
+when memory is tight and refuses what cannot fit."""
+import math
+
+import pytest
+
+from rfx_amd.adjoint import segment_count
+
+
+def _need(T, k, per_step, state):
+    return math.ceil(T / k) * per_step + (k * state if k > 1 else 0)
+
+
+def test_one_segment_when_the_record_fits():
+    assert segment_count(3000, 100, 10**6, 3000 * 100) == 1
+
+
+def test_forced_segments():
+    assert segment_count(3000, 100, 10**6, 1 << 40, forced=7) == 7
+    assert segment_count(5, 100, 10**6, 1 << 40, forced=9) == 5
+
+
+@pytest.mark.parametrize("T,per_step,state,budget", [
+    (2000, 36 * 533**3, 4 * (6 * 533**3 + 24 * 10 * 533**2), 400 << 30),   # whole 533^3 grid
+    (3000, 36 * 16384 * 8, 4 * 6 * 11_432_621, 5 << 30),                     # large region, tight
+    (4000, 1000, 50_000, 1_000_000),
+])
+def test_segments_fit_the_budget(T, per_step, state, budget):
+    k = segment_count(T, per_step, state, budget)
+    assert k >= 2
+    assert _need(T, k, per_step, state) <= budget
+    # the smallest fitting k: one fewer segment would not fit
+    assert k == 2 or _need(T, k - 1, per_step, state) > budget
+
+
+def test_whole_grid_533_is_refused_below_the_sqrt_minimum():
+    T, per_step, state = 2000, 36 * 533**3, 4 * 6 * 533**3
+    k_opt = round(math.sqrt(T * per_step / state))
+    floor = _need(T, k_opt, per_step, state)
+    with pytest.raises(MemoryError):
+        segment_count(T, per_step, state, int(0.9 * floor))
+    k = segment_count(T, per_step, state, int(1.01 * floor))
+    assert k <= k_opt + 1 and _need(T, k, per_step, state) <= 1.01 * floor
+
+
+@pytest.mark.parametrize("which", ["sigma_override", "mu_r_override"])
+def test_override_gradients_are_refused_not_dropped(which):
+    import numpy as np
+    import torch
+    import rfx_amd as R
+    sim = R.Simulation(10e9, (0.01, 0.01, 0.01), dx=1e-3, cpml_layers=4)
+    sim.add_source((0.005, 0.005, 0.005), "ez")
+    sim.add_probe((0.006, 0.005, 0.005), "ez")
+    shape = sim._build_grid().shape
+    t = torch.ones(shape, dtype=torch.float32, requires_grad=True)
+    with pytest.raises(NotImplementedError, match=which):
+        sim.forward(n_steps=4, **{which: t})

@@ -56,3 +56,32 @@ def test_axis_validation():
     with pytest.raises(TypeError):
         BoundarySpec(3, "cpml", "cpml")
     assert BoundarySpec(" CPML ", "cpml", "cpml").x == Boundary("cpml", "cpml")
+
+
+def test_reference_constructors_and_views():
+    # the reference's classmethod constructor and (axis, side, token) faces
+    spec = BoundarySpec.uniform("pec")
+    assert list(spec.faces()) == [(a, s, "pec") for a in "xyz" for s in ("lo", "hi")]
+    assert spec.pec_faces() == {f"{a}_{s}" for a in "xyz" for s in ("lo", "hi")}
+    assert spec.absorber_type is None
+    assert BoundarySpec.uniform("cpml").absorber_type == "cpml"
+    d = BoundarySpec("cpml", Boundary("cpml", "cpml", 5, 5), "cpml").to_dict()
+    assert BoundarySpec.from_dict(d) == BoundarySpec("cpml", Boundary("cpml", "cpml", 5, 5), "cpml")
+    from rfx_amd.boundary import normalize_boundary
+    assert normalize_boundary(None) == BoundarySpec.uniform("cpml")
+    assert normalize_boundary("pec") == spec
+    # BoundarySpec.uniform('pec') runs the PEC lane like the string form
+    g0, p0, _ = _problem("pec")
+    g1, p1, _ = _problem(BoundarySpec.uniform("pec"))
+    assert p0.shape == p1.shape and p0.faces == p1.faces
+
+
+def test_conformal_axis_is_validated_and_refused():
+    with pytest.raises(ValueError):
+        Boundary("cpml", "cpml", conformal=True)
+    spec = BoundarySpec(Boundary("pec", "pec", conformal=True), "pec", "pec")
+    assert spec.conformal_faces() == {"x_lo", "x_hi"}
+    with pytest.raises(NotImplementedError):
+        Simulation(10e9, (0.012, 0.010, 0.008), dx=0.001, boundary=spec)
+    with pytest.raises(ValueError):
+        BoundarySpec("cpml", "upml", "cpml")

@@ -1,0 +1,199 @@
+"""The benchmark workloads themselves on an MI355X: the 512^3 throughput
+workload (64 seeded random boxes, 16 probes, a 32-frequency DFT plane)
+and the design-region gradient workload, each scaled down far enough for
+the float64 oracle to finish in seconds, plus size-independent checks of
+the 533^3 run itself.
+
+Bars: every probe series and the DFT plane <= 1e-4 relative L2 against the
+float64 oracle; directional derivatives of the 5-frequency DFT objective
+within 1e-3 of the oracle's central differences (step 1e-3).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bench
+from oracle import cstep
+from oracle import fdtd_np as o
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import rfx_amd  # noqa: F401
+    return torch
+
+
+def _oracle_spec(spec, lane="run", eps=None):
+    out = bench.oracle_spec(spec)
+    out["lane"] = lane
+    if eps is not None:
+        out["eps_override"] = eps
+    return out
+
+
+def _variants_to_check():
+    # the default column step, the one-cell step and every single-launch
+    # variant the plan accepts; the autotuner may pick any of them
+    return [(4, 1), (0, 1), (7, 0)]
+
+
+@pytest.mark.parametrize("n_int", [48, 64])
+def test_c2_scaled_vs_oracle(torch_mod, n_int):
+    """c2_spec at an n_int^3 interior: 16 probes and the 32-frequency
+    z-plane DFT vs the float64 oracle, for each kernel variant."""
+    torch = torch_mod
+    from rfx_amd import autotune as at
+    from rfx_amd.engine import DeviceRun
+    n = 320
+    spec = bench.c2_spec(n_int)
+    _say(f"C2 scaled to {n_int}^3 interior, {n} steps")
+    sim = bench.build_sim(spec)
+    grid, prob, _ = sim._build_problem(n, "run")
+    g, cfg = o.build_uniform(_oracle_spec(spec), n)
+    assert tuple(g.shape) == tuple(grid.shape)
+    ref = cstep.run_c(cfg, n, want_fields=False)
+    dr = DeviceRun(prob, n, "cuda")
+    for v in _variants_to_check():
+        try:
+            at.set_variant(dr, *v)
+        except Exception as e:              # a variant this plan refuses
+            print("skip", v, e)
+            continue
+        _say(f"C2 {n_int}^3 variant {at.variant_name(v)}")
+        dr.reset()
+        dr.run(0, n)
+        torch.cuda.synchronize()
+        ts = dr.time_series().cpu().numpy()
+        assert ts.shape == (n, 16)
+        worst = 0.0
+        for p in range(16):
+            if np.abs(ref.time_series[:, p]).max() == 0.0:
+                assert np.abs(ts[:, p]).max() == 0.0
+                continue
+            e = cases.rel_l2(ts[:, p], ref.time_series[:, p])
+            worst = max(worst, e)
+            assert e <= TOL, (v, p, e)
+        acc = dr.plane_acc[0].cpu().numpy()
+        ep = cases.rel_l2(acc.view(np.float64), ref.dft_planes[0].view(np.float64))
+        print(at.variant_name(v), "worst probe", worst, "plane", ep)
+        assert ep <= TOL
+    dr.close()
+
+
+def test_c2_full_size_properties(torch_mod):
+    """The 533^3 workload itself, 48 steps: finite probes and plane, the
+    column step bit-identical to the one-cell step (fields, probes, plane)
+    and every other variant within float32 rounding of it."""
+    torch = torch_mod
+    from rfx_amd import autotune as at
+    from rfx_amd.engine import DeviceRun
+    n = 48
+    spec = bench.c2_spec()
+    _say("C2 533^3, 48 steps, variant identity")
+    sim = bench.build_sim(spec)
+    grid, prob, _ = sim._build_problem(n, "run")
+    assert grid.shape == (533, 533, 533)
+    dr = DeviceRun(prob, n, "cuda")
+    out = {}
+    for v in _variants_to_check():
+        try:
+            at.set_variant(dr, *v)
+        except Exception as e:
+            print("skip", v, e)
+            continue
+        _say(f"C2 533^3 variant {at.variant_name(v)}")
+        dr.reset()
+        dr.run(0, n)
+        torch.cuda.synchronize()
+        ts = dr.time_series().cpu().numpy().copy()
+        acc = dr.plane_acc[0].cpu().numpy().copy()
+        ez = dr.field("ez")
+        sl = ez[:, :, 266].cpu().numpy().copy()          # the plane's slice
+        out[v] = (ts, acc, sl)
+        assert np.all(np.isfinite(ts)) and np.all(np.isfinite(acc))
+    dr.close()
+    base = out[(0, 1)]
+    for v, (ts, acc, sl) in out.items():
+        if v[0] in (0, 4, 5, 6):
+            assert np.array_equal(ts, base[0]), v
+            assert np.array_equal(acc, base[1]), v
+            assert np.array_equal(sl, base[2]), v
+        else:
+            assert cases.rel_l2(sl, base[2]) <= 1e-6, v
+            assert cases.rel_l2(acc.view(np.float64), base[1].view(np.float64)) <= 1e-6, v
+
+
+def _c4_reduced():
+    n_int = (48, 48, 24)
+    spec = bench.c4_spec(n_int, cpml_layers=8)
+    sim = bench.build_sim(spec)
+    grid = sim._build_grid()
+    eps, region = bench.c4_design(grid.shape, region_n=(8, 8, 4))
+    return spec, sim, grid, eps, region
+
+
+def test_c4_reduced_gradient_vs_oracle_fd(torch_mod):
+    """The gradient workload at 48x48x24 with an 8x8x4 design region: line
+    source, 3 probes, 5-frequency DFT objective.  The hand adjoint's
+    dJ/d eps_r along 8 random directions (seed 44) vs central differences
+    (h = 1e-3) of the float64 oracle; the forward series vs the oracle."""
+    torch = torch_mod
+    spec, sim, grid, eps, region = _c4_reduced()
+    n = 500
+    _say(f"C4 reduced {grid.shape}, region {region}, {n} steps")
+    t = torch.tensor(eps, requires_grad=True)
+    res = sim.forward(eps_override=t, n_steps=n, grad_region=region)
+    ts = res.time_series.double()
+    ph = torch.exp(-2j * np.pi * torch.tensor(bench.C4_FREQS, dtype=torch.float64,
+                                              device=ts.device)[:, None]
+                   * (torch.arange(n, dtype=torch.float64, device=ts.device) * grid.dt)[None, :])
+    acc = ph @ ts.to(torch.complex128)
+    J = (acc.real ** 2 + acc.imag ** 2).sum()
+    J.backward()
+    grad = t.grad.numpy().astype(np.float64)
+    i0, i1, j0, j1, k0, k1 = region
+    outside = grad.copy()
+    outside[i0:i1, j0:j1, k0:k1] = 0.0
+    assert np.all(outside == 0.0)
+
+    def J_oracle(e):
+        gg, cfg = o.build_uniform(_oracle_spec(spec, "forward", e.astype(np.float32)), n)
+        out = cstep.run_c(cfg, n, want_fields=False)
+        return bench.c4_objective_np(out.time_series, gg.dt), out
+
+    J0, out0 = J_oracle(eps)
+    e_ts = cases.rel_l2(res.time_series.detach().cpu().numpy(), out0.time_series)
+    print("forward rel-L2", e_ts, "J", float(J), J0)
+    assert e_ts <= TOL
+    assert abs(float(J) - J0) / J0 <= 1e-4
+    rng = np.random.default_rng(44)
+    h = 1e-3
+    for trial in range(8):
+        d = np.zeros_like(grad)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        # the oracle keeps eps_r in float32 (as the reference does): use the
+        # direction the rounded perturbations actually take
+        ep = (eps + h * d).astype(np.float32)
+        em = (eps - h * d).astype(np.float32)
+        d_eff = (ep.astype(np.float64) - em.astype(np.float64)) / (2 * h)
+        Jp, _ = J_oracle(ep)
+        Jm, _ = J_oracle(em)
+        fd = (Jp - Jm) / (2 * h)
+        ad = float((grad * d_eff).sum())
+        scale = float(np.abs(grad * d_eff).sum())
+        print("direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 1e-3 * abs(fd) + 1e-4 * scale
