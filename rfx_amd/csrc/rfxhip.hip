@@ -930,6 +930,9 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
     const unsigned o = (unsigned)(i0 - 1) * P;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
+      // a lane whose cell lies outside the grid (k = -1, k >= nz, the row
+      // past ny) has no H to recompute: its CPML index would leave the slab
+      if (!(q ? v1 : v0)) continue;
       const unsigned cq = q ? c1 : c0;
       const unsigned cy = q ? c2 : c1;     // forward-y neighbour row
       const bool hy = q ? h2 : j + 1 < p.ny;
