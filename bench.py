@@ -452,7 +452,10 @@ def main():
     ms_e, n_e = dr.kernel_time_ms(1)
     ms_d, n_d = dr.kernel_time_ms(2)
     value = args.steps * n_total * cells / el / 1e6
-    if kern_v in (1, 2, 3, 7):
+    if kern_v == 8:
+        # one launch advances two steps: 2 x 48 B per cell
+        which, ms, unit_bytes = "k_step_tb2", ms_h, 2 * BYTES_PER_CELL_STEP * cells
+    elif kern_v in (1, 2, 3, 7):
         # one launch does the whole step: 48 B per cell
         which = {1: "k_step_fused", 2: "k_step_march", 3: "k_step_march", 7: "k_step_wmarch"}[kern_v]
         ms, unit_bytes = ms_h, BYTES_PER_CELL_STEP * cells

@@ -6,7 +6,8 @@ second field set, as one fused kernel (flat or XCD-band
 workgroup order), as the barrier-free wave-march kernel (each wavefront
 walks a pair of rows along x) or as the marching kernel (6- or 14-row tiles
 of the y-z plane walking along x), the last two in a chosen number of x
-segments (include/rfxhip.h ``rfxh_set_variant``).  All
+segments, or as the temporal-blocked march, which advances two time steps
+per sweep (include/rfxhip.h ``rfxh_set_variant``).  All
 variants compute the same step.  Which is fastest depends on the grid (CPML
 share, plane size, cache fit), so the tuner runs a few real steps of the
 actual problem under each variant, timed with events on the stream the run
@@ -45,16 +46,22 @@ def variants(dr):
     thread) and 4 / 6 / 5 (columns of 2 / 3 / 4 cells) with workgroup order
     1 (flat) or 0 (XCD bands); with two field sets also the fused step 1
     (same orders) and the marching steps 2 / 3 (6- / 14-row tiles) and the
-    wave-march 7 with their number of x segments (0 = automatic)."""
+    wave-march 7 with their number of x segments (0 = automatic), and the
+    temporal-blocked march 8 (two time steps per sweep: temporal-block depth
+    2 against depth 1 for every other variant) with its x segments."""
     out = [(0, 1), (0, 0)]
     for kern in (4, 6, 5):                  # columns of 2, 3, 4 cells
         out += [(kern, order) for order in (1, 0)]
     nx = dr.prob.shape[0]
     if len(dr.sets) == 2:
         out += [(1, 1), (1, 0)]
-        # the wave-march step stages the material table in LDS (at most 1024 ids)
-        for kern in (2, 3) + ((7,) if len(dr.flags) <= 1024 else ()):
+        # the wave-march and temporal-blocked steps stage the material table
+        # in LDS (at most 1024 ids)
+        small = len(dr.flags) <= 1024
+        for kern in (2, 3) + ((7,) if small else ()):
             out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32, 64) if s <= max(1, nx // 4)]
+        if small:
+            out += [(8, 0)] + [(8, s) for s in (1, 2, 3, 4, 6, 8) if s <= max(1, nx // 8)]
     return out
 
 
@@ -65,6 +72,8 @@ ORDERED = {0: "two-pass", 1: "fused", 4: "col2", 5: "col4", 6: "col3"}
 def variant_name(v) -> str:
     """two-pass-flat, col2-bands, col3-flat, fused-flat, march14-16, wmarch-8, ..."""
     k, prm = v
+    if k == 8:
+        return "tb2" if prm == 0 else f"tb2-{prm}"
     if k == 7:
         return "wmarch" if prm == 0 else f"wmarch-{prm}"
     if k in MARCH_ROWS:
@@ -74,6 +83,9 @@ def variant_name(v) -> str:
 
 
 def parse_variant(name: str):
+    if name.startswith("tb2"):
+        _, _, seg = name.partition("-")
+        return (8, int(seg) if seg else 0)
     if name.startswith("wmarch"):
         _, _, seg = name.partition("-")
         return (7, int(seg) if seg else 0)

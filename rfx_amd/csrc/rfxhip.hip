@@ -126,6 +126,9 @@ struct PsiIO {
 };
 
 constexpr int kDftK = 16;   // steps staged per fold of the DFT-plane accumulators
+// ring slots of a DFT plane: two groups of kDftK, so a two-step sweep can
+// stage the first step of the next group before the fold of the current one
+constexpr int kDftRing = 2 * kDftK;
 constexpr int kFoldF = 4;    // accumulators a fold keeps in flight
 
 struct PlaneParams {
@@ -133,7 +136,7 @@ struct PlaneParams {
   double2* acc;
   const double2* tw;  // [max_steps][nf] twiddles exp(-j 2 pi f t) dt
   long long* step;
-  float* snap;        // [kDftK][n1*n2] ring of plane values not yet folded in
+  float* snap;        // [kDftRing][n1*n2] ring of plane values not yet folded in
   const long long* acc_from;  // first step not yet folded into acc (device)
   long long max_steps;
   int nf;
@@ -297,20 +300,24 @@ __device__ __forceinline__ int slab_layer(const KParams& p, int face, int i, int
 
 // Port V/I sampling, source injection and probes at one cell, in the order
 // the host gave (rfx/simulation.py:1449-1567; rfx/nonuniform.py:1353-1542).
+// n_step < 0: the step of the device counter.  own = false (a halo thread
+// of the temporal-blocked march repeating another thread's cell): inject
+// only, record nothing.
 __device__ __forceinline__ void point_ops(const KParams& p, unsigned cell, float& e0, float& e1,
                                           float& e2, float hx, float hy, float hz, float hx_j,
                                           float hx_k, float hy_i, float hy_k, float hz_i,
-                                          float hz_j) {
+                                          float hz_j, long long n_step = -1, bool own = true) {
   int lo = 0, hi = p.n_ops;  // first op of this cell (ops sorted by cell, order)
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (p.ops[mid].cell < (long long)cell) lo = mid + 1;
     else hi = mid;
   }
-  const long long n = *p.step;
+  const long long n = n_step >= 0 ? n_step : *p.step;
   for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)cell; ++q) {
     const rfxh_point_op op = p.ops[q];
     if (op.kind == RFXH_OP_PORT) {
+      if (!own) continue;
       const float v = -pick3(op.comp, e0, e1, e2) * op.d_par;
       float cur;
       if (op.comp == 2)
@@ -327,7 +334,7 @@ __device__ __forceinline__ void point_ops(const KParams& p, unsigned cell, float
       if (op.comp == 0) e0 += w;
       else if (op.comp == 1) e1 += w;
       else e2 += w;
-    } else {
+    } else if (own) {
       p.probe_out[n * p.n_probe + op.slot] =
           op.comp < 3 ? pick3(op.comp, e0, e1, e2) : pick3(op.comp - 3, hx, hy, hz);
     }
@@ -1023,6 +1030,465 @@ __global__ __launch_bounds__(kBX * kWMW) void k_step_wmarch(KParams p, WMarchIO 
 }
 
 // ---------------------------------------------------------------------------
+// Temporal-blocked march: two time steps per sweep.  Reads set A (E^n,
+// H^{n-1/2}, CPML memory), writes set B (E^{n+2}, H^{n+3/2}, memory).
+//
+// A workgroup of kTBW wavefronts covers a tile of kTBW rows (y) x 64
+// columns (z) of the y-z plane, one (y, z) column per thread, and walks
+// along x through its segment of planes.  The kTBHalo = 2 outer rows and
+// columns on each side are halo: their threads repeat the work of the
+// neighbouring tiles' threads, so that the kTBJ x kTBK interior can take
+// both steps without waiting for another workgroup (only the interior
+// stores).  Along x the four half-steps form a pipeline: the iteration
+// that loads E^n of plane p updates
+//   D: E^{n+2}   on plane p-4   (H^{n+3/2} of p-4, p-5, E^{n+1} of p-4)
+//   B: E^{n+1}   on plane p-2   (H^{n+1/2} of p-2, p-3, E^n of p-2)
+//   C: H^{n+3/2} on plane p-3   (E^{n+1} of p-3, p-2, H^{n+1/2} of p-3)
+//   A: H^{n+1/2} on plane p-1   (E^n of p-1, p, H^{n-1/2} of p-1)
+// in that order.  Neighbours along x are the thread's own registers; along
+// y and z they come from the other threads through LDS, each stage reading
+// what the previous iteration wrote (two buffers, one barrier per plane).
+// The CPML memory of a cell stays in registers between the two steps (a
+// cell lies in at most one slab per axis, two terms each).
+//
+// Per two cell-steps a cell's E, H and material id are read once and E, H
+// written once: 50 B, against 2 x (74-80) B for two steps of the two-pass
+// passes.  Halo cells are read by two or four tiles (through the L2 when
+// the tiles run together).  Point operations of the halo copies inject but
+// record nothing; DFT planes are staged into their ring by the interior
+// thread of the plane cell (k_dft_fold2 folds them).
+// ---------------------------------------------------------------------------
+constexpr int kTBW = 16;                    // wavefronts (rows) per workgroup
+constexpr int kTBHalo = 2;                  // halo width: the steps per sweep
+constexpr int kTBJ = kTBW - 2 * kTBHalo;    // interior rows of a tile
+constexpr int kTBK = kBX - 2 * kTBHalo;     // interior columns of a tile
+constexpr int kTBMat = 1024;                // material table entries staged in LDS
+constexpr int kTBPlanes = 4;                // DFT planes the sweep stages itself
+
+struct TBPlane {
+  float* snap;     // ring [kDftRing][npts]
+  int axis, index, comp, n2, npts;
+};
+
+struct TBIO {
+  const float* eo[3];
+  const float* ho[3];
+  float* en[3];
+  float* hn[3];
+  int ntk, ntj, seg_len, n_mat;
+  int n_plane;
+  TBPlane pl[kTBPlanes];
+};
+
+// H at one cell with the CPML memory in registers: h_core's update reading
+// the memory from pin and leaving the new memory in pout (and, when store,
+// in ps.hw).
+template <bool GRADED>
+__device__ __forceinline__ float3 tb_h(const KParams& p, const PsiIO& ps, int i, int j, int k,
+                                       float ch, float ex, float ey, float ez, float ex_j,
+                                       float ez_j, float ex_k, float ey_k, float ey_i, float ez_i,
+                                       float hx, float hy, float hz, const float (&pin)[3][2],
+                                       float (&pout)[3][2], bool store) {
+  float ix, iy, iz;
+  if (GRADED) {
+    ix = gld(p.inv_h[0], i);
+    iy = gld(p.inv_h[1], j);
+    iz = gld(p.inv_h[2], k);
+  } else {
+    ix = iy = iz = p.inv_h0;
+  }
+  hx = hx - ch * ((ez_j - ez) * iy - (ey_k - ey) * iz);
+  hy = hy - ch * ((ex_k - ex) * iz - (ez_i - ez) * ix);
+  hz = hz - ch * ((ey_i - ey) * ix - (ex_j - ex) * iy);
+  if (in_slab(p, i, j, k)) {
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const float bb = gld(p.pb[face], l), cc = gld(p.pc[face], l), km1 = gld(p.pk[face], l);
+      const float inv = p.cinv[face];
+      float g0, g1;
+      if (ax == 0) {
+        g0 = (ez_i - ez) * inv;
+        g1 = (ey_i - ey) * inv;
+      } else if (ax == 1) {
+        g0 = (ez_j - ez) * inv;
+        g1 = (ex_j - ex) * inv;
+      } else {
+        g0 = (ey_k - ey) * inv;
+        g1 = (ex_k - ex) * inv;
+      }
+      const float s0 = bb * pin[ax][0] + cc * g0;
+      const float s1 = bb * pin[ax][1] + cc * g1;
+      pout[ax][0] = s0;
+      pout[ax][1] = s1;
+      if (store) {
+        const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+        stf(ps.hw[face][0], q, s0);
+        stf(ps.hw[face][1], q, s1);
+      }
+      const float d0 = ch * s0 + ch * km1 * g0;
+      const float d1 = ch * s1 + ch * km1 * g1;
+      if (ax == 0) {
+        hy += d0;
+        hz -= d1;
+      } else if (ax == 1) {
+        hx -= d0;
+        hz += d1;
+      } else {
+        hx += d0;
+        hy -= d1;
+      }
+    }
+  }
+  return make_float3(hx, hy, hz);
+}
+
+// E at one cell with the CPML memory in registers (e_core_co's update),
+// point operations at step n (recorded only when own).
+template <bool GRADED>
+__device__ __forceinline__ float3 tb_e(const KParams& p, const PsiIO& ps, int i, int j, int k,
+                                       unsigned c, float4 co, uint32_t fl, float e0, float e1,
+                                       float e2, float hx, float hy, float hz, float hy_i,
+                                       float hz_i, float hx_j, float hz_j, float hx_k, float hy_k,
+                                       const float (&pin)[3][2], float (&pout)[3][2], bool store,
+                                       long long n, bool own) {
+  float ix, iy, iz;
+  if (GRADED) {
+    ix = gld(p.inv_e[0], i);
+    iy = gld(p.inv_e[1], j);
+    iz = gld(p.inv_e[2], k);
+  } else {
+    ix = iy = iz = p.inv_e0;
+  }
+  const float ca = co.x, cb = co.y, ce = co.z;
+  e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
+  e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
+  e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
+  if (in_slab(p, i, j, k)) {
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const float bb = gld(p.pb[face], l), cc = gld(p.pc[face], l), km1 = gld(p.pk[face], l);
+      const float inv = p.cinv[face];
+      float g0, g1;
+      if (ax == 0) {
+        g0 = (hz - hz_i) * inv;
+        g1 = (hy - hy_i) * inv;
+      } else if (ax == 1) {
+        g0 = (hz - hz_j) * inv;
+        g1 = (hx - hx_j) * inv;
+      } else {
+        g0 = (hy - hy_k) * inv;
+        g1 = (hx - hx_k) * inv;
+      }
+      const float s0 = bb * pin[ax][0] + cc * g0;
+      const float s1 = bb * pin[ax][1] + cc * g1;
+      pout[ax][0] = s0;
+      pout[ax][1] = s1;
+      if (store) {
+        const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+        stf(ps.ew[face][0], q, s0);
+        stf(ps.ew[face][1], q, s1);
+      }
+      const float d0 = ce * s0 + ce * km1 * g0;
+      const float d1 = ce * s1 + ce * km1 * g1;
+      if (ax == 0) {
+        e1 -= d0;
+        e2 += d1;
+      } else if (ax == 1) {
+        e0 += d0;
+        e2 -= d1;
+      } else {
+        e0 -= d0;
+        e1 += d1;
+      }
+    }
+  }
+  if (p.pec_axes & 1) {
+    if (i == 0 || i == p.nx - 1) e1 = e2 = 0.f;
+  }
+  if (p.pec_axes & 2) {
+    if (j == 0 || j == p.ny - 1) e0 = e2 = 0.f;
+  }
+  if (p.pec_axes & 4) {
+    if (k == 0 || k == p.nz - 1) e0 = e1 = 0.f;
+    if (k == p.nz - 1) e2 = 0.f;
+  }
+  if (fl) {
+    if (fl & RFXH_ZERO_EX) e0 = 0.f;
+    if (fl & RFXH_ZERO_EY) e1 = 0.f;
+    if (fl & RFXH_ZERO_EZ) e2 = 0.f;
+    if (fl & RFXH_POINT)
+      point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j, n, own);
+  }
+  return make_float3(e0, e1, e2);
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
+                                                         const PsiIO* __restrict__ psp) {
+  // CPML pointers through device memory (read set A, write set B)
+  const PsiIO& ps = *psp;
+  __shared__ float4 s_coef[kTBMat];
+  __shared__ uint32_t s_flag[kTBMat];
+  // [buffer][stage: E^n, H^{n+1/2}, E^{n+1}, H^{n+3/2}][component][row][lane]
+  __shared__ float s_x[2][4][3][kTBW][kBX];
+  const int w = (int)threadIdx.y, l = (int)threadIdx.x;
+  const int tid = w * kBX + l;
+  // the plan rejects this variant for more than kTBMat materials
+  for (int t = tid; t < io.n_mat; t += kBX * kTBW) {
+    s_coef[t] = gld(p.coef, t);
+    s_flag[t] = gld(p.flags, t);
+  }
+  // the first iteration reads buffer 1 (no plane before the segment start)
+  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&s_x[1][0][0][0][0])[t] = 0.f;
+  __syncthreads();
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int tk = L % io.ntk;
+  const int r = L / io.ntk;
+  const int tj = r % io.ntj;
+  const int seg = r / io.ntj;
+  const int j = tj * kTBJ - kTBHalo + w;
+  const int k = tk * kTBK - kTBHalo + l;
+  const int i0 = seg * io.seg_len;
+  const int i1 = min(p.nx, i0 + io.seg_len);
+  if (i0 >= i1) return;   // whole workgroup
+  const bool valid = j >= 0 && j < p.ny && k >= 0 && k < p.nz;
+  const bool interior = valid && w >= kTBHalo && w < kTBW - kTBHalo && l >= kTBHalo &&
+                        l < kBX - kTBHalo;
+  const unsigned P = (unsigned)p.plane;
+  const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
+  const long long n = *p.step;
+  const float chc = p.ch_all;
+  auto pin = [&](int q) { return valid && q >= 0 && q < p.nx; };
+  auto own = [&](int q) { return interior && q >= i0 && q < i1; };
+  // neighbour value of stage st from LDS buffer b; zero outside the tile
+  auto nb = [&](int b, int st, int cmp, int ww, int ll) {
+    const bool ok = ww >= 0 && ww < kTBW && ll >= 0 && ll < kBX;
+    const float v = s_x[b][st][cmp][ok ? ww : 0][ok ? ll : 0];
+    return ok ? v : 0.f;
+  };
+  // psi terms of the cell at plane q from the read set (zero outside slabs)
+  auto ld_psi = [&](float (&dst)[3][2], const float* const (&src)[6][2], int q) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dst[a][0] = dst[a][1] = 0.f;
+    if (pin(q) && in_slab(p, q, j, k)) {
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        const int ly = slab_layer(p, face, q, j, k);
+        if (ly < 0) continue;
+        const unsigned qq = (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
+        dst[face >> 1][0] = ldf(src[face][0], qq);
+        dst[face >> 1][1] = ldf(src[face][1], qq);
+      }
+    }
+  };
+  auto stage_snap = [&](int q, int cmp0, const float3 v, long long st) {
+    for (int t = 0; t < io.n_plane; ++t) {
+      const TBPlane& d = io.pl[t];
+      if (d.comp < cmp0 || d.comp >= cmp0 + 3) continue;
+      const int pos = d.axis == 0 ? q : d.axis == 1 ? j : k;
+      if (pos != d.index) continue;
+      // plane cell (u, v): the two other indices in axis order
+      const int u = d.axis == 0 ? j : q, vv = d.axis == 2 ? j : k;
+      const float x = d.comp - cmp0 == 0 ? v.x : d.comp - cmp0 == 1 ? v.y : v.z;
+      d.snap[(st % kDftRing) * d.npts + (long long)u * d.n2 + vv] = x;
+    }
+  };
+
+  // pipeline state (see above); at iteration p:
+  float e0c[3], e0b[3], e0n[3];   // E^n of planes p-1, p-2, p
+  float h0n[3];                   // H^{n-1/2} of plane p-1
+  float h1a[3], h1b[3];           // H^{n+1/2} of planes p-2, p-3
+  float e1a[3], e1b[3];           // E^{n+1} of planes p-3, p-4
+  float h2a[3], h2by, h2bz;       // H^{n+3/2} of plane p-4, (y, z) of p-5
+  unsigned m1, m2, m3, m4;        // material ids of planes p-1 .. p-4
+  float phn[3][2], pen[3][2];     // memory read for A (plane p-1) and B (plane p-2)
+  float pha[3][2], phb[3][2];     // H memory after step 1, planes p-2, p-3
+  float pea[3][2], peb[3][2];     // E memory after step 1, planes p-3, p-4
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    e0c[c] = e0b[c] = h1a[c] = h1b[c] = e1a[c] = e1b[c] = h2a[c] = 0.f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) pha[c][t] = phb[c][t] = pea[c][t] = peb[c][t] = 0.f;
+  }
+  h2by = h2bz = 0.f;
+  m2 = m3 = m4 = 0u;
+  // prefetch for iteration pp + 1: E^n of plane pp+1; H^{n-1/2}, id and H
+  // memory of plane pp; E memory of plane pp-1
+  float fe[3], fh[3];
+  unsigned fm;
+  float fph[3][2], fpe[3][2];
+  auto prefetch = [&](int pp) {
+    const int q = pp + 1;
+    const unsigned oq = (unsigned)min(max(q, 0), p.nx - 1) * P + c0;
+    const unsigned op = (unsigned)min(max(pp, 0), p.nx - 1) * P + c0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      fe[c] = ldf(io.eo[c], oq);
+      fh[c] = ldf(io.ho[c], op);
+    }
+    fm = ldu16(p.mat, op);
+    ld_psi(fph, ps.hr, pp);
+    ld_psi(fpe, ps.er, pp - 1);
+    const bool vq = pin(q), vp = pin(pp);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      fe[c] = vq ? fe[c] : 0.f;
+      fh[c] = vp ? fh[c] : 0.f;
+    }
+  };
+  prefetch(i0 - 3);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    e0n[c] = fe[c];
+    h0n[c] = fh[c];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      phn[c][t] = fph[c][t];
+      pen[c][t] = fpe[c][t];
+    }
+  }
+  m1 = fm;
+
+  int it = 0;
+  for (int pp = i0 - 2; pp < i1 + 4; ++pp, ++it) {
+    const int rb = (it & 1) ^ 1, wb = it & 1;
+    prefetch(pp);
+    // ---- D: E^{n+2} on plane pp-4
+    float3 e2 = make_float3(0.f, 0.f, 0.f);
+    {
+      const int q = pp - 4;
+      const float hx_j = nb(rb, 3, 0, w - 1, l), hz_j = nb(rb, 3, 2, w - 1, l);
+      const float hx_k = nb(rb, 3, 0, w, l - 1), hy_k = nb(rb, 3, 1, w, l - 1);
+      if (pin(q)) {
+        const bool o = own(q);
+        const unsigned c = (unsigned)q * P + c0;
+        float dummy[3][2];
+        e2 = tb_e<GRADED>(p, ps, q, j, k, c, s_coef[m4], s_flag[m4], e1b[0], e1b[1], e1b[2],
+                          h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k, peb, dummy,
+                          o, n + 1, o);
+        if (o) {
+          stf(io.en[0], c, e2.x);
+          stf(io.en[1], c, e2.y);
+          stf(io.en[2], c, e2.z);
+          if (io.n_plane) stage_snap(q, 0, e2, n + 1);
+        }
+      }
+    }
+    // ---- B: E^{n+1} on plane pp-2
+    float3 e1 = make_float3(0.f, 0.f, 0.f);
+    float pe_new[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pe_new[a][0] = pe_new[a][1] = 0.f;
+    {
+      const int q = pp - 2;
+      const float hx_j = nb(rb, 1, 0, w - 1, l), hz_j = nb(rb, 1, 2, w - 1, l);
+      const float hx_k = nb(rb, 1, 0, w, l - 1), hy_k = nb(rb, 1, 1, w, l - 1);
+      if (pin(q)) {
+        const bool o = own(q);
+        const unsigned c = (unsigned)q * P + c0;
+        e1 = tb_e<GRADED>(p, ps, q, j, k, c, s_coef[m2], s_flag[m2], e0b[0], e0b[1], e0b[2],
+                          h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k, hy_k, pen,
+                          pe_new, false, n, o);
+        if (o && io.n_plane) stage_snap(q, 0, e1, n);
+      }
+    }
+    // ---- C: H^{n+3/2} on plane pp-3
+    float3 h2 = make_float3(0.f, 0.f, 0.f);
+    {
+      const int q = pp - 3;
+      const float ex_j = nb(rb, 2, 0, w + 1, l), ez_j = nb(rb, 2, 2, w + 1, l);
+      const float ex_k = nb(rb, 2, 0, w, l + 1), ey_k = nb(rb, 2, 1, w, l + 1);
+      if (pin(q)) {
+        const bool o = own(q);
+        const unsigned c = (unsigned)q * P + c0;
+        float dummy[3][2];
+        h2 = tb_h<GRADED>(p, ps, q, j, k, chc > 0.f ? chc : s_coef[m3].w, e1a[0], e1a[1],
+                          e1a[2], ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2],
+                          phb, dummy, o);
+        if (o) {
+          stf(io.hn[0], c, h2.x);
+          stf(io.hn[1], c, h2.y);
+          stf(io.hn[2], c, h2.z);
+          if (io.n_plane) stage_snap(q, 3, h2, n + 1);
+        }
+      }
+    }
+    // ---- A: H^{n+1/2} on plane pp-1
+    float3 h1 = make_float3(0.f, 0.f, 0.f);
+    float ph_new[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ph_new[a][0] = ph_new[a][1] = 0.f;
+    {
+      const int q = pp - 1;
+      const float ex_j = nb(rb, 0, 0, w + 1, l), ez_j = nb(rb, 0, 2, w + 1, l);
+      const float ex_k = nb(rb, 0, 0, w, l + 1), ey_k = nb(rb, 0, 1, w, l + 1);
+      if (pin(q)) {
+        h1 = tb_h<GRADED>(p, ps, q, j, k, chc > 0.f ? chc : s_coef[m1].w, e0c[0], e0c[1],
+                          e0c[2], ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1],
+                          h0n[2], phn, ph_new, false);
+        if (own(q) && io.n_plane) stage_snap(q, 3, h1, n);
+      }
+    }
+    // ---- publish this iteration's planes for the neighbours
+    s_x[wb][0][0][w][l] = e0n[0];
+    s_x[wb][0][1][w][l] = e0n[1];
+    s_x[wb][0][2][w][l] = e0n[2];
+    s_x[wb][1][0][w][l] = h1.x;
+    s_x[wb][1][1][w][l] = h1.y;
+    s_x[wb][1][2][w][l] = h1.z;
+    s_x[wb][2][0][w][l] = e1.x;
+    s_x[wb][2][1][w][l] = e1.y;
+    s_x[wb][2][2][w][l] = e1.z;
+    s_x[wb][3][0][w][l] = h2.x;
+    s_x[wb][3][1][w][l] = h2.y;
+    s_x[wb][3][2][w][l] = h2.z;
+    __syncthreads();
+    // ---- rotate
+    h2by = h2a[1];
+    h2bz = h2a[2];
+    h2a[0] = h2.x;
+    h2a[1] = h2.y;
+    h2a[2] = h2.z;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      e1b[c] = e1a[c];
+      h1b[c] = h1a[c];
+      e0b[c] = e0c[c];
+      e0c[c] = e0n[c];
+      e0n[c] = fe[c];
+      h0n[c] = fh[c];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        peb[c][t] = pea[c][t];
+        pea[c][t] = pe_new[c][t];
+        phb[c][t] = pha[c][t];
+        pha[c][t] = ph_new[c][t];
+        phn[c][t] = fph[c][t];
+        pen[c][t] = fpe[c][t];
+      }
+    }
+    e1a[0] = e1.x;
+    e1a[1] = e1.y;
+    e1a[2] = e1.z;
+    h1a[0] = h1.x;
+    h1a[1] = h1.y;
+    h1a[2] = h1.z;
+    m4 = m3;
+    m3 = m2;
+    m2 = m1;
+    m1 = fm;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Two-pass step, in place on p.f.
 // ---------------------------------------------------------------------------
 template <bool GRADED, bool ONE_MU = false>
@@ -1292,7 +1758,7 @@ __device__ __forceinline__ void dft_fold(const PlaneParams& q, long long pidx, l
   for (int m = 0; m < kDftK; ++m) {
     if (m < cnt) {
       const long long st = lo + m;
-      xs[m] = (hi_in_reg && st == hi) ? xhi : q.snap[(st % kDftK) * npts + pidx];
+      xs[m] = (hi_in_reg && st == hi) ? xhi : q.snap[(st % kDftRing) * npts + pidx];
     }
   }
   for (int f0 = 0; f0 < q.nf; f0 += kFoldF) {
@@ -1332,12 +1798,26 @@ __global__ __launch_bounds__(256) void k_dft_plane(PlaneParams q) {
   ii[a1] = u;
   ii[a2] = v;
   const float x = q.fld[((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2]];
-  const int slot = (int)(n % kDftK);
-  if (slot == kDftK - 1) {
+  if (n % kDftK == kDftK - 1) {
     const long long lo = max(*q.acc_from, n - (kDftK - 1));
     dft_fold(q, pidx, npts, lo, n, true, x);
   } else {
-    q.snap[slot * npts + pidx] = x;
+    q.snap[(n % kDftRing) * npts + pidx] = x;
+  }
+}
+
+// After a two-step sweep of the temporal-blocked march, which staged the
+// plane values of steps n and n+1 into the ring itself: fold a group that
+// one of them completes.
+__global__ __launch_bounds__(256) void k_dft_fold2(PlaneParams q) {
+  const long long npts = (long long)q.n1 * q.n2;
+  const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= npts) return;
+  const long long n = *q.step;
+  for (long long st = n; st <= n + 1; ++st) {
+    if (st < 0 || st >= q.max_steps || st % kDftK != kDftK - 1) continue;
+    const long long lo = max(*q.acc_from, st - (kDftK - 1));
+    dft_fold(q, pidx, npts, lo, st, false, 0.f);
   }
 }
 
@@ -1349,7 +1829,7 @@ __global__ __launch_bounds__(256) void k_dft_flush(PlaneParams q) {
   const long long last = *q.step - 1;
   if (last < 0) return;
   const int slot = (int)(last % kDftK);
-  if (slot == kDftK - 1) return;   // folded by k_dft_plane
+  if (slot == kDftK - 1) return;   // folded by k_dft_plane / k_dft_fold2
   const long long lo = max(*q.acc_from, last - slot);
   if (lo > last) return;
   dft_fold(q, pidx, npts, lo, last, false, 0.f);
@@ -1961,6 +2441,10 @@ __global__ void k_advance(long long* step) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *step = *step + 1;
 }
 
+__global__ void k_advance2(long long* step) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *step = *step + 2;
+}
+
 }  // namespace
 
 struct rfxh_plan {
@@ -1979,6 +2463,8 @@ struct rfxh_plan {
   int ncol = 1;                 // two-pass step: cells per thread along x (1, 2, 3, 4)
   bool wmarch = false;          // the ping-pong step uses the wave-march kernel
   int wmarch_seg = 0;           // its x segments (0: automatic)
+  bool tb = false;              // two steps per sweep (temporal-blocked march), ping-pong
+  int tb_seg = 0;               // its x segments (0: automatic)
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   long long* acc_from = nullptr;  // first step not yet folded into the DFT planes (device)
@@ -2234,6 +2720,71 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   launch_accumulators(pl, s, to);
   if (ev) HIPCHK(hipEventRecord(ev[3], s));
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Tiles and x segments of the temporal-blocked march.  Automatic segments:
+// at least about four workgroups per CU (one fits a CU), at least eight
+// planes per segment.
+void tb_blocks(const KParams& kp, int want_seg, TBIO* io, int* nseg_out) {
+  io->ntj = (kp.ny + kTBJ - 1) / kTBJ;
+  io->ntk = (kp.nz + kTBK - 1) / kTBK;
+  const long long tiles = (long long)io->ntj * io->ntk;
+  int nseg = want_seg;
+  if (nseg <= 0) {
+    nseg = (int)std::max<long long>(1, (1024 + tiles - 1) / tiles);
+    nseg = std::min(nseg, std::max(1, kp.nx / 8));
+  }
+  nseg = std::min(nseg, kp.nx);
+  io->seg_len = (kp.nx + nseg - 1) / nseg;
+  *nseg_out = (kp.nx + io->seg_len - 1) / io->seg_len;
+}
+
+// Two time steps starting from field set `from` (writes set 1-from): the
+// temporal-blocked sweep, the folds of the DFT groups it completes, and the
+// step counter.  ev (optional): four events as in launch_step; the sweep
+// kernel's interval covers both steps.
+int launch_tb(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
+  const int to = 1 - from;
+  const KParams& kp = pl->kp;
+  TBIO io;
+  std::memset(&io, 0, sizeof(io));
+  for (int a = 0; a < 3; ++a) {
+    io.eo[a] = pl->set[from][a];
+    io.ho[a] = pl->set[from][3 + a];
+    io.en[a] = pl->set[to][a];
+    io.hn[a] = pl->set[to][3 + a];
+  }
+  io.n_mat = pl->n_mat;
+  int nseg = 1;
+  tb_blocks(kp, pl->tb_seg, &io, &nseg);
+  io.n_plane = (int)pl->planes.size();
+  for (int q = 0; q < io.n_plane; ++q) {
+    const PlaneParams& pp = pl->planes[q];
+    io.pl[q].snap = pp.snap;
+    io.pl[q].axis = pp.axis;
+    io.pl[q].index = pp.index;
+    io.pl[q].comp = pl->plane_comp[q];
+    io.pl[q].n2 = pp.n2;
+    io.pl[q].npts = pp.n1 * pp.n2;
+  }
+  const dim3 g((unsigned)((long long)io.ntj * io.ntk * nseg)), blk(kBX, kTBW);
+  const PsiIO* psp = pl->d_psio + 2 + from;   // read set `from`, write set `to`
+  if (ev) HIPCHK(hipEventRecord(ev[0], s));
+  if (kp.graded) hipLaunchKernelGGL(k_step_tb2<true>, g, blk, 0, s, kp, io, psp);
+  else hipLaunchKernelGGL(k_step_tb2<false>, g, blk, 0, s, kp, io, psp);
+  if (ev) {
+    HIPCHK(hipEventRecord(ev[1], s));
+    HIPCHK(hipEventRecord(ev[2], s));
+  }
+  for (size_t q = 0; q < pl->planes.size(); ++q) {
+    const PlaneParams& pp = pl->planes[q];
+    const long long npts = (long long)pp.n1 * pp.n2;
+    hipLaunchKernelGGL(k_dft_fold2, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp);
+  }
+  if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  hipLaunchKernelGGL(k_advance2, dim3(1), dim3(64), 0, s, kp.step);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2520,7 +3071,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     pp.ny = d->ny;
     pp.nz = d->nz;
     {
-      const std::vector<float> ring((size_t)kDftK * pp.n1 * pp.n2, 0.f);
+      const std::vector<float> ring((size_t)kDftRing * pp.n1 * pp.n2, 0.f);
       if ((rc = dev_copy(pl, ring.data(), ring.size(), &pp.snap))) return bail(rc);
     }
     std::vector<double2> tw((size_t)d->max_steps * pd.n_freq);
@@ -2688,6 +3239,76 @@ int rfxh_kernel_time_ms(rfxh_plan* pl, int which, double* avg_ms, int64_t* count
   return RFXH_OK;
 }
 
+// rfxh_run with the temporal-blocked march: pairs of steps per sweep (each
+// flips the field set), an odd last step on the two-pass kernels in place.
+static int run_tb(rfxh_plan* pl, hipStream_t s, int64_t n_steps, int use_graph) {
+  const int64_t pairs = n_steps / 2;
+  if (!use_graph || pl->timing || n_steps < pl->graph_chunk) {
+    const int64_t every = std::max<int64_t>(1, pairs / kTimingSamples);
+    int n_samples = 0;
+    for (int64_t q = 0; q < pairs; ++q) {
+      hipEvent_t* ev = nullptr;
+      if (pl->timing && q % every == every - 1 && n_samples < kTimingSamples)
+        ev = pl->tev.data() + 4 * (n_samples++);
+      const int rc = launch_tb(pl, s, ev, pl->cur);
+      if (rc) return rc;
+      pl->cur ^= 1;
+    }
+    if (n_steps & 1) {
+      const int rc = launch_step(pl, s, nullptr, pl->cur);
+      if (rc) return rc;
+    }
+    dft_flush(pl, s);
+    HIPCHK(hipGetLastError());
+    if (n_samples) return collect_timing(pl, n_samples);
+    return RFXH_OK;
+  }
+  if (!pl->cap_stream) {
+    HIPCHK(hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&pl->ev_in, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&pl->ev_out, hipEventDisableTiming));
+  }
+  const int g = pl->cur;
+  const int sweeps = pl->graph_chunk / 2;   // even: a chunk returns the state to set g
+  if (!pl->graph_exec[g]) {
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(pl->cap_stream, hipStreamCaptureModeThreadLocal));
+    int from = g;
+    for (int q = 0; q < sweeps; ++q) {
+      const int rc = launch_tb(pl, pl->cap_stream, nullptr, from);
+      if (rc) {
+        (void)hipStreamEndCapture(pl->cap_stream, &graph);
+        return rc;
+      }
+      from ^= 1;
+    }
+    HIPCHK(hipStreamEndCapture(pl->cap_stream, &graph));
+    HIPCHK(hipGraphInstantiate(&pl->graph_exec[g], graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+  }
+  HIPCHK(hipEventRecord(pl->ev_in, s));
+  HIPCHK(hipStreamWaitEvent(pl->cap_stream, pl->ev_in, 0));
+  int64_t done = 0;
+  while (n_steps - done >= 2 * sweeps) {
+    HIPCHK(hipGraphLaunch(pl->graph_exec[g], pl->cap_stream));
+    done += 2 * sweeps;
+  }
+  for (; n_steps - done >= 2; done += 2) {
+    const int rc = launch_tb(pl, pl->cap_stream, nullptr, pl->cur);
+    if (rc) return rc;
+    pl->cur ^= 1;
+  }
+  if (done < n_steps) {
+    const int rc = launch_step(pl, pl->cap_stream, nullptr, pl->cur);
+    if (rc) return rc;
+  }
+  dft_flush(pl, pl->cap_stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(pl->ev_out, pl->cap_stream));
+  HIPCHK(hipStreamWaitEvent(s, pl->ev_out, 0));
+  return RFXH_OK;
+}
+
 int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int use_graph) {
   if (!pl) return fail(RFXH_EINVAL, "null plan");
   if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
@@ -2697,6 +3318,7 @@ int rfxh_run(rfxh_plan* pl, int64_t step0, int64_t n_steps, void* stream, int us
   dft_begin(pl, s, step0);
   HIPCHK(hipGetLastError());
   if (n_steps == 0) return RFXH_OK;
+  if (pl->tb) return run_tb(pl, s, n_steps, use_graph);
   const int flip = pl->fused ? 1 : 0;
   if (!use_graph || pl->timing || n_steps < pl->graph_chunk) {
     // timing: sample up to kTimingSamples evenly spaced steps; the host does
@@ -2801,7 +3423,32 @@ static PsiIO psi_io(const rfxh_plan* pl, int set) {
 }
 
 int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
-  if (!pl || kernel < 0 || kernel > 7) return fail(RFXH_EINVAL, "bad variant");
+  if (!pl || kernel < 0 || kernel > 8) return fail(RFXH_EINVAL, "bad variant");
+  if (kernel == 8) {
+    if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
+    if (!pl->set[1][0]) return fail(RFXH_EINVAL, "the temporal-blocked march needs a second field set");
+    if (pl->n_pole > 0)
+      return fail(RFXH_EINVAL, "dispersive (Debye / Lorentz) materials run on the two-pass kernels (0, 4, 5, 6)");
+    if (pl->n_mat > kTBMat)
+      return fail(RFXH_EINVAL, "the temporal-blocked march takes at most 1024 materials");
+    if ((int)pl->planes.size() > kTBPlanes)
+      return fail(RFXH_EINVAL, "the temporal-blocked march stages at most 4 DFT planes");
+    if (!pl->fluxes.empty())
+      return fail(RFXH_EINVAL, "flux monitors and NTFF boxes run on the single-step kernels");
+    const int N[3] = {pl->kp.nx, pl->kp.ny, pl->kp.nz};
+    for (int a = 0; a < 3; ++a)
+      if (pl->kp.cn[2 * a] + pl->kp.cn[2 * a + 1] > N[a])
+        return fail(RFXH_EINVAL, "the temporal-blocked march needs CPML slabs that do not overlap");
+    pl->tb = true;
+    pl->tb_seg = param;
+    pl->fused = pl->march = pl->wmarch = false;
+    for (int g = 0; g < 2; ++g)
+      if (pl->graph_exec[g]) {
+        (void)hipGraphExecDestroy(pl->graph_exec[g]);
+        pl->graph_exec[g] = nullptr;
+      }
+    return RFXH_OK;
+  }
   const bool pingpong = kernel == 1 || kernel == 2 || kernel == 3 || kernel == 7;
   if (pingpong && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
@@ -2833,6 +3480,7 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
   pl->fused = pingpong;
   pl->march = kernel == 2 || kernel == 3;
   pl->wmarch = kernel == 7;
+  pl->tb = false;
   for (int g = 0; g < 2; ++g)
     if (pl->graph_exec[g]) {
       (void)hipGraphExecDestroy(pl->graph_exec[g]);
@@ -2843,11 +3491,13 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
 
 int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
   if (!pl || !kernel || !param) return fail(RFXH_EINVAL, "bad arguments");
-  *kernel = pl->wmarch ? 7
+  *kernel = pl->tb ? 8
+          : pl->wmarch ? 7
           : pl->march ? 2 + pl->march_tile
           : pl->fused ? 1
           : pl->ncol == 2 ? 4 : pl->ncol == 4 ? 5 : pl->ncol == 3 ? 6 : 0;
-  *param = pl->wmarch ? pl->wmarch_seg : pl->march ? pl->march_seg : pl->kp.order;
+  *param = pl->tb ? pl->tb_seg
+         : pl->wmarch ? pl->wmarch_seg : pl->march ? pl->march_seg : pl->kp.order;
   return RFXH_OK;
 }
 

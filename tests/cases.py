@@ -342,3 +342,28 @@ def patch_case(n_xy=60):
                 ports=[((c - 0.003, c, z0), "ez", 50.0, o.Pulse(2.5e9, 1.2), True, 1.6e-3)],
                 probes=[((c + 0.008, c, z1 + 2e-3), "ez")],
                 s11_freqs=np.linspace(1e9, 4e9, 21))
+
+
+def tile_case(lane="run"):
+    """A uniform case larger than one tile of the temporal-blocked march in
+    y and z (57 x 47 x 92 cells: four tile rows, two tile columns): lossy,
+    magnetic and PEC boxes across tile edges, soft sources on three
+    components, a lumped port, E and H probes on and off tile edges, DFT
+    planes normal to x and z."""
+    return dict(freq_max=12e9, domain=(0.040, 0.030, 0.075), dx=1e-3, boundary="cpml",
+                cpml_layers=8,
+                materials={"d": (3.5, 0.03, 1.0), "m": (1.0, 1e8, 1.0), "mag": (2.0, 0.01, 1.7),
+                           "hi": (9.0, 0.0, 1.0)},
+                geometry=[((0.006, 0.004, 0.010), (0.020, 0.016, 0.040), "d"),
+                          ((0.022, 0.002, 0.050), (0.0235, 0.028, 0.0515), "m"),
+                          ((0.010, 0.018, 0.045), (0.030, 0.026, 0.070), "mag"),
+                          ((0.025, 0.008, 0.020), (0.035, 0.013, 0.030), "hi")],
+                ports=[((0.012, 0.010, 0.030), "ez", 0.0, o.Pulse(9e9, 0.8), True),
+                       ((0.030, 0.020, 0.055), "ex", 0.0, o.Pulse(7e9, 0.6), True),
+                       ((0.005, 0.025, 0.065), "ey", 0.0, o.Pulse(10e9, 0.9), True),
+                       ((0.028, 0.004, 0.015), "ez", 50.0, o.Pulse(8e9, 0.7), True)],
+                probes=[((0.020, 0.015, 0.038), "ez"), ((0.004, 0.004, 0.004), "ez"),
+                        ((0.015, 0.020, 0.052), "hx"), ((0.035, 0.027, 0.070), "ey"),
+                        ((0.018, 0.003, 0.060), "hz")],
+                dft_planes=[("z", 0.036, "ez", None, 4), ("x", 0.013, "hy", None, 3)],
+                s11_freqs=np.array([7e9, 9e9]), lane=lane)

@@ -57,7 +57,7 @@ def test_every_variant_matches_oracle_and_tuner_picks_one(R):
     for v in series:
         if v[0] in (4, 5, 6):                             # columns == one cell per thread
             assert np.array_equal(series[v], base), v
-        if v[0] in (2, 3, 7):                             # marching == fused arithmetic per cell
+        if v[0] in (2, 3, 7, 8):                          # marching == fused arithmetic per cell
             assert cases.rel_l2(series[v], series[(1, 1)]) <= 1e-6
     res = at.autotune(dr, trial_steps=16)
     print("autotune", res)
@@ -82,7 +82,7 @@ def test_march_graded_vs_oracle(R):
     ref = cstep.run_c(cfg, n, want_fields=False)
     dr = DeviceRun(prob, n, "cuda")
     for kern, seg in ((2, 0), (2, 1), (2, 3), (3, 0), (3, 7), (4, 1), (5, 0), (6, 1), (7, 0),
-                      (7, 1), (7, 5)):
+                      (7, 1), (7, 5), (8, 0), (8, 2)):
         at.set_variant(dr, kern, seg)
         dr.reset()
         dr.run(0, n)

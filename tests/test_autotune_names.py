@@ -24,13 +24,14 @@ def test_candidates_follow_the_plan():
     one = at.variants(_run(two_sets=False))
     assert {k for k, _ in one} == {0, 4, 5, 6}            # two-pass variants only
     two = at.variants(_run(two_sets=True))
-    assert {1, 2, 3, 7} <= {k for k, _ in two}
-    # the wave-march step stages at most 1024 material ids in LDS
+    assert {1, 2, 3, 7, 8} <= {k for k, _ in two}
+    # the wave-march and temporal-blocked steps stage at most 1024 material ids in LDS
     many = at.variants(_run(two_sets=True, n_mat=2000))
-    assert 7 not in {k for k, _ in many}
-    # segment counts never exceed a quarter of the planes
+    assert not {7, 8} & {k for k, _ in many}
+    # segment counts never exceed a quarter (an eighth for the two-step sweep) of the planes
     small = at.variants(_run(shape=(20, 8, 8), two_sets=True))
     assert all(seg <= 5 for k, seg in small if k in (2, 3, 7))
+    assert all(seg <= 2 for k, seg in small if k == 8)
 
 
 def test_known_names():
@@ -40,3 +41,5 @@ def test_known_names():
     assert at.variant_name((7, 0)) == "wmarch"
     assert at.parse_variant("wmarch-64") == (7, 64)
     assert at.parse_variant("col3-bands") == (6, 0)
+    assert at.variant_name((8, 3)) == "tb2-3"
+    assert at.parse_variant("tb2") == (8, 0)
