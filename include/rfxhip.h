@@ -204,8 +204,12 @@ int rfxh_kernel_time_ms(rfxh_plan* plan, int which, double* avg_ms, int64_t* cou
  * segments, 0 = automatic); 4 / 6 / 5: two-pass step with columns of
  * 2 / 3 / 4 cells along x per thread (param = workgroup order); 7: wave-march
  * step, each wavefront walking a pair of rows along x with no barriers
- * (param = x segments, 0 = automatic; at most 1024 material ids).  Kernels
- * 1-3 and 7 need the second field set.
+ * (param = x segments, 0 = automatic; at most 1024 material ids); 8:
+ * temporal-blocked march, two time steps per sweep (temporal-block depth 2;
+ * param = x segments, 0 = automatic; at most 1024 material ids and 4 DFT
+ * planes, no flux monitors, NTFF boxes or dispersion; an odd last step of
+ * a run takes the two-pass kernels; rfxh_kernel_time_ms which = 0 is then
+ * the two-step sweep).  Kernels 1-3, 7 and 8 need the second field set.
  * Variants compute the same step; they differ in speed only.  The
  * project's autotuner (rfx_amd/autotune.py) times them on the actual
  * problem.  No counterpart in the reference (its XLA lowering has no
