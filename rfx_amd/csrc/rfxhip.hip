@@ -1070,21 +1070,49 @@ struct TBPlane {
   int axis, index, comp, n2, npts;
 };
 
+// CPML memory and profiles of the sweep, addressed from one base pointer
+// per field set (a few scalar registers instead of one pointer per face
+// and term): face f's four terms (H0, H1, E0, E1) lie at psi + foff[f] +
+// t * fs[f], in the slab order of psi_idx; profile row l of face f is
+// prof[poff[f] + l] = (b, c, 1/kappa - 1, 0).
+struct TBCpml {
+  const float* psr;      // read set
+  float* psw;            // write set
+  const float4* prof;
+  unsigned foff[6];
+  unsigned fs[6];
+  int poff[6];
+};
+
+// x segments: [0, xa) holds the x-lo CPML slab (with the two planes past
+// it that a segment's first updates reach back to), [xb, nx) the x-hi slab;
+// [xa, xb) is cut into n_mid segments of seg_len planes.  A workgroup whose
+// planes and tile rows and columns (with their halos) miss every slab runs
+// the sweep without CPML code.
 struct TBIO {
   const float* eo[3];
   const float* ho[3];
   float* en[3];
   float* hn[3];
-  int ntk, ntj, seg_len, n_mat;
+  int ntk, ntj, n_mat;
+  int xa, xb, n_mid, seg_len;
   int n_plane;
   TBPlane pl[kTBPlanes];
+  TBCpml cm;
+};
+
+struct TBShared {
+  float4 coef[kTBMat];
+  uint32_t flag[kTBMat];
+  // [buffer][stage: E^n, H^{n+1/2}, E^{n+1}, H^{n+3/2}][component][row][lane]
+  float x[2][4][3][kTBW][kBX];
 };
 
 // H at one cell with the CPML memory in registers: h_core's update reading
 // the memory from pin and leaving the new memory in pout (and, when store,
 // in ps.hw).
-template <bool GRADED>
-__device__ __forceinline__ float3 tb_h(const KParams& p, const PsiIO& ps, int i, int j, int k,
+template <bool GRADED, bool CPML = true>
+__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, int i, int j, int k,
                                        float ch, float ex, float ey, float ez, float ex_j,
                                        float ez_j, float ex_k, float ey_k, float ey_i, float ez_i,
                                        float hx, float hy, float hz, const float (&pin)[3][2],
@@ -1100,14 +1128,15 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const PsiIO& ps, int i,
   hx = hx - ch * ((ez_j - ez) * iy - (ey_k - ey) * iz);
   hy = hy - ch * ((ex_k - ex) * iz - (ez_i - ez) * ix);
   hz = hz - ch * ((ey_i - ey) * ix - (ex_j - ex) * iy);
-  if (in_slab(p, i, j, k)) {
+  if (CPML && in_slab(p, i, j, k)) {
     __asm__ volatile("" ::: "memory");
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
       const int l = slab_layer(p, face, i, j, k);
       if (l < 0) continue;
       const int ax = face >> 1;
-      const float bb = gld(p.pb[face], l), cc = gld(p.pc[face], l), km1 = gld(p.pk[face], l);
+      const float4 pr = gld(cm.prof, cm.poff[face] + l);
+      const float bb = pr.x, cc = pr.y, km1 = pr.z;
       const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
@@ -1125,9 +1154,10 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const PsiIO& ps, int i,
       pout[ax][0] = s0;
       pout[ax][1] = s1;
       if (store) {
-        const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
-        stf(ps.hw[face][0], q, s0);
-        stf(ps.hw[face][1], q, s1);
+        const unsigned q =
+            cm.foff[face] + (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+        stf(cm.psw, q, s0);
+        stf(cm.psw, q + cm.fs[face], s1);
       }
       const float d0 = ch * s0 + ch * km1 * g0;
       const float d1 = ch * s1 + ch * km1 * g1;
@@ -1148,8 +1178,8 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const PsiIO& ps, int i,
 
 // E at one cell with the CPML memory in registers (e_core_co's update),
 // point operations at step n (recorded only when own).
-template <bool GRADED>
-__device__ __forceinline__ float3 tb_e(const KParams& p, const PsiIO& ps, int i, int j, int k,
+template <bool GRADED, bool CPML = true>
+__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, int i, int j, int k,
                                        unsigned c, float4 co, uint32_t fl, float e0, float e1,
                                        float e2, float hx, float hy, float hz, float hy_i,
                                        float hz_i, float hx_j, float hz_j, float hx_k, float hy_k,
@@ -1167,14 +1197,15 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const PsiIO& ps, int i,
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
   e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
   e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
-  if (in_slab(p, i, j, k)) {
+  if (CPML && in_slab(p, i, j, k)) {
     __asm__ volatile("" ::: "memory");
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
       const int l = slab_layer(p, face, i, j, k);
       if (l < 0) continue;
       const int ax = face >> 1;
-      const float bb = gld(p.pb[face], l), cc = gld(p.pc[face], l), km1 = gld(p.pk[face], l);
+      const float4 pr = gld(cm.prof, cm.poff[face] + l);
+      const float bb = pr.x, cc = pr.y, km1 = pr.z;
       const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
@@ -1192,9 +1223,10 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const PsiIO& ps, int i,
       pout[ax][0] = s0;
       pout[ax][1] = s1;
       if (store) {
-        const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
-        stf(ps.ew[face][0], q, s0);
-        stf(ps.ew[face][1], q, s1);
+        const unsigned q = cm.foff[face] + 2 * cm.fs[face] +
+                           (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+        stf(cm.psw, q, s0);
+        stf(cm.psw, q + cm.fs[face], s1);
       }
       const float d0 = ce * s0 + ce * km1 * g0;
       const float d1 = ce * s1 + ce * km1 * g1;
@@ -1230,35 +1262,16 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const PsiIO& ps, int i,
   return make_float3(e0, e1, e2);
 }
 
-template <bool GRADED>
-__global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
-                                                         const PsiIO* __restrict__ psp) {
-  // CPML pointers through device memory (read set A, write set B)
-  const PsiIO& ps = *psp;
-  __shared__ float4 s_coef[kTBMat];
-  __shared__ uint32_t s_flag[kTBMat];
-  // [buffer][stage: E^n, H^{n+1/2}, E^{n+1}, H^{n+3/2}][component][row][lane]
-  __shared__ float s_x[2][4][3][kTBW][kBX];
-  const int w = (int)threadIdx.y, l = (int)threadIdx.x;
-  const int tid = w * kBX + l;
-  // the plan rejects this variant for more than kTBMat materials
-  for (int t = tid; t < io.n_mat; t += kBX * kTBW) {
-    s_coef[t] = gld(p.coef, t);
-    s_flag[t] = gld(p.flags, t);
-  }
-  // the first iteration reads buffer 1 (no plane before the segment start)
-  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&s_x[1][0][0][0][0])[t] = 0.f;
-  __syncthreads();
-  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const int tk = L % io.ntk;
-  const int r = L / io.ntk;
-  const int tj = r % io.ntj;
-  const int seg = r / io.ntj;
-  const int j = tj * kTBJ - kTBHalo + w;
-  const int k = tk * kTBK - kTBHalo + l;
-  const int i0 = seg * io.seg_len;
-  const int i1 = min(p.nx, i0 + io.seg_len);
-  if (i0 >= i1) return;   // whole workgroup
+// The sweep of one workgroup over planes [i0, i1) of its tile; CPML =
+// false leaves out the memory terms and slab tests (the caller has checked
+// that the workgroup's needed cells miss every slab).
+template <bool GRADED, bool CPML>
+__device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBShared& sh, int w,
+                                          int l, int j, int k, int i0, int i1) {
+  const TBCpml& cm = io.cm;
+  float4* const s_coef = sh.coef;
+  uint32_t* const s_flag = sh.flag;
+  auto& s_x = sh.x;
   const bool valid = j >= 0 && j < p.ny && k >= 0 && k < p.nz;
   const bool interior = valid && w >= kTBHalo && w < kTBW - kTBHalo && l >= kTBHalo &&
                         l < kBX - kTBHalo;
@@ -1274,18 +1287,20 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
     const float v = s_x[b][st][cmp][ok ? ww : 0][ok ? ll : 0];
     return ok ? v : 0.f;
   };
-  // psi terms of the cell at plane q from the read set (zero outside slabs)
-  auto ld_psi = [&](float (&dst)[3][2], const float* const (&src)[6][2], int q) {
+  // memory terms (term0: 0 for H, 2 for E) of the cell at plane q from the
+  // read set (zero outside slabs)
+  auto ld_psi = [&](float (&dst)[3][2], unsigned term0, int q) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) dst[a][0] = dst[a][1] = 0.f;
-    if (pin(q) && in_slab(p, q, j, k)) {
+    if (CPML && pin(q) && in_slab(p, q, j, k)) {
 #pragma unroll
       for (int face = 0; face < 6; ++face) {
         const int ly = slab_layer(p, face, q, j, k);
         if (ly < 0) continue;
-        const unsigned qq = (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
-        dst[face >> 1][0] = ldf(src[face][0], qq);
-        dst[face >> 1][1] = ldf(src[face][1], qq);
+        const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
+                            (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
+        dst[face >> 1][0] = ldf(cm.psr, qq);
+        dst[face >> 1][1] = ldf(cm.psr, qq + cm.fs[face]);
       }
     }
   };
@@ -1335,8 +1350,8 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
       fh[c] = ldf(io.ho[c], op);
     }
     fm = ldu16(p.mat, op);
-    ld_psi(fph, ps.hr, pp);
-    ld_psi(fpe, ps.er, pp - 1);
+    ld_psi(fph, 0u, pp);
+    ld_psi(fpe, 2u, pp - 1);
     const bool vq = pin(q), vp = pin(pp);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1371,7 +1386,7 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
         const bool o = own(q);
         const unsigned c = (unsigned)q * P + c0;
         float dummy[3][2];
-        e2 = tb_e<GRADED>(p, ps, q, j, k, c, s_coef[m4], s_flag[m4], e1b[0], e1b[1], e1b[2],
+        e2 = tb_e<GRADED, CPML>(p, cm, q, j, k, c, s_coef[m4], s_flag[m4], e1b[0], e1b[1], e1b[2],
                           h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k, peb, dummy,
                           o, n + 1, o);
         if (o) {
@@ -1394,7 +1409,7 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
       if (pin(q)) {
         const bool o = own(q);
         const unsigned c = (unsigned)q * P + c0;
-        e1 = tb_e<GRADED>(p, ps, q, j, k, c, s_coef[m2], s_flag[m2], e0b[0], e0b[1], e0b[2],
+        e1 = tb_e<GRADED, CPML>(p, cm, q, j, k, c, s_coef[m2], s_flag[m2], e0b[0], e0b[1], e0b[2],
                           h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k, hy_k, pen,
                           pe_new, false, n, o);
         if (o && io.n_plane) stage_snap(q, 0, e1, n);
@@ -1410,7 +1425,7 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
         const bool o = own(q);
         const unsigned c = (unsigned)q * P + c0;
         float dummy[3][2];
-        h2 = tb_h<GRADED>(p, ps, q, j, k, chc > 0.f ? chc : s_coef[m3].w, e1a[0], e1a[1],
+        h2 = tb_h<GRADED, CPML>(p, cm, q, j, k, chc > 0.f ? chc : s_coef[m3].w, e1a[0], e1a[1],
                           e1a[2], ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2],
                           phb, dummy, o);
         if (o) {
@@ -1431,7 +1446,7 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
       const float ex_j = nb(rb, 0, 0, w + 1, l), ez_j = nb(rb, 0, 2, w + 1, l);
       const float ex_k = nb(rb, 0, 0, w, l + 1), ey_k = nb(rb, 0, 1, w, l + 1);
       if (pin(q)) {
-        h1 = tb_h<GRADED>(p, ps, q, j, k, chc > 0.f ? chc : s_coef[m1].w, e0c[0], e0c[1],
+        h1 = tb_h<GRADED, CPML>(p, cm, q, j, k, chc > 0.f ? chc : s_coef[m1].w, e0c[0], e0c[1],
                           e0c[2], ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1],
                           h0n[2], phn, ph_new, false);
         if (own(q) && io.n_plane) stage_snap(q, 3, h1, n);
@@ -1486,6 +1501,50 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io,
     m2 = m1;
     m1 = fm;
   }
+}
+
+// ANY_CPML = false: a plan without CPML.
+template <bool GRADED, bool ANY_CPML>
+__global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
+  __shared__ TBShared sh;
+  const int w = (int)threadIdx.y, l = (int)threadIdx.x;
+  const int tid = w * kBX + l;
+  // the plan rejects this variant for more than kTBMat materials
+  for (int t = tid; t < io.n_mat; t += kBX * kTBW) {
+    sh.coef[t] = gld(p.coef, t);
+    sh.flag[t] = gld(p.flags, t);
+  }
+  // the first iteration reads buffer 1 (no plane before the segment start)
+  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&sh.x[1][0][0][0][0])[t] = 0.f;
+  __syncthreads();
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int tk = L % io.ntk;
+  const int r = L / io.ntk;
+  const int tj = r % io.ntj;
+  const int seg = r / io.ntj;   // 0: x-lo slab, 1..n_mid: middle, n_mid + 1: x-hi slab
+  int i0, i1;
+  if (seg == 0) {
+    i0 = 0;
+    i1 = io.xa;
+  } else if (seg > io.n_mid) {
+    i0 = io.xb;
+    i1 = p.nx;
+  } else {
+    i0 = io.xa + (seg - 1) * io.seg_len;
+    i1 = min(io.xb, i0 + io.seg_len);
+  }
+  if (i0 >= i1) return;   // whole workgroup
+  const int j0 = tj * kTBJ, k0 = tk * kTBK;
+  const int j = j0 - kTBHalo + w;
+  const int k = k0 - kTBHalo + l;
+  // cells the interior's two steps need: planes [i0-2, i1], rows
+  // [j0-2, j0+kTBJ], columns [k0-2, k0+kTBK]
+  const bool slab =
+      ANY_CPML && ((p.cn[0] > 0 && i0 - 2 < p.slo[0]) || (p.cn[1] > 0 && i1 >= p.shi[0]) ||
+                   (p.cn[2] > 0 && j0 - 2 < p.slo[1]) || (p.cn[3] > 0 && j0 + kTBJ >= p.shi[1]) ||
+                   (p.cn[4] > 0 && k0 - 2 < p.slo[2]) || (p.cn[5] > 0 && k0 + kTBK >= p.shi[2]));
+  if (slab) tb2_sweep<GRADED, ANY_CPML>(p, io, sh, w, l, j, k, i0, i1);
+  else tb2_sweep<GRADED, false>(p, io, sh, w, l, j, k, i0, i1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2465,6 +2524,11 @@ struct rfxh_plan {
   int wmarch_seg = 0;           // its x segments (0: automatic)
   bool tb = false;              // two steps per sweep (temporal-blocked march), ping-pong
   int tb_seg = 0;               // its x segments (0: automatic)
+  float* psi_base = nullptr;    // CPML memory of set 0 (set 1 at + psi_set)
+  float4* cprof = nullptr;      // CPML profile rows (b, c, 1/kappa - 1, 0) of every face
+  int cprof_off[6] = {0, 0, 0, 0, 0, 0};
+  unsigned psi_foff[6] = {0, 0, 0, 0, 0, 0};  // float offset of each face's terms
+  unsigned psi_fs[6] = {0, 0, 0, 0, 0, 0};    // floats per term of each face
   float* set[2][6];             // field sets (set[1] null in two-pass mode)
   long long psi_set = 0;        // float offset of CPML set 1 from set 0
   long long* acc_from = nullptr;  // first step not yet folded into the DFT planes (device)
@@ -2727,18 +2791,24 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
 // Tiles and x segments of the temporal-blocked march.  Automatic segments:
 // at least about four workgroups per CU (one fits a CU), at least eight
 // planes per segment.
+// The x-slab segments and the middle ones (want_seg of them, 0: about
+// eight workgroups per CU, at least sixteen planes each).
 void tb_blocks(const KParams& kp, int want_seg, TBIO* io, int* nseg_out) {
   io->ntj = (kp.ny + kTBJ - 1) / kTBJ;
   io->ntk = (kp.nz + kTBK - 1) / kTBK;
   const long long tiles = (long long)io->ntj * io->ntk;
-  int nseg = want_seg;
-  if (nseg <= 0) {
-    nseg = (int)std::max<long long>(1, (1024 + tiles - 1) / tiles);
-    nseg = std::min(nseg, std::max(1, kp.nx / 8));
+  io->xa = kp.cn[0] > 0 ? std::min(kp.nx, kp.cn[0] + 2) : 0;
+  io->xb = kp.cn[1] > 0 ? std::max(io->xa, kp.nx - kp.cn[1] - 1) : kp.nx;
+  const int mid = io->xb - io->xa;
+  int nmid = want_seg;
+  if (nmid <= 0) {
+    nmid = (int)std::max<long long>(1, (2048 + tiles - 1) / tiles);
+    nmid = std::min(nmid, std::max(1, mid / 16));
   }
-  nseg = std::min(nseg, kp.nx);
-  io->seg_len = (kp.nx + nseg - 1) / nseg;
-  *nseg_out = (kp.nx + io->seg_len - 1) / io->seg_len;
+  nmid = std::max(1, std::min(nmid, std::max(1, mid)));
+  io->seg_len = std::max(1, (mid + nmid - 1) / nmid);
+  io->n_mid = mid > 0 ? (mid + io->seg_len - 1) / io->seg_len : 1;
+  *nseg_out = io->n_mid + 2;
 }
 
 // Two time steps starting from field set `from` (writes set 1-from): the
@@ -2769,11 +2839,26 @@ int launch_tb(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     io.pl[q].n2 = pp.n2;
     io.pl[q].npts = pp.n1 * pp.n2;
   }
+  if (pl->psi_base) {
+    io.cm.psr = pl->psi_base + from * pl->psi_set;   // read set `from`, write set `to`
+    io.cm.psw = pl->psi_base + to * pl->psi_set;
+  }
+  io.cm.prof = pl->cprof;
+  for (int f = 0; f < 6; ++f) {
+    io.cm.foff[f] = pl->psi_foff[f];
+    io.cm.fs[f] = pl->psi_fs[f];
+    io.cm.poff[f] = pl->cprof_off[f];
+  }
   const dim3 g((unsigned)((long long)io.ntj * io.ntk * nseg)), blk(kBX, kTBW);
-  const PsiIO* psp = pl->d_psio + 2 + from;   // read set `from`, write set `to`
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
-  if (kp.graded) hipLaunchKernelGGL(k_step_tb2<true>, g, blk, 0, s, kp, io, psp);
-  else hipLaunchKernelGGL(k_step_tb2<false>, g, blk, 0, s, kp, io, psp);
+  const bool cpml = pl->psi_base && pl->cprof;   // the plan has CPML faces
+  if (kp.graded) {
+    if (cpml) hipLaunchKernelGGL((k_step_tb2<true, true>), g, blk, 0, s, kp, io);
+    else hipLaunchKernelGGL((k_step_tb2<true, false>), g, blk, 0, s, kp, io);
+  } else {
+    if (cpml) hipLaunchKernelGGL((k_step_tb2<false, true>), g, blk, 0, s, kp, io);
+    else hipLaunchKernelGGL((k_step_tb2<false, false>), g, blk, 0, s, kp, io);
+  }
   if (ev) {
     HIPCHK(hipEventRecord(ev[1], s));
     HIPCHK(hipEventRecord(ev[2], s));
@@ -2961,10 +3046,17 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     kp.f[c] = d->d_f[c];
   }
   long long off = 0;
+  pl->psi_base = d->d_psi;
+  std::vector<float4> prof_rows;
   for (int f = 0; f < 6; ++f) {
     kp.cn[f] = d->cpml_n[f];
     kp.cinv[f] = d->cpml_inv_d[f];
     const long long s = face_slab_floats(d, f);
+    pl->psi_foff[f] = (unsigned)off;
+    pl->psi_fs[f] = (unsigned)s;
+    pl->cprof_off[f] = (int)prof_rows.size();
+    for (int l = 0; l < d->cpml_n[f]; ++l)
+      prof_rows.push_back(make_float4(d->cpml_b[f][l], d->cpml_c[f][l], d->cpml_km1[f][l], 0.f));
     if (d->cpml_n[f] > 0) {
       kp.psi_h[f][0] = d->d_psi + off;
       kp.psi_h[f][1] = d->d_psi + off + s;
@@ -2980,6 +3072,8 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     }
     off += 4 * s;
   }
+  if (!prof_rows.empty() && (rc = dev_copy(pl, prof_rows.data(), prof_rows.size(), &pl->cprof)))
+    return bail(rc);
   // Dispersive poles: cells whose material flags carry pole bits update their
   // polarisation in the box pol_lo + [0, pol_n)
   if (d->n_pole < 0 || d->n_pole > 16) return bail(fail(RFXH_EINVAL, "n_pole must be 0..16"));
@@ -3439,6 +3533,8 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
     for (int a = 0; a < 3; ++a)
       if (pl->kp.cn[2 * a] + pl->kp.cn[2 * a + 1] > N[a])
         return fail(RFXH_EINVAL, "the temporal-blocked march needs CPML slabs that do not overlap");
+    if ((long long)pl->psi_foff[5] + 4LL * pl->psi_fs[5] >= (1LL << 30))
+      return fail(RFXH_EINVAL, "the temporal-blocked march addresses CPML memory below 2^30 floats");
     pl->tb = true;
     pl->tb_seg = param;
     pl->fused = pl->march = pl->wmarch = false;

@@ -367,3 +367,22 @@ def tile_case(lane="run"):
                         ((0.018, 0.003, 0.060), "hz")],
                 dft_planes=[("z", 0.036, "ez", None, 4), ("x", 0.013, "hy", None, 3)],
                 s11_freqs=np.array([7e9, 9e9]), lane=lane)
+
+
+def tb_interior_case(lane="run"):
+    """A uniform case with tiles of the temporal-blocked march that miss every
+    CPML slab (43 x 34 x 130 cells, 4-layer CPML: the tile of rows 10-21 and
+    columns 60-119 runs its middle x segments without CPML code) beside
+    tiles that touch the slabs: boxes, sources, probes and a DFT plane on
+    both kinds of tile."""
+    return dict(freq_max=15e9, domain=(0.034, 0.025, 0.121), dx=1e-3, boundary="cpml",
+                cpml_layers=4,
+                materials={"d": (4.0, 0.02, 1.0), "mag": (1.5, 0.0, 2.5), "m": (1.0, 1e8, 1.0)},
+                geometry=[((0.008, 0.007, 0.060), (0.026, 0.017, 0.090), "d"),
+                          ((0.004, 0.002, 0.010), (0.012, 0.022, 0.030), "mag"),
+                          ((0.020, 0.009, 0.100), (0.021, 0.016, 0.110), "m")],
+                ports=[((0.017, 0.012, 0.075), "ez", 0.0, o.Pulse(10e9, 0.8), True),
+                       ((0.010, 0.020, 0.020), "ey", 0.0, o.Pulse(8e9, 0.7), True)],
+                probes=[((0.020, 0.012, 0.095), "ez"), ((0.017, 0.010, 0.070), "hy"),
+                        ((0.030, 0.004, 0.015), "ex")],
+                dft_planes=[("y", 0.012, "ez", None, 3)], lane=lane)

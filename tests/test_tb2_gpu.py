@@ -1,9 +1,13 @@
 """The temporal-blocked march (variant 8: two time steps per sweep) on an
 MI355X: probe series, port samples and DFT planes against the float64
-oracle (<= 1e-4 relative L2), against the single-step fused kernel (the
-same per-cell arithmetic, <= 1e-6), bit-identical over segmentations and
-between graph replay and direct launches, with an odd step count (the
-last step on the two-pass kernels), on a grid spanning several tiles, the
+oracle (<= 1e-4 relative L2) and against the single-step fused kernel
+(the same per-cell arithmetic with its multiply-adds formed in other
+places: within 4x the case's own float32 rounding level per output, the
+distance of the float32 C oracle from the float64 one), over
+segmentations, bit-
+identical between graph replay and direct launches, with odd step counts
+(the last step on the two-pass kernels), on grids spanning several tiles
+(with and without tiles that skip the CPML code, and with PEC walls), the
 mixed case on both lanes and the graded mixed case."""
 import sys
 
@@ -53,12 +57,28 @@ def _vs_oracle(out, ref, tol=TOL):
         assert e <= tol, ("plane", q, e)
 
 
-def _close(a, b, tol=1e-6):
-    assert cases.rel_l2(a["ts"], b["ts"]) <= tol
-    for x, y in zip(a["planes"], b["planes"]):
-        assert cases.rel_l2(x.view(np.float64), y.view(np.float64)) <= tol
-    for x, y in zip(a["fields"], b["fields"]):
-        assert cases.rel_l2(x, y) <= tol
+def _f32_levels(cfg, n):
+    """Relative L2 distance of the float32 C oracle from the float64 one on
+    this case, per output (series, each DFT plane, each final field): the
+    rounding level two float32 step kernels that form their multiply-adds
+    differently can drift apart by."""
+    a = cstep.run_c(cfg, n, double=True)
+    b = cstep.run_c(cfg, n, double=False)
+    return dict(ts=cases.rel_l2(b.time_series, a.time_series),
+                planes=[cases.rel_l2(x.view(np.float64), y.view(np.float64))
+                        for x, y in zip(b.dft_planes, a.dft_planes)],
+                fields=[cases.rel_l2(b.fields[c], a.fields[c]) for c in o.COMPS])
+
+
+def _close(a, b, lv):
+    """a and b within 4x the float32 rounding level of each output (at
+    least 1e-6)."""
+    tol = lambda x: max(1e-6, 4.0 * x)   # noqa: E731
+    assert cases.rel_l2(a["ts"], b["ts"]) <= tol(lv["ts"])
+    for x, y, t in zip(a["planes"], b["planes"], lv["planes"]):
+        assert cases.rel_l2(x.view(np.float64), y.view(np.float64)) <= tol(t)
+    for c, (x, y, t) in enumerate(zip(a["fields"], b["fields"], lv["fields"])):
+        assert cases.rel_l2(x, y) <= tol(t), (o.COMPS[c], cases.rel_l2(x, y), t)
 
 
 def _same(a, b):
@@ -70,34 +90,55 @@ def _same(a, b):
         np.testing.assert_array_equal(x, y)
 
 
-def test_tb2_tile_case(R):
+def _case_vs_fused(spec, n, segs, shape=None):
+    """The sweep at several middle-segment counts vs the oracle (1e-4) and
+    vs the fused single step (4x the float32 rounding level of the case,
+    at least 1e-6); graph replay vs direct launches bit-identical."""
     from rfx_amd import autotune as at
     from rfx_amd.engine import DeviceRun
-    spec = cases.tile_case()
-    n = 241
-    _say(f"tb2 tile case {n} steps")
     sim = cases.to_sim(spec)
     grid, prob, _ = sim._build_problem(n, "run")
     g, cfg = o.build_uniform(spec, n)
-    assert tuple(g.shape) == tuple(grid.shape) == (57, 47, 92)
+    if shape is not None:
+        assert tuple(g.shape) == tuple(grid.shape) == shape
     ref = cstep.run_c(cfg, n)
+    lv = _f32_levels(cfg, n)
     dr = DeviceRun(prob, n, "cuda")
     at.set_variant(dr, 1, 1)
     fused = _outputs(dr)
     outs = {}
-    for seg in (0, 1, 4):
+    for seg in segs:
         at.set_variant(dr, 8, seg)
         assert at.get_variant(dr) == (8, seg)
-        _say(f"tb2 tile case segments {seg}")
+        _say(f"tb2 {grid.shape} middle segments {seg}")
         outs[seg] = _outputs(dr)
         _vs_oracle(outs[seg], ref)
-        _close(outs[seg], fused)
-    _same(outs[0], outs[1])
-    _same(outs[0], outs[4])
-    _say("tb2 tile case, direct launches")
-    at.set_variant(dr, 8, 0)
-    _same(_outputs(dr, use_graph=False), outs[0])
+        _close(outs[seg], fused, lv)
+        _close(outs[seg], outs[segs[0]], lv)
+    _say(f"tb2 {grid.shape}, direct launches")
+    at.set_variant(dr, 8, segs[0])
+    _same(_outputs(dr, use_graph=False), outs[segs[0]])
     dr.close()
+
+
+def test_tb2_tile_case(R):
+    _say("tb2 tile case 241 steps")
+    _case_vs_fused(cases.tile_case(), 241, (0, 1, 4), (57, 47, 92))
+
+
+def test_tb2_interior_tiles(R):
+    """Tiles that run their middle segments without CPML code beside tiles
+    that touch the slabs."""
+    _say("tb2 interior-tile case 200 steps")
+    _case_vs_fused(cases.tb_interior_case(), 200, (0, 1, 3), (43, 34, 130))
+
+
+def test_tb2_pec_walls(R):
+    """A plan without CPML: every workgroup takes the CPML-free sweep."""
+    spec = cases.tb_interior_case()
+    spec["boundary"], spec["cpml_layers"] = "pec", 0
+    _say("tb2 interior-tile case with PEC walls")
+    _case_vs_fused(spec, 160, (0, 2))
 
 
 @pytest.mark.parametrize("lane", ["run", "forward"])
@@ -119,7 +160,7 @@ def test_tb2_mixed_case(R, lane):
         if np.abs(ref.port_v[q]).max() > 0:
             assert cases.rel_l2(out["port"][:, q, 0], ref.port_v[q]) <= TOL
     at.set_variant(dr, 1, 1)
-    _close(out, _outputs(dr))
+    _close(out, _outputs(dr), _f32_levels(cfg, n))
     dr.close()
 
 
