@@ -302,6 +302,21 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* desc, int32_t n_theta, const d
                         int32_t n_phi, const double* phi, double* e_theta, double* e_phi,
                         void* stream);
 
+/* The same transform with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Arguments and results as rfxh_ntff_far_field (the direction
+ * vectors are computed on the device, so results agree with it to float64
+ * rounding, not bit for bit).  Its tables live in one file-scope device array
+ * per GPU of 1,048,576 doubles (8 MiB, held until the process ends), which
+ * must hold 9 per direction + 2 per frequency + 2 per grid index of each axis
+ * + the theta and phi values; larger requests return RFXH_EINVAL.  Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU (other host
+ * threads, other streams) race.  Waits for its stream before returning. */
+int rfxh_ntff_far_field_noalloc(const rfxh_farfield_desc* desc, int32_t n_theta,
+                                const double* theta, int32_t n_phi, const double* phi,
+                                double* e_theta, double* e_phi, void* stream);
+
 const char* rfxh_last_error(void);
 int rfxh_version(void);
 

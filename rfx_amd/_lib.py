@@ -104,7 +104,7 @@ EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
            "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
            "rfxh_batch_run", "rfxh_set_variant", "rfxh_get_variant", "rfxh_ntff_far_field",
-           "rfxh_last_error", "rfxh_version")
+           "rfxh_ntff_far_field_noalloc", "rfxh_last_error", "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -155,6 +155,8 @@ def load():
                                         C.POINTER(C.c_double), C.c_int32,
                                         C.POINTER(C.c_double), _P, _P, _P]
     lib.rfxh_ntff_far_field.restype = C.c_int
+    lib.rfxh_ntff_far_field_noalloc.argtypes = lib.rfxh_ntff_far_field.argtypes
+    lib.rfxh_ntff_far_field_noalloc.restype = C.c_int
     lib.rfxh_last_error.argtypes = []
     lib.rfxh_last_error.restype = C.c_char_p
     lib.rfxh_version.argtypes = []

@@ -38,6 +38,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "rfxhip.h"
@@ -2100,6 +2101,36 @@ __global__ __launch_bounds__(kFFBlock) void k_far_field(FarFieldParams q) {
   q.e_phi[o] = make_double2(-g * b.y, g * b.x);     //  j g b
 }
 
+// Direction vectors and wave numbers of a far-field transform computed on
+// the device (the allocation-free entry point has no host tables to
+// upload): dirs[9 t] = r-hat, theta-hat, phi-hat of direction t = it n_phi +
+// ip; kw[f] = 2 pi f / c0.
+__global__ __launch_bounds__(256) void k_far_field_dirs(const double* __restrict__ theta,
+                                                        int n_theta,
+                                                        const double* __restrict__ phi, int n_phi,
+                                                        const double* __restrict__ freqs, int nf,
+                                                        double c0, double* __restrict__ dirs,
+                                                        double* __restrict__ kw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n_dir = (long long)n_theta * n_phi;
+  if (t < n_dir) {
+    const int it = (int)(t / n_phi), ip = (int)(t % n_phi);
+    const double st = sin(theta[it]), ct = cos(theta[it]);
+    const double sp = sin(phi[ip]), cp = cos(phi[ip]);
+    double* o = dirs + 9 * t;
+    o[0] = st * cp;
+    o[1] = st * sp;
+    o[2] = ct;
+    o[3] = ct * cp;
+    o[4] = ct * sp;
+    o[5] = -st;
+    o[6] = -sp;
+    o[7] = cp;
+    o[8] = 0.0;
+  }
+  if (t < nf) kw[t] = 2.0 * M_PI * freqs[t] / c0;
+}
+
 // ---------------------------------------------------------------------------
 // Reverse mode.  The forward step is linear in the state (E, H, CPML memory)
 // for fixed materials, so its adjoint is the transpose of each sub-step run
@@ -3986,6 +4017,115 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const do
     rc = fail(RFXH_EHIP, "far field: kernel");
   (void)hipFree(blk);
   return rc;
+}
+
+}  // extern "C"
+
+// ---- far-field transform without allocation ---------------------------------
+// The tables of rfxh_ntff_far_field_noalloc live in one file-scope device
+// array per GPU (8 MiB of float64, held until the process ends).  Host code
+// never passes the array's name to a kernel: each device's address comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+constexpr long long kFFScratch = 1LL << 20;   // doubles
+static __device__ __align__(256) double scratch[kFFScratch];
+
+namespace {
+constexpr int kScratchDevices = 64;
+double* g_scratch[kScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// Address of this device's scratch array (the current device's copy).
+int scratch_address(double** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail(RFXH_EINVAL, "far field: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    double* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(RFXH_EHIP, "far field: no address for the scratch array");
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int rfxh_ntff_far_field_noalloc(const rfxh_farfield_desc* dsc, int32_t n_theta,
+                                const double* theta, int32_t n_phi, const double* phi,
+                                double* e_theta, double* e_phi, void* stream) {
+  if (!dsc || !theta || !phi || !e_theta || !e_phi || n_theta < 1 || n_phi < 1)
+    return fail(RFXH_EINVAL, "far field: missing arguments");
+  if (dsc->n_freq < 1 || dsc->n_freq > 65535 || !dsc->freqs)
+    return fail(RFXH_EINVAL, "far field: need 1..65535 frequencies");
+  for (int a = 0; a < 3; ++a) {
+    if (!dsc->pos[a] || !dsc->width[a] || dsc->n_axis[a] < 1)
+      return fail(RFXH_EINVAL, "far field: missing position / width table");
+    if (dsc->lo[a] < 0 || dsc->hi[a] <= dsc->lo[a] || dsc->hi[a] >= dsc->n_axis[a])
+      return fail(RFXH_EINVAL, "far field: need 0 <= lo < hi < n on every axis");
+  }
+  for (int face = 0; face < 6; ++face)
+    if (!dsc->d_acc[face]) return fail(RFXH_EINVAL, "far field: face accumulator is NULL");
+  const long long n_dir = (long long)n_theta * n_phi;
+  // layout: pos / width per axis, directions, wave numbers, theta, phi, freqs
+  long long total = 9 * n_dir + 2LL * dsc->n_freq + n_theta + n_phi;
+  for (int a = 0; a < 3; ++a) total += 2LL * dsc->n_axis[a];
+  if (total > kFFScratch)
+    return fail(RFXH_EINVAL,
+                "far field (no allocation): tables need " + std::to_string(total) +
+                    " doubles, more than the 1048576 of the scratch array (9 per direction, "
+                    "2 per frequency, 2 per grid index); use rfxh_ntff_far_field");
+  double* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FarFieldParams q;
+  std::memset(&q, 0, sizeof(q));
+  long long off = 0;
+  bool ok = true;
+  auto put = [&](const double* src, long long n) -> double* {
+    double* dst = base + off;
+    off += n;
+    if (src)
+      ok = ok && hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s) ==
+                     hipSuccess;
+    return dst;
+  };
+  for (int a = 0; a < 3; ++a) {
+    q.pos[a] = put(dsc->pos[a], dsc->n_axis[a]);
+    q.width[a] = put(dsc->width[a], dsc->n_axis[a]);
+  }
+  double* dirs = put(nullptr, 9 * n_dir);
+  double* kw = put(nullptr, dsc->n_freq);
+  const double* d_theta = put(theta, n_theta);
+  const double* d_phi = put(phi, n_phi);
+  const double* d_freq = put(dsc->freqs, dsc->n_freq);
+  if (!ok) return fail(RFXH_EHIP, "far field: table upload");
+  const long long nt = std::max<long long>(n_dir, dsc->n_freq);
+  hipLaunchKernelGGL(k_far_field_dirs, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s,
+                     d_theta, n_theta, d_phi, n_phi, d_freq, dsc->n_freq, dsc->c0, dirs, kw);
+  q.dirs = dirs;
+  q.kwave = kw;
+  for (int face = 0; face < 6; ++face) q.acc[face] = reinterpret_cast<const double2*>(dsc->d_acc[face]);
+  for (int a = 0; a < 3; ++a) {
+    q.lo[a] = dsc->lo[a];
+    q.hi[a] = dsc->hi[a];
+  }
+  q.nf = dsc->n_freq;
+  q.n_dir = (int)n_dir;
+  q.eta0 = dsc->eta0;
+  q.e_theta = reinterpret_cast<double2*>(e_theta);
+  q.e_phi = reinterpret_cast<double2*>(e_phi);
+  hipLaunchKernelGGL(k_far_field, dim3((unsigned)((n_dir + kFFBlock - 1) / kFFBlock),
+                                       (unsigned)dsc->n_freq), dim3(kFFBlock), 0, s, q);
+  if (hipGetLastError() != hipSuccess) return fail(RFXH_EHIP, "far field: launch");
+  // the host tables are read by the copies: wait for them (and the kernel),
+  // as rfxh_ntff_far_field does
+  HIPCHK(hipStreamSynchronize(s));
+  return RFXH_OK;
 }
 
 const char* rfxh_last_error(void) { return g_err.c_str(); }
