@@ -1075,10 +1075,13 @@ struct TBPlane {
 // per field set (a few scalar registers instead of one pointer per face
 // and term): face f's four terms (H0, H1, E0, E1) lie at psi + foff[f] +
 // t * fs[f], in the slab order of psi_idx; profile row l of face f is
-// prof[poff[f] + l] = (b, c, 1/kappa - 1, 0).
+// prof[poff[f] + l] = (b, c, 1/kappa - 1, 0).  The memory after the first
+// step goes through `mid` rather than registers: the H terms are written by
+// every copy of a cell (the same value), the E terms by the cell's owner.
 struct TBCpml {
   const float* psr;      // read set
   float* psw;            // write set
+  float* mid;            // memory after the first of the two steps (same layout)
   const float4* prof;
   unsigned foff[6];
   unsigned fs[6];
@@ -1288,20 +1291,33 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     const float v = s_x[b][st][cmp][ok ? ww : 0][ok ? ll : 0];
     return ok ? v : 0.f;
   };
-  // memory terms (term0: 0 for H, 2 for E) of the cell at plane q from the
-  // read set (zero outside slabs)
-  auto ld_psi = [&](float (&dst)[3][2], unsigned term0, int q) {
+  // memory terms (term0: 0 for H, 2 for E) of the cell at plane q from
+  // `base` (zero outside slabs), and their store
+  auto ld_psi = [&](float (&dst)[3][2], const float* base, unsigned term0, int q, bool use) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) dst[a][0] = dst[a][1] = 0.f;
-    if (CPML && pin(q) && in_slab(p, q, j, k)) {
+    if (CPML && use && pin(q) && in_slab(p, q, j, k)) {
 #pragma unroll
       for (int face = 0; face < 6; ++face) {
         const int ly = slab_layer(p, face, q, j, k);
         if (ly < 0) continue;
         const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
                             (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
-        dst[face >> 1][0] = ldf(cm.psr, qq);
-        dst[face >> 1][1] = ldf(cm.psr, qq + cm.fs[face]);
+        dst[face >> 1][0] = ldf(base, qq);
+        dst[face >> 1][1] = ldf(base, qq + cm.fs[face]);
+      }
+    }
+  };
+  auto st_psi = [&](float* base, unsigned term0, int q, const float (&src)[3][2], bool use) {
+    if (CPML && use && pin(q) && in_slab(p, q, j, k)) {
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        const int ly = slab_layer(p, face, q, j, k);
+        if (ly < 0) continue;
+        const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
+                            (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
+        stf(base, qq, src[face >> 1][0]);
+        stf(base, qq + cm.fs[face], src[face >> 1][1]);
       }
     }
   };
@@ -1325,22 +1341,14 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   float e1a[3], e1b[3];           // E^{n+1} of planes p-3, p-4
   float h2a[3], h2by, h2bz;       // H^{n+3/2} of plane p-4, (y, z) of p-5
   unsigned m1, m2, m3, m4;        // material ids of planes p-1 .. p-4
-  float phn[3][2], pen[3][2];     // memory read for A (plane p-1) and B (plane p-2)
-  float pha[3][2], phb[3][2];     // H memory after step 1, planes p-2, p-3
-  float pea[3][2], peb[3][2];     // E memory after step 1, planes p-3, p-4
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    e0c[c] = e0b[c] = h1a[c] = h1b[c] = e1a[c] = e1b[c] = h2a[c] = 0.f;
-#pragma unroll
-    for (int t = 0; t < 2; ++t) pha[c][t] = phb[c][t] = pea[c][t] = peb[c][t] = 0.f;
-  }
+  for (int c = 0; c < 3; ++c) e0c[c] = e0b[c] = h1a[c] = h1b[c] = e1a[c] = e1b[c] = h2a[c] = 0.f;
   h2by = h2bz = 0.f;
   m2 = m3 = m4 = 0u;
-  // prefetch for iteration pp + 1: E^n of plane pp+1; H^{n-1/2}, id and H
-  // memory of plane pp; E memory of plane pp-1
+  // prefetch for iteration pp + 1: E^n of plane pp+1; H^{n-1/2} and id of
+  // plane pp
   float fe[3], fh[3];
   unsigned fm;
-  float fph[3][2], fpe[3][2];
   auto prefetch = [&](int pp) {
     const int q = pp + 1;
     const unsigned oq = (unsigned)min(max(q, 0), p.nx - 1) * P + c0;
@@ -1351,8 +1359,6 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       fh[c] = ldf(io.ho[c], op);
     }
     fm = ldu16(p.mat, op);
-    ld_psi(fph, 0u, pp);
-    ld_psi(fpe, 2u, pp - 1);
     const bool vq = pin(q), vp = pin(pp);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1365,17 +1371,20 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   for (int c = 0; c < 3; ++c) {
     e0n[c] = fe[c];
     h0n[c] = fh[c];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      phn[c][t] = fph[c][t];
-      pen[c][t] = fpe[c][t];
-    }
   }
   m1 = fm;
 
   int it = 0;
   for (int pp = i0 - 2; pp < i1 + 4; ++pp, ++it) {
     const int rb = (it & 1) ^ 1, wb = it & 1;
+    // CPML memory of this iteration's four updates, issued before the
+    // prefetch so that waiting for it leaves the prefetch in flight (vector
+    // loads complete in order)
+    float pD[3][2], pB[3][2], pC[3][2], pA[3][2];
+    ld_psi(pD, cm.mid, 2u, pp - 4, own(pp - 4));
+    ld_psi(pB, cm.psr, 2u, pp - 2, true);
+    ld_psi(pC, cm.mid, 0u, pp - 3, true);
+    ld_psi(pA, cm.psr, 0u, pp - 1, true);
     prefetch(pp);
     // ---- D: E^{n+2} on plane pp-4
     float3 e2 = make_float3(0.f, 0.f, 0.f);
@@ -1388,7 +1397,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
         const unsigned c = (unsigned)q * P + c0;
         float dummy[3][2];
         e2 = tb_e<GRADED, CPML>(p, cm, q, j, k, c, s_coef[m4], s_flag[m4], e1b[0], e1b[1], e1b[2],
-                          h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k, peb, dummy,
+                          h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k, pD, dummy,
                           o, n + 1, o);
         if (o) {
           stf(io.en[0], c, e2.x);
@@ -1411,8 +1420,9 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
         const bool o = own(q);
         const unsigned c = (unsigned)q * P + c0;
         e1 = tb_e<GRADED, CPML>(p, cm, q, j, k, c, s_coef[m2], s_flag[m2], e0b[0], e0b[1], e0b[2],
-                          h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k, hy_k, pen,
+                          h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k, hy_k, pB,
                           pe_new, false, n, o);
+        st_psi(cm.mid, 2u, q, pe_new, o);    // only the owner takes the second E update
         if (o && io.n_plane) stage_snap(q, 0, e1, n);
       }
     }
@@ -1428,7 +1438,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
         float dummy[3][2];
         h2 = tb_h<GRADED, CPML>(p, cm, q, j, k, chc > 0.f ? chc : s_coef[m3].w, e1a[0], e1a[1],
                           e1a[2], ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2],
-                          phb, dummy, o);
+                          pC, dummy, o);
         if (o) {
           stf(io.hn[0], c, h2.x);
           stf(io.hn[1], c, h2.y);
@@ -1449,7 +1459,11 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       if (pin(q)) {
         h1 = tb_h<GRADED, CPML>(p, cm, q, j, k, chc > 0.f ? chc : s_coef[m1].w, e0c[0], e0c[1],
                           e0c[2], ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1],
-                          h0n[2], phn, ph_new, false);
+                          h0n[2], pA, ph_new, false);
+        // every copy whose H is right stores it (halo rows take the second H
+        // update too): not the tile's last row and lane (their outer
+        // neighbours read as zero), not the plane before the pipeline fills
+        st_psi(cm.mid, 0u, q, ph_new, q >= i0 - 2 && w < kTBW - 1 && l < kBX - 1);
         if (own(q) && io.n_plane) stage_snap(q, 3, h1, n);
       }
     }
@@ -1481,15 +1495,6 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       e0c[c] = e0n[c];
       e0n[c] = fe[c];
       h0n[c] = fh[c];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        peb[c][t] = pea[c][t];
-        pea[c][t] = pe_new[c][t];
-        phb[c][t] = pha[c][t];
-        pha[c][t] = ph_new[c][t];
-        phn[c][t] = fph[c][t];
-        pen[c][t] = fpe[c][t];
-      }
     }
     e1a[0] = e1.x;
     e1a[1] = e1.y;
@@ -2556,6 +2561,8 @@ struct rfxh_plan {
   bool tb = false;              // two steps per sweep (temporal-blocked march), ping-pong
   int tb_seg = 0;               // its x segments (0: automatic)
   float* psi_base = nullptr;    // CPML memory of set 0 (set 1 at + psi_set)
+  float* psi_mid = nullptr;     // the sweep's memory after its first step (owned, same layout)
+  long long psi_floats = 0;
   float4* cprof = nullptr;      // CPML profile rows (b, c, 1/kappa - 1, 0) of every face
   int cprof_off[6] = {0, 0, 0, 0, 0, 0};
   unsigned psi_foff[6] = {0, 0, 0, 0, 0, 0};  // float offset of each face's terms
@@ -2875,6 +2882,7 @@ int launch_tb(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     io.cm.psw = pl->psi_base + to * pl->psi_set;
   }
   io.cm.prof = pl->cprof;
+  io.cm.mid = pl->psi_mid;
   for (int f = 0; f < 6; ++f) {
     io.cm.foff[f] = pl->psi_foff[f];
     io.cm.fs[f] = pl->psi_fs[f];
@@ -3105,6 +3113,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   }
   if (!prof_rows.empty() && (rc = dev_copy(pl, prof_rows.data(), prof_rows.size(), &pl->cprof)))
     return bail(rc);
+  pl->psi_floats = off;
   // Dispersive poles: cells whose material flags carry pole bits update their
   // polarisation in the box pol_lo + [0, pol_n)
   if (d->n_pole < 0 || d->n_pole > 16) return bail(fail(RFXH_EINVAL, "n_pole must be 0..16"));
@@ -3566,6 +3575,12 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
         return fail(RFXH_EINVAL, "the temporal-blocked march needs CPML slabs that do not overlap");
     if ((long long)pl->psi_foff[5] + 4LL * pl->psi_fs[5] >= (1LL << 30))
       return fail(RFXH_EINVAL, "the temporal-blocked march addresses CPML memory below 2^30 floats");
+    if (pl->cprof && !pl->psi_mid) {
+      void* m = nullptr;
+      HIPCHK(hipMalloc(&m, (size_t)pl->psi_floats * sizeof(float)));
+      pl->owned.push_back(m);
+      pl->psi_mid = static_cast<float*>(m);
+    }
     pl->tb = true;
     pl->tb_seg = param;
     pl->fused = pl->march = pl->wmarch = false;
