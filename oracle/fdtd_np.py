@@ -276,12 +276,19 @@ def extend_cpml_pad(eps_r, sigma, mu_r, p):
 PEC_SIGMA_THRESHOLD = 1e6  # rfx/api/_compile.py:118
 
 
-def assemble_materials(shape, coords, geometry, materials, cpml_pad):
+def assemble_materials(shape, coords, geometry, materials, cpml_pad, sweep=None):
     """rfx/api/_compile.py:121-367 (no dispersion / thin conductors).
 
     geometry: list of (corner_lo, corner_hi, material_name) in insertion
     order (later wins); materials: name -> (eps_r, sigma, mu_r).
     Returns float32 eps_r, sigma, mu_r and a bool pec_mask or None.
+
+    sweep = (name or None, field, value): one element of a material sweep
+    (rfx/vmap_sweep.py:265-390 _build_batched_materials).  With a material
+    name, every cell inside that material's shapes (later shapes of other
+    materials included) takes the value before the CPML pads are extended
+    (:328-373); without one, every cell whose field differs from vacuum
+    (eps_r, mu_r != 1, sigma != 0) takes it after the extension (:374-401).
     """
     eps_r = np.ones(shape, F32)
     sigma = np.zeros(shape, F32)
@@ -299,8 +306,29 @@ def assemble_materials(shape, coords, geometry, materials, cpml_pad):
             eps_r[sel] = F32(e)
             sigma[sel] = F32(s)
             mu_r[sel] = F32(m)
+    arrays = {"eps_r": eps_r, "sigma": sigma, "mu_r": mu_r}
+    if sweep is not None and sweep[0] is not None:
+        name, fld, value = sweep
+        mask = np.zeros(shape, bool)
+        for lo, hi, nm in geometry:
+            if nm == name:
+                mask[np.ix_(*[np.nonzero(_axis_mask(coords[a], lo[a], hi[a]))[0]
+                              for a in range(3)])] = True
+        arrays[fld] = np.where(mask, F32(value), arrays[fld]).astype(F32)
+    eps_r, sigma, mu_r = arrays["eps_r"], arrays["sigma"], arrays["mu_r"]
     if cpml_pad > 0:
         eps_r, sigma, mu_r = extend_cpml_pad(eps_r, sigma, mu_r, cpml_pad)
+    if sweep is not None and sweep[0] is None:
+        _, fld, value = sweep
+        arr = {"eps_r": eps_r, "sigma": sigma, "mu_r": mu_r}[fld]
+        vac = F32(0.0) if fld == "sigma" else F32(1.0)
+        arr = np.where(arr != vac, F32(value), arr).astype(F32)
+        if fld == "eps_r":
+            eps_r = arr
+        elif fld == "sigma":
+            sigma = arr
+        else:
+            mu_r = arr
     return eps_r, sigma, mu_r, (pec if pec.any() else None)
 
 
@@ -1039,7 +1067,7 @@ def build_uniform(spec, n_steps):
     coords = g.node_coords_f32()
     eps_r, sigma, mu_r, pec = assemble_materials(
         shape, coords, spec.get("geometry", []), spec.get("materials", {}),
-        layers)
+        layers, spec.get("material_sweep"))
     lane = spec.get("lane", "run")
     if lane == "forward" and spec.get("eps_override") is not None:
         # _execute.py:2800-2805: override replaces eps_r wholesale

@@ -437,7 +437,13 @@ class Simulation:
         return Grid(self._freq_max, self._domain, self._dx, self._cpml_layers)
 
     def _build_problem(self, n_steps: int, lane: str, eps_override=None,
-                       sigma_override=None, mu_r_override=None, s11_freqs=None):
+                       sigma_override=None, mu_r_override=None, s11_freqs=None,
+                       material_sweep=None):
+        """Host set-up of one run.  ``material_sweep = (name, field, value)``
+        replaces one material property as rfx/vmap_sweep.py:265-390 does:
+        with a material name, the cells inside that material's shapes take
+        the value before the CPML pads are extended; without one, every
+        material whose property differs from vacuum takes it, pads included."""
         grid = self._build_grid()
         graded = self._graded
         shape = grid.shape
@@ -453,8 +459,17 @@ class Simulation:
                 pec[np.ix_(*[np.nonzero(x)[0] for x in masks])] = True
             else:
                 mat.fill_box(masks, m.eps_r, m.sigma, m.mu_r)
+        if material_sweep is not None and material_sweep[0] is not None:
+            sw_name, sw_field, sw_value = material_sweep
+            sel = np.zeros(shape, bool)
+            for box, name in self._geometry:
+                if name == sw_name:
+                    sel[np.ix_(*[np.nonzero(x)[0] for x in box.axis_masks(coords)])] = True
+            mat.set_field_where(sel, sw_field, sw_value)
         if self._cpml_layers > 0:
             mat.extend_pads(grid.face_pads)
+        if material_sweep is not None and material_sweep[0] is None:
+            mat.set_field_all(material_sweep[1], material_sweep[2])
         # Debye pole masks (rfx/api/_compile.py:203-232, 352): per pole, keyed by
         # its value in order of first appearance, OR-ed over every shape of a
         # material listing it (a later shape does not remove it), not extended

@@ -1065,6 +1065,7 @@ constexpr int kTBJ = kTBW - 2 * kTBHalo;    // interior rows of a tile
 constexpr int kTBK = kBX - 2 * kTBHalo;     // interior columns of a tile
 constexpr int kTBMat = 1024;                // material table entries staged in LDS
 constexpr int kTBPlanes = 4;                // DFT planes the sweep stages itself
+constexpr int kTBProf = 256;                // CPML profile rows staged in LDS (all faces)
 
 struct TBPlane {
   float* snap;     // ring [kDftRing][npts]
@@ -1100,14 +1101,18 @@ struct TBIO {
   float* hn[3];
   int ntk, ntj, n_mat;
   int xa, xb, n_mid, seg_len;
+  int n_prof;            // CPML profile rows (all faces)
   int n_plane;
   TBPlane pl[kTBPlanes];
   TBCpml cm;
 };
 
 struct TBShared {
-  float4 coef[kTBMat];
-  uint32_t flag[kTBMat];
+  float4 ecoef[kTBMat];   // ca, cb, ce, flag bits (as float bits): the E update's row
+  float ch[kTBMat];
+  // CPML profile rows: a global load inside an update would wait for every
+  // plane prefetch issued before it (vector loads complete in order)
+  float4 prof[kTBProf];
   // [buffer][stage: E^n, H^{n+1/2}, E^{n+1}, H^{n+3/2}][component][row][lane]
   float x[2][4][3][kTBW][kBX];
 };
@@ -1116,16 +1121,19 @@ struct TBShared {
 // the memory from pin and leaving the new memory in pout (and, when store,
 // in ps.hw).
 template <bool GRADED, bool CPML = true>
-__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, int i, int j, int k,
+__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const float4* prof,
+                                       int i, int j, int k, float gy, float gz,
                                        float ch, float ex, float ey, float ez, float ex_j,
                                        float ez_j, float ex_k, float ey_k, float ey_i, float ez_i,
                                        float hx, float hy, float hz, const float (&pin)[3][2],
                                        float (&pout)[3][2], bool store) {
+  // graded: gy, gz are the thread's own inverse spacings along y and z
+  // (loaded once), the x one is wave-uniform
   float ix, iy, iz;
   if (GRADED) {
     ix = gld(p.inv_h[0], i);
-    iy = gld(p.inv_h[1], j);
-    iz = gld(p.inv_h[2], k);
+    iy = gy;
+    iz = gz;
   } else {
     ix = iy = iz = p.inv_h0;
   }
@@ -1139,7 +1147,7 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, int i
       const int l = slab_layer(p, face, i, j, k);
       if (l < 0) continue;
       const int ax = face >> 1;
-      const float4 pr = gld(cm.prof, cm.poff[face] + l);
+      const float4 pr = prof[cm.poff[face] + l];
       const float bb = pr.x, cc = pr.y, km1 = pr.z;
       const float inv = p.cinv[face];
       float g0, g1;
@@ -1183,7 +1191,8 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, int i
 // E at one cell with the CPML memory in registers (e_core_co's update),
 // point operations at step n (recorded only when own).
 template <bool GRADED, bool CPML = true>
-__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, int i, int j, int k,
+__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const float4* prof,
+                                       int i, int j, int k, float gy, float gz,
                                        unsigned c, float4 co, uint32_t fl, float e0, float e1,
                                        float e2, float hx, float hy, float hz, float hy_i,
                                        float hz_i, float hx_j, float hz_j, float hx_k, float hy_k,
@@ -1192,8 +1201,8 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, int i
   float ix, iy, iz;
   if (GRADED) {
     ix = gld(p.inv_e[0], i);
-    iy = gld(p.inv_e[1], j);
-    iz = gld(p.inv_e[2], k);
+    iy = gy;
+    iz = gz;
   } else {
     ix = iy = iz = p.inv_e0;
   }
@@ -1208,7 +1217,7 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, int i
       const int l = slab_layer(p, face, i, j, k);
       if (l < 0) continue;
       const int ax = face >> 1;
-      const float4 pr = gld(cm.prof, cm.poff[face] + l);
+      const float4 pr = prof[cm.poff[face] + l];
       const float bb = pr.x, cc = pr.y, km1 = pr.z;
       const float inv = p.cinv[face];
       float g0, g1;
@@ -1273,8 +1282,8 @@ template <bool GRADED, bool CPML>
 __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBShared& sh, int w,
                                           int l, int j, int k, int i0, int i1) {
   const TBCpml& cm = io.cm;
-  float4* const s_coef = sh.coef;
-  uint32_t* const s_flag = sh.flag;
+  const float4* const s_ecoef = sh.ecoef;
+  const float* const s_ch = sh.ch;
   auto& s_x = sh.x;
   const bool valid = j >= 0 && j < p.ny && k >= 0 && k < p.nz;
   const bool interior = valid && w >= kTBHalo && w < kTBW - kTBHalo && l >= kTBHalo &&
@@ -1283,6 +1292,14 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
   const long long n = *p.step;
   const float chc = p.ch_all;
+  // graded lane: this thread's inverse spacings along y and z
+  float ghy = 0.f, ghz = 0.f, gey = 0.f, gez = 0.f;
+  if (GRADED && valid) {
+    ghy = gld(p.inv_h[1], j);
+    ghz = gld(p.inv_h[2], k);
+    gey = gld(p.inv_e[1], j);
+    gez = gld(p.inv_e[2], k);
+  }
   auto pin = [&](int q) { return valid && q >= 0 && q < p.nx; };
   auto own = [&](int q) { return interior && q >= i0 && q < i1; };
   // neighbour value of stage st from LDS buffer b; zero outside the tile
@@ -1321,10 +1338,19 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       }
     }
   };
+  // DFT planes this thread's (j, k) can lie on (x-normal planes: all), so
+  // that the waves without plane cells skip the staging
+  unsigned snap_bits = 0u;
+  for (int t = 0; t < io.n_plane; ++t) {
+    const TBPlane& d = io.pl[t];
+    const bool hit = d.axis == 0 || (d.axis == 1 ? j == d.index : k == d.index);
+    if (valid && hit) snap_bits |= 1u << t;
+  }
   auto stage_snap = [&](int q, int cmp0, const float3 v, long long st) {
+    if (!snap_bits) return;
     for (int t = 0; t < io.n_plane; ++t) {
       const TBPlane& d = io.pl[t];
-      if (d.comp < cmp0 || d.comp >= cmp0 + 3) continue;
+      if (!((snap_bits >> t) & 1u) || d.comp < cmp0 || d.comp >= cmp0 + 3) continue;
       const int pos = d.axis == 0 ? q : d.axis == 1 ? j : k;
       if (pos != d.index) continue;
       // plane cell (u, v): the two other indices in axis order
@@ -1389,22 +1415,24 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     // ---- D: E^{n+2} on plane pp-4
     float3 e2 = make_float3(0.f, 0.f, 0.f);
     {
+      // every lane computes (no branch); lanes outside the grid or the
+      // planes keep zero, the boundary value their neighbours need
       const int q = pp - 4;
       const float hx_j = nb(rb, 3, 0, w - 1, l), hz_j = nb(rb, 3, 2, w - 1, l);
       const float hx_k = nb(rb, 3, 0, w, l - 1), hy_k = nb(rb, 3, 1, w, l - 1);
-      if (pin(q)) {
-        const bool o = own(q);
-        const unsigned c = (unsigned)q * P + c0;
-        float dummy[3][2];
-        e2 = tb_e<GRADED, CPML>(p, cm, q, j, k, c, s_coef[m4], s_flag[m4], e1b[0], e1b[1], e1b[2],
-                          h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k, pD, dummy,
-                          o, n + 1, o);
-        if (o) {
-          stf(io.en[0], c, e2.x);
-          stf(io.en[1], c, e2.y);
-          stf(io.en[2], c, e2.z);
-          if (io.n_plane) stage_snap(q, 0, e2, n + 1);
-        }
+      const bool o = own(q);
+      const unsigned c = (unsigned)q * P + c0;
+      float dummy[3][2];
+      const float4 row = s_ecoef[m4];
+      e2 = tb_e<GRADED, CPML>(p, cm, sh.prof, q, j, k, gey, gez, c, row, __float_as_uint(row.w), e1b[0], e1b[1],
+                              e1b[2], h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k,
+                              pD, dummy, o, n + 1, o);
+      if (!pin(q)) e2 = make_float3(0.f, 0.f, 0.f);
+      if (o) {
+        stf(io.en[0], c, e2.x);
+        stf(io.en[1], c, e2.y);
+        stf(io.en[2], c, e2.z);
+        stage_snap(q, 0, e2, n + 1);
       }
     }
     // ---- B: E^{n+1} on plane pp-2
@@ -1416,15 +1444,15 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const int q = pp - 2;
       const float hx_j = nb(rb, 1, 0, w - 1, l), hz_j = nb(rb, 1, 2, w - 1, l);
       const float hx_k = nb(rb, 1, 0, w, l - 1), hy_k = nb(rb, 1, 1, w, l - 1);
-      if (pin(q)) {
-        const bool o = own(q);
-        const unsigned c = (unsigned)q * P + c0;
-        e1 = tb_e<GRADED, CPML>(p, cm, q, j, k, c, s_coef[m2], s_flag[m2], e0b[0], e0b[1], e0b[2],
-                          h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k, hy_k, pB,
-                          pe_new, false, n, o);
-        st_psi(cm.mid, 2u, q, pe_new, o);    // only the owner takes the second E update
-        if (o && io.n_plane) stage_snap(q, 0, e1, n);
-      }
+      const bool o = own(q);
+      const unsigned c = (unsigned)q * P + c0;
+      const float4 row = s_ecoef[m2];
+      e1 = tb_e<GRADED, CPML>(p, cm, sh.prof, q, j, k, gey, gez, c, row, __float_as_uint(row.w), e0b[0], e0b[1],
+                              e0b[2], h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k,
+                              hy_k, pB, pe_new, false, n, o);
+      if (!pin(q)) e1 = make_float3(0.f, 0.f, 0.f);
+      st_psi(cm.mid, 2u, q, pe_new, o);    // only the owner takes the second E update
+      if (o) stage_snap(q, 0, e1, n);
     }
     // ---- C: H^{n+3/2} on plane pp-3
     float3 h2 = make_float3(0.f, 0.f, 0.f);
@@ -1432,19 +1460,18 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const int q = pp - 3;
       const float ex_j = nb(rb, 2, 0, w + 1, l), ez_j = nb(rb, 2, 2, w + 1, l);
       const float ex_k = nb(rb, 2, 0, w, l + 1), ey_k = nb(rb, 2, 1, w, l + 1);
-      if (pin(q)) {
-        const bool o = own(q);
-        const unsigned c = (unsigned)q * P + c0;
-        float dummy[3][2];
-        h2 = tb_h<GRADED, CPML>(p, cm, q, j, k, chc > 0.f ? chc : s_coef[m3].w, e1a[0], e1a[1],
-                          e1a[2], ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2],
-                          pC, dummy, o);
-        if (o) {
-          stf(io.hn[0], c, h2.x);
-          stf(io.hn[1], c, h2.y);
-          stf(io.hn[2], c, h2.z);
-          if (io.n_plane) stage_snap(q, 3, h2, n + 1);
-        }
+      const bool o = own(q);
+      const unsigned c = (unsigned)q * P + c0;
+      float dummy[3][2];
+      h2 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m3], e1a[0], e1a[1], e1a[2],
+                              ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2], pC,
+                              dummy, o);
+      if (!pin(q)) h2 = make_float3(0.f, 0.f, 0.f);
+      if (o) {
+        stf(io.hn[0], c, h2.x);
+        stf(io.hn[1], c, h2.y);
+        stf(io.hn[2], c, h2.z);
+        stage_snap(q, 3, h2, n + 1);
       }
     }
     // ---- A: H^{n+1/2} on plane pp-1
@@ -1456,16 +1483,15 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const int q = pp - 1;
       const float ex_j = nb(rb, 0, 0, w + 1, l), ez_j = nb(rb, 0, 2, w + 1, l);
       const float ex_k = nb(rb, 0, 0, w, l + 1), ey_k = nb(rb, 0, 1, w, l + 1);
-      if (pin(q)) {
-        h1 = tb_h<GRADED, CPML>(p, cm, q, j, k, chc > 0.f ? chc : s_coef[m1].w, e0c[0], e0c[1],
-                          e0c[2], ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1],
-                          h0n[2], pA, ph_new, false);
-        // every copy whose H is right stores it (halo rows take the second H
-        // update too): not the tile's last row and lane (their outer
-        // neighbours read as zero), not the plane before the pipeline fills
-        st_psi(cm.mid, 0u, q, ph_new, q >= i0 - 2 && w < kTBW - 1 && l < kBX - 1);
-        if (own(q) && io.n_plane) stage_snap(q, 3, h1, n);
-      }
+      h1 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m1], e0c[0], e0c[1], e0c[2],
+                              ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1], h0n[2], pA,
+                              ph_new, false);
+      if (!pin(q)) h1 = make_float3(0.f, 0.f, 0.f);
+      // every copy whose H is right stores it (halo rows take the second H
+      // update too): not the tile's last row and lane (their outer
+      // neighbours read as zero), not the plane before the pipeline fills
+      st_psi(cm.mid, 0u, q, ph_new, q >= i0 - 2 && w < kTBW - 1 && l < kBX - 1);
+      if (own(q)) stage_snap(q, 3, h1, n);
     }
     // ---- publish this iteration's planes for the neighbours
     s_x[wb][0][0][w][l] = e0n[0];
@@ -1517,9 +1543,11 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
   const int tid = w * kBX + l;
   // the plan rejects this variant for more than kTBMat materials
   for (int t = tid; t < io.n_mat; t += kBX * kTBW) {
-    sh.coef[t] = gld(p.coef, t);
-    sh.flag[t] = gld(p.flags, t);
+    sh.ecoef[t] = gld(p.ecoef, t);
+    sh.ch[t] = ld_ch(p.coef, (unsigned)t);
   }
+  // the plan rejects this variant for more than kTBProf CPML layers in all
+  for (int t = tid; t < io.n_prof; t += kBX * kTBW) sh.prof[t] = gld(io.cm.prof, t);
   // the first iteration reads buffer 1 (no plane before the segment start)
   for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&sh.x[1][0][0][0][0])[t] = 0.f;
   __syncthreads();
@@ -2830,7 +2858,9 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
 // at least about four workgroups per CU (one fits a CU), at least eight
 // planes per segment.
 // The x-slab segments and the middle ones (want_seg of them, 0: about
-// eight workgroups per CU, at least sixteen planes each).
+// 6144 workgroups, at least 24 planes each; measured at 533^3, 16 middle
+// segments (7290 workgroups) beat 4 by 22 % and 8 by 7 %: the workgroups
+// over CPML slabs take longer, and many small workgroups balance them).
 void tb_blocks(const KParams& kp, int want_seg, TBIO* io, int* nseg_out) {
   io->ntj = (kp.ny + kTBJ - 1) / kTBJ;
   io->ntk = (kp.nz + kTBK - 1) / kTBK;
@@ -2840,8 +2870,8 @@ void tb_blocks(const KParams& kp, int want_seg, TBIO* io, int* nseg_out) {
   const int mid = io->xb - io->xa;
   int nmid = want_seg;
   if (nmid <= 0) {
-    nmid = (int)std::max<long long>(1, (2048 + tiles - 1) / tiles);
-    nmid = std::min(nmid, std::max(1, mid / 16));
+    nmid = (int)std::max<long long>(1, (6144 + tiles - 1) / tiles);
+    nmid = std::min(nmid, std::max(1, mid / 24));
   }
   nmid = std::max(1, std::min(nmid, std::max(1, mid)));
   io->seg_len = std::max(1, (mid + nmid - 1) / nmid);
@@ -2883,6 +2913,7 @@ int launch_tb(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
   }
   io.cm.prof = pl->cprof;
   io.cm.mid = pl->psi_mid;
+  io.n_prof = kp.cn[0] + kp.cn[1] + kp.cn[2] + kp.cn[3] + kp.cn[4] + kp.cn[5];
   for (int f = 0; f < 6; ++f) {
     io.cm.foff[f] = pl->psi_foff[f];
     io.cm.fs[f] = pl->psi_fs[f];
@@ -3573,6 +3604,9 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
     for (int a = 0; a < 3; ++a)
       if (pl->kp.cn[2 * a] + pl->kp.cn[2 * a + 1] > N[a])
         return fail(RFXH_EINVAL, "the temporal-blocked march needs CPML slabs that do not overlap");
+    if (pl->kp.cn[0] + pl->kp.cn[1] + pl->kp.cn[2] + pl->kp.cn[3] + pl->kp.cn[4] + pl->kp.cn[5] >
+        kTBProf)
+      return fail(RFXH_EINVAL, "the temporal-blocked march takes at most 256 CPML layers in all");
     if ((long long)pl->psi_foff[5] + 4LL * pl->psi_fs[5] >= (1LL << 30))
       return fail(RFXH_EINVAL, "the temporal-blocked march addresses CPML memory below 2^30 floats");
     if (pl->cprof && !pl->psi_mid) {

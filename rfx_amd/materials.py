@@ -117,6 +117,41 @@ class MaterialMap:
                     ids[n - phi - 1] = src
                 ids[n - phi:] = src
 
+    # -- material sweeps (rfx/vmap_sweep.py:265-390) ----------------------------
+    _FIELD = {"eps_r": 0, "sigma": 1, "mu_r": 2}
+
+    def set_field_where(self, mask, field: str, value):
+        """Give the cells of ``mask`` the value of one property (eps_r, sigma
+        or mu_r), keeping their other properties and flags."""
+        where = np.nonzero(mask)
+        if len(where[0]) == 0:
+            return
+        f = self._FIELD[field]
+        uniq, inv = np.unique(self.ids[where], return_inverse=True)
+        new = np.empty(len(uniq), np.uint16)
+        for q, m in enumerate(uniq):
+            props = list(self.table[int(m)])
+            props[f] = F32(value)
+            new[q] = self.get_id(*props)
+        self.ids[where] = new[inv]
+
+    def set_field_all(self, field: str, value):
+        """Every material whose property differs from vacuum (eps_r, mu_r != 1,
+        sigma != 0) takes ``value`` for it."""
+        f = self._FIELD[field]
+        vac = F32(0.0) if field == "sigma" else F32(1.0)
+        table = []
+        for e in self.table:
+            e = list(e)
+            if e[f] != vac:
+                e[f] = F32(value)
+            table.append(tuple(e))
+        self.table = table
+        self._lookup = {}
+        for mid, (e, sg, mu, fl) in enumerate(table):
+            key = (F32(e).tobytes(), F32(sg).tobytes(), F32(mu).tobytes(), int(fl))
+            self._lookup.setdefault(key, mid)
+
     # -- per-cell edits ------------------------------------------------------
     def cell_props(self, idx):
         e, s, m, f = self.table[int(self.ids[idx])]

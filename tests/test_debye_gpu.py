@@ -68,7 +68,7 @@ def test_debye_variants_and_batch_bit_identical(R):
     _say("Debye: column, one-cell and batched steps")
     import torch
     from rfx_amd import autotune as at
-    from rfx_amd.batch import run_batch
+    from rfx_amd.batch import run_sims
     from rfx_amd.engine import DeviceRun
     spec = cases.debye_case(20e-12)
     n = 120
@@ -87,7 +87,7 @@ def test_debye_variants_and_batch_bit_identical(R):
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             np.testing.assert_array_equal(a, b)
-    res = run_batch([cases.to_sim(spec), cases.to_sim(cases.mixed_case("run"))], [n, 80])
+    res = run_sims([cases.to_sim(spec), cases.to_sim(cases.mixed_case("run"))], [n, 80])
     single = cases.to_sim(spec).run(n_steps=n, autotune=False)
     np.testing.assert_array_equal(res[0].time_series, single.time_series)
 

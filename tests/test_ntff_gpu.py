@@ -118,10 +118,10 @@ def test_batch_carries_the_ntff_sums(R):
     its own (the batched step and the column step give bit-identical
     fields)."""
     _say("NTFF in a batch of two")
-    from rfx_amd.batch import run_batch
+    from rfx_amd.batch import run_sims
     spec, n = cases.dipole_ntff_case()
     single = cases.to_sim(spec).run(n_steps=n)
-    res = run_batch([cases.to_sim(spec), cases.to_sim(cases.mixed_case("run"))], [n, 60])
+    res = run_sims([cases.to_sim(spec), cases.to_sim(cases.mixed_case("run"))], [n, 60])
     for f in o.NTFF_FACES:
         np.testing.assert_array_equal(getattr(res[0].ntff_data, f), getattr(single.ntff_data, f))
     assert res[1].ntff_data is None
