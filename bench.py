@@ -355,23 +355,132 @@ def oracle_spec(spec):
                 probes=spec["probes"], dft_planes=spec["dft_planes"], lane="run")
 
 
-def cpu_baseline(spec, n_total, cpu_steps):
-    """The C restatement (oracle/cstep.c, float32, OpenMP) timed on a prefix
-    of the same workload."""
-    from oracle import cstep
-    from oracle import fdtd_np as o
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    g, cfg = o.build_uniform(oracle_spec(spec), cpu_steps)
+_CPU_CHILD = r"""
+import json, sys, time
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import bench
+from oracle import cstep
+from oracle import fdtd_np as o
+n1, n2 = int(sys.argv[2]), int(sys.argv[3])
+spec = bench.c2_spec() if sys.argv[4] == "c2" else bench.c1_spec()
+out = {}
+for n in (n1, n2):
+    g, cfg = o.build_uniform(bench.oracle_spec(spec), n)
     t0 = time.perf_counter()
-    cstep.run_c(cfg, cpu_steps, double=False, want_fields=False)
-    el = time.perf_counter() - t0
-    cells = int(np.prod(g.shape))
-    return {"value": round(cpu_steps * cells / el / 1e6, 3), "unit": "Mcell-steps/s",
-            "cores": cores, "kind": "port",
-            "sample": f"first {cpu_steps} of {n_total} time steps of the same "
-                      f"{g.shape[0]}x{g.shape[1]}x{g.shape[2]} problem, C restatement "
-                      f"(oracle/cstep.c) in float32 with {cores} OpenMP threads, "
-                      f"{el:.1f} s"}
+    cstep.run_c(cfg, n, double=False, want_fields=False)
+    out[n] = time.perf_counter() - t0
+print(json.dumps({"cells": int(np.prod(g.shape)), "shape": list(g.shape),
+                  "t": [out[n1], out[n2]]}))
+"""
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_rate(config, threads, n1, n2):
+    """Mcell-steps/s of the C restatement (oracle/cstep.c, float32, OpenMP)
+    with `threads` threads: two runs of the same workload, n1 and n2 time
+    steps, in a child process; the rate is (n2 - n1) steps over the
+    difference of their times, so allocation and first-touch set-up cancel."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    r = subprocess.run([sys.executable, "-c", _CPU_CHILD, ROOT, str(n1), str(n2), config],
+                       env=env, capture_output=True, text=True, check=True)
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    dt = d["t"][1] - d["t"][0]
+    return (n2 - n1) * d["cells"] / dt / 1e6, d, dt
+
+
+def cpu_baseline(config, n_total, all_steps, one_steps):
+    """The C restatement timed on this host's cores, on rank 0: all the
+    cores this process may use, and one core."""
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    v_all, d, t_all = cpu_rate(config, cores, 1, 1 + all_steps)
+    v_one, _, t_one = cpu_rate(config, 1, 1, 1 + one_steps)
+    shape = "x".join(str(v) for v in d["shape"])
+    return {"value": round(v_all, 2), "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
+            "one_core": {"value": round(v_one, 2), "unit": "Mcell-steps/s", "cores": 1,
+                         "timed_steps": one_steps, "seconds": round(t_one, 2)},
+            "sample": f"{all_steps} time steps ({cores} OpenMP threads, {t_all:.1f} s) and "
+                      f"{one_steps} (1 thread, {t_one:.1f} s) of the same {shape} problem "
+                      f"of {n_total} steps, C restatement (oracle/cstep.c) in float32: each "
+                      f"the difference of a run of 1 + k steps and a run of 1 step, so set-up "
+                      f"and first touch cancel"}
+
+
+def c1_line():
+    """The README quick start (87x47x42 cells, 1200 steps) on the product
+    path: autotuned variant, graph replay, inputs resident."""
+    import torch
+    from rfx_amd import autotune as at
+    from rfx_amd.engine import DeviceRun
+    spec = c1_spec()
+    n = N_STEPS["c1"]
+    sim = build_sim(spec)
+    grid, prob, _ = sim._build_problem(n, "run")
+    dr = DeviceRun(prob, n, "cuda:0")
+    tune = at.autotune(dr, trial_steps=64)
+    for _ in range(2):
+        dr.reset()
+        dr.run(0, n)
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dr.reset()
+        dr.run(0, n)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    finite = bool(torch.isfinite(dr.time_series()).all())
+    dr.close()
+    cells = int(np.prod(grid.shape))
+    return {"workload": f"README quick start {grid.shape[0]}x{grid.shape[1]}x{grid.shape[2]} "
+                        f"cells, {n} time steps, graph replay, mean of {reps} runs",
+            "variant": at.variant_name(tuple(tune["variant"])),
+            "Mcell_steps_per_s": round(n * cells / el / 1e6, 1),
+            "ms_per_run": round(el * 1e3, 3), "us_per_time_step": round(el / n * 1e6, 2),
+            "series_finite": finite}
+
+
+def kernel_roofline(kern_v, ms_h, ms_e, cells):
+    """The dominant kernel of the variant, its average launch time and its
+    algorithmic bytes per launch (48 B per cell-step)."""
+    if kern_v == 8:
+        # one launch advances two steps: 2 x 48 B per cell
+        return "k_step_tb2", ms_h, 2 * BYTES_PER_CELL_STEP * cells
+    if kern_v in (1, 2, 3, 7):
+        # one launch does the whole step: 48 B per cell
+        which = {1: "k_step_fused", 2: "k_step_march", 3: "k_step_march",
+                 7: "k_step_wmarch"}[kern_v]
+        return which, ms_h, BYTES_PER_CELL_STEP * cells
+    # two passes, each advancing half the field components: 24 B per cell
+    sfx = "_col" if kern_v in (4, 5, 6) else ""
+    which, ms = (f"k_update_e{sfx}", ms_e) if ms_e >= ms_h else (f"k_update_h{sfx}", ms_h)
+    return which, ms, 0.5 * BYTES_PER_CELL_STEP * cells
+
+
+def traffic_of(config, variant, which):
+    """HBM bytes per launch of `which` from the committed counter profile of
+    this workload and variant (the profile named beside the figure), or None."""
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as fh:
+        tab = json.load(fh)
+    rec = tab.get(f"{config}:{variant}:{which}")
+    if rec is None and which == "k_step_tb2":
+        rec = tab.get(f"{config}:tb2:{which}")
+    return rec
 
 
 def main():
@@ -380,7 +489,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(N_STEPS))
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=12,
+                    help="time steps of the all-core CPU sample (the 1-core sample takes "
+                         "a twelfth of them, at least one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graded", action="store_true",
                     help="skip the graded-lane timing")
@@ -388,7 +499,10 @@ def main():
                     help="skip the batched-sweep timing")
     ap.add_argument("--no-grad", action="store_true",
                     help="skip the forward+adjoint / forward timing")
+    ap.add_argument("--no-c1", action="store_true", help="skip the README quick-start line")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-kernel-sample", action="store_true",
+                    help="skip the event-sampled run that times the kernels")
     ap.add_argument("--time-steps", type=int, default=None,
                     help="time steps per run (default: the workload's own count); "
                          "for short profiled runs")
@@ -434,42 +548,43 @@ def main():
         del src, dst
         print("[bench] copy calibration: 5 x 512 MiB device copies", file=sys.stderr,
               flush=True)
+    use_graph = not args.no_graph
     for w in range(args.warmup):
         dr.reset()
-        dr.run(0, n_total, use_graph=not args.no_graph)
+        dr.run(0, n_total, use_graph=use_graph)
         torch.cuda.synchronize()
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
-    dr.set_timing(True)
+    # the timed region: the product path (hipGraph replay of the step
+    # sequence), no event sampling
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         dr.reset()
-        dr.run(0, n_total, use_graph=not args.no_graph)
+        dr.run(0, n_total, use_graph=use_graph)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     print(f"[bench] timed {args.steps} runs in {el:.3f} s", file=sys.stderr, flush=True)
-    ms_h, n_h = dr.kernel_time_ms(0)
-    ms_e, n_e = dr.kernel_time_ms(1)
-    ms_d, n_d = dr.kernel_time_ms(2)
+    finite = bool(torch.isfinite(dr.time_series()).all()) and all(
+        bool(torch.isfinite(a).all()) for a in dr.plane_acc)
+    # kernel durations: HIP events on the run's stream around up to 64
+    # evenly spaced launches of one more run (direct launches: events
+    # cannot sit inside a replayed graph)
+    ms_h = ms_e = ms_d = 0.0
+    n_e = 0
+    if not args.no_kernel_sample:
+        dr.reset()
+        dr.set_timing(True)
+        dr.run(0, n_total, use_graph=False)
+        torch.cuda.synchronize()
+        ms_h, n_h = dr.kernel_time_ms(0)
+        ms_e, n_e = dr.kernel_time_ms(1)
+        ms_d, n_d = dr.kernel_time_ms(2)
+        dr.set_timing(False)      # (clears the sums)
     value = args.steps * n_total * cells / el / 1e6
-    if kern_v == 8:
-        # one launch advances two steps: 2 x 48 B per cell
-        which, ms, unit_bytes = "k_step_tb2", ms_h, 2 * BYTES_PER_CELL_STEP * cells
-    elif kern_v in (1, 2, 3, 7):
-        # one launch does the whole step: 48 B per cell
-        which = {1: "k_step_fused", 2: "k_step_march", 3: "k_step_march", 7: "k_step_wmarch"}[kern_v]
-        ms, unit_bytes = ms_h, BYTES_PER_CELL_STEP * cells
-    else:
-        # two passes, each advancing half the field components: 24 B per cell
-        sfx = "_col" if kern_v in (4, 5, 6) else ""
-        which, ms = (f"k_update_e{sfx}", ms_e) if ms_e >= ms_h else (f"k_update_h{sfx}", ms_h)
-        unit_bytes = 0.5 * BYTES_PER_CELL_STEP * cells
+    which, ms, unit_bytes = kernel_roofline(kern_v, ms_h, ms_e, cells)
     achieved = unit_bytes / (ms * 1e-3) / 1e9 if ms > 0 else None
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get(f"{args.config}:{variant}:{which}")
+    trec = traffic_of(args.config, variant, which)
+    launches_per_step = 0.5 if kern_v == 8 else 1.0
     out = {
         "metric": "Mcell-steps/s fp32 + % HBM roofline; fwd+adjoint grad time vs plain forward",
         "value": round(value, 1), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -483,24 +598,36 @@ def main():
                    if args.config == "c2" else
                    f"README quick start {grid.shape}, {n_total} time steps",
                    "cells": cells, "time_steps": n_total, "variant": variant,
+                   "timed_path": "hipGraph replay" if use_graph else "direct launches",
+                   "ms_per_time_step": round(el / args.steps / n_total * 1e3, 4),
+                   "outputs_finite": finite,
                    "autotune_ms_per_step": tuning["ms_per_step"] if tuning else None,
                    "pct_hbm_roofline_48B": round(100 * value * 1e6 * BYTES_PER_CELL_STEP
                                                  / (HBM_PEAK_GBS * 1e9), 2),
-                   "kernel_ms": {"h_pass_or_step": round(ms_h, 4), "e_pass": round(ms_e, 4),
-                                 "k_dft_plane": round(ms_d, 4), "samples": n_e}},
+                   "kernel_ms_event_sampled": {
+                       "step_kernel_or_h_pass": round(ms_h, 4), "e_pass": round(ms_e, 4),
+                       "dft_and_counter": round(ms_d, 4), "samples": n_e,
+                       "time_steps_per_launch": 1 / launches_per_step}},
         "roofline": {"bound": "hbm", "kernel": which,
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": traffic},
+                     "algorithmic_bytes_per_launch": unit_bytes,
+                     "traffic": trec["bytes_per_launch"] if trec else None,
+                     "traffic_profile": trec["profile"] if trec else None},
         "cpu_baseline": None,
         "grad_vs_forward": None,
         "batch_sweep": None,
         "graded_lane": None,
+        "readme_quick_start": None,
     }
     dr.close()
     del dr
     torch.cuda.empty_cache()
+    if not args.no_c1 and args.config == "c2":
+        print("[bench] README quick start ...", file=sys.stderr, flush=True)
+        out["readme_quick_start"] = c1_line()
+        print(f"[bench] c1 {out['readme_quick_start']}", file=sys.stderr, flush=True)
     if not args.no_grad and args.config == "c2":
         print("[bench] forward+adjoint vs forward ...", file=sys.stderr, flush=True)
         out["grad_vs_forward"] = grad_cost()
@@ -515,7 +642,9 @@ def main():
         print(f"[bench] batch {out['batch_sweep']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(spec, n_total, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.config, n_total, args.cpu_steps,
+                                           max(1, args.cpu_steps // 12))
+        print(f"[bench] cpu {out['cpu_baseline']}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

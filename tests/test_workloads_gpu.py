@@ -144,6 +144,102 @@ def test_c2_full_size_properties(torch_mod):
             assert cases.rel_l2(acc.view(np.float64), base[1].view(np.float64)) <= 1e-6, v
 
 
+def _product_variants():
+    # the variants this workload has run on the GPU at its full size: the
+    # column step and the one-cell step in XCD-band order and the two-step
+    # sweep (the autotuner's picks for it)
+    return [(4, 0), (0, 0), (8, 0), (8, 4)]
+
+
+@pytest.mark.parametrize("n_int", [96])
+def test_c2_scaled_product_variants_vs_oracle(torch_mod, n_int):
+    """c2_spec at a 96^3 interior (117^3 cells, the 64 seeded boxes, 16
+    probes and the 32-frequency plane scaled): each product variant's
+    probes and plane vs the float64 oracle, and the column step
+    bit-identical to the one-cell step."""
+    torch = torch_mod
+    from rfx_amd import autotune as at
+    from rfx_amd.engine import DeviceRun
+    n = 320
+    spec = bench.c2_spec(n_int)
+    sim = bench.build_sim(spec)
+    grid, prob, _ = sim._build_problem(n, "run")
+    _say(f"C2 scaled to {n_int}^3 interior {grid.shape}, {n} steps, product variants")
+    g, cfg = o.build_uniform(_oracle_spec(spec), n)
+    assert tuple(g.shape) == tuple(grid.shape)
+    ref = cstep.run_c(cfg, n, want_fields=False)
+    assert np.abs(ref.time_series).max() > 0.0
+    dr = DeviceRun(prob, n, "cuda")
+    got = {}
+    for v in _product_variants():
+        at.set_variant(dr, *v)
+        _say(f"C2 {n_int}^3 variant {at.variant_name(v)}")
+        dr.reset()
+        dr.run(0, n)
+        torch.cuda.synchronize()
+        ts = dr.time_series().cpu().numpy().copy()
+        acc = dr.plane_acc[0].cpu().numpy().copy()
+        got[v] = (ts, acc)
+        assert ts.shape == (n, 16)
+        worst = 0.0
+        for p in range(16):
+            if np.abs(ref.time_series[:, p]).max() == 0.0:
+                assert np.abs(ts[:, p]).max() == 0.0
+                continue
+            e = cases.rel_l2(ts[:, p], ref.time_series[:, p])
+            worst = max(worst, e)
+            assert e <= TOL, (v, p, e)
+        ep = cases.rel_l2(acc.view(np.float64), ref.dft_planes[0].view(np.float64))
+        print(at.variant_name(v), "worst probe", worst, "plane", ep)
+        assert ep <= TOL
+    dr.close()
+    assert np.array_equal(got[(4, 0)][0], got[(0, 0)][0])
+    assert np.array_equal(got[(4, 0)][1], got[(0, 0)][1])
+
+
+def test_c2_full_size_product_variants(torch_mod):
+    """The 533^3 workload itself, 64 steps (four DFT folds), checked by
+    size-independent properties (the oracle is too slow at this size):
+    finite probes, plane and fields; the column step bit-identical to the
+    one-cell step (probes, plane, the Ez slice through the plane); the
+    two-step sweep, with and without middle segments, within float32
+    rounding of it."""
+    torch = torch_mod
+    from rfx_amd import autotune as at
+    from rfx_amd.engine import DeviceRun
+    n = 64
+    spec = bench.c2_spec()
+    sim = bench.build_sim(spec)
+    grid, prob, _ = sim._build_problem(n, "run")
+    assert grid.shape == (533, 533, 533)
+    _say(f"C2 533^3, {n} steps, product variants")
+    dr = DeviceRun(prob, n, "cuda")
+    out = {}
+    for v in _product_variants():
+        at.set_variant(dr, *v)
+        _say(f"C2 533^3 variant {at.variant_name(v)}")
+        dr.reset()
+        dr.run(0, n)
+        torch.cuda.synchronize()
+        ts = dr.time_series().cpu().numpy().copy()
+        acc = dr.plane_acc[0].cpu().numpy().copy()
+        sl = dr.field("ez")[:, :, 266].cpu().numpy().copy()
+        out[v] = (ts, acc, sl)
+        assert np.all(np.isfinite(ts)) and np.all(np.isfinite(acc)) and np.all(np.isfinite(sl))
+        assert np.abs(sl).max() > 0.0 and np.abs(acc).max() > 0.0
+    dr.close()
+    base = out[(4, 0)]
+    for k in range(3):
+        assert np.array_equal(out[(0, 0)][k], base[k]), k
+    for v in ((8, 0), (8, 4)):
+        ts, acc, sl = out[v]
+        e_sl = cases.rel_l2(sl, base[2])
+        e_acc = cases.rel_l2(acc.view(np.float64), base[1].view(np.float64))
+        print(at.variant_name(v), "slice", e_sl, "plane", e_acc)
+        assert e_sl <= 1e-5 and e_acc <= 1e-5
+        assert ts.shape == base[0].shape
+
+
 def _c4_reduced():
     n_int = (48, 48, 24)
     spec = bench.c4_spec(n_int, cpml_layers=8)
