@@ -1066,6 +1066,20 @@ constexpr int kTBK = kBX - 2 * kTBHalo;     // interior columns of a tile
 constexpr int kTBMat = 1024;                // material table entries staged in LDS
 constexpr int kTBPlanes = 4;                // DFT planes the sweep stages itself
 constexpr int kTBProf = 256;                // CPML profile rows staged in LDS (all faces)
+constexpr int kTBOps = 256;                 // point operations one tile column may hold
+
+// A point operation of the workgroup's region, staged in LDS at the start
+// of the sweep with the source values of both steps, so that the sweep
+// issues no global loads besides its plane prefetch and CPML memory (a
+// load in a rarely taken branch would make the compiler wait for every
+// load in flight where the branches join).
+struct TBOp {
+  unsigned cell;
+  int kind, comp, slot;
+  float d_par;
+  float loop[3];
+  float w[2];      // inject: source values of steps n and n + 1
+};
 
 struct TBPlane {
   float* snap;     // ring [kDftRing][npts]
@@ -1115,14 +1129,200 @@ struct TBShared {
   float4 prof[kTBProf];
   // [buffer][stage: E^n, H^{n+1/2}, E^{n+1}, H^{n+3/2}][component][row][lane]
   float x[2][4][3][kTBW][kBX];
+  TBOp ops[kTBOps];   // the region's point operations, sorted by (cell, order)
+  int n_ops;
+  int wcnt[kTBW];
 };
+
+// Stable compaction of the point operations whose cells lie in planes
+// [qa, qb), rows [ja, jb) and columns [ka, kb) into sh.ops (the plan checks
+// that no tile column holds more than kTBOps).  All threads of the
+// workgroup call it.
+__device__ __forceinline__ void tb_stage_ops(const KParams& p, TBShared& sh, int w, int l,
+                                             long long n, int qa, int qb, int ja, int jb,
+                                             int ka, int kb) {
+  const int tid = w * kBX + l;
+  if (tid == 0) sh.n_ops = 0;
+  __syncthreads();
+  for (int base = 0; base < p.n_ops; base += kBX * kTBW) {
+    const int t = base + tid;
+    bool in = false;
+    rfxh_point_op op;
+    if (t < p.n_ops) {
+      op = p.ops[t];
+      const long long c = op.cell;
+      const int i = (int)(c / p.plane);
+      const int r = (int)(c - (long long)i * p.plane);
+      const int j = r / p.nz, k = r - (r / p.nz) * p.nz;
+      in = i >= qa && i < qb && j >= ja && j < jb && k >= ka && k < kb;
+    }
+    const unsigned long long bal = __ballot(in);
+    const int rank = __popcll(bal & ((1ull << l) - 1ull));
+    if (l == 0) sh.wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int off = sh.n_ops;
+    for (int u = 0; u < w; ++u) off += sh.wcnt[u];
+    if (in && off + rank < kTBOps) {
+      TBOp& d = sh.ops[off + rank];
+      d.cell = (unsigned)op.cell;
+      d.kind = op.kind;
+      d.comp = op.comp;
+      d.slot = op.slot;
+      d.d_par = op.d_par;
+      d.loop[0] = op.loop[0];
+      d.loop[1] = op.loop[1];
+      d.loop[2] = op.loop[2];
+      d.w[0] = d.w[1] = 0.f;
+      if (op.kind == RFXH_OP_INJECT) {
+        d.w[0] = p.src_wave[n * p.n_src + op.slot];
+        d.w[1] = p.src_wave[(n + 1) * p.n_src + op.slot];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int tot = sh.n_ops;
+      for (int u = 0; u < kTBW; ++u) tot += sh.wcnt[u];
+      sh.n_ops = min(tot, kTBOps);
+    }
+    __syncthreads();
+  }
+}
+
+// point_ops from the staged list: step n + s (s = 0, 1).
+__device__ __forceinline__ void tb_point_ops(const KParams& p, const TBShared& sh, unsigned cell,
+                                             float& e0, float& e1, float& e2, float hx, float hy,
+                                             float hz, float hx_j, float hx_k, float hy_i,
+                                             float hy_k, float hz_i, float hz_j, long long n,
+                                             int s, bool own) {
+  int lo = 0, hi = sh.n_ops;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sh.ops[mid].cell < cell) lo = mid + 1;
+    else hi = mid;
+  }
+  const long long ns = n + s;
+  for (int q = lo; q < sh.n_ops && sh.ops[q].cell == cell; ++q) {
+    const TBOp& op = sh.ops[q];
+    if (op.kind == RFXH_OP_PORT) {
+      if (!own) continue;
+      const float v = -pick3(op.comp, e0, e1, e2) * op.d_par;
+      float cur;
+      if (op.comp == 2)
+        cur = (hy - hy_i) * op.loop[1] - (hx - hx_j) * op.loop[0];
+      else if (op.comp == 0)
+        cur = (hz - hz_j) * op.loop[2] - (hy - hy_k) * op.loop[1];
+      else
+        cur = (hx - hx_k) * op.loop[0] - (hz - hz_i) * op.loop[2];
+      float* o = p.port_out + (ns * p.n_port + op.slot) * 2;
+      o[0] = v;
+      o[1] = cur;
+    } else if (op.kind == RFXH_OP_INJECT) {
+      const float w = s ? op.w[1] : op.w[0];
+      if (op.comp == 0) e0 += w;
+      else if (op.comp == 1) e1 += w;
+      else e2 += w;
+    } else if (own) {
+      p.probe_out[ns * p.n_probe + op.slot] =
+          op.comp < 3 ? pick3(op.comp, e0, e1, e2) : pick3(op.comp - 3, hx, hy, hz);
+    }
+  }
+}
+
+// CPML faces of the sweep's cells.  A slab is thinner than half its axis,
+// so a cell lies in at most one face per axis; the y and z faces of a
+// thread are fixed along the x march (per row and per lane) and resolved
+// once, the x face is the plane's (the same for the whole workgroup).
+struct TBCol {
+  int ly, lz;        // layer in the y / z face, -1: none
+  int fy, fz;        // that face (2 or 3, 4 or 5)
+  unsigned by, bz;   // psi index of term 0 at plane 0
+  unsigned sy, sz;   // psi index step per plane
+};
+
+__device__ __forceinline__ TBCol tb_col(const KParams& p, const TBCpml& cm, bool valid, int j,
+                                        int k) {
+  TBCol col;
+  col.ly = col.lz = -1;
+  col.fy = 2;
+  col.fz = 4;
+  col.by = col.bz = col.sy = col.sz = 0u;
+  if (!valid) return col;
+  if (j < p.cn[2]) {
+    col.ly = j;
+  } else if (j >= p.ny - p.cn[3]) {
+    col.fy = 3;
+    col.ly = j - (p.ny - p.cn[3]);
+  }
+  if (col.ly >= 0) {
+    // y face slab [i][l][k]
+    const unsigned n = (unsigned)(col.fy == 2 ? p.cn[2] : p.cn[3]);
+    col.by = (col.fy == 2 ? cm.foff[2] : cm.foff[3]) + (unsigned)col.ly * (unsigned)p.nz +
+             (unsigned)k;
+    col.sy = n * (unsigned)p.nz;
+  }
+  if (k < p.cn[4]) {
+    col.lz = k;
+  } else if (k >= p.nz - p.cn[5]) {
+    col.fz = 5;
+    col.lz = k - (p.nz - p.cn[5]);
+  }
+  if (col.lz >= 0) {
+    // z face slab [i][j][l]
+    const unsigned n = (unsigned)(col.fz == 4 ? p.cn[4] : p.cn[5]);
+    col.bz = (col.fz == 4 ? cm.foff[4] : cm.foff[5]) + (unsigned)j * n + (unsigned)col.lz;
+    col.sz = (unsigned)p.ny * n;
+  }
+  return col;
+}
+
+// The face on axis a holding cell (q, j, k) (c0 = j nz + k): psi index of
+// its term 0, the term stride, the profile row and the inverse spacing;
+// false when the cell lies in no face on that axis.
+__device__ __forceinline__ bool tb_face(const KParams& p, const TBCpml& cm, const TBCol& col,
+                                        int a, int q, unsigned c0, unsigned& idx, unsigned& ts,
+                                        int& prow, float& inv) {
+  if (a == 0) {
+    int f, lx;
+    if (q >= 0 && q < p.cn[0]) {
+      f = 0;
+      lx = q;
+    } else if (q >= p.nx - p.cn[1] && q < p.nx) {
+      f = 1;
+      lx = q - (p.nx - p.cn[1]);
+    } else {
+      return false;
+    }
+    // x face slab [l][j][k]
+    idx = (f == 0 ? cm.foff[0] : cm.foff[1]) + (unsigned)lx * (unsigned)p.plane + c0;
+    ts = f == 0 ? cm.fs[0] : cm.fs[1];
+    prow = (f == 0 ? cm.poff[0] : cm.poff[1]) + lx;
+    inv = f == 0 ? p.cinv[0] : p.cinv[1];
+    return true;
+  }
+  if (a == 1) {
+    if (col.ly < 0) return false;
+    const bool lo = col.fy == 2;
+    idx = col.by + (unsigned)q * col.sy;
+    ts = lo ? cm.fs[2] : cm.fs[3];
+    prow = (lo ? cm.poff[2] : cm.poff[3]) + col.ly;
+    inv = lo ? p.cinv[2] : p.cinv[3];
+    return true;
+  }
+  if (col.lz < 0) return false;
+  const bool lo = col.fz == 4;
+  idx = col.bz + (unsigned)q * col.sz;
+  ts = lo ? cm.fs[4] : cm.fs[5];
+  prow = (lo ? cm.poff[4] : cm.poff[5]) + col.lz;
+  inv = lo ? p.cinv[4] : p.cinv[5];
+  return true;
+}
 
 // H at one cell with the CPML memory in registers: h_core's update reading
 // the memory from pin and leaving the new memory in pout (and, when store,
-// in ps.hw).
+// in cm.psw).
 template <bool GRADED, bool CPML = true>
-__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const float4* prof,
-                                       int i, int j, int k, float gy, float gz,
+__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const TBCol& col,
+                                       const float4* prof, int i, unsigned c0, float gy, float gz,
                                        float ch, float ex, float ey, float ez, float ex_j,
                                        float ez_j, float ex_k, float ey_k, float ey_i, float ez_i,
                                        float hx, float hy, float hz, const float (&pin)[3][2],
@@ -1140,16 +1340,15 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const
   hx = hx - ch * ((ez_j - ez) * iy - (ey_k - ey) * iz);
   hy = hy - ch * ((ex_k - ex) * iz - (ez_i - ez) * ix);
   hz = hz - ch * ((ey_i - ey) * ix - (ex_j - ex) * iy);
-  if (CPML && in_slab(p, i, j, k)) {
-    __asm__ volatile("" ::: "memory");
+  if (CPML) {
 #pragma unroll
-    for (int face = 0; face < 6; ++face) {
-      const int l = slab_layer(p, face, i, j, k);
-      if (l < 0) continue;
-      const int ax = face >> 1;
-      const float4 pr = prof[cm.poff[face] + l];
+    for (int ax = 0; ax < 3; ++ax) {
+      unsigned q, ts;
+      int prow;
+      float inv;
+      if (!tb_face(p, cm, col, ax, i, c0, q, ts, prow, inv)) continue;
+      const float4 pr = prof[prow];
       const float bb = pr.x, cc = pr.y, km1 = pr.z;
-      const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
         g0 = (ez_i - ez) * inv;
@@ -1166,10 +1365,8 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const
       pout[ax][0] = s0;
       pout[ax][1] = s1;
       if (store) {
-        const unsigned q =
-            cm.foff[face] + (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
         stf(cm.psw, q, s0);
-        stf(cm.psw, q + cm.fs[face], s1);
+        stf(cm.psw, q + ts, s1);
       }
       const float d0 = ch * s0 + ch * km1 * g0;
       const float d1 = ch * s1 + ch * km1 * g1;
@@ -1191,13 +1388,13 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const
 // E at one cell with the CPML memory in registers (e_core_co's update),
 // point operations at step n (recorded only when own).
 template <bool GRADED, bool CPML = true>
-__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const float4* prof,
-                                       int i, int j, int k, float gy, float gz,
-                                       unsigned c, float4 co, uint32_t fl, float e0, float e1,
+__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const TBCol& col,
+                                       const float4* prof, int i, int j, int k, unsigned c0,
+                                       float gy, float gz, unsigned c, float4 co, uint32_t fl, float e0, float e1,
                                        float e2, float hx, float hy, float hz, float hy_i,
                                        float hz_i, float hx_j, float hz_j, float hx_k, float hy_k,
                                        const float (&pin)[3][2], float (&pout)[3][2], bool store,
-                                       long long n, bool own) {
+                                       const TBShared& sh, long long n, int s, bool own) {
   float ix, iy, iz;
   if (GRADED) {
     ix = gld(p.inv_e[0], i);
@@ -1210,16 +1407,15 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
   e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
   e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
-  if (CPML && in_slab(p, i, j, k)) {
-    __asm__ volatile("" ::: "memory");
+  if (CPML) {
 #pragma unroll
-    for (int face = 0; face < 6; ++face) {
-      const int l = slab_layer(p, face, i, j, k);
-      if (l < 0) continue;
-      const int ax = face >> 1;
-      const float4 pr = prof[cm.poff[face] + l];
+    for (int ax = 0; ax < 3; ++ax) {
+      unsigned q, ts;
+      int prow;
+      float inv;
+      if (!tb_face(p, cm, col, ax, i, c0, q, ts, prow, inv)) continue;
+      const float4 pr = prof[prow];
       const float bb = pr.x, cc = pr.y, km1 = pr.z;
-      const float inv = p.cinv[face];
       float g0, g1;
       if (ax == 0) {
         g0 = (hz - hz_i) * inv;
@@ -1236,10 +1432,8 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const
       pout[ax][0] = s0;
       pout[ax][1] = s1;
       if (store) {
-        const unsigned q = cm.foff[face] + 2 * cm.fs[face] +
-                           (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
-        stf(cm.psw, q, s0);
-        stf(cm.psw, q + cm.fs[face], s1);
+        stf(cm.psw, q + 2 * ts, s0);
+        stf(cm.psw, q + 3 * ts, s1);
       }
       const float d0 = ce * s0 + ce * km1 * g0;
       const float d1 = ce * s1 + ce * km1 * g1;
@@ -1270,7 +1464,8 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const
     if (fl & RFXH_ZERO_EY) e1 = 0.f;
     if (fl & RFXH_ZERO_EZ) e2 = 0.f;
     if (fl & RFXH_POINT)
-      point_ops(p, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j, n, own);
+      tb_point_ops(p, sh, c, e0, e1, e2, hx, hy, hz, hx_j, hx_k, hy_i, hy_k, hz_i, hz_j, n, s,
+                   own);
   }
   return make_float3(e0, e1, e2);
 }
@@ -1300,6 +1495,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     gey = gld(p.inv_e[1], j);
     gez = gld(p.inv_e[2], k);
   }
+  const TBCol col = tb_col(p, cm, CPML && valid, j, k);
   auto pin = [&](int q) { return valid && q >= 0 && q < p.nx; };
   auto own = [&](int q) { return interior && q >= i0 && q < i1; };
   // neighbour value of stage st from LDS buffer b; zero outside the tile
@@ -1313,28 +1509,30 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   auto ld_psi = [&](float (&dst)[3][2], const float* base, unsigned term0, int q, bool use) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) dst[a][0] = dst[a][1] = 0.f;
-    if (CPML && use && pin(q) && in_slab(p, q, j, k)) {
+    if (CPML && use && pin(q)) {
 #pragma unroll
-      for (int face = 0; face < 6; ++face) {
-        const int ly = slab_layer(p, face, q, j, k);
-        if (ly < 0) continue;
-        const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
-                            (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
-        dst[face >> 1][0] = ldf(base, qq);
-        dst[face >> 1][1] = ldf(base, qq + cm.fs[face]);
+      for (int a = 0; a < 3; ++a) {
+        unsigned qq, ts;
+        int prow;
+        float inv;
+        if (!tb_face(p, cm, col, a, q, c0, qq, ts, prow, inv)) continue;
+        qq += term0 * ts;
+        dst[a][0] = ldf(base, qq);
+        dst[a][1] = ldf(base, qq + ts);
       }
     }
   };
   auto st_psi = [&](float* base, unsigned term0, int q, const float (&src)[3][2], bool use) {
-    if (CPML && use && pin(q) && in_slab(p, q, j, k)) {
+    if (CPML && use && pin(q)) {
 #pragma unroll
-      for (int face = 0; face < 6; ++face) {
-        const int ly = slab_layer(p, face, q, j, k);
-        if (ly < 0) continue;
-        const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
-                            (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
-        stf(base, qq, src[face >> 1][0]);
-        stf(base, qq + cm.fs[face], src[face >> 1][1]);
+      for (int a = 0; a < 3; ++a) {
+        unsigned qq, ts;
+        int prow;
+        float inv;
+        if (!tb_face(p, cm, col, a, q, c0, qq, ts, prow, inv)) continue;
+        qq += term0 * ts;
+        stf(base, qq, src[a][0]);
+        stf(base, qq + ts, src[a][1]);
       }
     }
   };
@@ -1371,38 +1569,45 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   for (int c = 0; c < 3; ++c) e0c[c] = e0b[c] = h1a[c] = h1b[c] = e1a[c] = e1b[c] = h2a[c] = 0.f;
   h2by = h2bz = 0.f;
   m2 = m3 = m4 = 0u;
-  // prefetch for iteration pp + 1: E^n of plane pp+1; H^{n-1/2} and id of
-  // plane pp
-  float fe[3], fh[3];
-  unsigned fm;
-  auto prefetch = [&](int pp) {
+  // prefetch(pp): E^n of plane pp+1, H^{n-1/2} and id of plane pp, raw
+  // (clamped addresses; zeroed outside the grid where consumed).  Two
+  // slots: iteration pp fills one with prefetch(pp + 1) and consumes the
+  // other, filled one iteration earlier, so each load has two iterations
+  // to arrive and waiting for it never waits for the newer one.
+  float fe[2][3], fh[2][3];
+  unsigned fm[2];
+  auto prefetch = [&](int pp, float (&e)[3], float (&h)[3], unsigned& m) {
     const int q = pp + 1;
     const unsigned oq = (unsigned)min(max(q, 0), p.nx - 1) * P + c0;
     const unsigned op = (unsigned)min(max(pp, 0), p.nx - 1) * P + c0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      fe[c] = ldf(io.eo[c], oq);
-      fh[c] = ldf(io.ho[c], op);
+      e[c] = ldf(io.eo[c], oq);
+      h[c] = ldf(io.ho[c], op);
     }
-    fm = ldu16(p.mat, op);
-    const bool vq = pin(q), vp = pin(pp);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      fe[c] = vq ? fe[c] : 0.f;
-      fh[c] = vp ? fh[c] : 0.f;
-    }
+    m = ldu16(p.mat, op);
   };
-  prefetch(i0 - 3);
+  prefetch(i0 - 3, fe[0], fh[0], fm[0]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    e0n[c] = fe[c];
-    h0n[c] = fh[c];
+    e0n[c] = pin(i0 - 2) ? fe[0][c] : 0.f;
+    h0n[c] = pin(i0 - 3) ? fh[0][c] : 0.f;
   }
-  m1 = fm;
+  m1 = fm[0];
+  // The CPML body loads its memory terms under branches, and the compiler
+  // then waits for every load in flight before using them: a second slot
+  // would only cost registers there, so it prefetches one plane ahead.
+  constexpr bool kDeep = !CPML;
+  if (kDeep) prefetch(i0 - 2, fe[1], fh[1], fm[1]);
 
-  int it = 0;
-  for (int pp = i0 - 2; pp < i1 + 4; ++pp, ++it) {
-    const int rb = (it & 1) ^ 1, wb = it & 1;
+  // one iteration: planes pp-4 .. pp-1; LDS buffer wb written, 1 - wb read;
+  // (ce, chh, cmm) the slot holding prefetch(pp) at the iteration's end,
+  // (ne, nh, nm) the slot it fills with prefetch(fill) -- pp + 1 into the
+  // other slot, or pp into the same one
+  auto iteration = [&](int pp, int wb, const float (&ce)[3], const float (&chh)[3],
+                       const unsigned& cmm, float (&ne)[3], float (&nh)[3], unsigned& nm,
+                       int fill) {
+    const int rb = wb ^ 1;
     // CPML memory of this iteration's four updates, issued before the
     // prefetch so that waiting for it leaves the prefetch in flight (vector
     // loads complete in order)
@@ -1411,7 +1616,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     ld_psi(pB, cm.psr, 2u, pp - 2, true);
     ld_psi(pC, cm.mid, 0u, pp - 3, true);
     ld_psi(pA, cm.psr, 0u, pp - 1, true);
-    prefetch(pp);
+    prefetch(fill, ne, nh, nm);
     // ---- D: E^{n+2} on plane pp-4
     float3 e2 = make_float3(0.f, 0.f, 0.f);
     {
@@ -1424,9 +1629,9 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const unsigned c = (unsigned)q * P + c0;
       float dummy[3][2];
       const float4 row = s_ecoef[m4];
-      e2 = tb_e<GRADED, CPML>(p, cm, sh.prof, q, j, k, gey, gez, c, row, __float_as_uint(row.w), e1b[0], e1b[1],
+      e2 = tb_e<GRADED, CPML>(p, cm, col, sh.prof, q, j, k, c0, gey, gez, c, row, __float_as_uint(row.w), e1b[0], e1b[1],
                               e1b[2], h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k,
-                              pD, dummy, o, n + 1, o);
+                              pD, dummy, o, sh, n, 1, o);
       if (!pin(q)) e2 = make_float3(0.f, 0.f, 0.f);
       if (o) {
         stf(io.en[0], c, e2.x);
@@ -1447,9 +1652,9 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       const float4 row = s_ecoef[m2];
-      e1 = tb_e<GRADED, CPML>(p, cm, sh.prof, q, j, k, gey, gez, c, row, __float_as_uint(row.w), e0b[0], e0b[1],
+      e1 = tb_e<GRADED, CPML>(p, cm, col, sh.prof, q, j, k, c0, gey, gez, c, row, __float_as_uint(row.w), e0b[0], e0b[1],
                               e0b[2], h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k,
-                              hy_k, pB, pe_new, false, n, o);
+                              hy_k, pB, pe_new, false, sh, n, 0, o);
       if (!pin(q)) e1 = make_float3(0.f, 0.f, 0.f);
       st_psi(cm.mid, 2u, q, pe_new, o);    // only the owner takes the second E update
       if (o) stage_snap(q, 0, e1, n);
@@ -1463,7 +1668,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       float dummy[3][2];
-      h2 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m3], e1a[0], e1a[1], e1a[2],
+      h2 = tb_h<GRADED, CPML>(p, cm, col, sh.prof, q, c0, ghy, ghz, chc > 0.f ? chc : s_ch[m3], e1a[0], e1a[1], e1a[2],
                               ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2], pC,
                               dummy, o);
       if (!pin(q)) h2 = make_float3(0.f, 0.f, 0.f);
@@ -1483,7 +1688,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const int q = pp - 1;
       const float ex_j = nb(rb, 0, 0, w + 1, l), ez_j = nb(rb, 0, 2, w + 1, l);
       const float ex_k = nb(rb, 0, 0, w, l + 1), ey_k = nb(rb, 0, 1, w, l + 1);
-      h1 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m1], e0c[0], e0c[1], e0c[2],
+      h1 = tb_h<GRADED, CPML>(p, cm, col, sh.prof, q, c0, ghy, ghz, chc > 0.f ? chc : s_ch[m1], e0c[0], e0c[1], e0c[2],
                               ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1], h0n[2], pA,
                               ph_new, false);
       if (!pin(q)) h1 = make_float3(0.f, 0.f, 0.f);
@@ -1513,14 +1718,15 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     h2a[0] = h2.x;
     h2a[1] = h2.y;
     h2a[2] = h2.z;
+    const bool vq = pin(pp + 1), vp = pin(pp);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       e1b[c] = e1a[c];
       h1b[c] = h1a[c];
       e0b[c] = e0c[c];
       e0c[c] = e0n[c];
-      e0n[c] = fe[c];
-      h0n[c] = fh[c];
+      e0n[c] = vq ? ce[c] : 0.f;
+      h0n[c] = vp ? chh[c] : 0.f;
     }
     e1a[0] = e1.x;
     e1a[1] = e1.y;
@@ -1531,15 +1737,29 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     m4 = m3;
     m3 = m2;
     m2 = m1;
-    m1 = fm;
+    m1 = cmm;
+  };
+  // an even number of iterations (one past the last plane's when the count
+  // is odd: it stores nothing the workgroup owns, and the H memory it may
+  // store is the value every other copy of that cell stores)
+  for (int pp = i0 - 2; pp < i1 + 4; pp += 2) {
+    if (kDeep) {
+      iteration(pp, 0, fe[1], fh[1], fm[1], fe[0], fh[0], fm[0], pp + 1);
+      iteration(pp + 1, 1, fe[0], fh[0], fm[0], fe[1], fh[1], fm[1], pp + 2);
+    } else {
+      iteration(pp, 0, fe[0], fh[0], fm[0], fe[0], fh[0], fm[0], pp);
+      iteration(pp + 1, 1, fe[0], fh[0], fm[0], fe[0], fh[0], fm[0], pp + 1);
+    }
   }
 }
+
 
 // ANY_CPML = false: a plan without CPML.
 template <bool GRADED, bool ANY_CPML>
 __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
   __shared__ TBShared sh;
-  const int w = (int)threadIdx.y, l = (int)threadIdx.x;
+  // one wavefront per row: the row index is wave-uniform (scalar registers)
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y), l = (int)threadIdx.x;
   const int tid = w * kBX + l;
   // the plan rejects this variant for more than kTBMat materials
   for (int t = tid; t < io.n_mat; t += kBX * kTBW) {
@@ -1571,6 +1791,11 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
   const int j0 = tj * kTBJ, k0 = tk * kTBK;
   const int j = j0 - kTBHalo + w;
   const int k = k0 - kTBHalo + l;
+  // point operations of every cell the sweep's stages visit
+  // (without any, no cell carries the point flag and the list is unused)
+  if (p.n_ops > 0)
+    tb_stage_ops(p, sh, w, l, *p.step, i0 - 6, i1 + 6, j0 - kTBHalo, j0 - kTBHalo + kTBW,
+                 k0 - kTBHalo, k0 - kTBHalo + kBX);
   // cells the interior's two steps need: planes [i0-2, i1], rows
   // [j0-2, j0+kTBJ], columns [k0-2, k0+kTBK]
   const bool slab =
@@ -2588,6 +2813,7 @@ struct rfxh_plan {
   int wmarch_seg = 0;           // its x segments (0: automatic)
   bool tb = false;              // two steps per sweep (temporal-blocked march), ping-pong
   int tb_seg = 0;               // its x segments (0: automatic)
+  int tb_ops_max = 0;           // most point operations in one of its tile columns
   float* psi_base = nullptr;    // CPML memory of set 0 (set 1 at + psi_set)
   float* psi_mid = nullptr;     // the sweep's memory after its first step (owned, same layout)
   long long psi_floats = 0;
@@ -3044,6 +3270,25 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   });
 
   rfxh_plan* pl = new rfxh_plan();
+  {
+    // point operations per tile column of the temporal-blocked march (tile
+    // rows [tj kTBJ - halo, + kTBW), columns [tk kTBK - halo, + kBX), all x)
+    const int ntj = (d->ny + kTBJ - 1) / kTBJ, ntk = (d->nz + kTBK - 1) / kTBK;
+    std::vector<int> cnt((size_t)ntj * ntk, 0);
+    const long long P = (long long)d->ny * d->nz;
+    for (const rfxh_point_op& op : ops) {
+      const int r = (int)(op.cell % P), j = r / d->nz, k = r % d->nz;
+      for (int tj = std::max(0, (j - kTBHalo) / kTBJ - 1); tj < ntj; ++tj) {
+        if (tj * kTBJ - kTBHalo > j) break;
+        if (j >= tj * kTBJ - kTBHalo + kTBW) continue;
+        for (int tk = std::max(0, (k - kTBHalo) / kTBK - 1); tk < ntk; ++tk) {
+          if (tk * kTBK - kTBHalo > k) break;
+          if (k >= tk * kTBK - kTBHalo + kBX) continue;
+          pl->tb_ops_max = std::max(pl->tb_ops_max, ++cnt[(size_t)tj * ntk + tk]);
+        }
+      }
+    }
+  }
   KParams& kp = pl->kp;
   std::memset(&kp, 0, sizeof(kp));
   kp.nx = d->nx;
@@ -3598,6 +3843,9 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
       return fail(RFXH_EINVAL, "the temporal-blocked march takes at most 1024 materials");
     if ((int)pl->planes.size() > kTBPlanes)
       return fail(RFXH_EINVAL, "the temporal-blocked march stages at most 4 DFT planes");
+    if (pl->tb_ops_max > kTBOps)
+      return fail(RFXH_EINVAL, "the temporal-blocked march stages at most 256 point operations "
+                               "(sources, probes, ports) per 16 x 64 tile column");
     if (!pl->fluxes.empty())
       return fail(RFXH_EINVAL, "flux monitors and NTFF boxes run on the single-step kernels");
     const int N[3] = {pl->kp.nx, pl->kp.ny, pl->kp.nz};

@@ -205,3 +205,17 @@ def test_tb2_refuses_what_it_does_not_run(R):
     with pytest.raises(_lib.RfxHipError):
         at.set_variant(fl, 8, 0)
     fl.close()
+    # more point operations in one 16 x 64 tile column than the sweep stages
+    import rfx_amd as Rx
+    sim = Rx.Simulation(10e9, (0.030, 0.010, 0.010), dx=1e-3, boundary="cpml", cpml_layers=2)
+    for a in range(1, 30):
+        for b in range(1, 10):
+            sim.add_source((a * 1e-3, 5e-3, b * 1e-3), "ez", waveform=Rx.GaussianPulse(f0=5e9))
+    _, prob, _ = sim._build_problem(20, "run")
+    assert len(prob.sources) == 29 * 9
+    dense = DeviceRun(prob, 20, "cuda")
+    with pytest.raises(_lib.RfxHipError):
+        at.set_variant(dense, 8, 0)
+    at.set_variant(dense, 4, 0)
+    dense.run(0, 20)
+    dense.close()
