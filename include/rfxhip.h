@@ -41,6 +41,9 @@ extern "C" {
 #define RFXH_ZERO_EY 2u
 #define RFXH_ZERO_EZ 4u
 #define RFXH_POINT 8u     /* cell carries point operations (source/probe/port) */
+#define RFXH_ANISO 16u    /* subpixel smoothing: per-component eps at the cell; ca, cb of
+                             mat_coef hold the Ex coefficients, aniso_coef the Ey, Ez ones
+                             (rfx/core/yee.py:824-901, graded update_e_nu_aniso :668-719) */
 #define RFXH_POLES 0xFFFF0000u /* bit 16 + q: dispersion pole q acts at the cell
                                    (per-pole masks, rfx/materials/debye.py:76-179,
                                    rfx/materials/lorentz.py:103-198) */
@@ -165,6 +168,11 @@ typedef struct {
   int32_t pol_n[3];
   float* d_pol;            /* device, zeroed by the caller: per Debye pole P[3][box cells],
                               then per Lorentz pole P[3][box cells], P^{n-1}[3][box cells] */
+  /* subpixel smoothing (rfx/runners/uniform.py:271-278, rfx/geometry/smoothing.py):
+     per id flagged RFXH_ANISO, (ca, cb) of Ey and of Ez from that component's eps_r
+     and the id's sigma, the ce of the CPML correction staying the isotropic one;
+     such plans run the two-pass kernels (0, 4, 5, 6) and have no reverse mode */
+  const float* aniso_coef; /* host [n_mat][4] = ca_y, cb_y, ca_z, cb_z, or NULL */
 } rfxh_desc;
 
 typedef struct rfxh_plan rfxh_plan;
@@ -316,6 +324,47 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* desc, int32_t n_theta, const d
 int rfxh_ntff_far_field_noalloc(const rfxh_farfield_desc* desc, int32_t n_theta,
                                 const double* theta, int32_t n_phi, const double* phi,
                                 double* e_theta, double* e_phi, void* stream);
+
+/* ---- Device-side material assembly ----------------------------------------
+ * The per-cell uint16 material ids, built on the GPU in the reference's
+ * order: box fill in insertion order, later boxes winning
+ * (rfx/api/_compile.py:206-224 with Box.mask_on_coords, rfx/geometry/
+ * csg.py:235-307: each box's node set is a product of three index ranges);
+ * CPML pad extension (rfx/geometry/rasterize_grid.py:237-414); tangential
+ * PEC zero bits (rfx/boundaries/pec.py:90-176), folded into the ids through
+ * a table the caller builds from the (id, bits) pairs that occur.  Replaces
+ * the host loops of _assemble_materials (rfx/api/_compile.py:121-367). */
+typedef struct {
+  int32_t lo[3], hi[3];  /* half-open node-index range per axis (lo >= hi: empty) */
+  int32_t id;            /* material id the box paints */
+  int32_t pec;           /* 1: the box marks perfect-conductor cells (sigma >= 1e6)
+                            instead of painting (rfx/api/_compile.py:118, 216-220) */
+} rfxh_asm_box;
+
+/* d_ids (device uint16 [nx][ny][nz]) <- 0 (vacuum) painted by the non-PEC
+ * boxes in order; d_pec (device uint8, same shape) <- 1 where a PEC box
+ * covers the cell.  pads (host [6], or NULL: none) = CPML pad widths lo_x,
+ * hi_x, lo_y, hi_y, lo_z, hi_z: along x, then y, then z, the lo pad copies
+ * the first interior slice and the hi pad the last one, except where that
+ * slice is vacuum and the one inside it is not (then the inner slice, also
+ * written onto the last interior slice).  vacuum: host uint8 [n_mat], 1 for
+ * ids whose (eps_r, sigma, mu_r) is (1, 0, 1).  boxes: host [n_box]. */
+int rfxh_assemble_fill(int32_t nx, int32_t ny, int32_t nz, const rfxh_asm_box* boxes,
+                       int32_t n_box, const int32_t* pads, const uint8_t* vacuum, int32_t n_mat,
+                       uint16_t* d_ids, uint8_t* d_pec, void* stream);
+
+/* The tangential-zero bits of the PEC cells (bit a: E along axis a is zeroed
+ * because a neighbour along a is PEC too, none past the domain): bit b of
+ * d_used[id] (device uint32 [n_mat], caller-zeroed) is set when a PEC cell
+ * with that id carries bits b != 0. */
+int rfxh_assemble_pec_pairs(int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_ids,
+                            const uint8_t* d_pec, uint32_t* d_used, int32_t n_mat, void* stream);
+
+/* Each PEC cell with bits b != 0 takes id map[id * 8 + b] (host uint16
+ * [n_mat * 8]; the caller fills the pairs d_used reported). */
+int rfxh_assemble_pec_apply(int32_t nx, int32_t ny, int32_t nz, uint16_t* d_ids,
+                            const uint8_t* d_pec, const uint16_t* map, int32_t n_mat,
+                            void* stream);
 
 const char* rfxh_last_error(void);
 int rfxh_version(void);

@@ -51,6 +51,7 @@ typedef struct {
   const float *const *plane_freqs;
   double *const *plane_acc; /* complex128 accumulators [nf][n1][n2] */
   const double *const *init_fields; /* optional 6 initial fields */
+  const float *aniso_eps[3];  /* per-component eps_r (subpixel smoothing), NULL = eps_r */
 } oracle_desc;
 
 typedef struct {

@@ -44,6 +44,7 @@ class _Desc(C.Structure):
         ("plane_comp", _I), ("plane_nf", _I),
         ("plane_freqs", C.POINTER(_F)), ("plane_acc", C.POINTER(_D)),
         ("init_fields", C.POINTER(_D)),
+        ("aniso_eps", _F * 3),
     ]
 
 
@@ -103,6 +104,9 @@ def run_c(cfg: o.StepConfig, n_steps: int, double: bool = True,
     d.eps_r = _p(K(np.ascontiguousarray(cfg.eps_r, np.float32)), _F)
     d.sigma = _p(K(np.ascontiguousarray(cfg.sigma, np.float32)), _F)
     d.mu_r = _p(K(np.ascontiguousarray(cfg.mu_r, np.float32)), _F)
+    if cfg.aniso_eps is not None:
+        for q in range(3):
+            d.aniso_eps[q] = _p(K(np.ascontiguousarray(cfg.aniso_eps[q], np.float32)), _F)
     d.cpml_n = cfg.cpml_n
     d.cpml_axes = sum(1 << a for a in range(3) if "xyz"[a] in cfg.cpml_axes) if cfg.cpml_n else 0
     for (ax, side), (b, c, k) in cfg.cpml_profiles.items():

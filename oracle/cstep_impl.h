@@ -16,6 +16,7 @@
 typedef struct {
   REAL *f[6];        /* ex ey ez hx hy hz */
   REAL *ca, *cb, *ch, *ce_p, *ch_p;
+  REAL *ca3[3], *cb3[3];  /* per component (subpixel smoothing), else ca, cb */
   REAL *psi[3][2][2]; /* [axis][side][term]: H psi */
   REAL *psie[3][2][2];/* E psi */
 } FN(state_t);
@@ -121,9 +122,9 @@ static void FN(step_fields)(FN(state_t) *s, const oracle_desc *d) {
         REAL cx = FN(dbwd)(hz, d, i, j, k, 1) - FN(dbwd)(hy, d, i, j, k, 2);
         REAL cy = FN(dbwd)(hx, d, i, j, k, 2) - FN(dbwd)(hz, d, i, j, k, 0);
         REAL cz = FN(dbwd)(hy, d, i, j, k, 0) - FN(dbwd)(hx, d, i, j, k, 1);
-        ex[c] = s->ca[c] * ex[c] + s->cb[c] * cx;
-        ey[c] = s->ca[c] * ey[c] + s->cb[c] * cy;
-        ez[c] = s->ca[c] * ez[c] + s->cb[c] * cz;
+        ex[c] = s->ca3[0][c] * ex[c] + s->cb3[0][c] * cx;
+        ey[c] = s->ca3[1][c] * ey[c] + s->cb3[1][c] * cy;
+        ez[c] = s->ca3[2][c] * ez[c] + s->cb3[2][c] * cz;
       }
   if (d->cpml_n > 0) FN(cpml)(s, d, 0);
   /* pec.py:13-48 */
@@ -225,6 +226,24 @@ static int FN(run)(const oracle_desc *d, oracle_out *out) {
     s.ce_p[c] = dt / ((REAL)d->eps_r[c] * (REAL)ORACLE_EPS0);
     s.ch_p[c] = dt / ((REAL)d->mu_r[c] * (REAL)ORACLE_MU0);
   }
+  /* rfx/core/yee.py:824-901 (update_e_aniso): per-component eps with the
+     cell's isotropic sigma; the CPML correction keeps eps_r (ce_p) */
+  for (int q = 0; q < 3; ++q) {
+    if (!d->aniso_eps[q]) {
+      s.ca3[q] = s.ca;
+      s.cb3[q] = s.cb;
+      continue;
+    }
+    s.ca3[q] = (REAL *)malloc(ncell * sizeof(REAL));
+    s.cb3[q] = (REAL *)malloc(ncell * sizeof(REAL));
+    if (!s.ca3[q] || !s.cb3[q]) return -1;
+    for (long c = 0; c < ncell; ++c) {
+      REAL eps = (REAL)d->aniso_eps[q][c] * (REAL)ORACLE_EPS0;
+      REAL L = (REAL)d->sigma[c] * dt / ((REAL)2 * eps);
+      s.ca3[q][c] = ((REAL)1 - L) / ((REAL)1 + L);
+      s.cb3[q][c] = (dt / eps) / ((REAL)1 + L);
+    }
+  }
   long N[3] = {d->nx, d->ny, d->nz};
   if (d->cpml_n > 0)
     for (int ax = 0; ax < 3; ++ax)
@@ -267,6 +286,11 @@ static int FN(run)(const oracle_desc *d, oracle_out *out) {
     for (int c = 0; c < 6; ++c)
       for (long q = 0; q < ncell; ++q) out->fields[c][q] = (double)s.f[c][q];
   for (int c = 0; c < 6; ++c) free(s.f[c]);
+  for (int q = 0; q < 3; ++q)
+    if (d->aniso_eps[q]) {
+      free(s.ca3[q]);
+      free(s.cb3[q]);
+    }
   for (int c = 0; c < 5; ++c) free(*co[c]);
   if (d->cpml_n > 0)
     for (int ax = 0; ax < 3; ++ax)

@@ -511,6 +511,117 @@ CPML_E_TERMS = {
 }
 
 
+# ---------------------------------------------------------------------------
+# Subpixel smoothing: Kottke-averaged eps per E component
+# (rfx/geometry/smoothing.py:454-611 uniform, :296-452 graded; box SDF
+# :49-65, box normal :113-154, tensor average :244-293).  float32 throughout
+# as the reference computes it with x64 off.
+# ---------------------------------------------------------------------------
+def _sdf_box_f32(X, Y, Z, lo, hi):
+    lo = np.asarray(lo, F32)
+    hi = np.asarray(hi, F32)
+    c = (lo + hi) / F32(2.0)
+    h = (hi - lo) / F32(2.0)
+    dx = np.abs(X - c[0]) - h[0]
+    dy = np.abs(Y - c[1]) - h[1]
+    dz = np.abs(Z - c[2]) - h[2]
+    ox, oy, oz = np.maximum(dx, F32(0)), np.maximum(dy, F32(0)), np.maximum(dz, F32(0))
+    outside = np.sqrt(ox ** 2 + oy ** 2 + oz ** 2)
+    inside = np.minimum(np.maximum(np.maximum(dx, dy), dz), F32(0))
+    return (outside + inside).astype(F32)
+
+
+def _normal_box_f32(X, Y, Z, lo, hi):
+    lo = np.asarray(lo, F32)
+    hi = np.asarray(hi, F32)
+    c = (lo + hi) / F32(2.0)
+    h = (hi - lo) / F32(2.0)
+    rx = np.abs(X - c[0]) - h[0]
+    ry = np.abs(Y - c[1]) - h[1]
+    rz = np.abs(Z - c[2]) - h[2]
+    is_x = (rx >= ry) & (rx >= rz)
+    is_y = (ry > rx) & (ry >= rz)
+    nx = np.where(is_x, np.sign(X - c[0]), F32(0)).astype(F32)
+    ny = np.where(is_y, np.sign(Y - c[1]), F32(0)).astype(F32)
+    nz = np.where(is_x | is_y, F32(0), np.sign(Z - c[2])).astype(F32)
+    mag = np.sqrt(nx ** 2 + ny ** 2 + nz ** 2 + F32(1e-30))
+    return nx / mag, ny / mag, nz / mag
+
+
+def _kottke_f32(f, eps_in, eps_out, n):
+    """Diagonal of the Kottke-averaged eps tensor (smoothing.py:244-293)."""
+    e_in = F32(eps_in)
+    inv_perp = f / e_in + (F32(1.0) - f) / (eps_out + F32(1e-30))
+    eps_par = f * e_in + (F32(1.0) - f) * eps_out
+    inv_par = F32(1.0) / (eps_par + F32(1e-30))
+    out = []
+    for nc in n:
+        nn = nc * nc
+        inv = nn * inv_perp + (F32(1.0) - nn) * inv_par
+        out.append((F32(1.0) / (inv + F32(1e-30))).astype(F32))
+    return out
+
+
+def smoothed_eps(positions, length, shape, boxes, background=1.0):
+    """Per-component eps_r of Kottke subpixel smoothing for axis-aligned
+    boxes.  positions[comp] = (X, Y, Z) coordinate arrays (broadcastable to
+    the grid) of that component's sample points, length = the cell length
+    dividing the SDF (dx, or the graded lane's local geometric mean);
+    boxes = [(lo, hi, eps_r)] in insertion order.  Boxes with the same eps_r
+    form one group: union SDF (minimum), the normal of the box whose surface
+    is nearest (first on ties); per group and component, cells with fill
+    fraction f = clip(0.5 - sdf/length, 0, 1) of 1 take eps_r, cells with
+    0 < f < 1 the Kottke average against the current value."""
+    eps = [np.full(shape, F32(background), F32) for _ in range(3)]
+    groups = {}
+    for lo, hi, e in boxes:
+        groups.setdefault(float(e), []).append((lo, hi))
+    for e_r, members in groups.items():
+        for comp in range(3):
+            X, Y, Z = positions[comp]
+            sdf = best = None
+            nrm = None
+            for lo, hi in members:
+                sd = np.broadcast_to(_sdf_box_f32(X, Y, Z, lo, hi), shape)
+                nv = [np.broadcast_to(v, shape) for v in _normal_box_f32(X, Y, Z, lo, hi)]
+                if sdf is None:
+                    sdf, best, nrm = sd, np.abs(sd), nv
+                else:
+                    sdf = np.minimum(sdf, sd)
+                    closer = np.abs(sd) < best
+                    best = np.where(closer, np.abs(sd), best)
+                    nrm = [np.where(closer, a, b) for a, b in zip(nv, nrm)]
+            f = np.clip(F32(0.5) - sdf / length, F32(0.0), F32(1.0)).astype(F32)
+            inside = f >= F32(1.0)
+            bnd = (f > F32(0.0)) & (f < F32(1.0))
+            smooth = _kottke_f32(f, e_r, eps[comp], nrm)[comp]
+            eps[comp] = np.where(inside, F32(e_r), np.where(bnd, smooth, eps[comp])).astype(F32)
+    return tuple(eps)
+
+
+def smoothed_eps_uniform(g, boxes):
+    """smoothing.py:454-611: node coordinates f32(i - pad) * f32(dx), the
+    component's axis shifted by half a cell, SDF over dx."""
+    x, y, z = g.node_coords_f32()
+    half = F32(g.dx * 0.5)
+    X, Y, Z = x[:, None, None], y[None, :, None], z[None, None, :]
+    pos = [(X + half, Y, Z), (X, Y + half, Z), (X, Y, Z + half)]
+    return smoothed_eps(pos, F32(g.dx), g.shape, boxes)
+
+
+def smoothed_eps_graded(g, boxes):
+    """smoothing.py:296-452: centres node + d/2 along the component's axis,
+    SDF over the local geometric mean (dx dy dz)^(1/3)."""
+    nodes = g.node_coords_f32()
+    d = [np.asarray(v, F32) for v in g.d32]
+    cen = [nodes[a] + d[a] / F32(2.0) for a in range(3)]
+    X, Y, Z = nodes[0][:, None, None], nodes[1][None, :, None], nodes[2][None, None, :]
+    Xc, Yc, Zc = cen[0][:, None, None], cen[1][None, :, None], cen[2][None, None, :]
+    length = (d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :]) ** F32(1.0 / 3.0)
+    pos = [(Xc, Y, Z), (X, Yc, Z), (X, Y, Zc)]
+    return smoothed_eps(pos, length.astype(F32), g.shape, boxes)
+
+
 @dataclass
 class StepConfig:
     """Everything the time loop needs, already set up on the host."""
@@ -541,6 +652,9 @@ class StepConfig:
     debye_masks: list = field(default_factory=list)
     lorentz_poles: list = field(default_factory=list)
     lorentz_masks: list = field(default_factory=list)
+    # subpixel smoothing: per-component eps_r (eps_ex, eps_ey, eps_ez) for
+    # the E update's ca, cb (rfx/core/yee.py:824-901, graded :668-719)
+    aniso_eps: tuple | None = None
 
 
 class Stepper:
@@ -564,6 +678,15 @@ class Stepper:
         self.ch_cpml = dt / (cfg.mu_r.astype(dtype) * dtype(MU_0))
         self.f = {c: np.zeros(cfg.shape, dtype) for c in COMPS}
         self.debye = self.lorentz = None
+        self.aniso = None
+        if cfg.aniso_eps is not None and not cfg.debye_poles and not cfg.lorentz_poles:
+            # yee.py:880-895: per component, sigma isotropic
+            self.aniso = []
+            for e_c in cfg.aniso_eps:
+                ea = e_c.astype(dtype) * dtype(EPS_0)
+                La = sig * dt / (dtype(2.0) * ea)
+                self.aniso.append(((dtype(1.0) - La) / (dtype(1.0) + La),
+                                   (dt / ea) / (dtype(1.0) + La)))
         z3 = lambda: {c: np.zeros(cfg.shape, dtype) for c in ("ex", "ey", "ez")}  # noqa: E731
         if cfg.debye_poles:
             # with any pole every cell takes the dispersive ca, cb
@@ -633,6 +756,12 @@ class Stepper:
         cy = self._dbwd(f["hx"], 2) - self._dbwd(f["hz"], 0)
         cz = self._dbwd(f["hy"], 0) - self._dbwd(f["hx"], 1)
         if self.debye is None and self.lorentz is None:
+            if self.aniso is not None:
+                (ca0, cb0), (ca1, cb1), (ca2, cb2) = self.aniso
+                f["ex"] = ca0 * f["ex"] + cb0 * cx
+                f["ey"] = ca1 * f["ey"] + cb1 * cy
+                f["ez"] = ca2 * f["ez"] + cb2 * cz
+                return
             f["ex"] = self.ca * f["ex"] + self.cb * cx
             f["ey"] = self.ca * f["ey"] + self.cb * cy
             f["ez"] = self.ca * f["ez"] + self.cb * cz
@@ -1042,6 +1171,11 @@ class Pulse:
     cutoff: float = 3.0
 
 
+def _smoothing_boxes(spec):
+    mats = spec.get("materials", {})
+    return [(lo, hi, mats[name][0]) for lo, hi, name in spec.get("geometry", [])]
+
+
 def build_uniform(spec, n_steps):
     """Build a StepConfig for the uniform lane from a high-level spec.
 
@@ -1142,6 +1276,10 @@ def build_uniform(spec, n_steps):
                      probes=probes, dft_planes=planes, ports_vi=ports_vi, flux=flux)
     cfg.flux_area = areas
     cfg.ntff = _ntff_entries(spec, g)
+    if spec.get("subpixel") and lane == "run":
+        # rfx/runners/uniform.py:271-278: every geometry entry with its
+        # material's eps_r, background 1
+        cfg.aniso_eps = smoothed_eps_uniform(g, _smoothing_boxes(spec))
     if spec.get("debye"):
         cfg.debye_poles, cfg.debye_masks = debye_pole_masks(
             shape, coords, spec.get("geometry", []), spec["debye"])
@@ -1321,6 +1459,9 @@ def build_graded(spec, n_steps):
                      ports_vi=ports_vi, record_order="graded")
     cfg.flux, cfg.flux_area = _flux_entries(spec, g, n_steps, True)
     cfg.ntff = _ntff_entries(spec, g)
+    if spec.get("subpixel") and not spec.get("debye") and not spec.get("lorentz"):
+        # rfx/runners/nonuniform.py:648-672 (not with dispersion)
+        cfg.aniso_eps = smoothed_eps_graded(g, _smoothing_boxes(spec))
     if spec.get("debye"):
         cfg.debye_poles, cfg.debye_masks = debye_pole_masks(
             shape, coords, spec.get("geometry", []), spec["debye"])

@@ -20,6 +20,7 @@ RFXH_ZERO_EX = 1
 RFXH_ZERO_EY = 2
 RFXH_ZERO_EZ = 4
 RFXH_POINT = 8
+RFXH_ANISO = 16
 RFXH_POLES = 0xFFFF0000
 OP_PORT, OP_INJECT, OP_PROBE = 0, 1, 2
 
@@ -81,6 +82,7 @@ class Desc(C.Structure):
         ("n_ntff", C.c_int32), ("ntff", C.POINTER(Ntff)),
         ("n_pole", C.c_int32), ("n_debye", C.c_int32), ("pole_coef", _FP), ("pol_lo", C.c_int32 * 3),
         ("pol_n", C.c_int32 * 3), ("d_pol", _P),
+        ("aniso_coef", _FP),
     ]
 
 
@@ -98,13 +100,19 @@ class AdjointDesc(C.Structure):
     ]
 
 
+class AsmBox(C.Structure):
+    _fields_ = [("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3), ("id", C.c_int32),
+                ("pec", C.c_int32)]
+
+
 _lib = None
 
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
            "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
            "rfxh_batch_run", "rfxh_set_variant", "rfxh_get_variant", "rfxh_ntff_far_field",
-           "rfxh_ntff_far_field_noalloc", "rfxh_last_error", "rfxh_version")
+           "rfxh_ntff_far_field_noalloc", "rfxh_assemble_fill", "rfxh_assemble_pec_pairs",
+           "rfxh_assemble_pec_apply", "rfxh_last_error", "rfxh_version")
 
 
 class RfxHipError(RuntimeError):
@@ -157,6 +165,16 @@ def load():
     lib.rfxh_ntff_far_field.restype = C.c_int
     lib.rfxh_ntff_far_field_noalloc.argtypes = lib.rfxh_ntff_far_field.argtypes
     lib.rfxh_ntff_far_field_noalloc.restype = C.c_int
+    lib.rfxh_assemble_fill.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(AsmBox),
+                                       C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
+                                       C.c_int32, _P, _P, _P]
+    lib.rfxh_assemble_fill.restype = C.c_int
+    lib.rfxh_assemble_pec_pairs.argtypes = [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P,
+                                            C.c_int32, _P]
+    lib.rfxh_assemble_pec_pairs.restype = C.c_int
+    lib.rfxh_assemble_pec_apply.argtypes = [C.c_int32, C.c_int32, C.c_int32, _P, _P,
+                                            C.POINTER(C.c_uint16), C.c_int32, _P]
+    lib.rfxh_assemble_pec_apply.restype = C.c_int
     lib.rfxh_last_error.argtypes = []
     lib.rfxh_last_error.restype = C.c_char_p
     lib.rfxh_version.argtypes = []

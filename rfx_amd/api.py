@@ -450,7 +450,25 @@ class Simulation:
         mat = MaterialMap(shape)
         coords = grid.node_coords()
         pec = None
-        for box, name in self._geometry:
+        has_poles = any(self._materials[name].debye or self._materials[name].lorentz
+                        for _, name in self._geometry)
+        overridden = (eps_override is not None or sigma_override is not None
+                      or mu_r_override is not None)
+        dasm = None
+        if self._use_device_assembly(shape, material_sweep is not None or has_poles or overridden):
+            # box fill, pads and PEC bits on the GPU; single-cell edits (ports)
+            # go through the same interface as on the host map
+            from .assembly import DeviceAssembly
+            dasm = DeviceAssembly(mat, self._device or "cuda")
+            boxes = []
+            for box, name in self._geometry:
+                m = self._materials[name]
+                props = None if m.sigma >= PEC_SIGMA_THRESHOLD else (m.eps_r, m.sigma, m.mu_r)
+                boxes.append((box.axis_masks(coords), props))
+            dasm.fill(boxes, grid.face_pads if self._cpml_layers > 0 else None)
+            pec = dasm.pec
+            mat = dasm
+        for box, name in ([] if dasm is not None else self._geometry):
             m = self._materials[name]
             masks = box.axis_masks(coords)
             if m.sigma >= PEC_SIGMA_THRESHOLD:
@@ -466,7 +484,7 @@ class Simulation:
                 if name == sw_name:
                     sel[np.ix_(*[np.nonzero(x)[0] for x in box.axis_masks(coords)])] = True
             mat.set_field_where(sel, sw_field, sw_value)
-        if self._cpml_layers > 0:
+        if self._cpml_layers > 0 and dasm is None:
             mat.extend_pads(grid.face_pads)
         if material_sweep is not None and material_sweep[0] is None:
             mat.set_field_all(material_sweep[1], material_sweep[2])
@@ -536,7 +554,9 @@ class Simulation:
             if s11_freqs is not None:
                 ports.append((idx, pe.component, float(grid.dx), (grid.dx,) * 3, pe.impedance,
                               pe.excite))
-        if pec is not None:
+        if dasm is not None:
+            mat = dasm.finish()
+        elif pec is not None:
             mat.add_flags(tangential_edge_bits(pec))
         probes = [(grid.position_to_index(p.position), p.component) for p in self._probes]
         if not probes and lane == "forward" and self._ports:
@@ -574,6 +594,30 @@ class Simulation:
         prob.flux_areas = areas
         prob.ntff_box = box
         return grid, prob, ports
+
+    # material assembly on the GPU: "auto" (grids of DEVICE_ASSEMBLY_CELLS cells
+    # or more, when a GPU is visible), "device" or "host"
+    material_assembly = "auto"
+    DEVICE_ASSEMBLY_CELLS = 1 << 21
+
+    def _use_device_assembly(self, shape, host_only: bool) -> bool:
+        """Device assembly covers box fill, pads, port cells and PEC bits;
+        material sweeps, dispersion poles and overrides edit the ids on the
+        host, so those problems keep the host assembly."""
+        how = self.material_assembly
+        if how not in ("auto", "device", "host"):
+            raise ValueError(f"material_assembly must be 'auto', 'device' or 'host', got {how!r}")
+        if how == "host" or host_only:
+            if how == "device" and host_only:
+                raise NotImplementedError("device material assembly does not cover material "
+                                          "sweeps, dispersion poles or overrides")
+            return False
+        if not torch.cuda.is_available():
+            if how == "device":
+                from ._lib import RfxHipError
+                raise RfxHipError("device material assembly needs a GPU")
+            return False
+        return how == "device" or int(np.prod(shape)) >= self.DEVICE_ASSEMBLY_CELLS
 
     def _port_cells(self, grid, pe):
         """Cells a port entry touches (wire ports: every extent cell)."""

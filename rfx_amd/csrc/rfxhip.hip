@@ -101,6 +101,7 @@ struct KParams {
   int slo[3], shi[3];            // CPML slab extents: cells i < slo or i >= shi per axis
   float ch_all;                  // > 0: every material has this ch (one mu_r), and the
                                  // column H pass reads no material id
+  const float4* acoef;           // RFXH_ANISO ids: ca, cb of Ey and of Ez
 };
 
 struct FusedIO {
@@ -535,8 +536,15 @@ __device__ __forceinline__ float3 e_core_co(const KParams& p, const PsiIO& ps, i
   const float ca = co.x, cb = co.y, ce = co.z;
   const float o0 = e0, o1 = e1, o2 = e2;
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
-  e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
-  e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
+  if (DISP && (fl & RFXH_ANISO)) {
+    // per-component eps (subpixel smoothing): the row holds Ex's ca, cb
+    const float4 a = gld(p.acoef, (int)ldu16(p.mat, c));
+    e1 = a.x * e1 + a.y * ((hx - hx_k) * iz - (hz - hz_i) * ix);
+    e2 = a.z * e2 + a.w * ((hy - hy_i) * ix - (hx - hx_j) * iy);
+  } else {
+    e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
+    e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
+  }
   if (DISP && (fl & RFXH_POLES)) dispersion_cell(p, ps, i, j, k, c, o0, o1, o2, e0, e1, e2);
   if (in_slab(p, i, j, k)) {
     // keep the CPML pointer and profile loads inside the slab branch: the
@@ -1228,110 +1236,22 @@ __device__ __forceinline__ void tb_point_ops(const KParams& p, const TBShared& s
   }
 }
 
-// CPML faces of the sweep's cells.  A slab is thinner than half its axis,
-// so a cell lies in at most one face per axis; the y and z faces of a
-// thread are fixed along the x march (per row and per lane) and resolved
-// once, the x face is the plane's (the same for the whole workgroup).
-struct TBCol {
-  int ly, lz;        // layer in the y / z face, -1: none
-  int fy, fz;        // that face (2 or 3, 4 or 5)
-  unsigned by, bz;   // psi index of term 0 at plane 0
-  unsigned sy, sz;   // psi index step per plane
-};
-
-__device__ __forceinline__ TBCol tb_col(const KParams& p, const TBCpml& cm, bool valid, int j,
-                                        int k) {
-  TBCol col;
-  col.ly = col.lz = -1;
-  col.fy = 2;
-  col.fz = 4;
-  col.by = col.bz = col.sy = col.sz = 0u;
-  if (!valid) return col;
-  if (j < p.cn[2]) {
-    col.ly = j;
-  } else if (j >= p.ny - p.cn[3]) {
-    col.fy = 3;
-    col.ly = j - (p.ny - p.cn[3]);
-  }
-  if (col.ly >= 0) {
-    // y face slab [i][l][k]
-    const unsigned n = (unsigned)(col.fy == 2 ? p.cn[2] : p.cn[3]);
-    col.by = (col.fy == 2 ? cm.foff[2] : cm.foff[3]) + (unsigned)col.ly * (unsigned)p.nz +
-             (unsigned)k;
-    col.sy = n * (unsigned)p.nz;
-  }
-  if (k < p.cn[4]) {
-    col.lz = k;
-  } else if (k >= p.nz - p.cn[5]) {
-    col.fz = 5;
-    col.lz = k - (p.nz - p.cn[5]);
-  }
-  if (col.lz >= 0) {
-    // z face slab [i][j][l]
-    const unsigned n = (unsigned)(col.fz == 4 ? p.cn[4] : p.cn[5]);
-    col.bz = (col.fz == 4 ? cm.foff[4] : cm.foff[5]) + (unsigned)j * n + (unsigned)col.lz;
-    col.sz = (unsigned)p.ny * n;
-  }
-  return col;
-}
-
-// The face on axis a holding cell (q, j, k) (c0 = j nz + k): psi index of
-// its term 0, the term stride, the profile row and the inverse spacing;
-// false when the cell lies in no face on that axis.
-__device__ __forceinline__ bool tb_face(const KParams& p, const TBCpml& cm, const TBCol& col,
-                                        int a, int q, unsigned c0, unsigned& idx, unsigned& ts,
-                                        int& prow, float& inv) {
-  if (a == 0) {
-    int f, lx;
-    if (q >= 0 && q < p.cn[0]) {
-      f = 0;
-      lx = q;
-    } else if (q >= p.nx - p.cn[1] && q < p.nx) {
-      f = 1;
-      lx = q - (p.nx - p.cn[1]);
-    } else {
-      return false;
-    }
-    // x face slab [l][j][k]
-    idx = (f == 0 ? cm.foff[0] : cm.foff[1]) + (unsigned)lx * (unsigned)p.plane + c0;
-    ts = f == 0 ? cm.fs[0] : cm.fs[1];
-    prow = (f == 0 ? cm.poff[0] : cm.poff[1]) + lx;
-    inv = f == 0 ? p.cinv[0] : p.cinv[1];
-    return true;
-  }
-  if (a == 1) {
-    if (col.ly < 0) return false;
-    const bool lo = col.fy == 2;
-    idx = col.by + (unsigned)q * col.sy;
-    ts = lo ? cm.fs[2] : cm.fs[3];
-    prow = (lo ? cm.poff[2] : cm.poff[3]) + col.ly;
-    inv = lo ? p.cinv[2] : p.cinv[3];
-    return true;
-  }
-  if (col.lz < 0) return false;
-  const bool lo = col.fz == 4;
-  idx = col.bz + (unsigned)q * col.sz;
-  ts = lo ? cm.fs[4] : cm.fs[5];
-  prow = (lo ? cm.poff[4] : cm.poff[5]) + col.lz;
-  inv = lo ? p.cinv[4] : p.cinv[5];
-  return true;
-}
-
 // H at one cell with the CPML memory in registers: h_core's update reading
 // the memory from pin and leaving the new memory in pout (and, when store,
 // in cm.psw).
 template <bool GRADED, bool CPML = true>
-__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const TBCol& col,
-                                       const float4* prof, int i, unsigned c0, float gy, float gz,
+__device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const float4* prof,
+                                       int i, int j, int k, float gy, float gz,
                                        float ch, float ex, float ey, float ez, float ex_j,
                                        float ez_j, float ex_k, float ey_k, float ey_i, float ez_i,
                                        float hx, float hy, float hz, const float (&pin)[3][2],
                                        float (&pout)[3][2], bool store) {
   // graded: gy, gz are the thread's own inverse spacings along y and z
-  // (loaded once), the x one is wave-uniform
+  // (loaded once), the x one is wave-uniform; the pipeline also visits
+  // planes outside the grid (results discarded): clamp the index
   float ix, iy, iz;
   if (GRADED) {
-    ix = gld(p.inv_h[0], i);
+    ix = gld(p.inv_h[0], min(max(i, 0), p.nx - 1));
     iy = gy;
     iz = gz;
   } else {
@@ -1340,14 +1260,17 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const
   hx = hx - ch * ((ez_j - ez) * iy - (ey_k - ey) * iz);
   hy = hy - ch * ((ex_k - ex) * iz - (ez_i - ez) * ix);
   hz = hz - ch * ((ey_i - ey) * ix - (ex_j - ex) * iy);
-  if (CPML) {
+  if (CPML && in_slab(p, i, j, k)) {
+    __asm__ volatile("" ::: "memory");
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      unsigned q, ts;
-      int prow;
-      float inv;
-      if (!tb_face(p, cm, col, ax, i, c0, q, ts, prow, inv)) continue;
-      const float4 pr = prof[prow];
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const unsigned q = cm.foff[face] + (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+      const unsigned ts = cm.fs[face];
+      const float4 pr = prof[cm.poff[face] + l];
+      const float inv = p.cinv[face];
       const float bb = pr.x, cc = pr.y, km1 = pr.z;
       float g0, g1;
       if (ax == 0) {
@@ -1388,16 +1311,16 @@ __device__ __forceinline__ float3 tb_h(const KParams& p, const TBCpml& cm, const
 // E at one cell with the CPML memory in registers (e_core_co's update),
 // point operations at step n (recorded only when own).
 template <bool GRADED, bool CPML = true>
-__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const TBCol& col,
-                                       const float4* prof, int i, int j, int k, unsigned c0,
-                                       float gy, float gz, unsigned c, float4 co, uint32_t fl, float e0, float e1,
+__device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const float4* prof,
+                                       int i, int j, int k, float gy, float gz, unsigned c,
+                                       float4 co, uint32_t fl, float e0, float e1,
                                        float e2, float hx, float hy, float hz, float hy_i,
                                        float hz_i, float hx_j, float hz_j, float hx_k, float hy_k,
                                        const float (&pin)[3][2], float (&pout)[3][2], bool store,
                                        const TBShared& sh, long long n, int s, bool own) {
   float ix, iy, iz;
   if (GRADED) {
-    ix = gld(p.inv_e[0], i);
+    ix = gld(p.inv_e[0], min(max(i, 0), p.nx - 1));
     iy = gy;
     iz = gz;
   } else {
@@ -1407,14 +1330,17 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const
   e0 = ca * e0 + cb * ((hz - hz_j) * iy - (hy - hy_k) * iz);
   e1 = ca * e1 + cb * ((hx - hx_k) * iz - (hz - hz_i) * ix);
   e2 = ca * e2 + cb * ((hy - hy_i) * ix - (hx - hx_j) * iy);
-  if (CPML) {
+  if (CPML && in_slab(p, i, j, k)) {
+    __asm__ volatile("" ::: "memory");
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      unsigned q, ts;
-      int prow;
-      float inv;
-      if (!tb_face(p, cm, col, ax, i, c0, q, ts, prow, inv)) continue;
-      const float4 pr = prof[prow];
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const unsigned q = cm.foff[face] + (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+      const unsigned ts = cm.fs[face];
+      const float4 pr = prof[cm.poff[face] + l];
+      const float inv = p.cinv[face];
       const float bb = pr.x, cc = pr.y, km1 = pr.z;
       float g0, g1;
       if (ax == 0) {
@@ -1495,7 +1421,6 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     gey = gld(p.inv_e[1], j);
     gez = gld(p.inv_e[2], k);
   }
-  const TBCol col = tb_col(p, cm, CPML && valid, j, k);
   auto pin = [&](int q) { return valid && q >= 0 && q < p.nx; };
   auto own = [&](int q) { return interior && q >= i0 && q < i1; };
   // neighbour value of stage st from LDS buffer b; zero outside the tile
@@ -1509,30 +1434,28 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   auto ld_psi = [&](float (&dst)[3][2], const float* base, unsigned term0, int q, bool use) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) dst[a][0] = dst[a][1] = 0.f;
-    if (CPML && use && pin(q)) {
+    if (CPML && use && pin(q) && in_slab(p, q, j, k)) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        unsigned qq, ts;
-        int prow;
-        float inv;
-        if (!tb_face(p, cm, col, a, q, c0, qq, ts, prow, inv)) continue;
-        qq += term0 * ts;
-        dst[a][0] = ldf(base, qq);
-        dst[a][1] = ldf(base, qq + ts);
+      for (int face = 0; face < 6; ++face) {
+        const int ly = slab_layer(p, face, q, j, k);
+        if (ly < 0) continue;
+        const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
+                            (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
+        dst[face >> 1][0] = ldf(base, qq);
+        dst[face >> 1][1] = ldf(base, qq + cm.fs[face]);
       }
     }
   };
   auto st_psi = [&](float* base, unsigned term0, int q, const float (&src)[3][2], bool use) {
-    if (CPML && use && pin(q)) {
+    if (CPML && use && pin(q) && in_slab(p, q, j, k)) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        unsigned qq, ts;
-        int prow;
-        float inv;
-        if (!tb_face(p, cm, col, a, q, c0, qq, ts, prow, inv)) continue;
-        qq += term0 * ts;
-        stf(base, qq, src[a][0]);
-        stf(base, qq + ts, src[a][1]);
+      for (int face = 0; face < 6; ++face) {
+        const int ly = slab_layer(p, face, q, j, k);
+        if (ly < 0) continue;
+        const unsigned qq = cm.foff[face] + term0 * cm.fs[face] +
+                            (unsigned)psi_idx(face, ly, q, j, k, p.cn[face], p.ny, p.nz);
+        stf(base, qq, src[face >> 1][0]);
+        stf(base, qq + cm.fs[face], src[face >> 1][1]);
       }
     }
   };
@@ -1594,11 +1517,28 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     h0n[c] = pin(i0 - 3) ? fh[0][c] : 0.f;
   }
   m1 = fm[0];
+  // Priming in place of a first iteration at pp = i0-2 (whose updates no
+  // output needs): publish E^n of plane i0-2 for the first H update's y/z
+  // neighbours and shift the planes by one.  The other stages of buffer 0
+  // hold the zeros written at the kernel's start.
+  s_x[0][0][0][w][l] = e0n[0];
+  s_x[0][0][1][w][l] = e0n[1];
+  s_x[0][0][2][w][l] = e0n[2];
+  prefetch(i0 - 2, fe[1], fh[1], fm[1]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    e0c[c] = e0n[c];
+    e0n[c] = pin(i0 - 1) ? fe[1][c] : 0.f;
+    h0n[c] = pin(i0 - 2) ? fh[1][c] : 0.f;
+  }
+  m2 = m1;
+  m1 = fm[1];
   // The CPML body loads its memory terms under branches, and the compiler
   // then waits for every load in flight before using them: a second slot
   // would only cost registers there, so it prefetches one plane ahead.
   constexpr bool kDeep = !CPML;
-  if (kDeep) prefetch(i0 - 2, fe[1], fh[1], fm[1]);
+  if (kDeep) prefetch(i0 - 1, fe[0], fh[0], fm[0]);
+  __syncthreads();
 
   // one iteration: planes pp-4 .. pp-1; LDS buffer wb written, 1 - wb read;
   // (ce, chh, cmm) the slot holding prefetch(pp) at the iteration's end,
@@ -1629,7 +1569,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const unsigned c = (unsigned)q * P + c0;
       float dummy[3][2];
       const float4 row = s_ecoef[m4];
-      e2 = tb_e<GRADED, CPML>(p, cm, col, sh.prof, q, j, k, c0, gey, gez, c, row, __float_as_uint(row.w), e1b[0], e1b[1],
+      e2 = tb_e<GRADED, CPML>(p, cm, sh.prof, q, j, k, gey, gez, c, row, __float_as_uint(row.w), e1b[0], e1b[1],
                               e1b[2], h2a[0], h2a[1], h2a[2], h2by, h2bz, hx_j, hz_j, hx_k, hy_k,
                               pD, dummy, o, sh, n, 1, o);
       if (!pin(q)) e2 = make_float3(0.f, 0.f, 0.f);
@@ -1652,7 +1592,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       const float4 row = s_ecoef[m2];
-      e1 = tb_e<GRADED, CPML>(p, cm, col, sh.prof, q, j, k, c0, gey, gez, c, row, __float_as_uint(row.w), e0b[0], e0b[1],
+      e1 = tb_e<GRADED, CPML>(p, cm, sh.prof, q, j, k, gey, gez, c, row, __float_as_uint(row.w), e0b[0], e0b[1],
                               e0b[2], h1a[0], h1a[1], h1a[2], h1b[1], h1b[2], hx_j, hz_j, hx_k,
                               hy_k, pB, pe_new, false, sh, n, 0, o);
       if (!pin(q)) e1 = make_float3(0.f, 0.f, 0.f);
@@ -1668,7 +1608,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       float dummy[3][2];
-      h2 = tb_h<GRADED, CPML>(p, cm, col, sh.prof, q, c0, ghy, ghz, chc > 0.f ? chc : s_ch[m3], e1a[0], e1a[1], e1a[2],
+      h2 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m3], e1a[0], e1a[1], e1a[2],
                               ex_j, ez_j, ex_k, ey_k, e1.y, e1.z, h1b[0], h1b[1], h1b[2], pC,
                               dummy, o);
       if (!pin(q)) h2 = make_float3(0.f, 0.f, 0.f);
@@ -1688,7 +1628,7 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       const int q = pp - 1;
       const float ex_j = nb(rb, 0, 0, w + 1, l), ez_j = nb(rb, 0, 2, w + 1, l);
       const float ex_k = nb(rb, 0, 0, w, l + 1), ey_k = nb(rb, 0, 1, w, l + 1);
-      h1 = tb_h<GRADED, CPML>(p, cm, col, sh.prof, q, c0, ghy, ghz, chc > 0.f ? chc : s_ch[m1], e0c[0], e0c[1], e0c[2],
+      h1 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m1], e0c[0], e0c[1], e0c[2],
                               ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1], h0n[2], pA,
                               ph_new, false);
       if (!pin(q)) h1 = make_float3(0.f, 0.f, 0.f);
@@ -1742,13 +1682,13 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   // an even number of iterations (one past the last plane's when the count
   // is odd: it stores nothing the workgroup owns, and the H memory it may
   // store is the value every other copy of that cell stores)
-  for (int pp = i0 - 2; pp < i1 + 4; pp += 2) {
+  for (int pp = i0 - 1; pp < i1 + 4; pp += 2) {
     if (kDeep) {
-      iteration(pp, 0, fe[1], fh[1], fm[1], fe[0], fh[0], fm[0], pp + 1);
-      iteration(pp + 1, 1, fe[0], fh[0], fm[0], fe[1], fh[1], fm[1], pp + 2);
+      iteration(pp, 1, fe[0], fh[0], fm[0], fe[1], fh[1], fm[1], pp + 1);
+      iteration(pp + 1, 0, fe[1], fh[1], fm[1], fe[0], fh[0], fm[0], pp + 2);
     } else {
-      iteration(pp, 0, fe[0], fh[0], fm[0], fe[0], fh[0], fm[0], pp);
-      iteration(pp + 1, 1, fe[0], fh[0], fm[0], fe[0], fh[0], fm[0], pp + 1);
+      iteration(pp, 1, fe[0], fh[0], fm[0], fe[0], fh[0], fm[0], pp);
+      iteration(pp + 1, 0, fe[0], fh[0], fm[0], fe[0], fh[0], fm[0], pp + 1);
     }
   }
 }
@@ -1768,14 +1708,19 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
   }
   // the plan rejects this variant for more than kTBProf CPML layers in all
   for (int t = tid; t < io.n_prof; t += kBX * kTBW) sh.prof[t] = gld(io.cm.prof, t);
-  // the first iteration reads buffer 1 (no plane before the segment start)
-  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&sh.x[1][0][0][0][0])[t] = 0.f;
+  // the first iteration reads buffer 0 (no plane before the segment start
+  // but the primed E^n)
+  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&sh.x[0][0][0][0][0])[t] = 0.f;
   __syncthreads();
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
   const int tk = L % io.ntk;
   const int r = L / io.ntk;
   const int tj = r % io.ntj;
-  const int seg = r / io.ntj;   // 0: x-lo slab, 1..n_mid: middle, n_mid + 1: x-hi slab
+  // dispatch order: the two x-slab segments (the slowest workgroups, CPML
+  // on every cell) first, so that they do not form the launch's tail;
+  // then the middle ones.  seg: 0 x-lo slab, 1..n_mid middle, n_mid + 1 x-hi
+  const int so = r / io.ntj;
+  const int seg = so == 0 ? 0 : so == 1 ? io.n_mid + 1 : so - 1;
   int i0, i1;
   if (seg == 0) {
     i0 = 0;
@@ -2811,6 +2756,7 @@ struct rfxh_plan {
   int ncol = 1;                 // two-pass step: cells per thread along x (1, 2, 3, 4)
   bool wmarch = false;          // the ping-pong step uses the wave-march kernel
   int wmarch_seg = 0;           // its x segments (0: automatic)
+  bool aniso = false;           // ids with per-component eps (subpixel smoothing)
   bool tb = false;              // two steps per sweep (temporal-blocked march), ping-pong
   int tb_seg = 0;               // its x segments (0: automatic)
   int tb_ops_max = 0;           // most point operations in one of its tile columns
@@ -2938,7 +2884,8 @@ void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const 
       if (kp.ch_all > 0.f) hipLaunchKernelGGL((k_update_h<GRADED, true>), g, blk, 0, s, kp, ps);
       else hipLaunchKernelGGL((k_update_h<GRADED, false>), g, blk, 0, s, kp, ps);
     }
-    else if (pl->n_pole > 0) hipLaunchKernelGGL((k_update_e<GRADED, true>), g, blk, 0, s, kp, ps);
+    else if (pl->n_pole > 0 || pl->aniso)
+      hipLaunchKernelGGL((k_update_e<GRADED, true>), g, blk, 0, s, kp, ps);
     else hipLaunchKernelGGL((k_update_e<GRADED, false>), g, blk, 0, s, kp, ps);
   } else {
     const PsiIO* psp = pl->d_psio + set;
@@ -2948,7 +2895,7 @@ void launch_pass_n(const rfxh_plan* pl, hipStream_t s, const KParams& kp, const 
       else
         hipLaunchKernelGGL((k_update_h_col<GRADED, NP, false>), g, blk, 0, s, kp, psp);
     } else {
-      if (pl->n_pole > 0)
+      if (pl->n_pole > 0 || pl->aniso)
         hipLaunchKernelGGL((k_update_e_col<GRADED, NP, true>), g, blk, 0, s, kp, psp);
       else
         hipLaunchKernelGGL((k_update_e_col<GRADED, NP, false>), g, blk, 0, s, kp, psp);
@@ -3352,7 +3299,7 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   // need the second field set); rfxh_set_variant changes it later
   const char* kern = std::getenv("RFXH_KERNEL");
   const std::string kname = kern ? kern : "";
-  pl->fused = has2 && d->n_pole == 0 && (kname == "fused" || kname == "march");
+  pl->fused = has2 && d->n_pole == 0 && !d->aniso_coef && (kname == "fused" || kname == "march");
   pl->march = has2 && kname == "march";
   pl->ncol = (pl->fused || kname == "onecell") ? 1 : 2;
   for (int c = 0; c < 6; ++c) {
@@ -3420,6 +3367,21 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
     for (int m = 0; m < d->n_mat; ++m)
       if (d->mat_flags[m] & RFXH_POLES)
         return bail(fail(RFXH_EINVAL, "material flags name poles but n_pole is 0"));
+  }
+  // subpixel smoothing: rows (ca, cb of Ey, of Ez) for the flagged ids
+  {
+    bool flagged = false;
+    for (int m = 0; m < d->n_mat; ++m) flagged = flagged || (d->mat_flags[m] & RFXH_ANISO);
+    if (flagged && !d->aniso_coef)
+      return bail(fail(RFXH_EINVAL, "material flags name per-component eps but aniso_coef is NULL"));
+    if (d->aniso_coef) {
+      float4* ac;
+      if ((rc = dev_copy(pl, reinterpret_cast<const float4*>(d->aniso_coef), (size_t)d->n_mat,
+                         &ac)))
+        return bail(rc);
+      kp.acoef = ac;
+      pl->aniso = true;
+    }
   }
   // CPML set 1 is addressed as set-0 pointer + psi_set floats
   pl->psi_set = (has2 && any_cpml) ? (long long)(d->d_psi2 - d->d_psi) : 0;
@@ -3837,8 +3799,9 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
   if (kernel == 8) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     if (!pl->set[1][0]) return fail(RFXH_EINVAL, "the temporal-blocked march needs a second field set");
-    if (pl->n_pole > 0)
-      return fail(RFXH_EINVAL, "dispersive (Debye / Lorentz) materials run on the two-pass kernels (0, 4, 5, 6)");
+    if (pl->n_pole > 0 || pl->aniso)
+      return fail(RFXH_EINVAL, "dispersive (Debye / Lorentz) and subpixel-smoothed materials run on "
+                               "the two-pass kernels (0, 4, 5, 6)");
     if (pl->n_mat > kTBMat)
       return fail(RFXH_EINVAL, "the temporal-blocked march takes at most 1024 materials");
     if ((int)pl->planes.size() > kTBPlanes)
@@ -3876,8 +3839,9 @@ int rfxh_set_variant(rfxh_plan* pl, int kernel, int param) {
   const bool pingpong = kernel == 1 || kernel == 2 || kernel == 3 || kernel == 7;
   if (pingpong && !pl->set[1][0])
     return fail(RFXH_EINVAL, "the fused and marching steps need a second field set");
-  if (pingpong && pl->n_pole > 0)
-    return fail(RFXH_EINVAL, "dispersive (Debye / Lorentz) materials run on the two-pass kernels (0, 4, 5, 6)");
+  if (pingpong && (pl->n_pole > 0 || pl->aniso))
+    return fail(RFXH_EINVAL, "dispersive (Debye / Lorentz) and subpixel-smoothed materials run on "
+                             "the two-pass kernels (0, 4, 5, 6)");
   if (kernel == 2 || kernel == 3 || kernel == 7) {
     if (param < 0) return fail(RFXH_EINVAL, "bad segment count");
     if (kernel == 7) {
@@ -3928,8 +3892,9 @@ int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
 int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
                     float* d_rec, void* stream) {
   if (!pl) return fail(RFXH_EINVAL, "null plan");
-  if (pl->n_pole > 0)
-    return fail(RFXH_EINVAL, "dispersive materials: the reverse mode covers non-dispersive plans only");
+  if (pl->n_pole > 0 || pl->aniso)
+    return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode covers "
+                             "plain isotropic plans only");
   if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
     return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
   int rc = check_region(pl, region);
@@ -3962,8 +3927,9 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
 
 int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
   if (!pl || !ad) return fail(RFXH_EINVAL, "null plan or adjoint descriptor");
-  if (pl->n_pole > 0)
-    return fail(RFXH_EINVAL, "dispersive materials: the reverse mode covers non-dispersive plans only");
+  if (pl->n_pole > 0 || pl->aniso)
+    return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode covers "
+                             "plain isotropic plans only");
   if (lo < 0 || hi < lo || hi > pl->max_steps) return fail(RFXH_EINVAL, "bad step range");
   for (int c = 0; c < 6; ++c)
     if (!ad->d_adj[c]) return fail(RFXH_EINVAL, "adjoint field pointer missing");
@@ -4070,7 +4036,7 @@ int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
   bool one_mu = true, disp = false;
   for (int q = 0; q < n_act; ++q) {
     one_mu = one_mu && b->plans[b->order[q]]->kp.ch_all > 0.f;
-    disp = disp || b->plans[b->order[q]]->n_pole > 0;
+    disp = disp || b->plans[b->order[q]]->n_pole > 0 || b->plans[b->order[q]]->aniso;
   }
   if (one_mu)
     hipLaunchKernelGGL((k_batch_pass<0, true, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
@@ -4423,6 +4389,207 @@ int rfxh_ntff_far_field_noalloc(const rfxh_farfield_desc* dsc, int32_t n_theta,
   // as rfxh_ntff_far_field does
   HIPCHK(hipStreamSynchronize(s));
   return RFXH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-side material assembly: box fill, CPML pad extension, tangential
+// PEC bits on the uint16 id array (one thread per cell or per column).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Box fill: the last non-PEC box covering the cell paints it (insertion
+// order, later wins); any PEC box covering it marks it.
+__global__ __launch_bounds__(256) void k_asm_fill(const rfxh_asm_box* __restrict__ boxes,
+                                                  int n_box, int nx, int ny, int nz,
+                                                  uint16_t* __restrict__ ids,
+                                                  uint8_t* __restrict__ pec) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long P = (long long)ny * nz;
+  if (c >= (long long)nx * P) return;
+  const int i = (int)(c / P);
+  const int r = (int)(c - (long long)i * P);
+  const int j = r / nz, k = r - (r / nz) * nz;
+  int id = 0;
+  bool painted = false, p = false;
+  for (int b = n_box - 1; b >= 0; --b) {
+    const rfxh_asm_box& bx = boxes[b];
+    const bool in = i >= bx.lo[0] && i < bx.hi[0] && j >= bx.lo[1] && j < bx.hi[1] &&
+                    k >= bx.lo[2] && k < bx.hi[2];
+    if (!in) continue;
+    if (bx.pec) {
+      p = true;
+    } else if (!painted) {
+      id = bx.id;
+      painted = true;
+    }
+  }
+  ids[c] = (uint16_t)id;
+  pec[c] = p ? 1 : 0;
+}
+
+// Pad extension along axis a: one thread per line along a (the other two
+// indices), lo pad first, then the hi pad with its vacuum rule.
+__global__ __launch_bounds__(256) void k_asm_pads(int a, int plo, int phi, int nx, int ny, int nz,
+                                                  const uint8_t* __restrict__ vacuum, int n_mat,
+                                                  uint16_t* __restrict__ ids) {
+  const int N[3] = {nx, ny, nz};
+  const long long stride[3] = {(long long)ny * nz, (long long)nz, 1};
+  const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)N[a1] * N[a2]) return;
+  const int u = (int)(t / N[a2]), v = (int)(t - (t / N[a2]) * N[a2]);
+  const long long base = u * stride[a1] + v * stride[a2];
+  const long long sa = stride[a];
+  const int n = N[a];
+  if (plo > 0) {
+    const uint16_t first = ids[base + plo * sa];
+    for (int q = 0; q < plo; ++q) ids[base + q * sa] = first;
+  }
+  if (phi > 0) {
+    const uint16_t outer = ids[base + (long long)(n - phi - 1) * sa];
+    uint16_t src = outer;
+    if (n - plo - phi >= 2) {
+      const uint16_t inner = ids[base + (long long)(n - phi - 2) * sa];
+      // (ids come from the fill, below n_mat)
+      if (outer < n_mat && inner < n_mat && vacuum[outer] && !vacuum[inner]) src = inner;
+      ids[base + (long long)(n - phi - 1) * sa] = src;
+    }
+    for (int q = n - phi; q < n; ++q) ids[base + q * sa] = src;
+  }
+}
+
+// Tangential-zero bits of a PEC cell: bit a when a neighbour along a is
+// PEC (none past the domain).
+__device__ __forceinline__ unsigned asm_pec_bits(const uint8_t* __restrict__ pec, long long c,
+                                                 int i, int j, int k, int nx, int ny, int nz) {
+  const long long P = (long long)ny * nz;
+  unsigned bits = 0;
+  if ((i > 0 && pec[c - P]) || (i < nx - 1 && pec[c + P])) bits |= 1u;
+  if ((j > 0 && pec[c - nz]) || (j < ny - 1 && pec[c + nz])) bits |= 2u;
+  if ((k > 0 && pec[c - 1]) || (k < nz - 1 && pec[c + 1])) bits |= 4u;
+  return bits;
+}
+
+template <bool APPLY>
+__global__ __launch_bounds__(256) void k_asm_pec(int nx, int ny, int nz, uint16_t* __restrict__ ids,
+                                                 const uint8_t* __restrict__ pec,
+                                                 uint32_t* __restrict__ used,
+                                                 const uint16_t* __restrict__ map, int n_mat) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long P = (long long)ny * nz;
+  if (c >= (long long)nx * P || !pec[c]) return;
+  const int i = (int)(c / P);
+  const int r = (int)(c - (long long)i * P);
+  const int j = r / nz, k = r - (r / nz) * nz;
+  const unsigned bits = asm_pec_bits(pec, c, i, j, k, nx, ny, nz);
+  if (!bits) return;
+  const unsigned id = ids[c];
+  if (id >= (unsigned)n_mat) return;   // not an id of the table given
+  if (APPLY) ids[c] = map[id * 8u + bits];
+  else atomicOr(&used[id], 1u << bits);
+}
+
+int asm_check(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1 || (long long)nx * ny * nz >= (1LL << 30))
+    return fail(RFXH_EINVAL, "assembly: grid must have 1 .. 2^30 cells");
+  return 0;
+}
+
+// host table -> device (synchronous: set-up path); *out freed by the caller
+template <typename T>
+int asm_upload(const T* host, size_t n, T** out) {
+  *out = nullptr;
+  if (n == 0) return 0;
+  if (hipMalloc((void**)out, n * sizeof(T)) != hipSuccess)
+    return fail(RFXH_ENOMEM, "assembly: device table");
+  if (hipMemcpy(*out, host, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(*out);
+    *out = nullptr;
+    return fail(RFXH_EHIP, "assembly: table upload");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rfxh_assemble_fill(int32_t nx, int32_t ny, int32_t nz, const rfxh_asm_box* boxes,
+                       int32_t n_box, const int32_t* pads, const uint8_t* vacuum, int32_t n_mat,
+                       uint16_t* d_ids, uint8_t* d_pec, void* stream) {
+  int rc = asm_check(nx, ny, nz);
+  if (rc) return rc;
+  if (!d_ids || !d_pec || n_box < 0 || (n_box > 0 && !boxes) || n_mat < 1 || n_mat > 65536 ||
+      !vacuum)
+    return fail(RFXH_EINVAL, "assembly: bad arguments");
+  for (int b = 0; b < n_box; ++b)
+    if (!boxes[b].pec && (boxes[b].id < 0 || boxes[b].id >= n_mat))
+      return fail(RFXH_EINVAL, "assembly: box material id outside the table");
+  const int N[3] = {nx, ny, nz};
+  if (pads)
+    for (int a = 0; a < 3; ++a)
+      if (pads[2 * a] < 0 || pads[2 * a + 1] < 0 || pads[2 * a] + pads[2 * a + 1] >= N[a])
+        return fail(RFXH_EINVAL, "assembly: pads leave no interior");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  rfxh_asm_box* dbox = nullptr;
+  uint8_t* dvac = nullptr;
+  if ((rc = asm_upload(boxes, (size_t)n_box, &dbox))) return rc;
+  if ((rc = asm_upload(vacuum, (size_t)n_mat, &dvac))) {
+    (void)hipFree(dbox);
+    return rc;
+  }
+  const long long cells = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_asm_fill, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, dbox,
+                     n_box, nx, ny, nz, d_ids, d_pec);
+  if (pads) {
+    for (int a = 0; a < 3; ++a) {
+      if (pads[2 * a] == 0 && pads[2 * a + 1] == 0) continue;
+      const long long lines = cells / N[a];
+      hipLaunchKernelGGL(k_asm_pads, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, a,
+                         pads[2 * a], pads[2 * a + 1], nx, ny, nz, dvac, n_mat, d_ids);
+    }
+  }
+  rc = hipGetLastError() == hipSuccess ? 0 : fail(RFXH_EHIP, "assembly: launch");
+  // the tables are freed once the kernels are done with them
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(RFXH_EHIP, "assembly: fill");
+  (void)hipFree(dbox);
+  (void)hipFree(dvac);
+  return rc;
+}
+
+int rfxh_assemble_pec_pairs(int32_t nx, int32_t ny, int32_t nz, const uint16_t* d_ids,
+                            const uint8_t* d_pec, uint32_t* d_used, int32_t n_mat, void* stream) {
+  int rc = asm_check(nx, ny, nz);
+  if (rc) return rc;
+  if (!d_ids || !d_pec || !d_used || n_mat < 1 || n_mat > 65536)
+    return fail(RFXH_EINVAL, "assembly: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const long long cells = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_asm_pec<false>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, nx,
+                     ny, nz, const_cast<uint16_t*>(d_ids), d_pec, d_used, nullptr, n_mat);
+  HIPCHK(hipGetLastError());
+  return RFXH_OK;
+}
+
+int rfxh_assemble_pec_apply(int32_t nx, int32_t ny, int32_t nz, uint16_t* d_ids,
+                            const uint8_t* d_pec, const uint16_t* map, int32_t n_mat,
+                            void* stream) {
+  int rc = asm_check(nx, ny, nz);
+  if (rc) return rc;
+  if (!d_ids || !d_pec || !map || n_mat < 1 || n_mat > 65536)
+    return fail(RFXH_EINVAL, "assembly: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint16_t* dmap = nullptr;
+  if ((rc = asm_upload(map, (size_t)n_mat * 8, &dmap))) return rc;
+  const long long cells = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_asm_pec<true>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, nx,
+                     ny, nz, d_ids, d_pec, nullptr, dmap, n_mat);
+  rc = hipGetLastError() == hipSuccess ? 0 : fail(RFXH_EHIP, "assembly: launch");
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = fail(RFXH_EHIP, "assembly: pec apply");
+  (void)hipFree(dmap);
+  return rc;
 }
 
 const char* rfxh_last_error(void) { return g_err.c_str(); }
