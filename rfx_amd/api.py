@@ -438,7 +438,7 @@ class Simulation:
 
     def _build_problem(self, n_steps: int, lane: str, eps_override=None,
                        sigma_override=None, mu_r_override=None, s11_freqs=None,
-                       material_sweep=None):
+                       material_sweep=None, subpixel_smoothing=False):
         """Host set-up of one run.  ``material_sweep = (name, field, value)``
         replaces one material property as rfx/vmap_sweep.py:265-390 does:
         with a material name, the cells inside that material's shapes take
@@ -558,6 +558,8 @@ class Simulation:
             mat = dasm.finish()
         elif pec is not None:
             mat.add_flags(tangential_edge_bits(pec))
+        if subpixel_smoothing:
+            self._apply_subpixel(grid, mat, subpixel_smoothing, bool(poles or lorentz))
         probes = [(grid.position_to_index(p.position), p.component) for p in self._probes]
         if not probes and lane == "forward" and self._ports:
             probes = [(grid.position_to_index(p.position), p.component) for p in self._ports]
@@ -594,6 +596,27 @@ class Simulation:
         prob.flux_areas = areas
         prob.ntff_box = box
         return grid, prob, ports
+
+    def _apply_subpixel(self, grid, mat, how, dispersive: bool):
+        """Per-component eps at the interface cells (rfx/runners/uniform.py:
+        234-278, graded rfx/runners/nonuniform.py:648-672): every geometry
+        entry with its material's eps_r, background 1.  Dispersive problems
+        keep the plain update (the reference's dispersive E update does not
+        read the smoothed eps)."""
+        if how == "kottke_pec":
+            raise NotImplementedError("subpixel_smoothing='kottke_pec' (the inverse-permittivity "
+                                      "PEC tensor) is not supported by the HIP backend")
+        if how is not True:
+            raise ValueError(f"subpixel_smoothing must be True, False or 'kottke_pec', got {how!r}")
+        if dispersive:
+            warnings.warn("subpixel_smoothing=True has no effect with dispersive (Debye / "
+                          "Lorentz) materials", stacklevel=3)
+            return
+        from .smoothing import smoothed_eps
+        boxes = [(box.corner_lo, box.corner_hi, self._materials[name].eps_r)
+                 for box, name in self._geometry]
+        if boxes:
+            mat.apply_aniso(*smoothed_eps(grid, boxes))
 
     # material assembly on the GPU: "auto" (grids of DEVICE_ASSEMBLY_CELLS cells
     # or more, when a GPU is visible), "device" or "host"
@@ -682,13 +705,15 @@ class Simulation:
     # ---- run / forward -----------------------------------------------------------
     def run(self, n_steps: int | None = None, *, num_periods: float = 20.0,
             use_graph: bool = True, s_param_freqs=None, autotune: bool = False,
-            **unsupported):
+            subpixel_smoothing: bool | str = False, **unsupported):
         """Run the time loop (rfx/api/_execute.py:2862 -> runners/uniform.py:140,
         graded z -> runners/nonuniform.py:465).  On the graded lane
         ``s_param_freqs`` gives the wire-port S matrix S[j, k] = b_j / a_k
         (rfx/nonuniform.py:1900-1935).  ``autotune=True`` first picks the
         fastest kernel variant for this problem (a few timed steps, then a
-        restart from zero)."""
+        restart from zero).  ``subpixel_smoothing=True`` gives the cells at
+        material interfaces per-component Kottke-averaged eps
+        (rfx/geometry/smoothing.py; those plans run the two-pass kernels)."""
         _reject(unsupported, "run")
         graded = self._graded
         grid = self._build_grid()
@@ -700,7 +725,8 @@ class Simulation:
         sf = None
         if graded and s_param_freqs is not None:
             sf = np.asarray(s_param_freqs, F32)
-        grid, prob, ports = self._build_problem(int(n_steps), "run", s11_freqs=sf)
+        grid, prob, ports = self._build_problem(int(n_steps), "run", s11_freqs=sf,
+                                                subpixel_smoothing=subpixel_smoothing)
         dr = DeviceRun(prob, int(n_steps), self._device)
         if autotune:
             # time the kernel variants on this problem's first steps, keep

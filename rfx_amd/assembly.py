@@ -105,9 +105,8 @@ class DeviceAssembly:
         return self.mat.table[int(self.ids[tuple(idx)].item()) & 0xFFFF]
 
     def set_cell(self, idx, eps=None, sig=None, mu=None, add_flags=0):
-        e, s, m, f = self.cell_props(idx)
-        mid = self.mat.get_id(e if eps is None else eps, s if sig is None else sig,
-                              m if mu is None else mu, f | add_flags)
+        cur = int(self.ids[tuple(idx)].item()) & 0xFFFF
+        mid = self.mat._derive(cur, eps, sig, mu, add_flags)
         self.ids[tuple(idx)] = int(np.uint16(mid).view(np.int16))
 
     # -- PEC bits and hand-over ----------------------------------------------------

@@ -101,6 +101,8 @@ class DeviceRun:
         else:
             self.coef, self.flags = mat.coefficient_table(prob.dt)
             self.pole_rows, self.pol_box = None, None
+        # subpixel smoothing: Ey, Ez coefficients of the per-component ids
+        self.acoef = mat.aniso_table(prob.dt) if not (prob.poles or prob.lorentz) else None
         self._ops = (_lib.PointOp * max(len(ops), 1))()
         for q, (cell, kind, comp, slot, dpar, loop, _) in enumerate(ops):
             o = self._ops[q]
@@ -247,6 +249,8 @@ class DeviceRun:
                 d.pol_lo[a] = self.pol_box[0][a]
                 d.pol_n[a] = self.pol_box[1][a]
             d.d_pol = self.pol.data_ptr()
+        if self.acoef is not None:
+            d.aniso_coef = _f32p(self.acoef)
         self.desc = d
         self.plan = C.c_void_p()
         with torch.cuda.device(dev):

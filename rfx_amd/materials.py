@@ -21,6 +21,7 @@ from .constants import EPS_0, MU_0
 
 F32 = np.float32
 MAX_IDS = 65536
+ANISO_FLAG = 16      # the id carries per-component eps (subpixel smoothing)
 POLE_SHIFT = 16      # flag bit 16 + q: Debye pole q acts at the cell
 MAX_POLES = 16
 
@@ -56,12 +57,21 @@ class MaterialMap:
         self.shape = tuple(shape)
         self.ids = np.zeros(self.shape, np.uint16)
         self.table = []           # (eps_r, sigma, mu_r, flags)
+        self.aniso = {}           # id -> (eps_x, eps_y, eps_z) of ANISO_FLAG ids
         self._lookup = {}
         self.get_id(1.0, 0.0, 1.0, 0)
 
     # -- table -------------------------------------------------------------
-    def get_id(self, eps, sig, mu, flags=0) -> int:
-        key = (F32(eps).tobytes(), F32(sig).tobytes(), F32(mu).tobytes(), int(flags))
+    def get_id(self, eps, sig, mu, flags=0, aniso=None) -> int:
+        """Id of (eps_r, sigma, mu_r, flags[, per-component eps]); a new
+        entry when the combination is new."""
+        flags = int(flags)
+        tail = b""
+        if aniso is not None:
+            flags |= ANISO_FLAG
+            aniso = tuple(F32(v) for v in aniso)
+            tail = b"".join(v.tobytes() for v in aniso)
+        key = (F32(eps).tobytes(), F32(sig).tobytes(), F32(mu).tobytes(), flags, tail)
         mid = self._lookup.get(key)
         if mid is None:
             mid = len(self.table)
@@ -69,9 +79,19 @@ class MaterialMap:
                 raise ValueError(
                     f"more than {MAX_IDS} distinct (eps_r, sigma, mu_r) combinations; "
                     "the material-id layout cannot represent this problem")
-            self.table.append((F32(eps), F32(sig), F32(mu), int(flags)))
+            self.table.append((F32(eps), F32(sig), F32(mu), flags))
+            if aniso is not None:
+                self.aniso[mid] = aniso
             self._lookup[key] = mid
         return mid
+
+    def _derive(self, mid, eps=None, sig=None, mu=None, add_flags=0) -> int:
+        """Id of material ``mid`` with some properties replaced and flags
+        added (its per-component eps kept)."""
+        e, s, m, f = self.table[int(mid)]
+        return self.get_id(e if eps is None else eps, s if sig is None else sig,
+                           m if mu is None else mu, f | int(add_flags),
+                           self.aniso.get(int(mid)))
 
     def props(self):
         t = self.table
@@ -149,7 +169,8 @@ class MaterialMap:
         self.table = table
         self._lookup = {}
         for mid, (e, sg, mu, fl) in enumerate(table):
-            key = (F32(e).tobytes(), F32(sg).tobytes(), F32(mu).tobytes(), int(fl))
+            tail = b"".join(v.tobytes() for v in self.aniso[mid]) if mid in self.aniso else b""
+            key = (F32(e).tobytes(), F32(sg).tobytes(), F32(mu).tobytes(), int(fl), tail)
             self._lookup.setdefault(key, mid)
 
     # -- per-cell edits ------------------------------------------------------
@@ -158,9 +179,7 @@ class MaterialMap:
         return e, s, m, f
 
     def set_cell(self, idx, eps=None, sig=None, mu=None, add_flags=0):
-        e, s, m, f = self.cell_props(idx)
-        self.ids[idx] = self.get_id(e if eps is None else eps, s if sig is None else sig,
-                                    m if mu is None else mu, f | add_flags)
+        self.ids[idx] = self._derive(self.ids[idx], eps, sig, mu, add_flags)
 
     def add_flags(self, cell_mask_bits: np.ndarray):
         """OR per-cell flag bits (integer array of the grid shape, up to 32
@@ -174,9 +193,30 @@ class MaterialMap:
         uniq, inv = np.unique(key, return_inverse=True)
         new = np.empty(len(uniq), np.uint16)
         for q, kv in enumerate(uniq):
-            e, s, m, f = self.table[int(kv >> 32)]
-            new[q] = self.get_id(e, s, m, f | int(kv & 0xFFFFFFFF))
+            new[q] = self._derive(int(kv >> 32), add_flags=int(kv & 0xFFFFFFFF))
         self.ids[where] = new[inv]
+
+    def apply_aniso(self, eps_x, eps_y, eps_z) -> int:
+        """Give every cell whose per-component eps differs from its eps_r an
+        id carrying the three values (subpixel smoothing); sigma, mu_r and
+        flags stay the cell's.  Returns the number of such cells."""
+        eps, _, _, _ = self.props()
+        e_cell = eps[self.ids]
+        where = np.nonzero((eps_x != e_cell) | (eps_y != e_cell) | (eps_z != e_cell))
+        if len(where[0]) == 0:
+            return 0
+        rows = np.stack([self.ids[where].astype(np.uint32),
+                         np.asarray(eps_x, F32)[where].view(np.uint32),
+                         np.asarray(eps_y, F32)[where].view(np.uint32),
+                         np.asarray(eps_z, F32)[where].view(np.uint32)], axis=1)
+        uniq, inv = np.unique(rows, axis=0, return_inverse=True)
+        new = np.empty(len(uniq), np.uint16)
+        for q, (mid, ex, ey, ez) in enumerate(uniq):
+            e, s, m, f = self.table[int(mid)]
+            comp = tuple(np.uint32(v).view(F32) for v in (ex, ey, ez))
+            new[q] = self.get_id(e, s, m, f, comp)
+        self.ids[where] = new[np.asarray(inv).ravel()]
+        return len(where[0])
 
     def override(self, eps_r=None, sigma=None, mu_r=None):
         """Replace per-cell material values wholesale (forward(eps_override=...),
@@ -200,6 +240,8 @@ class MaterialMap:
         if len(ukey) > MAX_IDS:
             raise ValueError(
                 f"override produces {len(ukey)} distinct materials (max {MAX_IDS})")
+        if self.aniso:
+            raise NotImplementedError("overrides of subpixel-smoothed materials")
         self.table, self._lookup = [], {}
         self.get_id(1.0, 0.0, 1.0, 0)
         new = np.empty(len(ukey), np.uint16)
@@ -219,17 +261,41 @@ class MaterialMap:
         return eps[self.ids], sig[self.ids], mu[self.ids]
 
     # -- kernel table ----------------------------------------------------------
-    def coefficient_table(self, dt: float):
-        eps_r, sig, mu_r, flg = self.props()
-        dt32 = F32(dt)
+    @staticmethod
+    def _ca_cb(eps_r, sig, dt32):
+        """ca, cb of the E update in the reference's float32 order
+        (rfx/core/yee.py:386-393, per component :880-895)."""
         eps = eps_r * F32(EPS_0)
         loss = sig * dt32 / (F32(2.0) * eps)
-        ca = (F32(1.0) - loss) / (F32(1.0) + loss)
-        cb = (dt32 / eps) / (F32(1.0) + loss)
+        return (F32(1.0) - loss) / (F32(1.0) + loss), (dt32 / eps) / (F32(1.0) + loss)
+
+    def coefficient_table(self, dt: float):
+        """Per id (ca, cb, ce, ch) and flags; ids with per-component eps
+        carry the Ex coefficients here (the CPML ce keeps eps_r)."""
+        eps_r, sig, mu_r, flg = self.props()
+        dt32 = F32(dt)
+        eps_x = eps_r.copy()
+        for mid, comp in self.aniso.items():
+            eps_x[mid] = comp[0]
+        ca, cb = self._ca_cb(eps_x, sig, dt32)
         ce = dt32 / (eps_r * F32(EPS_0))
         ch = dt32 / (mu_r * F32(MU_0))
         coef = np.stack([ca, cb, ce, ch], axis=1).astype(F32)
         return np.ascontiguousarray(coef), np.ascontiguousarray(flg)
+
+    def aniso_table(self, dt: float):
+        """Per id (ca_y, cb_y, ca_z, cb_z) of the per-component ids, zero
+        rows elsewhere; None when no id has per-component eps."""
+        if not self.aniso:
+            return None
+        _, sig, _, _ = self.props()
+        dt32 = F32(dt)
+        rows = np.zeros((len(self.table), 4), F32)
+        for mid, (_, ey, ez) in self.aniso.items():
+            s = sig[mid:mid + 1]
+            rows[mid, 0:2] = [v[0] for v in self._ca_cb(np.array([ey], F32), s, dt32)]
+            rows[mid, 2:4] = [v[0] for v in self._ca_cb(np.array([ez], F32), s, dt32)]
+        return np.ascontiguousarray(rows)
 
 
 def dispersion_tables(mat: MaterialMap, dt: float, debye_poles, lorentz_poles=()):
