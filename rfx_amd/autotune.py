@@ -61,7 +61,7 @@ def variants(dr):
         for kern in (2, 3) + ((7,) if small else ()):
             out += [(kern, 0)] + [(kern, s) for s in (4, 8, 16, 32, 64) if s <= max(1, nx // 4)]
         if small:
-            out += [(8, 0)] + [(8, s) for s in (4, 8, 12, 14, 16, 18, 20, 24, 32)
+            out += [(8, 0)] + [(8, s) for s in (4, 8, 10, 11, 12, 13, 14, 15, 16, 17, 18, 20, 24, 32)
                                if s <= max(1, nx // 16)]
     return out
 

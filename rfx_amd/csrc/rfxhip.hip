@@ -1075,6 +1075,11 @@ constexpr int kTBMat = 1024;                // material table entries staged in 
 constexpr int kTBPlanes = 4;                // DFT planes the sweep stages itself
 constexpr int kTBProf = 256;                // CPML profile rows staged in LDS (all faces)
 constexpr int kTBOps = 256;                 // point operations one tile column may hold
+// z-edge tiles: 64 rows x 16 columns (four rows per wavefront), their inner
+// 12 columns over a z face's CPML slab, so that the 16 x 64 tiles between
+// them need no z-face CPML code
+constexpr int kTBEdgeC = 16;
+constexpr int kTBEdgeR = kTBW * kBX / kTBEdgeC;
 
 // A point operation of the workgroup's region, staged in LDS at the start
 // of the sweep with the source values of both steps, so that the sweep
@@ -1122,6 +1127,8 @@ struct TBIO {
   float* en[3];
   float* hn[3];
   int ntk, ntj, n_mat;
+  int kz0, kz1;          // the 16 x 64 tiles own columns [kz0, kz1)
+  int ntj_e;             // rows of z-edge tiles per z face (0: none)
   int xa, xb, n_mid, seg_len;
   int n_prof;            // CPML profile rows (all faces)
   int n_plane;
@@ -1136,7 +1143,7 @@ struct TBShared {
   // plane prefetch issued before it (vector loads complete in order)
   float4 prof[kTBProf];
   // [buffer][stage: E^n, H^{n+1/2}, E^{n+1}, H^{n+3/2}][component][row][lane]
-  float x[2][4][3][kTBW][kBX];
+  float x[2][4][3][kTBW * kBX];   // [row * tile columns + column]
   TBOp ops[kTBOps];   // the region's point operations, sorted by (cell, order)
   int n_ops;
   int wcnt[kTBW];
@@ -1399,16 +1406,22 @@ __device__ __forceinline__ float3 tb_e(const KParams& p, const TBCpml& cm, const
 // The sweep of one workgroup over planes [i0, i1) of its tile; CPML =
 // false leaves out the memory terms and slab tests (the caller has checked
 // that the workgroup's needed cells miss every slab).
-template <bool GRADED, bool CPML>
-__device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBShared& sh, int w,
-                                          int l, int j, int k, int i0, int i1) {
+template <bool GRADED, bool CPML, bool EDGE>
+__device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBShared& sh,
+                                          int row, int col, int j, int k, int i0, int i1,
+                                          int kown1) {
+  // tile: TR rows x TC columns (16 x 64, one row per wavefront; z-edge
+  // tiles 64 x 16, four rows of 16 columns per wavefront); (row, col) is this
+  // thread's place in it, the outer kTBHalo rows and columns are halo
+  constexpr int TR = EDGE ? kTBW * kBX / 16 : kTBW;
+  constexpr int TC = EDGE ? 16 : kBX;
   const TBCpml& cm = io.cm;
   const float4* const s_ecoef = sh.ecoef;
   const float* const s_ch = sh.ch;
   auto& s_x = sh.x;
   const bool valid = j >= 0 && j < p.ny && k >= 0 && k < p.nz;
-  const bool interior = valid && w >= kTBHalo && w < kTBW - kTBHalo && l >= kTBHalo &&
-                        l < kBX - kTBHalo;
+  const bool interior = valid && row >= kTBHalo && row < TR - kTBHalo && col >= kTBHalo &&
+                        col < TC - kTBHalo && k < kown1;
   const unsigned P = (unsigned)p.plane;
   const unsigned c0 = valid ? (unsigned)j * (unsigned)p.nz + (unsigned)k : 0u;
   const long long n = *p.step;
@@ -1424,11 +1437,12 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   auto pin = [&](int q) { return valid && q >= 0 && q < p.nx; };
   auto own = [&](int q) { return interior && q >= i0 && q < i1; };
   // neighbour value of stage st from LDS buffer b; zero outside the tile
-  auto nb = [&](int b, int st, int cmp, int ww, int ll) {
-    const bool ok = ww >= 0 && ww < kTBW && ll >= 0 && ll < kBX;
-    const float v = s_x[b][st][cmp][ok ? ww : 0][ok ? ll : 0];
+  auto nb = [&](int b, int st, int cmp, int rr, int cc) {
+    const bool ok = rr >= 0 && rr < TR && cc >= 0 && cc < TC;
+    const float v = s_x[b][st][cmp][(ok ? rr : 0) * TC + (ok ? cc : 0)];
     return ok ? v : 0.f;
   };
+  const int me = row * TC + col;   // this thread's LDS slot
   // memory terms (term0: 0 for H, 2 for E) of the cell at plane q from
   // `base` (zero outside slabs), and their store
   auto ld_psi = [&](float (&dst)[3][2], const float* base, unsigned term0, int q, bool use) {
@@ -1521,9 +1535,9 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
   // output needs): publish E^n of plane i0-2 for the first H update's y/z
   // neighbours and shift the planes by one.  The other stages of buffer 0
   // hold the zeros written at the kernel's start.
-  s_x[0][0][0][w][l] = e0n[0];
-  s_x[0][0][1][w][l] = e0n[1];
-  s_x[0][0][2][w][l] = e0n[2];
+  s_x[0][0][0][me] = e0n[0];
+  s_x[0][0][1][me] = e0n[1];
+  s_x[0][0][2][me] = e0n[2];
   prefetch(i0 - 2, fe[1], fh[1], fm[1]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -1563,8 +1577,8 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       // every lane computes (no branch); lanes outside the grid or the
       // planes keep zero, the boundary value their neighbours need
       const int q = pp - 4;
-      const float hx_j = nb(rb, 3, 0, w - 1, l), hz_j = nb(rb, 3, 2, w - 1, l);
-      const float hx_k = nb(rb, 3, 0, w, l - 1), hy_k = nb(rb, 3, 1, w, l - 1);
+      const float hx_j = nb(rb, 3, 0, row - 1, col), hz_j = nb(rb, 3, 2, row - 1, col);
+      const float hx_k = nb(rb, 3, 0, row, col - 1), hy_k = nb(rb, 3, 1, row, col - 1);
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       float dummy[3][2];
@@ -1587,8 +1601,8 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     for (int a = 0; a < 3; ++a) pe_new[a][0] = pe_new[a][1] = 0.f;
     {
       const int q = pp - 2;
-      const float hx_j = nb(rb, 1, 0, w - 1, l), hz_j = nb(rb, 1, 2, w - 1, l);
-      const float hx_k = nb(rb, 1, 0, w, l - 1), hy_k = nb(rb, 1, 1, w, l - 1);
+      const float hx_j = nb(rb, 1, 0, row - 1, col), hz_j = nb(rb, 1, 2, row - 1, col);
+      const float hx_k = nb(rb, 1, 0, row, col - 1), hy_k = nb(rb, 1, 1, row, col - 1);
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       const float4 row = s_ecoef[m2];
@@ -1603,8 +1617,8 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     float3 h2 = make_float3(0.f, 0.f, 0.f);
     {
       const int q = pp - 3;
-      const float ex_j = nb(rb, 2, 0, w + 1, l), ez_j = nb(rb, 2, 2, w + 1, l);
-      const float ex_k = nb(rb, 2, 0, w, l + 1), ey_k = nb(rb, 2, 1, w, l + 1);
+      const float ex_j = nb(rb, 2, 0, row + 1, col), ez_j = nb(rb, 2, 2, row + 1, col);
+      const float ex_k = nb(rb, 2, 0, row, col + 1), ey_k = nb(rb, 2, 1, row, col + 1);
       const bool o = own(q);
       const unsigned c = (unsigned)q * P + c0;
       float dummy[3][2];
@@ -1626,8 +1640,8 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
     for (int a = 0; a < 3; ++a) ph_new[a][0] = ph_new[a][1] = 0.f;
     {
       const int q = pp - 1;
-      const float ex_j = nb(rb, 0, 0, w + 1, l), ez_j = nb(rb, 0, 2, w + 1, l);
-      const float ex_k = nb(rb, 0, 0, w, l + 1), ey_k = nb(rb, 0, 1, w, l + 1);
+      const float ex_j = nb(rb, 0, 0, row + 1, col), ez_j = nb(rb, 0, 2, row + 1, col);
+      const float ex_k = nb(rb, 0, 0, row, col + 1), ey_k = nb(rb, 0, 1, row, col + 1);
       h1 = tb_h<GRADED, CPML>(p, cm, sh.prof, q, j, k, ghy, ghz, chc > 0.f ? chc : s_ch[m1], e0c[0], e0c[1], e0c[2],
                               ex_j, ez_j, ex_k, ey_k, e0n[1], e0n[2], h0n[0], h0n[1], h0n[2], pA,
                               ph_new, false);
@@ -1635,22 +1649,22 @@ __device__ __forceinline__ void tb2_sweep(const KParams& p, const TBIO& io, TBSh
       // every copy whose H is right stores it (halo rows take the second H
       // update too): not the tile's last row and lane (their outer
       // neighbours read as zero), not the plane before the pipeline fills
-      st_psi(cm.mid, 0u, q, ph_new, q >= i0 - 2 && w < kTBW - 1 && l < kBX - 1);
+      st_psi(cm.mid, 0u, q, ph_new, q >= i0 - 2 && row < TR - 1 && col < TC - 1);
       if (own(q)) stage_snap(q, 3, h1, n);
     }
     // ---- publish this iteration's planes for the neighbours
-    s_x[wb][0][0][w][l] = e0n[0];
-    s_x[wb][0][1][w][l] = e0n[1];
-    s_x[wb][0][2][w][l] = e0n[2];
-    s_x[wb][1][0][w][l] = h1.x;
-    s_x[wb][1][1][w][l] = h1.y;
-    s_x[wb][1][2][w][l] = h1.z;
-    s_x[wb][2][0][w][l] = e1.x;
-    s_x[wb][2][1][w][l] = e1.y;
-    s_x[wb][2][2][w][l] = e1.z;
-    s_x[wb][3][0][w][l] = h2.x;
-    s_x[wb][3][1][w][l] = h2.y;
-    s_x[wb][3][2][w][l] = h2.z;
+    s_x[wb][0][0][me] = e0n[0];
+    s_x[wb][0][1][me] = e0n[1];
+    s_x[wb][0][2][me] = e0n[2];
+    s_x[wb][1][0][me] = h1.x;
+    s_x[wb][1][1][me] = h1.y;
+    s_x[wb][1][2][me] = h1.z;
+    s_x[wb][2][0][me] = e1.x;
+    s_x[wb][2][1][me] = e1.y;
+    s_x[wb][2][2][me] = e1.z;
+    s_x[wb][3][0][me] = h2.x;
+    s_x[wb][3][1][me] = h2.y;
+    s_x[wb][3][2][me] = h2.z;
     __syncthreads();
     // ---- rotate
     h2by = h2a[1];
@@ -1710,16 +1724,18 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
   for (int t = tid; t < io.n_prof; t += kBX * kTBW) sh.prof[t] = gld(io.cm.prof, t);
   // the first iteration reads buffer 0 (no plane before the segment start
   // but the primed E^n)
-  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&sh.x[0][0][0][0][0])[t] = 0.f;
+  for (int t = tid; t < 4 * 3 * kTBW * kBX; t += kBX * kTBW) (&sh.x[0][0][0][0])[t] = 0.f;
   __syncthreads();
   const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  const int tk = L % io.ntk;
-  const int r = L / io.ntk;
-  const int tj = r % io.ntj;
+  // the tiles of one x segment: first the z-edge tiles (slower: CPML on
+  // most of their cells), then the 16 x 64 tiles
+  const int n_edge = 2 * io.ntj_e;
+  const int tiles = n_edge + io.ntj * io.ntk;
+  const int t = L % tiles;
   // dispatch order: the two x-slab segments (the slowest workgroups, CPML
   // on every cell) first, so that they do not form the launch's tail;
   // then the middle ones.  seg: 0 x-lo slab, 1..n_mid middle, n_mid + 1 x-hi
-  const int so = r / io.ntj;
+  const int so = L / tiles;
   const int seg = so == 0 ? 0 : so == 1 ? io.n_mid + 1 : so - 1;
   int i0, i1;
   if (seg == 0) {
@@ -1733,22 +1749,48 @@ __global__ __launch_bounds__(kBX * kTBW) void k_step_tb2(KParams p, TBIO io) {
     i1 = min(io.xb, i0 + io.seg_len);
   }
   if (i0 >= i1) return;   // whole workgroup
-  const int j0 = tj * kTBJ, k0 = tk * kTBK;
-  const int j = j0 - kTBHalo + w;
-  const int k = k0 - kTBHalo + l;
+  const bool edge = t < n_edge;
+  int j0, k0, row, col, kown1, tr, tc;
+  if (edge) {
+    // 64 x 16 tile over a z face's slab: its own columns are [0, kz0) or
+    // [kz1, nz)
+    const int side = t / io.ntj_e;
+    j0 = (t - side * io.ntj_e) * (kTBEdgeR - 2 * kTBHalo);
+    k0 = side == 0 ? 0 : io.kz1;
+    row = w * (kBX / kTBEdgeC) + l / kTBEdgeC;
+    col = l % kTBEdgeC;
+    kown1 = side == 0 ? io.kz0 : p.nz;
+    tr = kTBEdgeR;
+    tc = kTBEdgeC;
+  } else {
+    const int u = t - n_edge;
+    const int tk = u % io.ntk;
+    j0 = (u / io.ntk) * kTBJ;
+    k0 = io.kz0 + tk * kTBK;
+    row = w;
+    col = l;
+    kown1 = io.kz1;
+    tr = kTBW;
+    tc = kBX;
+  }
+  const int j = j0 - kTBHalo + row;
+  const int k = k0 - kTBHalo + col;
   // point operations of every cell the sweep's stages visit
   // (without any, no cell carries the point flag and the list is unused)
   if (p.n_ops > 0)
-    tb_stage_ops(p, sh, w, l, *p.step, i0 - 6, i1 + 6, j0 - kTBHalo, j0 - kTBHalo + kTBW,
-                 k0 - kTBHalo, k0 - kTBHalo + kBX);
+    tb_stage_ops(p, sh, w, l, *p.step, i0 - 6, i1 + 6, j0 - kTBHalo, j0 - kTBHalo + tr,
+                 k0 - kTBHalo, k0 - kTBHalo + tc);
   // cells the interior's two steps need: planes [i0-2, i1], rows
-  // [j0-2, j0+kTBJ], columns [k0-2, k0+kTBK]
+  // [j0-2, j0 + inner rows], columns [k0-2, min(k0 + inner columns, own end)]
+  const int kneed = min(k0 + tc - 2 * kTBHalo, kown1);
   const bool slab =
       ANY_CPML && ((p.cn[0] > 0 && i0 - 2 < p.slo[0]) || (p.cn[1] > 0 && i1 >= p.shi[0]) ||
-                   (p.cn[2] > 0 && j0 - 2 < p.slo[1]) || (p.cn[3] > 0 && j0 + kTBJ >= p.shi[1]) ||
-                   (p.cn[4] > 0 && k0 - 2 < p.slo[2]) || (p.cn[5] > 0 && k0 + kTBK >= p.shi[2]));
-  if (slab) tb2_sweep<GRADED, ANY_CPML>(p, io, sh, w, l, j, k, i0, i1);
-  else tb2_sweep<GRADED, false>(p, io, sh, w, l, j, k, i0, i1);
+                   (p.cn[2] > 0 && j0 - 2 < p.slo[1]) ||
+                   (p.cn[3] > 0 && j0 + tr - 2 * kTBHalo >= p.shi[1]) ||
+                   (p.cn[4] > 0 && k0 - 2 < p.slo[2]) || (p.cn[5] > 0 && kneed >= p.shi[2]));
+  if (edge) tb2_sweep<GRADED, ANY_CPML, true>(p, io, sh, row, col, j, k, i0, i1, kown1);
+  else if (slab) tb2_sweep<GRADED, ANY_CPML, false>(p, io, sh, row, col, j, k, i0, i1, kown1);
+  else tb2_sweep<GRADED, false, false>(p, io, sh, row, col, j, k, i0, i1, kown1);
 }
 
 // ---------------------------------------------------------------------------
@@ -3035,9 +3077,17 @@ int launch_step(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
 // segments (7290 workgroups) beat 4 by 22 % and 8 by 7 %: the workgroups
 // over CPML slabs take longer, and many small workgroups balance them).
 void tb_blocks(const KParams& kp, int want_seg, TBIO* io, int* nseg_out) {
+  // z-edge tiles when both z faces have CPML slabs no deeper than an edge
+  // tile's inner columns less the halo
+  const int ei = kTBEdgeC - 2 * kTBHalo;
+  const bool edges = kp.cn[4] > 0 && kp.cn[5] > 0 && std::max(kp.cn[4], kp.cn[5]) + kTBHalo <= ei &&
+                     kp.nz > 2 * ei;
+  io->kz0 = edges ? ei : 0;
+  io->kz1 = edges ? kp.nz - ei : kp.nz;
   io->ntj = (kp.ny + kTBJ - 1) / kTBJ;
-  io->ntk = (kp.nz + kTBK - 1) / kTBK;
-  const long long tiles = (long long)io->ntj * io->ntk;
+  io->ntk = (io->kz1 - io->kz0 + kTBK - 1) / kTBK;
+  io->ntj_e = edges ? (kp.ny + kTBEdgeR - 2 * kTBHalo - 1) / (kTBEdgeR - 2 * kTBHalo) : 0;
+  const long long tiles = (long long)io->ntj * io->ntk + 2LL * io->ntj_e;
   io->xa = kp.cn[0] > 0 ? std::min(kp.nx, kp.cn[0] + 2) : 0;
   io->xb = kp.cn[1] > 0 ? std::max(io->xa, kp.nx - kp.cn[1] - 1) : kp.nx;
   const int mid = io->xb - io->xa;
@@ -3092,7 +3142,7 @@ int launch_tb(rfxh_plan* pl, hipStream_t s, hipEvent_t* ev, int from) {
     io.cm.fs[f] = pl->psi_fs[f];
     io.cm.poff[f] = pl->cprof_off[f];
   }
-  const dim3 g((unsigned)((long long)io.ntj * io.ntk * nseg)), blk(kBX, kTBW);
+  const dim3 g((unsigned)(((long long)io.ntj * io.ntk + 2LL * io.ntj_e) * nseg)), blk(kBX, kTBW);
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
   const bool cpml = pl->psi_base && pl->cprof;   // the plan has CPML faces
   if (kp.graded) {
@@ -3218,20 +3268,39 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
 
   rfxh_plan* pl = new rfxh_plan();
   {
-    // point operations per tile column of the temporal-blocked march (tile
-    // rows [tj kTBJ - halo, + kTBW), columns [tk kTBK - halo, + kBX), all x)
-    const int ntj = (d->ny + kTBJ - 1) / kTBJ, ntk = (d->nz + kTBK - 1) / kTBK;
-    std::vector<int> cnt((size_t)ntj * ntk, 0);
+    // point operations per tile column of the temporal-blocked march, over
+    // all x: 16 x 64 tiles (rows [j0 - halo, + kTBW), columns [k0 - halo,
+    // + kBX)) and, with z-edge tiles, 64 x 16 ones
+    KParams tk_p;
+    std::memset(&tk_p, 0, sizeof(tk_p));
+    tk_p.nx = d->nx;
+    tk_p.ny = d->ny;
+    tk_p.nz = d->nz;
+    for (int f = 0; f < 6; ++f) tk_p.cn[f] = d->cpml_n[f];
+    TBIO tio;
+    std::memset(&tio, 0, sizeof(tio));
+    int nseg = 1;
+    tb_blocks(tk_p, 0, &tio, &nseg);
+    std::map<long long, int> cnt;
     const long long P = (long long)d->ny * d->nz;
+    auto count = [&](long long key) { pl->tb_ops_max = std::max(pl->tb_ops_max, ++cnt[key]); };
     for (const rfxh_point_op& op : ops) {
       const int r = (int)(op.cell % P), j = r / d->nz, k = r % d->nz;
-      for (int tj = std::max(0, (j - kTBHalo) / kTBJ - 1); tj < ntj; ++tj) {
+      for (int tj = std::max(0, (j - kTBHalo) / kTBJ - 1); tj < tio.ntj; ++tj) {
         if (tj * kTBJ - kTBHalo > j) break;
         if (j >= tj * kTBJ - kTBHalo + kTBW) continue;
-        for (int tk = std::max(0, (k - kTBHalo) / kTBK - 1); tk < ntk; ++tk) {
-          if (tk * kTBK - kTBHalo > k) break;
-          if (k >= tk * kTBK - kTBHalo + kBX) continue;
-          pl->tb_ops_max = std::max(pl->tb_ops_max, ++cnt[(size_t)tj * ntk + tk]);
+        for (int tk = 0; tk < tio.ntk; ++tk) {
+          const int k0 = tio.kz0 + tk * kTBK;
+          if (k >= k0 - kTBHalo && k < k0 - kTBHalo + kBX) count((long long)tj * tio.ntk + tk);
+        }
+      }
+      for (int side = 0; side < 2 && tio.ntj_e > 0; ++side) {
+        const int k0 = side == 0 ? 0 : tio.kz1;
+        if (k < k0 - kTBHalo || k >= k0 - kTBHalo + kTBEdgeC) continue;
+        for (int te = 0; te < tio.ntj_e; ++te) {
+          const int j0 = te * (kTBEdgeR - 2 * kTBHalo);
+          if (j >= j0 - kTBHalo && j < j0 - kTBHalo + kTBEdgeR)
+            count(-1 - (long long)(side * tio.ntj_e + te));
         }
       }
     }
