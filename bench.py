@@ -243,40 +243,46 @@ def graded_run(n_steps=8000):
             "port_samples_finite": finite}
 
 
-def c5_specs(n_inst=48):
+def c5_specs(n_inst=48, sizes=None, steps=(1500, 4001), thick=(4, 21), seed=5):
     """Batched-sweep workload: 48 instances, interior sizes from {64^3, 96^3,
     128x128x64, 160x96x96, 192x192x48}, dx = 1 mm, 8-layer CPML, one
     dielectric slab (eps_r ~ U(2,12), 4-20 cells thick, normal to a random
     axis), an Ez pulse source (f0 ~ U(2,20) GHz), 4 Ez probes, 1500-4000
-    steps; seed 5."""
-    rng = np.random.default_rng(5)
-    sizes = [(64, 64, 64), (96, 96, 96), (128, 128, 64), (160, 96, 96), (192, 192, 48)]
+    steps; seed 5.  (sizes, steps, thick, seed: the same recipe at other
+    sizes, for tests.)"""
+    rng = np.random.default_rng(seed)
+    if sizes is None:
+        sizes = [(64, 64, 64), (96, 96, 96), (128, 128, 64), (160, 96, 96), (192, 192, 48)]
     dx = 1e-3
     out = []
     for q in range(n_inst):
         n = sizes[int(rng.integers(0, len(sizes)))]
         ax = int(rng.integers(0, 3))
-        th = int(rng.integers(4, 21))
+        th = int(rng.integers(*thick))
         c0 = int(rng.integers(1, n[ax] - th - 1))
         lo, hi = [0.0, 0.0, 0.0], [n[0] * dx, n[1] * dx, n[2] * dx]
         lo[ax], hi[ax] = (c0 + 0.5) * dx, (c0 + th + 0.5) * dx
         eps = float(rng.uniform(2, 12))
         f0 = float(rng.uniform(2e9, 20e9))
-        steps = int(rng.integers(1500, 4001))
+        n_steps = int(rng.integers(*steps))
         src = tuple(0.3 * v * dx for v in n)
         probes = [(tuple(float(rng.uniform(0.2, 0.8)) * v * dx for v in n), "ez") for _ in range(4)]
         out.append((dict(freq_max=25e9, domain=tuple(v * dx for v in n), dx=dx, boundary="cpml",
                          cpml_layers=8, materials={"slab": (eps, 0.0, 1.0)},
                          geometry=[(tuple(lo), tuple(hi), "slab")],
-                         sources=[(src, "ez", f0, 0.5)], probes=probes, dft_planes=[]), steps))
+                         sources=[(src, "ez", f0, 0.5)], probes=probes, dft_planes=[]), n_steps))
     return out
 
 
 def batch_sweep():
-    """Whole-sweep throughput of the batched launch vs the same instances
-    run one after another (each with its own captured graphs)."""
+    """The batched-sweep workload: 48 forwards, and for instances 0, 2, 4, ...
+    d(sum of squared probe samples)/d eps_r over the whole grid.  Timed
+    batched (one launch per sub-step over all active instances; the
+    gradient instances by batched reverse mode) and one instance after
+    another (each with its own captured graphs; per-instance reverse mode)."""
     import torch
     from rfx_amd.batch import Batch
+    from rfx_amd.adjoint import AdjointRunner, batch_gradients
     specs = c5_specs()
     sims = [build_sim(sp) for sp, _ in specs]
     steps = [n for _, n in specs]
@@ -295,36 +301,71 @@ def batch_sweep():
         r.reset()
         r.run(0, min(n, 64))
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    t_one = []
     for r, n in zip(b.runs, steps):
+        t0 = time.perf_counter()
         r.reset()
         r.run(0, n)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t_one.append(time.perf_counter() - t0)
+    t_seq = sum(t_one)
     problems = [r.prob for r in b.runs]
     b.close()
+    del b
     torch.cuda.empty_cache()
-    # instances 0, 2, 4, ... also need d(sum of squared probe samples)/d eps_r
-    # over the whole grid: forward with checkpoints + reverse sweep each
-    from rfx_amd.adjoint import AdjointRunner
+    grad_ids = list(range(0, len(problems), 2))
+    other = [q for q in range(len(problems)) if q not in grad_ids]
+    # the forward-only half, batched
+    fb = Batch([sims[q] for q in other], [steps[q] for q in other], "cuda:0")
+    at.autotune_batch(fb, trial_steps=64)
+    fb.run()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    n_adj = 0
-    for q in range(0, len(problems), 2):
+    fb.run()
+    torch.cuda.synchronize()
+    t_other = time.perf_counter() - t0
+    fb.close()
+    del fb
+    torch.cuda.empty_cache()
+    # the gradient half, batched reverse mode (forward with checkpoints,
+    # recomputation with recording, reverse sweep; setup included)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ts_b, grads_b = batch_gradients([problems[q] for q in grad_ids], [steps[q] for q in grad_ids],
+                                    lambda q, ts: 2.0 * ts, "cuda:0")
+    torch.cuda.synchronize()
+    t_adj_b = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(g).all()) for g in grads_b)
+    del ts_b, grads_b
+    torch.cuda.empty_cache()
+    # the gradient half, one instance after another
+    t0 = time.perf_counter()
+    for q in grad_ids:
         runner = AdjointRunner(problems[q], steps[q], "cuda:0")
         ts = runner.forward()
         grad = runner.backward(2.0 * ts)
-        n_adj += 1
+        runner.dr.close()
         del runner, grad
     torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
+    t_adj_s = time.perf_counter() - t0
     torch.cuda.empty_cache()
+    whole_b = t_other + t_adj_b
+    whole_s = sum(t_one[q] for q in other) + t_adj_s
     return {"workload": f"{len(specs)} instances, {min(steps)}-{max(steps)} steps, "
-                        f"{cell_steps / 1e9:.1f} G cell-steps",
+                        f"{cell_steps / 1e9:.1f} G cell-steps; d(sum ts^2)/d eps_r of "
+                        f"{len(grad_ids)} of them",
             "batched_variant": tune["variant"],
             "batched_s": round(t_batch, 3), "sequential_s": round(t_seq, 3),
             "batched_Mcell_steps_per_s": round(cell_steps / t_batch / 1e6, 1),
             "sequential_Mcell_steps_per_s": round(cell_steps / t_seq / 1e6, 1),
-            "adjoint_instances": n_adj, "adjoint_s_incl_setup": round(t_adj, 3)}
+            "adjoint_instances": len(grad_ids),
+            "adjoint_batched_s": round(t_adj_b, 3),
+            "adjoint_sequential_s": round(t_adj_s, 3),
+            "adjoint_gain": round(t_adj_s / t_adj_b, 2),
+            "gradients_finite": finite,
+            "whole_workload_batched_s": round(whole_b, 3),
+            "whole_workload_sequential_s": round(whole_s, 3),
+            "whole_workload_gain": round(whole_s / whole_b, 2)}
 
 
 def build_sim(spec):

@@ -279,6 +279,27 @@ int rfxh_batch_destroy(rfxh_batch* batch);
 int rfxh_batch_run(rfxh_batch* batch, const int64_t* step0, const int64_t* n_steps, void* stream,
                    int use_graph);
 
+/* Batched reverse mode: the gradients of a sweep's instances computed
+ * together, where the reference takes jax.grad of each instance's run
+ * (rfx/vmap_sweep.py:1017-1022 maps the gradient over the sweep;
+ * rfx/api/_execute.py:2447-2858 per instance).  Same semantics as
+ * rfxh_run_record / rfxh_adjoint per instance; every time step is one
+ * launch per sub-step covering all instances still active.
+ *
+ * Record: instance q advances n_steps[q] steps from step0[q] (0 = sits
+ * out) recording its region regions[q] into d_rec[q] (device, as
+ * rfxh_run_record); accumulate != 0 also takes the probe / DFT outputs of
+ * those steps, as rfxh_run_record does (0: a recomputation whose outputs
+ * are already known). */
+int rfxh_batch_run_record(rfxh_batch* batch, const int64_t* step0, const int64_t* n_steps,
+                          const rfxh_region* regions, float* const* d_rec, int accumulate,
+                          void* stream);
+/* Reverse sweep of instance q over steps hi[q]-1 down to lo[q] with
+ * descs[q] (NULL = sits out), as rfxh_adjoint; instances sweep their own
+ * step ranges side by side. */
+int rfxh_batch_adjoint(rfxh_batch* batch, const rfxh_adjoint_desc* const* descs,
+                       const int64_t* lo, const int64_t* hi, void* stream);
+
 /* ---- near-to-far-field transform ----
  * Far-field radiation of an NTFF box from its accumulated face DFTs
  * (rfx/farfield.py:393-556 compute_far_field, post-processing in NumPy in

@@ -110,7 +110,8 @@ _lib = None
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
            "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
-           "rfxh_batch_run", "rfxh_set_variant", "rfxh_get_variant", "rfxh_ntff_far_field",
+           "rfxh_batch_run", "rfxh_batch_run_record", "rfxh_batch_adjoint", "rfxh_set_variant",
+           "rfxh_get_variant", "rfxh_ntff_far_field",
            "rfxh_ntff_far_field_noalloc", "rfxh_assemble_fill", "rfxh_assemble_pec_pairs",
            "rfxh_assemble_pec_apply", "rfxh_last_error", "rfxh_version")
 
@@ -155,6 +156,12 @@ def load():
     lib.rfxh_batch_destroy.restype = C.c_int
     lib.rfxh_batch_run.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, C.c_int]
     lib.rfxh_batch_run.restype = C.c_int
+    lib.rfxh_batch_run_record.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(Region), C.POINTER(_P), C.c_int, _P]
+    lib.rfxh_batch_run_record.restype = C.c_int
+    lib.rfxh_batch_adjoint.argtypes = [_P, C.POINTER(C.POINTER(AdjointDesc)),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P]
+    lib.rfxh_batch_adjoint.restype = C.c_int
     lib.rfxh_set_variant.argtypes = [_P, C.c_int, C.c_int]
     lib.rfxh_set_variant.restype = C.c_int
     lib.rfxh_get_variant.argtypes = [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]

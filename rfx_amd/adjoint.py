@@ -143,29 +143,9 @@ class AdjointRunner:
 
     # -- reverse -------------------------------------------------------------------
     def backward(self, g_ts: torch.Tensor) -> torch.Tensor:
-        dev = self.dr.device
-        n_probe = len(self.prob.probes)
-        T = self.n_steps
-        probe_adj = torch.zeros((T, max(n_probe, 1)), dtype=torch.float32, device=dev)
-        if n_probe:
-            probe_adj[:, :n_probe] = g_ts.to(dev, torch.float32).reshape(T, n_probe)
-        n_src = len(self.prob.sources)
-        src_adj = torch.zeros((T, max(n_src, 1)), dtype=torch.float32, device=dev)
-        npsi = int(self.dr.psis[0].numel())
-        adj = [torch.zeros(self.prob.shape, dtype=torch.float32, device=dev) for _ in range(6)]
-        adj_psi = torch.zeros(npsi, dtype=torch.float32, device=dev)
-        gbar = torch.zeros(npsi, dtype=torch.float32, device=dev)
-        grad = torch.zeros(self.nreg, dtype=torch.float32, device=dev)
-        d = _lib.AdjointDesc()
-        for c in range(6):
-            d.d_adj[c] = adj[c].data_ptr()
-        d.d_adj_psi = adj_psi.data_ptr()
-        d.d_gbar = gbar.data_ptr()
-        d.d_probe_adj = probe_adj.data_ptr()
-        d.d_src_adj = src_adj.data_ptr()
-        d.region = self.region
-        d.d_grad = grad.data_ptr()
-        d.dcoef = self.dcoef.ctypes.data_as(C.POINTER(C.c_float))
+        st = _ReverseState(self.prob, self.dr, self.n_steps, g_ts, self.region, self.nreg,
+                           self.dcoef)
+        d = st.desc
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
             if len(self.bounds) > 1:
@@ -175,31 +155,296 @@ class AdjointRunner:
             d.d_rec = self.rec.data_ptr()
             d.rec_step0 = lo
             d.rec_rows = hi - lo
-            with torch.cuda.device(dev):
+            with torch.cuda.device(self.dr.device):
                 _lib.check(self.lib.rfxh_adjoint(self.dr.plan, C.byref(d), lo, hi,
                                                  self.dr.stream_ptr()))
-        torch.cuda.synchronize(dev)
-        i0, i1, j0, j1, k0, k1 = self.box
-        full = torch.zeros(self.prob.shape, dtype=torch.float32, device=dev)
-        full[i0:i1, j0:j1, k0:k1] = grad.reshape(i1 - i0, j1 - j0, k1 - k0)
+        torch.cuda.synchronize(self.dr.device)
+        return st.full_gradient(self.box)
+
+
+class _ReverseState:
+    """One instance's reverse-mode buffers (adjoint fields, adjoint CPML
+    memory and its g-bar, dJ/dy of the probes, dJ/dw of the sources, the
+    region gradient) and the descriptor that points at them."""
+
+    def __init__(self, prob: Problem, dr: DeviceRun, T: int, g_ts: torch.Tensor, region,
+                 nreg: int, dcoef: np.ndarray):
+        dev = dr.device
+        self.prob = prob
+        self.T = T
+        n_probe = len(prob.probes)
+        self.probe_adj = torch.zeros((T, max(n_probe, 1)), dtype=torch.float32, device=dev)
+        if n_probe:
+            self.probe_adj[:, :n_probe] = g_ts.to(dev, torch.float32).reshape(T, n_probe)
+        self.n_src = len(prob.sources)
+        self.src_adj = torch.zeros((T, max(self.n_src, 1)), dtype=torch.float32, device=dev)
+        npsi = int(dr.psis[0].numel())
+        self.adj = [torch.zeros(prob.shape, dtype=torch.float32, device=dev) for _ in range(6)]
+        self.adj_psi = torch.zeros(npsi, dtype=torch.float32, device=dev)
+        self.gbar = torch.zeros(npsi, dtype=torch.float32, device=dev)
+        self.grad = torch.zeros(nreg, dtype=torch.float32, device=dev)
+        self.dcoef = dcoef
+        d = _lib.AdjointDesc()
+        for c in range(6):
+            d.d_adj[c] = self.adj[c].data_ptr()
+        d.d_adj_psi = self.adj_psi.data_ptr()
+        d.d_gbar = self.gbar.data_ptr()
+        d.d_probe_adj = self.probe_adj.data_ptr()
+        d.d_src_adj = self.src_adj.data_ptr()
+        d.region = region
+        d.d_grad = self.grad.data_ptr()
+        d.dcoef = self.dcoef.ctypes.data_as(C.POINTER(C.c_float))
+        self.desc = d
+
+    def full_gradient(self, box) -> torch.Tensor:
+        """dJ/d eps_r on the whole grid (zero outside the region box)."""
+        prob = self.prob
+        i0, i1, j0, j1, k0, k1 = box
+        full = torch.zeros(prob.shape, dtype=torch.float32, device=self.grad.device)
+        full[i0:i1, j0:j1, k0:k1] = self.grad.reshape(i1 - i0, j1 - j0, k1 - k0)
         # uniform lane: source tables depend on eps_r through Cb at the
         # source cell (rfx/simulation.py:181-194, 275-284): w[n] = Cb * s[n].
         # The graded lane normalises sources with the materials before the
         # override (rfx/runners/nonuniform.py:630-643): no dependence.
-        if n_src and not self.prob.graded:
-            sa = src_adj[:, :n_src].double().cpu().numpy()
-            eps_r, sig, _, _ = self.prob.mat.props()
-            for q, (idx, comp, wave) in enumerate(self.prob.sources):
-                m = int(self.prob.mat.ids[tuple(idx)])
+        if self.n_src and not prob.graded:
+            sa = self.src_adj[:, :self.n_src].double().cpu().numpy()
+            eps_r, sig, _, _ = prob.mat.props()
+            for q, (idx, comp, wave) in enumerate(prob.sources):
+                m = int(prob.mat.ids[tuple(idx)])
                 e = float(eps_r[m])
                 s = float(sig[m])
                 eps = e * EPS_0
-                L = s * self.prob.dt / (2.0 * eps)
-                cb = (self.prob.dt / eps) / (1.0 + L)
+                L = s * prob.dt / (2.0 * eps)
+                cb = (prob.dt / eps) / (1.0 + L)
                 dcb = -cb / (e * (1.0 + L))
-                contrib = dcb / cb * float(np.dot(sa[:, q], np.asarray(wave[:T], np.float64)))
+                contrib = dcb / cb * float(np.dot(sa[:, q], np.asarray(wave[:self.T], np.float64)))
                 full[tuple(idx)] += contrib
         return full
+
+
+DEFAULT_BATCH_BUDGET = 192 << 30  # bytes: records and checkpoints of a batched reverse run
+
+
+def batch_segment_steps(T, per_step, state, budget: int, forced: int | None = None) -> int:
+    """Segment length S (steps) of a batched reverse run.
+
+    Instance q (T[q] steps, record per_step[q] bytes per step, state
+    checkpoint state[q] bytes) is cut into ceil(T[q]/S) segments; a
+    segment's recomputation records all instances at once, so the
+    device memory is  S * sum(per_step) + sum((ceil(T/S) - 1) * state)
+    (the state at step 0 is the zero state, never stored).  S = max(T)
+    (the forward records every step; no recomputation) when that fits,
+    else the largest S that fits (fewest segments); MemoryError when no S
+    does (fewer instances per batch, or smaller gradient regions, fit)."""
+    T = np.asarray(T, np.int64)
+    per = np.asarray(per_step, np.float64)
+    st = np.asarray(state, np.float64)
+    t_max = int(T.max())
+    if forced is not None and int(forced) > 0:
+        return max(1, min(int(forced), t_max))
+    A = float(per.sum())
+    if t_max * A <= budget:
+        return t_max
+    S = np.arange(1, t_max + 1, dtype=np.int64)
+    need = S * A + ((np.ceil(T[None, :] / S[:, None]) - 1.0) * st[None, :]).sum(axis=1)
+    ok = np.nonzero(need <= budget)[0]
+    if len(ok) == 0:
+        best = int(np.argmin(need))
+        raise MemoryError(
+            f"batched reverse mode needs at least {need[best] / 2**30:.1f} GiB (segments of "
+            f"{best + 1} steps) but the budget is {budget / 2**30:.1f} GiB; split the batch "
+            "or pass smaller gradient regions")
+    return int(S[ok[-1]])
+
+
+class BatchAdjointRunner:
+    """Reverse mode of many instances together (include/rfxhip.h
+    ``rfxh_batch_run_record`` / ``rfxh_batch_adjoint``), the batched
+    counterpart of AdjointRunner: where the reference maps jax.grad over a
+    sweep (rfx/vmap_sweep.py:1017-1022), every sub-step here is one launch
+    over all instances still active.
+
+    The instances' steps are cut into segments of S steps
+    (``batch_segment_steps``).  The forward runs the batch segment by
+    segment, keeping each instance's state at the segment starts; the
+    backward, from the last segment to the first, restores every
+    instance's state there, recomputes the segment for all of them with
+    recording, and sweeps it in reverse for all of them.  With one segment
+    the forward records directly and nothing is recomputed."""
+
+    def __init__(self, problems, n_steps, device, regions=None,
+                 record_budget: int = DEFAULT_BATCH_BUDGET, segment_steps: int | None = None):
+        if not problems:
+            raise ValueError("empty batch")
+        self.problems = list(problems)
+        n = len(self.problems)
+        self.T = [int(v) for v in n_steps] if not np.isscalar(n_steps) else [int(n_steps)] * n
+        if len(self.T) != n:
+            raise ValueError("one step count per problem")
+        for prob in self.problems:
+            if prob.ports:
+                raise NotImplementedError("gradients through lumped-port V/I samples are not "
+                                          "supported")
+        regions = [None] * n if regions is None else list(regions)
+        self.runs = [DeviceRun(p, t, device, two_sets=False) for p, t in zip(self.problems, self.T)]
+        self.device = self.runs[0].device
+        self.lib = self.runs[0].lib
+        self.boxes, self.regions, self.nreg = [], [], []
+        for prob, reg in zip(self.problems, regions):
+            shape = prob.shape
+            if reg is None:
+                reg = (0, shape[0], 0, shape[1], 0, shape[2])
+            i0, i1, j0, j1, k0, k1 = (int(v) for v in reg)
+            if not (0 <= i0 < i1 <= shape[0] and 0 <= j0 < j1 <= shape[1]
+                    and 0 <= k0 < k1 <= shape[2]):
+                raise ValueError(f"gradient region {reg} outside the grid {shape}")
+            self.boxes.append((i0, i1, j0, j1, k0, k1))
+            self.regions.append(_lib.Region(i0, j0, k0, i1 - i0, j1 - j0, k1 - k0))
+            self.nreg.append((i1 - i0) * (j1 - j0) * (k1 - k0))
+        per_step = [9 * 4 * v for v in self.nreg]
+        state = [4 * (6 * int(np.prod(p.shape)) + int(r.psis[0].numel()))
+                 for p, r in zip(self.problems, self.runs)]
+        budget = int(record_budget)
+        if self.device.type == "cuda":
+            free, _ = torch.cuda.mem_get_info(self.device)
+            budget = min(budget, int(0.8 * free))
+        self.S = batch_segment_steps(self.T, per_step, state, budget, segment_steps)
+        self.single = self.S >= max(self.T)
+        self.bounds = [[(lo, min(t, lo + self.S)) for lo in range(0, t, self.S)] for t in self.T]
+        self.dcoef = [eps_derivative_table(p.mat, p.dt) for p in self.problems]
+        self.rec = [None] * n
+        self.checkpoints = [dict() for _ in range(n)]
+        plans = (C.c_void_p * n)(*[r.plan.value for r in self.runs])
+        self.handle = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.rfxh_batch_create(plans, n, C.byref(self.handle)))
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _record(self, step0, counts, accumulate):
+        n = len(self.runs)
+        recs = (C.c_void_p * n)()
+        for q in range(n):
+            if counts[q] == 0:
+                continue
+            rows = min(self.S, self.T[q])
+            if self.rec[q] is None:
+                self.rec[q] = torch.empty((rows, 9, self.nreg[q]), dtype=torch.float32,
+                                          device=self.device)
+            recs[q] = self.rec[q].data_ptr()
+        regs = (_lib.Region * n)(*self.regions)
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.rfxh_batch_run_record(
+                self.handle, (C.c_int64 * n)(*step0), (C.c_int64 * n)(*counts), regs, recs,
+                1 if accumulate else 0, self._stream()))
+
+    # -- forward -----------------------------------------------------------------
+    def forward(self) -> list:
+        n = len(self.runs)
+        for r in self.runs:
+            r.reset()
+        if self.single:
+            self._record([0] * n, self.T, True)
+        else:
+            for c in range(max(len(b) for b in self.bounds)):
+                step0, counts = [], []
+                for q, r in enumerate(self.runs):
+                    if c < len(self.bounds[q]):
+                        lo, hi = self.bounds[q][c]
+                        if c > 0:
+                            self.checkpoints[q][c] = ([f.clone() for f in r.fields], r.psi.clone())
+                        step0.append(lo)
+                        counts.append(hi - lo)
+                    else:
+                        step0.append(self.T[q])
+                        counts.append(0)
+                with torch.cuda.device(self.device):
+                    _lib.check(self.lib.rfxh_batch_run(
+                        self.handle, (C.c_int64 * n)(*step0), (C.c_int64 * n)(*counts),
+                        self._stream(), 1))
+        return [r.time_series().clone() for r in self.runs]
+
+    # -- reverse -------------------------------------------------------------------
+    def backward(self, g_ts) -> list:
+        """g_ts[q]: dJ/d(time series of instance q); returns dJ/d eps_r per
+        instance on its whole grid (zero outside its region)."""
+        n = len(self.runs)
+        states = [_ReverseState(p, r, t, g, reg, nr, dc) for p, r, t, g, reg, nr, dc in
+                  zip(self.problems, self.runs, self.T, g_ts, self.regions, self.nreg, self.dcoef)]
+        for c in range(max(len(b) for b in self.bounds) - 1, -1, -1):
+            part = [q for q in range(n) if c < len(self.bounds[q])]
+            lo = [0] * n
+            hi = [0] * n
+            for q in part:
+                lo[q], hi[q] = self.bounds[q][c]
+            if not self.single:
+                for q in part:
+                    r = self.runs[q]
+                    if c == 0:
+                        r.reset()
+                    else:
+                        r.load_state(*self.checkpoints[q][c])
+                self._record(lo, [h - l for l, h in zip(lo, hi)], False)
+            descs = (C.POINTER(_lib.AdjointDesc) * n)()
+            for q in part:
+                d = states[q].desc
+                d.d_rec = self.rec[q].data_ptr()
+                d.rec_step0 = lo[q]
+                d.rec_rows = hi[q] - lo[q]
+                descs[q] = C.pointer(d)
+            with torch.cuda.device(self.device):
+                _lib.check(self.lib.rfxh_batch_adjoint(
+                    self.handle, descs, (C.c_int64 * n)(*lo), (C.c_int64 * n)(*hi),
+                    self._stream()))
+        torch.cuda.synchronize(self.device)
+        return [st.full_gradient(box) for st, box in zip(states, self.boxes)]
+
+    def close(self):
+        if self.handle:
+            self.lib.rfxh_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+        for r in self.runs:
+            r.close()
+        self.rec = [None] * len(self.runs)
+        self.checkpoints = [dict() for _ in self.runs]
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_gradients(problems, n_steps, dj_dts, device, regions=None,
+                    record_budget: int = DEFAULT_BATCH_BUDGET):
+    """Time series and dJ/d eps_r of many instances by batched reverse mode.
+
+    dj_dts(q, ts) returns dJ/d(time series) of instance q from its time
+    series.  The instances run as one BatchAdjointRunner when its records
+    and checkpoints fit the budget, else as consecutive halves (recursively).
+    Returns (time series, gradients), one of each per instance, in order."""
+    probs = list(problems)
+    steps = [int(v) for v in n_steps] if not np.isscalar(n_steps) else [int(n_steps)] * len(probs)
+    regs = [None] * len(probs) if regions is None else list(regions)
+
+    def run(lo, hi):
+        try:
+            r = BatchAdjointRunner(probs[lo:hi], steps[lo:hi], device, regs[lo:hi], record_budget)
+        except MemoryError:
+            if hi - lo == 1:
+                raise
+            mid = (lo + hi) // 2
+            a, b = run(lo, mid), run(mid, hi)
+            return a[0] + b[0], a[1] + b[1]
+        try:
+            ts = r.forward()
+            grads = r.backward([dj_dts(lo + q, t) for q, t in enumerate(ts)])
+            return ts, grads
+        finally:
+            r.close()
+
+    return run(0, len(probs))
 
 
 class _ForwardFn(torch.autograd.Function):

@@ -2417,10 +2417,10 @@ __device__ __forceinline__ long long region_index(const RegionParams& r, int i, 
   return ((long long)a * r.nj + b) * r.nk + c;
 }
 
+// the record of region cell q (row-major in the region box) into rec
 template <bool GRADED>
-__global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParams r, float* rec) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= r.n) return;
+__device__ __forceinline__ void record_cell(const KParams& p, const PsiIO& ps,
+                                            const RegionParams& r, long long q, float* rec) {
   const int c3 = (int)(q % r.nk);
   const long long t = q / r.nk;
   const int i = r.i0 + (int)(t / r.nj), j = r.j0 + (int)(t % r.nj), k = r.k0 + c3;
@@ -2481,8 +2481,19 @@ __global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParam
                          (hx - hx_k) * iz - (hz - hz_i) * ix,
                          (hy - hy_i) * ix - (hx - hx_j) * iy,
                          cp[0], cp[1], cp[2]};
+  // the CPML-E rows only for cells in a slab (zero elsewhere; the reverse
+  // sweep reads them only there)
+  const int nv = in_slab(p, i, j, k) ? 9 : 6;
 #pragma unroll
-  for (int v = 0; v < 9; ++v) rec[(long long)v * r.n + q] = vals[v];
+  for (int v = 0; v < 9; ++v)
+    if (v < nv) rec[(long long)v * r.n + q] = vals[v];
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParams r, float* rec) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= r.n) return;
+  record_cell<GRADED>(p, ps, r, q, rec);
 }
 
 // ch or cb of the cell's material (w = ch, y = cb)
@@ -2491,9 +2502,8 @@ __device__ __forceinline__ float4 coef_at(const KParams& p, unsigned c) {
 }
 
 template <bool GRADED>
-__global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long n) {
-  int i, j, k;
-  if (!cell_of(p, i, j, k)) return;
+__device__ __forceinline__ void adj_h_cell(const KParams& p, const AdjParams& a, int i, int j,
+                                           int k, long long n) {
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
   const unsigned P = (unsigned)p.plane;
   const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
@@ -2611,9 +2621,15 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
 }
 
 template <bool GRADED>
-__global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gather, long long m) {
+__global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long n) {
   int i, j, k;
   if (!cell_of(p, i, j, k)) return;
+  adj_h_cell<GRADED>(p, a, i, j, k, n);
+}
+
+template <bool GRADED>
+__device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a, int i, int j,
+                                           int k, int gather, long long m) {
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
   const unsigned P = (unsigned)p.plane;
   const unsigned mid0 = ldu16(p.mat, c);
@@ -2757,7 +2773,9 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
         const long long N = a.reg.n;
         const float A = ex * rr[0] + ey * rr[N] + ez * rr[2 * N];
         const float B = ex * rr[3 * N] + ey * rr[4 * N] + ez * rr[5 * N];
-        const float C = ex * rr[6 * N] + ey * rr[7 * N] + ez * rr[8 * N];
+        // the record holds the CPML-E rows only for slab cells
+        const float C = in_slab(p, i, j, k) ? ex * rr[6 * N] + ey * rr[7 * N] + ez * rr[8 * N]
+                                            : 0.f;
         const float4 dc = a.dcoef[mid0];
         a.grad[r] += dc.x * A + dc.y * B + dc.z * C;
       }
@@ -2766,6 +2784,79 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
   stf(a.a[0], c, ex);
   stf(a.a[1], c, ey);
   stf(a.a[2], c, ez);
+}
+
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gather, long long m) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  adj_e_cell<GRADED>(p, a, i, j, k, gather, m);
+}
+
+// ---------------------------------------------------------------------------
+// Batched reverse mode: the forward record and the reverse sweep of many
+// instances per launch, over a batch's instance table as k_batch_pass does
+// for the forward (instance q owns logical workgroups [start[q],
+// start[q+1]), all instance parameters read through the table).
+// ---------------------------------------------------------------------------
+struct BatchAdjEntry {
+  AdjParams a;
+  long long lo, hi;      // this instance's reverse sweep: steps hi-1 down to lo
+};
+
+struct BatchRecEntry {
+  RegionParams r;
+  float* rec;            // [rows][9][r.n]
+  long long rec_step0;   // step of row 0
+};
+
+// E pass of the active instances, each cell of an instance's region first
+// recording its E^n, curl H^{n+1/2} and CPML-E term at the instance's
+// current step (the update then finds the same inputs in cache).  Plain
+// isotropic plans only (the record path refuses the others).
+__global__ __launch_bounds__(kWG) void k_batch_e_rec(const BatchEntry* __restrict__ ents,
+                                                     const BatchRecEntry* __restrict__ recs,
+                                                     const int* __restrict__ start, int n_act) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int q = batch_instance(start, n_act, L);
+  const KParams& p = ents[q].p;
+  int i, j, k;
+  if (!cell_at(p, L - gld(start, q), i, j, k)) return;
+  const BatchRecEntry& e = recs[q];
+  const long long t = region_index(e.r, i, j, k);
+  if (t >= 0) {
+    float* rec = e.rec + (*p.step - e.rec_step0) * 9 * e.r.n;
+    if (p.graded) record_cell<true>(p, ents[q].ps, e.r, t, rec);
+    else record_cell<false>(p, ents[q].ps, e.r, t, rec);
+  }
+  if (p.graded) two_pass_e<true, false>(p, ents[q].ps, i, j, k);
+  else two_pass_e<false, false>(p, ents[q].ps, i, j, k);
+}
+
+// Reverse step r (counted from each instance's hi) of the active instances:
+// E = false the H-bar gather of step n = hi-1-r; E = true the E-bar gather
+// and the prologue of step n-1 (gather = 0: only the prologue of step
+// hi-1, before the first reverse step).
+template <bool E>
+__global__ __launch_bounds__(kWG) void k_batch_adj(const BatchEntry* __restrict__ ents,
+                                                   const BatchAdjEntry* __restrict__ adj,
+                                                   const int* __restrict__ start, int n_act,
+                                                   int gather, long long r) {
+  const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int q = batch_instance(start, n_act, L);
+  const KParams& p = ents[q].p;
+  int i, j, k;
+  if (!cell_at(p, L - gld(start, q), i, j, k)) return;
+  const BatchAdjEntry& e = adj[q];
+  const long long n = e.hi - 1 - r;
+  if (E) {
+    const long long m = !gather ? n : n - 1 >= e.lo ? n - 1 : -1;
+    if (p.graded) adj_e_cell<true>(p, e.a, i, j, k, gather, m);
+    else adj_e_cell<false>(p, e.a, i, j, k, gather, m);
+  } else {
+    if (p.graded) adj_h_cell<true>(p, e.a, i, j, k, n);
+    else adj_h_cell<false>(p, e.a, i, j, k, n);
+  }
 }
 
 __global__ void k_set_step(long long* step, long long value) {
@@ -3994,7 +4085,10 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
   return RFXH_OK;
 }
 
-int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
+// Checks of one instance's reverse sweep over steps [lo, hi) and its
+// kernel parameters.
+static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi,
+                          AdjParams* out) {
   if (!pl || !ad) return fail(RFXH_EINVAL, "null plan or adjoint descriptor");
   if (pl->n_pole > 0 || pl->aniso)
     return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode covers "
@@ -4011,8 +4105,7 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
     if (rc) return rc;
     if (!ad->d_rec || !ad->dcoef || ad->rec_rows < 0) return fail(RFXH_EINVAL, "forward record missing");
   }
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  AdjParams a;
+  AdjParams& a = *out;
   std::memset(&a, 0, sizeof(a));
   for (int c = 0; c < 6; ++c) a.a[c] = ad->d_adj[c];
   // adjoint CPML memory and g-bar share the psi layout
@@ -4056,6 +4149,14 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
     a.grad = ad->d_grad;
     a.dcoef = pl->dcoef;
   }
+  return RFXH_OK;
+}
+
+int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
+  AdjParams a;
+  const int rc = adjoint_params(pl, ad, lo, hi, &a);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
   if (hi == lo) return RFXH_OK;
   // prologue of step hi-1 (E-bar holds the adjoint of E^{hi})
   if (pl->kp.graded)
@@ -4090,6 +4191,13 @@ struct rfxh_batch {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::map<int, hipGraphExec_t> graphs;  // keyed by active count
   int graph_chunk = 16;
+  // tables of the record and reverse-mode calls (the run's captured graphs
+  // read d_ents, so these calls keep their own)
+  BatchEntry* d_xents = nullptr;
+  int* d_xstart = nullptr;
+  long long* d_xsteps = nullptr;
+  BatchRecEntry* d_rents = nullptr;
+  BatchAdjEntry* d_aents = nullptr;
 };
 
 namespace {
@@ -4099,26 +4207,60 @@ void batch_drop_graphs(rfxh_batch* b) {
   b->graphs.clear();
 }
 
+// Instance table rows (current field sets) and first logical workgroups of
+// the plans in `order`.
+int batch_table(const rfxh_batch* b, const std::vector<int>& order, std::vector<BatchEntry>& ents,
+                std::vector<int>& start) {
+  const int n = (int)order.size();
+  ents.assign(n, BatchEntry());
+  start.assign(n + 1, 0);
+  for (int r = 0; r < n; ++r) {
+    rfxh_plan* pl = b->plans[order[r]];
+    ents[r].p = pl->kp;
+    for (int c = 0; c < 6; ++c) ents[r].p.f[c] = pl->set[pl->cur][c];
+    ents[r].ps = psi_io(pl, pl->cur);
+    const KParams& kq = pl->kp;
+    const long long flat = kq.plane, nxb = kq.nx;
+    const long long blk = kq.order == 0 ? 8LL * nxb * (((flat + 7) / 8 + kWG - 1) / kWG)
+                                        : nxb * ((flat + kWG - 1) / kWG);
+    const long long nb = (long long)start[r] + blk;
+    if (nb > 0x7fffffffLL) return fail(RFXH_EINVAL, "batch too large for one launch");
+    start[r + 1] = (int)nb;
+  }
+  return RFXH_OK;
+}
+
+// One pass (0: H, 1: E) of the first n_act instances of a table whose rows
+// are the plans in `order`; nb = their logical workgroups.
+void batch_pass(const rfxh_batch* b, hipStream_t s, const BatchEntry* d_ents, const int* d_start,
+                const std::vector<int>& order, unsigned nb, int n_act, int pass) {
+  if (pass == 0) {
+    bool one_mu = true;
+    for (int q = 0; q < n_act; ++q) one_mu = one_mu && b->plans[order[q]]->kp.ch_all > 0.f;
+    if (one_mu)
+      hipLaunchKernelGGL((k_batch_pass<0, true, false>), dim3(nb), dim3(kBX, kTY), 0, s, d_ents,
+                         d_start, n_act);
+    else
+      hipLaunchKernelGGL((k_batch_pass<0, false, false>), dim3(nb), dim3(kBX, kTY), 0, s, d_ents,
+                         d_start, n_act);
+  } else {
+    bool disp = false;
+    for (int q = 0; q < n_act; ++q)
+      disp = disp || b->plans[order[q]]->n_pole > 0 || b->plans[order[q]]->aniso;
+    if (disp)
+      hipLaunchKernelGGL((k_batch_pass<1, false, true>), dim3(nb), dim3(kBX, kTY), 0, s, d_ents,
+                         d_start, n_act);
+    else
+      hipLaunchKernelGGL((k_batch_pass<1, false, false>), dim3(nb), dim3(kBX, kTY), 0, s, d_ents,
+                         d_start, n_act);
+  }
+}
+
 // One step of the first n_act instances (run order).
 int batch_step(rfxh_batch* b, hipStream_t s, int n_act) {
   const unsigned nb = (unsigned)b->start[n_act];
-  bool one_mu = true, disp = false;
-  for (int q = 0; q < n_act; ++q) {
-    one_mu = one_mu && b->plans[b->order[q]]->kp.ch_all > 0.f;
-    disp = disp || b->plans[b->order[q]]->n_pole > 0 || b->plans[b->order[q]]->aniso;
-  }
-  if (one_mu)
-    hipLaunchKernelGGL((k_batch_pass<0, true, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
-                       b->d_start, n_act);
-  else
-    hipLaunchKernelGGL((k_batch_pass<0, false, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
-                       b->d_start, n_act);
-  if (disp)
-    hipLaunchKernelGGL((k_batch_pass<1, false, true>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
-                       b->d_start, n_act);
-  else
-    hipLaunchKernelGGL((k_batch_pass<1, false, false>), dim3(nb), dim3(kBX, kTY), 0, s, b->d_ents,
-                       b->d_start, n_act);
+  batch_pass(b, s, b->d_ents, b->d_start, b->order, nb, n_act, 0);
+  batch_pass(b, s, b->d_ents, b->d_start, b->order, nb, n_act, 1);
   for (int q = 0; q < n_act; ++q) {
     rfxh_plan* pl = b->plans[b->order[q]];
     launch_accumulators(pl, s, pl->cur);
@@ -4146,7 +4288,12 @@ int rfxh_batch_create(rfxh_plan* const* plans, int n_plans, rfxh_batch** out) {
   };
   if (hipMalloc(&b->d_ents, sizeof(BatchEntry) * n_plans) != hipSuccess ||
       hipMalloc(&b->d_start, sizeof(int) * (n_plans + 1)) != hipSuccess ||
-      hipMalloc(&b->d_steps, sizeof(long long) * n_plans) != hipSuccess)
+      hipMalloc(&b->d_steps, sizeof(long long) * n_plans) != hipSuccess ||
+      hipMalloc(&b->d_xents, sizeof(BatchEntry) * n_plans) != hipSuccess ||
+      hipMalloc(&b->d_xstart, sizeof(int) * (n_plans + 1)) != hipSuccess ||
+      hipMalloc(&b->d_xsteps, sizeof(long long) * n_plans) != hipSuccess ||
+      hipMalloc(&b->d_rents, sizeof(BatchRecEntry) * n_plans) != hipSuccess ||
+      hipMalloc(&b->d_aents, sizeof(BatchAdjEntry) * n_plans) != hipSuccess)
     return bail(fail(RFXH_ENOMEM, "batch tables"));
   *out = b;
   return RFXH_OK;
@@ -4161,6 +4308,9 @@ int rfxh_batch_destroy(rfxh_batch* b) {
   if (b->d_ents) (void)hipFree(b->d_ents);
   if (b->d_start) (void)hipFree(b->d_start);
   if (b->d_steps) (void)hipFree(b->d_steps);
+  for (void* d : {(void*)b->d_xents, (void*)b->d_xstart, (void*)b->d_xsteps, (void*)b->d_rents,
+                  (void*)b->d_aents})
+    if (d) (void)hipFree(d);
   delete b;
   return RFXH_OK;
 }
@@ -4180,23 +4330,12 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
   for (int q = 0; q < n; ++q) order[q] = q;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int c) { return n_steps[a] > n_steps[c]; });
-  std::vector<BatchEntry> ents(n);
-  std::vector<int> start(n + 1, 0);
+  std::vector<BatchEntry> ents;
+  std::vector<int> start;
+  const int trc = batch_table(b, order, ents, start);
+  if (trc) return trc;
   std::vector<long long> st(n);
-  for (int r = 0; r < n; ++r) {
-    rfxh_plan* pl = b->plans[order[r]];
-    ents[r].p = pl->kp;
-    for (int c = 0; c < 6; ++c) ents[r].p.f[c] = pl->set[pl->cur][c];
-    ents[r].ps = psi_io(pl, pl->cur);
-    const KParams& kq = pl->kp;
-    const long long flat = kq.plane, nxb = kq.nx;
-    const long long blk = kq.order == 0 ? 8LL * nxb * (((flat + 7) / 8 + kWG - 1) / kWG)
-                                        : nxb * ((flat + kWG - 1) / kWG);
-    const long long nb = (long long)start[r] + blk;
-    if (nb > 0x7fffffffLL) return fail(RFXH_EINVAL, "batch too large for one launch");
-    start[r + 1] = (int)nb;
-    st[r] = step0[order[r]];
-  }
+  for (int r = 0; r < n; ++r) st[r] = step0[order[r]];
   // the captured graphs launch per-instance plane kernels: keep them only
   // while the instance table is unchanged
   bool same = order == b->order && start == b->start;
@@ -4268,6 +4407,116 @@ int rfxh_batch_run(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps, 
     HIPCHK(hipEventRecord(b->ev_out, b->cap_stream));
     HIPCHK(hipStreamWaitEvent(s, b->ev_out, 0));
   }
+  return RFXH_OK;
+}
+
+int rfxh_batch_run_record(rfxh_batch* b, const int64_t* step0, const int64_t* n_steps,
+                          const rfxh_region* regions, float* const* d_rec, int accumulate,
+                          void* stream) {
+  if (!b || !step0 || !n_steps || !regions || !d_rec)
+    return fail(RFXH_EINVAL, "bad batch record arguments");
+  const int n = (int)b->plans.size();
+  std::vector<int> order;
+  for (int q = 0; q < n; ++q) {
+    rfxh_plan* pl = b->plans[q];
+    if (step0[q] < 0 || n_steps[q] < 0 || step0[q] + n_steps[q] > pl->max_steps)
+      return fail(RFXH_EINVAL, "batch instance steps outside its tables (max_steps)");
+    if (n_steps[q] == 0) continue;
+    if (pl->n_pole > 0 || pl->aniso)
+      return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode "
+                               "covers plain isotropic plans only");
+    const int rc = check_region(pl, &regions[q]);
+    if (rc) return rc;
+    if (!d_rec[q]) return fail(RFXH_EINVAL, "record buffer missing");
+    order.push_back(q);
+  }
+  if (order.empty()) return RFXH_OK;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return n_steps[a] > n_steps[c]; });
+  const int na = (int)order.size();
+  std::vector<BatchEntry> ents;
+  std::vector<int> start;
+  int rc = batch_table(b, order, ents, start);
+  if (rc) return rc;
+  std::vector<BatchRecEntry> recs(na);
+  std::vector<long long> st(na);
+  for (int r = 0; r < na; ++r) {
+    const int q = order[r];
+    recs[r].r = region_params(&regions[q]);
+    recs[r].rec = d_rec[q];
+    recs[r].rec_step0 = step0[q];
+    st[r] = step0[q];
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIPCHK(hipMemcpyAsync(b->d_xents, ents.data(), sizeof(BatchEntry) * na, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_xstart, start.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_xsteps, st.data(), sizeof(long long) * na, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_rents, recs.data(), sizeof(BatchRecEntry) * na, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_batch_set_steps, dim3((unsigned)((na + 63) / 64)), dim3(64), 0, s, b->d_xents,
+                     b->d_xsteps, na);
+  if (accumulate)
+    for (int q : order) dft_begin(b->plans[q], s, step0[q]);
+  int act = na;
+  for (int64_t t = 0; t < n_steps[order[0]]; ++t) {
+    while (act > 0 && n_steps[order[act - 1]] <= t) --act;
+    const unsigned nb = (unsigned)start[act];
+    batch_pass(b, s, b->d_xents, b->d_xstart, order, nb, act, 0);
+    hipLaunchKernelGGL(k_batch_e_rec, dim3(nb), dim3(kBX, kTY), 0, s, b->d_xents, b->d_rents,
+                       b->d_xstart, act);
+    if (accumulate)
+      for (int r = 0; r < act; ++r) launch_accumulators(b->plans[order[r]], s, b->plans[order[r]]->cur);
+    hipLaunchKernelGGL(k_batch_advance, dim3((unsigned)((act + 63) / 64)), dim3(64), 0, s,
+                       b->d_xents, act);
+  }
+  if (accumulate)
+    for (int q : order) dft_flush(b->plans[q], s);
+  HIPCHK(hipGetLastError());
+  return RFXH_OK;
+}
+
+int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, const int64_t* lo,
+                       const int64_t* hi, void* stream) {
+  if (!b || !descs || !lo || !hi) return fail(RFXH_EINVAL, "bad batch adjoint arguments");
+  const int n = (int)b->plans.size();
+  std::vector<int> order;
+  std::vector<BatchAdjEntry> all(n);
+  for (int q = 0; q < n; ++q) {
+    if (!descs[q]) continue;
+    const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);
+    if (rc) return rc;
+    all[q].lo = lo[q];
+    all[q].hi = hi[q];
+    if (hi[q] > lo[q]) order.push_back(q);
+  }
+  if (order.empty()) return RFXH_OK;
+  // longest sweep first: the active instances are always a prefix
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return hi[a] - lo[a] > hi[c] - lo[c]; });
+  const int na = (int)order.size();
+  std::vector<BatchEntry> ents;
+  std::vector<int> start;
+  int rc = batch_table(b, order, ents, start);
+  if (rc) return rc;
+  std::vector<BatchAdjEntry> adj(na);
+  for (int r = 0; r < na; ++r) adj[r] = all[order[r]];
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIPCHK(hipMemcpyAsync(b->d_xents, ents.data(), sizeof(BatchEntry) * na, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_xstart, start.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(b->d_aents, adj.data(), sizeof(BatchAdjEntry) * na, hipMemcpyHostToDevice, s));
+  // prologue of step hi-1 of every instance
+  hipLaunchKernelGGL(k_batch_adj<true>, dim3((unsigned)start[na]), dim3(kBX, kTY), 0, s, b->d_xents,
+                     b->d_aents, b->d_xstart, na, 0, 0LL);
+  int act = na;
+  const long long len0 = hi[order[0]] - lo[order[0]];
+  for (long long r = 0; r < len0; ++r) {
+    while (act > 0 && hi[order[act - 1]] - lo[order[act - 1]] <= r) --act;
+    const unsigned nb = (unsigned)start[act];
+    hipLaunchKernelGGL(k_batch_adj<false>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_xents, b->d_aents,
+                       b->d_xstart, act, 1, r);
+    hipLaunchKernelGGL(k_batch_adj<true>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_xents, b->d_aents,
+                       b->d_xstart, act, 1, r);
+  }
+  HIPCHK(hipGetLastError());
   return RFXH_OK;
 }
 

@@ -65,3 +65,24 @@ def test_named_sweep_reaches_the_later_box_and_the_pads():
     spec["material_sweep"] = (None, "eps_r", 9.0)
     _, cfg = o.build_uniform(spec, 10)
     assert np.all(cfg.eps_r[~vac] == np.float32(9.0)) and np.all(cfg.eps_r[vac] == 1.0)
+
+
+def test_batch_segment_steps():
+    """Segment length of a batched reverse run: all steps when the record
+    fits, else the largest S whose records and checkpoints fit, else
+    MemoryError."""
+    import math
+    from rfx_amd.adjoint import batch_segment_steps
+    T, per, st = [100, 70, 40], [10, 20, 30], [500, 400, 300]
+    assert batch_segment_steps(T, per, st, 100 * 60) == 100
+
+    def need(S):
+        return S * sum(per) + sum((math.ceil(t / S) - 1) * b for t, b in zip(T, st))
+
+    budget = 5000
+    S = batch_segment_steps(T, per, st, budget)
+    assert need(S) <= budget
+    assert all(need(s) > budget for s in range(S + 1, 101))
+    assert batch_segment_steps(T, per, st, budget, forced=7) == 7
+    with pytest.raises(MemoryError):
+        batch_segment_steps(T, per, st, 100)
