@@ -2379,10 +2379,11 @@ __global__ __launch_bounds__(256) void k_far_field_dirs(const double* __restrict
 // ---------------------------------------------------------------------------
 // Reverse mode.  The forward step is linear in the state (E, H, CPML memory)
 // for fixed materials, so its adjoint is the transpose of each sub-step run
-// in reverse order.  A reverse step is two gather kernels:
-//   k_adj_h: H-bar += (E update)^T E-bar + (CPML-E)^T g-bar_E, then the H-probe
+// in reverse order.  A reverse step is two gather kernels (k_adj_h, k_adj_e;
+// k_batch_adj<false / true> over the instances of a batch):
+//   H-bar: H-bar += (E update)^T E-bar + (CPML-E)^T g-bar_E, then the H-probe
 //            adjoint and the CPML-H memory adjoint (writes g-bar_H);
-//   k_adj_e: E-bar <- ca E-bar + (H update)^T H-bar + (CPML-H)^T g-bar_H, then
+//   E-bar:   E-bar <- ca E-bar + (H update)^T H-bar + (CPML-H)^T g-bar_H, then
 //            the prologue of the previous step: E-probe adjoint, source
 //            adjoint record, PEC projection, CPML-E memory adjoint (writes
 //            g-bar_E) and the eps_r gradient from the forward record.
@@ -2621,13 +2622,6 @@ __device__ __forceinline__ void adj_h_cell(const KParams& p, const AdjParams& a,
 }
 
 template <bool GRADED>
-__global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long n) {
-  int i, j, k;
-  if (!cell_of(p, i, j, k)) return;
-  adj_h_cell<GRADED>(p, a, i, j, k, n);
-}
-
-template <bool GRADED>
 __device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a, int i, int j,
                                            int k, int gather, long long m) {
   const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
@@ -2786,11 +2780,289 @@ __device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a,
   stf(a.a[2], c, ez);
 }
 
+// The single-instance reverse kernels: the bodies of adj_h_cell and
+// adj_e_cell written out in the kernels (the same arithmetic; through a
+// device function taking the by-value kernel arguments by reference the
+// compiler copies them into registers whole and spills, 4 % slower on a
+// 277 x 277 x 149 grid).  The batched kernels below call the functions.
+template <bool GRADED>
+__global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long n) {
+  int i, j, k;
+  if (!cell_of(p, i, j, k)) return;
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned P = (unsigned)p.plane;
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
+  float ix, iy, iz, ixn, iyn, izn;  // E-update spacings at the cell and at its forward neighbour
+  if (GRADED) {
+    ix = p.inv_e[0][i];
+    iy = p.inv_e[1][j];
+    iz = p.inv_e[2][k];
+    ixn = hi ? p.inv_e[0][i + 1] : 0.f;
+    iyn = hj ? p.inv_e[1][j + 1] : 0.f;
+    izn = hk ? p.inv_e[2][k + 1] : 0.f;
+  } else {
+    ix = iy = iz = p.inv_e0;
+    ixn = hi ? p.inv_e0 : 0.f;
+    iyn = hj ? p.inv_e0 : 0.f;
+    izn = hk ? p.inv_e0 : 0.f;
+  }
+  // v = cb * E-bar at the cell and its forward neighbours
+  const float cb0 = coef_at(p, c).y;
+  const float vx = cb0 * ldf(a.a[0], c), vy = cb0 * ldf(a.a[1], c), vz = cb0 * ldf(a.a[2], c);
+  const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
+  const float cbj = coef_at(p, cj).y, cbk = coef_at(p, ck).y, cbi = coef_at(p, ci).y;
+  const float vx_j = cbj * ldf(a.a[0], cj), vz_j = cbj * ldf(a.a[2], cj);
+  const float vx_k = cbk * ldf(a.a[0], ck), vy_k = cbk * ldf(a.a[1], ck);
+  const float vy_i = cbi * ldf(a.a[1], ci), vz_i = cbi * ldf(a.a[2], ci);
+  float hx = ldf(a.a[3], c), hy = ldf(a.a[4], c), hz = ldf(a.a[5], c);
+  // transpose of E += cb curl_bwd H (yee.py:379-404)
+  hz += vx * iy - vx_j * iyn - vy * ix + vy_i * ixn;
+  hy += -vx * iz + vx_k * izn + vz * ix - vz_i * ixn;
+  hx += vy * iz - vy_k * izn - vz * iy + vz_j * iyn;
+  // transpose of the CPML-E differences g = (H(x) - H(x - e)) / d
+  if (in_slab(p, i, j, k) || in_slab(p, hi ? i + 1 : i, hj ? j + 1 : j, hk ? k + 1 : k)) {
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      if (p.cn[face] == 0) continue;
+      const int ax = face >> 1;
+      const float inv = p.cinv[face];
+      const int l0 = slab_layer(p, face, i, j, k);
+      const bool nb = ax == 0 ? hi : ax == 1 ? hj : hk;
+      const int ii = i + (ax == 0), jj = j + (ax == 1), kk = k + (ax == 2);
+      const int l1 = nb ? slab_layer(p, face, ii, jj, kk) : -1;
+      float d0 = 0.f, d1 = 0.f;
+      if (l0 >= 0) {
+        const unsigned q = (unsigned)psi_idx(face, l0, i, j, k, p.cn[face], p.ny, p.nz);
+        d0 += ldf(a.ge[face][0], q);
+        d1 += ldf(a.ge[face][1], q);
+      }
+      if (l1 >= 0) {
+        const unsigned q = (unsigned)psi_idx(face, l1, ii, jj, kk, p.cn[face], p.ny, p.nz);
+        d0 -= ldf(a.ge[face][0], q);
+        d1 -= ldf(a.ge[face][1], q);
+      }
+      d0 *= inv;
+      d1 *= inv;
+      if (ax == 0) {
+        hz += d0;
+        hy += d1;
+      } else if (ax == 1) {
+        hz += d0;
+        hx += d1;
+      } else {
+        hy += d0;
+        hx += d1;
+      }
+    }
+  }
+  // H-probe adjoints of step n
+  const uint32_t fl = p.flags[ldu16(p.mat, c)];
+  if ((fl & RFXH_POINT) && a.probe_adj) {
+    int lo = 0, hi2 = p.n_ops;
+    while (lo < hi2) {
+      const int mid = (lo + hi2) >> 1;
+      if (p.ops[mid].cell < (long long)c) lo = mid + 1;
+      else hi2 = mid;
+    }
+    for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+      const rfxh_point_op op = p.ops[q];
+      if (op.kind == RFXH_OP_PROBE && op.comp >= 3) {
+        const float g = a.probe_adj[n * p.n_probe + op.slot];
+        if (op.comp == 3) hx += g;
+        else if (op.comp == 4) hy += g;
+        else hz += g;
+      }
+    }
+  }
+  // CPML-H memory adjoint (cpml.py:875-1043 transposed)
+  if (in_slab(p, i, j, k)) {
+    const float ch = coef_at(p, c).w;
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+      const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+      float s0, s1, h0, h1;  // signs and adjoints of the two H components the face corrects
+      if (ax == 0) {
+        s0 = 1.f; h0 = hy; s1 = -1.f; h1 = hz;
+      } else if (ax == 1) {
+        s0 = -1.f; h0 = hx; s1 = 1.f; h1 = hz;
+      } else {
+        s0 = 1.f; h0 = hx; s1 = -1.f; h1 = hy;
+      }
+      const float t0 = ldf(a.ph[face][0], q) + s0 * ch * h0;
+      const float t1 = ldf(a.ph[face][1], q) + s1 * ch * h1;
+      stf(a.gh[face][0], q, cc * t0 + s0 * ch * km1 * h0);
+      stf(a.gh[face][1], q, cc * t1 + s1 * ch * km1 * h1);
+      stf(a.ph[face][0], q, bb * t0);
+      stf(a.ph[face][1], q, bb * t1);
+    }
+  }
+  stf(a.a[3], c, hx);
+  stf(a.a[4], c, hy);
+  stf(a.a[5], c, hz);
+}
+
 template <bool GRADED>
 __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gather, long long m) {
   int i, j, k;
   if (!cell_of(p, i, j, k)) return;
-  adj_e_cell<GRADED>(p, a, i, j, k, gather, m);
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const unsigned P = (unsigned)p.plane;
+  const unsigned mid0 = ldu16(p.mat, c);
+  const float4 co = p.coef[mid0];
+  float ex = ldf(a.a[0], c), ey = ldf(a.a[1], c), ez = ldf(a.a[2], c);
+  if (gather) {
+    const bool hj = j > 0, hk = k > 0, hi = i > 0;
+    float ix, iy, iz, ixp, iyp, izp;  // H-update spacings at the cell and its backward neighbour
+    if (GRADED) {
+      ix = p.inv_h[0][i];
+      iy = p.inv_h[1][j];
+      iz = p.inv_h[2][k];
+      ixp = hi ? p.inv_h[0][i - 1] : 0.f;
+      iyp = hj ? p.inv_h[1][j - 1] : 0.f;
+      izp = hk ? p.inv_h[2][k - 1] : 0.f;
+    } else {
+      ix = iy = iz = p.inv_h0;
+      ixp = hi ? p.inv_h0 : 0.f;
+      iyp = hj ? p.inv_h0 : 0.f;
+      izp = hk ? p.inv_h0 : 0.f;
+    }
+    const float ch0 = co.w;
+    const float ux = ch0 * ldf(a.a[3], c), uy = ch0 * ldf(a.a[4], c), uz = ch0 * ldf(a.a[5], c);
+    const unsigned cj = hj ? c - p.nz : c, ck = hk ? c - 1 : c, ci = hi ? c - P : c;
+    const float chj = coef_at(p, cj).w, chk = coef_at(p, ck).w, chi = coef_at(p, ci).w;
+    const float ux_j = chj * ldf(a.a[3], cj), uz_j = chj * ldf(a.a[5], cj);
+    const float ux_k = chk * ldf(a.a[3], ck), uy_k = chk * ldf(a.a[4], ck);
+    const float uy_i = chi * ldf(a.a[4], ci), uz_i = chi * ldf(a.a[5], ci);
+    // transpose of H -= ch curl_fwd E (yee.py:272-289), plus ca E-bar
+    ex = co.x * ex + uy * iz - uy_k * izp - uz * iy + uz_j * iyp;
+    ey = co.x * ey - ux * iz + ux_k * izp + uz * ix - uz_i * ixp;
+    ez = co.x * ez + ux * iy - ux_j * iyp - uy * ix + uy_i * ixp;
+    // transpose of the CPML-H differences g = (E(x + e) - E(x)) / d
+    if (in_slab(p, i, j, k) || in_slab(p, hi ? i - 1 : i, hj ? j - 1 : j, hk ? k - 1 : k)) {
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        if (p.cn[face] == 0) continue;
+        const int ax = face >> 1;
+        const float inv = p.cinv[face];
+        const int l0 = slab_layer(p, face, i, j, k);
+        const bool nb = ax == 0 ? hi : ax == 1 ? hj : hk;
+        const int ii = i - (ax == 0), jj = j - (ax == 1), kk = k - (ax == 2);
+        const int l1 = nb ? slab_layer(p, face, ii, jj, kk) : -1;
+        float d0 = 0.f, d1 = 0.f;
+        if (l1 >= 0) {
+          const unsigned q = (unsigned)psi_idx(face, l1, ii, jj, kk, p.cn[face], p.ny, p.nz);
+          d0 += ldf(a.gh[face][0], q);
+          d1 += ldf(a.gh[face][1], q);
+        }
+        if (l0 >= 0) {
+          const unsigned q = (unsigned)psi_idx(face, l0, i, j, k, p.cn[face], p.ny, p.nz);
+          d0 -= ldf(a.gh[face][0], q);
+          d1 -= ldf(a.gh[face][1], q);
+        }
+        d0 *= inv;
+        d1 *= inv;
+        if (ax == 0) {
+          ez += d0;
+          ey += d1;
+        } else if (ax == 1) {
+          ez += d0;
+          ex += d1;
+        } else {
+          ey += d0;
+          ex += d1;
+        }
+      }
+    }
+  }
+  if (m >= 0) {
+    // prologue of step m: probes read E after injection, injection follows
+    // the PEC projection (rfx/simulation.py:1392-1567)
+    const uint32_t fl = p.flags[mid0];
+    if (fl & RFXH_POINT) {
+      int lo = 0, hi2 = p.n_ops;
+      while (lo < hi2) {
+        const int md = (lo + hi2) >> 1;
+        if (p.ops[md].cell < (long long)c) lo = md + 1;
+        else hi2 = md;
+      }
+      for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+        const rfxh_point_op op = p.ops[q];
+        if (op.kind == RFXH_OP_PROBE && op.comp < 3 && a.probe_adj) {
+          const float g = a.probe_adj[m * p.n_probe + op.slot];
+          if (op.comp == 0) ex += g;
+          else if (op.comp == 1) ey += g;
+          else ez += g;
+        }
+      }
+      for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+        const rfxh_point_op op = p.ops[q];
+        if (op.kind == RFXH_OP_INJECT && a.src_adj)
+          a.src_adj[m * p.n_src + op.slot] = pick3(op.comp, ex, ey, ez);
+      }
+    }
+    if (p.pec_axes & 1) {
+      if (i == 0 || i == p.nx - 1) ey = ez = 0.f;
+    }
+    if (p.pec_axes & 2) {
+      if (j == 0 || j == p.ny - 1) ex = ez = 0.f;
+    }
+    if (p.pec_axes & 4) {
+      if (k == 0 || k == p.nz - 1) ex = ey = 0.f;
+      if (k == p.nz - 1) ez = 0.f;
+    }
+    if (fl & RFXH_ZERO_EX) ex = 0.f;
+    if (fl & RFXH_ZERO_EY) ey = 0.f;
+    if (fl & RFXH_ZERO_EZ) ez = 0.f;
+    // CPML-E memory adjoint (cpml.py:608-777 transposed)
+    if (in_slab(p, i, j, k)) {
+      const float ce = co.z;
+#pragma unroll
+      for (int face = 0; face < 6; ++face) {
+        const int l = slab_layer(p, face, i, j, k);
+        if (l < 0) continue;
+        const int ax = face >> 1;
+        const unsigned q = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+        const float bb = p.pb[face][l], cc = p.pc[face][l], km1 = p.pk[face][l];
+        float s0, s1, e0v, e1v;
+        if (ax == 0) {
+          s0 = -1.f; e0v = ey; s1 = 1.f; e1v = ez;
+        } else if (ax == 1) {
+          s0 = 1.f; e0v = ex; s1 = -1.f; e1v = ez;
+        } else {
+          s0 = -1.f; e0v = ex; s1 = 1.f; e1v = ey;
+        }
+        const float t0 = ldf(a.pe[face][0], q) + s0 * ce * e0v;
+        const float t1 = ldf(a.pe[face][1], q) + s1 * ce * e1v;
+        stf(a.ge[face][0], q, cc * t0 + s0 * ce * km1 * e0v);
+        stf(a.ge[face][1], q, cc * t1 + s1 * ce * km1 * e1v);
+        stf(a.pe[face][0], q, bb * t0);
+        stf(a.pe[face][1], q, bb * t1);
+      }
+    }
+    // eps_r gradient from the forward record of step m
+    if (a.grad) {
+      const long long r = region_index(a.reg, i, j, k);
+      const long long row = m - a.rec_step0;
+      if (r >= 0 && row >= 0 && row < a.rec_rows) {
+        const float* rr = a.rec + row * 9 * a.reg.n + r;
+        const long long N = a.reg.n;
+        const float A = ex * rr[0] + ey * rr[N] + ez * rr[2 * N];
+        const float B = ex * rr[3 * N] + ey * rr[4 * N] + ez * rr[5 * N];
+        // the record holds the CPML-E rows only for slab cells
+        const float C = in_slab(p, i, j, k) ? ex * rr[6 * N] + ey * rr[7 * N] + ez * rr[8 * N]
+                                            : 0.f;
+        const float4 dc = a.dcoef[mid0];
+        a.grad[r] += dc.x * A + dc.y * B + dc.z * C;
+      }
+    }
+  }
+  stf(a.a[0], c, ex);
+  stf(a.a[1], c, ey);
+  stf(a.a[2], c, ez);
 }
 
 // ---------------------------------------------------------------------------
@@ -2836,8 +3108,10 @@ __global__ __launch_bounds__(kWG) void k_batch_e_rec(const BatchEntry* __restric
 // Reverse step r (counted from each instance's hi) of the active instances:
 // E = false the H-bar gather of step n = hi-1-r; E = true the E-bar gather
 // and the prologue of step n-1 (gather = 0: only the prologue of step
-// hi-1, before the first reverse step).
-template <bool E>
+// hi-1, before the first reverse step).  GM: 0 every instance uniform, 1
+// every instance graded, 2 mixed (one body per lane kind, chosen per
+// instance).
+template <bool E, int GM>
 __global__ __launch_bounds__(kWG) void k_batch_adj(const BatchEntry* __restrict__ ents,
                                                    const BatchAdjEntry* __restrict__ adj,
                                                    const int* __restrict__ start, int n_act,
@@ -2849,13 +3123,14 @@ __global__ __launch_bounds__(kWG) void k_batch_adj(const BatchEntry* __restrict_
   if (!cell_at(p, L - gld(start, q), i, j, k)) return;
   const BatchAdjEntry& e = adj[q];
   const long long n = e.hi - 1 - r;
+  const bool graded = GM == 2 ? p.graded != 0 : GM == 1;
   if (E) {
     const long long m = !gather ? n : n - 1 >= e.lo ? n - 1 : -1;
-    if (p.graded) adj_e_cell<true>(p, e.a, i, j, k, gather, m);
-    else adj_e_cell<false>(p, e.a, i, j, k, gather, m);
+    if (GM != 0 && graded) adj_e_cell<true>(p, e.a, i, j, k, gather, m);
+    else if (GM != 1) adj_e_cell<false>(p, e.a, i, j, k, gather, m);
   } else {
-    if (p.graded) adj_h_cell<true>(p, e.a, i, j, k, n);
-    else adj_h_cell<false>(p, e.a, i, j, k, n);
+    if (GM != 0 && graded) adj_h_cell<true>(p, e.a, i, j, k, n);
+    else if (GM != 1) adj_h_cell<false>(p, e.a, i, j, k, n);
   }
 }
 
@@ -2922,6 +3197,33 @@ struct rfxh_plan {
   float4* dcoef = nullptr;      // per material d(ca, cb, ce)/d eps_r (reverse mode)
   int n_mat = 0;
 };
+
+// One reverse-mode launch (E-bar pass when e) over the first n_act
+// instances of a table; gm as k_batch_adj's GM.
+static void launch_batch_adj(hipStream_t s, bool e, int gm, unsigned nb, const BatchEntry* ents,
+                             const BatchAdjEntry* adj, const int* start, int n_act, int gather,
+                             long long r) {
+  const dim3 g(nb), blk(kBX, kTY);
+  if (e) {
+    if (gm == 0) hipLaunchKernelGGL((k_batch_adj<true, 0>), g, blk, 0, s, ents, adj, start, n_act, gather, r);
+    else if (gm == 1) hipLaunchKernelGGL((k_batch_adj<true, 1>), g, blk, 0, s, ents, adj, start, n_act, gather, r);
+    else hipLaunchKernelGGL((k_batch_adj<true, 2>), g, blk, 0, s, ents, adj, start, n_act, gather, r);
+  } else {
+    if (gm == 0) hipLaunchKernelGGL((k_batch_adj<false, 0>), g, blk, 0, s, ents, adj, start, n_act, gather, r);
+    else if (gm == 1) hipLaunchKernelGGL((k_batch_adj<false, 1>), g, blk, 0, s, ents, adj, start, n_act, gather, r);
+    else hipLaunchKernelGGL((k_batch_adj<false, 2>), g, blk, 0, s, ents, adj, start, n_act, gather, r);
+  }
+}
+
+// Logical workgroups of one instance in a batched launch (its plan's
+// workgroup order, one cell per thread).
+static long long batch_blocks(const rfxh_plan* pl) {
+  const KParams& kq = pl->kp;
+  const long long flat = kq.plane, nxb = kq.nx;
+  return kq.order == 0 ? 8LL * nxb * (((flat + 7) / 8 + kWG - 1) / kWG)
+                       : nxb * ((flat + kWG - 1) / kWG);
+}
+
 
 namespace {
 
@@ -4219,11 +4521,7 @@ int batch_table(const rfxh_batch* b, const std::vector<int>& order, std::vector<
     ents[r].p = pl->kp;
     for (int c = 0; c < 6; ++c) ents[r].p.f[c] = pl->set[pl->cur][c];
     ents[r].ps = psi_io(pl, pl->cur);
-    const KParams& kq = pl->kp;
-    const long long flat = kq.plane, nxb = kq.nx;
-    const long long blk = kq.order == 0 ? 8LL * nxb * (((flat + 7) / 8 + kWG - 1) / kWG)
-                                        : nxb * ((flat + kWG - 1) / kWG);
-    const long long nb = (long long)start[r] + blk;
+    const long long nb = (long long)start[r] + batch_blocks(pl);
     if (nb > 0x7fffffffLL) return fail(RFXH_EINVAL, "batch too large for one launch");
     start[r + 1] = (int)nb;
   }
@@ -4503,18 +4801,19 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   HIPCHK(hipMemcpyAsync(b->d_xents, ents.data(), sizeof(BatchEntry) * na, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(b->d_xstart, start.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(b->d_aents, adj.data(), sizeof(BatchAdjEntry) * na, hipMemcpyHostToDevice, s));
+  int n_graded = 0;
+  for (int q : order) n_graded += b->plans[q]->kp.graded ? 1 : 0;
+  const int gm = n_graded == 0 ? 0 : n_graded == na ? 1 : 2;
   // prologue of step hi-1 of every instance
-  hipLaunchKernelGGL(k_batch_adj<true>, dim3((unsigned)start[na]), dim3(kBX, kTY), 0, s, b->d_xents,
-                     b->d_aents, b->d_xstart, na, 0, 0LL);
+  launch_batch_adj(s, true, gm, (unsigned)start[na], b->d_xents, b->d_aents, b->d_xstart, na, 0,
+                   0LL);
   int act = na;
   const long long len0 = hi[order[0]] - lo[order[0]];
   for (long long r = 0; r < len0; ++r) {
     while (act > 0 && hi[order[act - 1]] - lo[order[act - 1]] <= r) --act;
     const unsigned nb = (unsigned)start[act];
-    hipLaunchKernelGGL(k_batch_adj<false>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_xents, b->d_aents,
-                       b->d_xstart, act, 1, r);
-    hipLaunchKernelGGL(k_batch_adj<true>, dim3(nb), dim3(kBX, kTY), 0, s, b->d_xents, b->d_aents,
-                       b->d_xstart, act, 1, r);
+    launch_batch_adj(s, false, gm, nb, b->d_xents, b->d_aents, b->d_xstart, act, 1, r);
+    launch_batch_adj(s, true, gm, nb, b->d_xents, b->d_aents, b->d_xstart, act, 1, r);
   }
   HIPCHK(hipGetLastError());
   return RFXH_OK;
