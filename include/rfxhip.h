@@ -253,6 +253,11 @@ typedef struct {
   rfxh_region region;
   float* d_grad;             /* [region cells] dJ/d eps_r, accumulated (+=); NULL: no gradient */
   const float* dcoef;        /* host [n_mat][3]: d ca, d cb, d ce / d eps_r per material id */
+  /* dJ/d sigma on the same region (the gradient of sigma_override,
+   * rfx/api/_execute.py:2800-2805), from the same record: ca and cb depend
+   * on sigma, ce does not.  NULL: not computed (needs d_grad). */
+  float* d_grad_sigma;       /* [region cells], accumulated (+=) */
+  const float* dcoef_sigma;  /* host [n_mat][2]: d ca, d cb / d sigma per material id */
 } rfxh_adjoint_desc;
 
 /* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries

@@ -97,6 +97,7 @@ class AdjointDesc(C.Structure):
         ("d_probe_adj", _P), ("d_src_adj", _P),
         ("d_rec", _P), ("rec_step0", C.c_int64), ("rec_rows", C.c_int64),
         ("region", Region), ("d_grad", _P), ("dcoef", _FP),
+        ("d_grad_sigma", _P), ("dcoef_sigma", _FP),
     ]
 
 

@@ -86,12 +86,25 @@ def eps_derivative_table(mat, dt: float) -> np.ndarray:
     return np.ascontiguousarray(np.stack([dca, dcb, dce], axis=1).astype(np.float32))
 
 
+def sigma_derivative_table(mat, dt: float) -> np.ndarray:
+    """Per material id: d ca, d cb / d sigma (float64 -> float32); ce does not
+    depend on sigma.  With L = sigma dt/(2 eps): d ca/d sigma = -dt/(eps (1+L)^2),
+    d cb/d sigma = -cb dt/(2 eps (1+L))."""
+    eps_r, sig, _, _ = mat.props()
+    eps = eps_r.astype(np.float64) * EPS_0
+    L = sig.astype(np.float64) * dt / (2.0 * eps)
+    cb = (dt / eps) / (1.0 + L)
+    dca = -dt / (eps * (1.0 + L) ** 2)
+    dcb = -cb * dt / (2.0 * eps * (1.0 + L))
+    return np.ascontiguousarray(np.stack([dca, dcb], axis=1).astype(np.float32))
+
+
 class AdjointRunner:
     """Forward run with recording / checkpoints, then the reverse sweep."""
 
     def __init__(self, prob: Problem, n_steps: int, device, region=None,
                  checkpoint_segments: int | None = None,
-                 record_budget: int = DEFAULT_RECORD_BUDGET):
+                 record_budget: int = DEFAULT_RECORD_BUDGET, want_sigma: bool = False):
         if prob.ports:
             raise NotImplementedError("gradients through lumped-port V/I samples are not supported")
         self.prob = prob
@@ -118,6 +131,8 @@ class AdjointRunner:
         self.bounds = [(lo, min(self.n_steps, lo + self.seg_len))
                        for lo in range(0, self.n_steps, self.seg_len)]
         self.dcoef = eps_derivative_table(prob.mat, prob.dt)
+        self.dcoef_sigma = sigma_derivative_table(prob.mat, prob.dt) if want_sigma else None
+        self.grad_sigma = None
         self.rec = None
         self.checkpoints = []
 
@@ -143,8 +158,10 @@ class AdjointRunner:
 
     # -- reverse -------------------------------------------------------------------
     def backward(self, g_ts: torch.Tensor) -> torch.Tensor:
+        """dJ/d eps_r on the whole grid; with want_sigma also dJ/d sigma, left
+        in ``self.grad_sigma``."""
         st = _ReverseState(self.prob, self.dr, self.n_steps, g_ts, self.region, self.nreg,
-                           self.dcoef)
+                           self.dcoef, self.dcoef_sigma)
         d = st.desc
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
@@ -159,6 +176,8 @@ class AdjointRunner:
                 _lib.check(self.lib.rfxh_adjoint(self.dr.plan, C.byref(d), lo, hi,
                                                  self.dr.stream_ptr()))
         torch.cuda.synchronize(self.dr.device)
+        if self.dcoef_sigma is not None:
+            self.grad_sigma = st.full_gradient(self.box, sigma=True)
         return st.full_gradient(self.box)
 
 
@@ -168,7 +187,7 @@ class _ReverseState:
     region gradient) and the descriptor that points at them."""
 
     def __init__(self, prob: Problem, dr: DeviceRun, T: int, g_ts: torch.Tensor, region,
-                 nreg: int, dcoef: np.ndarray):
+                 nreg: int, dcoef: np.ndarray, dcoef_sigma: np.ndarray | None = None):
         dev = dr.device
         self.prob = prob
         self.T = T
@@ -194,14 +213,22 @@ class _ReverseState:
         d.region = region
         d.d_grad = self.grad.data_ptr()
         d.dcoef = self.dcoef.ctypes.data_as(C.POINTER(C.c_float))
+        self.dcoef_sigma = dcoef_sigma
+        self.grad_s = None
+        if dcoef_sigma is not None:
+            self.grad_s = torch.zeros(nreg, dtype=torch.float32, device=dev)
+            d.d_grad_sigma = self.grad_s.data_ptr()
+            d.dcoef_sigma = dcoef_sigma.ctypes.data_as(C.POINTER(C.c_float))
         self.desc = d
 
-    def full_gradient(self, box) -> torch.Tensor:
-        """dJ/d eps_r on the whole grid (zero outside the region box)."""
+    def full_gradient(self, box, sigma: bool = False) -> torch.Tensor:
+        """dJ/d eps_r (sigma: dJ/d sigma) on the whole grid (zero outside the
+        region box)."""
         prob = self.prob
         i0, i1, j0, j1, k0, k1 = box
-        full = torch.zeros(prob.shape, dtype=torch.float32, device=self.grad.device)
-        full[i0:i1, j0:j1, k0:k1] = self.grad.reshape(i1 - i0, j1 - j0, k1 - k0)
+        g = self.grad_s if sigma else self.grad
+        full = torch.zeros(prob.shape, dtype=torch.float32, device=g.device)
+        full[i0:i1, j0:j1, k0:k1] = g.reshape(i1 - i0, j1 - j0, k1 - k0)
         # uniform lane: source tables depend on eps_r through Cb at the
         # source cell (rfx/simulation.py:181-194, 275-284): w[n] = Cb * s[n].
         # The graded lane normalises sources with the materials before the
@@ -216,7 +243,8 @@ class _ReverseState:
                 eps = e * EPS_0
                 L = s * prob.dt / (2.0 * eps)
                 cb = (prob.dt / eps) / (1.0 + L)
-                dcb = -cb / (e * (1.0 + L))
+                # d cb / d eps_r, or d cb / d sigma
+                dcb = -cb * prob.dt / (2.0 * eps * (1.0 + L)) if sigma else -cb / (e * (1.0 + L))
                 contrib = dcb / cb * float(np.dot(sa[:, q], np.asarray(wave[:self.T], np.float64)))
                 full[tuple(idx)] += contrib
         return full
@@ -449,20 +477,29 @@ def batch_gradients(problems, n_steps, dj_dts, device, regions=None,
 
 class _ForwardFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, eps, runner):
+    def forward(ctx, eps, sigma, runner):
         ctx.runner = runner
-        ctx.eps_device = eps.device
-        ctx.eps_shape = eps.shape
+        ctx.shapes = (None if eps is None else (eps.shape, eps.device),
+                      None if sigma is None else (sigma.shape, sigma.device))
         return runner.forward()
 
     @staticmethod
     def backward(ctx, g):
         grad = ctx.runner.backward(g)
-        return grad.reshape(ctx.eps_shape).to(ctx.eps_device), None
+        out = []
+        for which, sh in zip((grad, ctx.runner.grad_sigma), ctx.shapes):
+            out.append(None if sh is None or which is None else which.reshape(sh[0]).to(sh[1]))
+        return out[0], out[1], None
 
 
-def forward_with_grad(sim, prob: Problem, grid, eps_tensor: torch.Tensor, n_steps: int,
-                      checkpoint_segments=None, grad_region=None):
-    runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments)
-    ts = _ForwardFn.apply(eps_tensor, runner)
+def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
+                      checkpoint_segments=None, grad_region=None, sigma_tensor=None):
+    """The time series as a function of eps_override and / or
+    sigma_override (tensors that require grad; either may be None)."""
+    want_sigma = isinstance(sigma_tensor, torch.Tensor) and sigma_tensor.requires_grad
+    runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments,
+                           want_sigma=want_sigma)
+    eps_in = eps_tensor if isinstance(eps_tensor, torch.Tensor) and eps_tensor.requires_grad \
+        else None
+    ts = _ForwardFn.apply(eps_in, sigma_tensor if want_sigma else None, runner)
     return ts, runner.dr

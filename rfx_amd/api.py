@@ -768,8 +768,9 @@ class Simulation:
                 skip_preflight: bool = False, grad_region=None, **unsupported):
         """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
         PEC on axes covered by CPML (_execute.py:1420-1421); the returned
-        time series carries gradients w.r.t. ``eps_override`` through the
-        hand-written adjoint.  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
+        time series carries gradients w.r.t. ``eps_override`` and
+        ``sigma_override`` through the hand-written adjoint (not
+        ``mu_r_override``: that raises).  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
         limits where dJ/d eps_r is evaluated (default: the whole grid);
         ``checkpoint_segments`` = K forces K recompute segments."""
         _reject(unsupported, "forward")
@@ -778,12 +779,15 @@ class Simulation:
             n_steps = grid.num_timesteps(num_periods)
         n_steps = int(n_steps)
         s11 = None if port_s11_freqs is None else np.asarray(port_s11_freqs, F32)
-        needs_grad = isinstance(eps_override, torch.Tensor) and eps_override.requires_grad
-        for nm, ov in (("sigma_override", sigma_override), ("mu_r_override", mu_r_override)):
-            if isinstance(ov, torch.Tensor) and ov.requires_grad:
-                raise NotImplementedError(
-                    f"gradients w.r.t. {nm} are not supported by the reverse mode (it "
-                    "differentiates eps_override); pass a tensor without requires_grad")
+        def _req(t):
+            return isinstance(t, torch.Tensor) and t.requires_grad
+
+        needs_grad = _req(eps_override) or _req(sigma_override)
+        if _req(mu_r_override):
+            raise NotImplementedError(
+                "gradients w.r.t. mu_r_override are not supported by the reverse mode (it "
+                "differentiates eps_override and sigma_override); pass a tensor without "
+                "requires_grad")
         eps_np = _to_np(eps_override)
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),
@@ -794,7 +798,8 @@ class Simulation:
         if needs_grad:
             from .adjoint import forward_with_grad
             ts, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
-                                       checkpoint_segments, grad_region)
+                                       checkpoint_segments, grad_region,
+                                       sigma_tensor=sigma_override)
         else:
             dr = DeviceRun(prob, n_steps, self._device)
             dr.run(0, n_steps)

@@ -2410,6 +2410,8 @@ struct AdjParams {
   RegionParams reg;
   float* grad;            // [region n] dJ/d eps_r
   const float4* dcoef;    // per material id: d ca, d cb, d ce / d eps_r
+  float* grad_s;          // [region n] dJ/d sigma (null: not computed)
+  const float2* dcoef_s;  // per material id: d ca, d cb / d sigma
 };
 
 __device__ __forceinline__ long long region_index(const RegionParams& r, int i, int j, int k) {
@@ -2772,6 +2774,10 @@ __device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a,
                                             : 0.f;
         const float4 dc = a.dcoef[mid0];
         a.grad[r] += dc.x * A + dc.y * B + dc.z * C;
+        if (a.grad_s) {
+          const float2 ds = a.dcoef_s[mid0];
+          a.grad_s[r] += ds.x * A + ds.y * B;
+        }
       }
     }
   }
@@ -3057,6 +3063,10 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
                                             : 0.f;
         const float4 dc = a.dcoef[mid0];
         a.grad[r] += dc.x * A + dc.y * B + dc.z * C;
+        if (a.grad_s) {
+          const float2 ds = a.dcoef_s[mid0];
+          a.grad_s[r] += ds.x * A + ds.y * B;
+        }
       }
     }
   }
@@ -3195,6 +3205,7 @@ struct rfxh_plan {
   long long t_cnt[3] = {0, 0, 0};
   std::vector<hipEvent_t> tev;
   float4* dcoef = nullptr;      // per material d(ca, cb, ce)/d eps_r (reverse mode)
+  float2* dcoef_s = nullptr;    // per material d(ca, cb)/d sigma (reverse mode)
   int n_mat = 0;
 };
 
@@ -4450,6 +4461,18 @@ static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo
     a.reg = region_params(&ad->region);
     a.grad = ad->d_grad;
     a.dcoef = pl->dcoef;
+    if (ad->d_grad_sigma) {
+      if (!ad->dcoef_sigma) return fail(RFXH_EINVAL, "sigma derivative table missing");
+      if (!pl->dcoef_s) {
+        std::vector<float2> tmp(pl->n_mat);
+        for (int m = 0; m < pl->n_mat; ++m)
+          tmp[m] = make_float2(ad->dcoef_sigma[2 * m], ad->dcoef_sigma[2 * m + 1]);
+        int rc = dev_copy(pl, tmp.data(), tmp.size(), &pl->dcoef_s);
+        if (rc) return rc;
+      }
+      a.grad_s = ad->d_grad_sigma;
+      a.dcoef_s = pl->dcoef_s;
+    }
   }
   return RFXH_OK;
 }

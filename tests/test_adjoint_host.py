@@ -44,7 +44,34 @@ def test_whole_grid_533_is_refused_below_the_sqrt_minimum():
     assert k <= k_opt + 1 and _need(T, k, per_step, state) <= 1.01 * floor
 
 
-@pytest.mark.parametrize("which", ["sigma_override", "mu_r_override"])
+def test_sigma_derivative_table_matches_finite_differences():
+    """d ca / d sigma and d cb / d sigma against float64 central differences
+    of ca = (1-L)/(1+L), cb = (dt/eps)/(1+L), L = sigma dt / (2 eps)."""
+    import numpy as np
+    from rfx_amd.adjoint import sigma_derivative_table
+    from rfx_amd.constants import EPS_0
+    from rfx_amd.materials import MaterialMap
+    m = MaterialMap((2, 2, 2))
+    for e, sg in ((1.0, 0.0), (4.0, 0.02), (11.7, 0.5), (2.2, 3.0)):
+        m.get_id(e, sg, 1.0)
+    dt = 1.9e-12
+    tab = sigma_derivative_table(m, dt).astype(np.float64)
+    eps_r, sig, _, _ = m.props()
+
+    def coef(e, s):
+        eps = e * EPS_0
+        L = s * dt / (2 * eps)
+        return (1 - L) / (1 + L), (dt / eps) / (1 + L)
+
+    for q in range(len(m.table)):
+        e, s = float(eps_r[q]), float(sig[q])
+        h = 1e-4 * max(s, 1.0)
+        (a1, b1), (a0, b0) = coef(e, s + h), coef(e, s - h)
+        np.testing.assert_allclose(tab[q], [(a1 - a0) / (2 * h), (b1 - b0) / (2 * h)],
+                                   rtol=1e-5)
+
+
+@pytest.mark.parametrize("which", ["mu_r_override"])
 def test_override_gradients_are_refused_not_dropped(which):
     import numpy as np
     import torch
