@@ -241,6 +241,12 @@ typedef struct {
  * d_rec[n][9][region] (row n = step step0 + n). */
 int rfxh_run_record(rfxh_plan* plan, int64_t step0, int64_t n_steps, const rfxh_region* region,
                     float* d_rec, void* stream);
+/* The same, also recording into d_rec_h[n][3][region] (device; NULL: not
+ * recorded) the H update's term R^n = -curl_fwd E^n + CPML-H term, with
+ * H^{n+1/2} = H^{n-1/2} + ch R^n, which dJ/d mu_r needs (the gradient of
+ * mu_r_override, rfx/api/_execute.py:2800-2805). */
+int rfxh_run_record_h(rfxh_plan* plan, int64_t step0, int64_t n_steps, const rfxh_region* region,
+                      float* d_rec, float* d_rec_h, void* stream);
 
 typedef struct {
   float* d_adj[6];           /* adjoint fields E-bar x,y,z, H-bar x,y,z; zeroed before the first call */
@@ -258,6 +264,11 @@ typedef struct {
    * on sigma, ce does not.  NULL: not computed (needs d_grad). */
   float* d_grad_sigma;       /* [region cells], accumulated (+=) */
   const float* dcoef_sigma;  /* host [n_mat][2]: d ca, d cb / d sigma per material id */
+  /* dJ/d mu_r on the same region from the H record of rfxh_run_record_h
+   * (same rows as d_rec).  NULL: not computed (needs d_grad). */
+  const float* d_rec_h;      /* [rec_rows][3][region cells] */
+  float* d_grad_mu;          /* [region cells], accumulated (+=) */
+  const float* dcoef_mu;     /* host [n_mat]: d ch / d mu_r per material id */
 } rfxh_adjoint_desc;
 
 /* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries

@@ -1184,7 +1184,9 @@ def build_uniform(spec, n_steps):
     [(lo, hi, name)], sources [(pos, comp, Pulse)], ports
     [(pos, comp, z0, Pulse|None, excite)], probes [(pos, comp)],
     dft_planes [(axis, coord, comp, freqs|None, n_freqs)], lane
-    ("run"|"forward"), eps_override (array|None), s11_freqs.
+    ("run"|"forward"), eps_override, sigma_override, mu_r_override (arrays
+    or None; forward lane: each replaces its array wholesale,
+    rfx/api/_execute.py:2800-2805), s11_freqs.
 
     Lane semantics: ``run`` applies outer-face PEC on every axis
     (rfx/runners/uniform.py:321 passes pec_axes=None -> all axes,
@@ -1208,6 +1210,8 @@ def build_uniform(spec, n_steps):
         eps_r = np.asarray(spec["eps_override"], F32).reshape(shape).copy()
     if lane == "forward" and spec.get("sigma_override") is not None:
         sigma = np.asarray(spec["sigma_override"], F32).reshape(shape).copy()
+    if lane == "forward" and spec.get("mu_r_override") is not None:
+        mu_r = np.asarray(spec["mu_r_override"], F32).reshape(shape).copy()
     dt = g.dt
     tms = times_f32(n_steps, dt)
     sources, ports_vi = [], []

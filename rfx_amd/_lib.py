@@ -98,6 +98,7 @@ class AdjointDesc(C.Structure):
         ("d_rec", _P), ("rec_step0", C.c_int64), ("rec_rows", C.c_int64),
         ("region", Region), ("d_grad", _P), ("dcoef", _FP),
         ("d_grad_sigma", _P), ("dcoef_sigma", _FP),
+        ("d_rec_h", _P), ("d_grad_mu", _P), ("dcoef_mu", _FP),
     ]
 
 
@@ -110,7 +111,8 @@ _lib = None
 
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
-           "rfxh_run_record", "rfxh_adjoint", "rfxh_batch_create", "rfxh_batch_destroy",
+           "rfxh_run_record", "rfxh_run_record_h", "rfxh_adjoint", "rfxh_batch_create",
+           "rfxh_batch_destroy",
            "rfxh_batch_run", "rfxh_batch_run_record", "rfxh_batch_adjoint", "rfxh_set_variant",
            "rfxh_get_variant", "rfxh_ntff_far_field",
            "rfxh_ntff_far_field_noalloc", "rfxh_assemble_fill", "rfxh_assemble_pec_pairs",
@@ -149,6 +151,8 @@ def load():
     lib.rfxh_kernel_time_ms.restype = C.c_int
     lib.rfxh_run_record.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(Region), _P, _P]
     lib.rfxh_run_record.restype = C.c_int
+    lib.rfxh_run_record_h.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(Region), _P, _P, _P]
+    lib.rfxh_run_record_h.restype = C.c_int
     lib.rfxh_adjoint.argtypes = [_P, C.POINTER(AdjointDesc), C.c_int64, C.c_int64, _P]
     lib.rfxh_adjoint.restype = C.c_int
     lib.rfxh_batch_create.argtypes = [C.POINTER(_P), C.c_int, C.POINTER(_P)]

@@ -99,12 +99,22 @@ def sigma_derivative_table(mat, dt: float) -> np.ndarray:
     return np.ascontiguousarray(np.stack([dca, dcb], axis=1).astype(np.float32))
 
 
+def mu_derivative_table(mat, dt: float) -> np.ndarray:
+    """Per material id: d ch / d mu_r with ch = dt / (mu_r mu0), i.e. -ch/mu_r
+    (float64 -> float32)."""
+    _, _, mu_r, _ = mat.props()
+    mu = mu_r.astype(np.float64)
+    ch = dt / (mu * MU_0)
+    return np.ascontiguousarray((-ch / mu).astype(np.float32))
+
+
 class AdjointRunner:
     """Forward run with recording / checkpoints, then the reverse sweep."""
 
     def __init__(self, prob: Problem, n_steps: int, device, region=None,
                  checkpoint_segments: int | None = None,
-                 record_budget: int = DEFAULT_RECORD_BUDGET, want_sigma: bool = False):
+                 record_budget: int = DEFAULT_RECORD_BUDGET, want_sigma: bool = False,
+                 want_mu: bool = False):
         if prob.ports:
             raise NotImplementedError("gradients through lumped-port V/I samples are not supported")
         self.prob = prob
@@ -120,7 +130,7 @@ class AdjointRunner:
         self.box = (i0, i1, j0, j1, k0, k1)
         self.region = _lib.Region(i0, j0, k0, i1 - i0, j1 - j0, k1 - k0)
         self.nreg = (i1 - i0) * (j1 - j0) * (k1 - k0)
-        per_step = 9 * 4 * self.nreg
+        per_step = (12 if want_mu else 9) * 4 * self.nreg
         state = 4 * (6 * int(np.prod(shape)) + int(self.dr.psis[0].numel()))
         budget = int(record_budget)
         if self.dr.device.type == "cuda":
@@ -133,6 +143,9 @@ class AdjointRunner:
         self.dcoef = eps_derivative_table(prob.mat, prob.dt)
         self.dcoef_sigma = sigma_derivative_table(prob.mat, prob.dt) if want_sigma else None
         self.grad_sigma = None
+        self.dcoef_mu = mu_derivative_table(prob.mat, prob.dt) if want_mu else None
+        self.grad_mu = None
+        self.rec_h = None
         self.rec = None
         self.checkpoints = []
 
@@ -142,9 +155,13 @@ class AdjointRunner:
         if self.rec is None or self.rec.shape[0] < rows:
             self.rec = torch.empty((rows, 9, self.nreg), dtype=torch.float32,
                                    device=self.dr.device)
+            if self.dcoef_mu is not None:
+                self.rec_h = torch.empty((rows, 3, self.nreg), dtype=torch.float32,
+                                         device=self.dr.device)
+        rec_h = self.rec_h.data_ptr() if self.rec_h is not None else None
         with torch.cuda.device(self.dr.device):
-            _lib.check(self.lib.rfxh_run_record(
-                self.dr.plan, lo, hi - lo, C.byref(self.region), self.rec.data_ptr(),
+            _lib.check(self.lib.rfxh_run_record_h(
+                self.dr.plan, lo, hi - lo, C.byref(self.region), self.rec.data_ptr(), rec_h,
                 self.dr.stream_ptr()))
 
     def forward(self) -> torch.Tensor:
@@ -161,7 +178,7 @@ class AdjointRunner:
         """dJ/d eps_r on the whole grid; with want_sigma also dJ/d sigma, left
         in ``self.grad_sigma``."""
         st = _ReverseState(self.prob, self.dr, self.n_steps, g_ts, self.region, self.nreg,
-                           self.dcoef, self.dcoef_sigma)
+                           self.dcoef, self.dcoef_sigma, self.dcoef_mu)
         d = st.desc
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
@@ -170,6 +187,8 @@ class AdjointRunner:
                 self.dr.load_state(fields, psi)
                 self._record(lo, hi)
             d.d_rec = self.rec.data_ptr()
+            if self.rec_h is not None:
+                d.d_rec_h = self.rec_h.data_ptr()
             d.rec_step0 = lo
             d.rec_rows = hi - lo
             with torch.cuda.device(self.dr.device):
@@ -178,6 +197,8 @@ class AdjointRunner:
         torch.cuda.synchronize(self.dr.device)
         if self.dcoef_sigma is not None:
             self.grad_sigma = st.full_gradient(self.box, sigma=True)
+        if self.dcoef_mu is not None:
+            self.grad_mu = st.full_gradient(self.box, mu=True)
         return st.full_gradient(self.box)
 
 
@@ -187,7 +208,8 @@ class _ReverseState:
     region gradient) and the descriptor that points at them."""
 
     def __init__(self, prob: Problem, dr: DeviceRun, T: int, g_ts: torch.Tensor, region,
-                 nreg: int, dcoef: np.ndarray, dcoef_sigma: np.ndarray | None = None):
+                 nreg: int, dcoef: np.ndarray, dcoef_sigma: np.ndarray | None = None,
+                 dcoef_mu: np.ndarray | None = None):
         dev = dr.device
         self.prob = prob
         self.T = T
@@ -219,21 +241,27 @@ class _ReverseState:
             self.grad_s = torch.zeros(nreg, dtype=torch.float32, device=dev)
             d.d_grad_sigma = self.grad_s.data_ptr()
             d.dcoef_sigma = dcoef_sigma.ctypes.data_as(C.POINTER(C.c_float))
+        self.dcoef_mu = dcoef_mu
+        self.grad_m = None
+        if dcoef_mu is not None:
+            self.grad_m = torch.zeros(nreg, dtype=torch.float32, device=dev)
+            d.d_grad_mu = self.grad_m.data_ptr()
+            d.dcoef_mu = dcoef_mu.ctypes.data_as(C.POINTER(C.c_float))
         self.desc = d
 
-    def full_gradient(self, box, sigma: bool = False) -> torch.Tensor:
-        """dJ/d eps_r (sigma: dJ/d sigma) on the whole grid (zero outside the
-        region box)."""
+    def full_gradient(self, box, sigma: bool = False, mu: bool = False) -> torch.Tensor:
+        """dJ/d eps_r (sigma: dJ/d sigma, mu: dJ/d mu_r) on the whole grid
+        (zero outside the region box)."""
         prob = self.prob
         i0, i1, j0, j1, k0, k1 = box
-        g = self.grad_s if sigma else self.grad
+        g = self.grad_m if mu else self.grad_s if sigma else self.grad
         full = torch.zeros(prob.shape, dtype=torch.float32, device=g.device)
         full[i0:i1, j0:j1, k0:k1] = g.reshape(i1 - i0, j1 - j0, k1 - k0)
         # uniform lane: source tables depend on eps_r through Cb at the
         # source cell (rfx/simulation.py:181-194, 275-284): w[n] = Cb * s[n].
         # The graded lane normalises sources with the materials before the
         # override (rfx/runners/nonuniform.py:630-643): no dependence.
-        if self.n_src and not prob.graded:
+        if self.n_src and not prob.graded and not mu:    # sources do not depend on mu_r
             sa = self.src_adj[:, :self.n_src].double().cpu().numpy()
             eps_r, sig, _, _ = prob.mat.props()
             for q, (idx, comp, wave) in enumerate(prob.sources):
@@ -477,29 +505,29 @@ def batch_gradients(problems, n_steps, dj_dts, device, regions=None,
 
 class _ForwardFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, eps, sigma, runner):
+    def forward(ctx, eps, sigma, mu, runner):
         ctx.runner = runner
-        ctx.shapes = (None if eps is None else (eps.shape, eps.device),
-                      None if sigma is None else (sigma.shape, sigma.device))
+        ctx.shapes = tuple(None if t is None else (t.shape, t.device) for t in (eps, sigma, mu))
         return runner.forward()
 
     @staticmethod
     def backward(ctx, g):
-        grad = ctx.runner.backward(g)
-        out = []
-        for which, sh in zip((grad, ctx.runner.grad_sigma), ctx.shapes):
-            out.append(None if sh is None or which is None else which.reshape(sh[0]).to(sh[1]))
-        return out[0], out[1], None
+        grads = (ctx.runner.backward(g), ctx.runner.grad_sigma, ctx.runner.grad_mu)
+        out = [None if sh is None or w is None else w.reshape(sh[0]).to(sh[1])
+               for w, sh in zip(grads, ctx.shapes)]
+        return out[0], out[1], out[2], None
 
 
 def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
-                      checkpoint_segments=None, grad_region=None, sigma_tensor=None):
-    """The time series as a function of eps_override and / or
-    sigma_override (tensors that require grad; either may be None)."""
-    want_sigma = isinstance(sigma_tensor, torch.Tensor) and sigma_tensor.requires_grad
+                      checkpoint_segments=None, grad_region=None, sigma_tensor=None,
+                      mu_tensor=None):
+    """The time series as a function of eps_override, sigma_override and
+    mu_r_override (the tensors among them that require grad)."""
+    def req(t):
+        return t if isinstance(t, torch.Tensor) and t.requires_grad else None
+
+    eps_in, sig_in, mu_in = req(eps_tensor), req(sigma_tensor), req(mu_tensor)
     runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments,
-                           want_sigma=want_sigma)
-    eps_in = eps_tensor if isinstance(eps_tensor, torch.Tensor) and eps_tensor.requires_grad \
-        else None
-    ts = _ForwardFn.apply(eps_in, sigma_tensor if want_sigma else None, runner)
+                           want_sigma=sig_in is not None, want_mu=mu_in is not None)
+    ts = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
     return ts, runner.dr

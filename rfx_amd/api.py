@@ -768,9 +768,9 @@ class Simulation:
                 skip_preflight: bool = False, grad_region=None, **unsupported):
         """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
         PEC on axes covered by CPML (_execute.py:1420-1421); the returned
-        time series carries gradients w.r.t. ``eps_override`` and
-        ``sigma_override`` through the hand-written adjoint (not
-        ``mu_r_override``: that raises).  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
+        time series carries gradients w.r.t. ``eps_override``,
+        ``sigma_override`` and ``mu_r_override`` through the hand-written
+        adjoint.  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
         limits where dJ/d eps_r is evaluated (default: the whole grid);
         ``checkpoint_segments`` = K forces K recompute segments."""
         _reject(unsupported, "forward")
@@ -782,12 +782,7 @@ class Simulation:
         def _req(t):
             return isinstance(t, torch.Tensor) and t.requires_grad
 
-        needs_grad = _req(eps_override) or _req(sigma_override)
-        if _req(mu_r_override):
-            raise NotImplementedError(
-                "gradients w.r.t. mu_r_override are not supported by the reverse mode (it "
-                "differentiates eps_override and sigma_override); pass a tensor without "
-                "requires_grad")
+        needs_grad = _req(eps_override) or _req(sigma_override) or _req(mu_r_override)
         eps_np = _to_np(eps_override)
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),
@@ -799,7 +794,7 @@ class Simulation:
             from .adjoint import forward_with_grad
             ts, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
                                        checkpoint_segments, grad_region,
-                                       sigma_tensor=sigma_override)
+                                       sigma_tensor=sigma_override, mu_tensor=mu_r_override)
         else:
             dr = DeviceRun(prob, n_steps, self._device)
             dr.run(0, n_steps)

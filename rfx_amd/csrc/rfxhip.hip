@@ -2412,6 +2412,9 @@ struct AdjParams {
   const float4* dcoef;    // per material id: d ca, d cb, d ce / d eps_r
   float* grad_s;          // [region n] dJ/d sigma (null: not computed)
   const float2* dcoef_s;  // per material id: d ca, d cb / d sigma
+  const float* rec_h;     // [rec_rows][3][region n]: the H update's R^n
+  float* grad_mu;         // [region n] dJ/d mu_r (null: not computed)
+  const float* dmu;       // per material id: d ch / d mu_r
 };
 
 __device__ __forceinline__ long long region_index(const RegionParams& r, int i, int j, int k) {
@@ -2421,6 +2424,69 @@ __device__ __forceinline__ long long region_index(const RegionParams& r, int i, 
 }
 
 // the record of region cell q (row-major in the region box) into rec
+// R = -curl_fwd E^n + CPML-H term at a cell, with H^{n+1/2} = H^{n-1/2} +
+// ch R (h_core's update; the CPML memory already holds its new values);
+// stored at rec_h[v * r.n + q].
+template <bool GRADED>
+__device__ __forceinline__ void record_h_cell(const KParams& p, const PsiIO& ps,
+                                              const RegionParams& r, long long q, int i, int j,
+                                              int k, float* rec_h) {
+  const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+  const bool hj = j + 1 < p.ny, hk = k + 1 < p.nz, hi = i + 1 < p.nx;
+  const float ex = ldf(p.f[0], c), ey = ldf(p.f[1], c), ez = ldf(p.f[2], c);
+  const float ex_j = hj ? ldf(p.f[0], c + p.nz) : 0.f, ez_j = hj ? ldf(p.f[2], c + p.nz) : 0.f;
+  const float ex_k = hk ? ldf(p.f[0], c + 1) : 0.f, ey_k = hk ? ldf(p.f[1], c + 1) : 0.f;
+  const float ey_i = hi ? ldf(p.f[1], c + (unsigned)p.plane) : 0.f;
+  const float ez_i = hi ? ldf(p.f[2], c + (unsigned)p.plane) : 0.f;
+  float ix, iy, iz;
+  if (GRADED) {
+    ix = p.inv_h[0][i];
+    iy = p.inv_h[1][j];
+    iz = p.inv_h[2][k];
+  } else {
+    ix = iy = iz = p.inv_h0;
+  }
+  float rx = -((ez_j - ez) * iy - (ey_k - ey) * iz);
+  float ry = -((ex_k - ex) * iz - (ez_i - ez) * ix);
+  float rz = -((ey_i - ey) * ix - (ex_j - ex) * iy);
+  if (in_slab(p, i, j, k)) {
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      const int l = slab_layer(p, face, i, j, k);
+      if (l < 0) continue;
+      const int ax = face >> 1;
+      const unsigned pq = (unsigned)psi_idx(face, l, i, j, k, p.cn[face], p.ny, p.nz);
+      const float km1 = p.pk[face][l], inv = p.cinv[face];
+      float g0, g1;
+      if (ax == 0) {
+        g0 = (ez_i - ez) * inv;
+        g1 = (ey_i - ey) * inv;
+      } else if (ax == 1) {
+        g0 = (ez_j - ez) * inv;
+        g1 = (ex_j - ex) * inv;
+      } else {
+        g0 = (ey_k - ey) * inv;
+        g1 = (ex_k - ex) * inv;
+      }
+      const float d0 = ldf(ps.hr[face][0], pq) + km1 * g0;
+      const float d1 = ldf(ps.hr[face][1], pq) + km1 * g1;
+      if (ax == 0) {
+        ry += d0;
+        rz -= d1;
+      } else if (ax == 1) {
+        rx -= d0;
+        rz += d1;
+      } else {
+        rx += d0;
+        ry -= d1;
+      }
+    }
+  }
+  rec_h[q] = rx;
+  rec_h[r.n + q] = ry;
+  rec_h[2 * r.n + q] = rz;
+}
+
 template <bool GRADED>
 __device__ __forceinline__ void record_cell(const KParams& p, const PsiIO& ps,
                                             const RegionParams& r, long long q, float* rec) {
@@ -2493,10 +2559,17 @@ __device__ __forceinline__ void record_cell(const KParams& p, const PsiIO& ps,
 }
 
 template <bool GRADED>
-__global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParams r, float* rec) {
+__global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParams r, float* rec,
+                                                float* rec_h) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= r.n) return;
   record_cell<GRADED>(p, ps, r, q, rec);
+  if (rec_h) {
+    const int c3 = (int)(q % r.nk);
+    const long long t = q / r.nk;
+    record_h_cell<GRADED>(p, ps, r, q, r.i0 + (int)(t / r.nj), r.j0 + (int)(t % r.nj), r.k0 + c3,
+                          rec_h);
+  }
 }
 
 // ch or cb of the cell's material (w = ch, y = cb)
@@ -2616,6 +2689,16 @@ __device__ __forceinline__ void adj_h_cell(const KParams& p, const AdjParams& a,
       stf(a.gh[face][1], q, cc * t1 + s1 * ch * km1 * h1);
       stf(a.ph[face][0], q, bb * t0);
       stf(a.ph[face][1], q, bb * t1);
+    }
+  }
+  // mu_r gradient: dJ/dH^{n+1/2} . R^n times d ch / d mu_r
+  if (a.grad_mu) {
+    const long long r = region_index(a.reg, i, j, k);
+    const long long row = n - a.rec_step0;
+    if (r >= 0 && row >= 0 && row < a.rec_rows) {
+      const long long N = a.reg.n;
+      const float* rr = a.rec_h + row * 3 * N + r;
+      a.grad_mu[r] += a.dmu[ldu16(p.mat, c)] * (hx * rr[0] + hy * rr[N] + hz * rr[2 * N]);
     }
   }
   stf(a.a[3], c, hx);
@@ -2904,6 +2987,16 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
       stf(a.gh[face][1], q, cc * t1 + s1 * ch * km1 * h1);
       stf(a.ph[face][0], q, bb * t0);
       stf(a.ph[face][1], q, bb * t1);
+    }
+  }
+  // mu_r gradient: dJ/dH^{n+1/2} . R^n times d ch / d mu_r
+  if (a.grad_mu) {
+    const long long r = region_index(a.reg, i, j, k);
+    const long long row = n - a.rec_step0;
+    if (r >= 0 && row >= 0 && row < a.rec_rows) {
+      const long long N = a.reg.n;
+      const float* rr = a.rec_h + row * 3 * N + r;
+      a.grad_mu[r] += a.dmu[ldu16(p.mat, c)] * (hx * rr[0] + hy * rr[N] + hz * rr[2 * N]);
     }
   }
   stf(a.a[3], c, hx);
@@ -3206,6 +3299,7 @@ struct rfxh_plan {
   std::vector<hipEvent_t> tev;
   float4* dcoef = nullptr;      // per material d(ca, cb, ce)/d eps_r (reverse mode)
   float2* dcoef_s = nullptr;    // per material d(ca, cb)/d sigma (reverse mode)
+  float* dmu = nullptr;         // per material d ch / d mu_r (reverse mode)
   int n_mat = 0;
 };
 
@@ -4364,6 +4458,11 @@ int rfxh_get_variant(rfxh_plan* pl, int* kernel, int* param) {
 
 int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
                     float* d_rec, void* stream) {
+  return rfxh_run_record_h(pl, step0, n_steps, region, d_rec, nullptr, stream);
+}
+
+int rfxh_run_record_h(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
+                      float* d_rec, float* d_rec_h, void* stream) {
   if (!pl) return fail(RFXH_EINVAL, "null plan");
   if (pl->n_pole > 0 || pl->aniso)
     return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode covers "
@@ -4386,9 +4485,11 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
   for (int64_t n = 0; n < n_steps; ++n) {
     launch_pass(pl, s, kp, ps, 0, pl->cur);
     if (kp.graded)
-      hipLaunchKernelGGL(k_record<true>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
+      hipLaunchKernelGGL(k_record<true>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n,
+                         d_rec_h ? d_rec_h + n * 3 * rp.n : nullptr);
     else
-      hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n);
+      hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n,
+                         d_rec_h ? d_rec_h + n * 3 * rp.n : nullptr);
     launch_pass(pl, s, kp, ps, 1, pl->cur);
     launch_accumulators(pl, s, pl->cur);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
@@ -4472,6 +4573,16 @@ static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo
       }
       a.grad_s = ad->d_grad_sigma;
       a.dcoef_s = pl->dcoef_s;
+    }
+    if (ad->d_grad_mu) {
+      if (!ad->dcoef_mu || !ad->d_rec_h) return fail(RFXH_EINVAL, "mu_r gradient: H record or table missing");
+      if (!pl->dmu) {
+        int rc = dev_copy(pl, ad->dcoef_mu, (size_t)pl->n_mat, &pl->dmu);
+        if (rc) return rc;
+      }
+      a.rec_h = ad->d_rec_h;
+      a.grad_mu = ad->d_grad_mu;
+      a.dmu = pl->dmu;
     }
   }
   return RFXH_OK;
@@ -4803,6 +4914,8 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   std::vector<BatchAdjEntry> all(n);
   for (int q = 0; q < n; ++q) {
     if (!descs[q]) continue;
+    if (descs[q]->d_grad_mu)
+      return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients only");
     const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);
     if (rc) return rc;
     all[q].lo = lo[q];

@@ -211,6 +211,63 @@ def test_sigma_gradient_directions_vs_oracle_fd(torch_mod):
         assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
 
 
+def test_mu_gradient_directions_vs_oracle_fd(torch_mod):
+    """dJ/d mu_r_override on the mixed case (a magnetic box, CPML), region
+    across the CPML, from a sweep that also takes dJ/d eps_r and dJ/d sigma:
+    directional derivatives vs float64 oracle central differences; the eps
+    gradient equals the eps-only sweep's."""
+    torch = torch_mod
+    _say("mixed-case mu_r directional FD")
+    spec = cases.mixed_case("forward")
+    spec["ports"] = [p for p in spec["ports"] if p[2] == 0.0]
+    spec["s11_freqs"] = None
+    n = 160
+    sim = cases.to_sim(spec)
+    g0, cfg0 = o.build_uniform(spec, n)
+    base_e, base_s, base_m = cfg0.eps_r.copy(), cfg0.sigma.copy(), cfg0.mu_r.copy()
+    region = (2, 14, 3, 17, 2, 10)
+
+    def J_of(ts):
+        ts = ts.double()
+        wts = torch.linspace(0.5, 1.5, ts.shape[1], dtype=torch.float64, device=ts.device)
+        return (ts ** 2 * wts).sum()
+
+    eps = torch.tensor(base_e, requires_grad=True)
+    sig = torch.tensor(base_s, requires_grad=True)
+    mu = torch.tensor(base_m, requires_grad=True)
+    res = sim.forward(eps_override=eps, sigma_override=sig, mu_r_override=mu, n_steps=n,
+                      grad_region=region)
+    J_of(res.time_series).backward()
+    grad_m = mu.grad.numpy().astype(np.float64)
+    eps2 = torch.tensor(base_e, requires_grad=True)
+    res2 = sim.forward(eps_override=eps2, sigma_override=torch.tensor(base_s),
+                       mu_r_override=torch.tensor(base_m), n_steps=n, grad_region=region)
+    J_of(res2.time_series).backward()
+    np.testing.assert_allclose(eps.grad.numpy(), eps2.grad.numpy(), rtol=1e-6,
+                               atol=1e-7 * float(np.abs(eps2.grad.numpy()).max()))
+    assert np.abs(grad_m).max() > 0
+    rng = np.random.default_rng(6)
+    i0, i1, j0, j1, k0, k1 = region
+    for trial in range(3):
+        d = np.zeros_like(base_m, dtype=np.float64)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        h = 1e-3
+        Js = []
+        for sgn in (1, -1):
+            sp = dict(spec)
+            sp["eps_override"] = base_e
+            sp["sigma_override"] = base_s
+            sp["mu_r_override"] = (base_m + sgn * h * d).astype(np.float32)
+            gg, cfg = o.build_uniform(sp, n)
+            out = cstep.run_c(cfg, n, want_fields=False)
+            Js.append(float((out.time_series ** 2 * np.linspace(0.5, 1.5, out.time_series.shape[1])).sum()))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float((grad_m * d).sum())
+        scale = float(np.abs(grad_m * d).sum())
+        print("mu direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
+
+
 def test_segmented_gradient_equals_single(torch_mod):
     torch = torch_mod
     _say("checkpoint segments == single segment")

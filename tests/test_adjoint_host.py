@@ -71,15 +71,19 @@ def test_sigma_derivative_table_matches_finite_differences():
                                    rtol=1e-5)
 
 
-@pytest.mark.parametrize("which", ["mu_r_override"])
-def test_override_gradients_are_refused_not_dropped(which):
+def test_mu_derivative_table_matches_finite_differences():
+    """d ch / d mu_r against a float64 central difference of ch = dt/(mu_r mu0)."""
     import numpy as np
-    import torch
-    import rfx_amd as R
-    sim = R.Simulation(10e9, (0.01, 0.01, 0.01), dx=1e-3, cpml_layers=4)
-    sim.add_source((0.005, 0.005, 0.005), "ez")
-    sim.add_probe((0.006, 0.005, 0.005), "ez")
-    shape = sim._build_grid().shape
-    t = torch.ones(shape, dtype=torch.float32, requires_grad=True)
-    with pytest.raises(NotImplementedError, match=which):
-        sim.forward(n_steps=4, **{which: t})
+    from rfx_amd.adjoint import mu_derivative_table
+    from rfx_amd.constants import MU_0
+    from rfx_amd.materials import MaterialMap
+    m = MaterialMap((2, 2, 2))
+    for mu in (1.0, 2.0, 4.5, 100.0):
+        m.get_id(2.0, 0.0, mu)
+    dt = 1.9e-12
+    tab = mu_derivative_table(m, dt).astype(np.float64)
+    _, _, mu_r, _ = m.props()
+    for q in range(len(m.table)):
+        mu, h = float(mu_r[q]), 1e-5 * float(mu_r[q])
+        fd = (dt / ((mu + h) * MU_0) - dt / ((mu - h) * MU_0)) / (2 * h)
+        np.testing.assert_allclose(tab[q], fd, rtol=1e-5)
