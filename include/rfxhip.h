@@ -269,6 +269,10 @@ typedef struct {
   const float* d_rec_h;      /* [rec_rows][3][region cells] */
   float* d_grad_mu;          /* [region cells], accumulated (+=) */
   const float* dcoef_mu;     /* host [n_mat]: d ch / d mu_r per material id */
+  /* dJ/d(port samples) [max_steps][n_port][2] = dJ/dV, dJ/dI (device;
+   * NULL: none): objectives on lumped-port S-parameters (rfx/probes/
+   * probes.py:1240-1274 differentiated through the scan). */
+  const float* d_port_adj;
 } rfxh_adjoint_desc;
 
 /* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries

@@ -115,8 +115,6 @@ class AdjointRunner:
                  checkpoint_segments: int | None = None,
                  record_budget: int = DEFAULT_RECORD_BUDGET, want_sigma: bool = False,
                  want_mu: bool = False):
-        if prob.ports:
-            raise NotImplementedError("gradients through lumped-port V/I samples are not supported")
         self.prob = prob
         self.n_steps = int(n_steps)
         self.dr = DeviceRun(prob, n_steps, device, two_sets=False)
@@ -173,12 +171,17 @@ class AdjointRunner:
                 self.dr.run(lo, hi - lo, use_graph=True)
         return self.dr.time_series().clone()
 
+    def port_samples(self) -> torch.Tensor:
+        """The lumped ports' (V, I) samples of the forward run, (T, n_port, 2)."""
+        return self.dr.port_samples()[:self.n_steps].clone()
+
     # -- reverse -------------------------------------------------------------------
-    def backward(self, g_ts: torch.Tensor) -> torch.Tensor:
-        """dJ/d eps_r on the whole grid; with want_sigma also dJ/d sigma, left
-        in ``self.grad_sigma``."""
+    def backward(self, g_ts: torch.Tensor, g_ports: torch.Tensor | None = None) -> torch.Tensor:
+        """dJ/d eps_r on the whole grid from dJ/d(time series) and, with lumped
+        ports, dJ/d(port samples) (T, n_port, 2); with want_sigma / want_mu
+        also dJ/d sigma / dJ/d mu_r, left in ``grad_sigma`` / ``grad_mu``."""
         st = _ReverseState(self.prob, self.dr, self.n_steps, g_ts, self.region, self.nreg,
-                           self.dcoef, self.dcoef_sigma, self.dcoef_mu)
+                           self.dcoef, self.dcoef_sigma, self.dcoef_mu, g_ports)
         d = st.desc
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
@@ -209,7 +212,7 @@ class _ReverseState:
 
     def __init__(self, prob: Problem, dr: DeviceRun, T: int, g_ts: torch.Tensor, region,
                  nreg: int, dcoef: np.ndarray, dcoef_sigma: np.ndarray | None = None,
-                 dcoef_mu: np.ndarray | None = None):
+                 dcoef_mu: np.ndarray | None = None, g_ports: torch.Tensor | None = None):
         dev = dr.device
         self.prob = prob
         self.T = T
@@ -247,6 +250,15 @@ class _ReverseState:
             self.grad_m = torch.zeros(nreg, dtype=torch.float32, device=dev)
             d.d_grad_mu = self.grad_m.data_ptr()
             d.dcoef_mu = dcoef_mu.ctypes.data_as(C.POINTER(C.c_float))
+        self.port_adj = None
+        n_port = len(prob.ports)
+        if n_port and g_ports is not None:
+            # the plan's port table has max_steps rows: pad past T with zeros
+            pa = torch.zeros((dr.port_out.shape[0], dr.port_out.shape[1], 2), dtype=torch.float32,
+                             device=dev)
+            pa[:T, :n_port] = g_ports.to(dev, torch.float32).reshape(T, n_port, 2)
+            self.port_adj = pa
+            d.d_port_adj = pa.data_ptr()
         self.desc = d
 
     def full_gradient(self, box, sigma: bool = False, mu: bool = False) -> torch.Tensor:
@@ -339,8 +351,8 @@ class BatchAdjointRunner:
             raise ValueError("one step count per problem")
         for prob in self.problems:
             if prob.ports:
-                raise NotImplementedError("gradients through lumped-port V/I samples are not "
-                                          "supported")
+                raise NotImplementedError("the batched reverse mode covers probe objectives "
+                                          "(no lumped-port S-parameters)")
         regions = [None] * n if regions is None else list(regions)
         self.runs = [DeviceRun(p, t, device, two_sets=False) for p, t in zip(self.problems, self.T)]
         self.device = self.runs[0].device
@@ -508,11 +520,13 @@ class _ForwardFn(torch.autograd.Function):
     def forward(ctx, eps, sigma, mu, runner):
         ctx.runner = runner
         ctx.shapes = tuple(None if t is None else (t.shape, t.device) for t in (eps, sigma, mu))
-        return runner.forward()
+        ts = runner.forward()
+        return ts, runner.port_samples()
 
     @staticmethod
-    def backward(ctx, g):
-        grads = (ctx.runner.backward(g), ctx.runner.grad_sigma, ctx.runner.grad_mu)
+    def backward(ctx, g_ts, g_ports):
+        r = ctx.runner
+        grads = (r.backward(g_ts, g_ports if r.prob.ports else None), r.grad_sigma, r.grad_mu)
         out = [None if sh is None or w is None else w.reshape(sh[0]).to(sh[1])
                for w, sh in zip(grads, ctx.shapes)]
         return out[0], out[1], out[2], None
@@ -521,13 +535,14 @@ class _ForwardFn(torch.autograd.Function):
 def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
                       checkpoint_segments=None, grad_region=None, sigma_tensor=None,
                       mu_tensor=None):
-    """The time series as a function of eps_override, sigma_override and
-    mu_r_override (the tensors among them that require grad)."""
+    """The time series and the lumped ports' (V, I) samples (T, n_port, 2)
+    as functions of eps_override, sigma_override and mu_r_override (the
+    tensors among them that require grad)."""
     def req(t):
         return t if isinstance(t, torch.Tensor) and t.requires_grad else None
 
     eps_in, sig_in, mu_in = req(eps_tensor), req(sigma_tensor), req(mu_tensor)
     runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments,
                            want_sigma=sig_in is not None, want_mu=mu_in is not None)
-    ts = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
-    return ts, runner.dr
+    ts, ports = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
+    return ts, ports, runner.dr

@@ -188,6 +188,22 @@ def port_dft(samples: np.ndarray, freqs, dt: float, t_offset: int = 0):
     return ph @ np.asarray(samples, np.float64)
 
 
+def port_dft_torch(samples, freqs, dt: float, t_offset: int = 0):
+    """port_dft on a torch tensor (differentiable, complex128)."""
+    x = samples.to(torch.float64)
+    n = torch.arange(x.shape[0], dtype=torch.float64, device=x.device) + t_offset
+    f = torch.as_tensor(np.asarray(freqs, np.float64), device=x.device)
+    ph = torch.exp(-2j * np.pi * torch.outer(f, n * dt)) * dt
+    return ph @ x.to(torch.complex128)
+
+
+def extract_lumped_s11_torch(v_dft, i_dft, z0=50.0):
+    """extract_lumped_s11 on torch tensors."""
+    den = v_dft - z0 * i_dft
+    den = torch.where(den.abs() > 0.0, den, torch.ones_like(den))
+    return (v_dft + z0 * i_dft) / den
+
+
 def extract_lumped_s11(v_dft, i_dft, z0=50.0):
     """S11 = (V + Z0 I)/(V - Z0 I) (rfx/probes/probes.py:1239-1273)."""
     den = v_dft - z0 * i_dft
@@ -768,9 +784,10 @@ class Simulation:
                 skip_preflight: bool = False, grad_region=None, **unsupported):
         """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
         PEC on axes covered by CPML (_execute.py:1420-1421); the returned
-        time series carries gradients w.r.t. ``eps_override``,
-        ``sigma_override`` and ``mu_r_override`` through the hand-written
-        adjoint.  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
+        time series (and, with lumped ports and ``port_s11_freqs``, the
+        S-parameters, a torch tensor then) carry gradients w.r.t.
+        ``eps_override``, ``sigma_override`` and ``mu_r_override`` through the
+        hand-written adjoint.  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
         limits where dJ/d eps_r is evaluated (default: the whole grid);
         ``checkpoint_segments`` = K forces K recompute segments."""
         _reject(unsupported, "forward")
@@ -790,17 +807,26 @@ class Simulation:
         if needs_grad and (prob.poles or prob.lorentz):
             raise NotImplementedError("gradients through dispersive materials are not supported "
                                       "(the reverse mode covers non-dispersive problems)")
+        port_t = None
         if needs_grad:
             from .adjoint import forward_with_grad
-            ts, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
-                                       checkpoint_segments, grad_region,
-                                       sigma_tensor=sigma_override, mu_tensor=mu_r_override)
+            ts, port_t, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
+                                               checkpoint_segments, grad_region,
+                                               sigma_tensor=sigma_override,
+                                               mu_tensor=mu_r_override)
         else:
             dr = DeviceRun(prob, n_steps, self._device)
             dr.run(0, n_steps)
             ts = dr.time_series().clone()
         s_params = freqs = None
-        if ports:
+        if ports and port_t is not None:
+            # differentiable S11: the same DFT and extraction in torch (complex128)
+            s_list = [extract_lumped_s11_torch(port_dft_torch(port_t[:, q, 0], s11, grid.dt),
+                                               port_dft_torch(port_t[:, q, 1], s11, grid.dt),
+                                               p[4]) for q, p in enumerate(ports)]
+            s_params = s_list[0] if len(s_list) == 1 else torch.stack(s_list)
+            freqs = s11
+        elif ports:
             torch.cuda.synchronize(dr.device)
             samp = dr.port_samples().cpu().numpy().astype(np.float64)
             s_list = []

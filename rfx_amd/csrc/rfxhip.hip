@@ -2415,6 +2415,7 @@ struct AdjParams {
   const float* rec_h;     // [rec_rows][3][region n]: the H update's R^n
   float* grad_mu;         // [region n] dJ/d mu_r (null: not computed)
   const float* dmu;       // per material id: d ch / d mu_r
+  const float* port_adj;  // [max_steps][n_port][2] dJ/dV, dJ/dI (null: none)
 };
 
 __device__ __forceinline__ long long region_index(const RegionParams& r, int i, int j, int k) {
@@ -2803,6 +2804,17 @@ __device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a,
         if (op.kind == RFXH_OP_INJECT && a.src_adj)
           a.src_adj[m * p.n_src + op.slot] = pick3(op.comp, ex, ey, ez);
       }
+      // port voltage V = -E_c d_par, sampled before the injection: its
+      // adjoint is not part of dJ/d(source table)
+      if (a.port_adj)
+        for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+          const rfxh_point_op op = p.ops[q];
+          if (op.kind != RFXH_OP_PORT) continue;
+          const float g = -op.d_par * a.port_adj[(m * p.n_port + op.slot) * 2];
+          if (op.comp == 0) ex += g;
+          else if (op.comp == 1) ey += g;
+          else ez += g;
+        }
     }
     if (p.pec_axes & 1) {
       if (i == 0 || i == p.nx - 1) ey = ez = 0.f;
@@ -2867,6 +2879,41 @@ __device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a,
   stf(a.a[0], c, ex);
   stf(a.a[1], c, ey);
   stf(a.a[2], c, ez);
+}
+
+// Port current adjoint of step n into H-bar^{n+1/2}: the Ampere-loop
+// current I = sum of +-loop * H over the cell and its backward neighbours
+// (point_ops), transposed; one thread per point operation (few ports),
+// before the H-bar pass of step n, so that pass sees the whole
+// dJ/dH^{n+1/2}.
+__global__ void k_port_adj_h(KParams p, AdjParams a, long long n) {
+  const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= p.n_ops) return;
+  const rfxh_point_op op = p.ops[q];
+  if (op.kind != RFXH_OP_PORT) return;
+  const float g = a.port_adj[(n * p.n_port + op.slot) * 2 + 1];
+  const long long c = op.cell;
+  const int k = (int)(c % p.nz), j = (int)((c / p.nz) % p.ny), i = (int)(c / p.plane);
+  const long long P = p.plane;
+  float* const hx = a.a[3];
+  float* const hy = a.a[4];
+  float* const hz = a.a[5];
+  if (op.comp == 2) {         // I = (hy - hy_i) l1 - (hx - hx_j) l0
+    atomicAdd(hy + c, g * op.loop[1]);
+    if (i > 0) atomicAdd(hy + c - P, -g * op.loop[1]);
+    atomicAdd(hx + c, -g * op.loop[0]);
+    if (j > 0) atomicAdd(hx + c - p.nz, g * op.loop[0]);
+  } else if (op.comp == 0) {  // I = (hz - hz_j) l2 - (hy - hy_k) l1
+    atomicAdd(hz + c, g * op.loop[2]);
+    if (j > 0) atomicAdd(hz + c - p.nz, -g * op.loop[2]);
+    atomicAdd(hy + c, -g * op.loop[1]);
+    if (k > 0) atomicAdd(hy + c - 1, g * op.loop[1]);
+  } else {                    // I = (hx - hx_k) l0 - (hz - hz_i) l2
+    atomicAdd(hx + c, g * op.loop[0]);
+    if (k > 0) atomicAdd(hx + c - 1, -g * op.loop[0]);
+    atomicAdd(hz + c, -g * op.loop[2]);
+    if (i > 0) atomicAdd(hz + c - P, g * op.loop[2]);
+  }
 }
 
 // The single-instance reverse kernels: the bodies of adj_h_cell and
@@ -3102,6 +3149,17 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
         if (op.kind == RFXH_OP_INJECT && a.src_adj)
           a.src_adj[m * p.n_src + op.slot] = pick3(op.comp, ex, ey, ez);
       }
+      // port voltage V = -E_c d_par, sampled before the injection: its
+      // adjoint is not part of dJ/d(source table)
+      if (a.port_adj)
+        for (int q = lo; q < p.n_ops && p.ops[q].cell == (long long)c; ++q) {
+          const rfxh_point_op op = p.ops[q];
+          if (op.kind != RFXH_OP_PORT) continue;
+          const float g = -op.d_par * a.port_adj[(m * p.n_port + op.slot) * 2];
+          if (op.comp == 0) ex += g;
+          else if (op.comp == 1) ey += g;
+          else ez += g;
+        }
     }
     if (p.pec_axes & 1) {
       if (i == 0 || i == p.nx - 1) ey = ez = 0.f;
@@ -4548,6 +4606,7 @@ static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo
   }
   a.probe_adj = ad->d_probe_adj;
   a.src_adj = ad->d_src_adj;
+  a.port_adj = ad->d_port_adj;
   if (ad->d_grad) {
     if (!pl->dcoef) {
       std::vector<float4> tmp(pl->n_mat);
@@ -4599,8 +4658,12 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
     hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, 0, (long long)(hi - 1));
   else
     hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, 0, (long long)(hi - 1));
+  const bool ports = a.port_adj && pl->kp.n_port > 0;
   for (int64_t n = hi - 1; n >= lo; --n) {
     const long long m = n - 1 >= lo ? (long long)(n - 1) : -1;
+    if (ports)
+      hipLaunchKernelGGL(k_port_adj_h, dim3((unsigned)((pl->kp.n_ops + 63) / 64)), dim3(64), 0, s,
+                         pl->kp, a, (long long)n);
     if (pl->kp.graded) {
       hipLaunchKernelGGL(k_adj_h<true>, pl->grid, pl->block, 0, s, pl->kp, a, (long long)n);
       hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, 1, m);
@@ -4914,8 +4977,9 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   std::vector<BatchAdjEntry> all(n);
   for (int q = 0; q < n; ++q) {
     if (!descs[q]) continue;
-    if (descs[q]->d_grad_mu)
-      return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients only");
+    if (descs[q]->d_grad_mu || descs[q]->d_port_adj)
+      return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients of "
+                               "probe objectives only");
     const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);
     if (rc) return rc;
     all[q].lo = lo[q];

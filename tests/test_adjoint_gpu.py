@@ -268,6 +268,61 @@ def test_mu_gradient_directions_vs_oracle_fd(torch_mod):
         assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
 
 
+def test_s11_gradient_directions_vs_oracle_fd(torch_mod):
+    """J = sum |S11(f)|^2 of the mixed case's lumped ports (forward lane,
+    port V/I samples through the adjoint): directional derivatives w.r.t.
+    eps_r vs float64 oracle central differences."""
+    torch = torch_mod
+    _say("mixed-case S11 objective directional FD")
+    spec = cases.mixed_case("forward")
+    assert any(p[2] != 0.0 for p in spec["ports"]) and spec.get("s11_freqs") is not None
+    n = 200
+    sim = cases.to_sim(spec)
+    g0, cfg0 = o.build_uniform(spec, n)
+    base = cfg0.eps_r.copy()
+    region = (2, 14, 3, 17, 2, 10)
+    eps = torch.tensor(base, requires_grad=True)
+    res = sim.forward(eps_override=eps, n_steps=n, grad_region=region,
+                      port_s11_freqs=np.asarray(spec["s11_freqs"], np.float32))
+    s = res.s_params
+    assert isinstance(s, torch.Tensor) and s.requires_grad
+    J = (s.abs() ** 2).sum()
+    J.backward()
+    grad = eps.grad.numpy().astype(np.float64)
+    assert np.abs(grad).max() > 0
+
+    def J_oracle(sp):
+        gg, cfg = o.build_uniform(sp, n)
+        out = cstep.run_c(cfg, n, want_fields=False)
+        tot = 0.0
+        for q in range(len(out.port_v_dft)):     # the case's one port, Z0 = 50
+            tot += float(np.sum(np.abs(o.extract_lumped_s11(out.port_v_dft[q],
+                                                            out.port_i_dft[q])) ** 2))
+        return tot
+
+    sp0 = dict(spec)
+    sp0["eps_override"] = base
+    J0 = J_oracle(sp0)
+    print("J", float(J), J0)
+    assert abs(float(J) - J0) <= 1e-3 * abs(J0)
+    rng = np.random.default_rng(8)
+    i0, i1, j0, j1, k0, k1 = region
+    for trial in range(3):
+        d = np.zeros_like(base, dtype=np.float64)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        h = 1e-3
+        Js = []
+        for sgn in (1, -1):
+            sp = dict(spec)
+            sp["eps_override"] = (base + sgn * h * d).astype(np.float32)
+            Js.append(J_oracle(sp))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float((grad * d).sum())
+        scale = float(np.abs(grad * d).sum())
+        print("s11 direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
+
+
 def test_segmented_gradient_equals_single(torch_mod):
     torch = torch_mod
     _say("checkpoint segments == single segment")
