@@ -273,6 +273,11 @@ typedef struct {
    * NULL: none): objectives on lumped-port S-parameters (rfx/probes/
    * probes.py:1240-1274 differentiated through the scan). */
   const float* d_port_adj;
+  /* dJ/d(DFT plane accumulators): host array of the plan's n_plane device
+   * pointers (complex128 [n_freq][n1][n2] each, torch's gradient of the
+   * accumulator: dJ/dRe + j dJ/dIm; a NULL entry, or a NULL array: none)
+   * (rfx/probes/probes.py:379-473 differentiated through the scan). */
+  const double* const* d_plane_adj;
 } rfxh_adjoint_desc;
 
 /* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries

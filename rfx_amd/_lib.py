@@ -99,7 +99,7 @@ class AdjointDesc(C.Structure):
         ("region", Region), ("d_grad", _P), ("dcoef", _FP),
         ("d_grad_sigma", _P), ("dcoef_sigma", _FP),
         ("d_rec_h", _P), ("d_grad_mu", _P), ("dcoef_mu", _FP),
-        ("d_port_adj", _P),
+        ("d_port_adj", _P), ("d_plane_adj", C.POINTER(_P)),
     ]
 
 

@@ -175,13 +175,21 @@ class AdjointRunner:
         """The lumped ports' (V, I) samples of the forward run, (T, n_port, 2)."""
         return self.dr.port_samples()[:self.n_steps].clone()
 
+    def plane_values(self) -> list:
+        """The DFT planes' accumulators of the forward run (complex128
+        (n_freq, n1, n2) each)."""
+        return [a.clone() for a in self.dr.plane_acc]
+
     # -- reverse -------------------------------------------------------------------
-    def backward(self, g_ts: torch.Tensor, g_ports: torch.Tensor | None = None) -> torch.Tensor:
+    def backward(self, g_ts: torch.Tensor, g_ports: torch.Tensor | None = None,
+                 g_planes: list | None = None) -> torch.Tensor:
         """dJ/d eps_r on the whole grid from dJ/d(time series) and, with lumped
-        ports, dJ/d(port samples) (T, n_port, 2); with want_sigma / want_mu
-        also dJ/d sigma / dJ/d mu_r, left in ``grad_sigma`` / ``grad_mu``."""
+        ports, dJ/d(port samples) (T, n_port, 2), and with DFT planes,
+        dJ/d(accumulator) per plane (torch's complex gradient dJ/dRe + j
+        dJ/dIm, or None); with want_sigma / want_mu also dJ/d sigma /
+        dJ/d mu_r, left in ``grad_sigma`` / ``grad_mu``."""
         st = _ReverseState(self.prob, self.dr, self.n_steps, g_ts, self.region, self.nreg,
-                           self.dcoef, self.dcoef_sigma, self.dcoef_mu, g_ports)
+                           self.dcoef, self.dcoef_sigma, self.dcoef_mu, g_ports, g_planes)
         d = st.desc
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
@@ -212,13 +220,14 @@ class _ReverseState:
 
     def __init__(self, prob: Problem, dr: DeviceRun, T: int, g_ts: torch.Tensor, region,
                  nreg: int, dcoef: np.ndarray, dcoef_sigma: np.ndarray | None = None,
-                 dcoef_mu: np.ndarray | None = None, g_ports: torch.Tensor | None = None):
+                 dcoef_mu: np.ndarray | None = None, g_ports: torch.Tensor | None = None,
+                 g_planes: list | None = None):
         dev = dr.device
         self.prob = prob
         self.T = T
         n_probe = len(prob.probes)
         self.probe_adj = torch.zeros((T, max(n_probe, 1)), dtype=torch.float32, device=dev)
-        if n_probe:
+        if n_probe and g_ts is not None:
             self.probe_adj[:, :n_probe] = g_ts.to(dev, torch.float32).reshape(T, n_probe)
         self.n_src = len(prob.sources)
         self.src_adj = torch.zeros((T, max(self.n_src, 1)), dtype=torch.float32, device=dev)
@@ -259,6 +268,20 @@ class _ReverseState:
             pa[:T, :n_port] = g_ports.to(dev, torch.float32).reshape(T, n_port, 2)
             self.port_adj = pa
             d.d_port_adj = pa.data_ptr()
+        self.plane_adj = None
+        if g_planes is not None and any(g is not None for g in g_planes):
+            if len(g_planes) != len(dr.plane_acc):
+                raise ValueError(f"{len(g_planes)} plane gradients for {len(dr.plane_acc)} planes")
+            self.plane_adj = []
+            ptrs = (C.c_void_p * len(g_planes))()
+            for q, (g, acc) in enumerate(zip(g_planes, dr.plane_acc)):
+                if g is None:
+                    continue
+                g = g.to(dev, torch.complex128).reshape(acc.shape).contiguous()
+                self.plane_adj.append(g)
+                ptrs[q] = g.data_ptr()
+            self._plane_ptrs = ptrs
+            d.d_plane_adj = C.cast(ptrs, C.POINTER(C.c_void_p))
         self.desc = d
 
     def full_gradient(self, box, sigma: bool = False, mu: bool = False) -> torch.Tensor:
@@ -520,13 +543,15 @@ class _ForwardFn(torch.autograd.Function):
     def forward(ctx, eps, sigma, mu, runner):
         ctx.runner = runner
         ctx.shapes = tuple(None if t is None else (t.shape, t.device) for t in (eps, sigma, mu))
+        ctx.set_materialize_grads(False)   # outputs the objective ignores cost nothing
         ts = runner.forward()
-        return ts, runner.port_samples()
+        return (ts, runner.port_samples(), *runner.plane_values())
 
     @staticmethod
-    def backward(ctx, g_ts, g_ports):
+    def backward(ctx, g_ts, g_ports, *g_planes):
         r = ctx.runner
-        grads = (r.backward(g_ts, g_ports if r.prob.ports else None), r.grad_sigma, r.grad_mu)
+        grads = (r.backward(g_ts, g_ports if r.prob.ports else None, list(g_planes)),
+                 r.grad_sigma, r.grad_mu)
         out = [None if sh is None or w is None else w.reshape(sh[0]).to(sh[1])
                for w, sh in zip(grads, ctx.shapes)]
         return out[0], out[1], out[2], None
@@ -535,14 +560,15 @@ class _ForwardFn(torch.autograd.Function):
 def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
                       checkpoint_segments=None, grad_region=None, sigma_tensor=None,
                       mu_tensor=None):
-    """The time series and the lumped ports' (V, I) samples (T, n_port, 2)
-    as functions of eps_override, sigma_override and mu_r_override (the
-    tensors among them that require grad)."""
+    """The time series, the lumped ports' (V, I) samples (T, n_port, 2) and
+    the DFT planes' accumulators as functions of eps_override,
+    sigma_override and mu_r_override (the tensors among them that require
+    grad)."""
     def req(t):
         return t if isinstance(t, torch.Tensor) and t.requires_grad else None
 
     eps_in, sig_in, mu_in = req(eps_tensor), req(sigma_tensor), req(mu_tensor)
     runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments,
                            want_sigma=sig_in is not None, want_mu=mu_in is not None)
-    ts, ports = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
-    return ts, ports, runner.dr
+    ts, ports, *planes = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
+    return ts, ports, planes, runner.dr

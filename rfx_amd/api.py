@@ -143,7 +143,7 @@ class DFTPlaneProbe:
     index: int
     component: str
     freqs: np.ndarray
-    accumulator: np.ndarray   # complex128 (n_freqs, n1, n2)
+    accumulator: np.ndarray   # complex128 (n_freqs, n1, n2); a torch tensor when differentiated
 
 
 @dataclass
@@ -807,10 +807,10 @@ class Simulation:
         if needs_grad and (prob.poles or prob.lorentz):
             raise NotImplementedError("gradients through dispersive materials are not supported "
                                       "(the reverse mode covers non-dispersive problems)")
-        port_t = None
+        port_t = plane_t = None
         if needs_grad:
             from .adjoint import forward_with_grad
-            ts, port_t, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
+            ts, port_t, plane_t, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
                                                checkpoint_segments, grad_region,
                                                sigma_tensor=sigma_override,
                                                mu_tensor=mu_r_override)
@@ -838,9 +838,11 @@ class Simulation:
             freqs = s11
         planes = None
         if self._dft_planes:
-            planes = {e.name: DFTPlaneProbe(a, int(idx), comp, np.asarray(fr), acc.cpu().numpy())
-                      for e, (a, idx, comp, fr), acc in zip(self._dft_planes, prob.planes,
-                                                            dr.plane_acc)}
+            # with a gradient requested the accumulators are differentiable
+            # torch tensors (complex128), else numpy arrays
+            accs = plane_t if plane_t is not None else [a.cpu().numpy() for a in dr.plane_acc]
+            planes = {e.name: DFTPlaneProbe(a, int(idx), comp, np.asarray(fr), acc)
+                      for e, (a, idx, comp, fr), acc in zip(self._dft_planes, prob.planes, accs)}
         fluxes = self._flux_results(prob, dr, prob.flux_areas) if self._flux_monitors else None
         return ForwardResult(time_series=ts, grid=grid, s_params=s_params, freqs=freqs,
                              dft_planes=planes, flux_monitors=fluxes,

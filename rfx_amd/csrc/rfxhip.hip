@@ -2916,6 +2916,30 @@ __global__ void k_port_adj_h(KParams p, AdjParams a, long long n) {
   }
 }
 
+// DFT-plane adjoint of step n into the adjoint field `tgt` (the plane's
+// component): acc[f] += tw[n][f] x over the steps, so dJ/dx[n] = sum_f
+// Re(conj(G[f]) tw[n][f]) with G = dJ/dRe(acc) + j dJ/dIm(acc).
+__global__ __launch_bounds__(256) void k_plane_adj(PlaneParams q, const double2* __restrict__ G,
+                                                   long long n, float* tgt) {
+  const long long npts = (long long)q.n1 * q.n2;
+  const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= npts || n < 0 || n >= q.max_steps) return;
+  const int u = (int)(pidx / q.n2), v = (int)(pidx % q.n2);
+  int ii[3];
+  const int a1 = q.axis == 0 ? 1 : 0, a2 = q.axis == 2 ? 1 : 2;
+  ii[q.axis] = q.index;
+  ii[a1] = u;
+  ii[a2] = v;
+  double g = 0.0;
+  for (int f = 0; f < q.nf; ++f) {
+    const double2 gf = G[(long long)f * npts + pidx];
+    const double2 tw = q.tw[n * q.nf + f];
+    g += gf.x * tw.x + gf.y * tw.y;
+  }
+  const long long c = ((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2];
+  tgt[c] += (float)g;
+}
+
 // The single-instance reverse kernels: the bodies of adj_h_cell and
 // adj_e_cell written out in the kernels (the same arithmetic; through a
 // device function taking the by-value kernel arguments by reference the
@@ -4653,23 +4677,51 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (hi == lo) return RFXH_OK;
+  // DFT-plane adjoints: planes of an E component take theirs just before
+  // the prologue of their step (the sample is E^{n+1} after the injection,
+  // like a probe's), planes of an H component before the H-bar pass of
+  // their step
+  bool e_planes = false, h_planes = false;
+  if (ad->d_plane_adj)
+    for (size_t q = 0; q < pl->planes.size(); ++q)
+      if (ad->d_plane_adj[q]) (pl->plane_comp[q] < 3 ? e_planes : h_planes) = true;
+  auto plane_adj = [&](bool e, long long step) {
+    for (size_t q = 0; q < pl->planes.size(); ++q) {
+      const int comp = pl->plane_comp[q];
+      if (!ad->d_plane_adj[q] || (comp < 3) != e) continue;
+      const PlaneParams& pp = pl->planes[q];
+      const long long npts = (long long)pp.n1 * pp.n2;
+      hipLaunchKernelGGL(k_plane_adj, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp,
+                         reinterpret_cast<const double2*>(ad->d_plane_adj[q]), step, a.a[comp]);
+    }
+  };
+  auto adj_e = [&](int gather, long long m) {
+    if (pl->kp.graded)
+      hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+    else
+      hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+  };
   // prologue of step hi-1 (E-bar holds the adjoint of E^{hi})
-  if (pl->kp.graded)
-    hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, 0, (long long)(hi - 1));
-  else
-    hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, 0, (long long)(hi - 1));
+  if (e_planes) plane_adj(true, (long long)(hi - 1));
+  adj_e(0, (long long)(hi - 1));
   const bool ports = a.port_adj && pl->kp.n_port > 0;
   for (int64_t n = hi - 1; n >= lo; --n) {
     const long long m = n - 1 >= lo ? (long long)(n - 1) : -1;
     if (ports)
       hipLaunchKernelGGL(k_port_adj_h, dim3((unsigned)((pl->kp.n_ops + 63) / 64)), dim3(64), 0, s,
                          pl->kp, a, (long long)n);
-    if (pl->kp.graded) {
+    if (h_planes) plane_adj(false, (long long)n);
+    if (pl->kp.graded)
       hipLaunchKernelGGL(k_adj_h<true>, pl->grid, pl->block, 0, s, pl->kp, a, (long long)n);
-      hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, 1, m);
-    } else {
+    else
       hipLaunchKernelGGL(k_adj_h<false>, pl->grid, pl->block, 0, s, pl->kp, a, (long long)n);
-      hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, 1, m);
+    if (e_planes && m >= 0) {
+      // the gather, then the E-plane adjoints of step m, then its prologue
+      adj_e(1, -1);
+      plane_adj(true, m);
+      adj_e(0, m);
+    } else {
+      adj_e(1, m);
     }
   }
   HIPCHK(hipGetLastError());
@@ -4977,7 +5029,7 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   std::vector<BatchAdjEntry> all(n);
   for (int q = 0; q < n; ++q) {
     if (!descs[q]) continue;
-    if (descs[q]->d_grad_mu || descs[q]->d_port_adj)
+    if (descs[q]->d_grad_mu || descs[q]->d_port_adj || descs[q]->d_plane_adj)
       return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients of "
                                "probe objectives only");
     const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);

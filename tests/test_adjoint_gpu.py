@@ -323,6 +323,83 @@ def test_s11_gradient_directions_vs_oracle_fd(torch_mod):
         assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
 
 
+def test_dft_plane_gradient_directions_vs_oracle_fd(torch_mod):
+    """J = sum_q sum |acc_q|^2 / S_q over the mixed case's two DFT planes (an
+    E_z plane and an H_y plane; S_q the oracle's value at the base point),
+    through the differentiable plane accumulators: directional derivatives
+    w.r.t. eps_r vs float64 oracle central differences."""
+    torch = torch_mod
+    _say("mixed-case DFT-plane objective directional FD")
+    spec = cases.mixed_case("forward")
+    n = 200
+    sim = cases.to_sim(spec)
+    g0, cfg0 = o.build_uniform(spec, n)
+    base = cfg0.eps_r.copy()
+    S = [float(np.sum(np.abs(a) ** 2)) for a in cstep.run_c(cfg0, n, want_fields=False).dft_planes]
+    assert len(S) == 2 and min(S) > 0
+    region = (2, 14, 3, 17, 2, 10)
+    eps = torch.tensor(base, requires_grad=True)
+    res = sim.forward(eps_override=eps, n_steps=n, grad_region=region)
+    accs = [res.dft_planes[k].accumulator for k in res.dft_planes]
+    assert all(isinstance(a, torch.Tensor) and a.requires_grad for a in accs)
+    J = sum((a.abs() ** 2).sum() / s for a, s in zip(accs, S))
+    J.backward()
+    grad = eps.grad.numpy().astype(np.float64)
+    assert np.abs(grad).max() > 0
+
+    def J_oracle(sp):
+        _, cfg = o.build_uniform(sp, n)
+        out = cstep.run_c(cfg, n, want_fields=False)
+        return sum(float(np.sum(np.abs(a) ** 2)) / s for a, s in zip(out.dft_planes, S))
+
+    print("J", float(J), 2.0)
+    assert abs(float(J) - 2.0) <= 1e-3 * 2.0
+    rng = np.random.default_rng(12)
+    i0, i1, j0, j1, k0, k1 = region
+    for trial in range(3):
+        d = np.zeros_like(base, dtype=np.float64)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        h = 1e-3
+        Js = []
+        for sgn in (1, -1):
+            sp = dict(spec)
+            sp["eps_override"] = (base + sgn * h * d).astype(np.float32)
+            Js.append(J_oracle(sp))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float((grad * d).sum())
+        scale = float(np.abs(grad * d).sum())
+        print("plane direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
+
+
+def test_dft_plane_gradient_single_plane_and_segments(torch_mod):
+    """A gradient through one plane only (the other's gradient None) and
+    through 7 checkpoint segments equals the one-segment gradient; the H_y
+    plane's part and the E_z plane's part add up to the gradient of their
+    sum."""
+    torch = torch_mod
+    _say("DFT-plane gradient: single planes, segments")
+    spec = cases.mixed_case("forward")
+    n = 160
+    sim = cases.to_sim(spec)
+    _, cfg0 = o.build_uniform(spec, n)
+    base = cfg0.eps_r.copy()
+
+    def grad_of(which, segs=None):
+        eps = torch.tensor(base, requires_grad=True)
+        res = sim.forward(eps_override=eps, n_steps=n, checkpoint_segments=segs)
+        accs = [res.dft_planes[k].accumulator for k in res.dft_planes]
+        J = sum((accs[q].abs() ** 2).sum() / (accs[q].detach().abs() ** 2).sum() for q in which)
+        J.backward()
+        return eps.grad.numpy().copy()
+
+    g_e, g_h, g_both = grad_of([0]), grad_of([1]), grad_of([0, 1])
+    assert np.abs(g_e).max() > 0 and np.abs(g_h).max() > 0
+    np.testing.assert_allclose(g_e + g_h, g_both, rtol=1e-5, atol=1e-5 * np.abs(g_both).max())
+    g_seg = grad_of([0, 1], segs=7)
+    np.testing.assert_allclose(g_seg, g_both, rtol=1e-5, atol=1e-5 * np.abs(g_both).max())
+
+
 def test_segmented_gradient_equals_single(torch_mod):
     torch = torch_mod
     _say("checkpoint segments == single segment")
