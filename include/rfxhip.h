@@ -278,6 +278,13 @@ typedef struct {
    * accumulator: dJ/dRe + j dJ/dIm; a NULL entry, or a NULL array: none)
    * (rfx/probes/probes.py:379-473 differentiated through the scan). */
   const double* const* d_plane_adj;
+  /* dJ/d(flux accumulators): host array of one device pointer per flux
+   * monitor, in the plan's order, then six per NTFF box (faces x-lo, x-hi,
+   * y-lo, y-hi, z-lo, z-hi), each complex128 [4][n_freq][n1][n2] like the
+   * accumulator (dJ/dRe + j dJ/dIm); a NULL entry, or a NULL array: none
+   * (rfx/farfield.py:208-313 and the flux DFT of rfx/api/__init__.py:
+   * 2577-2645, differentiated through the scan). */
+  const double* const* d_flux_adj;
 } rfxh_adjoint_desc;
 
 /* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries
@@ -367,6 +374,20 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* desc, int32_t n_theta, const d
  * + the theta and phi values; larger requests return RFXH_EINVAL.  Every call
  * on a GPU shares that GPU's array: calls that overlap on one GPU (other host
  * threads, other streams) race.  Waits for its stream before returning. */
+/* Reverse mode of rfxh_ntff_far_field (the reference differentiates its
+ * far field with jax.grad through rfx/farfield.py:393-556): from g_theta,
+ * g_phi (device complex128 [n_freq][n_theta][n_phi], torch's gradient
+ * dJ/dRe + j dJ/dIm of E_theta and E_phi) writes dJ/d(face accumulator)
+ * into d_gacc[face] (device complex128 [4][n_freq][n1][n2] per face,
+ * caller-owned, overwritten).  desc->d_acc is not read.  The transform is
+ * linear: the result is the conjugate transpose of the forward map applied
+ * to (g_theta, g_phi).  Allocates its tables and scratch, waits for its
+ * stream before returning. */
+int rfxh_ntff_far_field_adjoint(const rfxh_farfield_desc* desc, int32_t n_theta,
+                                const double* theta, int32_t n_phi, const double* phi,
+                                const double* g_theta, const double* g_phi,
+                                double* const* d_gacc, void* stream);
+
 int rfxh_ntff_far_field_noalloc(const rfxh_farfield_desc* desc, int32_t n_theta,
                                 const double* theta, int32_t n_phi, const double* phi,
                                 double* e_theta, double* e_phi, void* stream);

@@ -99,7 +99,7 @@ class AdjointDesc(C.Structure):
         ("region", Region), ("d_grad", _P), ("dcoef", _FP),
         ("d_grad_sigma", _P), ("dcoef_sigma", _FP),
         ("d_rec_h", _P), ("d_grad_mu", _P), ("dcoef_mu", _FP),
-        ("d_port_adj", _P), ("d_plane_adj", C.POINTER(_P)),
+        ("d_port_adj", _P), ("d_plane_adj", C.POINTER(_P)), ("d_flux_adj", C.POINTER(_P)),
     ]
 
 
@@ -116,7 +116,7 @@ EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run
            "rfxh_batch_destroy",
            "rfxh_batch_run", "rfxh_batch_run_record", "rfxh_batch_adjoint", "rfxh_set_variant",
            "rfxh_get_variant", "rfxh_ntff_far_field",
-           "rfxh_ntff_far_field_noalloc", "rfxh_assemble_fill", "rfxh_assemble_pec_pairs",
+           "rfxh_ntff_far_field_noalloc", "rfxh_ntff_far_field_adjoint", "rfxh_assemble_fill", "rfxh_assemble_pec_pairs",
            "rfxh_assemble_pec_apply", "rfxh_last_error", "rfxh_version")
 
 
@@ -178,6 +178,10 @@ def load():
     lib.rfxh_ntff_far_field.restype = C.c_int
     lib.rfxh_ntff_far_field_noalloc.argtypes = lib.rfxh_ntff_far_field.argtypes
     lib.rfxh_ntff_far_field_noalloc.restype = C.c_int
+    lib.rfxh_ntff_far_field_adjoint.argtypes = [C.POINTER(FarFieldDesc), C.c_int32,
+                                                C.POINTER(C.c_double), C.c_int32,
+                                                C.POINTER(C.c_double), _P, _P, C.POINTER(_P), _P]
+    lib.rfxh_ntff_far_field_adjoint.restype = C.c_int
     lib.rfxh_assemble_fill.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(AsmBox),
                                        C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
                                        C.c_int32, _P, _P, _P]

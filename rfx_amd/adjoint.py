@@ -180,16 +180,23 @@ class AdjointRunner:
         (n_freq, n1, n2) each)."""
         return [a.clone() for a in self.dr.plane_acc]
 
+    def flux_values(self) -> list:
+        """The flux monitors' accumulators, then the six faces of each NTFF
+        box (complex128 (4, n_freq, n1, n2) each): the plan's flux order."""
+        return [a.clone() for a in _flux_list(self.dr)]
+
     # -- reverse -------------------------------------------------------------------
     def backward(self, g_ts: torch.Tensor, g_ports: torch.Tensor | None = None,
-                 g_planes: list | None = None) -> torch.Tensor:
+                 g_planes: list | None = None, g_fluxes: list | None = None) -> torch.Tensor:
         """dJ/d eps_r on the whole grid from dJ/d(time series) and, with lumped
-        ports, dJ/d(port samples) (T, n_port, 2), and with DFT planes,
-        dJ/d(accumulator) per plane (torch's complex gradient dJ/dRe + j
-        dJ/dIm, or None); with want_sigma / want_mu also dJ/d sigma /
-        dJ/d mu_r, left in ``grad_sigma`` / ``grad_mu``."""
+        ports, dJ/d(port samples) (T, n_port, 2), with DFT planes,
+        dJ/d(accumulator) per plane, with flux monitors / NTFF boxes,
+        dJ/d(accumulator) per entry of flux_values() (torch's complex
+        gradient dJ/dRe + j dJ/dIm, or None); with want_sigma / want_mu
+        also dJ/d sigma / dJ/d mu_r, left in ``grad_sigma`` / ``grad_mu``."""
         st = _ReverseState(self.prob, self.dr, self.n_steps, g_ts, self.region, self.nreg,
-                           self.dcoef, self.dcoef_sigma, self.dcoef_mu, g_ports, g_planes)
+                           self.dcoef, self.dcoef_sigma, self.dcoef_mu, g_ports, g_planes,
+                           g_fluxes)
         d = st.desc
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
@@ -221,7 +228,7 @@ class _ReverseState:
     def __init__(self, prob: Problem, dr: DeviceRun, T: int, g_ts: torch.Tensor, region,
                  nreg: int, dcoef: np.ndarray, dcoef_sigma: np.ndarray | None = None,
                  dcoef_mu: np.ndarray | None = None, g_ports: torch.Tensor | None = None,
-                 g_planes: list | None = None):
+                 g_planes: list | None = None, g_fluxes: list | None = None):
         dev = dr.device
         self.prob = prob
         self.T = T
@@ -268,21 +275,28 @@ class _ReverseState:
             pa[:T, :n_port] = g_ports.to(dev, torch.float32).reshape(T, n_port, 2)
             self.port_adj = pa
             d.d_port_adj = pa.data_ptr()
-        self.plane_adj = None
+        self.acc_adj = []     # keeps the gradient tensors the descriptor points at
+        self._ptr_tables = []
         if g_planes is not None and any(g is not None for g in g_planes):
-            if len(g_planes) != len(dr.plane_acc):
-                raise ValueError(f"{len(g_planes)} plane gradients for {len(dr.plane_acc)} planes")
-            self.plane_adj = []
-            ptrs = (C.c_void_p * len(g_planes))()
-            for q, (g, acc) in enumerate(zip(g_planes, dr.plane_acc)):
-                if g is None:
-                    continue
-                g = g.to(dev, torch.complex128).reshape(acc.shape).contiguous()
-                self.plane_adj.append(g)
-                ptrs[q] = g.data_ptr()
-            self._plane_ptrs = ptrs
-            d.d_plane_adj = C.cast(ptrs, C.POINTER(C.c_void_p))
+            d.d_plane_adj = self._pointer_table(g_planes, dr.plane_acc, "plane", dev)
+        if g_fluxes is not None and any(g is not None for g in g_fluxes):
+            d.d_flux_adj = self._pointer_table(g_fluxes, _flux_list(dr), "flux", dev)
         self.desc = d
+
+    def _pointer_table(self, grads, accs, what, dev):
+        """Host array of device pointers to the complex128 gradients (NULL
+        where a gradient is None), laid out like the accumulators."""
+        if len(grads) != len(accs):
+            raise ValueError(f"{len(grads)} {what} gradients for {len(accs)} accumulators")
+        ptrs = (C.c_void_p * len(grads))()
+        for q, (g, acc) in enumerate(zip(grads, accs)):
+            if g is None:
+                continue
+            g = g.to(dev, torch.complex128).reshape(acc.shape).contiguous()
+            self.acc_adj.append(g)
+            ptrs[q] = g.data_ptr()
+        self._ptr_tables.append(ptrs)
+        return C.cast(ptrs, C.POINTER(C.c_void_p))
 
     def full_gradient(self, box, sigma: bool = False, mu: bool = False) -> torch.Tensor:
         """dJ/d eps_r (sigma: dJ/d sigma, mu: dJ/d mu_r) on the whole grid
@@ -538,6 +552,12 @@ def batch_gradients(problems, n_steps, dj_dts, device, regions=None,
     return run(0, len(probs))
 
 
+def _flux_list(dr) -> list:
+    """A run's flux accumulators in the plan's order: the flux monitors,
+    then the six faces of each NTFF box."""
+    return list(dr.flux_acc) + [f for box in dr.ntff_acc for f in box]
+
+
 class _ForwardFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps, sigma, mu, runner):
@@ -545,12 +565,16 @@ class _ForwardFn(torch.autograd.Function):
         ctx.shapes = tuple(None if t is None else (t.shape, t.device) for t in (eps, sigma, mu))
         ctx.set_materialize_grads(False)   # outputs the objective ignores cost nothing
         ts = runner.forward()
-        return (ts, runner.port_samples(), *runner.plane_values())
+        planes = runner.plane_values()
+        ctx.n_plane = len(planes)
+        return (ts, runner.port_samples(), *planes, *runner.flux_values())
 
     @staticmethod
-    def backward(ctx, g_ts, g_ports, *g_planes):
+    def backward(ctx, g_ts, g_ports, *g_acc):
         r = ctx.runner
-        grads = (r.backward(g_ts, g_ports if r.prob.ports else None, list(g_planes)),
+        k = ctx.n_plane
+        grads = (r.backward(g_ts, g_ports if r.prob.ports else None, list(g_acc[:k]),
+                            list(g_acc[k:])),
                  r.grad_sigma, r.grad_mu)
         out = [None if sh is None or w is None else w.reshape(sh[0]).to(sh[1])
                for w, sh in zip(grads, ctx.shapes)]
@@ -560,15 +584,16 @@ class _ForwardFn(torch.autograd.Function):
 def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
                       checkpoint_segments=None, grad_region=None, sigma_tensor=None,
                       mu_tensor=None):
-    """The time series, the lumped ports' (V, I) samples (T, n_port, 2) and
-    the DFT planes' accumulators as functions of eps_override,
-    sigma_override and mu_r_override (the tensors among them that require
-    grad)."""
+    """The time series, the lumped ports' (V, I) samples (T, n_port, 2), the
+    DFT planes' accumulators and the flux accumulators (flux monitors, then
+    NTFF faces) as functions of eps_override, sigma_override and
+    mu_r_override (the tensors among them that require grad)."""
     def req(t):
         return t if isinstance(t, torch.Tensor) and t.requires_grad else None
 
     eps_in, sig_in, mu_in = req(eps_tensor), req(sigma_tensor), req(mu_tensor)
     runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments,
                            want_sigma=sig_in is not None, want_mu=mu_in is not None)
-    ts, ports, *planes = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
-    return ts, ports, planes, runner.dr
+    ts, ports, *acc = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
+    k = len(prob.planes)
+    return ts, ports, acc[:k], acc[k:], runner.dr

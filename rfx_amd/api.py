@@ -103,7 +103,13 @@ class FluxMonitor:
 
 def flux_spectrum(mon: FluxMonitor) -> np.ndarray:
     """Integral of Re(E x H*) . n dA per frequency, in float64
-    (rfx/probes/probes.py:615-660); positive = power along +axis."""
+    (rfx/probes/probes.py:615-660); positive = power along +axis.  A monitor
+    of differentiable torch tensors gives a torch tensor."""
+    if isinstance(mon.e1_dft, torch.Tensor):
+        dA = torch.as_tensor(np.asarray(mon.dA, np.float64), device=mon.e1_dft.device)
+        integrand = (mon.e1_dft * torch.conj(mon.h2_dft)
+                     - mon.e2_dft * torch.conj(mon.h1_dft))
+        return torch.real(torch.sum(integrand * dA, dim=(-2, -1)))
     integrand = mon.e1_dft * np.conj(mon.h2_dft) - mon.e2_dft * np.conj(mon.h1_dft)
     return np.real(np.sum(integrand * np.asarray(mon.dA, np.float64), axis=(-2, -1)))
 
@@ -391,10 +397,12 @@ class Simulation:
         check_box(box, grid.shape)
         return box
 
-    def _ntff_results(self, box, dr):
+    def _ntff_results(self, box, dr, faces=None):
         if box is None:
             return None
-        from .farfield import data_from_device
+        from .farfield import NTFFData, data_from_device
+        if faces is not None:     # differentiable (n_freqs, n1, n2, 4) views
+            return NTFFData(*[a.permute(1, 2, 3, 0) for a in faces])
         return data_from_device(dr.ntff_acc[0])
 
     def _flux_setup(self, grid, n_steps):
@@ -434,11 +442,11 @@ class Simulation:
                         dft_window_weights(n_steps, e.window, e.alpha)))
         return out, areas
 
-    def _flux_results(self, prob, dr, areas):
+    def _flux_results(self, prob, dr, areas, accs=None):
         res = {}
         for e, (a, idx, lo1, hi1, lo2, hi2, fr, _), acc, dA in zip(
-                self._flux_monitors, prob.fluxes, dr.flux_acc, areas):
-            x = acc.cpu().numpy()
+                self._flux_monitors, prob.fluxes, dr.flux_acc if accs is None else accs, areas):
+            x = acc.cpu().numpy() if accs is None else acc
             res[e.name] = FluxMonitor(x[0], x[1], x[2], x[3], np.asarray(fr), a, int(idx), dA,
                                       lo1, hi1, lo2, hi2)
         return res
@@ -807,10 +815,10 @@ class Simulation:
         if needs_grad and (prob.poles or prob.lorentz):
             raise NotImplementedError("gradients through dispersive materials are not supported "
                                       "(the reverse mode covers non-dispersive problems)")
-        port_t = plane_t = None
+        port_t = plane_t = flux_t = None
         if needs_grad:
             from .adjoint import forward_with_grad
-            ts, port_t, plane_t, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
+            ts, port_t, plane_t, flux_t, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
                                                checkpoint_segments, grad_region,
                                                sigma_tensor=sigma_override,
                                                mu_tensor=mu_r_override)
@@ -843,11 +851,17 @@ class Simulation:
             accs = plane_t if plane_t is not None else [a.cpu().numpy() for a in dr.plane_acc]
             planes = {e.name: DFTPlaneProbe(a, int(idx), comp, np.asarray(fr), acc)
                       for e, (a, idx, comp, fr), acc in zip(self._dft_planes, prob.planes, accs)}
-        fluxes = self._flux_results(prob, dr, prob.flux_areas) if self._flux_monitors else None
+        # with a gradient requested, flux monitors and NTFF faces hold
+        # differentiable torch tensors
+        n_mon = len(prob.fluxes)
+        fluxes = (self._flux_results(prob, dr, prob.flux_areas,
+                                     None if flux_t is None else flux_t[:n_mon])
+                  if self._flux_monitors else None)
+        ntff = self._ntff_results(prob.ntff_box, dr,
+                                  None if flux_t is None else flux_t[n_mon:n_mon + 6])
         return ForwardResult(time_series=ts, grid=grid, s_params=s_params, freqs=freqs,
                              dft_planes=planes, flux_monitors=fluxes,
-                             ntff_data=self._ntff_results(prob.ntff_box, dr),
-                             ntff_box=prob.ntff_box)
+                             ntff_data=ntff, ntff_box=prob.ntff_box)
 
 
 def _to_np(x):

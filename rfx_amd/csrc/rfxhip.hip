@@ -2346,6 +2346,100 @@ __global__ __launch_bounds__(kFFBlock) void k_far_field(FarFieldParams q) {
   q.e_phi[o] = make_double2(-g * b.y, g * b.x);     //  j g b
 }
 
+// Reverse mode of k_far_field, first half: per (direction, frequency) the
+// gradients of the twelve per-axis sums S[ax][t] from the gradients G_theta,
+// G_phi of the outputs (torch's complex convention: for y = a x, G_x =
+// conj(a) G_y).  E_theta = -j g A, E_phi = j g B give G_A = j g G_theta,
+// G_B = -j g G_phi; A = L_phi + eta0 N_theta, B = L_theta - eta0 N_phi give
+// G_L = phi-hat G_A + theta-hat G_B, G_N = eta0 (theta-hat G_A - phi-hat G_B);
+// each S[ax][t] enters one component of N or L with sign +-1.
+__global__ __launch_bounds__(256) void k_far_field_adj_s(FarFieldParams q,
+                                                         const double2* __restrict__ g_th,
+                                                         const double2* __restrict__ g_ph,
+                                                         double2* __restrict__ gs) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (d >= q.n_dir) return;
+  const double g = q.kwave[f] / (4.0 * M_PI);
+  const long long o = (long long)f * q.n_dir + d;
+  const double2 gt = g_th[o], gp = g_ph[o];
+  const double2 GA = make_double2(-g * gt.y, g * gt.x);    //  j g G_theta
+  const double2 GB = make_double2(g * gp.y, -g * gp.x);    // -j g G_phi
+  const double* th = q.dirs + 9 * (long long)d + 3;
+  const double* ph = th + 3;
+  double2 GL[3], GN[3];
+  for (int c = 0; c < 3; ++c) {
+    GL[c] = make_double2(ph[c] * GA.x + th[c] * GB.x, ph[c] * GA.y + th[c] * GB.y);
+    GN[c] = make_double2(q.eta0 * (th[c] * GA.x - ph[c] * GB.x),
+                         q.eta0 * (th[c] * GA.y - ph[c] * GB.y));
+  }
+  auto neg = [](double2 v) { return make_double2(-v.x, -v.y); };
+  double2* out = gs + 12 * o;
+  // x: N1 -= S3, N2 += S2, L1 += S1, L2 -= S0
+  out[0] = neg(GL[2]); out[1] = GL[1]; out[2] = GN[2]; out[3] = neg(GN[1]);
+  // y: N0 += S3, N2 -= S2, L0 -= S1, L2 += S0
+  out[4] = GL[2]; out[5] = neg(GL[0]); out[6] = neg(GN[2]); out[7] = GN[0];
+  // z: N0 -= S3, N1 += S2, L0 += S1, L1 -= S0
+  out[8] = neg(GL[1]); out[9] = GL[0]; out[10] = GN[1]; out[11] = neg(GN[0]);
+}
+
+// Second half, one thread per (face cell, frequency): dJ/d acc[t][f][c] =
+// s dS(c) sum_d e^{-j k r_d.p_c} G_S[ax][t](f, d), the directions streaming
+// through LDS in chunks of kFFBlock (r-hat and the face axis's four G_S).
+__global__ __launch_bounds__(kFFBlock) void k_far_field_adj(FarFieldParams q,
+                                                            const double2* __restrict__ gs,
+                                                            int face, double2* __restrict__ gacc) {
+  __shared__ double s_r[3][kFFBlock];
+  __shared__ double2 s_g[4][kFFBlock];
+  const int ax = face / 2, side = face & 1;
+  const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const int n2 = q.hi[t2] - q.lo[t2];
+  const long long npts = (long long)(q.hi[t1] - q.lo[t1]) * n2;
+  const int tid = threadIdx.x;
+  const int f = blockIdx.y;
+  const long long c = (long long)blockIdx.x * kFFBlock + tid;
+  const bool live = c < npts;
+  const double k = q.kwave[f];
+  double p[3] = {0.0, 0.0, 0.0};
+  double ds = 0.0;
+  if (live) {
+    const int i1 = q.lo[t1] + (int)(c / n2), i2 = q.lo[t2] + (int)(c % n2);
+    p[ax] = q.pos[ax][side ? q.hi[ax] : q.lo[ax]];
+    p[t1] = q.pos[t1][i1];
+    p[t2] = q.pos[t2][i2];
+    ds = (side ? 1.0 : -1.0) * (q.width[t1][i1] * q.width[t2][i2]);
+  }
+  double2 acc[4];
+  for (int t = 0; t < 4; ++t) acc[t] = make_double2(0.0, 0.0);
+  for (int base = 0; base < q.n_dir; base += kFFBlock) {
+    __syncthreads();   // the previous chunk has been consumed
+    const int d = base + tid;
+    if (d < q.n_dir) {
+      for (int a = 0; a < 3; ++a) s_r[a][tid] = q.dirs[9 * (long long)d + a];
+      const double2* g = gs + 12 * ((long long)f * q.n_dir + d) + 4 * ax;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) s_g[t][tid] = g[t];
+    }
+    __syncthreads();
+    const int m = min(kFFBlock, q.n_dir - base);
+    for (int e = 0; e < m; ++e) {
+      const double ph = k * (s_r[0][e] * p[0] + s_r[1][e] * p[1] + s_r[2][e] * p[2]);
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {   // e^{-j ph} G
+        const double2 v = s_g[t][e];
+        acc[t].x += cs * v.x + sn * v.y;
+        acc[t].y += cs * v.y - sn * v.x;
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    gacc[((long long)t * q.nf + f) * npts + c] = make_double2(acc[t].x * ds, acc[t].y * ds);
+}
+
 // Direction vectors and wave numbers of a far-field transform computed on
 // the device (the allocation-free entry point has no host tables to
 // upload): dirs[9 t] = r-hat, theta-hat, phi-hat of direction t = it n_phi +
@@ -2938,6 +3032,43 @@ __global__ __launch_bounds__(256) void k_plane_adj(PlaneParams q, const double2*
   }
   const long long c = ((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2];
   tgt[c] += (float)g;
+}
+
+// Flux-monitor / NTFF-face adjoint of step n: component t of the monitor
+// (E1, E2 when e_part, else H1, H2; blockIdx.y picks one of the two) takes
+// g = sum_f Re(conj(G[t][f]) tw[n][f]) into its adjoint field tgt[y]; the
+// H components were sampled as the mean of planes index_m1 and index, so
+// each gets half (all of it when the two planes coincide).
+__global__ __launch_bounds__(256) void k_flux_adj(FluxParams q, const double2* G, long long n,
+                                                  int e_part, float* tgt0, float* tgt1) {
+  const long long npts = (long long)q.n1 * q.n2;
+  const long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx >= npts || n < 0 || n >= q.max_steps) return;
+  const int y = (int)blockIdx.y;
+  const int t = (e_part ? 0 : 2) + y;
+  float* tgt = y ? tgt1 : tgt0;
+  const int u = (int)(pidx / q.n2) + q.lo1, v = (int)(pidx % q.n2) + q.lo2;
+  int ii[3], im[3];
+  const int a1 = q.axis == 0 ? 1 : 0, a2 = q.axis == 2 ? 1 : 2;
+  ii[q.axis] = q.index;
+  im[q.axis] = q.index_m1;
+  ii[a1] = im[a1] = u;
+  ii[a2] = im[a2] = v;
+  const double2* tw = t < 2 ? q.twe : q.twh;
+  double g = 0.0;
+  for (int f = 0; f < q.nf; ++f) {
+    const double2 gf = G[((long long)t * q.nf + f) * npts + pidx];
+    const double2 w = tw[n * q.nf + f];
+    g += gf.x * w.x + gf.y * w.y;
+  }
+  const long long c = ((long long)ii[0] * q.ny + ii[1]) * q.nz + ii[2];
+  const long long cm = ((long long)im[0] * q.ny + im[1]) * q.nz + im[2];
+  if (t < 2 || cm == c) {
+    tgt[c] += (float)g;
+  } else {
+    tgt[cm] += (float)(0.5 * g);
+    tgt[c] += (float)(0.5 * g);
+  }
 }
 
 // The single-instance reverse kernels: the bodies of adj_h_cell and
@@ -4681,19 +4812,34 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
   // the prologue of their step (the sample is E^{n+1} after the injection,
   // like a probe's), planes of an H component before the H-bar pass of
   // their step
+  // (flux monitors and NTFF faces the same, with E and H parts both)
   bool e_planes = false, h_planes = false;
   if (ad->d_plane_adj)
     for (size_t q = 0; q < pl->planes.size(); ++q)
       if (ad->d_plane_adj[q]) (pl->plane_comp[q] < 3 ? e_planes : h_planes) = true;
+  if (ad->d_flux_adj)
+    for (size_t q = 0; q < pl->fluxes.size(); ++q)
+      if (ad->d_flux_adj[q]) e_planes = h_planes = true;
   auto plane_adj = [&](bool e, long long step) {
-    for (size_t q = 0; q < pl->planes.size(); ++q) {
-      const int comp = pl->plane_comp[q];
-      if (!ad->d_plane_adj[q] || (comp < 3) != e) continue;
-      const PlaneParams& pp = pl->planes[q];
-      const long long npts = (long long)pp.n1 * pp.n2;
-      hipLaunchKernelGGL(k_plane_adj, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp,
-                         reinterpret_cast<const double2*>(ad->d_plane_adj[q]), step, a.a[comp]);
-    }
+    if (ad->d_plane_adj)
+      for (size_t q = 0; q < pl->planes.size(); ++q) {
+        const int comp = pl->plane_comp[q];
+        if (!ad->d_plane_adj[q] || (comp < 3) != e) continue;
+        const PlaneParams& pp = pl->planes[q];
+        const long long npts = (long long)pp.n1 * pp.n2;
+        hipLaunchKernelGGL(k_plane_adj, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, pp,
+                           reinterpret_cast<const double2*>(ad->d_plane_adj[q]), step, a.a[comp]);
+      }
+    if (ad->d_flux_adj)
+      for (size_t q = 0; q < pl->fluxes.size(); ++q) {
+        if (!ad->d_flux_adj[q]) continue;
+        const FluxParams& fp = pl->fluxes[q];
+        const long long npts = (long long)fp.n1 * fp.n2;
+        const int* comp = &pl->flux_comp[4 * q + (e ? 0 : 2)];
+        hipLaunchKernelGGL(k_flux_adj, dim3((unsigned)((npts + 255) / 256), 2), dim3(256), 0, s,
+                           fp, reinterpret_cast<const double2*>(ad->d_flux_adj[q]), step,
+                           e ? 1 : 0, a.a[comp[0]], a.a[comp[1]]);
+      }
   };
   auto adj_e = [&](int gather, long long m) {
     if (pl->kp.graded)
@@ -5029,7 +5175,8 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   std::vector<BatchAdjEntry> all(n);
   for (int q = 0; q < n; ++q) {
     if (!descs[q]) continue;
-    if (descs[q]->d_grad_mu || descs[q]->d_port_adj || descs[q]->d_plane_adj)
+    if (descs[q]->d_grad_mu || descs[q]->d_port_adj || descs[q]->d_plane_adj ||
+        descs[q]->d_flux_adj)
       return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients of "
                                "probe objectives only");
     const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);
@@ -5071,11 +5218,16 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   return RFXH_OK;
 }
 
-int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const double* theta,
-                        int32_t n_phi, const double* phi, double* e_theta, double* e_phi,
-                        void* stream) {
-  if (!dsc || !theta || !phi || !e_theta || !e_phi || n_theta < 1 || n_phi < 1)
-    return fail(RFXH_EINVAL, "far field: missing arguments");
+}  // extern "C"
+
+// Checks of a far-field request and its device tables (positions and widths
+// per axis, directions, wave numbers) in one allocation `*blk` (extra:
+// further doubles reserved after them, at *extra_at), uploaded on stream s;
+// fills q except the accumulators and outputs.
+static int far_field_setup(const rfxh_farfield_desc* dsc, int32_t n_theta, const double* theta,
+                           int32_t n_phi, const double* phi, hipStream_t s, size_t extra,
+                           FarFieldParams& q, double** blk, double** extra_at) {
+  *blk = nullptr;
   if (dsc->n_freq < 1 || dsc->n_freq > 65535 || !dsc->freqs)
     return fail(RFXH_EINVAL, "far field: need 1..65535 frequencies");
   if ((long long)n_theta * n_phi > (1LL << 30))
@@ -5086,8 +5238,6 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const do
     if (dsc->lo[a] < 0 || dsc->hi[a] <= dsc->lo[a] || dsc->hi[a] >= dsc->n_axis[a])
       return fail(RFXH_EINVAL, "far field: need 0 <= lo < hi < n on every axis");
   }
-  for (int face = 0; face < 6; ++face)
-    if (!dsc->d_acc[face]) return fail(RFXH_EINVAL, "far field: face accumulator is NULL");
   const int n_dir = n_theta * n_phi;
   std::vector<double> dirs((size_t)n_dir * 9);
   for (int it = 0; it < n_theta; ++it) {
@@ -5106,15 +5256,15 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const do
   // directions and the wave numbers
   size_t total = dirs.size() + kw.size();
   for (int a = 0; a < 3; ++a) total += 2 * (size_t)dsc->n_axis[a];
-  double* blk = nullptr;
-  if (hipMalloc(&blk, total * sizeof(double)) != hipSuccess)
+  total = (total + 1) & ~(size_t)1;   // the extra space holds complex128: 16-byte aligned
+  if (hipMalloc(blk, (total + extra) * sizeof(double)) != hipSuccess) {
+    *blk = nullptr;
     return fail(RFXH_ENOMEM, "far field: device tables");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  FarFieldParams q;
+  }
   std::memset(&q, 0, sizeof(q));
   size_t off = 0;
   auto put = [&](const double* src, size_t n) -> const double* {
-    double* dst = blk + off;
+    double* dst = *blk + off;
     off += n;
     return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s) == hipSuccess
                ? dst : nullptr;
@@ -5126,27 +5276,82 @@ int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const do
   }
   ok = ok && (q.dirs = put(dirs.data(), dirs.size())) != nullptr;
   ok = ok && (q.kwave = put(kw.data(), kw.size())) != nullptr;
-  int rc = RFXH_OK;
-  if (!ok) {
-    rc = fail(RFXH_EHIP, "far field: table upload");
-  } else {
+  if (extra_at) *extra_at = *blk + total;
+  for (int a = 0; a < 3; ++a) {
+    q.lo[a] = dsc->lo[a];
+    q.hi[a] = dsc->hi[a];
+  }
+  q.nf = dsc->n_freq;
+  q.n_dir = n_dir;
+  q.eta0 = dsc->eta0;
+  // the caller waits for the stream before freeing *blk
+  return ok ? RFXH_OK : fail(RFXH_EHIP, "far field: table upload");
+}
+
+extern "C" {
+
+int rfxh_ntff_far_field(const rfxh_farfield_desc* dsc, int32_t n_theta, const double* theta,
+                        int32_t n_phi, const double* phi, double* e_theta, double* e_phi,
+                        void* stream) {
+  if (!dsc || !theta || !phi || !e_theta || !e_phi || n_theta < 1 || n_phi < 1)
+    return fail(RFXH_EINVAL, "far field: missing arguments");
+  for (int face = 0; face < 6; ++face)
+    if (!dsc->d_acc[face]) return fail(RFXH_EINVAL, "far field: face accumulator is NULL");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FarFieldParams q;
+  double* blk = nullptr;
+  int rc = far_field_setup(dsc, n_theta, theta, n_phi, phi, s, 0, q, &blk, nullptr);
+  if (rc == RFXH_OK) {
     for (int face = 0; face < 6; ++face) q.acc[face] = reinterpret_cast<const double2*>(dsc->d_acc[face]);
-    for (int a = 0; a < 3; ++a) {
-      q.lo[a] = dsc->lo[a];
-      q.hi[a] = dsc->hi[a];
-    }
-    q.nf = dsc->n_freq;
-    q.n_dir = n_dir;
-    q.eta0 = dsc->eta0;
     q.e_theta = reinterpret_cast<double2*>(e_theta);
     q.e_phi = reinterpret_cast<double2*>(e_phi);
-    hipLaunchKernelGGL(k_far_field, dim3((unsigned)((n_dir + kFFBlock - 1) / kFFBlock),
+    hipLaunchKernelGGL(k_far_field, dim3((unsigned)((q.n_dir + kFFBlock - 1) / kFFBlock),
                                          (unsigned)dsc->n_freq), dim3(kFFBlock), 0, s, q);
     if (hipGetLastError() != hipSuccess) rc = fail(RFXH_EHIP, "far field: launch");
   }
+  if (!blk) return rc;
   // the uploads and the kernel must finish before the tables are freed
   if (hipStreamSynchronize(s) != hipSuccess && rc == RFXH_OK)
     rc = fail(RFXH_EHIP, "far field: kernel");
+  (void)hipFree(blk);
+  return rc;
+}
+
+int rfxh_ntff_far_field_adjoint(const rfxh_farfield_desc* dsc, int32_t n_theta,
+                                const double* theta, int32_t n_phi, const double* phi,
+                                const double* g_theta, const double* g_phi,
+                                double* const* d_gacc, void* stream) {
+  if (!dsc || !theta || !phi || !g_theta || !g_phi || !d_gacc || n_theta < 1 || n_phi < 1)
+    return fail(RFXH_EINVAL, "far field adjoint: missing arguments");
+  for (int face = 0; face < 6; ++face)
+    if (!d_gacc[face]) return fail(RFXH_EINVAL, "far field adjoint: face gradient is NULL");
+  if ((long long)n_theta * n_phi > (1LL << 30))
+    return fail(RFXH_EINVAL, "far field: too many directions");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FarFieldParams q;
+  double* blk = nullptr;
+  double* gs = nullptr;
+  // scratch: twelve complex sums per (frequency, direction)
+  const size_t extra = (size_t)24 * (size_t)std::max(dsc->n_freq, 0) * (size_t)n_theta * n_phi;
+  int rc = far_field_setup(dsc, n_theta, theta, n_phi, phi, s, extra, q, &blk, &gs);
+  if (rc == RFXH_OK) {
+    hipLaunchKernelGGL(k_far_field_adj_s, dim3((unsigned)((q.n_dir + 255) / 256), (unsigned)q.nf),
+                       dim3(256), 0, s, q, reinterpret_cast<const double2*>(g_theta),
+                       reinterpret_cast<const double2*>(g_phi), reinterpret_cast<double2*>(gs));
+    for (int face = 0; face < 6; ++face) {
+      const int ax = face / 2;
+      const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+      const long long npts = (long long)(q.hi[t1] - q.lo[t1]) * (q.hi[t2] - q.lo[t2]);
+      hipLaunchKernelGGL(k_far_field_adj, dim3((unsigned)((npts + kFFBlock - 1) / kFFBlock),
+                                               (unsigned)q.nf), dim3(kFFBlock), 0, s, q,
+                         reinterpret_cast<const double2*>(gs), face,
+                         reinterpret_cast<double2*>(d_gacc[face]));
+    }
+    if (hipGetLastError() != hipSuccess) rc = fail(RFXH_EHIP, "far field adjoint: launch");
+  }
+  if (!blk) return rc;
+  if (hipStreamSynchronize(s) != hipSuccess && rc == RFXH_OK)
+    rc = fail(RFXH_EHIP, "far field adjoint: kernel");
   (void)hipFree(blk);
   return rc;
 }

@@ -134,25 +134,9 @@ def _axis_tables(box: NTFFBox, grid):
     return pos, width
 
 
-def compute_far_field(ntff_data: NTFFData, box: NTFFBox, grid, theta, phi,
-                      device=None) -> FarFieldResult:
-    """E_theta, E_phi on the (theta, phi) grid (rfx/farfield.py:393-556),
-    computed by the rfxh_ntff_far_field kernel in float64."""
-    _lib.require_device()
-    lib = _lib.load()
-    dev = torch.device(device if device is not None else "cuda")
-    theta = np.ascontiguousarray(np.asarray(theta, np.float64).ravel())
-    phi = np.ascontiguousarray(np.asarray(phi, np.float64).ravel())
-    freqs = np.ascontiguousarray(np.asarray(box.freqs, np.float64).ravel())
-    check_box(box, grid.shape)
-    nf = len(freqs)
-    accs = []
-    for name, shp in zip(FACES, face_shapes(box)):
-        x = np.asarray(getattr(ntff_data, name))
-        if x.shape != (nf,) + tuple(shp) + (4,):
-            raise ValueError(f"NTFF face {name}: shape {x.shape}, expected {(nf,) + tuple(shp) + (4,)}")
-        accs.append(torch.from_numpy(np.ascontiguousarray(
-            x.astype(np.complex128).transpose(3, 0, 1, 2))).to(dev))
+def _far_field_desc(box: NTFFBox, grid, freqs):
+    """The transform's descriptor without accumulators, and the host tables
+    it points at (kept alive by the caller for the call)."""
     pos, width = _axis_tables(box, grid)
     d = _lib.FarFieldDesc()
     for a in range(3):
@@ -160,24 +144,103 @@ def compute_far_field(ntff_data: NTFFData, box: NTFFBox, grid, theta, phi,
         d.n_axis[a] = grid.shape[a]
         d.pos[a] = pos[a].ctypes.data_as(C.POINTER(C.c_double))
         d.width[a] = width[a].ctypes.data_as(C.POINTER(C.c_double))
-    d.n_freq = nf
+    d.n_freq = len(freqs)
     d.freqs = freqs.ctypes.data_as(C.POINTER(C.c_double))
+    d.c0, d.eta0 = C0, ETA_0
+    return d, (pos, width, freqs)
+
+
+def _run_far_field(box, grid, theta, phi, freqs, accs, dev):
+    """E_theta, E_phi on the device from six device accumulators
+    [4][n_freqs][n1][n2] (rfxh_ntff_far_field)."""
+    lib = _lib.load()
+    d, keep = _far_field_desc(box, grid, freqs)
     for q in range(6):
         d.d_acc[q] = accs[q].data_ptr()
-    d.c0, d.eta0 = C0, ETA_0
-    e_th = torch.empty((nf, len(theta), len(phi)), dtype=torch.complex128, device=dev)
+    e_th = torch.empty((len(freqs), len(theta), len(phi)), dtype=torch.complex128, device=dev)
     e_ph = torch.empty_like(e_th)
     dp = C.POINTER(C.c_double)
     with torch.cuda.device(dev):
         _lib.check(lib.rfxh_ntff_far_field(
             C.byref(d), len(theta), theta.ctypes.data_as(dp), len(phi), phi.ctypes.data_as(dp),
             e_th.data_ptr(), e_ph.data_ptr(), torch.cuda.current_stream(dev).cuda_stream))
+    del keep
+    return e_th, e_ph
+
+
+class _FarFieldFn(torch.autograd.Function):
+    """The transform as a torch function of the six face accumulators: the
+    forward is rfxh_ntff_far_field, the backward its conjugate transpose
+    (rfxh_ntff_far_field_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, setup, *accs):
+        box, grid, theta, phi, freqs, dev = setup
+        ctx.setup = setup
+        ctx.shapes = [a.shape for a in accs]
+        ctx.set_materialize_grads(False)
+        return _run_far_field(box, grid, theta, phi, freqs, [a.contiguous() for a in accs], dev)
+
+    @staticmethod
+    def backward(ctx, g_th, g_ph):
+        box, grid, theta, phi, freqs, dev = ctx.setup
+        shape = (len(freqs), len(theta), len(phi))
+        zero = torch.zeros(shape, dtype=torch.complex128, device=dev)
+        g_th = zero if g_th is None else g_th.to(dev, torch.complex128).contiguous()
+        g_ph = zero if g_ph is None else g_ph.to(dev, torch.complex128).contiguous()
+        out = [torch.empty(sh, dtype=torch.complex128, device=dev) for sh in ctx.shapes]
+        ptrs = (C.c_void_p * 6)(*[o.data_ptr() for o in out])
+        lib = _lib.load()
+        d, keep = _far_field_desc(box, grid, freqs)
+        dp = C.POINTER(C.c_double)
+        with torch.cuda.device(dev):
+            _lib.check(lib.rfxh_ntff_far_field_adjoint(
+                C.byref(d), len(theta), theta.ctypes.data_as(dp), len(phi),
+                phi.ctypes.data_as(dp), g_th.data_ptr(), g_ph.data_ptr(),
+                C.cast(ptrs, C.POINTER(C.c_void_p)), torch.cuda.current_stream(dev).cuda_stream))
+        del keep
+        return (None, *out)
+
+
+def compute_far_field(ntff_data: NTFFData, box: NTFFBox, grid, theta, phi,
+                      device=None) -> FarFieldResult:
+    """E_theta, E_phi on the (theta, phi) grid (rfx/farfield.py:393-556),
+    computed by the rfxh_ntff_far_field kernel in float64.  Face data held
+    as torch tensors (``forward()`` with a gradient requested gives such
+    data) give torch results on their device, differentiable through the
+    transform's adjoint (rfxh_ntff_far_field_adjoint)."""
+    _lib.require_device()
+    theta = np.ascontiguousarray(np.asarray(theta, np.float64).ravel())
+    phi = np.ascontiguousarray(np.asarray(phi, np.float64).ravel())
+    freqs = np.ascontiguousarray(np.asarray(box.freqs, np.float64).ravel())
+    check_box(box, grid.shape)
+    nf = len(freqs)
+    faces = [getattr(ntff_data, name) for name in FACES]
+    for name, shp, x in zip(FACES, face_shapes(box), faces):
+        if tuple(x.shape) != (nf,) + tuple(shp) + (4,):
+            raise ValueError(f"NTFF face {name}: shape {tuple(x.shape)}, "
+                             f"expected {(nf,) + tuple(shp) + (4,)}")
+    if any(isinstance(x, torch.Tensor) for x in faces):
+        dev = torch.device(device) if device is not None else next(
+            x.device for x in faces if isinstance(x, torch.Tensor))
+        accs = [torch.as_tensor(x).to(dev, torch.complex128).permute(3, 0, 1, 2) for x in faces]
+        e_th, e_ph = _FarFieldFn.apply((box, grid, theta, phi, freqs, dev), *accs)
+        return FarFieldResult(e_th, e_ph, theta, phi, freqs)
+    dev = torch.device(device if device is not None else "cuda")
+    accs = [torch.from_numpy(np.ascontiguousarray(
+        np.asarray(x).astype(np.complex128).transpose(3, 0, 1, 2))).to(dev) for x in faces]
+    e_th, e_ph = _run_far_field(box, grid, theta, phi, freqs, accs, dev)
     return FarFieldResult(e_th.cpu().numpy(), e_ph.cpu().numpy(), theta, phi, freqs)
 
 
 def radiation_pattern(ff: FarFieldResult) -> np.ndarray:
     """Normalised pattern in dB, (n_freqs, n_theta, n_phi)
-    (rfx/farfield.py:769-777)."""
+    (rfx/farfield.py:769-777); torch results give a torch tensor."""
+    if isinstance(ff.E_theta, torch.Tensor):
+        power = ff.E_theta.abs() ** 2 + ff.E_phi.abs() ** 2
+        peak = power.amax(dim=(1, 2), keepdim=True)
+        ratio = power / torch.where(peak > 0, peak, torch.ones_like(peak))
+        return 10 * torch.log10(torch.clamp(ratio, min=1e-10))
     power = np.abs(ff.E_theta) ** 2 + np.abs(ff.E_phi) ** 2
     peak = np.max(power, axis=(1, 2), keepdims=True)
     return 10 * np.log10(np.maximum(power / np.where(peak > 0, peak, 1.0), 1e-10))
@@ -186,10 +249,18 @@ def radiation_pattern(ff: FarFieldResult) -> np.ndarray:
 def directivity(ff: FarFieldResult) -> np.ndarray:
     """D = 4 pi U_max / P_rad in dBi per frequency, P_rad by the sampled
     sum of U sin(theta) dtheta dphi with np.gradient spacings
-    (rfx/farfield.py:780-802)."""
-    power = np.abs(ff.E_theta) ** 2 + np.abs(ff.E_phi) ** 2
+    (rfx/farfield.py:780-802); torch results give a torch tensor."""
     dth = np.gradient(ff.theta) if len(ff.theta) > 1 else np.array([np.pi])
     dph = np.gradient(ff.phi) if len(ff.phi) > 1 else np.array([2 * np.pi])
+    if isinstance(ff.E_theta, torch.Tensor):
+        power = ff.E_theta.abs() ** 2 + ff.E_phi.abs() ** 2
+        w = torch.as_tensor(np.sin(ff.theta)[:, None] * dth[:, None] * dph[None, :],
+                            device=power.device)
+        p_rad = torch.sum(power * w[None], dim=(1, 2))
+        u_max = power.amax(dim=(1, 2))
+        d = 4 * np.pi * u_max / torch.where(p_rad > 0, p_rad, torch.ones_like(p_rad))
+        return 10 * torch.log10(torch.clamp(d, min=1e-10))
+    power = np.abs(ff.E_theta) ** 2 + np.abs(ff.E_phi) ** 2
     p_rad = np.sum(power * np.sin(ff.theta)[None, :, None] * dth[None, :, None]
                    * dph[None, None, :], axis=(1, 2))
     u_max = np.max(power, axis=(1, 2))
