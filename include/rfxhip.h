@@ -247,6 +247,14 @@ int rfxh_run_record(rfxh_plan* plan, int64_t step0, int64_t n_steps, const rfxh_
  * mu_r_override, rfx/api/_execute.py:2800-2805). */
 int rfxh_run_record_h(rfxh_plan* plan, int64_t step0, int64_t n_steps, const rfxh_region* region,
                       float* d_rec, float* d_rec_h, void* stream);
+/* rfxh_run_record_h plus, for plans with Debye / Lorentz poles (required
+ * there, ignored elsewhere), d_rec_k[n][3][region]: the dispersive part of
+ * the E update, K^n = sum_d cc_d P_d^n - cc_L sum_l (P_l^{n+1} - P_l^n)
+ * (zero at cells without poles), which the eps_r and sigma gradients of
+ * dispersive cells need (the reference differentiates its ADE,
+ * rfx/materials/debye.py:182-250 and lorentz.py:200-275, with jax.grad). */
+int rfxh_run_record_ex(rfxh_plan* plan, int64_t step0, int64_t n_steps, const rfxh_region* region,
+                       float* d_rec, float* d_rec_h, float* d_rec_k, void* stream);
 
 typedef struct {
   float* d_adj[6];           /* adjoint fields E-bar x,y,z, H-bar x,y,z; zeroed before the first call */
@@ -285,6 +293,16 @@ typedef struct {
    * (rfx/farfield.py:208-313 and the flux DFT of rfx/api/__init__.py:
    * 2577-2645, differentiated through the scan). */
   const double* const* d_flux_adj;
+  /* Plans with Debye / Lorentz poles: the adjoint polarisation, device
+   * float32 laid out like the plan's polarisation state (d_pol), zeroed by
+   * the caller before the first (latest) segment and carried between
+   * segments; with d_grad also the dispersive record of rfxh_run_record_ex
+   * (rows as d_rec) and per material id the host pair (-(d gamma/d eps_r)/
+   * gamma, -(d gamma/d sigma)/gamma) of the ADE normaliser gamma = eps_inf +
+   * sum_d beta_d + sigma dt/2. */
+  float* d_pol_adj;
+  const float* d_rec_k;
+  const float* dcoef_disp;
 } rfxh_adjoint_desc;
 
 /* Reverse sweep over steps hi-1 down to lo.  The adjoint state carries

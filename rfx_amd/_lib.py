@@ -100,6 +100,7 @@ class AdjointDesc(C.Structure):
         ("d_grad_sigma", _P), ("dcoef_sigma", _FP),
         ("d_rec_h", _P), ("d_grad_mu", _P), ("dcoef_mu", _FP),
         ("d_port_adj", _P), ("d_plane_adj", C.POINTER(_P)), ("d_flux_adj", C.POINTER(_P)),
+        ("d_pol_adj", _P), ("d_rec_k", _P), ("dcoef_disp", _FP),
     ]
 
 
@@ -112,11 +113,12 @@ _lib = None
 
 EXPORTS = ("rfxh_psi_floats", "rfxh_plan_create", "rfxh_plan_destroy", "rfxh_run",
            "rfxh_state_set", "rfxh_set_state_set", "rfxh_set_timing", "rfxh_kernel_time_ms",
-           "rfxh_run_record", "rfxh_run_record_h", "rfxh_adjoint", "rfxh_batch_create",
+           "rfxh_run_record", "rfxh_run_record_h", "rfxh_run_record_ex", "rfxh_adjoint", "rfxh_batch_create",
            "rfxh_batch_destroy",
            "rfxh_batch_run", "rfxh_batch_run_record", "rfxh_batch_adjoint", "rfxh_set_variant",
            "rfxh_get_variant", "rfxh_ntff_far_field",
-           "rfxh_ntff_far_field_noalloc", "rfxh_ntff_far_field_adjoint", "rfxh_assemble_fill", "rfxh_assemble_pec_pairs",
+           "rfxh_ntff_far_field_noalloc", "rfxh_ntff_far_field_adjoint", "rfxh_assemble_fill",
+           "rfxh_assemble_pec_pairs",
            "rfxh_assemble_pec_apply", "rfxh_last_error", "rfxh_version")
 
 
@@ -154,6 +156,8 @@ def load():
     lib.rfxh_run_record.restype = C.c_int
     lib.rfxh_run_record_h.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(Region), _P, _P, _P]
     lib.rfxh_run_record_h.restype = C.c_int
+    lib.rfxh_run_record_ex.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(Region), _P, _P, _P, _P]
+    lib.rfxh_run_record_ex.restype = C.c_int
     lib.rfxh_adjoint.argtypes = [_P, C.POINTER(AdjointDesc), C.c_int64, C.c_int64, _P]
     lib.rfxh_adjoint.restype = C.c_int
     lib.rfxh_batch_create.argtypes = [C.POINTER(_P), C.c_int, C.POINTER(_P)]

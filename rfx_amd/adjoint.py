@@ -99,6 +99,40 @@ def sigma_derivative_table(mat, dt: float) -> np.ndarray:
     return np.ascontiguousarray(np.stack([dca, dcb], axis=1).astype(np.float32))
 
 
+def dispersive_derivative_tables(prob, dt: float):
+    """Derivative tables of a plan with Debye / Lorentz poles (float64 ->
+    float32), replacing eps_derivative_table / sigma_derivative_table.
+
+    With eps_inf = eps_r eps0, h = sigma dt/2, B = sum_d beta_d of the
+    material's Debye poles, gamma = eps_inf + B + h and the dispersive ca =
+    (eps_inf - B - h)/gamma, cb = dt/gamma (the coefficients of
+    rfx_amd.materials.dispersion_tables, rfx/materials/debye.py:76-179 and
+    lorentz.py:103-198; every pole-free material has B = 0 and the plain
+    values): d ca/d eps_r = 2 eps0 (B + h)/gamma^2, d cb/d eps_r = -dt
+    eps0/gamma^2, d ca/d sigma = -(dt/2)(gamma + eps_inf - B - h)/gamma^2,
+    d cb/d sigma = -dt (dt/2)/gamma^2; ce = dt/(eps_r eps0) as without
+    poles.  Every pole's cc is a constant over gamma, so the ADE part K of
+    the E update has dK/dx = -(gamma'(x)/gamma) K: the third table holds
+    (-eps0/gamma, -(dt/2)/gamma).  Returns (eps table [n_mat][3], sigma
+    table [n_mat][2], K table [n_mat][2])."""
+    from .materials import dispersion_tables
+    mat = prob.mat
+    _, _, rows = dispersion_tables(mat, dt, prob.poles, prob.lorentz)
+    eps_r, sig, _, _ = mat.props()
+    e = eps_r.astype(np.float64)
+    einf = e * EPS_0
+    h = sig.astype(np.float64) * dt / 2.0
+    nd = len(prob.poles)
+    B = rows[:, :nd, 1].astype(np.float64).sum(axis=1) if nd else np.zeros_like(e)
+    g = einf + B + h
+    num = einf - B - h
+    ce = dt / einf
+    d_eps = np.stack([2.0 * EPS_0 * (B + h) / g ** 2, -dt * EPS_0 / g ** 2, -ce / e], axis=1)
+    d_sig = np.stack([-(dt / 2.0) * (g + num) / g ** 2, -dt * (dt / 2.0) / g ** 2], axis=1)
+    d_k = np.stack([-EPS_0 / g, -(dt / 2.0) / g], axis=1)
+    return tuple(np.ascontiguousarray(t.astype(np.float32)) for t in (d_eps, d_sig, d_k))
+
+
 def mu_derivative_table(mat, dt: float) -> np.ndarray:
     """Per material id: d ch / d mu_r with ch = dt / (mu_r mu0), i.e. -ch/mu_r
     (float64 -> float32)."""
@@ -128,8 +162,11 @@ class AdjointRunner:
         self.box = (i0, i1, j0, j1, k0, k1)
         self.region = _lib.Region(i0, j0, k0, i1 - i0, j1 - j0, k1 - k0)
         self.nreg = (i1 - i0) * (j1 - j0) * (k1 - k0)
-        per_step = (12 if want_mu else 9) * 4 * self.nreg
+        self.disp = bool(prob.poles or prob.lorentz)
+        per_step = (9 + (3 if want_mu else 0) + (3 if self.disp else 0)) * 4 * self.nreg
         state = 4 * (6 * int(np.prod(shape)) + int(self.dr.psis[0].numel()))
+        if self.dr.pol is not None:
+            state += 4 * int(self.dr.pol.numel())
         budget = int(record_budget)
         if self.dr.device.type == "cuda":
             free, _ = torch.cuda.mem_get_info(self.dr.device)
@@ -138,12 +175,18 @@ class AdjointRunner:
         self.seg_len = math.ceil(self.n_steps / nseg)
         self.bounds = [(lo, min(self.n_steps, lo + self.seg_len))
                        for lo in range(0, self.n_steps, self.seg_len)]
-        self.dcoef = eps_derivative_table(prob.mat, prob.dt)
-        self.dcoef_sigma = sigma_derivative_table(prob.mat, prob.dt) if want_sigma else None
+        self.dcoef_disp = None
+        if self.disp:
+            self.dcoef, dsig, self.dcoef_disp = dispersive_derivative_tables(prob, prob.dt)
+            self.dcoef_sigma = dsig if want_sigma else None
+        else:
+            self.dcoef = eps_derivative_table(prob.mat, prob.dt)
+            self.dcoef_sigma = sigma_derivative_table(prob.mat, prob.dt) if want_sigma else None
         self.grad_sigma = None
         self.dcoef_mu = mu_derivative_table(prob.mat, prob.dt) if want_mu else None
         self.grad_mu = None
         self.rec_h = None
+        self.rec_k = None
         self.rec = None
         self.checkpoints = []
 
@@ -156,18 +199,24 @@ class AdjointRunner:
             if self.dcoef_mu is not None:
                 self.rec_h = torch.empty((rows, 3, self.nreg), dtype=torch.float32,
                                          device=self.dr.device)
+            if self.disp:
+                self.rec_k = torch.empty((rows, 3, self.nreg), dtype=torch.float32,
+                                         device=self.dr.device)
         rec_h = self.rec_h.data_ptr() if self.rec_h is not None else None
+        rec_k = self.rec_k.data_ptr() if self.rec_k is not None else None
         with torch.cuda.device(self.dr.device):
-            _lib.check(self.lib.rfxh_run_record_h(
+            _lib.check(self.lib.rfxh_run_record_ex(
                 self.dr.plan, lo, hi - lo, C.byref(self.region), self.rec.data_ptr(), rec_h,
-                self.dr.stream_ptr()))
+                rec_k, self.dr.stream_ptr()))
 
     def forward(self) -> torch.Tensor:
         if len(self.bounds) == 1:
             self._record(0, self.n_steps)
         else:
             for lo, hi in self.bounds:
-                self.checkpoints.append(([f.clone() for f in self.dr.fields], self.dr.psi.clone()))
+                pol = self.dr.pol.clone() if self.dr.pol is not None else None
+                self.checkpoints.append(([f.clone() for f in self.dr.fields], self.dr.psi.clone(),
+                                         pol))
                 self.dr.run(lo, hi - lo, use_graph=True)
         return self.dr.time_series().clone()
 
@@ -198,15 +247,21 @@ class AdjointRunner:
                            self.dcoef, self.dcoef_sigma, self.dcoef_mu, g_ports, g_planes,
                            g_fluxes)
         d = st.desc
+        if self.dcoef_disp is not None:
+            d.dcoef_disp = self.dcoef_disp.ctypes.data_as(C.POINTER(C.c_float))
         for s_idx in range(len(self.bounds) - 1, -1, -1):
             lo, hi = self.bounds[s_idx]
             if len(self.bounds) > 1:
-                fields, psi = self.checkpoints[s_idx]
+                fields, psi, pol = self.checkpoints[s_idx]
                 self.dr.load_state(fields, psi)
+                if pol is not None:
+                    self.dr.pol.copy_(pol)
                 self._record(lo, hi)
             d.d_rec = self.rec.data_ptr()
             if self.rec_h is not None:
                 d.d_rec_h = self.rec_h.data_ptr()
+            if self.rec_k is not None:
+                d.d_rec_k = self.rec_k.data_ptr()
             d.rec_step0 = lo
             d.rec_rows = hi - lo
             with torch.cuda.device(self.dr.device):
@@ -275,6 +330,10 @@ class _ReverseState:
             pa[:T, :n_port] = g_ports.to(dev, torch.float32).reshape(T, n_port, 2)
             self.port_adj = pa
             d.d_port_adj = pa.data_ptr()
+        self.pol_adj = None
+        if dr.pol is not None:
+            self.pol_adj = torch.zeros_like(dr.pol)
+            d.d_pol_adj = self.pol_adj.data_ptr()
         self.acc_adj = []     # keeps the gradient tensors the descriptor points at
         self._ptr_tables = []
         if g_planes is not None and any(g is not None for g in g_planes):

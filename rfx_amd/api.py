@@ -812,9 +812,6 @@ class Simulation:
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),
                                                 s11)
-        if needs_grad and (prob.poles or prob.lorentz):
-            raise NotImplementedError("gradients through dispersive materials are not supported "
-                                      "(the reverse mode covers non-dispersive problems)")
         port_t = plane_t = flux_t = None
         if needs_grad:
             from .adjoint import forward_with_grad

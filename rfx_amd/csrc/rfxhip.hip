@@ -2510,6 +2510,15 @@ struct AdjParams {
   float* grad_mu;         // [region n] dJ/d mu_r (null: not computed)
   const float* dmu;       // per material id: d ch / d mu_r
   const float* port_adj;  // [max_steps][n_port][2] dJ/dV, dJ/dI (null: none)
+  // dispersive plans: the adjoint polarisation (the forward state's layout),
+  // the plan's pole rows and box, the dispersive record [rec_rows][3][region
+  // n] and per material (-gamma'/gamma w.r.t. eps_r, w.r.t. sigma)
+  float* pol_adj;
+  const float* pcoef;
+  const float* rec_k;
+  const float2* dkdis;
+  int n_pole, n_debye;
+  int pol_lo[3], pol_n[3];
 };
 
 __device__ __forceinline__ long long region_index(const RegionParams& r, int i, int j, int k) {
@@ -2653,17 +2662,62 @@ __device__ __forceinline__ void record_cell(const KParams& p, const PsiIO& ps,
     if (v < nv) rec[(long long)v * r.n + q] = vals[v];
 }
 
+// The dispersive part of the E update at one cell and step n, from the state
+// before the E pass (E^n, P_d^n, P_l^n, P_l^{n-1}): K = sum_d cc_d P_d^n -
+// cc_L sum_l (P_l^{n+1} - P_l^n), P_l^{n+1} = a P_l^n + b P_l^{n-1} + c E^n
+// (dispersion_cell's terms); zero for a cell without poles.  Every cc is
+// 1/gamma times a constant, so d K / d(eps_r or sigma) = -(gamma'/gamma) K.
+__device__ void disp_k_cell(const KParams& p, const PsiIO& ps, int i, int j, int k, unsigned c,
+                            float* out, long long stride) {
+  float k0 = 0.f, k1 = 0.f, k2 = 0.f;
+  const int li = i - ps.pol_lo[0], lj = j - ps.pol_lo[1], lk = k - ps.pol_lo[2];
+  const unsigned m = ldu16(p.mat, c);
+  if ((p.flags[m] & RFXH_POLES) && ps.pol && li >= 0 && lj >= 0 && lk >= 0 &&
+      li < ps.pol_n[0] && lj < ps.pol_n[1] && lk < ps.pol_n[2]) {
+    const unsigned R = (unsigned)ps.pol_n[0] * (unsigned)ps.pol_n[1] * (unsigned)ps.pol_n[2];
+    const unsigned r = ((unsigned)li * (unsigned)ps.pol_n[1] + (unsigned)lj) *
+                       (unsigned)ps.pol_n[2] + (unsigned)lk;
+    const float* pc = ps.pcoef + (size_t)m * ps.n_pole * 4;
+    const int nd = ps.n_debye;
+    const float e0 = ldf(p.f[0], c), e1 = ldf(p.f[1], c), e2 = ldf(p.f[2], c);
+    for (int q = 0; q < nd; ++q) {
+      const float cc = pc[4 * q + 2];
+      const float* P = ps.pol + (size_t)q * 3 * R;
+      k0 += cc * P[r];
+      k1 += cc * P[R + r];
+      k2 += cc * P[2 * R + r];
+    }
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f, ccl = 0.f;
+    for (int q = nd; q < ps.n_pole; ++q) {
+      const float a = pc[4 * q], b = pc[4 * q + 1], cl = pc[4 * q + 2];
+      ccl = pc[4 * q + 3];
+      const float* P = ps.pol + (size_t)nd * 3 * R + (size_t)(q - nd) * 6 * R;
+      d0 += a * P[r] + b * P[3 * R + r] + cl * e0 - P[r];
+      d1 += a * P[R + r] + b * P[4 * R + r] + cl * e1 - P[R + r];
+      d2 += a * P[2 * R + r] + b * P[5 * R + r] + cl * e2 - P[2 * R + r];
+    }
+    k0 -= ccl * d0;
+    k1 -= ccl * d1;
+    k2 -= ccl * d2;
+  }
+  out[0] = k0;
+  out[stride] = k1;
+  out[2 * stride] = k2;
+}
+
 template <bool GRADED>
 __global__ __launch_bounds__(256) void k_record(KParams p, PsiIO ps, RegionParams r, float* rec,
-                                                float* rec_h) {
+                                                float* rec_h, float* rec_k) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= r.n) return;
   record_cell<GRADED>(p, ps, r, q, rec);
-  if (rec_h) {
-    const int c3 = (int)(q % r.nk);
-    const long long t = q / r.nk;
-    record_h_cell<GRADED>(p, ps, r, q, r.i0 + (int)(t / r.nj), r.j0 + (int)(t % r.nj), r.k0 + c3,
-                          rec_h);
+  const int c3 = (int)(q % r.nk);
+  const long long t = q / r.nk;
+  const int i = r.i0 + (int)(t / r.nj), j = r.j0 + (int)(t % r.nj), k = r.k0 + c3;
+  if (rec_h) record_h_cell<GRADED>(p, ps, r, q, i, j, k, rec_h);
+  if (rec_k) {
+    const unsigned c = ((unsigned)i * (unsigned)p.ny + (unsigned)j) * (unsigned)p.nz + (unsigned)k;
+    disp_k_cell(p, ps, i, j, k, c, rec_k + q, r.n);
   }
 }
 
@@ -3206,7 +3260,19 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
   stf(a.a[5], c, hz);
 }
 
-template <bool GRADED>
+// Row of a cell with poles in the adjoint polarisation box (-1: none); R =
+// the box's cell count.
+__device__ __forceinline__ long long adj_pol_row(const AdjParams& a, int i, int j, int k,
+                                                 uint32_t fl, unsigned& R) {
+  const int li = i - a.pol_lo[0], lj = j - a.pol_lo[1], lk = k - a.pol_lo[2];
+  R = (unsigned)a.pol_n[0] * (unsigned)a.pol_n[1] * (unsigned)a.pol_n[2];
+  if (!(fl & RFXH_POLES) || !a.pol_adj || li < 0 || lj < 0 || lk < 0 || li >= a.pol_n[0] ||
+      lj >= a.pol_n[1] || lk >= a.pol_n[2])
+    return -1;
+  return ((long long)li * a.pol_n[1] + lj) * a.pol_n[2] + lk;
+}
+
+template <bool GRADED, bool DISP = false>
 __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gather, long long m) {
   int i, j, k;
   if (!cell_of(p, i, j, k)) return;
@@ -3215,6 +3281,7 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
   const unsigned mid0 = ldu16(p.mat, c);
   const float4 co = p.coef[mid0];
   float ex = ldf(a.a[0], c), ey = ldf(a.a[1], c), ez = ldf(a.a[2], c);
+  const float bx = ex, by = ey, bz = ez;   // adjoint of E^{m+2} before the E update's transpose
   if (gather) {
     const bool hj = j > 0, hk = k > 0, hi = i > 0;
     float ix, iy, iz, ixp, iyp, izp;  // H-update spacings at the cell and its backward neighbour
@@ -3276,6 +3343,47 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
           ey += d0;
           ex += d1;
         }
+      }
+    }
+    if (DISP) {
+      // transpose of the ADE (dispersion_cell) at this cell: with e-bar the
+      // adjoint of the new E before the CPML term, P-bar / Q-bar those of
+      // the new polarisation: Debye P' = alpha P + beta (E' + E) and E' +=
+      // cc P give E-bar += beta P-bar', P-bar = alpha P-bar' + cc e-bar;
+      // Lorentz P' = a P + b Q + c E, Q' = P and E' -= ccl (P' - P) give
+      // E-bar += c P-bar' - ccl c e-bar, P-bar = a P-bar' + Q-bar' - ccl
+      // (a - 1) e-bar, Q-bar = b P-bar' - ccl b e-bar
+      unsigned R;
+      const long long r = adj_pol_row(a, i, j, k, p.flags[mid0], R);
+      if (r >= 0) {
+        const float* pc = a.pcoef + (size_t)mid0 * a.n_pole * 4;
+        const int nd = a.n_debye;
+        const float eb[3] = {bx, by, bz};
+        float ed[3] = {0.f, 0.f, 0.f};
+        for (int q = 0; q < nd; ++q) {
+          const float al = pc[4 * q], be = pc[4 * q + 1], cc = pc[4 * q + 2];
+          float* Pb = a.pol_adj + (size_t)q * 3 * R;
+#pragma unroll
+          for (int v = 0; v < 3; ++v) {
+            const float pb = Pb[(size_t)v * R + r];
+            ed[v] += be * pb;
+            Pb[(size_t)v * R + r] = al * pb + cc * eb[v];
+          }
+        }
+        for (int q = nd; q < a.n_pole; ++q) {
+          const float A = pc[4 * q], B = pc[4 * q + 1], cl = pc[4 * q + 2], ccl = pc[4 * q + 3];
+          float* Pb = a.pol_adj + (size_t)nd * 3 * R + (size_t)(q - nd) * 6 * R;
+#pragma unroll
+          for (int v = 0; v < 3; ++v) {
+            const float pb = Pb[(size_t)v * R + r], qb = Pb[(size_t)(3 + v) * R + r];
+            ed[v] += cl * pb - ccl * cl * eb[v];
+            Pb[(size_t)v * R + r] = A * pb + qb - ccl * (A - 1.f) * eb[v];
+            Pb[(size_t)(3 + v) * R + r] = B * pb - ccl * B * eb[v];
+          }
+        }
+        ex += ed[0];
+        ey += ed[1];
+        ez += ed[2];
       }
     }
   }
@@ -3355,6 +3463,23 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
         stf(a.pe[face][1], q, bb * t1);
       }
     }
+    // the CPML term's adjoint is E-bar as it stands; the Debye poles read
+    // the new E before that term: E-bar of it adds sum_d beta_d P-bar_d
+    const float fx = ex, fy = ey, fz = ez;
+    if (DISP) {
+      unsigned R;
+      const long long r = adj_pol_row(a, i, j, k, fl, R);
+      if (r >= 0) {
+        const float* pc = a.pcoef + (size_t)mid0 * a.n_pole * 4;
+        for (int q = 0; q < a.n_debye; ++q) {
+          const float be = pc[4 * q + 1];
+          const float* Pb = a.pol_adj + (size_t)q * 3 * R;
+          ex += be * Pb[r];
+          ey += be * Pb[R + r];
+          ez += be * Pb[2 * (size_t)R + r];
+        }
+      }
+    }
     // eps_r gradient from the forward record of step m
     if (a.grad) {
       const long long r = region_index(a.reg, i, j, k);
@@ -3365,13 +3490,18 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
         const float A = ex * rr[0] + ey * rr[N] + ez * rr[2 * N];
         const float B = ex * rr[3 * N] + ey * rr[4 * N] + ez * rr[5 * N];
         // the record holds the CPML-E rows only for slab cells
-        const float C = in_slab(p, i, j, k) ? ex * rr[6 * N] + ey * rr[7 * N] + ez * rr[8 * N]
+        const float C = in_slab(p, i, j, k) ? fx * rr[6 * N] + fy * rr[7 * N] + fz * rr[8 * N]
                                             : 0.f;
         const float4 dc = a.dcoef[mid0];
-        a.grad[r] += dc.x * A + dc.y * B + dc.z * C;
+        float K = 0.f;
+        if (DISP && a.rec_k) {
+          const float* rk = a.rec_k + row * 3 * N + r;
+          K = ex * rk[0] + ey * rk[N] + ez * rk[2 * N];
+        }
+        a.grad[r] += dc.x * A + dc.y * B + dc.z * C + (DISP ? a.dkdis[mid0].x * K : 0.f);
         if (a.grad_s) {
           const float2 ds = a.dcoef_s[mid0];
-          a.grad_s[r] += ds.x * A + ds.y * B;
+          a.grad_s[r] += ds.x * A + ds.y * B + (DISP ? a.dkdis[mid0].y * K : 0.f);
         }
       }
     }
@@ -3512,6 +3642,7 @@ struct rfxh_plan {
   std::vector<hipEvent_t> tev;
   float4* dcoef = nullptr;      // per material d(ca, cb, ce)/d eps_r (reverse mode)
   float2* dcoef_s = nullptr;    // per material d(ca, cb)/d sigma (reverse mode)
+  float2* dkdis = nullptr;      // per material -gamma'/gamma of the ADE (reverse mode)
   float* dmu = nullptr;         // per material d ch / d mu_r (reverse mode)
   int n_mat = 0;
 };
@@ -4676,10 +4807,17 @@ int rfxh_run_record(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_re
 
 int rfxh_run_record_h(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
                       float* d_rec, float* d_rec_h, void* stream) {
+  return rfxh_run_record_ex(pl, step0, n_steps, region, d_rec, d_rec_h, nullptr, stream);
+}
+
+int rfxh_run_record_ex(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_region* region,
+                       float* d_rec, float* d_rec_h, float* d_rec_k, void* stream) {
   if (!pl) return fail(RFXH_EINVAL, "null plan");
-  if (pl->n_pole > 0 || pl->aniso)
-    return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode covers "
-                             "plain isotropic plans only");
+  if (pl->aniso)
+    return fail(RFXH_EINVAL, "subpixel-smoothed materials: the reverse mode covers isotropic "
+                             "plans only");
+  if (pl->n_pole > 0 && !d_rec_k)
+    return fail(RFXH_EINVAL, "dispersive plan: the dispersive record (d_rec_k) is missing");
   if (step0 < 0 || n_steps < 0 || step0 + n_steps > pl->max_steps)
     return fail(RFXH_EINVAL, "steps outside the source/probe tables (max_steps)");
   int rc = check_region(pl, region);
@@ -4697,12 +4835,14 @@ int rfxh_run_record_h(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_
   const unsigned rb = (unsigned)((rp.n + 255) / 256);
   for (int64_t n = 0; n < n_steps; ++n) {
     launch_pass(pl, s, kp, ps, 0, pl->cur);
+    float* rh = d_rec_h ? d_rec_h + n * 3 * rp.n : nullptr;
+    float* rk = (d_rec_k && pl->n_pole > 0) ? d_rec_k + n * 3 * rp.n : nullptr;
     if (kp.graded)
       hipLaunchKernelGGL(k_record<true>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n,
-                         d_rec_h ? d_rec_h + n * 3 * rp.n : nullptr);
+                         rh, rk);
     else
       hipLaunchKernelGGL(k_record<false>, dim3(rb), dim3(256), 0, s, kp, ps, rp, d_rec + n * 9 * rp.n,
-                         d_rec_h ? d_rec_h + n * 3 * rp.n : nullptr);
+                         rh, rk);
     launch_pass(pl, s, kp, ps, 1, pl->cur);
     launch_accumulators(pl, s, pl->cur);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, pl->kp.step);
@@ -4717,9 +4857,13 @@ int rfxh_run_record_h(rfxh_plan* pl, int64_t step0, int64_t n_steps, const rfxh_
 static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi,
                           AdjParams* out) {
   if (!pl || !ad) return fail(RFXH_EINVAL, "null plan or adjoint descriptor");
-  if (pl->n_pole > 0 || pl->aniso)
-    return fail(RFXH_EINVAL, "dispersive or subpixel-smoothed materials: the reverse mode covers "
-                             "plain isotropic plans only");
+  if (pl->aniso)
+    return fail(RFXH_EINVAL, "subpixel-smoothed materials: the reverse mode covers isotropic "
+                             "plans only");
+  if (pl->n_pole > 0 && !ad->d_pol_adj)
+    return fail(RFXH_EINVAL, "dispersive plan: the adjoint polarisation (d_pol_adj) is missing");
+  if (pl->n_pole > 0 && ad->d_grad && (!ad->d_rec_k || !ad->dcoef_disp))
+    return fail(RFXH_EINVAL, "dispersive plan: the dispersive record or its table is missing");
   if (lo < 0 || hi < lo || hi > pl->max_steps) return fail(RFXH_EINVAL, "bad step range");
   for (int c = 0; c < 6; ++c)
     if (!ad->d_adj[c]) return fail(RFXH_EINVAL, "adjoint field pointer missing");
@@ -4762,6 +4906,27 @@ static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo
   a.probe_adj = ad->d_probe_adj;
   a.src_adj = ad->d_src_adj;
   a.port_adj = ad->d_port_adj;
+  if (pl->n_pole > 0) {
+    a.pol_adj = ad->d_pol_adj;
+    a.pcoef = pl->pcoef;
+    a.n_pole = pl->n_pole;
+    a.n_debye = pl->n_debye;
+    for (int c = 0; c < 3; ++c) {
+      a.pol_lo[c] = pl->pol_lo[c];
+      a.pol_n[c] = pl->pol_n[c];
+    }
+    if (ad->d_grad) {
+      if (!pl->dkdis) {
+        std::vector<float2> tmp(pl->n_mat);
+        for (int m = 0; m < pl->n_mat; ++m)
+          tmp[m] = make_float2(ad->dcoef_disp[2 * m], ad->dcoef_disp[2 * m + 1]);
+        int rc = dev_copy(pl, tmp.data(), tmp.size(), &pl->dkdis);
+        if (rc) return rc;
+      }
+      a.rec_k = ad->d_rec_k;
+      a.dkdis = pl->dkdis;
+    }
+  }
   if (ad->d_grad) {
     if (!pl->dcoef) {
       std::vector<float4> tmp(pl->n_mat);
@@ -4841,11 +5006,19 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
                            e ? 1 : 0, a.a[comp[0]], a.a[comp[1]]);
       }
   };
+  const bool disp = pl->n_pole > 0;
   auto adj_e = [&](int gather, long long m) {
-    if (pl->kp.graded)
-      hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
-    else
-      hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+    if (pl->kp.graded) {
+      if (disp)
+        hipLaunchKernelGGL((k_adj_e<true, true>), pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+      else
+        hipLaunchKernelGGL(k_adj_e<true>, pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+    } else {
+      if (disp)
+        hipLaunchKernelGGL((k_adj_e<false, true>), pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+      else
+        hipLaunchKernelGGL(k_adj_e<false>, pl->grid, pl->block, 0, s, pl->kp, a, gather, m);
+    }
   };
   // prologue of step hi-1 (E-bar holds the adjoint of E^{hi})
   if (e_planes) plane_adj(true, (long long)(hi - 1));
@@ -5176,7 +5349,7 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   for (int q = 0; q < n; ++q) {
     if (!descs[q]) continue;
     if (descs[q]->d_grad_mu || descs[q]->d_port_adj || descs[q]->d_plane_adj ||
-        descs[q]->d_flux_adj)
+        descs[q]->d_flux_adj || b->plans[q]->n_pole > 0)
       return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients of "
                                "probe objectives only");
     const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);

@@ -87,3 +87,58 @@ def test_mu_derivative_table_matches_finite_differences():
         mu, h = float(mu_r[q]), 1e-5 * float(mu_r[q])
         fd = (dt / ((mu + h) * MU_0) - dt / ((mu - h) * MU_0)) / (2 * h)
         np.testing.assert_allclose(tab[q], fd, rtol=1e-5)
+
+
+@pytest.mark.parametrize("debye,lorentz", [
+    ([(3.0, 8e-12)], []),
+    ([], [(2 * 3.14159e10, 1e9, 2.0)]),
+    ([(1.5, 5e-12), (0.7, 2e-11)], [(2 * 3.14159e10, 1e9, 2.0)]),
+])
+def test_dispersive_derivative_tables_match_finite_differences(debye, lorentz):
+    """The dispersive reverse mode's tables against float64 central
+    differences of the forward's own coefficient function
+    (rfx_amd.materials.dispersion_tables) in eps_r and sigma: d ca, d cb
+    (and d ce for eps_r), and d cc / dx = K-factor * cc for every pole's cc
+    (the Debye cc_d and the Lorentz cc_L), on pole-free and pole-carrying
+    materials with and without loss."""
+    import numpy as np
+    from types import SimpleNamespace
+    from rfx_amd.adjoint import dispersive_derivative_tables
+    from rfx_amd.materials import POLE_SHIFT, MaterialMap, dispersion_tables
+    n_pole = len(debye) + len(lorentz)
+    all_poles = ((1 << n_pole) - 1) << POLE_SHIFT
+    entries = [(1.0, 0.0, 0), (4.0, 0.02, all_poles), (2.5, 0.0, all_poles), (6.0, 0.3, 0),
+               (3.0, 0.1, (1 << POLE_SHIFT))]
+    dt = 1.9e-12
+
+    def tables(idx, de=0.0, ds=0.0):
+        m = MaterialMap((2, 2, 2))
+        for q, (e, s, f) in enumerate(entries):
+            m.get_id(e + (de if q == idx else 0.0), s + (ds if q == idx else 0.0), 1.0, f)
+        coef, _, rows = dispersion_tables(m, dt, debye, lorentz)
+        return m, coef.astype(np.float64), rows.astype(np.float64)
+
+    m0, _, _ = tables(-1)
+    prob = SimpleNamespace(mat=m0, poles=debye, lorentz=lorentz)
+    d_eps, d_sig, d_k = (t.astype(np.float64) for t in dispersive_derivative_tables(prob, dt))
+    nd = len(debye)
+    for q in range(1, len(entries)):
+        e, s, _ = entries[q]
+        # steps well above the float32 rounding of the coefficients and well
+        # below their scales of variation (eps_r itself; 2 eps_inf / dt ~ 30
+        # S/m for sigma)
+        for which, h in ((0, 2e-2 * e), (1, 0.5)):
+            _, c1, r1 = tables(q, *((h, 0.0) if which == 0 else (0.0, h)))
+            _, c0, r0 = tables(q, *((-h, 0.0) if which == 0 else (0.0, -h)))
+            fd = (c1[q] - c0[q]) / (2 * h)
+            tab = d_eps[q] if which == 0 else d_sig[q]
+            nc = 3 if which == 0 else 2
+            np.testing.assert_allclose(tab[:nc], fd[:nc], rtol=2e-3, atol=1e-3 * abs(fd[:nc]).max())
+            # every cc: the Debye rows' column 2, the Lorentz rows' column 3
+            _, cm, rm = tables(-1)
+            for p in range(n_pole):
+                col = 2 if p < nd else 3
+                cc = rm[q, p, col]
+                dcc = (r1[q, p, col] - r0[q, p, col]) / (2 * h)
+                np.testing.assert_allclose(d_k[q, which] * cc, dcc, rtol=2e-3,
+                                           atol=1e-6 * abs(cc) + 1e-30)

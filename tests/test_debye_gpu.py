@@ -107,16 +107,6 @@ def test_debye_fast_relaxation_equals_static_dielectric(R):
     assert e <= TOL
 
 
-def test_debye_gradient_refused(R):
-    import torch
-    spec = cases.debye_case(20e-12, lane="forward")
-    sim = cases.to_sim(spec)
-    g, _ = o.build_uniform(spec, 4)
-    eps = torch.ones(g.shape, dtype=torch.float32, requires_grad=True)
-    with pytest.raises(NotImplementedError):
-        sim.forward(eps_override=eps, n_steps=4)
-
-
 def test_lorentz_cavity_polaritons_gpu(R):
     _say("Lorentz-filled PEC cavity 21x17x4 x 6000")
     from tests.spectral import matrix_pencil_freqs
