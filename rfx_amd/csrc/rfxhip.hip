@@ -2519,6 +2519,12 @@ struct AdjParams {
   const float2* dkdis;
   int n_pole, n_debye;
   int pol_lo[3], pol_n[3];
+  // single-instance sweep: per-cell cb and ch (the neighbours' coefficients
+  // without an id load and a table gather each) and whether any probe
+  // samples an H component
+  const float* cbv;
+  const float* chv;
+  int h_probe;
 };
 
 __device__ __forceinline__ long long region_index(const RegionParams& r, int i, int j, int k) {
@@ -3152,10 +3158,10 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
     izn = hk ? p.inv_e0 : 0.f;
   }
   // v = cb * E-bar at the cell and its forward neighbours
-  const float cb0 = coef_at(p, c).y;
+  const float cb0 = ldf(a.cbv, c);
   const float vx = cb0 * ldf(a.a[0], c), vy = cb0 * ldf(a.a[1], c), vz = cb0 * ldf(a.a[2], c);
   const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
-  const float cbj = coef_at(p, cj).y, cbk = coef_at(p, ck).y, cbi = coef_at(p, ci).y;
+  const float cbj = ldf(a.cbv, cj), cbk = ldf(a.cbv, ck), cbi = ldf(a.cbv, ci);
   const float vx_j = cbj * ldf(a.a[0], cj), vz_j = cbj * ldf(a.a[2], cj);
   const float vx_k = cbk * ldf(a.a[0], ck), vy_k = cbk * ldf(a.a[1], ck);
   const float vy_i = cbi * ldf(a.a[1], ci), vz_i = cbi * ldf(a.a[2], ci);
@@ -3201,8 +3207,7 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
     }
   }
   // H-probe adjoints of step n
-  const uint32_t fl = p.flags[ldu16(p.mat, c)];
-  if ((fl & RFXH_POINT) && a.probe_adj) {
+  if (a.h_probe && (p.flags[ldu16(p.mat, c)] & RFXH_POINT) && a.probe_adj) {
     int lo = 0, hi2 = p.n_ops;
     while (lo < hi2) {
       const int mid = (lo + hi2) >> 1;
@@ -3221,7 +3226,7 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
   }
   // CPML-H memory adjoint (cpml.py:875-1043 transposed)
   if (in_slab(p, i, j, k)) {
-    const float ch = coef_at(p, c).w;
+    const float ch = ldf(a.chv, c);
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
       const int l = slab_layer(p, face, i, j, k);
@@ -3258,6 +3263,15 @@ __global__ __launch_bounds__(kWG) void k_adj_h(KParams p, AdjParams a, long long
   stf(a.a[3], c, hx);
   stf(a.a[4], c, hy);
   stf(a.a[5], c, hz);
+}
+
+// Per-cell cb and ch of the single-instance reverse sweep.
+__global__ __launch_bounds__(256) void k_fill_cbch(KParams p, float* cbv, float* chv, long long n) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const float4 co = p.coef[ldu16(p.mat, (unsigned)c)];
+  cbv[c] = co.y;
+  chv[c] = co.w;
 }
 
 // Row of a cell with poles in the adjoint polarisation box (-1: none); R =
@@ -3301,7 +3315,7 @@ __global__ __launch_bounds__(kWG) void k_adj_e(KParams p, AdjParams a, int gathe
     const float ch0 = co.w;
     const float ux = ch0 * ldf(a.a[3], c), uy = ch0 * ldf(a.a[4], c), uz = ch0 * ldf(a.a[5], c);
     const unsigned cj = hj ? c - p.nz : c, ck = hk ? c - 1 : c, ci = hi ? c - P : c;
-    const float chj = coef_at(p, cj).w, chk = coef_at(p, ck).w, chi = coef_at(p, ci).w;
+    const float chj = ldf(a.chv, cj), chk = ldf(a.chv, ck), chi = ldf(a.chv, ci);
     const float ux_j = chj * ldf(a.a[3], cj), uz_j = chj * ldf(a.a[5], cj);
     const float ux_k = chk * ldf(a.a[3], ck), uy_k = chk * ldf(a.a[4], ck);
     const float uy_i = chi * ldf(a.a[4], ci), uz_i = chi * ldf(a.a[5], ci);
@@ -3643,6 +3657,9 @@ struct rfxh_plan {
   float4* dcoef = nullptr;      // per material d(ca, cb, ce)/d eps_r (reverse mode)
   float2* dcoef_s = nullptr;    // per material d(ca, cb)/d sigma (reverse mode)
   float2* dkdis = nullptr;      // per material -gamma'/gamma of the ADE (reverse mode)
+  float* cbv = nullptr;         // per-cell cb, ch of the reverse sweep (built on first use)
+  float* chv = nullptr;
+  int h_probe = 0;              // some probe samples an H component
   float* dmu = nullptr;         // per material d ch / d mu_r (reverse mode)
   int n_mat = 0;
 };
@@ -4109,6 +4126,8 @@ int rfxh_plan_create(const rfxh_desc* d, rfxh_plan** out) {
   });
 
   rfxh_plan* pl = new rfxh_plan();
+  for (const rfxh_point_op& op : ops)
+    if (op.kind == RFXH_OP_PROBE && op.comp >= 3) pl->h_probe = 1;
   {
     // point operations per tile column of the temporal-blocked march, over
     // all x: 16 x 64 tiles (rows [j0 - halo, + kTBW), columns [k0 - halo,
@@ -4973,6 +4992,20 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (hi == lo) return RFXH_OK;
+  if (!pl->cbv) {
+    // per-cell cb and ch, built once per plan (the material ids are fixed)
+    const long long ncell = (long long)pl->kp.nx * pl->kp.ny * pl->kp.nz;
+    void* d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * (size_t)ncell * sizeof(float)));
+    pl->owned.push_back(d);
+    pl->cbv = static_cast<float*>(d);
+    pl->chv = pl->cbv + ncell;
+    hipLaunchKernelGGL(k_fill_cbch, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, pl->kp,
+                       pl->cbv, pl->chv, ncell);
+  }
+  a.cbv = pl->cbv;
+  a.chv = pl->chv;
+  a.h_probe = pl->h_probe;
   // DFT-plane adjoints: planes of an E component take theirs just before
   // the prologue of their step (the sample is E^{n+1} after the injection,
   // like a probe's), planes of an H component before the H-bar pass of
