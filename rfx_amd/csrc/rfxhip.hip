@@ -2753,10 +2753,10 @@ __device__ __forceinline__ void adj_h_cell(const KParams& p, const AdjParams& a,
     izn = hk ? p.inv_e0 : 0.f;
   }
   // v = cb * E-bar at the cell and its forward neighbours
-  const float cb0 = coef_at(p, c).y;
+  const float cb0 = ldf(a.cbv, c);
   const float vx = cb0 * ldf(a.a[0], c), vy = cb0 * ldf(a.a[1], c), vz = cb0 * ldf(a.a[2], c);
   const unsigned cj = hj ? c + p.nz : c, ck = hk ? c + 1 : c, ci = hi ? c + P : c;
-  const float cbj = coef_at(p, cj).y, cbk = coef_at(p, ck).y, cbi = coef_at(p, ci).y;
+  const float cbj = ldf(a.cbv, cj), cbk = ldf(a.cbv, ck), cbi = ldf(a.cbv, ci);
   const float vx_j = cbj * ldf(a.a[0], cj), vz_j = cbj * ldf(a.a[2], cj);
   const float vx_k = cbk * ldf(a.a[0], ck), vy_k = cbk * ldf(a.a[1], ck);
   const float vy_i = cbi * ldf(a.a[1], ci), vz_i = cbi * ldf(a.a[2], ci);
@@ -2802,8 +2802,7 @@ __device__ __forceinline__ void adj_h_cell(const KParams& p, const AdjParams& a,
     }
   }
   // H-probe adjoints of step n
-  const uint32_t fl = p.flags[ldu16(p.mat, c)];
-  if ((fl & RFXH_POINT) && a.probe_adj) {
+  if (a.h_probe && (p.flags[ldu16(p.mat, c)] & RFXH_POINT) && a.probe_adj) {
     int lo = 0, hi2 = p.n_ops;
     while (lo < hi2) {
       const int mid = (lo + hi2) >> 1;
@@ -2822,7 +2821,7 @@ __device__ __forceinline__ void adj_h_cell(const KParams& p, const AdjParams& a,
   }
   // CPML-H memory adjoint (cpml.py:875-1043 transposed)
   if (in_slab(p, i, j, k)) {
-    const float ch = coef_at(p, c).w;
+    const float ch = ldf(a.chv, c);
 #pragma unroll
     for (int face = 0; face < 6; ++face) {
       const int l = slab_layer(p, face, i, j, k);
@@ -2888,7 +2887,7 @@ __device__ __forceinline__ void adj_e_cell(const KParams& p, const AdjParams& a,
     const float ch0 = co.w;
     const float ux = ch0 * ldf(a.a[3], c), uy = ch0 * ldf(a.a[4], c), uz = ch0 * ldf(a.a[5], c);
     const unsigned cj = hj ? c - p.nz : c, ck = hk ? c - 1 : c, ci = hi ? c - P : c;
-    const float chj = coef_at(p, cj).w, chk = coef_at(p, ck).w, chi = coef_at(p, ci).w;
+    const float chj = ldf(a.chv, cj), chk = ldf(a.chv, ck), chi = ldf(a.chv, ci);
     const float ux_j = chj * ldf(a.a[3], cj), uz_j = chj * ldf(a.a[5], cj);
     const float ux_k = chk * ldf(a.a[3], ck), uy_k = chk * ldf(a.a[4], ck);
     const float uy_i = chi * ldf(a.a[4], ci), uz_i = chi * ldf(a.a[5], ci);
@@ -4986,14 +4985,12 @@ static int adjoint_params(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo
   return RFXH_OK;
 }
 
-int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
-  AdjParams a;
-  const int rc = adjoint_params(pl, ad, lo, hi, &a);
-  if (rc) return rc;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (hi == lo) return RFXH_OK;
+}  // extern "C"
+
+// Per-cell cb and ch of a plan's reverse sweep, built once (the material ids
+// are fixed), into a.
+static int use_cbch(rfxh_plan* pl, hipStream_t s, AdjParams& a) {
   if (!pl->cbv) {
-    // per-cell cb and ch, built once per plan (the material ids are fixed)
     const long long ncell = (long long)pl->kp.nx * pl->kp.ny * pl->kp.nz;
     void* d = nullptr;
     HIPCHK(hipMalloc(&d, 2 * (size_t)ncell * sizeof(float)));
@@ -5006,6 +5003,21 @@ int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t
   a.cbv = pl->cbv;
   a.chv = pl->chv;
   a.h_probe = pl->h_probe;
+  return RFXH_OK;
+}
+
+extern "C" {
+
+int rfxh_adjoint(rfxh_plan* pl, const rfxh_adjoint_desc* ad, int64_t lo, int64_t hi, void* stream) {
+  AdjParams a;
+  const int rc = adjoint_params(pl, ad, lo, hi, &a);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (hi == lo) return RFXH_OK;
+  {
+    const int rc2 = use_cbch(pl, s, a);
+    if (rc2) return rc2;
+  }
   // DFT-plane adjoints: planes of an E component take theirs just before
   // the prologue of their step (the sample is E^{n+1} after the injection,
   // like a probe's), planes of an H component before the H-bar pass of
@@ -5376,6 +5388,7 @@ int rfxh_batch_run_record(rfxh_batch* b, const int64_t* step0, const int64_t* n_
 int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, const int64_t* lo,
                        const int64_t* hi, void* stream) {
   if (!b || !descs || !lo || !hi) return fail(RFXH_EINVAL, "bad batch adjoint arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
   const int n = (int)b->plans.size();
   std::vector<int> order;
   std::vector<BatchAdjEntry> all(n);
@@ -5385,8 +5398,9 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
         descs[q]->d_flux_adj || b->plans[q]->n_pole > 0)
       return fail(RFXH_EINVAL, "the batched reverse mode computes eps_r and sigma gradients of "
                                "probe objectives only");
-    const int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);
+    int rc = adjoint_params(b->plans[q], descs[q], lo[q], hi[q], &all[q].a);
     if (rc) return rc;
+    if ((rc = use_cbch(b->plans[q], s, all[q].a))) return rc;
     all[q].lo = lo[q];
     all[q].hi = hi[q];
     if (hi[q] > lo[q]) order.push_back(q);
@@ -5402,7 +5416,6 @@ int rfxh_batch_adjoint(rfxh_batch* b, const rfxh_adjoint_desc* const* descs, con
   if (rc) return rc;
   std::vector<BatchAdjEntry> adj(na);
   for (int r = 0; r < na; ++r) adj[r] = all[order[r]];
-  hipStream_t s = static_cast<hipStream_t>(stream);
   HIPCHK(hipMemcpyAsync(b->d_xents, ents.data(), sizeof(BatchEntry) * na, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(b->d_xstart, start.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(b->d_aents, adj.data(), sizeof(BatchAdjEntry) * na, hipMemcpyHostToDevice, s));
