@@ -1212,6 +1212,11 @@ def build_uniform(spec, n_steps):
         sigma = np.asarray(spec["sigma_override"], F32).reshape(shape).copy()
     if lane == "forward" and spec.get("mu_r_override") is not None:
         mu_r = np.asarray(spec["mu_r_override"], F32).reshape(shape).copy()
+    if lane == "forward" and spec.get("pec_mask_override") is not None:
+        # _execute.py:2807-2808: merged with the geometry's PEC cells; the
+        # ports below clear their own cells as for geometry PEC
+        extra = np.asarray(spec["pec_mask_override"], bool).reshape(shape)
+        pec = extra.copy() if pec is None else (pec | extra)
     dt = g.dt
     tms = times_f32(n_steps, dt)
     sources, ports_vi = [], []

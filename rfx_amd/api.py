@@ -462,7 +462,7 @@ class Simulation:
 
     def _build_problem(self, n_steps: int, lane: str, eps_override=None,
                        sigma_override=None, mu_r_override=None, s11_freqs=None,
-                       material_sweep=None, subpixel_smoothing=False):
+                       material_sweep=None, subpixel_smoothing=False, pec_mask_override=None):
         """Host set-up of one run.  ``material_sweep = (name, field, value)``
         replaces one material property as rfx/vmap_sweep.py:265-390 does:
         with a material name, the cells inside that material's shapes take
@@ -477,7 +477,7 @@ class Simulation:
         has_poles = any(self._materials[name].debye or self._materials[name].lorentz
                         for _, name in self._geometry)
         overridden = (eps_override is not None or sigma_override is not None
-                      or mu_r_override is not None)
+                      or mu_r_override is not None or pec_mask_override is not None)
         dasm = None
         if self._use_device_assembly(shape, material_sweep is not None or has_poles or overridden):
             # box fill, pads and PEC bits on the GPU; single-cell edits (ports)
@@ -501,6 +501,12 @@ class Simulation:
                 pec[np.ix_(*[np.nonzero(x)[0] for x in masks])] = True
             else:
                 mat.fill_box(masks, m.eps_r, m.sigma, m.mu_r)
+        if pec_mask_override is not None:
+            # merged with the geometry's PEC cells (rfx/api/_execute.py:2807-2808)
+            extra = np.asarray(pec_mask_override, bool)
+            if extra.shape != tuple(shape):
+                raise ValueError(f"pec_mask_override shape {extra.shape} != grid shape {shape}")
+            pec = extra.copy() if pec is None else (pec | extra)
         if material_sweep is not None and material_sweep[0] is not None:
             sw_name, sw_field, sw_value = material_sweep
             sel = np.zeros(shape, bool)
@@ -787,15 +793,18 @@ class Simulation:
                       ntff_data=self._ntff_results(prob.ntff_box, dr), ntff_box=prob.ntff_box)
 
     def forward(self, *, eps_override=None, sigma_override=None, mu_r_override=None,
-                n_steps: int | None = None, num_periods: float = 20.0, checkpoint: bool = True,
-                checkpoint_segments: int | None = None, port_s11_freqs=None,
-                skip_preflight: bool = False, grad_region=None, **unsupported):
+                pec_mask_override=None, n_steps: int | None = None, num_periods: float = 20.0,
+                checkpoint: bool = True, checkpoint_segments: int | None = None,
+                port_s11_freqs=None, skip_preflight: bool = False, grad_region=None,
+                **unsupported):
         """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
         PEC on axes covered by CPML (_execute.py:1420-1421); the returned
         time series (and, with lumped ports and ``port_s11_freqs``, the
         S-parameters, a torch tensor then) carry gradients w.r.t.
         ``eps_override``, ``sigma_override`` and ``mu_r_override`` through the
-        hand-written adjoint.  ``grad_region`` = (i0, i1, j0, j1, k0, k1)
+        hand-written adjoint.  ``pec_mask_override`` (bool, grid shape) adds
+        hard PEC cells to the geometry's (rfx/api/_execute.py:2807-2808).
+        ``grad_region`` = (i0, i1, j0, j1, k0, k1)
         limits where dJ/d eps_r is evaluated (default: the whole grid);
         ``checkpoint_segments`` = K forces K recompute segments."""
         _reject(unsupported, "forward")
@@ -809,9 +818,11 @@ class Simulation:
 
         needs_grad = _req(eps_override) or _req(sigma_override) or _req(mu_r_override)
         eps_np = _to_np(eps_override)
+        if isinstance(pec_mask_override, torch.Tensor):
+            pec_mask_override = pec_mask_override.detach().cpu().numpy()
         grid, prob, ports = self._build_problem(n_steps, "forward", eps_np,
                                                 _to_np(sigma_override), _to_np(mu_r_override),
-                                                s11)
+                                                s11, pec_mask_override=pec_mask_override)
         port_t = plane_t = flux_t = None
         if needs_grad:
             from .adjoint import forward_with_grad
