@@ -148,7 +148,7 @@ class AdjointRunner:
     def __init__(self, prob: Problem, n_steps: int, device, region=None,
                  checkpoint_segments: int | None = None,
                  record_budget: int = DEFAULT_RECORD_BUDGET, want_sigma: bool = False,
-                 want_mu: bool = False):
+                 want_mu: bool = False, n_warmup: int = 0):
         self.prob = prob
         self.n_steps = int(n_steps)
         self.dr = DeviceRun(prob, n_steps, device, two_sets=False)
@@ -171,10 +171,16 @@ class AdjointRunner:
         if self.dr.device.type == "cuda":
             free, _ = torch.cuda.mem_get_info(self.dr.device)
             budget = min(budget, int(0.8 * free))
-        nseg = segment_count(self.n_steps, per_step, state, budget, checkpoint_segments)
-        self.seg_len = math.ceil(self.n_steps / nseg)
+        # steps [0, n_warmup) run without a record: the state after them is a
+        # constant of the gradient (rfx/nonuniform.py:1734-1752)
+        self.n_warmup = int(n_warmup)
+        if not 0 <= self.n_warmup < max(self.n_steps, 1):
+            raise ValueError(f"n_warmup ({self.n_warmup}) must be in [0, n_steps={self.n_steps})")
+        n_opt = self.n_steps - self.n_warmup
+        nseg = segment_count(n_opt, per_step, state, budget, checkpoint_segments)
+        self.seg_len = math.ceil(n_opt / nseg)
         self.bounds = [(lo, min(self.n_steps, lo + self.seg_len))
-                       for lo in range(0, self.n_steps, self.seg_len)]
+                       for lo in range(self.n_warmup, self.n_steps, self.seg_len)]
         self.dcoef_disp = None
         if self.disp:
             self.dcoef, dsig, self.dcoef_disp = dispersive_derivative_tables(prob, prob.dt)
@@ -210,8 +216,10 @@ class AdjointRunner:
                 rec_k, self.dr.stream_ptr()))
 
     def forward(self) -> torch.Tensor:
+        if self.n_warmup:
+            self.dr.run(0, self.n_warmup, use_graph=True)
         if len(self.bounds) == 1:
-            self._record(0, self.n_steps)
+            self._record(self.n_warmup, self.n_steps)
         else:
             for lo, hi in self.bounds:
                 pol = self.dr.pol.clone() if self.dr.pol is not None else None
@@ -642,7 +650,7 @@ class _ForwardFn(torch.autograd.Function):
 
 def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
                       checkpoint_segments=None, grad_region=None, sigma_tensor=None,
-                      mu_tensor=None):
+                      mu_tensor=None, n_warmup: int = 0):
     """The time series, the lumped ports' (V, I) samples (T, n_port, 2), the
     DFT planes' accumulators and the flux accumulators (flux monitors, then
     NTFF faces) as functions of eps_override, sigma_override and
@@ -652,7 +660,8 @@ def forward_with_grad(sim, prob: Problem, grid, eps_tensor, n_steps: int,
 
     eps_in, sig_in, mu_in = req(eps_tensor), req(sigma_tensor), req(mu_tensor)
     runner = AdjointRunner(prob, n_steps, sim._device, grad_region, checkpoint_segments,
-                           want_sigma=sig_in is not None, want_mu=mu_in is not None)
+                           want_sigma=sig_in is not None, want_mu=mu_in is not None,
+                           n_warmup=n_warmup)
     ts, ports, *acc = _ForwardFn.apply(eps_in, sig_in, mu_in, runner)
     k = len(prob.planes)
     return ts, ports, acc[:k], acc[k:], runner.dr

@@ -10,6 +10,7 @@ NotImplementedError instead of running a different algorithm.
 """
 from __future__ import annotations
 
+import math
 import warnings
 from dataclasses import dataclass, field
 from typing import NamedTuple
@@ -796,7 +797,8 @@ class Simulation:
                 pec_mask_override=None, n_steps: int | None = None, num_periods: float = 20.0,
                 checkpoint: bool = True, checkpoint_segments: int | None = None,
                 port_s11_freqs=None, skip_preflight: bool = False, grad_region=None,
-                **unsupported):
+                emit_time_series: bool = True, checkpoint_every: int | None = None,
+                n_warmup: int = 0, **unsupported):
         """Differentiable forward (rfx/api/_execute.py:2447-2858): no outer-face
         PEC on axes covered by CPML (_execute.py:1420-1421); the returned
         time series (and, with lumped ports and ``port_s11_freqs``, the
@@ -806,12 +808,29 @@ class Simulation:
         hard PEC cells to the geometry's (rfx/api/_execute.py:2807-2808).
         ``grad_region`` = (i0, i1, j0, j1, k0, k1)
         limits where dJ/d eps_r is evaluated (default: the whole grid);
-        ``checkpoint_segments`` = K forces K recompute segments."""
+        ``checkpoint_segments`` = K forces K recompute segments.  On the
+        graded lane, as in the reference (rfx/nonuniform.py:1589-1804):
+        ``checkpoint_every`` = recompute segment length, ``n_warmup`` = first
+        steps whose end state is a constant of the gradient (a truncated
+        gradient; outputs unchanged), ``emit_time_series=False`` = no probe
+        series; the uniform lane refuses them as the reference's does
+        (rfx/api/_execute.py:2331-2357)."""
         _reject(unsupported, "forward")
+        if not self._graded and (not emit_time_series or checkpoint_every is not None
+                                 or n_warmup > 0):
+            raise NotImplementedError(
+                "emit_time_series=False, checkpoint_every and n_warmup are supported on the "
+                "non-uniform (graded) forward path only; on the uniform lane use "
+                "checkpoint_segments")
         grid = self._build_grid()
         if n_steps is None:
             n_steps = grid.num_timesteps(num_periods)
         n_steps = int(n_steps)
+        n_warmup = int(n_warmup)
+        if n_warmup and n_warmup >= n_steps:
+            raise ValueError(f"n_warmup ({n_warmup}) must be < n_steps ({n_steps})")
+        if checkpoint_every is not None and 0 < int(checkpoint_every) < n_steps - n_warmup:
+            checkpoint_segments = math.ceil((n_steps - n_warmup) / int(checkpoint_every))
         s11 = None if port_s11_freqs is None else np.asarray(port_s11_freqs, F32)
         def _req(t):
             return isinstance(t, torch.Tensor) and t.requires_grad
@@ -829,7 +848,7 @@ class Simulation:
             ts, port_t, plane_t, flux_t, dr = forward_with_grad(self, prob, grid, eps_override, n_steps,
                                                checkpoint_segments, grad_region,
                                                sigma_tensor=sigma_override,
-                                               mu_tensor=mu_r_override)
+                                               mu_tensor=mu_r_override, n_warmup=n_warmup)
         else:
             dr = DeviceRun(prob, n_steps, self._device)
             dr.run(0, n_steps)
@@ -867,6 +886,8 @@ class Simulation:
                   if self._flux_monitors else None)
         ntff = self._ntff_results(prob.ntff_box, dr,
                                   None if flux_t is None else flux_t[n_mon:n_mon + 6])
+        if not emit_time_series:      # no probe series (rfx/nonuniform.py:1538-1542)
+            ts = ts[:, :0]
         return ForwardResult(time_series=ts, grid=grid, s_params=s_params, freqs=freqs,
                              dft_planes=planes, flux_monitors=fluxes,
                              ntff_data=ntff, ntff_box=prob.ntff_box)

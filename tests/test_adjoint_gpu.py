@@ -496,6 +496,48 @@ def test_pec_mask_override_forward_and_gradient(torch_mod):
     assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
 
 
+def test_graded_forward_options(torch_mod):
+    """The graded lane's forward options (rfx/nonuniform.py:1589-1804):
+    n_warmup leaves the outputs unchanged and, with the gradient region
+    outside the fields' reach during the warm-up, the gradient too;
+    checkpoint_every = 37 equals one segment; emit_time_series=False gives
+    no probe series; the uniform lane refuses all three."""
+    torch = torch_mod
+    _say("graded forward options: n_warmup, checkpoint_every, emit_time_series")
+    spec = cases.mixed_graded_case()
+    spec["ports"] = [p for p in spec["ports"] if p[2] == 0.0]
+    spec["s11_freqs"] = None
+    n = 160
+    sim = cases.to_sim(spec)
+    _, cfg0 = o.build_graded(spec, n)
+    base = cfg0.eps_r.copy()
+    region = (17, 21, 4, 14, 2, 12)    # >= 6 cells from the source at x index 10
+
+    def run(**kw):
+        eps = torch.tensor(base, requires_grad=True)
+        res = sim.forward(eps_override=eps, n_steps=n, grad_region=region, **kw)
+        accs = [res.dft_planes[k].accumulator for k in res.dft_planes]
+        J = (res.time_series.double() ** 2).sum() if res.time_series.shape[1] else 0.0
+        J = J + sum((a.abs() ** 2).sum() / (a.detach().abs() ** 2).sum() for a in accs)
+        J.backward()
+        return res, eps.grad.numpy().copy()
+
+    r0, g0 = run()
+    r1, g1 = run(n_warmup=2)        # the fields reach at most 4 cells in 2 steps
+    np.testing.assert_array_equal(r1.time_series.detach().cpu().numpy(),
+                                  r0.time_series.detach().cpu().numpy())
+    assert np.abs(g0).max() > 0
+    np.testing.assert_allclose(g1, g0, rtol=1e-6, atol=1e-7 * np.abs(g0).max())
+    r2, g2 = run(checkpoint_every=37)
+    np.testing.assert_allclose(g2, g0, rtol=1e-5, atol=1e-6 * np.abs(g0).max())
+    r3, g3 = run(emit_time_series=False)
+    assert tuple(r3.time_series.shape) == (n, 0)
+    usim = cases.to_sim(cases.mixed_case("forward"))
+    for kw in ({"n_warmup": 3}, {"checkpoint_every": 10}, {"emit_time_series": False}):
+        with pytest.raises(NotImplementedError):
+            usim.forward(n_steps=20, **kw)
+
+
 def test_segmented_gradient_equals_single(torch_mod):
     torch = torch_mod
     _say("checkpoint segments == single segment")
