@@ -538,6 +538,60 @@ def test_graded_forward_options(torch_mod):
             usim.forward(n_steps=20, **kw)
 
 
+def test_graded_s11_gradient_vs_oracle_fd(torch_mod):
+    """Graded-z lane with its single-cell and wire ports (V/I sampled after
+    the injection on this lane): J = sum |S|^2 over the ports and
+    frequencies, directional derivatives w.r.t. eps_r vs float64 oracle
+    central differences."""
+    torch = torch_mod
+    _say("graded S-parameter objective directional FD")
+    spec = cases.mixed_graded_case()
+    n = 200
+    sim = cases.to_sim(spec)
+    _, cfg0 = o.build_graded(spec, n)
+    base = cfg0.eps_r.copy()
+    z0 = [p[2] for p in spec["ports"] if p[2] != 0.0]
+    region = (3, 15, 4, 14, 2, 12)
+    eps = torch.tensor(base, requires_grad=True)
+    res = sim.forward(eps_override=eps, n_steps=n, grad_region=region,
+                      port_s11_freqs=np.asarray(spec["s11_freqs"], np.float32))
+    s = res.s_params
+    assert isinstance(s, torch.Tensor) and s.requires_grad
+    J = (s.abs() ** 2).sum()
+
+    def J_oracle(sp):
+        _, cfg = o.build_graded(sp, n)
+        out = cstep.run_c(cfg, n, want_fields=False)
+        return sum(float(np.sum(np.abs(o.extract_lumped_s11(out.port_v_dft[q], out.port_i_dft[q],
+                                                             z0[q])) ** 2))
+                   for q in range(len(out.port_v_dft)))
+
+    sp0 = dict(spec)
+    sp0["eps_override"] = base
+    J0 = J_oracle(sp0)
+    print("J", float(J.detach()), J0)
+    assert abs(float(J.detach()) - J0) <= 1e-3 * abs(J0)
+    J.backward()
+    grad = eps.grad.numpy().astype(np.float64)
+    assert np.abs(grad).max() > 0
+    rng = np.random.default_rng(23)
+    i0, i1, j0, j1, k0, k1 = region
+    for trial in range(2):
+        d = np.zeros_like(base, dtype=np.float64)
+        d[i0:i1, j0:j1, k0:k1] = rng.uniform(-1, 1, (i1 - i0, j1 - j0, k1 - k0))
+        h = 1e-3
+        Js = []
+        for sgn in (1, -1):
+            sp = dict(spec)
+            sp["eps_override"] = (base + sgn * h * d).astype(np.float32)
+            Js.append(J_oracle(sp))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float((grad * d).sum())
+        scale = float(np.abs(grad * d).sum())
+        print("graded s11 direction", trial, ad, fd, abs(ad - fd) / abs(fd), abs(ad - fd) / scale)
+        assert abs(ad - fd) <= 2e-3 * abs(fd) + 1e-4 * scale
+
+
 def test_segmented_gradient_equals_single(torch_mod):
     torch = torch_mod
     _say("checkpoint segments == single segment")
